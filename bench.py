@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark: assembled nnz/s of the mixed-precision MFMA path vs the plain FP64
+GPU path on the 3D linear-elasticity workload (h1xyz P2 on 64^3 x 6 Kuhn
+tetrahedra, jittered, E lognormal per element, nu = 0.3).
+
+One step = one numeric assembly (Jacobians, Gauss-point coefficients, element
+contractions, deterministic FP64 reduction into the cached CSR pattern + rhs)
+with the node coordinates and parameters already resident in HBM.
+
+Prints one JSON line (see the keys below).  One GPU only: --gpus > 1 exits.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAKS = {  # MI355X published dense peaks
+    "hbm_GBps": 8000.0,
+    "fp16_TFLOPs": 2500.0,
+    "bf16_TFLOPs": 2500.0,
+    "fp32_TFLOPs": 157.3,
+    "fp64_TFLOPs": 78.6,
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def element_flops(c, n, d, ngp):
+    """Canonical B^T D B contraction flops per element: 2 n_gp (c n)^2 (c d)."""
+    return 2.0 * ngp * (c * n) ** 2 * (c * d)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64, help="cells per cube edge (workload: 64)")
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--mixed", default="f16split", help="mixed precision mode")
+    ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU baseline sample size (0: auto)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    import sparselizard_amd as sa
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    ctx = sa.Context(0)
+    spec = "n=%d;order=%d" % (args.n, args.order)
+    t0 = time.time()
+    prob = sa.Problem(ctx, "elasticity3d", spec)
+    info = prob.info()
+    t_host = time.time() - t0
+    log("problem built in %.1f s: %s" % (t_host, info))
+    t0 = time.time()
+    plan = prob.plan()
+    t_sym = time.time() - t0
+    shape = plan.shape()
+    stats = plan.stats()
+    nnz = shape["nnzA"] + shape["nnzD"]
+    log("plan (symbolic) in %.1f s: %s %s" % (t_sym, shape, stats))
+
+    nd = prob.numeric()
+    nn = info["ndofs"]
+    # device-resident inputs
+    import ctypes as C
+    xyz_host = np.ctypeslib.as_array(nd.node_xyz, shape=(3 * prob.n_nodes(),)).copy()
+    par_host = np.ctypeslib.as_array(nd.params, shape=(nd.n_params * nd.param_len,)).copy()
+    d_xyz = torch.from_numpy(xyz_host).to(dev)
+    d_par = torch.from_numpy(par_host).to(dev)
+    stream = torch.cuda.current_stream()
+    sptr = C.c_void_p(stream.cuda_stream)
+
+    def step(prec):
+        plan.assemble_device(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None, prec, sptr)
+
+    def timed(prec):
+        for _ in range(args.warmup):
+            step(prec)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t_wall = time.time()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            step(prec)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        t_wall = time.time() - t_wall
+        return ev0.elapsed_time(ev1) / args.steps, t_wall
+
+    def csr_now():
+        return plan.csr(nn)
+
+    # plain FP64 GPU path
+    ms64, _ = timed("fp64")
+    ref = csr_now()
+    log("fp64: %.3f ms/step" % ms64)
+    # mixed precision path (the headline)
+    msmix, wall = timed(args.mixed)
+    mix = csr_now()
+    log("%s: %.3f ms/step" % (args.mixed, msmix))
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    err_A = rel(mix.Aval, ref.Aval)
+    err_D = rel(mix.Dval, ref.Dval) if ref.Dval.size else 0.0
+    err_b = rel(mix.rhs, ref.rhs)
+    glob = float(np.sqrt(np.sum((mix.Aval - ref.Aval) ** 2) + np.sum((mix.Dval - ref.Dval) ** 2)) /
+                 np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2)))
+
+    # per-phase timing (HIP events on the plan's stream) for the roofline
+    t_elem, t_red = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
+                                 args.mixed, iters=max(3, args.steps // 2))
+    t64_elem, t64_red = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
+                                     "fp64", iters=max(3, args.steps // 2))
+    n_elem = count_tets(args.n)
+    n = (args.order + 1) * (args.order + 2) * (args.order + 3) // 6
+    ngp = {1: 11, 2: 24, 3: 43}.get(args.order, 24)
+    flops = element_flops(3, n, 3, ngp) * n_elem
+    ke_bytes = 4 if args.mixed in ("f16split", "bf16split", "fp32") else 8
+    red_bytes = stats["n_coo"] * (ke_bytes + 4) + stats["n_elem_rows"] * 8 + nnz * 8 + nn * 8
+    mixed_kind = {"f16split": "fp16", "bf16split": "bf16", "fp32": "fp32"}.get(args.mixed, "fp32")
+    mfma_used = os.environ.get("SLX_MIXED_IS_VALU", "") == ""
+    peak_elem = PEAKS[mixed_kind + "_TFLOPs"]
+    if t_elem >= t_red:
+        achieved = flops / (t_elem * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_elem, "unit": "TFLOP/s",
+                "frac": round(achieved / peak_elem, 5), "traffic": None, "kernel": "element contraction",
+                "kernel_ms": round(t_elem, 4)}
+    else:
+        achieved = red_bytes / (t_red * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
+                "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "csr reduction",
+                "kernel_ms": round(t_red, 4)}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(traffic_file):
+        try:
+            tr = json.load(open(traffic_file))
+            key = "%s_%s_n%d" % (roof["kernel"].replace(" ", "_"), args.mixed, args.n)
+            if key in tr:
+                roof["traffic"] = tr[key]
+        except Exception:
+            pass
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sa, args)
+
+    out = {
+        "metric": "assembled nnz/s: MFMA-batched mixed-prec vs plain FP64 GPU at rel err <=1e-6",
+        "value": nnz / (msmix * 1e-3),
+        "unit": "nnz/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": msmix,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"f16split": "f16x2-split/f32acc/f64reduce", "bf16split": "bf16x3-split/f32acc/f64reduce",
+                  "fp32": "f32/f64reduce"}.get(args.mixed, args.mixed),
+        "data": "synthetic",
+        "config": {"workload": "3D linear elasticity h1xyz P%d, %d^3 x 6 jittered tets (%d), E lognormal(0,0.5) per "
+                               "element, nu=0.3, clamped x=0" % (args.order, args.n, n_elem),
+                   "n_elem": n_elem, "ndofs": nn, "nnzA": shape["nnzA"], "nnzD": shape["nnzD"],
+                   "coo_entries": stats["n_coo"], "parallelism": "single GPU"},
+        "fp64_gpu": {"ms_per_step": ms64, "nnz_per_s": nnz / (ms64 * 1e-3), "elem_ms": t64_elem, "reduce_ms": t64_red},
+        "mixed_gpu": {"mode": args.mixed, "ms_per_step": msmix, "elem_ms": t_elem, "reduce_ms": t_red},
+        "ratio_mixed_vs_fp64": ms64 / msmix,
+        "rel_err_vs_fp64_gpu": {"global_frobenius": glob, "A": err_A, "D": err_D, "rhs": err_b},
+        "symbolic_s": t_sym,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def count_tets(n):
+    return 6 * n ** 3
+
+
+def cpu_baseline(sa, args):
+    """The CPU oracle (restated generate()+A()+b(), FP64, 1 thread) on a bounded
+    sample of the same workload; nnz/s is reported for that sample."""
+    from oracle import py_oracle
+    import numpy as np
+
+    def run(n):
+        p = sa.Problem(None, "elasticity3d", "n=%d;order=%d" % (n, args.order))
+        info = p.info()
+        nd = p.numeric()
+        t0 = time.time()
+        o = py_oracle.assemble(p.desc_ptr(), nd, info["ndofs"], info["nfree"], nthreads=1)
+        dt = time.time() - t0
+        return dt, len(o.Aval) + len(o.Dval), count_tets(n)
+
+    n = args.cpu_sample_n
+    if n <= 0:
+        dt, nnz, ne = run(4)
+        per_elem = dt / ne
+        # aim at ~15 s of CPU work
+        n = max(4, int(round((15.0 / max(per_elem, 1e-9) / 6) ** (1.0 / 3))))
+        n = min(n, 20)
+    dt, nnz, ne = run(n)
+    return {"value": nnz / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
+            "sample": "oracle generate()+A()+b() FP64, 1 thread, elasticity P%d on %d^3 x 6 tets (%d elements), %.1f s"
+                      % (args.order, n, ne, dt)}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,217 @@
+/* slx — MI355X-native weak-form assembly: C ABI.
+ *
+ * Two layers, both plain C (no C++ or torch types cross this boundary):
+ *
+ *  1. Assembly plan layer (the hot path).  A plan holds the symbolic state
+ *     of one formulation on one mesh: element buckets with their Gauss
+ *     tables, shape-function tables, DOF address tables, term tables and
+ *     coefficient programs, plus the cached CSR pattern.  slx_assemble runs
+ *     the numeric phase on the GPU: Jacobians, Gauss-point coefficients,
+ *     element contractions and the deterministic reduction into the CSR.
+ *       replaces contribution::generate (src/formulation/contribution.cpp:20-307)
+ *       and rawmat::process (src/formulation/rawmat.cpp:127-298).
+ *
+ *  2. Formulation layer, mirroring the reference objects a user drives:
+ *     mesh, field, formulation (+= integral), generate(), A(), b().
+ *       replaces formulation::operator+= (src/formulation/formulation.cpp:51-149),
+ *       formulation::generate() (formulation.cpp:211-218),
+ *       formulation::A()/b() (formulation.cpp:358-422).
+ *
+ * Every function returns SLX_OK (0) or a negative status; the message of
+ * the last failure on the calling thread is returned by slx_last_error().
+ * Errors never propagate as C++ exceptions (the reference throws
+ * std::runtime_error from logs::error(), src/logs.cpp:9-12).
+ */
+#ifndef SLX_H
+#define SLX_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t slx_status;
+#define SLX_OK 0
+#define SLX_ERR_INVALID -1     /* bad argument or malformed description */
+#define SLX_ERR_DEVICE -2      /* HIP runtime failure */
+#define SLX_ERR_UNSUPPORTED -3 /* feature outside the implemented subset */
+#define SLX_ERR_NOMEM -4       /* device or host allocation failed */
+
+/* Precision of the numeric phase. */
+#define SLX_PREC_FP64 0        /* plain FP64 element kernel (FP64 everywhere) */
+#define SLX_PREC_F16_SPLIT 1   /* FP16 hi/lo split MFMA, FP32 accumulation, FP64 reduction */
+#define SLX_PREC_BF16_SPLIT 2  /* BF16 hi/mid/lo split MFMA, FP32 accumulation, FP64 reduction */
+#define SLX_PREC_FP32 3        /* FP32 VALU element kernel, FP64 reduction */
+
+/* Coefficient program opcodes (one instruction = {op, dst, a, b, imm}). */
+enum slx_opcode {
+    SLX_OP_CONST = 0,   /* dst = consts[imm] */
+    SLX_OP_PARAM = 1,   /* dst = elem_params[imm][element] */
+    SLX_OP_JAC = 2,     /* dst = J[a][b]  (a: reference derivative, b: physical coordinate) */
+    SLX_OP_INVJAC = 3,  /* dst = inverse(J)[a][b] */
+    SLX_OP_DETJ = 4,    /* dst = det(J) (signed) */
+    SLX_OP_COORD = 5,   /* dst = physical coordinate a at the Gauss point */
+    SLX_OP_FIELD = 6,   /* dst = field imm, space derivative a (0 value, 1..3 d/dx,y,z) */
+    SLX_OP_ADD = 7,     /* dst = r[a] + r[b] */
+    SLX_OP_MUL = 8,     /* dst = r[a] * r[b] */
+    SLX_OP_INV = 9,     /* dst = 1 / r[a] */
+    SLX_OP_POW = 10,    /* dst = pow(r[a], r[b]) */
+    SLX_OP_ABS = 11,
+    SLX_OP_EXP = 12,
+    SLX_OP_LOG = 13,
+    SLX_OP_SIN = 14,
+    SLX_OP_COS = 15,
+    SLX_OP_TAN = 16,
+    SLX_OP_SQRT = 17,
+    SLX_OP_OUT = 18,    /* coefficient slot imm = r[a] (the kernel multiplies by |det J|) */
+    SLX_OP_COUNT = 19
+};
+
+typedef struct slx_instr {
+    int16_t op, dst, a, b;
+    int32_t imm;
+} slx_instr;
+
+/* One term of a slice: tf derivative x dof derivative with a coefficient
+ * register (extractdoftf entries, src/expression/expression.cpp:2253-2281). */
+typedef struct slx_term {
+    int32_t tf_deriv;   /* 0 value, 1 d/dki, 2 d/deta, 3 d/dphi */
+    int32_t dof_deriv;  /* same; ignored for right-hand-side slices */
+    int32_t coef;       /* coefficient slot written by SLX_OP_OUT */
+} slx_term;
+
+/* A slice = one (tf field, dof field) pair of a contribution inside a bucket. */
+typedef struct slx_slice_desc {
+    int32_t frag_rank;        /* position of this fragment in the reference
+                                 accumulation order (rawmat fragment order) */
+    int32_t is_rhs;           /* 1: no dof, contributes -coef*tf to the rhs */
+    int32_t n_tf, n_dof;      /* local form functions (rows / columns) */
+    const int32_t* tf_addr;   /* [n_tf][n_elem] row-major, -1 = skipped   */
+    const int32_t* dof_addr;  /* [n_dof][n_elem], NULL for rhs slices    */
+    int32_t n_terms;
+    const slx_term* terms;
+    int32_t tf_table, dof_table; /* indices into the bucket's shape tables */
+} slx_slice_desc;
+
+/* A bucket: elements of one type sharing Gauss rule, orders and (when the
+ * basis depends on it) total orientation (elementselector.cpp:4-133). */
+typedef struct slx_bucket_desc {
+    int32_t elem_type;          /* 1 line, 2 tri, 3 quad, 4 tet, 5 hex */
+    int32_t n_elem;
+    int32_t n_geo;              /* corner (order-1 geometry) nodes per element */
+    const int32_t* geo_nodes;   /* [n_elem][n_geo] node numbers */
+    const int32_t* elem_ids;    /* [n_elem] element numbers (index into per-element parameters) */
+    int32_t n_gp;
+    const double* gp_coords;    /* [n_gp][3] reference coordinates */
+    const double* gp_weights;   /* [n_gp] */
+    int32_t n_tables;
+    const int32_t* table_nff;   /* [n_tables] functions per table */
+    const double* const* tables;/* [n_tables] -> [4 derivs][nff][n_gp] */
+    int32_t n_slices;
+    const slx_slice_desc* slices;
+    int32_t n_instr;
+    const slx_instr* prog;      /* coefficient program, run per (element, Gauss point) */
+    int32_t n_regs;             /* registers the program uses */
+    int32_t n_coefs;            /* coefficient slots written by SLX_OP_OUT */
+    int32_t q_invariant;        /* 1: program value identical at every Gauss point */
+} slx_bucket_desc;
+
+typedef struct slx_plan_desc {
+    int32_t mesh_dim;           /* problem (mesh) dimension, 1..3 */
+    int64_t n_nodes;
+    int64_t n_dofs;             /* dofmanager::countdofs */
+    const uint8_t* constrained; /* [n_dofs] 1 = Dirichlet-constrained DOF */
+    int32_t n_consts;
+    const double* consts;
+    int32_t n_buckets;
+    const slx_bucket_desc* buckets;
+    int32_t n_fields;           /* fields read by SLX_OP_FIELD */
+    const int32_t* field_nff;   /* per field: form functions per element, per bucket [n_fields*n_buckets] */
+    const int32_t* const* field_addr;   /* [n_fields*n_buckets] -> [nff][n_elem] addresses into field data */
+    const double* const* field_tables;  /* [n_fields*n_buckets] -> [4][nff][n_gp] */
+    int32_t has_rhs, has_matrix;
+} slx_plan_desc;
+
+/* Numeric inputs of one assembly (host pointers; copied to the device). */
+typedef struct slx_numeric_desc {
+    const double* node_xyz;       /* [n_nodes][3] */
+    int32_t n_params;
+    int64_t param_len;            /* entries per parameter array */
+    const double* params;         /* [n_params][param_len] per-element values */
+    const double* const* field_data;  /* [n_fields] coefficient arrays */
+    const int64_t* field_len;     /* [n_fields] */
+    int32_t precision;            /* SLX_PREC_* */
+} slx_numeric_desc;
+
+typedef struct slx_ctx_s* slx_ctx;
+typedef struct slx_plan_s* slx_plan;
+
+const char* slx_last_error(void);
+
+slx_status slx_create(int device, slx_ctx* out);
+slx_status slx_destroy(slx_ctx ctx);
+
+/* Symbolic phase: copies the description, builds the element-row lists, the
+ * CSR pattern (per-row sorted unique columns = sorted COO keys), the A/D
+ * split with local renumbering and the reduction maps.  Pattern equals
+ * rawmat::process (rawmat.cpp:127-298) on the same address tables. */
+slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out);
+slx_status slx_plan_shape(slx_plan plan, int64_t* nfree, int64_t* ncons, int64_t* nnzA, int64_t* nnzD);
+/* Numeric phase: uploads the numeric inputs and assembles A, D and b on the
+ * plan's stream (asynchronous; slx_plan_sync waits). */
+slx_status slx_assemble(slx_plan plan, const slx_numeric_desc* in);
+/* Numeric phase on inputs already resident on the device (no host copies,
+ * no allocation): device pointers laid out like slx_numeric_desc.  This is
+ * the entry the timed loops and graph capture use. */
+slx_status slx_assemble_device(slx_plan plan, const double* d_node_xyz, const double* d_params,
+                               int64_t param_len, const double* const* d_field_data, int32_t precision,
+                               void* hip_stream);
+slx_status slx_plan_sync(slx_plan plan);
+/* Phase timing on the plan's stream with HIP events: runs `iters` numeric
+ * phases on device-resident inputs and returns the average milliseconds of
+ * the element kernels (ms[0]) and of the reduction (ms[1]). */
+slx_status slx_plan_profile(slx_plan plan, const double* d_node_xyz, const double* d_params, int64_t param_len,
+                            const double* const* d_field_data, int32_t precision, int32_t iters, double* ms);
+/* Device views of the assembled CSR (valid until slx_plan_destroy). */
+slx_status slx_csr_device(slx_plan plan, const int64_t** Arow, const int32_t** Acol, const double** Aval,
+                          const int64_t** Drow, const int32_t** Dcol, const double** Dval, const double** rhs);
+/* Copies to caller-owned host arrays (any pointer may be NULL). */
+slx_status slx_csr_copyout(slx_plan plan, int64_t* Arow, int32_t* Acol, double* Aval, int64_t* Drow,
+                           int32_t* Dcol, double* Dval, double* rhs);
+/* Element matrices of the last assembly, in the reduction buffer layout
+ * (for error measurement); n_out receives the entry count. */
+slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, int64_t* n_coo);
+slx_status slx_plan_destroy(slx_plan plan);
+
+/* ---- Formulation layer ---------------------------------------------------- */
+typedef struct slx_problem_s* slx_problem;
+
+/* Synthetic workloads of the benchmark suite and small test problems.
+ * kind: "poisson2d" (P1 tris, unit square grid n x n, jitter, seed),
+ *       "elasticity3d" (P2 tets, h1xyz, n^3 cubes x 6 tets),
+ *       "hppoisson3d" (hex, per-element order 1..5),
+ *       "quadbatch" (independent quad instances, K + k^2 M),
+ *       "newton2d" (P2 tris, nonlinear reluctivity), "msh" (a gmsh 2.2 file).
+ * The spec string is "key=value;key=value" (n, seed, jitter, order, ...). */
+slx_status slx_problem_create(slx_ctx ctx, const char* kind, const char* spec, slx_problem* out);
+slx_status slx_problem_destroy(slx_problem p);
+/* Build (or reuse) the assembly plan of the problem's formulation. */
+slx_status slx_problem_plan(slx_problem p, slx_plan* plan_out);
+/* The numeric inputs of the problem (host pointers owned by the problem). */
+slx_status slx_problem_numeric(slx_problem p, slx_numeric_desc* out);
+/* formulation::generate() + A()/b(): assemble with the given precision. */
+slx_status slx_problem_generate(slx_problem p, int32_t precision);
+/* Sizes: dofs, free dofs, elements, buckets. */
+slx_status slx_problem_info(slx_problem p, int64_t* ndofs, int64_t* nfree, int64_t* nelem, int64_t* nbuckets);
+/* The plan description (for the CPU oracle), valid while the problem lives. */
+slx_status slx_problem_desc(slx_problem p, const slx_plan_desc** desc);
+/* Newton: set the field data used by SLX_OP_FIELD (setdata). */
+slx_status slx_problem_setdata(slx_problem p, int32_t field, const double* values, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SLX_H */

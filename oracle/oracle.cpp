@@ -1,0 +1,399 @@
+// CPU FP64 oracle for the weak-form assembly path.  TEST INFRASTRUCTURE ONLY:
+// it is loaded by tests/, by __graft_entry__.smoke() and by the cpu_baseline
+// leg of bench.py, as the checker and the CPU baseline; the product never
+// calls it.
+//
+// Restates, on the plan description of include/slx.h:
+//   contribution::generate   src/formulation/contribution.cpp:133-306
+//     - Jacobian, |det J|, inverse         src/field/jacobian.cpp:4-505
+//     - coefficient per term at every GP   contribution.cpp:155-163 (operation::interpolate)
+//     - tf*w (multiplycolumns), tf x dof   contribution.cpp:172-187, densemat.cpp:608-636
+//     - coef*|detJ|, transpose, GEMM       contribution.cpp:195-211, densemat.cpp:228-311
+//     - first term copied, next ones added contribution.cpp:241-245 (getproduct/addproduct)
+//     - rhs: minus, VecSetValues ADD       contribution.cpp:293-297, rawvec.cpp:216-254
+//   rawmat::process                        src/formulation/rawmat.cpp:127-298
+//     - per-row bucketing in fragment -> row r -> element order
+//     - per-row stable sort by column (ties by position), sequential FP64 sum
+//     - split into A (free columns) and D (constrained columns), local numbers
+// The fragments are processed in frag_rank order, which is the order in which
+// formulation::generate pushes them into the rawmat (formulation.cpp:176-218).
+//
+// The GEMM inner product runs over the Gauss points in increasing order with
+// plain multiply-add, like the reference's non-BLAS path (densemat.cpp:285-307);
+// OpenBLAS's summation order for larger shapes is not pinned (parity unpinned
+// at the dgemm boundary: values are compared with a relative tolerance).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+#include "../include/slx.h"
+
+namespace {
+
+struct Frag {
+    int bucket, slice, rank;
+};
+
+void jacobian(const slx_plan_desc& d, const slx_bucket_desc& b, const double* xyz, const std::vector<double>& geo, int e, int q,
+              double J[3][3], double inv[3][3], double& det) {
+    int edim = b.elem_type == 1 ? 1 : (b.elem_type <= 3 ? 2 : 3);
+    int md = d.mesh_dim;
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) J[r][c] = 0.0;
+    for (int r = 0; r < edim; r++)
+        for (int c = 0; c < md; c++) {
+            double s = 0.0;
+            for (int k = 0; k < b.n_geo; k++) {
+                int node = b.geo_nodes[(size_t)e * b.n_geo + k];
+                s += xyz[3 * (size_t)node + c] * geo[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q];
+            }
+            J[r][c] = s;
+        }
+    if (md == 1) {
+        det = J[0][0];
+        inv[0][0] = 1.0 / J[0][0];
+    } else if (md == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        inv[0][0] = J[1][1] / det;
+        inv[0][1] = -J[0][1] / det;
+        inv[1][0] = -J[1][0] / det;
+        inv[1][1] = J[0][0] / det;
+    } else {
+        det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
+        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+}
+
+// Order-1 Lagrange functions and reference derivatives: out[deriv][node][pt].
+void lagrange1(int type, const double* pts, int np, std::vector<double>& out) {
+    int nn = type == 1 ? 2 : type == 2 ? 3 : type == 3 ? 4 : type == 4 ? 4 : 8;
+    out.assign((size_t)4 * nn * np, 0.0);
+    for (int p = 0; p < np; p++) {
+        double k = pts[3 * p], e = pts[3 * p + 1], f = pts[3 * p + 2];
+        double v[8][4];
+        if (type == 1) {
+            double x = 0.5 * (k + 1);
+            double vv[2][4] = {{1 - x, -0.5, 0, 0}, {x, 0.5, 0, 0}};
+            std::memcpy(v, vv, sizeof(vv));
+        } else if (type == 2) {
+            double vv[3][4] = {{1 - k - e, -1, -1, 0}, {k, 1, 0, 0}, {e, 0, 1, 0}};
+            std::memcpy(v, vv, sizeof(vv));
+        } else if (type == 4) {
+            double vv[4][4] = {{1 - k - e - f, -1, -1, -1}, {k, 1, 0, 0}, {e, 0, 1, 0}, {f, 0, 0, 1}};
+            std::memcpy(v, vv, sizeof(vv));
+        } else if (type == 3) {
+            double sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+            for (int n = 0; n < 4; n++) {
+                v[n][0] = 0.25 * (1 + sx[n] * k) * (1 + sy[n] * e);
+                v[n][1] = 0.25 * sx[n] * (1 + sy[n] * e);
+                v[n][2] = 0.25 * sy[n] * (1 + sx[n] * k);
+                v[n][3] = 0;
+            }
+        } else {
+            double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1}, sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+            for (int n = 0; n < 8; n++) {
+                v[n][0] = 0.125 * (1 + sx[n] * k) * (1 + sy[n] * e) * (1 + sz[n] * f);
+                v[n][1] = 0.125 * sx[n] * (1 + sy[n] * e) * (1 + sz[n] * f);
+                v[n][2] = 0.125 * sy[n] * (1 + sx[n] * k) * (1 + sz[n] * f);
+                v[n][3] = 0.125 * sz[n] * (1 + sx[n] * k) * (1 + sy[n] * e);
+            }
+        }
+        for (int n = 0; n < nn; n++)
+            for (int dd = 0; dd < 4; dd++) out[((size_t)dd * nn + n) * np + p] = v[n][dd];
+    }
+}
+
+// operation::interpolate of every coefficient at one (element, GP):
+// evaluates the program's expression DAG in FP64.
+void coefficients(const slx_plan_desc& d, const slx_bucket_desc& b, const slx_numeric_desc& in, const std::vector<double>& geo,
+                  int e, int q, const double J[3][3], const double inv[3][3], double det, std::vector<double>& coef) {
+    std::vector<double> reg(std::max(b.n_regs, 1));
+    int edim = b.elem_type == 1 ? 1 : (b.elem_type <= 3 ? 2 : 3);
+    for (int pc = 0; pc < b.n_instr; pc++) {
+        const slx_instr& ins = b.prog[pc];
+        double va = (ins.a >= 0 && ins.a < b.n_regs) ? reg[ins.a] : 0.0;
+        double vb = (ins.b >= 0 && ins.b < b.n_regs) ? reg[ins.b] : 0.0;
+        double r = 0.0;
+        switch (ins.op) {
+            case SLX_OP_CONST: r = d.consts[ins.imm]; break;
+            case SLX_OP_PARAM: r = in.params[(size_t)ins.imm * in.param_len + b.elem_ids[e]]; break;
+            case SLX_OP_JAC: r = J[ins.a][ins.b]; break;
+            case SLX_OP_INVJAC: r = inv[ins.a][ins.b]; break;
+            case SLX_OP_DETJ: r = det; break;
+            case SLX_OP_COORD: {
+                for (int k = 0; k < b.n_geo; k++) {
+                    int node = b.geo_nodes[(size_t)e * b.n_geo + k];
+                    r += in.node_xyz[3 * (size_t)node + ins.a] * geo[(size_t)k * b.n_gp + q];
+                }
+                break;
+            }
+            case SLX_OP_FIELD: {
+                int bidx = (int)(&b - d.buckets);
+                size_t fk = (size_t)ins.imm * d.n_buckets + bidx;
+                int nff = d.field_nff[fk];
+                const int32_t* addr = d.field_addr[fk];
+                const double* tab = d.field_tables[fk];
+                double v[4] = {0, 0, 0, 0};
+                for (int k = 0; k < nff; k++) {
+                    int a = addr[(size_t)k * b.n_elem + e];
+                    if (a < 0) continue;
+                    double c = in.field_data[ins.imm][a];
+                    for (int dd = 0; dd < 4; dd++) v[dd] += c * tab[((size_t)dd * nff + k) * b.n_gp + q];
+                }
+                if (ins.a == 0) r = v[0];
+                else
+                    for (int i = 0; i < edim; i++) r += inv[ins.a - 1][i] * v[1 + i];
+                break;
+            }
+            case SLX_OP_ADD: r = va + vb; break;
+            case SLX_OP_MUL: r = va * vb; break;
+            case SLX_OP_INV: r = 1.0 / va; break;
+            case SLX_OP_POW: r = std::pow(va, vb); break;
+            case SLX_OP_ABS: r = std::fabs(va); break;
+            case SLX_OP_EXP: r = std::exp(va); break;
+            case SLX_OP_LOG: r = std::log(va); break;
+            case SLX_OP_SIN: r = std::sin(va); break;
+            case SLX_OP_COS: r = std::cos(va); break;
+            case SLX_OP_TAN: r = std::tan(va); break;
+            case SLX_OP_SQRT: r = std::sqrt(va); break;
+            case SLX_OP_OUT: coef[ins.imm] = va; continue;
+        }
+        reg[ins.dst] = r;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Full restatement: returns 0 on success.  Output arrays are sized by the
+// caller from oracle_shape(); rhs has n_dofs entries (constrained entries 0).
+// Element matrices of every fragment are also returned when ke_out is not
+// NULL, in the Ke layout of the plan (bucket, slice, element, i, j).
+int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nthreads, int64_t* nnzA_out, int64_t* nnzD_out,
+                    int64_t** Arow_out, int32_t** Acol_out, double** Aval_out, int64_t** Drow_out, int32_t** Dcol_out,
+                    double** Dval_out, double* rhs, double* ke_out);
+void oracle_free(void* p);
+
+int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nthreads, int64_t* nnzA_out, int64_t* nnzD_out,
+                    int64_t** Arow_out, int32_t** Acol_out, double** Aval_out, int64_t** Drow_out, int32_t** Dcol_out,
+                    double** Dval_out, double* rhs, double* ke_out) {
+    const slx_plan_desc& d = *dp;
+    const slx_numeric_desc& in = *inp;
+    const int64_t ndofs = d.n_dofs;
+    std::vector<Frag> frags;
+    std::vector<std::vector<int64_t>> kebase(d.n_buckets);
+    int64_t keoff = 0;
+    for (int b = 0; b < d.n_buckets; b++) {
+        kebase[b].resize(d.buckets[b].n_slices);
+        for (int s = 0; s < d.buckets[b].n_slices; s++) {
+            const slx_slice_desc& sd = d.buckets[b].slices[s];
+            frags.push_back({b, s, sd.frag_rank});
+            kebase[b][s] = keoff;
+            keoff += (int64_t)d.buckets[b].n_elem * sd.n_tf * (sd.is_rhs ? 1 : sd.n_dof);
+        }
+    }
+    // element matrices of each fragment: vals[(i*n_dof + j) * n_elem + e] (rawmat row-major)
+    std::vector<std::vector<double>> vals(frags.size());
+    for (size_t f = 0; f < frags.size(); f++) {
+        const slx_bucket_desc& b = d.buckets[frags[f].bucket];
+        const slx_slice_desc& sd = b.slices[frags[f].slice];
+        int ne = b.n_elem, ng = b.n_gp, nt = sd.n_tf, ndf = sd.is_rhs ? 1 : sd.n_dof;
+        std::vector<double> geo;
+        lagrange1(b.elem_type, b.gp_coords, ng, geo);
+        // coefficient of every term at every (e, q): coefs[t][e][q]
+        std::vector<std::vector<double>> C(sd.n_terms, std::vector<double>((size_t)ne * ng));
+        std::vector<double> cbuf(std::max(b.n_coefs, 1));
+        for (int e = 0; e < ne; e++)
+            for (int q = 0; q < ng; q++) {
+                double J[3][3], inv[3][3], det;
+                jacobian(d, b, in.node_xyz, geo, e, q, J, inv, det);
+                coefficients(d, b, in, geo, e, q, J, inv, det, cbuf);
+                for (int t = 0; t < sd.n_terms; t++) C[t][(size_t)e * ng + q] = cbuf[sd.terms[t].coef] * std::fabs(det);
+            }
+        std::vector<double>& V = vals[f];
+        V.assign((size_t)nt * ndf * ne, 0.0);
+        const double* tft = b.tables[sd.tf_table];
+        const double* dft = sd.is_rhs ? nullptr : b.tables[sd.dof_table];
+        int nff_tf = b.table_nff[sd.tf_table], nff_dof = sd.is_rhs ? 1 : b.table_nff[sd.dof_table];
+        std::vector<double> phi((size_t)nt * ndf * ng), term((size_t)nt * ndf * ne);
+        for (int t = 0; t < sd.n_terms; t++) {
+            const slx_term& tm = sd.terms[t];
+            // Phi[(i*ndf + j)][q] = tf[i][q] * w[q] * dof[j][q]
+            for (int i = 0; i < nt; i++)
+                for (int j = 0; j < ndf; j++)
+                    for (int q = 0; q < ng; q++) {
+                        double tv = tft[((size_t)tm.tf_deriv * nff_tf + i) * ng + q] * b.gp_weights[q];
+                        double dv = sd.is_rhs ? 1.0 : dft[((size_t)tm.dof_deriv * nff_dof + j) * ng + q];
+                        phi[((size_t)i * ndf + j) * ng + q] = tv * dv;
+                    }
+            // term = Phi * C^T  (M = nt*ndf, K = ng, N = ne)
+            for (int r = 0; r < nt * ndf; r++)
+                for (int e = 0; e < ne; e++) {
+                    double s = 0.0;
+                    for (int q = 0; q < ng; q++) s += phi[(size_t)r * ng + q] * C[t][(size_t)e * ng + q];
+                    if (t == 0) V[(size_t)r * ne + e] = s;
+                    else V[(size_t)r * ne + e] += s;
+                }
+        }
+        if (sd.is_rhs)
+            for (double& v : V) v = -v;
+        if (ke_out) {
+            for (int e = 0; e < ne; e++)
+                for (int i = 0; i < nt; i++)
+                    for (int j = 0; j < ndf; j++)
+                        ke_out[kebase[frags[f].bucket][frags[f].slice] + ((int64_t)e * nt + i) * ndf + j] = V[((size_t)i * ndf + j) * ne + e];
+        }
+    }
+    std::vector<size_t> order(frags.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return frags[a].rank < frags[b].rank; });
+    // rhs: rawvec::setvalues "add" in fragment order, tf-function-major
+    if (rhs) {
+        std::fill(rhs, rhs + ndofs, 0.0);
+        for (size_t oi : order) {
+            const slx_bucket_desc& b = d.buckets[frags[oi].bucket];
+            const slx_slice_desc& sd = b.slices[frags[oi].slice];
+            if (!sd.is_rhs) continue;
+            for (int i = 0; i < sd.n_tf; i++)
+                for (int e = 0; e < b.n_elem; e++) {
+                    int a = sd.tf_addr[(size_t)i * b.n_elem + e];
+                    if (a >= 0) rhs[a] += vals[oi][(size_t)i * b.n_elem + e];
+                }
+        }
+        // vec::updateconstraints: constrained entries take the (zero) Dirichlet value
+        for (int64_t i = 0; i < ndofs; i++)
+            if (d.constrained && d.constrained[i]) rhs[i] = 0.0;
+    }
+    // rawmat::process
+    std::vector<char> isc(ndofs, 0);
+    for (int64_t i = 0; i < ndofs; i++) isc[i] = d.constrained ? d.constrained[i] : 0;
+    std::vector<int64_t> renum(ndofs);
+    int64_t nt_ = 0, nf_ = 0;
+    for (int64_t i = 0; i < ndofs; i++) renum[i] = isc[i] ? nt_++ : nf_++;
+    std::vector<int64_t> maxnnz(ndofs, 0);
+    int64_t total = 0;
+    for (size_t oi : order) {
+        const slx_bucket_desc& b = d.buckets[frags[oi].bucket];
+        const slx_slice_desc& sd = b.slices[frags[oi].slice];
+        if (sd.is_rhs) continue;
+        int nr = sd.n_tf * sd.n_dof, nc = b.n_elem;
+        for (int r = 0; r < nr; r++)
+            for (int c = 0; c < nc; c++) {
+                int cr = sd.tf_addr[(size_t)(r / sd.n_dof) * nc + c], cc = sd.dof_addr[(size_t)(r % sd.n_dof) * nc + c];
+                if (cr >= 0 && cc >= 0 && !isc[cr]) { maxnnz[cr]++; total++; }
+            }
+    }
+    std::vector<int64_t> ads(ndofs + 1, 0);
+    for (int64_t i = 0; i < ndofs; i++) ads[i + 1] = ads[i] + maxnnz[i];
+    std::vector<std::pair<int, double>> pairs(total);
+    std::vector<int64_t> fill(ads.begin(), ads.end() - 1);
+    for (size_t oi : order) {
+        const slx_bucket_desc& b = d.buckets[frags[oi].bucket];
+        const slx_slice_desc& sd = b.slices[frags[oi].slice];
+        if (sd.is_rhs) continue;
+        int nr = sd.n_tf * sd.n_dof, nc = b.n_elem;
+        for (int r = 0; r < nr; r++)
+            for (int c = 0; c < nc; c++) {
+                int cr = sd.tf_addr[(size_t)(r / sd.n_dof) * nc + c], cc = sd.dof_addr[(size_t)(r % sd.n_dof) * nc + c];
+                if (cr >= 0 && cc >= 0 && !isc[cr]) pairs[fill[cr]++] = {cc, vals[oi][(size_t)r * nc + c]};
+            }
+    }
+    // processrows: stable per-row sort by column, sequential duplicate sum
+    std::vector<int64_t> rowlen(ndofs, 0);
+    auto processrows = [&](int64_t r0, int64_t r1) {
+        std::vector<int> idx;
+        std::vector<std::pair<int, double>> tmp;
+        for (int64_t i = r0; i < r1; i++) {
+            int64_t n = maxnnz[i];
+            std::pair<int, double>* p = pairs.data() + ads[i];
+            idx.resize(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+                if (p[a].first != p[b].first) return p[a].first < p[b].first;
+                return a < b;
+            });
+            tmp.assign(p, p + n);
+            int pc = -1;
+            int64_t ind = -1;
+            for (int64_t j = 0; j < n; j++) {
+                int cc = tmp[idx[j]].first;
+                double cv = tmp[idx[j]].second;
+                if (cc == pc) p[ind].second += cv;
+                else {
+                    ind++;
+                    p[ind] = {cc, cv};
+                }
+                pc = cc;
+            }
+            rowlen[i] = ind + 1;
+        }
+    };
+    if (nthreads <= 1) processrows(0, ndofs);
+    else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nthreads; t++) th.emplace_back(processrows, ndofs * t / nthreads, ndofs * (t + 1) / nthreads);
+        for (auto& x : th) x.join();
+    }
+    // A / D split
+    int64_t nnzA = 0, nnzD = 0;
+    for (int64_t i = 0; i < ndofs; i++) {
+        if (isc[i]) continue;
+        for (int64_t j = 0; j < rowlen[i]; j++) (isc[pairs[ads[i] + j].first] ? nnzD : nnzA)++;
+    }
+    int64_t nfree = nf_;
+    int64_t* Arow = (int64_t*)std::malloc(sizeof(int64_t) * (nfree + 1));
+    int64_t* Drow = (int64_t*)std::malloc(sizeof(int64_t) * (nfree + 1));
+    int32_t* Acol = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(nnzA, 1));
+    int32_t* Dcol = (int32_t*)std::malloc(sizeof(int32_t) * std::max<int64_t>(nnzD, 1));
+    double* Aval = (double*)std::malloc(sizeof(double) * std::max<int64_t>(nnzA, 1));
+    double* Dval = (double*)std::malloc(sizeof(double) * std::max<int64_t>(nnzD, 1));
+    Arow[0] = 0;
+    Drow[0] = 0;
+    int64_t index = 0;
+    for (int64_t i = 0; i < ndofs; i++) {
+        if (isc[i]) continue;
+        Arow[index + 1] = Arow[index];
+        Drow[index + 1] = Drow[index];
+        for (int64_t j = 0; j < rowlen[i]; j++) {
+            int cc = pairs[ads[i] + j].first;
+            double cv = pairs[ads[i] + j].second;
+            if (isc[cc]) {
+                Dcol[Drow[index + 1]] = (int32_t)renum[cc];
+                Dval[Drow[index + 1]] = cv;
+                Drow[index + 1]++;
+            } else {
+                Acol[Arow[index + 1]] = (int32_t)renum[cc];
+                Aval[Arow[index + 1]] = cv;
+                Arow[index + 1]++;
+            }
+        }
+        index++;
+    }
+    *nnzA_out = nnzA;
+    *nnzD_out = nnzD;
+    *Arow_out = Arow;
+    *Acol_out = Acol;
+    *Aval_out = Aval;
+    *Drow_out = Drow;
+    *Dcol_out = Dcol;
+    *Dval_out = Dval;
+    return 0;
+}
+
+void oracle_free(void* p) { std::free(p); }
+
+}  // extern "C"

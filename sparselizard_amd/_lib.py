@@ -1,0 +1,249 @@
+"""ctypes binding of the slx C ABI (include/slx.h).
+
+The library is built in-tree (sparselizard_amd/lib/libslx.so).  Loading
+fails loudly when it is missing: there is no CPU fallback for the assembly
+path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libslx.so")
+
+SLX_OK = 0
+PREC_FP64 = 0
+PREC_F16_SPLIT = 1
+PREC_BF16_SPLIT = 2
+PREC_FP32 = 3
+PRECISIONS = {"fp64": PREC_FP64, "f16split": PREC_F16_SPLIT, "bf16split": PREC_BF16_SPLIT, "fp32": PREC_FP32}
+
+
+class NumericDesc(C.Structure):
+    _fields_ = [
+        ("node_xyz", C.POINTER(C.c_double)),
+        ("n_params", C.c_int32),
+        ("param_len", C.c_int64),
+        ("params", C.POINTER(C.c_double)),
+        ("field_data", C.POINTER(C.POINTER(C.c_double))),
+        ("field_len", C.POINTER(C.c_int64)),
+        ("precision", C.c_int32),
+    ]
+
+
+class PlanDescHead(C.Structure):
+    """Leading fields of slx_plan_desc."""
+    _fields_ = [("mesh_dim", C.c_int32), ("n_nodes", C.c_int64), ("n_dofs", C.c_int64)]
+
+
+class SlxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libslx.so (once)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SlxError(
+            "libslx.so is not built (%s); run __graft_entry__.build() or `make -C sparselizard_amd`" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    i32, i64 = C.c_int32, C.c_int64
+    p64, p32, pd = C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.POINTER(C.c_double)
+    sig = {
+        "slx_last_error": (C.c_char_p, []),
+        "slx_create": (i32, [C.c_int, C.POINTER(vp)]),
+        "slx_destroy": (i32, [vp]),
+        "slx_plan_create": (i32, [vp, vp, C.POINTER(vp)]),
+        "slx_plan_shape": (i32, [vp, p64, p64, p64, p64]),
+        "slx_plan_stats": (i32, [vp, p64, p64, p64]),
+        "slx_assemble": (i32, [vp, C.POINTER(NumericDesc)]),
+        "slx_assemble_device": (i32, [vp, vp, vp, i64, vp, i32, vp]),
+        "slx_plan_sync": (i32, [vp]),
+        "slx_plan_profile": (i32, [vp, vp, vp, i64, vp, i32, i32, pd]),
+        "slx_csr_device": (i32, [vp] + [C.POINTER(vp)] * 7),
+        "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
+        "slx_plan_destroy": (i32, [vp]),
+        "slx_problem_create": (i32, [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]),
+        "slx_problem_destroy": (i32, [vp]),
+        "slx_problem_plan": (i32, [vp, C.POINTER(vp)]),
+        "slx_problem_numeric": (i32, [vp, C.POINTER(NumericDesc)]),
+        "slx_problem_generate": (i32, [vp, i32]),
+        "slx_problem_info": (i32, [vp, p64, p64, p64, p64]),
+        "slx_problem_desc": (i32, [vp, C.POINTER(vp)]),
+        "slx_problem_setdata": (i32, [vp, i32, pd, i64]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+EXPORTED = [
+    "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
+    "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
+    "slx_plan_destroy", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata",
+]
+
+
+def check(status):
+    if status != SLX_OK:
+        raise SlxError("slx error %d: %s" % (status, lib().slx_last_error().decode()))
+
+
+def _p(arr, ctype):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+class Context:
+    """One device + stream (slx_create)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().slx_create(device, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().slx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class CSR:
+    """Host copy of an assembled A/D block pair and the rhs."""
+
+    def __init__(self, Arow, Acol, Aval, Drow, Dcol, Dval, rhs):
+        self.Arow, self.Acol, self.Aval = Arow, Acol, Aval
+        self.Drow, self.Dcol, self.Dval = Drow, Dcol, Dval
+        self.rhs = rhs
+
+    @property
+    def nfree(self):
+        return len(self.Arow) - 1
+
+
+class Problem:
+    """A synthetic workload built by the C++ host layer (slx_problem_*)."""
+
+    def __init__(self, ctx, kind, spec=""):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().slx_problem_create(ctx.h if ctx else None, kind.encode(), spec.encode(), C.byref(self.h)))
+        self._plan = None
+
+    def info(self):
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().slx_problem_info(self.h, *[C.byref(x) for x in v]))
+        return dict(ndofs=v[0].value, nfree=v[1].value, nelem=v[2].value, nbuckets=v[3].value)
+
+    def desc_ptr(self):
+        d = C.c_void_p()
+        check(lib().slx_problem_desc(self.h, C.byref(d)))
+        return d
+
+    def n_nodes(self):
+        return C.cast(self.desc_ptr(), C.POINTER(PlanDescHead)).contents.n_nodes
+
+    def numeric(self):
+        nd = NumericDesc()
+        check(lib().slx_problem_numeric(self.h, C.byref(nd)))
+        return nd
+
+    def plan(self):
+        if self._plan is None:
+            p = C.c_void_p()
+            check(lib().slx_problem_plan(self.h, C.byref(p)))
+            self._plan = Plan(p, owned=False)
+        return self._plan
+
+    def generate(self, precision="fp64"):
+        check(lib().slx_problem_generate(self.h, PRECISIONS[precision] if isinstance(precision, str) else precision))
+        return self.plan()
+
+    def setdata(self, field, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().slx_problem_setdata(self.h, field, _p(v, C.c_double), len(v)))
+
+    def close(self):
+        if self.h:
+            lib().slx_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Plan:
+    def __init__(self, h, owned=True):
+        self.h = h
+        self.owned = owned
+
+    def shape(self):
+        v = [C.c_int64() for _ in range(4)]
+        check(lib().slx_plan_shape(self.h, *[C.byref(x) for x in v]))
+        return dict(nfree=v[0].value, ncons=v[1].value, nnzA=v[2].value, nnzD=v[3].value)
+
+    def stats(self):
+        v = [C.c_int64() for _ in range(3)]
+        check(lib().slx_plan_stats(self.h, *[C.byref(x) for x in v]))
+        return dict(n_elem_rows=v[0].value, n_ke=v[1].value, n_coo=v[2].value)
+
+    def assemble(self, numeric, precision):
+        numeric.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        check(lib().slx_assemble(self.h, C.byref(numeric)))
+
+    def assemble_device(self, d_xyz, d_params, param_len, d_fields, precision, stream=None):
+        check(lib().slx_assemble_device(self.h, d_xyz, d_params, param_len, d_fields,
+                                        PRECISIONS[precision] if isinstance(precision, str) else precision, stream))
+
+    def profile(self, d_xyz, d_params, param_len, d_fields, precision, iters=5):
+        ms = np.zeros(2, np.float64)
+        check(lib().slx_plan_profile(self.h, d_xyz, d_params, param_len, d_fields,
+                                     PRECISIONS[precision] if isinstance(precision, str) else precision, iters,
+                                     _p(ms, C.c_double)))
+        return float(ms[0]), float(ms[1])
+
+    def sync(self):
+        check(lib().slx_plan_sync(self.h))
+
+    def csr(self, ndofs):
+        s = self.shape()
+        nf = s["nfree"]
+        Arow = np.zeros(nf + 1, np.int64)
+        Drow = np.zeros(nf + 1, np.int64)
+        Acol = np.zeros(max(s["nnzA"], 1), np.int32)
+        Dcol = np.zeros(max(s["nnzD"], 1), np.int32)
+        Aval = np.zeros(max(s["nnzA"], 1), np.float64)
+        Dval = np.zeros(max(s["nnzD"], 1), np.float64)
+        rhs = np.zeros(max(ndofs, 1), np.float64)
+        check(lib().slx_csr_copyout(self.h, _p(Arow, C.c_int64), _p(Acol, C.c_int32), _p(Aval, C.c_double),
+                                    _p(Drow, C.c_int64), _p(Dcol, C.c_int32), _p(Dval, C.c_double),
+                                    _p(rhs, C.c_double)))
+        return CSR(Arow, Acol[:s["nnzA"]], Aval[:s["nnzA"]], Drow, Dcol[:s["nnzD"]], Dval[:s["nnzD"]], rhs[:ndofs])
+
+    def device_ptrs(self):
+        ptrs = [C.c_void_p() for _ in range(7)]
+        check(lib().slx_csr_device(self.h, *[C.byref(x) for x in ptrs]))
+        return dict(zip(["Arow", "Acol", "Aval", "Drow", "Dcol", "Dval", "rhs"], [p.value for p in ptrs]))
+
+    def close(self):
+        if self.owned and self.h:
+            lib().slx_plan_destroy(self.h)
+        self.h = None

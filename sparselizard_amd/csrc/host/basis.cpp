@@ -1,0 +1,337 @@
+// Hierarchical H1 functions evaluated with forward-mode derivatives.
+//
+// The reference builds each function as an explicit 3-variate polynomial and
+// evaluates the polynomial and its derivative polynomials
+// (hierarchicalformfunctioncontainer.cpp:49-76).  Here each function is
+// evaluated directly from its defining formula on dual numbers carrying the
+// three reference partial derivatives; values agree to rounding.
+#include "basis.h"
+#include <cmath>
+#include <functional>
+
+namespace slx {
+
+namespace {
+
+struct D {
+    double v, d[3];
+    D(double x = 0.0) : v(x), d{0, 0, 0} {}
+};
+inline D var(double x, int k) { D r(x); r.d[k] = 1.0; return r; }
+inline D operator+(const D& a, const D& b) { D r(a.v + b.v); for (int k = 0; k < 3; k++) r.d[k] = a.d[k] + b.d[k]; return r; }
+inline D operator-(const D& a, const D& b) { D r(a.v - b.v); for (int k = 0; k < 3; k++) r.d[k] = a.d[k] - b.d[k]; return r; }
+inline D operator*(const D& a, const D& b) { D r(a.v * b.v); for (int k = 0; k < 3; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k]; return r; }
+inline D operator*(double s, const D& a) { D r(s * a.v); for (int k = 0; k < 3; k++) r.d[k] = s * a.d[k]; return r; }
+[[maybe_unused]] inline D operator+(double s, const D& a) { D r(a); r.v += s; return r; }
+inline D operator-(double s, const D& a) { D r(s - a.v); for (int k = 0; k < 3; k++) r.d[k] = -a.d[k]; return r; }
+
+// Legendre families of src/shapefunction/hierarchical/legendre.cpp.
+// l: Legendre polynomials; L: integrated Legendre; ls/Ls: their scaled forms.
+std::vector<D> leg_l(int maxn, const D& x) {
+    std::vector<D> l(maxn + 1);
+    l[0] = D(1.0);
+    if (maxn > 0) l[1] = x;
+    for (int n = 1; n < maxn; n++) l[n + 1] = (1.0 / (n + 1)) * ((double)(2 * n + 1) * l[n] * x - (double)n * l[n - 1]);
+    return l;
+}
+std::vector<D> leg_L(int maxn, const D& x) {
+    std::vector<D> L(std::max(maxn + 1, 3));
+    L[1] = x;
+    L[2] = 0.5 * (x * x - D(1.0));
+    for (int n = 2; n < maxn; n++) L[n + 1] = (1.0 / (n + 1)) * ((double)(2 * n - 1) * x * L[n] - (double)(n - 2) * L[n - 1]);
+    return L;
+}
+std::vector<D> leg_ls(int maxn, const D& x, const D& t) {
+    std::vector<D> ls(maxn + 1);
+    ls[0] = D(1.0);
+    if (maxn > 0) ls[1] = x;
+    for (int n = 1; n < maxn; n++) ls[n + 1] = (1.0 / (n + 1)) * ((double)(2 * n + 1) * x * ls[n] - (double)n * t * t * ls[n - 1]);
+    return ls;
+}
+std::vector<D> leg_Ls(int maxn, const D& x, const D& t) {
+    std::vector<D> Ls(std::max(maxn + 1, 3));
+    Ls[1] = x;
+    Ls[2] = 0.5 * (x * x - t * t);
+    for (int n = 2; n < maxn; n++) Ls[n + 1] = (1.0 / (n + 1)) * ((double)(2 * n - 1) * x * Ls[n] - (double)(n - 2) * t * t * Ls[n - 1]);
+    return Ls;
+}
+
+// Node reorderings bringing an edge / face to its reference orientation 0
+// (orientation.cpp:164-180).
+const int edge_reorder[2][2] = {{0, 1}, {1, 0}};
+const int tri_reorder[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}, {1, 0, 2}, {2, 0, 1}, {2, 1, 0}};
+const int quad_reorder[8][4] = {{0, 1, 2, 3}, {0, 3, 2, 1}, {1, 2, 3, 0}, {1, 0, 3, 2},
+                                {2, 3, 0, 1}, {2, 1, 0, 3}, {3, 0, 1, 2}, {3, 2, 1, 0}};
+
+// Barycentric / bilinear 'lambda' and 'sigma' functions of the element (1-based
+// like the reference; index 0 unused).
+void lambdas(int type, const D& ki, const D& eta, const D& phi, std::vector<D>& lam, std::vector<D>& sig) {
+    lam.assign(9, D(0.0));
+    sig.assign(9, D(0.0));
+    if (type == LINE) {
+        D x = 0.5 * (ki + 1.0);
+        lam[1] = 1.0 - x; lam[2] = x;
+        sig[1] = 1.0 - x; sig[2] = x;
+    } else if (type == TRI) {
+        lam[1] = 1.0 - ki - eta; lam[2] = ki; lam[3] = eta;
+    } else if (type == TET) {
+        lam[1] = 1.0 - ki - eta - phi; lam[2] = ki; lam[3] = eta; lam[4] = phi;
+    } else if (type == QUAD) {
+        D x = 0.5 * (ki + 1.0), y = 0.5 * (eta + 1.0);
+        lam[1] = (1.0 - x) * (1.0 - y); lam[2] = x * (1.0 - y); lam[3] = x * y; lam[4] = (1.0 - x) * y;
+        sig[1] = (1.0 - x) + (1.0 - y); sig[2] = x + (1.0 - y); sig[3] = x + y; sig[4] = (1.0 - x) + y;
+    } else if (type == HEX) {
+        D x = 0.5 * (ki + 1.0), y = 0.5 * (eta + 1.0), z = 0.5 * (phi + 1.0);
+        lam[1] = (1.0 - x) * (1.0 - y) * (1.0 - z); lam[2] = x * (1.0 - y) * (1.0 - z);
+        lam[3] = x * y * (1.0 - z); lam[4] = (1.0 - x) * y * (1.0 - z);
+        lam[5] = (1.0 - x) * (1.0 - y) * z; lam[6] = x * (1.0 - y) * z;
+        lam[7] = x * y * z; lam[8] = (1.0 - x) * y * z;
+        sig[1] = (1.0 - x) + (1.0 - y) + (1.0 - z); sig[2] = x + (1.0 - y) + (1.0 - z);
+        sig[3] = x + y + (1.0 - z); sig[4] = (1.0 - x) + y + (1.0 - z);
+        sig[5] = (1.0 - x) + (1.0 - y) + z; sig[6] = x + (1.0 - y) + z;
+        sig[7] = x + y + z; sig[8] = (1.0 - x) + y + z;
+    } else {
+        fail("H1 functions are provided for line, triangle, quadrangle, tetrahedron and hexahedron elements");
+    }
+}
+
+bool simplex(int type) { return type == TRI || type == TET; }
+
+// Evaluate form function `f` (for entity orientation `orient`) at one point.
+D eval_ff(int type, int maxorder, const FormFunction& f, int orient, const D& ki, const D& eta, const D& phi) {
+    std::vector<D> lam, sig;
+    lambdas(type, ki, eta, phi, lam, sig);
+    const int h = f.order;
+    if (f.dim == 0) return lam[f.num + 1];
+    if (f.dim == 1 && type != HEX && type != QUAD && type != LINE && type != TRI && type != TET) fail("bad type");
+    if (f.dim == 1) {
+        const std::vector<int>& en = topo::edge_nodes(type);
+        int e1 = en[2 * f.num + edge_reorder[orient][0]] + 1;
+        int e2 = en[2 * f.num + edge_reorder[orient][1]] + 1;
+        if (simplex(type)) {
+            std::vector<D> Ls = leg_Ls(maxorder, lam[e1] - lam[e2], lam[e1] + lam[e2]);
+            return Ls[h];
+        }
+        std::vector<D> L = leg_L(maxorder, sig[e1] - sig[e2]);
+        return L[h] * (lam[e1] + lam[e2]);
+    }
+    if (f.dim == 2 && (type == TRI || type == TET)) {
+        const std::vector<int>& fn = topo::face_nodes(type);
+        int f1 = fn[3 * f.num + tri_reorder[orient][0]] + 1;
+        int f2 = fn[3 * f.num + tri_reorder[orient][1]] + 1;
+        int f3 = fn[3 * f.num + tri_reorder[orient][2]] + 1;
+        D lamF = lam[f1] + lam[f2] + lam[f3];
+        std::vector<D> Ls = leg_Ls(maxorder, lam[f1] - lam[f2], lam[f1] + lam[f2]);
+        std::vector<D> ls = leg_ls(maxorder, 2.0 * lam[f3] - lamF, lamF);
+        // (i, j) with i + j = h - 3, i outer (h1triangle.cpp:159-185)
+        int i = f.l, j = h - 3 - f.l;
+        return Ls[i + 2] * lam[f3] * ls[j];
+    }
+    if (f.dim == 2) {  // quadrangle face (QUAD itself or a HEX face)
+        const std::vector<int>& fn = topo::face_nodes(type);
+        int f1 = fn[4 * f.num + quad_reorder[orient][0]] + 1;
+        int f2 = fn[4 * f.num + quad_reorder[orient][1]] + 1;
+        int f3 = fn[4 * f.num + quad_reorder[orient][2]] + 1;
+        int f4 = fn[4 * f.num + quad_reorder[orient][3]] + 1;
+        D lamF = lam[f1] + lam[f2] + lam[f3] + lam[f4];
+        std::vector<D> Lx = leg_L(maxorder, sig[f1] - sig[f2]);
+        std::vector<D> Ly = leg_L(maxorder, sig[f1] - sig[f4]);
+        // pairs (i, j) in [0, h-2]^2 with i == h-2 or j == h-2, i outer (h1quadrangle.cpp:176-201)
+        int cnt = 0;
+        for (int i = 0; i <= h - 2; i++)
+            for (int j = 0; j <= h - 2; j++) {
+                if (i != h - 2 && j != h - 2) continue;
+                if (cnt == f.l) return Lx[i + 2] * Ly[j + 2] * lamF;
+                cnt++;
+            }
+        fail("quad face function index out of range");
+    }
+    // volume functions
+    if (type == TET) {
+        std::vector<D> Ls = leg_Ls(maxorder, lam[1] - lam[2], lam[1] + lam[2]);
+        std::vector<D> ls = leg_ls(maxorder, 2.0 * lam[3] - (1.0 - lam[4]), 1.0 - lam[4]);
+        std::vector<D> l = leg_l(maxorder, 2.0 * lam[4] - 1.0);
+        int cnt = 0;
+        for (int i = 0; i <= h - 4; i++)
+            for (int j = 0; j <= h - 4; j++)
+                for (int k = 0; k <= h - 4; k++) {
+                    if (i + j + k != h - 4) continue;
+                    if (cnt == f.l) return Ls[i + 2] * lam[3] * ls[j] * lam[4] * l[k];
+                    cnt++;
+                }
+        fail("tet volume function index out of range");
+    }
+    if (type == HEX) {
+        D x = 0.5 * (ki + 1.0), y = 0.5 * (eta + 1.0), z = 0.5 * (phi + 1.0);
+        std::vector<D> Lx = leg_L(maxorder, 2.0 * x - 1.0);
+        std::vector<D> Ly = leg_L(maxorder, 2.0 * y - 1.0);
+        std::vector<D> Lz = leg_L(maxorder, 2.0 * z - 1.0);
+        int cnt = 0;
+        for (int i = 0; i <= h - 2; i++)
+            for (int j = 0; j <= h - 2; j++)
+                for (int k = 0; k <= h - 2; k++) {
+                    if (i != h - 2 && j != h - 2 && k != h - 2) continue;
+                    if (cnt == f.l) return Lx[i + 2] * Ly[j + 2] * Lz[k + 2];
+                    cnt++;
+                }
+        fail("hex volume function index out of range");
+    }
+    fail("no volume functions for this element type");
+}
+
+}  // namespace
+
+int h1_count(int type, int order, int dim) {
+    if (order <= 0) return 0;
+    switch (dim) {
+        case 0: return 1;
+        case 1: return order - 1;
+        case 2:
+            if (type == TRI || type == TET) return (order - 2) * (order - 1) / 2;
+            if (type == QUAD || type == HEX) return (order - 1) * (order - 1);
+            return 0;
+        case 3:
+            if (type == TET) return (order - 3) * (order - 2) * (order - 1) / 6;
+            if (type == HEX) return (order - 1) * (order - 1) * (order - 1);
+            return 0;
+    }
+    return 0;
+}
+
+int h1_count(int type, int order) {
+    if (order <= 0) return 0;
+    switch (type) {
+        case POINT: return 1;
+        case LINE: return order + 1;
+        case TRI: return (order + 1) * (order + 2) / 2;
+        case QUAD: return (order + 1) * (order + 1);
+        case TET: return (order + 1) * (order + 2) * (order + 3) / 6;
+        case HEX: return (order + 1) * (order + 1) * (order + 1);
+    }
+    fail("H1 count: unsupported element type " + std::to_string(type));
+}
+
+std::vector<FormFunction> h1_iterate(int type, int order) {
+    std::vector<FormFunction> out;
+    for (int dim = 0; dim <= 3; dim++) {
+        int nent = topo::count_dim(type, dim);
+        for (int num = 0; num < nent; num++) {
+            int nff = h1_count(type, order, dim);
+            // entity counts are only defined up to the element dimension
+            if (dim > topo::dimension(type)) nff = 0;
+            for (int idx = 0; idx < nff; idx++) {
+                FormFunction f;
+                f.dim = dim;
+                f.num = num;
+                f.index = idx;
+                int h = 0;
+                while (idx >= h1_count(type, h, dim)) h++;
+                f.order = h;
+                f.l = idx - h1_count(type, h - 1, dim);
+                if (dim == 0) f.assoc_type = POINT;
+                else if (dim == 1) f.assoc_type = LINE;
+                else if (dim == 2) f.assoc_type = topo::is_triangular_face(type, num) ? TRI : QUAD;
+                else f.assoc_type = type;
+                out.push_back(f);
+            }
+        }
+    }
+    SLX_CHECK((int)out.size() == h1_count(type, order), "H1 iterator count mismatch");
+    return out;
+}
+
+int orientation_of_edge(long long a, long long b) { return a > b ? 0 : 1; }
+
+int orientation_of_triangle(const long long* n) {
+    int maxpos = 0;
+    for (int i = 1; i < 3; i++)
+        if (n[i] > n[maxpos]) maxpos = i;
+    return n[(maxpos + 1) % 3] > n[(maxpos + 2) % 3] ? 2 * maxpos : 2 * maxpos + 1;
+}
+
+int orientation_of_quadrangle(const long long* n) {
+    int maxpos = 0;
+    for (int i = 1; i < 4; i++)
+        if (n[i] > n[maxpos]) maxpos = i;
+    return n[(maxpos + 1) % 4] > n[(maxpos + 3) % 4] ? 2 * maxpos : 2 * maxpos + 1;
+}
+
+int total_orientation(int type, const long long* nodes) {
+    const std::vector<int>& en = topo::edge_nodes(type);
+    const std::vector<int>& fn = topo::face_nodes(type);
+    int ne = topo::count_edges(type), ntf = topo::count_tri_faces(type), nqf = topo::count_quad_faces(type);
+    int total = 0, factor = 1;
+    for (int i = 0; i < ne; i++) {
+        total += orientation_of_edge(nodes[en[2 * i]], nodes[en[2 * i + 1]]) * factor;
+        factor *= 2;
+    }
+    for (int i = 0; i < ntf + nqf; i++) {
+        int o;
+        if (i < ntf) {
+            long long f[3] = {nodes[fn[3 * i]], nodes[fn[3 * i + 1]], nodes[fn[3 * i + 2]]};
+            o = orientation_of_triangle(f);
+        } else {
+            int off = 3 * ntf + 4 * (i - ntf);
+            long long f[4] = {nodes[fn[off]], nodes[fn[off + 1]], nodes[fn[off + 2]], nodes[fn[off + 3]]};
+            o = orientation_of_quadrangle(f);
+        }
+        total += o * factor;
+        factor *= 8;
+    }
+    return total;
+}
+
+void orientations_from_total(int type, int total, std::vector<int>& edges, std::vector<int>& faces) {
+    int ne = topo::count_edges(type), nf = topo::count_faces(type);
+    edges.resize(ne);
+    faces.resize(nf);
+    for (int e = 0; e < ne; e++) { edges[e] = total % 2; total /= 2; }
+    for (int f = 0; f < nf; f++) { faces[f] = total % 8; total /= 8; }
+}
+
+void h1_tables(int type, int order, int totalorientation, const std::vector<double>& pts, std::vector<double>& out) {
+    std::vector<FormFunction> ffs = h1_iterate(type, order);
+    int nff = (int)ffs.size(), np = (int)pts.size() / 3;
+    std::vector<int> eo, fo;
+    orientations_from_total(type, totalorientation, eo, fo);
+    out.assign((size_t)4 * nff * np, 0.0);
+    for (int p = 0; p < np; p++) {
+        D ki = var(pts[3 * p], 0), eta = var(pts[3 * p + 1], 1), phi = var(pts[3 * p + 2], 2);
+        for (int f = 0; f < nff; f++) {
+            const FormFunction& ff = ffs[f];
+            int orient = 0;
+            // Only edges and faces carry an orientation (container.cpp:96-105);
+            // the element's own edge (line) and face (tri/quad) included.
+            if (ff.dim == 1) orient = eo[ff.num];
+            if (ff.dim == 2) orient = fo[ff.num];
+            D v = eval_ff(type, std::max(order, 2), ff, orient, ki, eta, phi);
+            out[((size_t)0 * nff + f) * np + p] = v.v;
+            out[((size_t)1 * nff + f) * np + p] = v.d[0];
+            out[((size_t)2 * nff + f) * np + p] = v.d[1];
+            out[((size_t)3 * nff + f) * np + p] = v.d[2];
+        }
+    }
+}
+
+void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out) {
+    int nn = topo::count_nodes(type), np = (int)pts.size() / 3;
+    out.assign((size_t)4 * nn * np, 0.0);
+    for (int p = 0; p < np; p++) {
+        D ki = var(pts[3 * p], 0), eta = var(pts[3 * p + 1], 1), phi = var(pts[3 * p + 2], 2);
+        std::vector<D> lam, sig;
+        if (type == POINT) {
+            out[p] = 1.0;
+            continue;
+        }
+        lambdas(type, ki, eta, phi, lam, sig);
+        for (int n = 0; n < nn; n++) {
+            const D& v = lam[n + 1];
+            out[((size_t)0 * nn + n) * np + p] = v.v;
+            out[((size_t)1 * nn + n) * np + p] = v.d[0];
+            out[((size_t)2 * nn + n) * np + p] = v.d[1];
+            out[((size_t)3 * nn + n) * np + p] = v.d[2];
+        }
+    }
+}
+
+}  // namespace slx

@@ -1,0 +1,54 @@
+// Hierarchical H1 shape functions (Zaglmayr) for line, triangle, quadrangle,
+// tetrahedron and hexahedron elements, evaluated with their reference
+// derivatives at arbitrary points.
+//
+// Follows src/shapefunction/hierarchical/h1/h1{line,triangle,quadrangle,
+// tetrahedron,hexahedron}.cpp, legendre.cpp, orientation.cpp and the form
+// function iterator order of hierarchicalformfunctioniterator.cpp:20-56
+// (vertex, edge, face, volume; inside an entity by increasing order).
+#pragma once
+#include <vector>
+#include "common.h"
+
+namespace slx {
+
+// Number of H1 functions of order <= `order` on the whole element
+// (h1*::count(int), e.g. src/shapefunction/hierarchical/h1/h1triangle.cpp:6-12).
+int h1_count(int type, int order);
+// Number of H1 functions of order <= `order` on ONE sub-entity of dimension `dim`
+// (h1*::count(order, dim, num), e.g. h1triangle.cpp:14-48).
+int h1_count(int type, int order, int dim);
+// h1*::isorientationdependent (h1triangle.h:44): orders above 2.
+inline bool h1_orientation_dependent(int order) { return order > 2; }
+
+struct FormFunction {
+    int dim;            // 0 vertex, 1 edge, 2 face, 3 volume
+    int num;            // node/edge/face index in the element
+    int index;          // index of the function inside its entity
+    int order;          // polynomial order of the function
+    int l;              // index among the functions of that order in the entity
+    int assoc_type;     // element type of the sub-entity carrying the function
+};
+
+// Form functions in the reference iterator order.
+std::vector<FormFunction> h1_iterate(int type, int order);
+
+// Total orientation helpers (src/shapefunction/hierarchical/orientation.cpp:66-291).
+int orientation_of_edge(long long a, long long b);
+int orientation_of_triangle(const long long* n);
+int orientation_of_quadrangle(const long long* n);
+int total_orientation(int type, const long long* corner_nodes);
+void orientations_from_total(int type, int total, std::vector<int>& edges, std::vector<int>& faces);
+
+// tomatrix (hierarchicalformfunctioncontainer.cpp:78-112): values of every form
+// function (rows, iterator order) at every point (columns), for the given total
+// orientation and derivative (0 value, 1 d/dki, 2 d/deta, 3 d/dphi).
+// `pts` holds 3 reference coordinates per point.  out[deriv][ff][pt].
+void h1_tables(int type, int order, int totalorientation, const std::vector<double>& pts,
+               std::vector<double>& out);
+
+// Order-1 Lagrange geometry functions (corner nodes) and their reference
+// derivatives: out[deriv][node][pt] (src/shapefunction/lagrange/lagrange*.cpp, order 1).
+void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out);
+
+}  // namespace slx

@@ -1,0 +1,739 @@
+// Fields, parameters, DOF manager and the formulation front end (see fem.h).
+#include "fem.h"
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <numeric>
+#include <unordered_map>
+#include <cstring>
+#include "basis.h"
+#include "gauss.h"
+
+namespace slx {
+
+// ---------------------------------------------------------------- fields ----
+
+RawField::RawField(const Mesh* m, const std::string& type, const std::string& name) : mesh_(m), type_(type), name_(name) {
+    orders_.assign(m->count_disjregs(), 1);
+    constrained_.assign(m->count_disjregs(), 0);
+}
+
+void RawField::setorder(int physreg, int order) {
+    SLX_CHECK(order >= 1, "H1 fields need an interpolation order of at least 1");
+    for (auto& s : subfields_) s->setorder(physreg, order);
+    if (!subfields_.empty()) return;
+    int regdim = mesh_->physreg_dim(physreg);
+    SLX_CHECK(regdim == mesh_->dimension(), "cannot set the interpolation order on a " << regdim << "D region in a "
+                                                                                        << mesh_->dimension() << "D problem");
+    for (int dr : mesh_->physreg_disjregs(physreg, -1)) orders_[dr] = order;
+}
+
+void RawField::setconstraint(int physreg) {
+    for (auto& s : subfields_) s->setconstraint(physreg);
+    if (!subfields_.empty()) return;
+    for (int dr : mesh_->physreg_disjregs(physreg, -1)) constrained_[dr] = 1;
+}
+
+void RawField::build_data_numbering() {
+    // The field's own coefficient numbering: every disjoint region in order,
+    // form functions in order, elements in order (like addtostructure on all
+    // disjoint regions, dofmanager.cpp:35-123).
+    int nd = mesh_->count_disjregs();
+    data_rangebegin.assign(nd, {});
+    int64_t n = 0;
+    for (int dr = 0; dr < nd; dr++) {
+        int t = mesh_->disjreg_type(dr);
+        int nff = h1_count(t, orders_[dr], topo::dimension(t));
+        for (int ff = 0; ff < nff; ff++) {
+            data_rangebegin[dr].push_back((int)n);
+            n += mesh_->disjreg_count(dr);
+        }
+    }
+    data_size_ = n;
+    data.assign(n, 0.0);
+}
+
+field::field(const Mesh& m, const std::string& type, const std::string& name) {
+    int ncomp = 1;
+    std::string base = type;
+    if (type.size() > 2 && type.compare(type.size() - 3, 3, "xyz") == 0) { ncomp = 3; base = type.substr(0, type.size() - 3); }
+    else if (type.size() > 2 && type.compare(type.size() - 2, 2, "xy") == 0) { ncomp = 2; base = type.substr(0, type.size() - 2); }
+    SLX_CHECK(base == "h1", "only h1 fields are provided (got '" << type << "')");
+    raw_ = std::make_shared<RawField>(&m, base, name);
+    raw_->setself(raw_);
+    if (ncomp > 1) {
+        const char* comp = "xyz";
+        for (int i = 0; i < ncomp; i++) {
+            auto s = std::make_shared<RawField>(&m, base, name + "." + comp[i]);
+            s->setself(s);
+            raw_->addsub(s);
+        }
+    }
+}
+
+void field::setorder(int physreg, int order) { raw_->setorder(physreg, order); }
+void field::setconstraint(int physreg) { raw_->setconstraint(physreg); }
+
+static OpP field_op(const std::shared_ptr<RawField>& f) {
+    auto o = std::make_shared<Op>();
+    o->kind = OpKind::Field;
+    o->field = f;
+    return o;
+}
+
+field::operator expression() const {
+    int n = raw_->ncomponents();
+    if (n == 1) return expression(field_op(raw_));
+    std::vector<expression> e;
+    for (int i = 0; i < n; i++) e.push_back(expression(field_op(raw_->component(i))));
+    return expression(n, 1, e);
+}
+
+expression field::comp(int i) const { return expression(field_op(raw_->component(i))); }
+
+parameter::operator expression() const {
+    auto o = std::make_shared<Op>();
+    o->kind = OpKind::Parameter;
+    o->param = raw_;
+    return expression(o);
+}
+
+// ------------------------------------------------------------ sl functions ----
+
+namespace sl {
+expression dof(const expression& e, int physreg) { return e.dof(physreg); }
+expression tf(const expression& e, int physreg) { return e.tf(physreg); }
+expression dx(const expression& e) { return e.spacederivative(1); }
+expression dy(const expression& e) { return e.spacederivative(2); }
+expression dz(const expression& e) { return e.spacederivative(3); }
+
+static int meshdim_of(const expression& e) {
+    for (int r = 0; r < e.countrows(); r++)
+        for (int c = 0; c < e.countcolumns(); c++)
+            if (e.op(r, c)->field) return e.op(r, c)->field->meshdim();
+    fail("cannot take the gradient of an expression without fields");
+}
+
+expression grad(const expression& in) {
+    // sl::grad (sl.cpp:1088-1128): entry (comp, i) = d/dx_i of component comp.
+    SLX_CHECK(in.countcolumns() == 1 && in.countrows() <= 3, "can only take the gradient of a scalar or an up to length 3 column vector");
+    int pd = meshdim_of(in);
+    std::vector<expression> ex;
+    for (int comp = 0; comp < in.countrows(); comp++)
+        for (int i = 0; i < pd; i++) ex.push_back(in.spacederivative(i + 1).at(comp, 0));
+    expression out(in.countrows(), pd, ex);
+    if (in.countrows() == 1) out = out.transpose();
+    return out;
+}
+
+expression strain(const expression& input) {
+    // sl::strain (sl.cpp:2284-2303)
+    expression g = input.countcolumns() == 1 ? grad(input) : input;
+    if (input.countrows() == 2) return expression(3, 1, {g.at(0, 0), g.at(1, 1), g.at(1, 0) + g.at(0, 1)});
+    SLX_CHECK(input.countrows() == 3, "can only compute the strains of a 2x1 or 3x1 column vector");
+    return expression(6, 1, {g.at(0, 0), g.at(1, 1), g.at(2, 2), g.at(2, 1) + g.at(1, 2), g.at(0, 2) + g.at(2, 0), g.at(0, 1) + g.at(1, 0)});
+}
+
+expression predefinedelasticity(const expression& dofu, const expression& tfu, const expression& E, const expression& nu) {
+    // sl::predefinedelasticity (sl.cpp:2672-2752), 3D isotropic case.
+    SLX_CHECK(dofu.countrows() == 3 && tfu.countrows() == 3, "only the 3D isotropic elasticity form is provided");
+    expression one(1.0), half(0.5), zero(0.0), two(2.0);
+    expression H(6, 6, {one - nu, nu, nu, zero, zero, zero, nu, one - nu, nu, zero, zero, zero, nu, nu, one - nu, zero, zero, zero,
+                        zero, zero, zero, half * (one - two * nu), zero, zero, zero, zero, zero, zero, half * (one - two * nu), zero,
+                        zero, zero, zero, zero, zero, half * (one - two * nu)});
+    expression coef = E / (one + nu) / (one - two * nu);
+    coef.reuseit();
+    H = coef * H;
+    return -(H * strain(dofu)) * strain(tfu);
+}
+
+expression pow(const expression& b, const expression& e) { return expression(make_op(OpKind::Power, {b.op(), e.op()})); }
+expression abs(const expression& e) { return expression(make_op(OpKind::Abs, {e.op()})); }
+expression sqrt(const expression& e) { return pow(e, expression(0.5)); }
+expression exp(const expression& e) { return expression(make_op(OpKind::Exp, {e.op()})); }
+expression sin(const expression& e) { return expression(make_op(OpKind::Sin, {e.op()})); }
+expression cos(const expression& e) { return expression(make_op(OpKind::Cos, {e.op()})); }
+expression norm(const expression& e) {
+    expression s(0.0);
+    bool first = true;
+    for (int r = 0; r < e.countrows(); r++)
+        for (int c = 0; c < e.countcolumns(); c++) {
+            expression t = e.at(r, c) * e.at(r, c);
+            s = first ? t : s + t;
+            first = false;
+        }
+    return sqrt(s);
+}
+static expression coord(int axis) {
+    auto o = std::make_shared<Op>();
+    o->kind = OpKind::Coord;
+    o->row = axis;
+    return expression(o);
+}
+expression x() { return coord(0); }
+expression y() { return coord(1); }
+expression z() { return coord(2); }
+integration integral(int physreg, const expression& e, int delta) { return integration{physreg, e, delta}; }
+}  // namespace sl
+
+// ------------------------------------------------------------- dofmanager ----
+
+int DofManager::fieldindex(const RawField* f) const {
+    for (size_t i = 0; i < fields_.size(); i++)
+        if (fields_[i].get() == f) return (int)i;
+    return -1;
+}
+
+void DofManager::addtostructure(const std::shared_ptr<RawField>& f, int physreg) {
+    addtostructure(f, mesh_->physreg_disjregs(physreg, -1));
+}
+
+void DofManager::addtostructure(const std::shared_ptr<RawField>& f, const std::vector<int>& disjregs) {
+    // dofmanager::addtostructure (dofmanager.cpp:35-123)
+    int fi = fieldindex(f.get());
+    if (fi < 0) {
+        fi = (int)fields_.size();
+        fields_.push_back(f);
+        orders_at_add_.push_back(f->orders());
+        rangebegin_.emplace_back(mesh_->count_disjregs());
+    } else
+        SLX_CHECK(f->orders() == orders_at_add_[fi], "field order was changed and does not match dof structure anymore");
+    for (int dr : disjregs) {
+        int t = mesh_->disjreg_type(dr);
+        int nff = h1_count(t, f->order(dr), topo::dimension(t));
+        std::vector<int>& rb = rangebegin_[fi][dr];
+        int before = (int)rb.size();
+        if (before >= nff) continue;
+        rb.resize(nff);
+        int n = mesh_->disjreg_count(dr);
+        for (int ff = before; ff < nff; ff++) {
+            rb[ff] = (int)ndofs_;
+            ndofs_ += n;
+        }
+    }
+    SLX_CHECK(ndofs_ < ((int64_t)1 << 31), "more than 2^31 DOFs: the reference's int addressing overflows");
+}
+
+std::vector<uint8_t> DofManager::isconstrained() const {
+    // dofmanager::isconstrained / getdisjregconstrainedindexes (dofmanager.cpp:366-451)
+    std::vector<uint8_t> out(ndofs_, 0);
+    for (size_t fi = 0; fi < fields_.size(); fi++)
+        for (int dr = 0; dr < (int)rangebegin_[fi].size(); dr++) {
+            if (rangebegin_[fi][dr].empty() || !fields_[fi]->constrained(dr)) continue;
+            int n = mesh_->disjreg_count(dr);
+            for (int rb : rangebegin_[fi][dr])
+                for (int i = 0; i < n; i++) out[rb + i] = 1;
+        }
+    return out;
+}
+
+std::vector<int32_t> DofManager::getaddresses(const std::shared_ptr<RawField>& f, int order, int elemtype,
+                                              const std::vector<int>& elems, int fieldphysreg) const {
+    // dofmanager::getaddresses (dofmanager.cpp:846-918)
+    int fi = fieldindex(f.get());
+    SLX_CHECK(fi >= 0, "selected field is not defined in the dof manager");
+    std::vector<char> defined(mesh_->count_disjregs(), 0);
+    for (int dr : mesh_->physreg_disjregs(fieldphysreg, -1)) defined[dr] = 1;
+    std::vector<FormFunction> ffs = h1_iterate(elemtype, order);
+    int ne = (int)elems.size();
+    std::vector<int32_t> out((size_t)ffs.size() * ne);
+    for (size_t k = 0; k < ffs.size(); k++) {
+        const FormFunction& ff = ffs[k];
+        int prevdr = -1, cnt = 0;
+        for (int i = 0; i < ne; i++) {
+            int sub = mesh_->subelement(ff.assoc_type, elemtype, elems[i], ff.num);
+            int dr = mesh_->disjreg_of(ff.assoc_type, sub);
+            if (i == 0 || dr != prevdr) cnt = h1_count(elemtype, f->order(dr), ff.dim);
+            prevdr = dr;
+            int32_t a = -1;
+            if (defined[dr] && ff.index < cnt) a = rangebegin_[fi][dr][ff.index] + (sub - mesh_->disjreg_begin(dr));
+            out[k * ne + i] = a;
+        }
+    }
+    return out;
+}
+
+// ------------------------------------------------------------ formulation ----
+
+formulation::formulation(const Mesh& m) : mesh_(&m), dofm_(std::make_shared<DofManager>(&m)) {}
+
+formulation& formulation::operator+=(const integration& in) {
+    // formulation::operator+=(integration) (formulation.cpp:51-149)
+    int elementdimension = mesh_->physreg_dim(in.physreg);
+    expression e = in.expr;
+    SLX_CHECK(e.isscalar(), "expand is only defined for scalar expressions (did you try to define a nonscalar formulation term?)");
+    e.op() = op_expand(e.op());
+    DofTfSlices s = extract_dof_tf(e.op(), elementdimension);
+    int id = nintegrals_++;
+    for (size_t sl = 0; sl < s.tfs.size(); sl++) {
+        Contribution c;
+        c.integral_id = id;
+        c.doffield = s.dofs[sl][0] ? s.dofs[sl][0]->field : nullptr;
+        c.tffield = s.tfs[sl][0]->field;
+        c.coeffs = s.coeffs[sl];
+        c.dofs = s.dofs[sl];
+        c.tfs = s.tfs[sl];
+        c.intphysreg = in.physreg;
+        c.dofphysreg = (c.doffield && s.dofs[sl][0]->physreg >= 0) ? s.dofs[sl][0]->physreg : in.physreg;
+        c.tfphysreg = s.tfs[sl][0]->physreg >= 0 ? s.tfs[sl][0]->physreg : in.physreg;
+        c.delta = in.delta;
+        if (c.doffield) dofm_->addtostructure(c.doffield, c.dofphysreg);
+        dofm_->addtostructure(c.tffield, c.tfphysreg);
+        contribs_[c.doffield ? 1 : 0].push_back(c);
+    }
+    return *this;
+}
+
+namespace {
+
+// Coefficient program compiler: CSE by structural signature, left-to-right
+// evaluation of sums and products like operation::interpolate
+// (opsum.cpp:15-58, opproduct.cpp:4-99), linear-scan register reuse.
+struct Compiler {
+    std::vector<slx_instr> code;
+    std::vector<double>* consts;
+    std::unordered_map<std::string, int> memo;       // signature -> value id
+    std::vector<std::string> sigs;
+    int nvals = 0;
+    bool uses_q = false;      // coordinates or fields: varies per Gauss point
+    bool uses_geo = false;    // Jacobian entries
+    std::vector<std::shared_ptr<RawParameter>>* params;
+    std::vector<std::shared_ptr<RawField>>* fields;
+    const Mesh* mesh;
+    std::vector<int> disjregs;
+
+    int emit(int16_t op, int a, int b, int imm, const std::string& sig) {
+        auto it = memo.find(sig);
+        if (it != memo.end()) return it->second;
+        slx_instr in;
+        in.op = op;
+        in.dst = (int16_t)nvals;
+        in.a = (int16_t)a;
+        in.b = (int16_t)b;
+        in.imm = imm;
+        code.push_back(in);
+        memo[sig] = nvals;
+        return nvals++;
+    }
+    int constant(double v) {
+        int k = -1;
+        for (size_t i = 0; i < consts->size(); i++)
+            if (std::memcmp(&(*consts)[i], &v, sizeof(double)) == 0) k = (int)i;
+        if (k < 0) { k = (int)consts->size(); consts->push_back(v); }
+        std::ostringstream s;
+        s << "c" << k;
+        return emit(SLX_OP_CONST, 0, 0, k, s.str());
+    }
+    int compile(const OpP& op) {
+        std::ostringstream s;
+        switch (op->kind) {
+            case OpKind::Constant: return constant(op->value);
+            case OpKind::Parameter: {
+                auto& p = op->param;
+                if (!p->element_values.empty()) {
+                    int k = -1;
+                    for (size_t i = 0; i < params->size(); i++)
+                        if ((*params)[i] == p) k = (int)i;
+                    if (k < 0) { k = (int)params->size(); params->push_back(p); }
+                    s << "p" << k;
+                    return emit(SLX_OP_PARAM, 0, 0, k, s.str());
+                }
+                // region-wise value: constant on the integration disjoint regions
+                // (opparameter::simplify resolves it per disjoint region group)
+                double v = 0;
+                bool found = false;
+                for (auto& kv : p->region_values) {
+                    std::vector<int> drs = mesh->physreg_disjregs(kv.first, -1);
+                    bool all = true;
+                    for (int d : disjregs) all = all && std::find(drs.begin(), drs.end(), d) != drs.end();
+                    if (all) { v = kv.second; found = true; }
+                }
+                SLX_CHECK(found, "parameter is undefined on the integration region");
+                return constant(v);
+            }
+            case OpKind::InvJac:
+                uses_geo = true;
+                s << "ij" << op->row << op->col;
+                return emit(SLX_OP_INVJAC, op->row, op->col, 0, s.str());
+            case OpKind::Jac:
+                uses_geo = true;
+                s << "j" << op->row << op->col;
+                return emit(SLX_OP_JAC, op->row, op->col, 0, s.str());
+            case OpKind::DetJac:
+                uses_geo = true;
+                return emit(SLX_OP_DETJ, 0, 0, 0, "dj");
+            case OpKind::Coord:
+                uses_q = true;
+                s << "x" << op->row;
+                return emit(SLX_OP_COORD, op->row, 0, 0, s.str());
+            case OpKind::Field: {
+                uses_q = true;
+                int k = -1;
+                for (size_t i = 0; i < fields->size(); i++)
+                    if ((*fields)[i] == op->field) k = (int)i;
+                if (k < 0) { k = (int)fields->size(); fields->push_back(op->field); }
+                SLX_CHECK(op->kietaphideriv == 0, "reference-coordinate derivatives of fields are not supported in coefficients");
+                s << "f" << k << "_" << op->spacederiv;
+                return emit(SLX_OP_FIELD, op->spacederiv, 0, k, s.str());
+            }
+            case OpKind::Sum:
+            case OpKind::Product: {
+                int r = compile(op->args[0]);
+                std::string acc = sigs_of(r);
+                for (size_t i = 1; i < op->args.size(); i++) {
+                    int b = compile(op->args[i]);
+                    std::string sig = std::string(op->kind == OpKind::Sum ? "+(" : "*(") + acc + "," + sigs_of(b) + ")";
+                    r = emit(op->kind == OpKind::Sum ? SLX_OP_ADD : SLX_OP_MUL, r, b, 0, sig);
+                    acc = sig;
+                }
+                return r;
+            }
+            case OpKind::Power: {
+                int a = compile(op->args[0]), b = compile(op->args[1]);
+                return emit(SLX_OP_POW, a, b, 0, "^(" + sigs_of(a) + "," + sigs_of(b) + ")");
+            }
+            default: {
+                int16_t code_;
+                const char* nm;
+                switch (op->kind) {
+                    case OpKind::Inversion: code_ = SLX_OP_INV; nm = "inv"; break;
+                    case OpKind::Abs: code_ = SLX_OP_ABS; nm = "abs"; break;
+                    case OpKind::Exp: code_ = SLX_OP_EXP; nm = "exp"; break;
+                    case OpKind::Log: code_ = SLX_OP_LOG; nm = "log"; break;
+                    case OpKind::Sin: code_ = SLX_OP_SIN; nm = "sin"; break;
+                    case OpKind::Cos: code_ = SLX_OP_COS; nm = "cos"; break;
+                    case OpKind::Tan: code_ = SLX_OP_TAN; nm = "tan"; break;
+                    case OpKind::Sqrt: code_ = SLX_OP_SQRT; nm = "sqrt"; break;
+                    default: fail("operation not supported in coefficients: " + op_str(op));
+                }
+                int a = compile(op->args[0]);
+                return emit(code_, a, 0, 0, std::string(nm) + "(" + sigs_of(a) + ")");
+            }
+        }
+    }
+    std::string sigs_of(int v) {
+        for (auto& kv : memo)
+            if (kv.second == v) return kv.first;
+        return "?";
+    }
+};
+
+// Linear-scan register allocation over the SSA program (values die at last use;
+// OUT instructions read their operand).
+int allocate_registers(std::vector<slx_instr>& code) {
+    int n = (int)code.size();
+    std::vector<int> lastuse(n, -1);
+    auto reads = [](const slx_instr& in, int& na) {
+        switch (in.op) {
+            case SLX_OP_ADD: case SLX_OP_MUL: case SLX_OP_POW: na = 2; break;
+            case SLX_OP_INV: case SLX_OP_ABS: case SLX_OP_EXP: case SLX_OP_LOG: case SLX_OP_SIN:
+            case SLX_OP_COS: case SLX_OP_TAN: case SLX_OP_SQRT: case SLX_OP_OUT: na = 1; break;
+            default: na = 0;
+        }
+    };
+    // value id v is produced by the instruction whose dst == v (before allocation)
+    std::vector<int> producer(n + 1, -1);
+    for (int i = 0; i < n; i++)
+        if (code[i].op != SLX_OP_OUT) producer[code[i].dst] = i;
+    for (int i = 0; i < n; i++) {
+        int na;
+        reads(code[i], na);
+        if (na >= 1) lastuse[code[i].a] = i;
+        if (na >= 2) lastuse[code[i].b] = i;
+    }
+    std::vector<int> reg_of(n + 1, -1);
+    std::vector<int> freeregs;
+    int nregs = 0;
+    for (int i = 0; i < n; i++) {
+        slx_instr& in = code[i];
+        int na;
+        reads(in, na);
+        int va = in.a, vb = in.b;
+        if (na >= 1) in.a = (int16_t)reg_of[va];
+        if (na >= 2) in.b = (int16_t)reg_of[vb];
+        // free operands dying here
+        if (na >= 1 && lastuse[va] == i) freeregs.push_back(reg_of[va]);
+        if (na >= 2 && lastuse[vb] == i && vb != va) freeregs.push_back(reg_of[vb]);
+        if (in.op == SLX_OP_OUT) continue;
+        int v = in.dst, r;
+        if (!freeregs.empty()) { r = freeregs.back(); freeregs.pop_back(); }
+        else r = nregs++;
+        reg_of[v] = r;
+        in.dst = (int16_t)r;
+        if (lastuse[v] < 0) freeregs.push_back(r);  // dead value
+    }
+    return std::max(nregs, 1);
+}
+
+// disjointregionselector (disjointregionselector.cpp:4-53)
+std::vector<std::vector<int>> select_groups(const Mesh& m, const std::vector<int>& drs, const std::vector<int>& c1,
+                                            const std::vector<int>& c2) {
+    std::vector<int> idx(drs.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    auto key = [&](int i) { return std::make_tuple(c1[i], c2[i], m.disjreg_type(drs[i])); };
+    std::sort(idx.begin(), idx.end(), [&](int a, int b) {
+        if (key(a) != key(b)) return key(a) < key(b);
+        return a < b;
+    });
+    std::vector<std::vector<int>> groups;
+    for (size_t k = 0; k < idx.size(); k++) {
+        if (k == 0 || key(idx[k]) != key(idx[k - 1])) groups.emplace_back();
+        groups.back().push_back(drs[idx[k]]);
+    }
+    return groups;
+}
+
+}  // namespace
+
+std::shared_ptr<PlanStorage> formulation::buildplan() {
+    auto ps = std::make_shared<PlanStorage>();
+    struct Frag {
+        const Contribution* c;
+        int m, type, ptf, pdof, intorder, orient;
+        bool orientdep;
+        std::vector<int> elems, disjregs;
+    };
+    std::vector<Frag> frags;
+    // formulation::generate (formulation.cpp:211-218) loops rhs first, then K;
+    // contribution::generate (contribution.cpp:44-133) builds groups and
+    // orientation batches.
+    for (int m = 0; m < 2; m++)
+        for (const Contribution& c : contribs_[m]) {
+            std::vector<int> drs = mesh_->physreg_disjregs(c.intphysreg, -2);
+            std::vector<int> tfo(drs.size()), dofo(drs.size(), 0);
+            for (size_t i = 0; i < drs.size(); i++) {
+                tfo[i] = c.tffield->order(drs[i]);
+                if (c.doffield) dofo[i] = c.doffield->order(drs[i]);
+            }
+            for (auto& g : select_groups(*mesh_, drs, tfo, dofo)) {
+                Frag f;
+                f.c = &c;
+                f.m = m;
+                f.type = mesh_->disjreg_type(g[0]);
+                f.ptf = c.tffield->order(g[0]);
+                f.pdof = c.doffield ? c.doffield->order(g[0]) : f.ptf;
+                f.intorder = f.ptf + f.pdof + 2 + c.delta;
+                SLX_CHECK(f.intorder >= 0, "trying to integrate at negative order " << f.intorder);
+                f.orientdep = h1_orientation_dependent(f.ptf) || (c.doffield && h1_orientation_dependent(f.pdof));
+                std::vector<int> sorted = g;
+                std::sort(sorted.begin(), sorted.end());
+                std::vector<int> elems;
+                for (int dr : sorted)
+                    for (int e = mesh_->disjreg_begin(dr); e <= mesh_->disjreg_end(dr); e++) elems.push_back(e);
+                f.disjregs = sorted;
+                if (!f.orientdep) {
+                    f.elems = elems;
+                    f.orient = mesh_->total_orientation(f.type, elems[0]);
+                    frags.push_back(f);
+                    continue;
+                }
+                // elementselector::prepare: stable sort by total orientation, batch by value
+                std::vector<int> to(elems.size());
+                for (size_t i = 0; i < elems.size(); i++) to[i] = mesh_->total_orientation(f.type, elems[i]);
+                std::vector<int> ro(elems.size());
+                std::iota(ro.begin(), ro.end(), 0);
+                std::stable_sort(ro.begin(), ro.end(), [&](int a, int b) { return to[a] < to[b]; });
+                size_t b = 0;
+                while (b < ro.size()) {
+                    size_t e = b;
+                    while (e < ro.size() && to[ro[e]] == to[ro[b]]) e++;
+                    Frag fb = f;
+                    fb.orient = to[ro[b]];
+                    for (size_t k = b; k < e; k++) fb.elems.push_back(elems[ro[k]]);
+                    frags.push_back(fb);
+                    b = e;
+                }
+            }
+        }
+
+    // Merge fragments of one integral over the same element batch into buckets.
+    std::vector<int> bucket_of(frags.size(), -1);
+    std::vector<std::vector<int>> bucket_frags;
+    for (size_t i = 0; i < frags.size(); i++) {
+        const Frag& f = frags[i];
+        for (size_t b = 0; b < bucket_frags.size(); b++) {
+            const Frag& g = frags[bucket_frags[b][0]];
+            if (g.c->integral_id == f.c->integral_id && g.type == f.type && g.intorder == f.intorder && g.ptf == f.ptf &&
+                g.pdof == f.pdof && g.orient == f.orient && g.elems == f.elems) {
+                bucket_of[i] = (int)b;
+                break;
+            }
+        }
+        if (bucket_of[i] < 0) {
+            bucket_of[i] = (int)bucket_frags.size();
+            bucket_frags.emplace_back();
+        }
+        bucket_frags[bucket_of[i]].push_back((int)i);
+    }
+
+    std::vector<std::shared_ptr<RawParameter>> params;
+    std::vector<std::shared_ptr<RawField>> fields;
+    int nb = (int)bucket_frags.size();
+    ps->buckets.resize(nb);
+    ps->slices.resize(nb);
+    ps->table_ptrs.resize(nb);
+    ps->progs.resize(nb);
+    int max_elem_id = 0;
+    int rank_rhs = 0, rank_mat = 0;
+    std::vector<int> frag_rank(frags.size());
+    for (size_t i = 0; i < frags.size(); i++) frag_rank[i] = frags[i].m == 0 ? rank_rhs++ : rank_mat++;
+    for (int b = 0; b < nb; b++) {
+        const Frag& f0 = frags[bucket_frags[b][0]];
+        slx_bucket_desc& bd = ps->buckets[b];
+        int ne = (int)f0.elems.size();
+        bd.elem_type = f0.type;
+        bd.n_elem = ne;
+        bd.n_geo = topo::count_nodes(f0.type);
+        ps->ints.emplace_back((size_t)ne * bd.n_geo);
+        for (int e = 0; e < ne; e++)
+            for (int k = 0; k < bd.n_geo; k++) ps->ints.back()[(size_t)e * bd.n_geo + k] = mesh_->subelement(0, f0.type, f0.elems[e], k);
+        bd.geo_nodes = ps->ints.back().data();
+        ps->ints.emplace_back(f0.elems.begin(), f0.elems.end());
+        bd.elem_ids = ps->ints.back().data();
+        for (int e : f0.elems) max_elem_id = std::max(max_elem_id, e + 1);
+        GaussRule rule = gauss_rule(f0.type, f0.intorder);
+        bd.n_gp = rule.count();
+        ps->dbls.push_back(rule.coords);
+        bd.gp_coords = ps->dbls.back().data();
+        ps->dbls.push_back(rule.weights);
+        bd.gp_weights = ps->dbls.back().data();
+        // shape tables for the tf and dof orders (orientation of the batch)
+        std::vector<int> table_orders;
+        auto table_for = [&](int order) {
+            for (size_t t = 0; t < table_orders.size(); t++)
+                if (table_orders[t] == order) return (int)t;
+            table_orders.push_back(order);
+            std::vector<double> tab;
+            h1_tables(f0.type, order, f0.orient, rule.coords, tab);
+            ps->dbls.push_back(tab);
+            ps->table_ptrs[b].push_back(ps->dbls.back().data());
+            return (int)table_orders.size() - 1;
+        };
+        Compiler comp;
+        comp.consts = &ps->consts;
+        comp.params = &params;
+        comp.fields = &fields;
+        comp.mesh = mesh_;
+        comp.disjregs = f0.disjregs;
+        int ncoefs = 0;
+        for (int fi : bucket_frags[b]) {
+            const Frag& f = frags[fi];
+            const Contribution& c = *f.c;
+            slx_slice_desc sd{};
+            sd.frag_rank = frag_rank[fi];
+            sd.is_rhs = c.doffield ? 0 : 1;
+            sd.tf_table = table_for(f.ptf);
+            sd.n_tf = h1_count(f.type, f.ptf);
+            ps->ints.push_back(dofm_->getaddresses(c.tffield, f.ptf, f.type, f.elems, c.tfphysreg));
+            sd.tf_addr = ps->ints.back().data();
+            if (c.doffield) {
+                sd.dof_table = table_for(f.pdof);
+                sd.n_dof = h1_count(f.type, f.pdof);
+                ps->ints.push_back(dofm_->getaddresses(c.doffield, f.pdof, f.type, f.elems, c.dofphysreg));
+                sd.dof_addr = ps->ints.back().data();
+            } else {
+                sd.dof_table = -1;
+                sd.n_dof = 1;
+                sd.dof_addr = nullptr;
+            }
+            ps->terms.emplace_back();
+            for (size_t t = 0; t < c.tfs.size(); t++) {
+                OpP co = op_simplify(c.coeffs[t]->copy());
+                int v = comp.compile(co);
+                slx_instr out;
+                out.op = SLX_OP_OUT;
+                out.dst = 0;
+                out.a = (int16_t)v;
+                out.b = 0;
+                out.imm = ncoefs;
+                comp.code.push_back(out);
+                slx_term term;
+                term.tf_deriv = c.tfs[t]->kietaphideriv;
+                term.dof_deriv = c.dofs[t] ? c.dofs[t]->kietaphideriv : 0;
+                term.coef = ncoefs++;
+                ps->terms.back().push_back(term);
+            }
+            sd.n_terms = (int)ps->terms.back().size();
+            sd.terms = ps->terms.back().data();
+            ps->slices[b].push_back(sd);
+        }
+        bd.n_slices = (int)ps->slices[b].size();
+        bd.slices = ps->slices[b].data();
+        ps->ints.emplace_back();
+        for (int o : table_orders) ps->ints.back().push_back(h1_count(f0.type, o));
+        bd.n_tables = (int)table_orders.size();
+        bd.table_nff = ps->ints.back().data();
+        bd.tables = ps->table_ptrs[b].data();
+        ps->progs[b] = comp.code;
+        bd.n_regs = allocate_registers(ps->progs[b]);
+        bd.prog = ps->progs[b].data();
+        bd.n_instr = (int)ps->progs[b].size();
+        bd.n_coefs = ncoefs;
+        bool affine = f0.type == TRI || f0.type == TET || f0.type == LINE;
+        bd.q_invariant = (!comp.uses_q && affine) ? 1 : 0;
+    }
+    // numeric parameter arrays
+    ps->n_params = (int)params.size();
+    ps->param_len = max_elem_id;
+    ps->params.assign((size_t)ps->n_params * max_elem_id, 0.0);
+    for (int k = 0; k < ps->n_params; k++) {
+        const std::vector<double>& v = params[k]->element_values;
+        SLX_CHECK((int64_t)v.size() >= max_elem_id, "per-element parameter has " << v.size() << " values, needs " << max_elem_id);
+        std::copy(v.begin(), v.begin() + max_elem_id, ps->params.begin() + (size_t)k * max_elem_id);
+    }
+    // fields read by coefficient programs (Newton re-assembly)
+    ps->plan_fields = fields;
+    int nf = (int)fields.size();
+    ps->field_nff.assign((size_t)nf * nb, 0);
+    ps->field_addr.assign((size_t)nf * nb, nullptr);
+    ps->field_tables.assign((size_t)nf * nb, nullptr);
+    for (int k = 0; k < nf; k++) {
+        auto& fl = fields[k];
+        if (fl->data_size() == 0) fl->build_data_numbering();
+        for (int b = 0; b < nb; b++) {
+            const Frag& f0 = frags[bucket_frags[b][0]];
+            const slx_bucket_desc& bd = ps->buckets[b];
+            int order = fl->order(f0.disjregs[0]);
+            std::vector<FormFunction> ffl = h1_iterate(f0.type, order);
+            std::vector<int32_t> addr((size_t)ffl.size() * bd.n_elem);
+            for (size_t q = 0; q < ffl.size(); q++)
+                for (int i = 0; i < bd.n_elem; i++) {
+                    int e = f0.elems[i];
+                    int sub = mesh_->subelement(ffl[q].assoc_type, f0.type, e, ffl[q].num);
+                    int dr = mesh_->disjreg_of(ffl[q].assoc_type, sub);
+                    int cnt = h1_count(f0.type, fl->order(dr), ffl[q].dim);
+                    addr[q * bd.n_elem + i] = ffl[q].index < cnt ? fl->data_rangebegin[dr][ffl[q].index] + (sub - mesh_->disjreg_begin(dr)) : -1;
+                }
+            ps->ints.push_back(addr);
+            ps->field_addr[(size_t)k * nb + b] = ps->ints.back().data();
+            ps->field_nff[(size_t)k * nb + b] = (int)ffl.size();
+            std::vector<double> tab;
+            GaussRule rule = gauss_rule(f0.type, f0.intorder);
+            h1_tables(f0.type, order, f0.orient, rule.coords, tab);
+            ps->dbls.push_back(tab);
+            ps->field_tables[(size_t)k * nb + b] = ps->dbls.back().data();
+        }
+        ps->field_data_ptrs.push_back(fl->data.data());
+        ps->field_len.push_back(fl->data_size());
+    }
+    ps->constrained = dofm_->isconstrained();
+    slx_plan_desc& d = ps->desc;
+    d.mesh_dim = mesh_->dimension();
+    d.n_nodes = mesh_->count(0);
+    d.n_dofs = dofm_->countdofs();
+    d.constrained = ps->constrained.data();
+    d.n_consts = (int)ps->consts.size();
+    d.consts = ps->consts.data();
+    d.n_buckets = nb;
+    d.buckets = ps->buckets.data();
+    d.n_fields = nf;
+    d.field_nff = ps->field_nff.data();
+    d.field_addr = ps->field_addr.data();
+    d.field_tables = ps->field_tables.data();
+    d.has_rhs = !contribs_[0].empty();
+    d.has_matrix = !contribs_[1].empty();
+    for (auto& b : ps->buckets) ps->n_elem_total += b.n_elem;
+    return ps;
+}
+
+}  // namespace slx

@@ -1,0 +1,288 @@
+// Synthetic benchmark workloads (see problems.h).
+#include "problems.h"
+#include <cmath>
+#include <sstream>
+
+namespace slx {
+
+static uint64_t splitmix64(uint64_t& x) {
+    uint64_t z = (x += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+Rng::Rng(uint64_t seed) {
+    uint64_t x = seed;
+    for (auto& s : s_) s = splitmix64(x);
+}
+
+static inline uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+uint64_t Rng::next() {
+    uint64_t r = rotl(s_[1] * 5, 7) * 9;
+    uint64_t t = s_[1] << 17;
+    s_[2] ^= s_[0];
+    s_[3] ^= s_[1];
+    s_[1] ^= s_[2];
+    s_[0] ^= s_[3];
+    s_[2] ^= t;
+    s_[3] = rotl(s_[3], 45);
+    return r;
+}
+
+double Rng::uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+
+int Rng::uniform_int(int lo, int hi) { return lo + (int)(next() % (uint64_t)(hi - lo + 1)); }
+
+double Rng::normal() {
+    if (has_spare_) {
+        has_spare_ = false;
+        return spare_;
+    }
+    double u1 = uniform(), u2 = uniform();
+    if (u1 < 1e-300) u1 = 1e-300;
+    double r = std::sqrt(-2.0 * std::log(u1)), th = 2.0 * M_PI * u2;
+    spare_ = r * std::sin(th);
+    has_spare_ = true;
+    return r * std::cos(th);
+}
+
+std::map<std::string, std::string> parse_spec(const std::string& spec) {
+    std::map<std::string, std::string> out;
+    std::stringstream ss(spec);
+    std::string item;
+    while (std::getline(ss, item, ';')) {
+        auto eq = item.find('=');
+        if (eq == std::string::npos) continue;
+        out[item.substr(0, eq)] = item.substr(eq + 1);
+    }
+    return out;
+}
+
+namespace {
+
+double getd(const std::map<std::string, std::string>& s, const char* k, double dflt) {
+    auto it = s.find(k);
+    return it == s.end() ? dflt : std::stod(it->second);
+}
+int geti(const std::map<std::string, std::string>& s, const char* k, int dflt) {
+    auto it = s.find(k);
+    return it == s.end() ? dflt : std::stoi(it->second);
+}
+
+// Unit-square grid of n x n cells, lexicographic vertices, interior jitter.
+std::vector<double> square_nodes(int n, double jitter, Rng& rng, double x0 = 0, double y0 = 0) {
+    std::vector<double> xyz;
+    double h = 1.0 / n;
+    xyz.reserve(3 * (size_t)(n + 1) * (n + 1));
+    for (int j = 0; j <= n; j++)
+        for (int i = 0; i <= n; i++) {
+            double x = i * h, y = j * h;
+            if (i > 0 && i < n && j > 0 && j < n && jitter > 0) {
+                x += rng.uniform(-jitter * h, jitter * h);
+                y += rng.uniform(-jitter * h, jitter * h);
+            }
+            xyz.insert(xyz.end(), {x0 + x, y0 + y, 0.0});
+        }
+    return xyz;
+}
+
+void square_boundary(Mesh& m, int n, int base, int physreg) {
+    auto v = [&](int i, int j) { return base + i + (n + 1) * j; };
+    for (int i = 0; i < n; i++) m.add_element(LINE, {v(i, 0), v(i + 1, 0)}, physreg);
+    for (int j = 0; j < n; j++) m.add_element(LINE, {v(n, j), v(n, j + 1)}, physreg);
+    for (int i = n; i > 0; i--) m.add_element(LINE, {v(i, n), v(i - 1, n)}, physreg);
+    for (int j = n; j > 0; j--) m.add_element(LINE, {v(0, j), v(0, j - 1)}, physreg);
+}
+
+void square_tris(Mesh& m, int n, int base, int physreg) {
+    auto v = [&](int i, int j) { return base + i + (n + 1) * j; };
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) {
+            m.add_element(TRI, {v(i, j), v(i + 1, j), v(i + 1, j + 1)}, physreg);
+            m.add_element(TRI, {v(i, j), v(i + 1, j + 1), v(i, j + 1)}, physreg);
+        }
+}
+
+// Config: 2D Poisson, P1 triangles, u = 0 on the boundary, f constant.
+std::shared_ptr<Problem> poisson2d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 200), order = geti(s, "order", 1);
+    double jit = getd(s, "jitter", 0.2), f = getd(s, "f", 1.0);
+    Rng rng((uint64_t)geti(s, "seed", 0));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    m.set_nodes(square_nodes(n, jit, rng));
+    square_boundary(m, n, 0, 2);
+    square_tris(m, n, 0, 1);
+    m.finalize();
+    field u(m, "h1", "u");
+    if (order != 1) u.setorder(1, order);
+    u.setconstraint(2);
+    p->fields = {u};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    *p->form += integral(1, grad(dof(u)) * grad(tf(u)));
+    *p->form += integral(1, -expression(f) * tf(u));
+    std::ostringstream d;
+    d << "2D Poisson P" << order << ", " << n << "x" << n << "x2 triangles, jitter " << jit;
+    p->describe = d.str();
+    return p;
+}
+
+// Config: 3D isotropic elasticity, h1xyz order p on Kuhn tets of an n^3 grid,
+// E per element lognormal(0, sigma), nu, clamped on x = 0, body force -z.
+std::shared_ptr<Problem> elasticity3d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 64), order = geti(s, "order", 2);
+    double jit = getd(s, "jitter", 0.15), sigma = getd(s, "sigma", 0.5), nu = getd(s, "nu", 0.3);
+    Rng rng((uint64_t)geti(s, "seed", 1));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    double h = 1.0 / n;
+    int n1 = n + 1;
+    std::vector<double> xyz;
+    xyz.reserve(3 * (size_t)n1 * n1 * n1);
+    for (int k = 0; k <= n; k++)
+        for (int j = 0; j <= n; j++)
+            for (int i = 0; i <= n; i++) {
+                double c[3] = {i * h, j * h, k * h};
+                bool interior = i > 0 && i < n && j > 0 && j < n && k > 0 && k < n;
+                if (interior && jit > 0)
+                    for (int d = 0; d < 3; d++) c[d] += rng.uniform(-jit * h, jit * h);
+                xyz.insert(xyz.end(), {c[0], c[1], c[2]});
+            }
+    m.set_nodes(xyz);
+    auto v = [&](int i, int j, int k) { return i + n1 * (j + n1 * k); };
+    // clamped face x = 0, split along the Kuhn face diagonal (j,k)-(j+1,k+1)
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++) {
+            m.add_element(TRI, {v(0, j, k), v(0, j + 1, k), v(0, j + 1, k + 1)}, 2);
+            m.add_element(TRI, {v(0, j, k), v(0, j, k + 1), v(0, j + 1, k + 1)}, 2);
+        }
+    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                for (auto& pm : perms) {
+                    int c[3] = {i, j, k};
+                    std::vector<int> tet = {v(c[0], c[1], c[2])};
+                    for (int a = 0; a < 2; a++) {
+                        c[pm[a]]++;
+                        tet.push_back(v(c[0], c[1], c[2]));
+                    }
+                    tet.push_back(v(i + 1, j + 1, k + 1));
+                    m.add_element(TET, tet, 1);
+                }
+    m.finalize();
+    int ntet = m.count(TET);
+    parameter E;
+    std::vector<double> ev(ntet);
+    for (int e = 0; e < ntet; e++) ev[e] = std::exp(sigma * rng.normal());
+    E.setelementvalues(ev);
+    field u(m, "h1xyz", "u");
+    u.setorder(1, order);
+    u.setconstraint(2);
+    p->fields = {u};
+    p->params = {E};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    *p->form += integral(1, predefinedelasticity(dof(u), tf(u), E, expression(nu)));
+    expression f(3, 1, {expression(0.0), expression(0.0), expression(-1.0)});
+    *p->form += integral(1, f * tf(u));
+    std::ostringstream d;
+    d << "3D elasticity h1xyz P" << order << ", " << n << "^3 x 6 tets (" << ntet << "), jitter " << jit;
+    p->describe = d.str();
+    return p;
+}
+
+// Config: many independent quad instances, Q-order Lagrange-equivalent H1,
+// K + k^2 M with k^2 uniform per instance, assembled as one block-diagonal set.
+std::shared_ptr<Problem> quadbatch(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int ninst = geti(s, "instances", 512), nmin = geti(s, "nmin", 16), nmax = geti(s, "nmax", 160);
+    int order = geti(s, "order", 3);
+    Rng rng((uint64_t)geti(s, "seed", 3));
+    std::vector<int> sizes(ninst);
+    std::vector<double> k2(ninst);
+    for (int i = 0; i < ninst; i++) {
+        sizes[i] = rng.uniform_int(nmin, nmax);
+        k2[i] = rng.uniform(1.0, 50.0);
+    }
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    std::vector<double> xyz;
+    for (int i = 0; i < ninst; i++) {
+        std::vector<double> sq = square_nodes(sizes[i], 0.0, rng, 2.0 * i, 0.0);
+        xyz.insert(xyz.end(), sq.begin(), sq.end());
+    }
+    m.set_nodes(xyz);
+    std::vector<double> kelem;
+    int base = 0;
+    for (int i = 0; i < ninst; i++) {
+        int n = sizes[i];
+        auto v = [&](int a, int b) { return base + a + (n + 1) * b; };
+        for (int j = 0; j < n; j++)
+            for (int a = 0; a < n; a++) {
+                m.add_element(QUAD, {v(a, j), v(a + 1, j), v(a + 1, j + 1), v(a, j + 1)}, 1);
+                kelem.push_back(k2[i]);
+            }
+        base += (n + 1) * (n + 1);
+    }
+    m.finalize();
+    parameter K2;
+    K2.setelementvalues(kelem);
+    field u(m, "h1", "u");
+    u.setorder(1, order);
+    p->fields = {u};
+    p->params = {K2};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    *p->form += integral(1, grad(dof(u)) * grad(tf(u)) + expression(K2) * dof(u) * tf(u));
+    std::ostringstream d;
+    d << ninst << " quad instances Q" << order << " (" << m.count(QUAD) << " quads), K + k^2 M";
+    p->describe = d.str();
+    return p;
+}
+
+std::shared_ptr<Problem> from_msh(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    auto it = s.find("file");
+    SLX_CHECK(it != s.end(), "msh problem needs file=<path>");
+    p->mesh = std::make_shared<Mesh>(Mesh::read_msh(it->second));
+    Mesh& m = *p->mesh;
+    int vol = geti(s, "vol", 1), order = geti(s, "order", 1);
+    field u(m, "h1", "u");
+    u.setorder(vol, order);
+    if (s.count("dirichlet")) u.setconstraint(geti(s, "dirichlet", 2));
+    p->fields = {u};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    std::string integrand = s.count("integrand") ? s.at("integrand") : "poisson";
+    if (integrand == "mass")
+        *p->form += integral(vol, dof(u) * tf(u), geti(s, "delta", 0));
+    else
+        *p->form += integral(vol, grad(dof(u)) * grad(tf(u)), geti(s, "delta", 0));
+    *p->form += integral(vol, -expression(getd(s, "f", 1.0)) * tf(u), geti(s, "delta", 0));
+    p->describe = "mesh file " + it->second;
+    return p;
+}
+
+}  // namespace
+
+std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec) {
+    auto s = parse_spec(spec);
+    std::shared_ptr<Problem> p;
+    if (kind == "poisson2d") p = poisson2d(s);
+    else if (kind == "elasticity3d") p = elasticity3d(s);
+    else if (kind == "quadbatch") p = quadbatch(s);
+    else if (kind == "msh") p = from_msh(s);
+    else fail("unknown problem kind '" + kind + "'");
+    p->kind = kind;
+    p->spec = s;
+    p->plan = p->form->buildplan();
+    return p;
+}
+
+}  // namespace slx

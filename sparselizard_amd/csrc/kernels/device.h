@@ -1,0 +1,62 @@
+// Device-side descriptions shared by the HIP kernels and the host plan code.
+#pragma once
+#include <stdint.h>
+#include "../../../include/slx.h"
+
+#define SLX_MAX_SLICES 12
+#define SLX_MAX_FIELDS 4
+#define SLX_MAX_REGS 48
+#define SLX_MAX_GEO 8
+
+namespace slx {
+
+struct DevSlice {
+    int32_t n_tf, n_dof, is_rhs, n_terms;
+    const slx_term* terms;      // device
+    const double* tf_tab;       // [4][n_tf][n_gp]
+    const double* dof_tab;      // [4][n_dof][n_gp] or null
+    int64_t ke_base;            // Ke entry (e, i, j) at ke_base + (e*n_tf + i)*n_dof + j
+};
+
+struct DevField {
+    int32_t nff;
+    const int32_t* addr;        // [nff][n_elem]
+    const double* tab;          // [4][nff][n_gp]
+};
+
+struct DevBucket {
+    int32_t type, n_elem, n_geo, n_gp, elemdim, meshdim;
+    int32_t n_instr, n_regs, n_coefs, q_inv, n_slices, n_fields;
+    const int32_t* geo_nodes;   // [n_elem][n_geo]
+    const int32_t* elem_ids;    // [n_elem]
+    const double* gp_w;         // [n_gp]
+    const double* geo_tab;      // [4][n_geo][n_gp] order-1 Lagrange
+    const slx_instr* prog;
+    DevSlice slices[SLX_MAX_SLICES];
+    DevField fields[SLX_MAX_FIELDS];
+};
+
+struct DevInputs {
+    const double* xyz;          // [n_nodes][3]
+    const double* params;       // [n_params][param_len]
+    int64_t param_len;
+    const double* consts;
+    const double* field_data[SLX_MAX_FIELDS];
+};
+
+// Reduction maps of the cached CSR pattern.
+struct DevReduce {
+    int64_t nrows;                  // free rows
+    const int64_t* erow_ptr;        // [nrows+1] into erows
+    const uint64_t* erows;          // packed (len << 48) | ke offset, in accumulation order
+    const int64_t* slot_ptr;        // [nrows+1] into slots
+    const int32_t* slots;           // per Ke entry of the row: A position, lenA + D position, or -1
+    const int64_t* Arow;            // [nrows+1]
+    const int64_t* Drow;            // [nrows+1]
+    int32_t max_row_len;            // max (lenA + lenD)
+    const int64_t* rhs_erow_ptr;    // [nrows+1]
+    const uint64_t* rhs_erows;      // packed (1 << 48) | ke offset
+    const int64_t* free_dofs;       // [nrows] global DOF of each free row
+};
+
+}  // namespace slx

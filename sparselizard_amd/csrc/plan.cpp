@@ -1,0 +1,758 @@
+// Assembly plan: symbolic phase (CSR pattern and reduction maps) and the
+// device buffers of the numeric phase, plus the C ABI of include/slx.h.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <thread>
+#include <vector>
+#include "../../include/slx.h"
+#include "host/basis.h"
+#include "host/problems.h"
+#include "kernels/device.h"
+
+namespace slx {
+hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st);
+hipError_t launch_elem_mfma(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, bool* handled);
+hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
+                         double* rhs, bool has_matrix, bool has_rhs, hipStream_t st);
+}  // namespace slx
+
+using namespace slx;
+
+namespace {
+thread_local std::string g_last_error;
+
+#define HIPCHECK(x)                                                                                  \
+    do {                                                                                             \
+        hipError_t _e = (x);                                                                         \
+        if (_e != hipSuccess) throw DeviceError(std::string(#x) + ": " + hipGetErrorString(_e));   \
+    } while (0)
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+template <typename F>
+slx_status guarded(F&& f) {
+    try {
+        f();
+        return SLX_OK;
+    } catch (const DeviceError& e) {
+        g_last_error = e.what();
+        return SLX_ERR_DEVICE;
+    } catch (const std::bad_alloc& e) {
+        g_last_error = "allocation failed";
+        return SLX_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return SLX_ERR_INVALID;
+    }
+}
+
+// Device allocation owned by a plan.
+struct DBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    ~DBuf() {
+        if (p) (void)hipFree(p);
+    }
+    void alloc(size_t bytes) {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = bytes;
+        if (bytes) HIPCHECK(hipMalloc(&p, bytes));
+    }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
+// Synchronous upload: the host staging vectors of the symbolic phase are
+// temporaries, so the copy completes before returning.
+template <typename T>
+void upload(DBuf& d, const T* src, size_t count, hipStream_t st) {
+    (void)st;
+    d.alloc(count * sizeof(T));
+    if (count) HIPCHECK(hipMemcpy(d.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+}
+
+}  // namespace
+
+struct slx_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+};
+
+struct slx_plan_s {
+    slx_ctx ctx = nullptr;
+    hipStream_t stream = nullptr;
+    const slx_plan_desc* src = nullptr;
+    int64_t n_dofs = 0, nfree = 0, ncons = 0, nnzA = 0, nnzD = 0;
+    int64_t n_ke = 0, n_erows = 0, n_coo = 0;
+    int mesh_dim = 0, n_fields = 0;
+    bool has_matrix = false, has_rhs = false;
+    int max_erow_len = 1;
+    // host CSR pattern
+    std::vector<int64_t> Arow, Drow;
+    std::vector<int32_t> Acol, Dcol;
+    // device: per-bucket data
+    std::vector<std::unique_ptr<DBuf>> bufs;
+    std::vector<DevBucket> dbuckets;
+    DBuf consts;
+    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_erow_ptr, d_erows, d_slot_ptr, d_slots, d_rhs_ptr, d_rhs_erows, d_free;
+    DBuf d_Aval, d_Dval, d_rhs, d_ke;
+    size_t ke_capacity_bytes = 0;
+    DevReduce rd{};
+    // numeric inputs staged by slx_assemble
+    DBuf d_xyz, d_params;
+    std::vector<std::unique_ptr<DBuf>> d_fields;
+    int last_precision = -1;
+    int max_param = -1;            // highest parameter index read by the programs
+    int64_t max_elem_id = 0;       // parameter arrays must hold this many entries
+    bool uses_fields = false;
+
+    DBuf* newbuf() {
+        bufs.emplace_back(new DBuf());
+        return bufs.back().get();
+    }
+    template <typename T>
+    const T* dev(const T* src, size_t count) {
+        DBuf* b = newbuf();
+        upload(*b, src, count, stream);
+        return b->as<T>();
+    }
+};
+
+namespace {
+
+struct SliceRef {
+    int bucket, slice;
+    int64_t ke_base;
+};
+
+// Symbolic phase on the host: element-row lists in the reference
+// accumulation order, per-row sorted unique columns, A/D split with local
+// renumbering (gentools::findtruefalse, gentools.cpp:2040-2068), slots.
+void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
+    const int64_t nd = d.n_dofs;
+    std::vector<int64_t> freeidx(nd), localidx(nd);
+    int64_t nf = 0, nc = 0;
+    for (int64_t i = 0; i < nd; i++) {
+        if (d.constrained && d.constrained[i]) localidx[i] = nc++;
+        else { localidx[i] = nf; freeidx[i] = nf++; }
+        if (d.constrained && d.constrained[i]) freeidx[i] = -1;
+    }
+    P.nfree = nf;
+    P.ncons = nc;
+    std::vector<int64_t> free_dofs(nf);
+    for (int64_t i = 0; i < nd; i++)
+        if (freeidx[i] >= 0) free_dofs[freeidx[i]] = i;
+    // Ke layout and the slices in fragment (accumulation) order
+    std::vector<SliceRef> mat, rhs;
+    int64_t ke = 0;
+    std::vector<std::vector<int64_t>> kebase(d.n_buckets);
+    for (int b = 0; b < d.n_buckets; b++) {
+        const slx_bucket_desc& bd = d.buckets[b];
+        kebase[b].resize(bd.n_slices);
+        for (int s = 0; s < bd.n_slices; s++) {
+            const slx_slice_desc& sd = bd.slices[s];
+            kebase[b][s] = ke;
+            ke += (int64_t)bd.n_elem * sd.n_tf * (sd.is_rhs ? 1 : sd.n_dof);
+            (sd.is_rhs ? rhs : mat).push_back({b, s, kebase[b][s]});
+        }
+    }
+    P.n_ke = ke;
+    auto rank = [&](const SliceRef& r) { return d.buckets[r.bucket].slices[r.slice].frag_rank; };
+    std::stable_sort(mat.begin(), mat.end(), [&](const SliceRef& a, const SliceRef& b) { return rank(a) < rank(b); });
+    std::stable_sort(rhs.begin(), rhs.end(), [&](const SliceRef& a, const SliceRef& b) { return rank(a) < rank(b); });
+
+    // element rows per free row: counting sort keeps (fragment, i, element) order
+    auto collect = [&](const std::vector<SliceRef>& refs, std::vector<int64_t>& ptr, std::vector<uint64_t>& erows,
+                       std::vector<int32_t>* ref_idx, std::vector<int32_t>* elem_idx) {
+        ptr.assign(nf + 1, 0);
+        for (const SliceRef& r : refs) {
+            const slx_bucket_desc& bd = d.buckets[r.bucket];
+            const slx_slice_desc& sd = bd.slices[r.slice];
+            for (int i = 0; i < sd.n_tf; i++)
+                for (int e = 0; e < bd.n_elem; e++) {
+                    int32_t a = sd.tf_addr[(size_t)i * bd.n_elem + e];
+                    if (a < 0 || (d.constrained && d.constrained[a])) continue;
+                    ptr[freeidx[a] + 1]++;
+                }
+        }
+        for (int64_t i = 0; i < nf; i++) ptr[i + 1] += ptr[i];
+        erows.resize(ptr[nf]);
+        if (ref_idx) ref_idx->resize(ptr[nf]);
+        if (elem_idx) elem_idx->resize(ptr[nf]);
+        std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
+        for (size_t ri = 0; ri < refs.size(); ri++) {
+            const SliceRef& r = refs[ri];
+            const slx_bucket_desc& bd = d.buckets[r.bucket];
+            const slx_slice_desc& sd = bd.slices[r.slice];
+            int64_t len = sd.is_rhs ? 1 : sd.n_dof;
+            for (int i = 0; i < sd.n_tf; i++)
+                for (int e = 0; e < bd.n_elem; e++) {
+                    int32_t a = sd.tf_addr[(size_t)i * bd.n_elem + e];
+                    if (a < 0 || (d.constrained && d.constrained[a])) continue;
+                    int64_t pos = fill[freeidx[a]]++;
+                    int64_t off = r.ke_base + ((int64_t)e * sd.n_tf + i) * len;
+                    erows[pos] = ((uint64_t)len << 48) | (uint64_t)off;
+                    if (ref_idx) (*ref_idx)[pos] = (int32_t)ri;
+                    if (elem_idx) (*elem_idx)[pos] = e;
+                }
+        }
+    };
+    std::vector<int64_t> erow_ptr, rhs_ptr;
+    std::vector<uint64_t> erows, rhs_erows;
+    std::vector<int32_t> eref, eelem;
+    collect(mat, erow_ptr, erows, &eref, &eelem);
+    collect(rhs, rhs_ptr, rhs_erows, nullptr, nullptr);
+    P.n_erows = (int64_t)erows.size();
+    // slot pointers
+    std::vector<int64_t> slot_ptr(nf + 1, 0);
+    for (int64_t r = 0; r < nf; r++) {
+        int64_t s = 0;
+        for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) s += (int64_t)(erows[k] >> 48);
+        slot_ptr[r + 1] = slot_ptr[r] + s;
+    }
+    P.n_coo = slot_ptr[nf];
+    std::vector<int32_t> slots(slot_ptr[nf]);
+    std::vector<int32_t> lenA(nf), lenD(nf);
+    std::vector<std::vector<int32_t>> colsA_chunks, colsD_chunks;
+    int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nf < 20000) nthreads = 1;
+    std::vector<std::vector<int32_t>> tA(nthreads), tD(nthreads);
+    std::vector<int> tmax(nthreads, 1), tmaxrow(nthreads, 0);
+    std::atomic<bool> dup_error{false};
+    auto work = [&](int t) {
+        int64_t r0 = nf * t / nthreads, r1 = nf * (t + 1) / nthreads;
+        std::vector<int64_t> cols;
+        std::vector<int32_t> rankv;
+        for (int64_t r = r0; r < r1; r++) {
+            cols.clear();
+            for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
+                const SliceRef& ref = mat[eref[k]];
+                const slx_bucket_desc& bd = d.buckets[ref.bucket];
+                const slx_slice_desc& sd = bd.slices[ref.slice];
+                int e = eelem[k];
+                tmax[t] = std::max(tmax[t], sd.n_dof);
+                for (int j = 0; j < sd.n_dof; j++) {
+                    int32_t c = sd.dof_addr[(size_t)j * bd.n_elem + e];
+                    if (c >= 0) cols.push_back(c);
+                }
+            }
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            // A (free) columns first, then D (constrained), both ascending
+            int la = 0, ldd = 0;
+            for (int64_t c : cols) (d.constrained && d.constrained[c]) ? ldd++ : la++;
+            lenA[r] = la;
+            lenD[r] = ldd;
+            tmaxrow[t] = std::max(tmaxrow[t], la + ldd);
+            for (int64_t c : cols)
+                if (!(d.constrained && d.constrained[c])) tA[t].push_back((int32_t)localidx[c]);
+            for (int64_t c : cols)
+                if (d.constrained && d.constrained[c]) tD[t].push_back((int32_t)localidx[c]);
+            // rank of every column among the A or the D columns of the row
+            rankv.resize(cols.size());
+            {
+                int na = 0, ndd = 0;
+                for (size_t q = 0; q < cols.size(); q++) {
+                    bool qD = d.constrained && d.constrained[cols[q]];
+                    rankv[q] = qD ? la + ndd++ : na++;
+                }
+            }
+            int64_t sp = slot_ptr[r];
+            for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
+                const SliceRef& ref = mat[eref[k]];
+                const slx_bucket_desc& bd = d.buckets[ref.bucket];
+                const slx_slice_desc& sd = bd.slices[ref.slice];
+                int e = eelem[k];
+                for (int j = 0; j < sd.n_dof; j++) {
+                    int32_t c = sd.dof_addr[(size_t)j * bd.n_elem + e];
+                    int32_t slot = -1;
+                    if (c >= 0) slot = rankv[std::lower_bound(cols.begin(), cols.end(), (int64_t)c) - cols.begin()];
+                    slots[sp + j] = slot;
+                }
+                // slots of one element row must be distinct for the lockstep reduction
+                for (int j = 0; j < sd.n_dof; j++)
+                    for (int j2 = j + 1; j2 < sd.n_dof; j2++)
+                        if (slots[sp + j] >= 0 && slots[sp + j] == slots[sp + j2]) dup_error = true;
+                sp += sd.n_dof;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++) th.emplace_back(work, t);
+    for (auto& x : th) x.join();
+    if (dup_error) fail("an element row maps two local functions to the same DOF (non-conforming addresses are not supported)");
+    P.Arow.assign(nf + 1, 0);
+    P.Drow.assign(nf + 1, 0);
+    for (int64_t r = 0; r < nf; r++) {
+        P.Arow[r + 1] = P.Arow[r] + lenA[r];
+        P.Drow[r + 1] = P.Drow[r] + lenD[r];
+    }
+    P.Acol.clear();
+    P.Dcol.clear();
+    P.Acol.reserve(P.Arow[nf]);
+    P.Dcol.reserve(P.Drow[nf]);
+    for (int t = 0; t < nthreads; t++) {
+        P.Acol.insert(P.Acol.end(), tA[t].begin(), tA[t].end());
+        P.Dcol.insert(P.Dcol.end(), tD[t].begin(), tD[t].end());
+    }
+    P.nnzA = P.Arow[nf];
+    P.nnzD = P.Drow[nf];
+    int maxrow = 1;
+    for (int t = 0; t < nthreads; t++) {
+        P.max_erow_len = std::max(P.max_erow_len, tmax[t]);
+        maxrow = std::max(maxrow, tmaxrow[t]);
+    }
+    // device copies
+    hipStream_t st = P.stream;
+    upload(P.d_Arow, P.Arow.data(), P.Arow.size(), st);
+    upload(P.d_Drow, P.Drow.data(), P.Drow.size(), st);
+    upload(P.d_Acol, P.Acol.data(), P.Acol.size(), st);
+    upload(P.d_Dcol, P.Dcol.data(), P.Dcol.size(), st);
+    upload(P.d_erow_ptr, erow_ptr.data(), erow_ptr.size(), st);
+    upload(P.d_erows, erows.data(), erows.size(), st);
+    upload(P.d_slot_ptr, slot_ptr.data(), slot_ptr.size(), st);
+    upload(P.d_slots, slots.data(), slots.size(), st);
+    upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
+    upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
+    upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
+    P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
+    P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
+    P.d_rhs.alloc(std::max<int64_t>(nd, 1) * sizeof(double));
+    HIPCHECK(hipMemsetAsync(P.d_rhs.p, 0, std::max<int64_t>(nd, 1) * sizeof(double), st));
+    P.rd.nrows = nf;
+    P.rd.erow_ptr = P.d_erow_ptr.as<int64_t>();
+    P.rd.erows = P.d_erows.as<uint64_t>();
+    P.rd.slot_ptr = P.d_slot_ptr.as<int64_t>();
+    P.rd.slots = P.d_slots.as<int32_t>();
+    P.rd.Arow = P.d_Arow.as<int64_t>();
+    P.rd.Drow = P.d_Drow.as<int64_t>();
+    P.rd.max_row_len = maxrow;
+    P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();
+    P.rd.rhs_erows = P.d_rhs_erows.as<uint64_t>();
+    P.rd.free_dofs = P.d_free.as<int64_t>();
+    // Ke buffer (FP64 capacity serves every precision)
+    P.ke_capacity_bytes = std::max<int64_t>(P.n_ke, 1) * sizeof(double);
+    P.d_ke.alloc(P.ke_capacity_bytes);
+}
+
+void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
+    P.dbuckets.resize(d.n_buckets);
+    std::vector<std::vector<int64_t>> kebase(d.n_buckets);
+    int64_t ke = 0;
+    for (int b = 0; b < d.n_buckets; b++) {
+        const slx_bucket_desc& bd = d.buckets[b];
+        DevBucket& db = P.dbuckets[b];
+        std::memset(&db, 0, sizeof(db));
+        SLX_CHECK(bd.n_slices <= SLX_MAX_SLICES, "bucket has " << bd.n_slices << " slices (max " << SLX_MAX_SLICES << ")");
+        SLX_CHECK(bd.n_regs <= SLX_MAX_REGS, "coefficient program needs " << bd.n_regs << " registers (max " << SLX_MAX_REGS << ")");
+        SLX_CHECK(bd.n_geo <= SLX_MAX_GEO, "too many geometry nodes");
+        SLX_CHECK(d.n_fields <= SLX_MAX_FIELDS, "too many fields in coefficient programs");
+        int edim = bd.elem_type == 1 ? 1 : (bd.elem_type <= 3 ? 2 : 3);
+        SLX_CHECK(edim == d.mesh_dim, "only integrals over elements of the mesh dimension are provided");
+        db.type = bd.elem_type;
+        db.n_elem = bd.n_elem;
+        db.n_geo = bd.n_geo;
+        db.n_gp = bd.n_gp;
+        db.elemdim = edim;
+        db.meshdim = d.mesh_dim;
+        db.n_instr = bd.n_instr;
+        db.n_regs = bd.n_regs;
+        db.n_coefs = bd.n_coefs;
+        db.q_inv = bd.q_invariant;
+        db.n_slices = bd.n_slices;
+        db.n_fields = d.n_fields;
+        db.geo_nodes = P.dev(bd.geo_nodes, (size_t)bd.n_elem * bd.n_geo);
+        db.elem_ids = P.dev(bd.elem_ids, (size_t)bd.n_elem);
+        for (int e = 0; e < bd.n_elem; e++) {
+            SLX_CHECK(bd.elem_ids[e] >= 0, "negative element id");
+            P.max_elem_id = std::max<int64_t>(P.max_elem_id, (int64_t)bd.elem_ids[e] + 1);
+        }
+        db.gp_w = P.dev(bd.gp_weights, (size_t)bd.n_gp);
+        std::vector<double> pts(bd.gp_coords, bd.gp_coords + 3 * bd.n_gp), geo;
+        lagrange1_tables(bd.elem_type, pts, geo);
+        db.geo_tab = P.dev(geo.data(), geo.size());
+        db.prog = P.dev(bd.prog, (size_t)bd.n_instr);
+        std::vector<const double*> dtabs(bd.n_tables);
+        for (int t = 0; t < bd.n_tables; t++) dtabs[t] = P.dev(bd.tables[t], (size_t)4 * bd.table_nff[t] * bd.n_gp);
+        for (int s = 0; s < bd.n_slices; s++) {
+            const slx_slice_desc& sd = bd.slices[s];
+            DevSlice& ds = db.slices[s];
+            ds.n_tf = sd.n_tf;
+            ds.n_dof = sd.is_rhs ? 1 : sd.n_dof;
+            ds.is_rhs = sd.is_rhs;
+            ds.n_terms = sd.n_terms;
+            ds.terms = P.dev(sd.terms, (size_t)sd.n_terms);
+            SLX_CHECK(sd.tf_table >= 0 && sd.tf_table < bd.n_tables && bd.table_nff[sd.tf_table] == sd.n_tf,
+                      "slice tf table does not match n_tf");
+            SLX_CHECK(sd.is_rhs || (sd.dof_table >= 0 && sd.dof_table < bd.n_tables && bd.table_nff[sd.dof_table] == sd.n_dof),
+                      "slice dof table does not match n_dof");
+            ds.tf_tab = dtabs[sd.tf_table];
+            ds.dof_tab = sd.is_rhs ? nullptr : dtabs[sd.dof_table];
+            ds.ke_base = ke;
+            ke += (int64_t)bd.n_elem * ds.n_tf * ds.n_dof;
+            for (int t = 0; t < sd.n_terms; t++) {
+                SLX_CHECK(sd.terms[t].coef >= 0 && sd.terms[t].coef < bd.n_coefs, "term coefficient slot out of range");
+                SLX_CHECK(sd.terms[t].tf_deriv >= 0 && sd.terms[t].tf_deriv <= 3 && sd.terms[t].dof_deriv >= 0 && sd.terms[t].dof_deriv <= 3,
+                          "term derivative out of range");
+            }
+        }
+        for (int f = 0; f < d.n_fields; f++) {
+            size_t k = (size_t)f * d.n_buckets + b;
+            db.fields[f].nff = d.field_nff[k];
+            db.fields[f].addr = P.dev(d.field_addr[k], (size_t)d.field_nff[k] * bd.n_elem);
+            db.fields[f].tab = P.dev(d.field_tables[k], (size_t)4 * d.field_nff[k] * bd.n_gp);
+        }
+        // program sanity: registers and opcodes in range
+        for (int i = 0; i < bd.n_instr; i++) {
+            const slx_instr& in = bd.prog[i];
+            SLX_CHECK(in.a >= 0 && in.b >= 0 && in.a < SLX_MAX_REGS && in.b < SLX_MAX_REGS, "bad operand register");
+            if (in.op == SLX_OP_PARAM) {
+                SLX_CHECK(in.imm >= 0, "bad parameter index");
+                P.max_param = std::max(P.max_param, in.imm);
+            }
+            if (in.op == SLX_OP_FIELD) P.uses_fields = true;
+            SLX_CHECK(in.op >= 0 && in.op < SLX_OP_COUNT, "bad opcode");
+            SLX_CHECK(in.dst >= 0 && in.dst < bd.n_regs, "bad destination register");
+            if (in.op == SLX_OP_OUT) SLX_CHECK(in.imm >= 0 && in.imm < bd.n_coefs, "bad coefficient slot");
+            if (in.op == SLX_OP_CONST) SLX_CHECK(in.imm >= 0 && in.imm < d.n_consts, "bad constant index");
+            if (in.op == SLX_OP_FIELD) SLX_CHECK(in.imm >= 0 && in.imm < d.n_fields && in.a >= 0 && in.a <= 3, "bad field reference");
+            if (in.op == SLX_OP_JAC || in.op == SLX_OP_INVJAC) SLX_CHECK(in.a >= 0 && in.a < 3 && in.b >= 0 && in.b < 3, "bad Jacobian entry");
+            if (in.op == SLX_OP_COORD) SLX_CHECK(in.a >= 0 && in.a < 3, "bad coordinate");
+        }
+        // element node numbers must be in range of the node array
+        for (size_t k = 0; k < (size_t)bd.n_elem * bd.n_geo; k++)
+            SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
+    }
+    upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
+}
+
+void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t plen, const double* const* fields, int precision,
+             hipStream_t st, hipEvent_t mid = nullptr) {
+    SLX_CHECK(precision >= 0 && precision <= 3, "unknown precision " << precision);
+    if (P.max_param >= 0) SLX_CHECK(params && plen >= P.max_elem_id, "per-element parameters missing or shorter than the element count");
+    if (P.uses_fields) {
+        SLX_CHECK(fields, "field data required by the coefficient programs");
+        for (int f = 0; f < P.n_fields; f++) SLX_CHECK(fields[f], "field data pointer is null");
+    }
+    DevInputs in{};
+    in.xyz = xyz;
+    in.params = params;
+    in.param_len = plen;
+    in.consts = P.consts.as<double>();
+    for (int f = 0; f < P.n_fields; f++) in.field_data[f] = fields ? fields[f] : nullptr;
+    bool ke64 = precision == SLX_PREC_FP64;
+    for (auto& b : P.dbuckets) {
+        bool handled = false;
+        if (precision == SLX_PREC_F16_SPLIT || precision == SLX_PREC_BF16_SPLIT) {
+            HIPCHECK(launch_elem_mfma(b, in, precision, P.d_ke.p, st, &handled));
+        }
+        if (!handled) {
+            int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
+            HIPCHECK(launch_elem(b, in, p, P.d_ke.p, st));
+        }
+    }
+    if (mid) HIPCHECK(hipEventRecord(mid, st));
+    HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
+                           P.has_matrix, P.has_rhs, st));
+    P.last_precision = precision;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* slx_last_error(void) { return g_last_error.c_str(); }
+
+slx_status slx_create(int device, slx_ctx* out) {
+    return guarded([&] {
+        SLX_CHECK(out, "null output pointer");
+        int n = 0;
+        HIPCHECK(hipGetDeviceCount(&n));
+        SLX_CHECK(device >= 0 && device < n, "device " << device << " not available (" << n << " devices)");
+        HIPCHECK(hipSetDevice(device));
+        auto c = new slx_ctx_s();
+        c->device = device;
+        if (hipStreamCreate(&c->stream) != hipSuccess) {
+            delete c;
+            throw DeviceError("hipStreamCreate failed");
+        }
+        *out = c;
+    });
+}
+
+slx_status slx_destroy(slx_ctx ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out) {
+    return guarded([&] {
+        SLX_CHECK(ctx && desc && out, "null argument");
+        HIPCHECK(hipSetDevice(ctx->device));
+        auto P = std::make_unique<slx_plan_s>();
+        P->ctx = ctx;
+        P->stream = ctx->stream;
+        P->src = desc;
+        P->n_dofs = desc->n_dofs;
+        P->mesh_dim = desc->mesh_dim;
+        P->n_fields = desc->n_fields;
+        P->has_matrix = desc->has_matrix != 0;
+        P->has_rhs = desc->has_rhs != 0;
+        SLX_CHECK(desc->n_dofs >= 0 && desc->n_nodes >= 0 && desc->n_buckets >= 0, "negative sizes in plan description");
+        for (int b = 0; b < desc->n_buckets; b++) {
+            const slx_bucket_desc& bd = desc->buckets[b];
+            for (int s = 0; s < bd.n_slices; s++) {
+                const slx_slice_desc& sd = bd.slices[s];
+                size_t ntf = (size_t)sd.n_tf * bd.n_elem, ndf = sd.is_rhs ? 0 : (size_t)sd.n_dof * bd.n_elem;
+                for (size_t k = 0; k < ntf; k++) SLX_CHECK(sd.tf_addr[k] < desc->n_dofs, "tf address out of range");
+                for (size_t k = 0; k < ndf; k++) SLX_CHECK(sd.dof_addr[k] < desc->n_dofs, "dof address out of range");
+            }
+        }
+        upload_buckets(*P, *desc);
+        build_symbolic(*P, *desc);
+        HIPCHECK(hipStreamSynchronize(P->stream));
+        *out = P.release();
+    });
+}
+
+slx_status slx_plan_shape(slx_plan p, int64_t* nfree, int64_t* ncons, int64_t* nnzA, int64_t* nnzD) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        if (nfree) *nfree = p->nfree;
+        if (ncons) *ncons = p->ncons;
+        if (nnzA) *nnzA = p->nnzA;
+        if (nnzD) *nnzD = p->nnzD;
+    });
+}
+
+slx_status slx_plan_stats(slx_plan p, int64_t* n_elem_rows, int64_t* n_ke, int64_t* n_coo) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        if (n_elem_rows) *n_elem_rows = p->n_erows;
+        if (n_ke) *n_ke = p->n_ke;
+        if (n_coo) *n_coo = p->n_coo;
+    });
+}
+
+slx_status slx_assemble(slx_plan p, const slx_numeric_desc* in) {
+    return guarded([&] {
+        SLX_CHECK(p && in, "null argument");
+        const slx_plan_desc& d = *p->src;
+        upload(p->d_xyz, in->node_xyz, (size_t)3 * d.n_nodes, p->stream);
+        SLX_CHECK(in->n_params > p->max_param, "numeric description has " << in->n_params << " parameters, programs read index " << p->max_param);
+        size_t np = (size_t)in->n_params * in->param_len;
+        upload(p->d_params, in->params, std::max<size_t>(np, 1), p->stream);
+        for (int b = 0; b < d.n_buckets; b++)
+            for (int e = 0; e < d.buckets[b].n_elem && in->n_params > 0; e++)
+                SLX_CHECK(d.buckets[b].elem_ids[e] >= 0 && d.buckets[b].elem_ids[e] < in->param_len, "element id beyond parameter length");
+        std::vector<const double*> fptr(p->n_fields);
+        p->d_fields.resize(p->n_fields);
+        for (int f = 0; f < p->n_fields; f++) {
+            if (!p->d_fields[f]) p->d_fields[f].reset(new DBuf());
+            SLX_CHECK(in->field_data && in->field_len, "field data required by the coefficient program");
+            upload(*p->d_fields[f], in->field_data[f], (size_t)std::max<int64_t>(in->field_len[f], 1), p->stream);
+            fptr[f] = p->d_fields[f]->as<double>();
+        }
+        numeric(*p, p->d_xyz.as<double>(), p->d_params.as<double>(), in->param_len, fptr.data(), in->precision, p->stream);
+    });
+}
+
+slx_status slx_assemble_device(slx_plan p, const double* d_node_xyz, const double* d_params, int64_t param_len,
+                               const double* const* d_field_data, int32_t precision, void* hip_stream) {
+    return guarded([&] {
+        SLX_CHECK(p && d_node_xyz, "null argument");
+        hipStream_t st = hip_stream ? (hipStream_t)hip_stream : p->stream;
+        numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, st);
+    });
+}
+
+slx_status slx_plan_profile(slx_plan p, const double* d_node_xyz, const double* d_params, int64_t param_len,
+                            const double* const* d_field_data, int32_t precision, int32_t iters, double* ms) {
+    return guarded([&] {
+        SLX_CHECK(p && d_node_xyz && ms && iters > 0, "bad argument");
+        hipEvent_t e0, e1, e2;
+        HIPCHECK(hipEventCreate(&e0));
+        HIPCHECK(hipEventCreate(&e1));
+        HIPCHECK(hipEventCreate(&e2));
+        double t_elem = 0, t_red = 0;
+        for (int it = 0; it < iters; it++) {
+            HIPCHECK(hipEventRecord(e0, p->stream));
+            numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, p->stream, e1);
+            HIPCHECK(hipEventRecord(e2, p->stream));
+            HIPCHECK(hipEventSynchronize(e2));
+            float a = 0, b = 0;
+            HIPCHECK(hipEventElapsedTime(&a, e0, e1));
+            HIPCHECK(hipEventElapsedTime(&b, e1, e2));
+            t_elem += a;
+            t_red += b;
+        }
+        ms[0] = t_elem / iters;
+        ms[1] = t_red / iters;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+    });
+}
+
+slx_status slx_plan_sync(slx_plan p) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        HIPCHECK(hipDeviceSynchronize());
+    });
+}
+
+slx_status slx_csr_device(slx_plan p, const int64_t** Arow, const int32_t** Acol, const double** Aval, const int64_t** Drow,
+                          const int32_t** Dcol, const double** Dval, const double** rhs) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        if (Arow) *Arow = p->d_Arow.as<int64_t>();
+        if (Acol) *Acol = p->d_Acol.as<int32_t>();
+        if (Aval) *Aval = p->d_Aval.as<double>();
+        if (Drow) *Drow = p->d_Drow.as<int64_t>();
+        if (Dcol) *Dcol = p->d_Dcol.as<int32_t>();
+        if (Dval) *Dval = p->d_Dval.as<double>();
+        if (rhs) *rhs = p->d_rhs.as<double>();
+    });
+}
+
+slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Aval, int64_t* Drow, int32_t* Dcol, double* Dval,
+                           double* rhs) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (Arow) std::copy(p->Arow.begin(), p->Arow.end(), Arow);
+        if (Drow) std::copy(p->Drow.begin(), p->Drow.end(), Drow);
+        if (Acol) std::copy(p->Acol.begin(), p->Acol.end(), Acol);
+        if (Dcol) std::copy(p->Dcol.begin(), p->Dcol.end(), Dcol);
+        if (Aval && p->nnzA) HIPCHECK(hipMemcpy(Aval, p->d_Aval.p, p->nnzA * sizeof(double), hipMemcpyDeviceToHost));
+        if (Dval && p->nnzD) HIPCHECK(hipMemcpy(Dval, p->d_Dval.p, p->nnzD * sizeof(double), hipMemcpyDeviceToHost));
+        if (rhs && p->n_dofs) HIPCHECK(hipMemcpy(rhs, p->d_rhs.p, p->n_dofs * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+slx_status slx_plan_destroy(slx_plan p) {
+    return guarded([&] {
+        if (!p) return;
+        (void)hipStreamSynchronize(p->stream);
+        delete p;
+    });
+}
+
+// ---- formulation layer ----
+
+struct slx_problem_s {
+    slx_ctx ctx = nullptr;
+    std::shared_ptr<Problem> prob;
+    slx_plan plan = nullptr;
+};
+
+slx_status slx_problem_create(slx_ctx ctx, const char* kind, const char* spec, slx_problem* out) {
+    return guarded([&] {
+        SLX_CHECK(kind && out, "null argument");
+        auto p = new slx_problem_s();
+        p->ctx = ctx;
+        try {
+            p->prob = make_problem(kind, spec ? spec : "");
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+    });
+}
+
+slx_status slx_problem_destroy(slx_problem p) {
+    return guarded([&] {
+        if (!p) return;
+        if (p->plan) slx_plan_destroy(p->plan);
+        delete p;
+    });
+}
+
+slx_status slx_problem_plan(slx_problem p, slx_plan* plan_out) {
+    return guarded([&] {
+        SLX_CHECK(p && p->ctx, "problem has no device context");
+        if (!p->plan) {
+            slx_plan pl = nullptr;
+            if (slx_plan_create(p->ctx, &p->prob->plan->desc, &pl) != SLX_OK) throw std::runtime_error(g_last_error);
+            p->plan = pl;
+        }
+        if (plan_out) *plan_out = p->plan;
+    });
+}
+
+slx_status slx_problem_numeric(slx_problem p, slx_numeric_desc* out) {
+    return guarded([&] {
+        SLX_CHECK(p && out, "null argument");
+        PlanStorage& ps = *p->prob->plan;
+        std::memset(out, 0, sizeof(*out));
+        out->node_xyz = p->prob->mesh->coords().data();
+        out->n_params = ps.n_params;
+        out->param_len = ps.param_len;
+        out->params = ps.params.data();
+        ps.field_data_ptrs.clear();
+        for (auto& f : ps.plan_fields) ps.field_data_ptrs.push_back(f->data.data());
+        out->field_data = ps.field_data_ptrs.data();
+        out->field_len = ps.field_len.data();
+        out->precision = SLX_PREC_FP64;
+    });
+}
+
+slx_status slx_problem_generate(slx_problem p, int32_t precision) {
+    return guarded([&] {
+        slx_plan pl = nullptr;
+        if (slx_problem_plan(p, &pl) != SLX_OK) throw std::runtime_error(g_last_error);
+        slx_numeric_desc nd;
+        if (slx_problem_numeric(p, &nd) != SLX_OK) throw std::runtime_error(g_last_error);
+        nd.precision = precision;
+        if (slx_assemble(pl, &nd) != SLX_OK) throw std::runtime_error(g_last_error);
+        if (slx_plan_sync(pl) != SLX_OK) throw std::runtime_error(g_last_error);
+    });
+}
+
+slx_status slx_problem_info(slx_problem p, int64_t* ndofs, int64_t* nfree, int64_t* nelem, int64_t* nbuckets) {
+    return guarded([&] {
+        SLX_CHECK(p, "null problem");
+        PlanStorage& ps = *p->prob->plan;
+        if (ndofs) *ndofs = ps.desc.n_dofs;
+        if (nfree) {
+            int64_t n = 0;
+            for (uint8_t c : ps.constrained) n += c ? 0 : 1;
+            *nfree = n;
+        }
+        if (nelem) *nelem = ps.n_elem_total;
+        if (nbuckets) *nbuckets = ps.desc.n_buckets;
+    });
+}
+
+slx_status slx_problem_desc(slx_problem p, const slx_plan_desc** desc) {
+    return guarded([&] {
+        SLX_CHECK(p && desc, "null argument");
+        *desc = &p->prob->plan->desc;
+    });
+}
+
+slx_status slx_problem_setdata(slx_problem p, int32_t field, const double* values, int64_t n) {
+    return guarded([&] {
+        SLX_CHECK(p && values, "null argument");
+        PlanStorage& ps = *p->prob->plan;
+        SLX_CHECK(field >= 0 && field < (int)ps.plan_fields.size(), "field index out of range");
+        auto& f = ps.plan_fields[field];
+        SLX_CHECK(n == f->data_size(), "field data has " << f->data_size() << " entries, got " << n);
+        std::copy(values, values + n, f->data.begin());
+    });
+}
+
+}  // extern "C"
