@@ -156,7 +156,9 @@ slx_status slx_destroy(slx_ctx ctx);
 /* Symbolic phase: copies the description, builds the element-row lists, the
  * CSR pattern (per-row sorted unique columns = sorted COO keys), the A/D
  * split with local renumbering and the reduction maps.  Pattern equals
- * rawmat::process (rawmat.cpp:127-298) on the same address tables. */
+ * rawmat::process (rawmat.cpp:127-298) on the same address tables.
+ * With ctx == NULL only the symbolic phase runs (host memory, no device):
+ * the pattern can be read with slx_csr_copyout (value pointers NULL). */
 slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out);
 slx_status slx_plan_shape(slx_plan plan, int64_t* nfree, int64_t* ncons, int64_t* nnzA, int64_t* nnzD);
 /* Numeric phase: uploads the numeric inputs and assembles A, D and b on the
@@ -184,6 +186,18 @@ slx_status slx_csr_copyout(slx_plan plan, int64_t* Arow, int32_t* Acol, double* 
  * (for error measurement); n_out receives the entry count. */
 slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, int64_t* n_coo);
 slx_status slx_plan_destroy(slx_plan plan);
+
+/* ---- Reference tables ------------------------------------------------------
+ * gausspoints(type, order) (src/gausspoint/gausspoints.cpp:4-54): writes up to
+ * `cap` points (3 coordinates each) and weights, *npts = count. */
+slx_status slx_gauss_rule(int32_t elem_type, int32_t order, int32_t cap, int32_t* npts, double* coords, double* weights);
+/* Number of H1 form functions of order <= `order` on the element
+ * (h1*::count, e.g. src/shapefunction/hierarchical/h1/h1triangle.cpp:6-12). */
+slx_status slx_h1_count(int32_t elem_type, int32_t order, int32_t* nff);
+/* hierarchicalformfunctioncontainer::tomatrix (hierarchicalformfunctioncontainer.cpp:78-112)
+ * for all four derivative slots: out[deriv][ff][pt], deriv 0 value, 1..3 d/dki,eta,phi. */
+slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t npts, const double* pts,
+                         double* out);
 
 /* ---- Formulation layer ---------------------------------------------------- */
 typedef struct slx_problem_s* slx_problem;

@@ -32,9 +32,38 @@ class NumericDesc(C.Structure):
     ]
 
 
-class PlanDescHead(C.Structure):
-    """Leading fields of slx_plan_desc."""
-    _fields_ = [("mesh_dim", C.c_int32), ("n_nodes", C.c_int64), ("n_dofs", C.c_int64)]
+class Instr(C.Structure):
+    _fields_ = [("op", C.c_int16), ("dst", C.c_int16), ("a", C.c_int16), ("b", C.c_int16), ("imm", C.c_int32)]
+
+
+class Term(C.Structure):
+    _fields_ = [("tf_deriv", C.c_int32), ("dof_deriv", C.c_int32), ("coef", C.c_int32)]
+
+
+class SliceDesc(C.Structure):
+    _fields_ = [("frag_rank", C.c_int32), ("is_rhs", C.c_int32), ("n_tf", C.c_int32), ("n_dof", C.c_int32),
+                ("tf_addr", C.POINTER(C.c_int32)), ("dof_addr", C.POINTER(C.c_int32)), ("n_terms", C.c_int32),
+                ("terms", C.POINTER(Term)), ("tf_table", C.c_int32), ("dof_table", C.c_int32)]
+
+
+class BucketDesc(C.Structure):
+    _fields_ = [("elem_type", C.c_int32), ("n_elem", C.c_int32), ("n_geo", C.c_int32),
+                ("geo_nodes", C.POINTER(C.c_int32)), ("elem_ids", C.POINTER(C.c_int32)), ("n_gp", C.c_int32),
+                ("gp_coords", C.POINTER(C.c_double)), ("gp_weights", C.POINTER(C.c_double)), ("n_tables", C.c_int32),
+                ("table_nff", C.POINTER(C.c_int32)), ("tables", C.POINTER(C.POINTER(C.c_double))),
+                ("n_slices", C.c_int32), ("slices", C.POINTER(SliceDesc)), ("n_instr", C.c_int32),
+                ("prog", C.POINTER(Instr)), ("n_regs", C.c_int32), ("n_coefs", C.c_int32), ("q_invariant", C.c_int32)]
+
+
+class PlanDesc(C.Structure):
+    _fields_ = [("mesh_dim", C.c_int32), ("n_nodes", C.c_int64), ("n_dofs", C.c_int64),
+                ("constrained", C.POINTER(C.c_uint8)), ("n_consts", C.c_int32), ("consts", C.POINTER(C.c_double)),
+                ("n_buckets", C.c_int32), ("buckets", C.POINTER(BucketDesc)), ("n_fields", C.c_int32),
+                ("field_nff", C.POINTER(C.c_int32)), ("field_addr", C.POINTER(C.POINTER(C.c_int32))),
+                ("field_tables", C.POINTER(C.POINTER(C.c_double))), ("has_rhs", C.c_int32), ("has_matrix", C.c_int32)]
+
+
+PlanDescHead = PlanDesc
 
 
 class SlxError(RuntimeError):
@@ -78,6 +107,9 @@ def lib():
         "slx_problem_info": (i32, [vp, p64, p64, p64, p64]),
         "slx_problem_desc": (i32, [vp, C.POINTER(vp)]),
         "slx_problem_setdata": (i32, [vp, i32, pd, i64]),
+        "slx_gauss_rule": (i32, [i32, i32, i32, p32, pd, pd]),
+        "slx_h1_count": (i32, [i32, i32, p32]),
+        "slx_h1_tables": (i32, [i32, i32, i32, i32, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -92,7 +124,33 @@ EXPORTED = [
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
     "slx_plan_destroy", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables",
 ]
+
+
+def gauss_rule(elem_type, order):
+    """(coords [n,3], weights [n]) of gausspoints(type, order)."""
+    n = C.c_int32()
+    check(lib().slx_gauss_rule(elem_type, order, 0, C.byref(n), None, None))
+    xyz = np.zeros((n.value, 3), np.float64)
+    w = np.zeros(n.value, np.float64)
+    check(lib().slx_gauss_rule(elem_type, order, n.value, C.byref(n), _p(xyz, C.c_double), _p(w, C.c_double)))
+    return xyz, w
+
+
+def h1_count(elem_type, order):
+    n = C.c_int32()
+    check(lib().slx_h1_count(elem_type, order, C.byref(n)))
+    return n.value
+
+
+def h1_tables(elem_type, order, pts, total_orientation=0):
+    """Values and reference derivatives: array [4, nff, npts]."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    nff = h1_count(elem_type, order)
+    out = np.zeros((4, nff, len(pts)), np.float64)
+    check(lib().slx_h1_tables(elem_type, order, total_orientation, len(pts), _p(pts, C.c_double), _p(out, C.c_double)))
+    return out
 
 
 def check(status):
@@ -155,8 +213,12 @@ class Problem:
         check(lib().slx_problem_desc(self.h, C.byref(d)))
         return d
 
+    def desc(self):
+        """The plan description (a ctypes view valid while the problem lives)."""
+        return C.cast(self.desc_ptr(), C.POINTER(PlanDesc)).contents
+
     def n_nodes(self):
-        return C.cast(self.desc_ptr(), C.POINTER(PlanDescHead)).contents.n_nodes
+        return self.desc().n_nodes
 
     def numeric(self):
         nd = NumericDesc()

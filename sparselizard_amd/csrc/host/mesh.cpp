@@ -68,6 +68,21 @@ int Mesh::add_element(int type, const std::vector<int>& nodes, int physreg) {
     return num;
 }
 
+void Mesh::add_to_physreg(int type, int elem, int physreg) {
+    SLX_CHECK(!finalized_, "mesh already finalized");
+    SLX_CHECK(elem >= 0 && elem < count(type), "element out of range");
+    int pi = physreg_index(physreg);
+    int dim = topo::dimension(type);
+    if (pi < 0) {
+        pi = (int)phys_number_.size();
+        phys_number_.push_back(physreg);
+        phys_dim_.push_back(dim);
+        phys_elems_.emplace_back();
+    }
+    SLX_CHECK(phys_dim_[pi] == dim, "physical region " << physreg << " mixes element dimensions");
+    phys_elems_[pi][type].push_back(elem);
+}
+
 void Mesh::renumber(int type, const std::vector<int>& renum) {
     // elements::renumber (elements.cpp:1005-1033): only node, line, triangle and
     // quadrangle references exist in the sub-element tables.
@@ -394,14 +409,7 @@ Mesh Mesh::read_msh(const std::string& filename) {
         }
         // A repeated element only joins the new physical region (gmshinterface.cpp:274-281).
         if (nodes == prev && type == prevtype) {
-            int pi = m.physreg_index(phys);
-            if (pi < 0) {
-                pi = (int)m.phys_number_.size();
-                m.phys_number_.push_back(phys);
-                m.phys_dim_.push_back(topo::dimension(type));
-                m.phys_elems_.emplace_back();
-            }
-            m.phys_elems_[pi][type].push_back(previdx);
+            m.add_to_physreg(type, previdx, phys);
             continue;
         }
         previdx = m.add_element(type, nodes, phys);

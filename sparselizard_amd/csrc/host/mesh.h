@@ -26,6 +26,9 @@ class Mesh {
     // elements::add + physicalregion::addelement for one element read from a
     // file (gmshinterface.cpp:239-288): returns the element number.
     int add_element(int type, const std::vector<int>& nodes, int physreg);
+    // A repeated element line of a file: the previous element also joins
+    // another physical region (gmshinterface.cpp:274-281).
+    void add_to_physreg(int type, int elem, int physreg);
     // Run the pipeline once every element has been added.
     void finalize();
     // Gmsh 2.2 ASCII reader / writer (gmshinterface.cpp:157-298).

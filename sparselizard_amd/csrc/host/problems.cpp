@@ -183,7 +183,7 @@ std::shared_ptr<Problem> elasticity3d(const std::map<std::string, std::string>& 
     E.setelementvalues(ev);
     field u(m, "h1xyz", "u");
     u.setorder(1, order);
-    u.setconstraint(2);
+    if (geti(s, "clamp", 1)) u.setconstraint(2);
     p->fields = {u};
     p->params = {E};
     p->form = std::make_shared<formulation>(m);
@@ -246,6 +246,147 @@ std::shared_ptr<Problem> quadbatch(const std::map<std::string, std::string>& s) 
     return p;
 }
 
+// Structured unit-square / unit-cube grids of one element type.
+void grid_mesh(Mesh& m, const std::string& elem, int n, double jit, Rng& rng, int volreg, int bndreg) {
+    bool is3d = elem == "tet" || elem == "hex";
+    int n1 = n + 1;
+    double h = 1.0 / n;
+    std::vector<double> xyz;
+    int nk = is3d ? n : 0;
+    for (int k = 0; k <= nk; k++)
+        for (int j = 0; j <= n; j++)
+            for (int i = 0; i <= n; i++) {
+                double c[3] = {i * h, j * h, is3d ? k * h : 0.0};
+                bool interior = i > 0 && i < n && j > 0 && j < n && (!is3d || (k > 0 && k < n));
+                if (interior && jit > 0)
+                    for (int d = 0; d < (is3d ? 3 : 2); d++) c[d] += rng.uniform(-jit * h, jit * h);
+                xyz.insert(xyz.end(), {c[0], c[1], c[2]});
+            }
+    m.set_nodes(xyz);
+    auto v2 = [&](int i, int j) { return i + n1 * j; };
+    auto v3 = [&](int i, int j, int k) { return i + n1 * (j + n1 * k); };
+    if (!is3d) {
+        if (bndreg >= 0) square_boundary(m, n, 0, bndreg);
+        if (elem == "tri") square_tris(m, n, 0, volreg);
+        else
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) m.add_element(QUAD, {v2(i, j), v2(i + 1, j), v2(i + 1, j + 1), v2(i, j + 1)}, volreg);
+        return;
+    }
+    if (elem == "hex") {
+        if (bndreg >= 0) {
+            // the six faces of the cube as quadrangles
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    m.add_element(QUAD, {v3(0, a, b), v3(0, a, b + 1), v3(0, a + 1, b + 1), v3(0, a + 1, b)}, bndreg);
+                    m.add_element(QUAD, {v3(n, a, b), v3(n, a + 1, b), v3(n, a + 1, b + 1), v3(n, a, b + 1)}, bndreg);
+                    m.add_element(QUAD, {v3(a, 0, b), v3(a + 1, 0, b), v3(a + 1, 0, b + 1), v3(a, 0, b + 1)}, bndreg);
+                    m.add_element(QUAD, {v3(a, n, b), v3(a, n, b + 1), v3(a + 1, n, b + 1), v3(a + 1, n, b)}, bndreg);
+                    m.add_element(QUAD, {v3(a, b, 0), v3(a, b + 1, 0), v3(a + 1, b + 1, 0), v3(a + 1, b, 0)}, bndreg);
+                    m.add_element(QUAD, {v3(a, b, n), v3(a + 1, b, n), v3(a + 1, b + 1, n), v3(a, b + 1, n)}, bndreg);
+                }
+        }
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++)
+                    m.add_element(HEX, {v3(i, j, k), v3(i + 1, j, k), v3(i + 1, j + 1, k), v3(i, j + 1, k), v3(i, j, k + 1),
+                                        v3(i + 1, j, k + 1), v3(i + 1, j + 1, k + 1), v3(i, j + 1, k + 1)},
+                                  volreg);
+        return;
+    }
+    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                for (auto& pm : perms) {
+                    int c[3] = {i, j, k};
+                    std::vector<int> tet = {v3(c[0], c[1], c[2])};
+                    for (int a = 0; a < 2; a++) {
+                        c[pm[a]]++;
+                        tet.push_back(v3(c[0], c[1], c[2]));
+                    }
+                    tet.push_back(v3(i + 1, j + 1, k + 1));
+                    m.add_element(TET, tet, volreg);
+                }
+}
+
+// L2 projection of a polynomial of degree `deg` (dof*tf - f*tf): with
+// deg <= order the projection reproduces f exactly, which pins bases,
+// orientations, numbering and assembly together (the validation of
+// examples/high-order-interpolations-2d/main.cpp:68 in exact form).
+std::shared_ptr<Problem> projection(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    std::string elem = s.count("elem") ? s.at("elem") : "tri";
+    int n = geti(s, "n", 4), order = geti(s, "order", 3), deg = geti(s, "deg", order);
+    double jit = getd(s, "jitter", (elem == "tri" || elem == "tet") ? 0.2 : 0.0);
+    Rng rng((uint64_t)geti(s, "seed", 7));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    grid_mesh(m, elem, n, jit, rng, 1, -1);
+    m.finalize();
+    field u(m, "h1", "u");
+    u.setorder(1, order);
+    p->fields = {u};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    bool is3d = elem == "tet" || elem == "hex";
+    expression f(0.0);
+    bool first = true;
+    for (int a = 0; a <= deg; a++)
+        for (int b = 0; a + b <= deg; b++)
+            for (int c = 0; a + b + c <= deg && (is3d || c == 0); c++) {
+                expression mono(1.0 / (1.0 + a + 2.0 * b + 3.0 * c));
+                for (int k = 0; k < a; k++) mono = mono * x();
+                for (int k = 0; k < b; k++) mono = mono * y();
+                for (int k = 0; k < c; k++) mono = mono * z();
+                f = first ? mono : f + mono;
+                first = false;
+            }
+    *p->form += integral(1, dof(u) * tf(u) - f * tf(u));
+    p->describe = "L2 projection of a degree-" + std::to_string(deg) + " polynomial, " + elem + " order " + std::to_string(order);
+    return p;
+}
+
+// Config: 3D hp-Poisson on hexahedra, per-element order uniform in
+// [pmin, pmax] (one physical region per order, orders set descendingly so
+// interfaces take the lower order, rawfield.cpp:670-676), coefficient
+// lognormal(0, sigma) per element, Gauss order 2p+1 (delta -1), u = 0 on the
+// cube boundary, f = 1.
+std::shared_ptr<Problem> hppoisson3d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 48), pmin = geti(s, "pmin", 1), pmax = geti(s, "pmax", 5);
+    double jit = getd(s, "jitter", 0.1), sigma = getd(s, "sigma", 1.0);
+    Rng rng((uint64_t)geti(s, "seed", 2));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    grid_mesh(m, "hex", n, jit, rng, 1, 2);
+    int nh = m.count(HEX);
+    std::vector<int> order(nh);
+    for (int e = 0; e < nh; e++) {
+        order[e] = rng.uniform_int(pmin, pmax);
+        m.add_to_physreg(HEX, e, 100 + order[e]);
+    }
+    m.finalize();
+    parameter c;
+    std::vector<double> cv(nh);
+    for (int e = 0; e < nh; e++) cv[e] = std::exp(sigma * rng.normal());
+    c.setelementvalues(cv);
+    field u(m, "h1", "u");
+    for (int q = pmax; q >= pmin; q--)
+        if (m.has_physreg(100 + q)) u.setorder(100 + q, q);
+    u.setconstraint(2);
+    p->fields = {u};
+    p->params = {c};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    *p->form += integral(1, expression(c) * grad(dof(u)) * grad(tf(u)), -1);
+    *p->form += integral(1, -expression(1.0) * tf(u), -1);
+    std::ostringstream d;
+    d << "3D hp-Poisson, " << n << "^3 hexahedra, orders " << pmin << ".." << pmax << ", jitter " << jit;
+    p->describe = d.str();
+    return p;
+}
+
 std::shared_ptr<Problem> from_msh(const std::map<std::string, std::string>& s) {
     auto p = std::make_shared<Problem>();
     auto it = s.find("file");
@@ -277,6 +418,8 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     if (kind == "poisson2d") p = poisson2d(s);
     else if (kind == "elasticity3d") p = elasticity3d(s);
     else if (kind == "quadbatch") p = quadbatch(s);
+    else if (kind == "projection") p = projection(s);
+    else if (kind == "hppoisson3d") p = hppoisson3d(s);
     else if (kind == "msh") p = from_msh(s);
     else fail("unknown problem kind '" + kind + "'");
     p->kind = kind;

@@ -5,7 +5,7 @@
 
 #define SLX_MAX_SLICES 12
 #define SLX_MAX_FIELDS 4
-#define SLX_MAX_REGS 48
+#define SLX_MAX_REGS 64
 #define SLX_MAX_GEO 8
 
 namespace slx {

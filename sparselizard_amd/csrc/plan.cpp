@@ -11,6 +11,7 @@
 #include <vector>
 #include "../../include/slx.h"
 #include "host/basis.h"
+#include "host/gauss.h"
 #include "host/problems.h"
 #include "kernels/device.h"
 
@@ -76,7 +77,8 @@ template <typename T>
 void upload(DBuf& d, const T* src, size_t count, hipStream_t st) {
     (void)st;
     d.alloc(count * sizeof(T));
-    if (count) HIPCHECK(hipMemcpy(d.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    if (count && src) HIPCHECK(hipMemcpy(d.p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    else if (count) HIPCHECK(hipMemset(d.p, 0, count * sizeof(T)));
 }
 
 }  // namespace
@@ -113,6 +115,7 @@ struct slx_plan_s {
     int max_param = -1;            // highest parameter index read by the programs
     int64_t max_elem_id = 0;       // parameter arrays must hold this many entries
     bool uses_fields = false;
+    bool host_only = false;        // symbolic phase only (no device): pattern inspection
 
     DBuf* newbuf() {
         bufs.emplace_back(new DBuf());
@@ -310,6 +313,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
         P.max_erow_len = std::max(P.max_erow_len, tmax[t]);
         maxrow = std::max(maxrow, tmaxrow[t]);
     }
+    if (P.host_only) return;
     // device copies
     hipStream_t st = P.stream;
     upload(P.d_Arow, P.Arow.data(), P.Arow.size(), st);
@@ -498,11 +502,12 @@ slx_status slx_destroy(slx_ctx ctx) {
 
 slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out) {
     return guarded([&] {
-        SLX_CHECK(ctx && desc && out, "null argument");
-        HIPCHECK(hipSetDevice(ctx->device));
+        SLX_CHECK(desc && out, "null argument");
         auto P = std::make_unique<slx_plan_s>();
+        P->host_only = (ctx == nullptr);
+        if (!P->host_only) HIPCHECK(hipSetDevice(ctx->device));
         P->ctx = ctx;
-        P->stream = ctx->stream;
+        P->stream = ctx ? ctx->stream : nullptr;
         P->src = desc;
         P->n_dofs = desc->n_dofs;
         P->mesh_dim = desc->mesh_dim;
@@ -519,9 +524,9 @@ slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out
                 for (size_t k = 0; k < ndf; k++) SLX_CHECK(sd.dof_addr[k] < desc->n_dofs, "dof address out of range");
             }
         }
-        upload_buckets(*P, *desc);
+        if (!P->host_only) upload_buckets(*P, *desc);
         build_symbolic(*P, *desc);
-        HIPCHECK(hipStreamSynchronize(P->stream));
+        if (!P->host_only) HIPCHECK(hipStreamSynchronize(P->stream));
         *out = P.release();
     });
 }
@@ -548,6 +553,7 @@ slx_status slx_plan_stats(slx_plan p, int64_t* n_elem_rows, int64_t* n_ke, int64
 slx_status slx_assemble(slx_plan p, const slx_numeric_desc* in) {
     return guarded([&] {
         SLX_CHECK(p && in, "null argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         const slx_plan_desc& d = *p->src;
         upload(p->d_xyz, in->node_xyz, (size_t)3 * d.n_nodes, p->stream);
         SLX_CHECK(in->n_params > p->max_param, "numeric description has " << in->n_params << " parameters, programs read index " << p->max_param);
@@ -572,6 +578,7 @@ slx_status slx_assemble_device(slx_plan p, const double* d_node_xyz, const doubl
                                const double* const* d_field_data, int32_t precision, void* hip_stream) {
     return guarded([&] {
         SLX_CHECK(p && d_node_xyz, "null argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         hipStream_t st = hip_stream ? (hipStream_t)hip_stream : p->stream;
         numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, st);
     });
@@ -581,6 +588,7 @@ slx_status slx_plan_profile(slx_plan p, const double* d_node_xyz, const double* 
                             const double* const* d_field_data, int32_t precision, int32_t iters, double* ms) {
     return guarded([&] {
         SLX_CHECK(p && d_node_xyz && ms && iters > 0, "bad argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         hipEvent_t e0, e1, e2;
         HIPCHECK(hipEventCreate(&e0));
         HIPCHECK(hipEventCreate(&e1));
@@ -608,6 +616,7 @@ slx_status slx_plan_profile(slx_plan p, const double* d_node_xyz, const double* 
 slx_status slx_plan_sync(slx_plan p) {
     return guarded([&] {
         SLX_CHECK(p, "null plan");
+        if (p->host_only) return;
         HIPCHECK(hipStreamSynchronize(p->stream));
         HIPCHECK(hipDeviceSynchronize());
     });
@@ -631,7 +640,8 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
                            double* rhs) {
     return guarded([&] {
         SLX_CHECK(p, "null plan");
-        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (!p->host_only) HIPCHECK(hipStreamSynchronize(p->stream));
+        SLX_CHECK(!p->host_only || (!Aval && !Dval && !rhs), "plan was created without a device context: only the pattern is available");
         if (Arow) std::copy(p->Arow.begin(), p->Arow.end(), Arow);
         if (Drow) std::copy(p->Drow.begin(), p->Drow.end(), Drow);
         if (Acol) std::copy(p->Acol.begin(), p->Acol.end(), Acol);
@@ -645,8 +655,37 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
 slx_status slx_plan_destroy(slx_plan p) {
     return guarded([&] {
         if (!p) return;
-        (void)hipStreamSynchronize(p->stream);
+        if (!p->host_only) (void)hipStreamSynchronize(p->stream);
         delete p;
+    });
+}
+
+// ---- reference tables ----
+
+slx_status slx_gauss_rule(int32_t elem_type, int32_t order, int32_t cap, int32_t* npts, double* coords, double* weights) {
+    return guarded([&] {
+        SLX_CHECK(npts, "null argument");
+        GaussRule r = gauss_rule(elem_type, order);
+        *npts = r.count();
+        SLX_CHECK(r.count() <= cap || !coords, "output capacity " << cap << " below " << r.count() << " points");
+        if (coords) std::copy(r.coords.begin(), r.coords.end(), coords);
+        if (weights) std::copy(r.weights.begin(), r.weights.end(), weights);
+    });
+}
+
+slx_status slx_h1_count(int32_t elem_type, int32_t order, int32_t* nff) {
+    return guarded([&] {
+        SLX_CHECK(nff, "null argument");
+        *nff = h1_count(elem_type, order);
+    });
+}
+
+slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t npts, const double* pts, double* out) {
+    return guarded([&] {
+        SLX_CHECK(pts && out && npts >= 0, "bad argument");
+        std::vector<double> p(pts, pts + 3 * (size_t)npts), o;
+        h1_tables(elem_type, order, total_orientation, p, o);
+        std::copy(o.begin(), o.end(), out);
     });
 }
 
@@ -683,7 +722,7 @@ slx_status slx_problem_destroy(slx_problem p) {
 
 slx_status slx_problem_plan(slx_problem p, slx_plan* plan_out) {
     return guarded([&] {
-        SLX_CHECK(p && p->ctx, "problem has no device context");
+        SLX_CHECK(p, "null problem");
         if (!p->plan) {
             slx_plan pl = nullptr;
             if (slx_plan_create(p->ctx, &p->prob->plan->desc, &pl) != SLX_OK) throw std::runtime_error(g_last_error);

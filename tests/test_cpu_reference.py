@@ -1,0 +1,413 @@
+"""CPU tests: the oracle pinned by closed forms and invariants, the host
+numbering pipeline, the C ABI exports and the host symbolic phase (CSR
+pattern) against the oracle's restatement of rawmat::process.
+
+No GPU is used here: problems are built by the C++ host layer and plans are
+created without a device context (pattern only).
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import sparselizard_amd as sa
+from sparselizard_amd import _lib as L
+from oracle import py_oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+TRI, QUAD, TET, HEX, LINE = 2, 3, 4, 5, 1
+
+
+def oracle_of(prob):
+    info = prob.info()
+    return info, py_oracle.assemble(prob.desc_ptr(), prob.numeric(), info["ndofs"], info["nfree"])
+
+
+def csr(o, nf, nc):
+    A = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(nf, nf))
+    D = sp.csr_matrix((o.Dval, o.Dcol, o.Drow), shape=(nf, nc))
+    return A, D
+
+
+# ---------------------------------------------------------------- ABI -------
+
+def test_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "slx.h")).read()
+    names = set(re.findall(r"\b(slx_[a-z0-9_]+)\s*\(", hdr))
+    lib = C.CDLL(L.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(L.EXPORTED) >= names
+
+
+def test_error_status_and_message():
+    with pytest.raises(sa.SlxError, match="unknown problem kind"):
+        sa.Problem(None, "nosuchkind", "")
+    with pytest.raises(sa.SlxError, match="Unable to open"):
+        sa.Problem(None, "msh", "file=/nonexistent.msh")
+
+
+def test_negative_integration_order_rejected(tmp_path):
+    msh = write_msh(tmp_path / "one.msh", [(0, 0), (1, 0), (0, 1)], [(TRI, 1, (1, 2, 3))])
+    with pytest.raises(sa.SlxError, match="negative order"):
+        sa.Problem(None, "msh", "file=%s;delta=-9" % msh)
+
+
+# ---------------------------------------------------------------- Gauss -----
+
+@pytest.mark.parametrize("etype,measure,maxorder", [(LINE, 2.0, 59), (TRI, 0.5, 20), (QUAD, 4.0, 20),
+                                                     (TET, 1 / 6, 13), (HEX, 8.0, 9)])
+def test_gauss_weight_sums(etype, measure, maxorder):
+    for order in range(0, maxorder + 1):
+        x, w = L.gauss_rule(etype, order)
+        assert abs(w.sum() - measure) < 1e-10 * measure, (etype, order)
+
+
+def _mono_integral(etype, a, b, c):
+    from math import factorial
+    if etype == TRI:
+        return factorial(a) * factorial(b) / factorial(a + b + 2)
+    if etype == TET:
+        return factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+    f = lambda k: 0.0 if k % 2 else 2.0 / (k + 1)
+    if etype == LINE:
+        return f(a)
+    if etype == QUAD:
+        return f(a) * f(b)
+    return f(a) * f(b) * f(c)
+
+
+@pytest.mark.parametrize("etype,orders,tol", [(LINE, range(0, 30), 1e-13), (TRI, range(0, 12), 1e-11),
+                                              (QUAD, range(0, 12), 1e-13), (TET, range(0, 10), 1e-10),
+                                              (HEX, range(0, 8), 1e-13)])
+def test_gauss_exactness(etype, orders, tol):
+    # the rule of order k integrates every monomial of total degree <= k exactly
+    for k in orders:
+        x, w = L.gauss_rule(etype, k)
+        dim = {LINE: 1, TRI: 2, QUAD: 2, TET: 3, HEX: 3}[etype]
+        for a in range(k + 1):
+            for b in range(k + 1 - a if dim > 1 else 1):
+                for c in range(k + 1 - a - b if dim > 2 else 1):
+                    val = np.sum(w * x[:, 0] ** a * x[:, 1] ** b * x[:, 2] ** c)
+                    ref = _mono_integral(etype, a, b, c)
+                    assert abs(val - ref) <= tol * max(1.0, abs(ref)), (etype, k, a, b, c, val, ref)
+
+
+def test_gauss_counts_of_configs():
+    # tri order 4 -> 6, tri order 6 -> 12, tet order 6 -> 24 (gptriangle.cpp:5, gptetrahedron.cpp:5)
+    assert len(L.gauss_rule(TRI, 4)[1]) == 6
+    assert len(L.gauss_rule(TRI, 6)[1]) == 12
+    assert len(L.gauss_rule(TET, 6)[1]) == 24
+    assert len(L.gauss_rule(QUAD, 8)[1]) == 25
+    for p in range(1, 6):
+        assert len(L.gauss_rule(HEX, 2 * p + 1)[1]) == (p + 1) ** 3
+
+
+# ---------------------------------------------------------------- H1 bases --
+
+REF_VERTS = {
+    TRI: np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float),
+    QUAD: np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], float),
+    TET: np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float),
+    HEX: np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float),
+}
+
+
+def interior_points(etype, n=7, seed=0):
+    rng = np.random.default_rng(seed)
+    if etype in (TRI, TET):
+        d = 2 if etype == TRI else 3
+        p = rng.dirichlet(np.ones(d + 1), size=n)[:, 1:]
+        out = np.zeros((n, 3))
+        out[:, :d] = p
+        return out
+    d = 2 if etype == QUAD else 3
+    out = np.zeros((n, 3))
+    out[:, :d] = rng.uniform(-0.9, 0.9, size=(n, d))
+    return out
+
+
+@pytest.mark.parametrize("etype", [TRI, QUAD, TET, HEX])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_h1_counts_partition_and_vertices(etype, order):
+    nv = len(REF_VERTS[etype])
+    nff = L.h1_count(etype, order)
+    expect = {TRI: (order + 1) * (order + 2) // 2, QUAD: (order + 1) ** 2,
+              TET: (order + 1) * (order + 2) * (order + 3) // 6, HEX: (order + 1) ** 3}[etype]
+    assert nff == expect
+    pts = interior_points(etype)
+    t = L.h1_tables(etype, order, pts)
+    # vertex functions are the barycentric / multilinear hats: partition of unity
+    assert np.allclose(t[0, :nv].sum(0), 1.0, atol=1e-14)
+    assert np.allclose(t[1:, :nv].sum(1), 0.0, atol=1e-13)
+    # at the vertices: hats are Kronecker deltas, every higher function vanishes
+    tv = L.h1_tables(etype, order, REF_VERTS[etype])
+    assert np.allclose(tv[0, :nv], np.eye(nv), atol=1e-14)
+    assert np.allclose(tv[0, nv:], 0.0, atol=1e-13)
+
+
+@pytest.mark.parametrize("etype", [TRI, QUAD, TET, HEX])
+@pytest.mark.parametrize("order", [2, 3, 5])
+def test_h1_derivatives_match_finite_differences(etype, order):
+    pts = interior_points(etype, n=4, seed=3)
+    t = L.h1_tables(etype, order, pts, total_orientation=5)
+    dim = 2 if etype in (TRI, QUAD) else 3
+    h = 1e-6
+    for k in range(dim):
+        dp = pts.copy(); dp[:, k] += h
+        dm = pts.copy(); dm[:, k] -= h
+        fd = (L.h1_tables(etype, order, dp, 5)[0] - L.h1_tables(etype, order, dm, 5)[0]) / (2 * h)
+        assert np.allclose(t[1 + k], fd, atol=2e-7 * max(1, np.abs(fd).max()))
+
+
+def test_h1_p2_edge_function_closed_forms():
+    # simplex edge functions of order 2: Ls_2(l1 - l2, l1 + l2) = -2 l1 l2 (legendre.cpp:48-63)
+    pts = interior_points(TRI, n=5)
+    t = L.h1_tables(TRI, 2, pts)
+    lam = np.stack([1 - pts[:, 0] - pts[:, 1], pts[:, 0], pts[:, 1]])
+    for e, (a, b) in enumerate([(0, 1), (1, 2), (2, 0)]):
+        assert np.allclose(t[0, 3 + e], -2 * lam[a] * lam[b], atol=1e-15)
+    # quadrangle edge 0 of order 2: L_2(s1 - s2)(l1 + l2), L_2(x) = (x^2 - 1)/2 on the shifted element
+    q = interior_points(QUAD, n=5)
+    tq = L.h1_tables(QUAD, 2, q)
+    x, y = 0.5 * (q[:, 0] + 1), 0.5 * (q[:, 1] + 1)
+    s1, s2 = (1 - x) + (1 - y), x + (1 - y)
+    l12 = (1 - x) * (1 - y) + x * (1 - y)
+    assert np.allclose(tq[0, 4], 0.5 * ((s1 - s2) ** 2 - 1) * l12, atol=1e-15)
+
+
+def test_h1_orientation_variants_are_permutations():
+    # order-3 edge functions change sign with the edge orientation: Ls_3 is odd
+    pts = interior_points(TRI, n=5)
+    t0 = L.h1_tables(TRI, 3, pts, total_orientation=0)
+    t1 = L.h1_tables(TRI, 3, pts, total_orientation=1)  # edge 0 flipped
+    # iterator order: 3 vertices, edge0 (p2, p3), edge1 (p2, p3), edge2 (p2, p3), face (p3)
+    assert np.allclose(t1[0, 3], t0[0, 3])       # order 2: even
+    assert np.allclose(t1[0, 4], -t0[0, 4])      # order 3: odd
+    assert np.allclose(t1[0, 5:], t0[0, 5:])
+
+
+# ---------------------------------------------------------------- numbering --
+
+def write_msh(path, nodes, elems):
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n%d\n" % len(nodes))
+        for i, p in enumerate(nodes):
+            x, y = p[0], p[1]
+            z = p[2] if len(p) > 2 else 0.0
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, x, y, z))
+        f.write("$EndNodes\n$Elements\n%d\n" % len(elems))
+        gm = {LINE: 1, TRI: 2, QUAD: 3, TET: 4, HEX: 5}
+        for i, (t, phys, nl) in enumerate(elems):
+            f.write("%d %d 2 %d %d %s\n" % (i + 1, gm[t], phys, phys, " ".join(map(str, nl))))
+        f.write("$EndElements\n")
+    return str(path)
+
+
+def test_numbering_two_triangles_order2(tmp_path):
+    """Hand-derived from the reference pipeline (elements.cpp:1115-1391,
+    dofmanager.cpp:35-123): nodes 0..3 form disjoint region 0 (in both regions),
+    the 4 boundary edges region 1 and the diagonal region 2.  DOFs: vertices
+    0..3, boundary edges 4..7 (in file order), diagonal 8.  Constraining the
+    boundary leaves the diagonal edge function as the only free DOF."""
+    nodes = [(0, 0), (1, 0), (0, 1), (1, 1)]
+    elems = [(LINE, 2, (1, 2)), (LINE, 2, (2, 4)), (LINE, 2, (4, 3)), (LINE, 2, (3, 1)),
+             (TRI, 1, (1, 2, 4)), (TRI, 1, (1, 4, 3))]
+    msh = write_msh(tmp_path / "two.msh", nodes, elems)
+    p = sa.Problem(None, "msh", "file=%s;order=2;dirichlet=2" % msh)
+    info = p.info()
+    assert info["ndofs"] == 9 and info["nfree"] == 1
+    d = p.desc()
+    b = d.buckets[1] if d.buckets[0].slices[0].is_rhs else d.buckets[0]
+    sl = b.slices[0]
+    ne = b.n_elem
+    tf = np.ctypeslib.as_array(sl.tf_addr, shape=(sl.n_tf * ne,)).reshape(sl.n_tf, ne)
+    # triangle 0 = nodes (0,1,3): vertices 0,1,3; edges (0,1)->line0->dof4, (1,3)->line1->dof5, (3,0)->diagonal->dof8
+    assert list(tf[:, 0]) == [0, 1, 3, 4, 5, 8]
+    # triangle 1 = nodes (0,3,2): edges (0,3)->diagonal 8, (3,2)->line2->dof6, (2,0)->line3->dof7
+    assert list(tf[:, 1]) == [0, 3, 2, 8, 6, 7]
+
+
+def test_numbering_free_dofs_last_for_dirichlet_square():
+    # boundary nodes sit in the first node disjoint region (they belong to both
+    # physical regions), so interior DOFs come last
+    p = sa.Problem(None, "poisson2d", "n=4")
+    info = p.info()
+    c = np.ctypeslib.as_array(p.desc().constrained, shape=(info["ndofs"],))
+    assert info["ndofs"] == 25 and info["nfree"] == 9
+    assert np.all(c[:16] == 1) and np.all(c[16:] == 0)
+
+
+def test_term_structure():
+    # extractdoftf: 2D grad.grad -> 4 entries; 3D elasticity -> 9 slices x 9 entries
+    p2 = sa.Problem(None, "poisson2d", "n=3")  # keep the problem alive: desc() is a view
+    d = p2.desc()
+    mats = [d.buckets[b] for b in range(d.n_buckets) if not d.buckets[b].slices[0].is_rhs]
+    assert len(mats) == 1 and mats[0].n_slices == 1 and mats[0].slices[0].n_terms == 4
+    p3 = sa.Problem(None, "elasticity3d", "n=1")
+    d = p3.desc()
+    mats = [d.buckets[b] for b in range(d.n_buckets) if not d.buckets[b].slices[0].is_rhs]
+    assert len(mats) == 1 and mats[0].n_slices == 9
+    assert all(mats[0].slices[s].n_terms == 9 for s in range(9))
+    assert mats[0].n_gp == 24 and mats[0].q_invariant == 1
+
+
+# ---------------------------------------------------------------- oracle -----
+
+@pytest.mark.parametrize("elem,order", [(e, o) for e in ("tri", "quad", "tet", "hex") for o in (1, 2, 3, 4)])
+def test_oracle_projection_reproduces_polynomials(elem, order):
+    """L2 projection of a degree-p polynomial onto the order-p space is exact:
+    pins bases, orientations (continuity), numbering, Gauss rules, Jacobians
+    and the assembly together.  Vertex DOFs must equal f at the vertices."""
+    n = 3 if elem in ("tet", "hex") else 4
+    p = sa.Problem(None, "projection", "elem=%s;n=%d;order=%d;deg=%d" % (elem, n, order, order))
+    info, o = oracle_of(p)
+    nf = info["nfree"]
+    M = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(nf, nf))
+    u = spl.spsolve(M.tocsc(), o.rhs)
+    nn = p.n_nodes()
+    xyz = np.ctypeslib.as_array(p.numeric().node_xyz, shape=(3 * nn,)).reshape(-1, 3)
+    is3 = elem in ("tet", "hex")
+    f = np.zeros(nn)
+    for a in range(order + 1):
+        for b in range(order + 1 - a):
+            for c in range(order + 1 - a - b):
+                if c and not is3:
+                    continue
+                f += 1 / (1 + a + 2 * b + 3 * c) * xyz[:, 0] ** a * xyz[:, 1] ** b * xyz[:, 2] ** c
+    assert np.abs(u[:nn] - f).max() < 1e-8
+
+
+def test_oracle_p1_stiffness_is_cotangent_formula(tmp_path):
+    # single triangle, no constraints: K_ij = -(cot of the angle opposite ij)/2, K_ii = -sum_j K_ij
+    P = np.array([(0.1, 0.2), (1.3, 0.1), (0.4, 0.9)])
+    msh = write_msh(tmp_path / "t.msh", [tuple(x) for x in P], [(TRI, 1, (1, 2, 3))])
+    p = sa.Problem(None, "msh", "file=%s;order=1" % msh)
+    info, o = oracle_of(p)
+    K = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(3, 3)).toarray()
+    ref = np.zeros((3, 3))
+    for k in range(3):
+        i, j = (k + 1) % 3, (k + 2) % 3
+        u, v = P[i] - P[k], P[j] - P[k]
+        cot = np.dot(u, v) / abs(u[0] * v[1] - u[1] * v[0])
+        ref[i, j] = ref[j, i] = -0.5 * cot
+    for i in range(3):
+        ref[i, i] = -ref[i].sum()
+    assert np.allclose(K, ref, rtol=1e-13, atol=1e-15)
+    # rhs of f = 1: each hat integrates to area / 3
+    area = 0.5 * abs((P[1] - P[0])[0] * (P[2] - P[0])[1] - (P[1] - P[0])[1] * (P[2] - P[0])[0])
+    assert np.allclose(o.rhs, area / 3, rtol=1e-13)
+
+
+def test_oracle_p1_mass_closed_form(tmp_path):
+    P = np.array([(0.0, 0.0), (2.0, 0.3), (0.5, 1.1)])
+    msh = write_msh(tmp_path / "m.msh", [tuple(x) for x in P], [(TRI, 1, (1, 2, 3))])
+    p = sa.Problem(None, "msh", "file=%s;order=1;integrand=mass" % msh)
+    info, o = oracle_of(p)
+    M = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(3, 3)).toarray()
+    area = 0.5 * abs((P[1] - P[0])[0] * (P[2] - P[0])[1] - (P[1] - P[0])[1] * (P[2] - P[0])[0])
+    assert np.allclose(M, area / 12 * (np.ones((3, 3)) + np.eye(3)), rtol=1e-13)
+
+
+def test_oracle_poisson_rows_sum_to_zero_and_symmetric():
+    p = sa.Problem(None, "poisson2d", "n=7;order=2")
+    info, o = oracle_of(p)
+    nf, nc = info["nfree"], info["ndofs"] - info["nfree"]
+    A, D = csr(o, nf, nc)
+    assert abs(A - A.T).max() < 1e-14
+    # constants are in the kernel: the constant function has coefficient 1 on
+    # every vertex DOF and 0 on the hierarchical edge/face DOFs
+    nn = p.n_nodes()
+    c = np.ctypeslib.as_array(p.desc().constrained, shape=(info["ndofs"],))
+    ones = np.zeros(info["ndofs"])
+    ones[:nn] = 1.0  # vertex DOFs come first (node disjoint regions are numbered first)
+    r = A @ ones[c == 0] + D @ ones[c == 1]
+    assert np.abs(r).max() < 1e-12
+
+
+def test_oracle_elasticity_rigid_body_modes():
+    # unconstrained 3D elasticity operator: exactly 6 zero eigenvalues, symmetric, negative semidefinite
+    p = sa.Problem(None, "elasticity3d", "n=1;clamp=0")
+    info, o = oracle_of(p)
+    nf = info["nfree"]
+    K = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(nf, nf)).toarray()
+    assert np.abs(K - K.T).max() < 1e-13 * np.abs(K).max()
+    ev = np.sort(sla.eigvalsh(-K))
+    assert np.all(np.abs(ev[:6]) < 1e-11 * ev[-1])
+    assert ev[6] > 1e-6 * ev[-1]
+
+
+def test_oracle_elasticity_p1_trace_matches_btcb():
+    """P1 tets: the isotropic stiffness integrates exactly, so the trace of the
+    assembled operator (independent of the DOF numbering) equals the sum of
+    the traces of independent B^T C B element matrices."""
+    p = sa.Problem(None, "elasticity3d", "n=1;order=1;clamp=0;sigma=0")
+    info, o = oracle_of(p)
+    nf = info["nfree"]
+    K = sp.csr_matrix((o.Aval, o.Acol, o.Arow), shape=(nf, nf)).toarray()
+    d = p.desc()
+    b = [d.buckets[i] for i in range(d.n_buckets) if not d.buckets[i].slices[0].is_rhs][0]
+    nn = p.n_nodes()
+    xyz = np.ctypeslib.as_array(p.numeric().node_xyz, shape=(3 * nn,)).reshape(-1, 3)
+    geo = np.ctypeslib.as_array(b.geo_nodes, shape=(b.n_elem * 4,)).reshape(-1, 4)
+    E, nu = 1.0, 0.3
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    Cm = np.zeros((6, 6))
+    Cm[:3, :3] = lam
+    Cm[np.arange(3), np.arange(3)] += 2 * mu
+    Cm[3:, 3:] = mu * np.eye(3)
+    tr = 0.0
+    for e in range(b.n_elem):
+        X = xyz[geo[e]]
+        Jm = np.array([X[1] - X[0], X[2] - X[0], X[3] - X[0]])
+        vol = abs(np.linalg.det(Jm)) / 6
+        G = np.linalg.solve(Jm, np.array([[-1, 1, 0, 0], [-1, 0, 1, 0], [-1, 0, 0, 1]], float))
+        B = np.zeros((6, 12))
+        for a in range(4):
+            gx, gy, gz = G[:, a]
+            B[:, 3 * a:3 * a + 3] = [[gx, 0, 0], [0, gy, 0], [0, 0, gz], [0, gz, gy], [gz, 0, gx], [gy, gx, 0]]
+        tr += np.trace(-vol * B.T @ Cm @ B)
+    # the reference's order-4 tetrahedron rule (11 points) is tabulated to ~15
+    # digits: its weights sum to 1/6 only within 3.4e-12 relative
+    assert abs(np.trace(K) - tr) < 1e-11 * abs(tr)
+
+
+def test_oracle_rhs_body_force_total():
+    # integral(f . tf(u)) with f = (0,0,-1): the rhs holds -(coef * tf) = +integral of the
+    # hats of the z block (contribution.cpp:296); P1 hats sum to 1, volume 1.  The
+    # reference's order-4 tetrahedron rule sums to 1/6 within 3.4e-12 relative.
+    p = sa.Problem(None, "elasticity3d", "n=2;order=1;clamp=0")
+    info, o = oracle_of(p)
+    assert abs(o.rhs.sum() - 1.0) < 1e-11
+
+
+# ---------------------------------------------------------------- pattern ----
+
+@pytest.mark.parametrize("kind,spec", [
+    ("poisson2d", "n=6"), ("poisson2d", "n=5;order=2"), ("poisson2d", "n=4;order=3"),
+    ("elasticity3d", "n=2"), ("quadbatch", "instances=3;nmin=2;nmax=4"),
+    ("hppoisson3d", "n=3"), ("projection", "elem=hex;n=2;order=3"),
+])
+def test_host_symbolic_pattern_equals_oracle(kind, spec):
+    """The plan's CSR pattern (per-row sorted unique columns, A/D split with
+    local numbering) equals the oracle's rawmat::process restatement."""
+    p = sa.Problem(None, kind, spec)
+    info, o = oracle_of(p)
+    plan = p.plan()
+    s = plan.shape()
+    assert s["nfree"] == info["nfree"]
+    assert s["nnzA"] == len(o.Aval) and s["nnzD"] == len(o.Dval)
+    nf = info["nfree"]
+    Arow = np.zeros(nf + 1, np.int64)
+    Drow = np.zeros(nf + 1, np.int64)
+    Acol = np.zeros(max(s["nnzA"], 1), np.int32)
+    Dcol = np.zeros(max(s["nnzD"], 1), np.int32)
+    L.check(L.lib().slx_csr_copyout(plan.h, L._p(Arow, C.c_int64), L._p(Acol, C.c_int32), None,
+                                    L._p(Drow, C.c_int64), L._p(Dcol, C.c_int32), None, None))
+    assert np.array_equal(Arow, o.Arow) and np.array_equal(Drow, o.Drow)
+    assert np.array_equal(Acol[:s["nnzA"]], o.Acol) and np.array_equal(Dcol[:s["nnzD"]], o.Dcol)
