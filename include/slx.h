@@ -185,7 +185,15 @@ slx_status slx_csr_copyout(slx_plan plan, int64_t* Arow, int32_t* Acol, double* 
 /* Element matrices of the last assembly, in the reduction buffer layout
  * (for error measurement); n_out receives the entry count. */
 slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, int64_t* n_coo);
+/* Buckets whose matrix slices run on the MFMA split kernel (the others and
+ * every right-hand-side slice use the VALU element kernel). */
+slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_buckets);
 slx_status slx_plan_destroy(slx_plan plan);
+
+/* Diagnostic: runs one v_mfma_f32_16x16x32_f16 on fixed operands and returns
+ * the 16x16 result (row-major) so the operand lane layout the kernels assume
+ * can be checked against a host product. */
+slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256);
 
 /* ---- Reference tables ------------------------------------------------------
  * gausspoints(type, order) (src/gausspoint/gausspoints.cpp:4-54): writes up to

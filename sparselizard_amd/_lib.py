@@ -99,6 +99,7 @@ def lib():
         "slx_csr_device": (i32, [vp] + [C.POINTER(vp)] * 7),
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
         "slx_plan_destroy": (i32, [vp]),
+        "slx_plan_kernels": (i32, [vp, p32, p32]),
         "slx_problem_create": (i32, [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]),
         "slx_problem_destroy": (i32, [vp]),
         "slx_problem_plan": (i32, [vp, C.POINTER(vp)]),
@@ -108,6 +109,7 @@ def lib():
         "slx_problem_desc": (i32, [vp, C.POINTER(vp)]),
         "slx_problem_setdata": (i32, [vp, i32, pd, i64]),
         "slx_gauss_rule": (i32, [i32, i32, i32, p32, pd, pd]),
+        "slx_selftest_mfma_layout": (i32, [vp, C.POINTER(C.c_float)]),
         "slx_h1_count": (i32, [i32, i32, p32]),
         "slx_h1_tables": (i32, [i32, i32, i32, i32, pd, pd]),
     }
@@ -122,10 +124,17 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata",
-    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout",
 ]
+
+
+def mfma_layout_probe(ctx):
+    """16x16 result of the MFMA layout probe (see slx_selftest_mfma_layout)."""
+    out = np.zeros(256, np.float32)
+    check(lib().slx_selftest_mfma_layout(ctx.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+    return out.reshape(16, 16)
 
 
 def gauss_rule(elem_type, order):
@@ -266,6 +275,11 @@ class Plan:
         v = [C.c_int64() for _ in range(3)]
         check(lib().slx_plan_stats(self.h, *[C.byref(x) for x in v]))
         return dict(n_elem_rows=v[0].value, n_ke=v[1].value, n_coo=v[2].value)
+
+    def kernels(self):
+        a, b = C.c_int32(), C.c_int32()
+        check(lib().slx_plan_kernels(self.h, C.byref(a), C.byref(b)))
+        return dict(mfma_buckets=a.value, buckets=b.value)
 
     def assemble(self, numeric, precision):
         numeric.precision = PRECISIONS[precision] if isinstance(precision, str) else precision

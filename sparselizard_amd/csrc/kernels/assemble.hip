@@ -147,7 +147,7 @@ __device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int 
 #define SLX_JBLK 16
 
 template <typename CT, typename KT>
-__global__ void __launch_bounds__(256) elem_kernel(DevBucket b, DevInputs in, int ne_tile, KT* __restrict__ ke) {
+__global__ void __launch_bounds__(256) elem_kernel(DevBucket b, DevInputs in, int ne_tile, uint32_t slice_mask, KT* __restrict__ ke) {
     extern __shared__ unsigned char smem_raw[];
     CT* C = reinterpret_cast<CT*>(smem_raw);  // [ne_tile][nq][n_coefs]
     const int nq = b.q_inv ? 1 : b.n_gp;
@@ -165,7 +165,8 @@ __global__ void __launch_bounds__(256) elem_kernel(DevBucket b, DevInputs in, in
     items_before[0] = 0;
     for (int s = 0; s < b.n_slices; s++) {
         int nj = (b.slices[s].n_dof + SLX_JBLK - 1) / SLX_JBLK;
-        items_before[s + 1] = items_before[s] + nloc * b.slices[s].n_tf * nj;
+        int n = ((slice_mask >> s) & 1u) ? nloc * b.slices[s].n_tf * nj : 0;
+        items_before[s + 1] = items_before[s] + n;
     }
     int total = items_before[b.n_slices];
     for (int it = threadIdx.x; it < total; it += blockDim.x) {
@@ -254,6 +255,104 @@ __global__ void reduce_rhs(DevReduce rd, const KT* __restrict__ ke, double* __re
     rhs[rd.free_dofs[r]] = s;
 }
 
+// FP64 element kernel, factorized and register-blocked ("lanes = elements").
+// A workgroup takes 64 elements; every wave walks work combos (slice, row
+// tile, column tile) that are uniform across its lanes, so the shape-function
+// tables are read through the scalar unit while each lane owns one element:
+//   H_r[j] = w_q sum_{t: r_t = r} C_t(e,q) dof_{s_t}[j][q]
+//   Ke[i][j] += sum_r tf_r[i][q] H_r[j]
+// which is the reference's sum over terms and Gauss points regrouped
+// (B^T D B with D gathered per tf derivative).
+template <int TI, int TJ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) elem64_tiled(DevBucket b, DevInputs in, uint32_t slice_mask, double* __restrict__ ke) {
+    extern __shared__ unsigned char smem_raw[];
+    double* C = reinterpret_cast<double*>(smem_raw);  // [64][nq][n_coefs]
+    const int NE = 64;
+    const int nq = b.q_inv ? 1 : b.n_gp;
+    const int e0 = blockIdx.x * NE;
+    const int nloc = min(NE, b.n_elem - e0);
+    for (int idx = threadIdx.x; idx < nloc * nq; idx += blockDim.x) {
+        int el = idx % nloc, q = idx / nloc;
+        Geo g;
+        geometry(b, in.xyz, e0 + el, q, g);
+        run_program<double>(b, in, e0 + el, q, g, C + ((size_t)el * nq + q) * b.n_coefs, 1);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // combos: for each masked slice, row tiles x column tiles
+    int ncombo = 0;
+    for (int s = 0; s < b.n_slices; s++)
+        if ((slice_mask >> s) & 1u) ncombo += ((b.slices[s].n_tf + TI - 1) / TI) * ((b.slices[s].n_dof + TJ - 1) / TJ);
+    for (int c = wave; c < ncombo; c += 4) {
+        // decode the combo (uniform)
+        int s = 0, rem = c;
+        for (; s < b.n_slices; s++) {
+            if (!((slice_mask >> s) & 1u)) continue;
+            int n = ((b.slices[s].n_tf + TI - 1) / TI) * ((b.slices[s].n_dof + TJ - 1) / TJ);
+            if (rem < n) break;
+            rem -= n;
+        }
+        const DevSlice& sl = b.slices[s];
+        const int njt = (sl.n_dof + TJ - 1) / TJ;
+        const int i0 = (rem / njt) * TI, j0 = (rem % njt) * TJ;
+        const int ni = min(TI, sl.n_tf - i0), nj = min(TJ, sl.n_dof - j0);
+        double acc[TI][TJ];
+#pragma unroll
+        for (int ii = 0; ii < TI; ii++)
+#pragma unroll
+            for (int jj = 0; jj < TJ; jj++) acc[ii][jj] = 0.0;
+        const int el = lane;
+        const double* Ce = C + (size_t)min(el, NE - 1) * nq * b.n_coefs;
+        for (int q = 0; q < b.n_gp; q++) {
+            const double* Cq = Ce + (size_t)(b.q_inv ? 0 : q) * b.n_coefs;
+            const double w = b.gp_w[q];
+            double h[4][TJ];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int jj = 0; jj < TJ; jj++) h[r][jj] = 0.0;
+            unsigned rmask = 0;
+            for (int t = 0; t < sl.n_terms; t++) {
+                const slx_term tm = sl.terms[t];
+                const double cw = Cq[tm.coef] * w;
+                rmask |= 1u << tm.tf_deriv;
+                if (sl.is_rhs) {
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (r == tm.tf_deriv) h[r][0] += cw;
+                } else {
+                    const double* dt = sl.dof_tab + ((size_t)tm.dof_deriv * sl.n_dof + j0) * b.n_gp + q;
+#pragma unroll
+                    for (int r = 0; r < 4; r++)
+                        if (r == tm.tf_deriv)
+#pragma unroll
+                            for (int jj = 0; jj < TJ; jj++) h[r][jj] += jj < nj ? cw * dt[(size_t)jj * b.n_gp] : 0.0;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                if (!((rmask >> r) & 1u)) continue;
+                const double* tt = sl.tf_tab + ((size_t)r * sl.n_tf + i0) * b.n_gp + q;
+#pragma unroll
+                for (int ii = 0; ii < TI; ii++) {
+                    const double tv = ii < ni ? tt[(size_t)ii * b.n_gp] : 0.0;
+#pragma unroll
+                    for (int jj = 0; jj < TJ; jj++) acc[ii][jj] += tv * h[r][jj];
+                }
+            }
+        }
+        if (el < nloc) {
+            const size_t base = sl.ke_base + ((size_t)(e0 + el) * sl.n_tf + i0) * sl.n_dof + j0;
+#pragma unroll
+            for (int ii = 0; ii < TI; ii++)
+#pragma unroll
+                for (int jj = 0; jj < TJ; jj++)
+                    if (ii < ni && jj < nj) ke[base + (size_t)ii * sl.n_dof + jj] = sl.is_rhs ? -acc[ii][jj] : acc[ii][jj];
+        }
+    }
+}
+
 // ---------------------------------------------------------------- launchers --
 
 int elem_tile_size(const DevBucket& b, size_t elem_bytes, size_t lds_budget) {
@@ -265,22 +364,40 @@ int elem_tile_size(const DevBucket& b, size_t elem_bytes, size_t lds_budget) {
     return t;
 }
 
-hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st) {
-    if (b.n_elem == 0) return hipSuccess;
+template <int T>
+static hipError_t launch_tiled64(const DevBucket& b, const DevInputs& in, uint32_t mask, double* ke, hipStream_t st, size_t lds) {
+    int grid = (b.n_elem + 63) / 64;
+    hipLaunchKernelGGL((elem64_tiled<T, T>), dim3(grid), dim3(256), lds, st, b, in, mask, ke);
+    return hipGetLastError();
+}
+
+hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, uint32_t slice_mask) {
+    if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
     if (precision == SLX_PREC_FP64) {
-        int tile = elem_tile_size(b, sizeof(double), 48 * 1024);
+        // lanes-as-elements kernel when the 64-element coefficient table fits
         int nq = b.q_inv ? 1 : b.n_gp;
+        size_t lds64 = (size_t)64 * nq * b.n_coefs * sizeof(double);
+        int nmax = 0;
+        for (int s = 0; s < b.n_slices; s++)
+            if ((slice_mask >> s) & 1u) nmax = max(nmax, max(b.slices[s].n_tf, b.slices[s].n_dof));
+        if (lds64 <= 48 * 1024 && b.n_coefs > 0) {
+            if (nmax <= 4) return launch_tiled64<4>(b, in, slice_mask, (double*)ke, st, lds64);
+            if (nmax <= 6) return launch_tiled64<6>(b, in, slice_mask, (double*)ke, st, lds64);
+            if (nmax <= 10) return launch_tiled64<10>(b, in, slice_mask, (double*)ke, st, lds64);
+            return launch_tiled64<8>(b, in, slice_mask, (double*)ke, st, lds64);
+        }
+        int tile = elem_tile_size(b, sizeof(double), 48 * 1024);
         size_t lds = (size_t)tile * nq * b.n_coefs * sizeof(double);
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         int grid = (b.n_elem + tile - 1) / tile;
-        hipLaunchKernelGGL((elem_kernel<double, double>), dim3(grid), dim3(256), lds, st, b, in, tile, (double*)ke);
+        hipLaunchKernelGGL((elem_kernel<double, double>), dim3(grid), dim3(256), lds, st, b, in, tile, slice_mask, (double*)ke);
     } else {
         int tile = elem_tile_size(b, sizeof(float), 48 * 1024);
         int nq = b.q_inv ? 1 : b.n_gp;
         size_t lds = (size_t)tile * nq * b.n_coefs * sizeof(float);
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         int grid = (b.n_elem + tile - 1) / tile;
-        hipLaunchKernelGGL((elem_kernel<float, float>), dim3(grid), dim3(256), lds, st, b, in, tile, (float*)ke);
+        hipLaunchKernelGGL((elem_kernel<float, float>), dim3(grid), dim3(256), lds, st, b, in, tile, slice_mask, (float*)ke);
     }
     return hipGetLastError();
 }

@@ -14,10 +14,11 @@
 #include "host/gauss.h"
 #include "host/problems.h"
 #include "kernels/device.h"
+#include "kernels/mfma.h"
+#include <cmath>
 
 namespace slx {
-hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st);
-hipError_t launch_elem_mfma(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, bool* handled);
+hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, uint32_t slice_mask);
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
                          double* rhs, bool has_matrix, bool has_rhs, hipStream_t st);
 }  // namespace slx
@@ -116,6 +117,10 @@ struct slx_plan_s {
     int64_t max_elem_id = 0;       // parameter arrays must hold this many entries
     bool uses_fields = false;
     bool host_only = false;        // symbolic phase only (no device): pattern inspection
+
+    std::vector<MfmaBucket> mfma;      // per bucket; M == 0 when not eligible
+    std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
+    int n_cu = 256;
 
     DBuf* newbuf() {
         bufs.emplace_back(new DBuf());
@@ -347,8 +352,150 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
     P.d_ke.alloc(P.ke_capacity_bytes);
 }
 
+// IEEE binary16 round-to-nearest-even conversion (host side of the split).
+uint16_t f32_to_f16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    int32_t exp = (int32_t)((x >> 23) & 0xff) - 127 + 15;
+    uint32_t mant = x & 0x7fffffu;
+    if (((x >> 23) & 0xff) == 0xff) return (uint16_t)(sign | 0x7c00u | (mant ? 0x200u : 0));
+    if (exp >= 31) return (uint16_t)(sign | 0x7c00u);
+    if (exp <= 0) {
+        if (exp < -10) return (uint16_t)sign;
+        mant |= 0x800000u;
+        int shift = 14 - exp;
+        uint32_t h = mant >> shift, rem = mant & ((1u << shift) - 1), half = 1u << (shift - 1);
+        if (rem > half || (rem == half && (h & 1u))) h++;
+        return (uint16_t)(sign | h);
+    }
+    uint32_t h = ((uint32_t)exp << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+float f16_to_f32(uint16_t h) {
+    uint32_t sign = (uint32_t)(h & 0x8000u) << 16, exp = (h >> 10) & 0x1f, mant = h & 0x3ffu, x;
+    if (exp == 0) {
+        if (mant == 0) x = sign;
+        else {
+            float v = std::ldexp((float)mant, -24);
+            std::memcpy(&x, &v, 4);
+            x |= sign;
+        }
+    } else if (exp == 31) x = sign | 0x7f800000u | (mant << 13);
+    else x = sign | ((exp - 15 + 127) << 23) | (mant << 13);
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// Builds the MFMA description of a bucket's matrix slices (all sharing one
+// Phi table) or leaves M = 0 when the bucket is not eligible.
+void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const std::vector<int64_t>& kebase) {
+    MfmaBucket mb;
+    std::memset(&mb, 0, sizeof(mb));
+    uint32_t mask = 0;
+    int tf_table = -1, dof_table = -1, n_tf = 0, n_dof = 0;
+    std::vector<std::pair<int, int>> pairs;
+    std::vector<int> ms;
+    for (int s = 0; s < bd.n_slices; s++) {
+        const slx_slice_desc& sd = bd.slices[s];
+        if (sd.is_rhs) continue;
+        if (tf_table < 0) {
+            tf_table = sd.tf_table;
+            dof_table = sd.dof_table;
+            n_tf = sd.n_tf;
+            n_dof = sd.n_dof;
+        }
+        if (sd.tf_table != tf_table || sd.dof_table != dof_table) return;  // different Phi per slice
+        ms.push_back(s);
+        for (int t = 0; t < sd.n_terms; t++) {
+            std::pair<int, int> pr(sd.terms[t].tf_deriv, sd.terms[t].dof_deriv);
+            if (std::find(pairs.begin(), pairs.end(), pr) == pairs.end()) pairs.push_back(pr);
+        }
+    }
+    if (ms.empty() || (int)ms.size() > SLX_MAX_SLICES) return;
+    mb.M = n_tf * n_dof;
+    mb.n_pairs = (int)pairs.size();
+    mb.K = mb.n_pairs * bd.n_gp;
+    mb.mtiles = (mb.M + 15) / 16;
+    mb.ksteps = (mb.K + 31) / 32;
+    mb.nq = bd.q_invariant ? 1 : bd.n_gp;
+    mb.n_mslices = (int)ms.size();
+    if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.mtw, &mb.maxk)) return;
+    // element tile: the largest of 64 / 32 / 16 with <= 48 KB of LDS
+    mb.ne_tile = 0;
+    for (int ne : {64, 32, 16}) {
+        mb.ne_tile = ne;
+        if (mfma_lds_bytes(mb, bd.n_coefs) <= 48 * 1024) break;
+        mb.ne_tile = 0;
+    }
+    if (mb.ne_tile == 0) return;
+    // coefficient routing: slot -> (s * n_pairs + p) * nq
+    std::vector<int32_t> route(std::max(bd.n_coefs, 1), -1);
+    for (size_t si = 0; si < ms.size(); si++) {
+        const slx_slice_desc& sd = bd.slices[ms[si]];
+        mb.slice_idx[si] = ms[si];
+        mb.ke_base[si] = kebase[ms[si]];
+        mask |= 1u << ms[si];
+        for (int t = 0; t < sd.n_terms; t++) {
+            std::pair<int, int> pr(sd.terms[t].tf_deriv, sd.terms[t].dof_deriv);
+            int p = (int)(std::find(pairs.begin(), pairs.end(), pr) - pairs.begin());
+            SLX_CHECK(route[sd.terms[t].coef] < 0, "coefficient slot shared by two matrix terms");
+            route[sd.terms[t].coef] = ((int32_t)si * mb.n_pairs + p) * mb.nq;
+        }
+    }
+    // Phi[m = i*n_dof + j][k = p*n_gp + q] = tf_r[i][q] w_q dof_s[j][q]
+    const double* tft = bd.tables[tf_table];
+    const double* dft = bd.tables[dof_table];
+    int ng = bd.n_gp;
+    std::vector<double> phi((size_t)mb.M * mb.K);
+    double mx = 0.0;
+    for (int i = 0; i < n_tf; i++)
+        for (int j = 0; j < n_dof; j++)
+            for (int p = 0; p < mb.n_pairs; p++)
+                for (int q = 0; q < ng; q++) {
+                    double v = tft[((size_t)pairs[p].first * n_tf + i) * ng + q] * bd.gp_weights[q] *
+                               dft[((size_t)pairs[p].second * n_dof + j) * ng + q];
+                    phi[((size_t)i * n_dof + j) * mb.K + p * ng + q] = v;
+                    mx = std::max(mx, std::fabs(v));
+                }
+    int ex = 0;
+    if (mx > 0) std::frexp(mx, &ex);
+    double sc = std::ldexp(1.0, -ex);
+    mb.a_unscale = (float)std::ldexp(1.0, ex);
+    // fragments [mt][ks][part][lane][8]
+    std::vector<uint16_t> frag((size_t)mb.mtiles * mb.ksteps * 2 * 64 * 8, 0);
+    for (int mt = 0; mt < mb.mtiles; mt++)
+        for (int ks = 0; ks < mb.ksteps; ks++)
+            for (int lane = 0; lane < 64; lane++)
+                for (int v = 0; v < 8; v++) {
+                    int m = mt * 16 + lane % 16, k = ks * 32 + 8 * (lane / 16) + v;
+                    float x = (m < mb.M && k < mb.K) ? (float)(phi[(size_t)m * mb.K + k] * sc) : 0.f;
+                    uint16_t hi = f32_to_f16(x);
+                    uint16_t lo = f32_to_f16((x - f16_to_f32(hi)) * 2048.f);
+                    size_t base = (((size_t)(mt * mb.ksteps + ks) * 2) * 64 + lane) * 8 + v;
+                    frag[base] = hi;
+                    frag[base + 64 * 8] = lo;
+                }
+    mb.afrag = P.dev(frag.data(), frag.size());
+    mb.route = P.dev(route.data(), route.size());
+    mb.max_blocks = P.n_cu * 2;
+    P.mfma[b] = mb;
+    P.mfma_mask[b] = mask;
+}
+
 void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
     P.dbuckets.resize(d.n_buckets);
+    P.mfma.assign(d.n_buckets, MfmaBucket());
+    for (auto& m : P.mfma) m.M = 0;
+    P.mfma_mask.assign(d.n_buckets, 0);
+    {
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, P.ctx->device) == hipSuccess && prop.multiProcessorCount > 0) P.n_cu = prop.multiProcessorCount;
+    }
     std::vector<std::vector<int64_t>> kebase(d.n_buckets);
     int64_t ke = 0;
     for (int b = 0; b < d.n_buckets; b++) {
@@ -401,6 +548,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
             ds.tf_tab = dtabs[sd.tf_table];
             ds.dof_tab = sd.is_rhs ? nullptr : dtabs[sd.dof_table];
             ds.ke_base = ke;
+            kebase[b].push_back(ke);
             ke += (int64_t)bd.n_elem * ds.n_tf * ds.n_dof;
             for (int t = 0; t < sd.n_terms; t++) {
                 SLX_CHECK(sd.terms[t].coef >= 0 && sd.terms[t].coef < bd.n_coefs, "term coefficient slot out of range");
@@ -434,6 +582,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
         // element node numbers must be in range of the node array
         for (size_t k = 0; k < (size_t)bd.n_elem * bd.n_geo; k++)
             SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
+        build_mfma(P, bd, b, kebase[b]);
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
 }
@@ -453,15 +602,16 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
     in.consts = P.consts.as<double>();
     for (int f = 0; f < P.n_fields; f++) in.field_data[f] = fields ? fields[f] : nullptr;
     bool ke64 = precision == SLX_PREC_FP64;
-    for (auto& b : P.dbuckets) {
-        bool handled = false;
-        if (precision == SLX_PREC_F16_SPLIT || precision == SLX_PREC_BF16_SPLIT) {
-            HIPCHECK(launch_elem_mfma(b, in, precision, P.d_ke.p, st, &handled));
+    for (size_t bi = 0; bi < P.dbuckets.size(); bi++) {
+        const DevBucket& b = P.dbuckets[bi];
+        uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
+        uint32_t done = 0;
+        if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
+            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], in, P.d_ke.as<float>(), st));
+            done = P.mfma_mask[bi];
         }
-        if (!handled) {
-            int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
-            HIPCHECK(launch_elem(b, in, p, P.d_ke.p, st));
-        }
+        int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
+        HIPCHECK(launch_elem(b, in, p, P.d_ke.p, st, all & ~done));
     }
     if (mid) HIPCHECK(hipEventRecord(mid, st));
     HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
@@ -652,11 +802,33 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
     });
 }
 
+slx_status slx_plan_kernels(slx_plan p, int32_t* n_mfma, int32_t* n_buckets) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        int n = 0;
+        for (auto& m : p->mfma) n += m.M > 0 ? 1 : 0;
+        if (n_mfma) *n_mfma = n;
+        if (n_buckets) *n_buckets = (int32_t)p->dbuckets.size();
+    });
+}
+
 slx_status slx_plan_destroy(slx_plan p) {
     return guarded([&] {
         if (!p) return;
         if (!p->host_only) (void)hipStreamSynchronize(p->stream);
         delete p;
+    });
+}
+
+slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256) {
+    return guarded([&] {
+        SLX_CHECK(ctx && out256, "null argument");
+        HIPCHECK(hipSetDevice(ctx->device));
+        DBuf d;
+        d.alloc(256 * sizeof(float));
+        HIPCHECK(launch_mfma_probe(d.as<float>(), ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        HIPCHECK(hipMemcpy(out256, d.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
     });
 }
 

@@ -64,9 +64,28 @@ def test_quadbatch_small_fp64(ctx):
     run_case(ctx, "quadbatch", "instances=3;nmin=2;nmax=5", "fp64", TOL_FP64)
 
 
-def test_deterministic_repeat(ctx):
+def test_mfma_operand_layout(ctx):
+    # A[m][k] = m + k/64, B selects rows n and n+16 of column n: D[m][n] = A[m][n] + A[m][n+16]
+    from sparselizard_amd import _lib as L
+    d = L.mfma_layout_probe(ctx)
+    m = np.arange(16)[:, None]
+    n = np.arange(16)[None, :]
+    expect = (m + n / 64.0) + (m + (n + 16) / 64.0)
+    assert np.array_equal(d, expect.astype(np.float32))
+
+
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=9"), ("poisson2d", "n=7;order=2"), ("elasticity3d", "n=3"),
+                                       ("elasticity3d", "n=4;order=1"), ("quadbatch", "instances=3;nmin=2;nmax=5")])
+def test_f16split_mfma_matches_oracle(ctx, kind, spec):
+    prob, plan, got, errs = run_case(ctx, kind, spec, "f16split", TOL_SPLIT)
+    assert plan.kernels()["mfma_buckets"] >= 1, "MFMA kernel not used"
+    print("errors", errs)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "f16split"])
+def test_deterministic_repeat(ctx, precision):
     prob = sa.Problem(ctx, "elasticity3d", "n=3")
     info = prob.info()
-    a = prob.generate("fp64").csr(info["ndofs"])
-    b = prob.generate("fp64").csr(info["ndofs"])
+    a = prob.generate(precision).csr(info["ndofs"])
+    b = prob.generate(precision).csr(info["ndofs"])
     assert np.array_equal(a.Aval, b.Aval) and np.array_equal(a.Dval, b.Dval) and np.array_equal(a.rhs, b.rhs)
