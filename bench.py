@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--mixed", default="f16split", help="mixed precision mode")
     ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus)
@@ -57,6 +59,9 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.init()
+    if args.copy_probe_mib > 0:
+        copy_probe(torch, dev, args.copy_probe_mib)
+        return
     ctx = sa.Context(0)
     spec = "n=%d;order=%d" % (args.n, args.order)
     t0 = time.time()
@@ -122,38 +127,40 @@ def main():
     glob = float(np.sqrt(np.sum((mix.Aval - ref.Aval) ** 2) + np.sum((mix.Dval - ref.Dval) ** 2)) /
                  np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2)))
 
-    # per-phase timing (HIP events on the plan's stream) for the roofline
-    t_elem, t_red = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
-                                 args.mixed, iters=max(3, args.steps // 2))
-    t64_elem, t64_red = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
-                                     "fp64", iters=max(3, args.steps // 2))
+    # per-kernel timing: HIP events around every launch on the plan's stream
+    prof_mix = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
+                            args.mixed, iters=max(3, args.steps // 2))
+    prof_64 = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
+                           "fp64", iters=max(3, args.steps // 2))
     n_elem = count_tets(args.n)
     n = (args.order + 1) * (args.order + 2) * (args.order + 3) // 6
     ngp = {1: 11, 2: 24, 3: 43}.get(args.order, 24)
     flops = element_flops(3, n, 3, ngp) * n_elem
     ke_bytes = 4 if args.mixed in ("f16split", "bf16split", "fp32") else 8
-    red_bytes = stats["n_coo"] * (ke_bytes + 4) + stats["n_elem_rows"] * 8 + nnz * 8 + nn * 8
+    # reduction: every element-matrix entry read once, every CSR value and rhs entry written once
+    red_bytes = stats["n_coo"] * ke_bytes + nnz * 8 + nn * 8
     mixed_kind = {"f16split": "fp16", "bf16split": "bf16", "fp32": "fp32"}.get(args.mixed, "fp32")
-    mfma_used = os.environ.get("SLX_MIXED_IS_VALU", "") == ""
     peak_elem = PEAKS[mixed_kind + "_TFLOPs"]
-    if t_elem >= t_red:
-        achieved = flops / (t_elem * 1e-3) / 1e12
+    t_contract = prof_mix["mfma"] if prof_mix["mfma"] > 0 else prof_mix["valu"]
+    if t_contract >= prof_mix["reduce"]:
+        achieved = flops / (t_contract * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_elem, "unit": "TFLOP/s",
-                "frac": round(achieved / peak_elem, 5), "traffic": None, "kernel": "element contraction",
-                "kernel_ms": round(t_elem, 4)}
+                "frac": round(achieved / peak_elem, 5), "traffic": None,
+                "kernel": "mfma_f16split_kernel" if prof_mix["mfma"] > 0 else "elem_tiled",
+                "kernel_ms": round(t_contract, 4)}
     else:
-        achieved = red_bytes / (t_red * 1e-3) / 1e9
+        achieved = red_bytes / (prof_mix["reduce"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
-                "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "csr reduction",
-                "kernel_ms": round(t_red, 4)}
+                "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "reduce_matrix",
+                "kernel_ms": round(prof_mix["reduce"], 4)}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         try:
             tr = json.load(open(traffic_file))
-            key = "%s_%s_n%d" % (roof["kernel"].replace(" ", "_"), args.mixed, args.n)
+            key = "%s|%s|n%d" % (roof["kernel"], args.mixed, args.n)
             if key in tr:
                 roof["traffic"] = tr[key]
-        except Exception:
+        except (OSError, ValueError):
             pass
 
     cpu = None
@@ -178,8 +185,8 @@ def main():
                                "element, nu=0.3, clamped x=0" % (args.order, args.n, n_elem),
                    "n_elem": n_elem, "ndofs": nn, "nnzA": shape["nnzA"], "nnzD": shape["nnzD"],
                    "coo_entries": stats["n_coo"], "parallelism": "single GPU"},
-        "fp64_gpu": {"ms_per_step": ms64, "nnz_per_s": nnz / (ms64 * 1e-3), "elem_ms": t64_elem, "reduce_ms": t64_red},
-        "mixed_gpu": {"mode": args.mixed, "ms_per_step": msmix, "elem_ms": t_elem, "reduce_ms": t_red},
+        "fp64_gpu": {"ms_per_step": ms64, "nnz_per_s": nnz / (ms64 * 1e-3), "kernel_ms": prof_64},
+        "mixed_gpu": {"mode": args.mixed, "ms_per_step": msmix, "kernel_ms": prof_mix},
         "ratio_mixed_vs_fp64": ms64 / msmix,
         "rel_err_vs_fp64_gpu": {"global_frobenius": glob, "A": err_A, "D": err_D, "rhs": err_b},
         "symbolic_s": t_sym,
@@ -187,6 +194,18 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def copy_probe(torch, dev, mib):
+    """A plain device-to-device copy of a known size: the FETCH_SIZE and
+    WRITE_SIZE counters of its kernel give the scale of those counters."""
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device=dev)
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
 def count_tets(n):

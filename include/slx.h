@@ -132,6 +132,9 @@ typedef struct slx_plan_desc {
     const int32_t* const* field_addr;   /* [n_fields*n_buckets] -> [nff][n_elem] addresses into field data */
     const double* const* field_tables;  /* [n_fields*n_buckets] -> [4][nff][n_gp] */
     int32_t has_rhs, has_matrix;
+    const double* dirichlet;    /* [n_dofs] or NULL: value of every constrained DOF, written into
+                                   the constrained rhs entries like formulation::b()
+                                   (vec::updateconstraints, src/formulation/vec.cpp:43-61) */
 } slx_plan_desc;
 
 /* Numeric inputs of one assembly (host pointers; copied to the device). */
@@ -171,9 +174,11 @@ slx_status slx_assemble_device(slx_plan plan, const double* d_node_xyz, const do
                                int64_t param_len, const double* const* d_field_data, int32_t precision,
                                void* hip_stream);
 slx_status slx_plan_sync(slx_plan plan);
-/* Phase timing on the plan's stream with HIP events: runs `iters` numeric
- * phases on device-resident inputs and returns the average milliseconds of
- * the element kernels (ms[0]) and of the reduction (ms[1]). */
+/* Kernel timing on the plan's stream with HIP events around every launch:
+ * runs `iters` numeric phases on device-resident inputs and returns the
+ * average milliseconds per phase of the Gauss-point coefficient kernels
+ * (ms[0]), the MFMA element contractions (ms[1]), the VALU element kernels
+ * (ms[2]) and the CSR/rhs reduction (ms[3]). */
 slx_status slx_plan_profile(slx_plan plan, const double* d_node_xyz, const double* d_params, int64_t param_len,
                             const double* const* d_field_data, int32_t precision, int32_t iters, double* ms);
 /* Device views of the assembled CSR (valid until slx_plan_destroy). */
@@ -182,13 +187,33 @@ slx_status slx_csr_device(slx_plan plan, const int64_t** Arow, const int32_t** A
 /* Copies to caller-owned host arrays (any pointer may be NULL). */
 slx_status slx_csr_copyout(slx_plan plan, int64_t* Arow, int32_t* Acol, double* Aval, int64_t* Drow,
                            int32_t* Dcol, double* Dval, double* rhs);
-/* Element matrices of the last assembly, in the reduction buffer layout
- * (for error measurement); n_out receives the entry count. */
+/* Segment counts of the reduction: element-row segments, element-matrix
+ * entries (including right-hand-side values) and COO entries. */
 slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, int64_t* n_coo);
 /* Buckets whose matrix slices run on the MFMA split kernel (the others and
  * every right-hand-side slice use the VALU element kernel). */
 slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_buckets);
 slx_status slx_plan_destroy(slx_plan plan);
+/* Element matrices of the last assembly (n_ke entries, as FP64) in the
+ * layout given by slx_plan_reduction_maps: for the element-matrix error of
+ * the mixed-precision paths. */
+slx_status slx_plan_element_matrices(slx_plan plan, double* out, int64_t cap);
+/* Gauss-point coefficient table of the last bucket of the last assembly
+ * (as FP64): out[(slot * nq + q) * n_elem + e] = coefficient * |det J|, with
+ * nq = 1 when the bucket's coefficients do not vary over the Gauss points.
+ * *n_out receives the entry count; out may be NULL to query it. */
+slx_status slx_plan_coefficients(slx_plan plan, double* out, int64_t cap, int64_t* n_out);
+/* Diagnostic: the reduction maps (pattern-only plans, ctx == NULL) and the
+ * element-matrix layout (every plan), so the reduction can be replayed on
+ * the host.
+ * sizes[0..3] = element-row segments, slot entries, rhs entries, slices.
+ * erow_ptr/slot_ptr/rhs_ptr have nfree + 1 entries; erows and rhs_erows are
+ * packed (length << 48 | offset); slots are column positions in the row
+ * (A columns first, then D), -1 = skipped; ke_base/ke_ld list every
+ * (bucket, slice): entry (e, i, j) at ke_base + (e*n_tf + i)*ke_ld + j.
+ * Any output pointer may be NULL. */
+slx_status slx_plan_reduction_maps(slx_plan plan, int64_t* sizes, int64_t* erow_ptr, uint64_t* erows, int64_t* slot_ptr,
+                                   int32_t* slots, int64_t* rhs_ptr, uint64_t* rhs_erows, int64_t* ke_base, int64_t* ke_ld);
 
 /* Diagnostic: runs one v_mfma_f32_16x16x32_f16 on fixed operands and returns
  * the 16x16 result (row-major) so the operand lane layout the kernels assume

@@ -11,6 +11,7 @@
 //     - coef*|detJ|, transpose, GEMM       contribution.cpp:195-211, densemat.cpp:228-311
 //     - first term copied, next ones added contribution.cpp:241-245 (getproduct/addproduct)
 //     - rhs: minus, VecSetValues ADD       contribution.cpp:293-297, rawvec.cpp:216-254
+//     - Dirichlet values in b()            formulation.cpp:361-385, vec.cpp:43-61
 //   rawmat::process                        src/formulation/rawmat.cpp:127-298
 //     - per-row bucketing in fragment -> row r -> element order
 //     - per-row stable sort by column (ties by position), sequential FP64 sum
@@ -179,8 +180,31 @@ void coefficients(const slx_plan_desc& d, const slx_bucket_desc& b, const slx_nu
 
 extern "C" {
 
+// Coefficient table of one bucket, as the device lays it out:
+// out[(slot * nq + q) * n_elem + e] = coefficient * |det J|, nq = 1 for
+// buckets whose coefficients do not vary over the Gauss points.
+int oracle_coefficients(const slx_plan_desc* dp, const slx_numeric_desc* inp, int bucket, double* out) {
+    const slx_plan_desc& d = *dp;
+    if (bucket < 0 || bucket >= d.n_buckets) return -1;
+    const slx_bucket_desc& b = d.buckets[bucket];
+    const int nq = b.q_invariant ? 1 : b.n_gp;
+    std::vector<double> geo;
+    lagrange1(b.elem_type, b.gp_coords, b.n_gp, geo);
+    std::vector<double> cbuf(std::max(b.n_coefs, 1));
+    for (int e = 0; e < b.n_elem; e++)
+        for (int q = 0; q < nq; q++) {
+            double J[3][3], inv[3][3], det;
+            jacobian(d, b, inp->node_xyz, geo, e, q, J, inv, det);
+            coefficients(d, b, *inp, geo, e, q, J, inv, det, cbuf);
+            for (int c = 0; c < b.n_coefs; c++) out[((size_t)c * nq + q) * b.n_elem + e] = cbuf[c] * std::fabs(det);
+        }
+    return 0;
+}
+
+
 // Full restatement: returns 0 on success.  Output arrays are sized by the
-// caller from oracle_shape(); rhs has n_dofs entries (constrained entries 0).
+// caller from oracle_shape(); rhs has n_dofs entries (constrained entries hold
+// the Dirichlet values, vec::updateconstraints, src/formulation/vec.cpp:43-61).
 // Element matrices of every fragment are also returned when ke_out is not
 // NULL, in the Ke layout of the plan (bucket, slice, element, i, j).
 int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nthreads, int64_t* nnzA_out, int64_t* nnzD_out,
@@ -276,7 +300,7 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
         }
         // vec::updateconstraints: constrained entries take the (zero) Dirichlet value
         for (int64_t i = 0; i < ndofs; i++)
-            if (d.constrained && d.constrained[i]) rhs[i] = 0.0;
+            if (d.constrained && d.constrained[i]) rhs[i] = d.dirichlet ? d.dirichlet[i] : 0.0;
     }
     // rawmat::process
     std::vector<char> isc(ndofs, 0);

@@ -30,8 +30,19 @@ def lib():
         L.oracle_assemble.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int64)] + \
             [C.POINTER(vp)] * 6 + [C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.oracle_free.argtypes = [vp]
+        L.oracle_coefficients.restype = C.c_int
+        L.oracle_coefficients.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def coefficients(desc_ptr, numeric, bucket, n):
+    """Coefficient table of one bucket: [(slot * nq + q) * n_elem + e]."""
+    out = np.zeros(max(n, 1), np.float64)
+    if lib().oracle_coefficients(desc_ptr, C.cast(C.byref(numeric), C.c_void_p), bucket,
+                                 out.ctypes.data_as(C.POINTER(C.c_double))) != 0:
+        raise RuntimeError("oracle_coefficients failed")
+    return out[:n]
 
 
 class OracleCSR:

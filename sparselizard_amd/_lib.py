@@ -60,7 +60,8 @@ class PlanDesc(C.Structure):
                 ("constrained", C.POINTER(C.c_uint8)), ("n_consts", C.c_int32), ("consts", C.POINTER(C.c_double)),
                 ("n_buckets", C.c_int32), ("buckets", C.POINTER(BucketDesc)), ("n_fields", C.c_int32),
                 ("field_nff", C.POINTER(C.c_int32)), ("field_addr", C.POINTER(C.POINTER(C.c_int32))),
-                ("field_tables", C.POINTER(C.POINTER(C.c_double))), ("has_rhs", C.c_int32), ("has_matrix", C.c_int32)]
+                ("field_tables", C.POINTER(C.POINTER(C.c_double))), ("has_rhs", C.c_int32), ("has_matrix", C.c_int32),
+                ("dirichlet", C.POINTER(C.c_double))]
 
 
 PlanDescHead = PlanDesc
@@ -100,6 +101,9 @@ def lib():
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
+        "slx_plan_element_matrices": (i32, [vp, pd, i64]),
+        "slx_plan_coefficients": (i32, [vp, pd, i64, p64]),
+        "slx_plan_reduction_maps": (i32, [vp, p64, p64, C.POINTER(C.c_uint64), p64, p32, p64, C.POINTER(C.c_uint64), p64, p64]),
         "slx_problem_create": (i32, [vp, C.c_char_p, C.c_char_p, C.POINTER(vp)]),
         "slx_problem_destroy": (i32, [vp]),
         "slx_problem_plan": (i32, [vp, C.POINTER(vp)]),
@@ -124,7 +128,7 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout",
 ]
@@ -281,6 +285,45 @@ class Plan:
         check(lib().slx_plan_kernels(self.h, C.byref(a), C.byref(b)))
         return dict(mfma_buckets=a.value, buckets=b.value)
 
+    def element_matrices(self):
+        n_ke = self.stats()["n_ke"]
+        out = np.zeros(max(n_ke, 1), np.float64)
+        check(lib().slx_plan_element_matrices(self.h, _p(out, C.c_double), out.size))
+        return out[:n_ke]
+
+    def coefficients(self):
+        """Coefficient table of the last bucket of the last assembly (FP64 copy)."""
+        n = C.c_int64()
+        check(lib().slx_plan_coefficients(self.h, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.float64)
+        check(lib().slx_plan_coefficients(self.h, _p(out, C.c_double), out.size, C.byref(n)))
+        return out[:n.value]
+
+    def ke_layout(self):
+        """(ke_base, ke_ld) of every (bucket, slice)."""
+        sz = np.zeros(4, np.int64)
+        check(lib().slx_plan_reduction_maps(self.h, _p(sz, C.c_int64), None, None, None, None, None, None, None, None))
+        kb, kl = np.zeros(sz[3], np.int64), np.zeros(sz[3], np.int64)
+        check(lib().slx_plan_reduction_maps(self.h, _p(sz, C.c_int64), None, None, None, None, None, None,
+                                            _p(kb, C.c_int64), _p(kl, C.c_int64)))
+        return kb, kl
+
+    def reduction_maps(self):
+        """Reduction maps and element-matrix layout of a pattern-only plan."""
+        sz = np.zeros(4, np.int64)
+        check(lib().slx_plan_reduction_maps(self.h, _p(sz, C.c_int64), None, None, None, None, None, None, None, None))
+        nf = self.shape()["nfree"]
+        out = dict(erow_ptr=np.zeros(nf + 1, np.int64), erows=np.zeros(sz[0], np.uint64),
+                   slot_ptr=np.zeros(nf + 1, np.int64), slots=np.zeros(sz[1], np.int32),
+                   rhs_ptr=np.zeros(nf + 1, np.int64), rhs_erows=np.zeros(sz[2], np.uint64),
+                   ke_base=np.zeros(sz[3], np.int64), ke_ld=np.zeros(sz[3], np.int64))
+        check(lib().slx_plan_reduction_maps(self.h, _p(sz, C.c_int64), _p(out["erow_ptr"], C.c_int64),
+                                            _p(out["erows"], C.c_uint64), _p(out["slot_ptr"], C.c_int64),
+                                            _p(out["slots"], C.c_int32), _p(out["rhs_ptr"], C.c_int64),
+                                            _p(out["rhs_erows"], C.c_uint64), _p(out["ke_base"], C.c_int64),
+                                            _p(out["ke_ld"], C.c_int64)))
+        return out
+
     def assemble(self, numeric, precision):
         numeric.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
         check(lib().slx_assemble(self.h, C.byref(numeric)))
@@ -290,11 +333,12 @@ class Plan:
                                         PRECISIONS[precision] if isinstance(precision, str) else precision, stream))
 
     def profile(self, d_xyz, d_params, param_len, d_fields, precision, iters=5):
-        ms = np.zeros(2, np.float64)
+        """Average ms per numeric phase: coef, mfma, valu, reduce (HIP events per launch)."""
+        ms = np.zeros(4, np.float64)
         check(lib().slx_plan_profile(self.h, d_xyz, d_params, param_len, d_fields,
                                      PRECISIONS[precision] if isinstance(precision, str) else precision, iters,
                                      _p(ms, C.c_double)))
-        return float(ms[0]), float(ms[1])
+        return {"coef": float(ms[0]), "mfma": float(ms[1]), "valu": float(ms[2]), "reduce": float(ms[3])}
 
     def sync(self):
         check(lib().slx_plan_sync(self.h))

@@ -16,6 +16,7 @@ namespace slx {
 RawField::RawField(const Mesh* m, const std::string& type, const std::string& name) : mesh_(m), type_(type), name_(name) {
     orders_.assign(m->count_disjregs(), 1);
     constrained_.assign(m->count_disjregs(), 0);
+    cvalue_.assign(m->count_disjregs(), 0.0);
 }
 
 void RawField::setorder(int physreg, int order) {
@@ -28,10 +29,13 @@ void RawField::setorder(int physreg, int order) {
     for (int dr : mesh_->physreg_disjregs(physreg, -1)) orders_[dr] = order;
 }
 
-void RawField::setconstraint(int physreg) {
-    for (auto& s : subfields_) s->setconstraint(physreg);
+void RawField::setconstraint(int physreg, double value) {
+    for (auto& s : subfields_) s->setconstraint(physreg, value);
     if (!subfields_.empty()) return;
-    for (int dr : mesh_->physreg_disjregs(physreg, -1)) constrained_[dr] = 1;
+    for (int dr : mesh_->physreg_disjregs(physreg, -1)) {
+        constrained_[dr] = 1;
+        cvalue_[dr] = value;
+    }
 }
 
 void RawField::build_data_numbering() {
@@ -72,7 +76,7 @@ field::field(const Mesh& m, const std::string& type, const std::string& name) {
 }
 
 void field::setorder(int physreg, int order) { raw_->setorder(physreg, order); }
-void field::setconstraint(int physreg) { raw_->setconstraint(physreg); }
+void field::setconstraint(int physreg, double value) { raw_->setconstraint(physreg, value); }
 
 static OpP field_op(const std::shared_ptr<RawField>& f) {
     auto o = std::make_shared<Op>();
@@ -223,6 +227,24 @@ std::vector<uint8_t> DofManager::isconstrained() const {
             int n = mesh_->disjreg_count(dr);
             for (int rb : rangebegin_[fi][dr])
                 for (int i = 0; i < n; i++) out[rb + i] = 1;
+        }
+    return out;
+}
+
+std::vector<double> DofManager::constraint_values() const {
+    // The reference projects the constraint expression onto the constrained
+    // region (rawfield::setdisjregconstraint, rawfield.cpp:1166-1248).  For a
+    // constant c the projection is exact in the hierarchical H1 basis: every
+    // vertex function (the nodal disjoint regions) gets c, every edge, face
+    // and volume function 0.
+    std::vector<double> out(ndofs_, 0.0);
+    for (size_t fi = 0; fi < fields_.size(); fi++)
+        for (int dr = 0; dr < (int)rangebegin_[fi].size(); dr++) {
+            if (rangebegin_[fi][dr].empty() || !fields_[fi]->constrained(dr)) continue;
+            if (mesh_->disjreg_type(dr) != 0) continue;
+            int n = mesh_->disjreg_count(dr);
+            int rb = rangebegin_[fi][dr][0];
+            for (int i = 0; i < n; i++) out[rb + i] = fields_[fi]->constraint_value(dr);
         }
     return out;
 }
@@ -717,6 +739,7 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         ps->field_len.push_back(fl->data_size());
     }
     ps->constrained = dofm_->isconstrained();
+    ps->dirichlet = dofm_->constraint_values();
     slx_plan_desc& d = ps->desc;
     d.mesh_dim = mesh_->dimension();
     d.n_nodes = mesh_->count(0);
@@ -732,6 +755,7 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
     d.field_tables = ps->field_tables.data();
     d.has_rhs = !contribs_[0].empty();
     d.has_matrix = !contribs_[1].empty();
+    d.dirichlet = ps->dirichlet.data();
     for (auto& b : ps->buckets) ps->n_elem_total += b.n_elem;
     return ps;
 }

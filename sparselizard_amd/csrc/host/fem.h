@@ -31,11 +31,12 @@ class RawField {
     int order(int disjreg) const { return orders_[disjreg]; }
     const std::vector<int>& orders() const { return orders_; }
     bool constrained(int disjreg) const { return constrained_[disjreg] != 0; }
+    double constraint_value(int disjreg) const { return cvalue_[disjreg]; }
     // rawfield::setorder (rawfield.cpp:652-722): all disjoint regions of the
     // physical region (every dimension) get the order; later calls override.
     void setorder(int physreg, int order);
-    // rawfield::setdisjregconstraint with a zero value (rawfield.cpp:1166-1248).
-    void setconstraint(int physreg);
+    // rawfield::setdisjregconstraint with a constant value (rawfield.cpp:1166-1248).
+    void setconstraint(int physreg, double value = 0.0);
     std::shared_ptr<RawField> self() { return self_.lock(); }
     void setself(const std::shared_ptr<RawField>& s) { self_ = s; }
     void addsub(const std::shared_ptr<RawField>& s) { subfields_.push_back(s); }
@@ -52,6 +53,7 @@ class RawField {
     std::vector<std::shared_ptr<RawField>> subfields_;
     std::vector<int> orders_;
     std::vector<char> constrained_;
+    std::vector<double> cvalue_;
     std::weak_ptr<RawField> self_;
     int64_t data_size_ = 0;
 };
@@ -62,7 +64,7 @@ class field {
     field() = default;
     field(const Mesh& m, const std::string& type, const std::string& name = "u");
     void setorder(int physreg, int order);
-    void setconstraint(int physreg);
+    void setconstraint(int physreg, double value = 0.0);
     std::shared_ptr<RawField> raw() const { return raw_; }
     operator expression() const;
     expression comp(int i) const;
@@ -129,6 +131,8 @@ class DofManager {
     void addtostructure(const std::shared_ptr<RawField>& f, const std::vector<int>& disjregs);
     int64_t countdofs() const { return ndofs_; }
     std::vector<uint8_t> isconstrained() const;
+    // Dirichlet values of the constrained DOFs (0 elsewhere).
+    std::vector<double> constraint_values() const;
     // dofmanager::getaddresses: [n_ff][elems.size()] row-major.
     std::vector<int32_t> getaddresses(const std::shared_ptr<RawField>& f, int order, int elemtype,
                                       const std::vector<int>& elems, int fieldphysreg) const;
@@ -155,6 +159,7 @@ struct PlanStorage {
     std::vector<std::vector<slx_instr>> progs;
     std::vector<double> consts;
     std::vector<uint8_t> constrained;
+    std::vector<double> dirichlet;
     std::vector<int32_t> field_nff;
     std::vector<const int32_t*> field_addr;
     std::vector<const double*> field_tables;

@@ -410,6 +410,50 @@ std::shared_ptr<Problem> from_msh(const std::map<std::string, std::string>& s) {
     return p;
 }
 
+// examples/electrostatics-1d/main.cpp:11-36 on its mesh file: P1 potential
+// on line region 1, 10 V on point region 2 and 2 V on point region 3,
+// integral(line, -epsilon grad(dof(v)) grad(tf(v))).
+std::shared_ptr<Problem> electrostatics1d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    auto it = s.find("file");
+    SLX_CHECK(it != s.end(), "electrostatics1d needs file=<path>");
+    p->mesh = std::make_shared<Mesh>(Mesh::read_msh(it->second));
+    Mesh& m = *p->mesh;
+    const int line = 1, left = 2, right = 3;
+    field v(m, "h1", "v");
+    v.setorder(line, geti(s, "order", 1));
+    v.setconstraint(left, 10.0);
+    v.setconstraint(right, 2.0);
+    p->fields = {v};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    const double epsilon = 8.854e-12;
+    *p->form += integral(line, -expression(epsilon) * grad(dof(v)) * grad(tf(v)));
+    p->describe = "electrostatics 1D, mesh file " + it->second;
+    return p;
+}
+
+// examples/high-order-interpolations-2d/main.cpp:17-48 on its mesh file:
+// L2 projection of sin(10x) sin(13y) onto h1 of the given order on face 1,
+// integration order +20.
+std::shared_ptr<Problem> highorderprojection2d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    auto it = s.find("file");
+    SLX_CHECK(it != s.end(), "hoprojection2d needs file=<path>");
+    p->mesh = std::make_shared<Mesh>(Mesh::read_msh(it->second));
+    Mesh& m = *p->mesh;
+    const int face = 1;
+    field v(m, "h1", "v");
+    v.setorder(face, geti(s, "order", 5));
+    p->fields = {v};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    expression toproject = sl::sin(expression(10.0) * x()) * sl::sin(expression(13.0) * y());
+    *p->form += integral(face, dof(v) * tf(v) - toproject * tf(v), geti(s, "delta", 20));
+    p->describe = "high-order projection 2D, mesh file " + it->second;
+    return p;
+}
+
 }  // namespace
 
 std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec) {
@@ -421,6 +465,8 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "projection") p = projection(s);
     else if (kind == "hppoisson3d") p = hppoisson3d(s);
     else if (kind == "msh") p = from_msh(s);
+    else if (kind == "electrostatics1d") p = electrostatics1d(s);
+    else if (kind == "hoprojection2d") p = highorderprojection2d(s);
     else fail("unknown problem kind '" + kind + "'");
     p->kind = kind;
     p->spec = s;

@@ -1,74 +1,101 @@
-// Element kernels (VALU path) and the deterministic CSR reduction.
+// Element kernels (VALU path), the Gauss-point coefficient kernel and the
+// deterministic CSR reduction.
 //
-// elem_kernel: one workgroup per tile of elements of a bucket.
-//   phase 1  per (element, Gauss point): FP64 Jacobian, determinant and
-//            inverse (jacobian.cpp:4-505), then the coefficient program
-//            (operation::interpolate of every term, contribution.cpp:155-163),
-//            times |det J| (contribution.cpp:147, 200) -> LDS table C.
-//   phase 2  per (element, slice, tf function i, block of dof functions j):
-//            Ke[i][j] = sum_q sum_t tf_t[i][q] w_q C_t(e,q) dof_t[j][q]
-//            (contribution.cpp:172-246; the per-term dgemm of densemat.cpp:228-311
-//            with the terms summed as in addproduct).  Right-hand-side slices
-//            write -sum (contribution.cpp:296).
-// reduce_matrix: one lane group per CSR row walks the row's element rows in
-//   the cached accumulation order and adds each entry into its column slot
-//   in LDS (FP64), then writes the A and D blocks (rawmat.cpp:67-125, 238-289).
+// coef_kernel: one thread per (element, Gauss point): FP64 Jacobian,
+//   determinant and inverse (jacobian.cpp:4-505), then the coefficient program
+//   (operation::interpolate of every term, contribution.cpp:155-163), times
+//   |det J| (contribution.cpp:147, 200), into the table
+//   ctab[(slot * nq + q) * n_elem + e]  (element fastest: coalesced for kernels
+//   that map lanes to elements).
+// elem_tiled: lanes = elements; per slice, tile of tf rows x dof columns:
+//   Ke[i][j] = sum_q sum_t tf_t[i][q] w_q C_t(e,q) dof_t[j][q]
+//   (contribution.cpp:172-246; the per-term dgemm of densemat.cpp:228-311 with
+//   the terms summed as in addproduct), right-hand-side slices write -sum
+//   (contribution.cpp:296).
+// reduce_matrix: one lane group per CSR row walks the row's element-row
+//   segments and adds each entry into its column slot in LDS (FP64), then
+//   writes the A and D blocks (rawmat.cpp:67-125, 238-289).
 #include <hip/hip_runtime.h>
 #include "device.h"
 
 namespace slx {
 
-template <typename T>
-__device__ inline T tabv(const double* tab, int deriv, int nff, int f, int ngp, int q) {
-    return (T)tab[((size_t)deriv * nff + f) * ngp + q];
-}
-
+// Jacobian, inverse and determinant of one (element, Gauss point).  Every
+// index below is a compile-time constant (loops fully unrolled, dimensions
+// guarded), so the values stay in registers: the program reads them through
+// a constant-index switch instead of indexing a private array.
 struct Geo {
-    double J[3][3], inv[3][3], det;
+    double J[9], inv[9], det;
 };
+
+__device__ inline double pick9(const double* v, int i) {
+    switch (i) {
+        case 0: return v[0];
+        case 1: return v[1];
+        case 2: return v[2];
+        case 3: return v[3];
+        case 4: return v[4];
+        case 5: return v[5];
+        case 6: return v[6];
+        case 7: return v[7];
+        case 8: return v[8];
+    }
+    return 0.0;
+}
 
 // J[r][c] = sum_k x_k^c dN_k/dki_r (rawfield.cpp:2442-2502); det and inverse
 // as jacobian.cpp:308-349, 396-505.
 __device__ inline void geometry(const DevBucket& b, const double* xyz, int e, int q, Geo& g) {
     double x[SLX_MAX_GEO][3];
-    for (int k = 0; k < b.n_geo; k++) {
-        int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-        x[k][0] = xyz[3 * (size_t)node];
-        x[k][1] = xyz[3 * (size_t)node + 1];
-        x[k][2] = xyz[3 * (size_t)node + 2];
-    }
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) g.J[r][c] = 0.0;
-    for (int r = 0; r < b.elemdim; r++)
-        for (int c = 0; c < b.meshdim; c++) {
-            double s = 0.0;
-            for (int k = 0; k < b.n_geo; k++) s += x[k][c] * b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q];
-            g.J[r][c] = s;
+#pragma unroll
+    for (int k = 0; k < SLX_MAX_GEO; k++) {
+        if (k < b.n_geo) {
+            const int node = b.geo_nodes[(size_t)e * b.n_geo + k];
+            x[k][0] = xyz[3 * (size_t)node];
+            x[k][1] = xyz[3 * (size_t)node + 1];
+            x[k][2] = xyz[3 * (size_t)node + 2];
+        } else {
+            x[k][0] = x[k][1] = x[k][2] = 0.0;
         }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+        double dn[SLX_MAX_GEO];
+#pragma unroll
+        for (int k = 0; k < SLX_MAX_GEO; k++)
+            dn[k] = (r < b.elemdim && k < b.n_geo) ? b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q] : 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < SLX_MAX_GEO; k++) s += x[k][c] * dn[k];
+            g.J[3 * r + c] = (c < b.meshdim) ? s : 0.0;
+        }
+    }
+    const double* J = g.J;
+#pragma unroll
+    for (int i = 0; i < 9; i++) g.inv[i] = 0.0;
     if (b.meshdim == 1) {
-        g.det = g.J[0][0];
-        g.inv[0][0] = 1.0 / g.J[0][0];
+        g.det = J[0];
+        g.inv[0] = 1.0 / J[0];
     } else if (b.meshdim == 2) {
-        const double(*J)[3] = g.J;
-        g.det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        g.inv[0][0] = J[1][1] / g.det;
-        g.inv[0][1] = -J[0][1] / g.det;
-        g.inv[1][0] = -J[1][0] / g.det;
-        g.inv[1][1] = J[0][0] / g.det;
+        g.det = J[0] * J[4] - J[1] * J[3];
+        g.inv[0] = J[4] / g.det;
+        g.inv[1] = -J[1] / g.det;
+        g.inv[3] = -J[3] / g.det;
+        g.inv[4] = J[0] / g.det;
     } else {
-        const double(*J)[3] = g.J;
-        g.det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
-                J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
-        double d = g.det;
-        g.inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / d;
-        g.inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / d;
-        g.inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / d;
-        g.inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / d;
-        g.inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / d;
-        g.inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / d;
-        g.inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / d;
-        g.inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / d;
-        g.inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / d;
+        g.det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) + J[2] * (J[3] * J[7] - J[6] * J[4]);
+        const double d = g.det;
+        g.inv[0] = (J[4] * J[8] - J[5] * J[7]) / d;
+        g.inv[1] = (J[2] * J[7] - J[1] * J[8]) / d;
+        g.inv[2] = (J[1] * J[5] - J[2] * J[4]) / d;
+        g.inv[3] = (J[5] * J[6] - J[3] * J[8]) / d;
+        g.inv[4] = (J[0] * J[8] - J[2] * J[6]) / d;
+        g.inv[5] = (J[2] * J[3] - J[0] * J[5]) / d;
+        g.inv[6] = (J[3] * J[7] - J[4] * J[6]) / d;
+        g.inv[7] = (J[1] * J[6] - J[0] * J[7]) / d;
+        g.inv[8] = (J[0] * J[4] - J[1] * J[3]) / d;
     }
 }
 
@@ -79,16 +106,20 @@ __device__ inline double field_value(const DevBucket& b, const DevInputs& in, in
     const DevField& fl = b.fields[f];
     const double* data = in.field_data[f];
     double v[4] = {0, 0, 0, 0};
-    int nd = sd == 0 ? 1 : 1 + b.elemdim;
+    const int nd = sd == 0 ? 1 : 1 + b.elemdim;
     for (int k = 0; k < fl.nff; k++) {
-        int a = fl.addr[(size_t)k * b.n_elem + e];
+        const int a = fl.addr[(size_t)k * b.n_elem + e];
         if (a < 0) continue;
-        double c = data[a];
-        for (int d = 0; d < nd; d++) v[d] += c * fl.tab[((size_t)d * fl.nff + k) * b.n_gp + q];
+        const double c = data[a];
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            if (d < nd) v[d] += c * fl.tab[((size_t)d * fl.nff + k) * b.n_gp + q];
     }
     if (sd == 0) return v[0];
     double s = 0.0;
-    for (int i = 0; i < b.elemdim; i++) s += g.inv[sd - 1][i] * v[1 + i];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (i < b.elemdim) s += pick9(g.inv, 3 * (sd - 1) + i) * v[1 + i];
     return s;
 }
 
@@ -110,7 +141,7 @@ __device__ inline T apply_unary(int op, T x) {
 // Coefficient program for one (element, Gauss point); outputs go to
 // out[slot] multiplied by |det J|.
 template <typename T>
-__device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int q, const Geo& g, T* out, int out_stride) {
+__device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int q, const Geo& g, T* out, int64_t out_stride) {
     T reg[SLX_MAX_REGS];
     T adet = (T)fabs(g.det);
     for (int pc = 0; pc < b.n_instr; pc++) {
@@ -120,8 +151,8 @@ __device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int 
         switch (ins.op) {
             case SLX_OP_CONST: r = (T)in.consts[ins.imm]; break;
             case SLX_OP_PARAM: r = (T)in.params[(size_t)ins.imm * in.param_len + b.elem_ids[e]]; break;
-            case SLX_OP_JAC: r = (T)g.J[ins.a][ins.b]; break;
-            case SLX_OP_INVJAC: r = (T)g.inv[ins.a][ins.b]; break;
+            case SLX_OP_JAC: r = (T)pick9(g.J, 3 * ins.a + ins.b); break;
+            case SLX_OP_INVJAC: r = (T)pick9(g.inv, 3 * ins.a + ins.b); break;
             case SLX_OP_DETJ: r = (T)g.det; break;
             case SLX_OP_COORD: {
                 double s = 0.0;
@@ -143,70 +174,128 @@ __device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int 
     }
 }
 
-// Work item decomposition of phase 2: (slice, element, row i, column block).
-#define SLX_JBLK 16
+// Coefficient table of one bucket (nq = 1 when the coefficients do not vary
+// over the Gauss points).
+template <typename T>
+__global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, int nq, T* __restrict__ ctab) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)b.n_elem * nq) return;
+    const int e = (int)(idx % b.n_elem), q = (int)(idx / b.n_elem);
+    Geo g;
+    geometry(b, in.xyz, e, q, g);
+    run_program<T>(b, in, e, q, g, ctab + (size_t)q * b.n_elem + e, (int64_t)nq * b.n_elem);
+}
 
-template <typename CT, typename KT>
-__global__ void __launch_bounds__(256) elem_kernel(DevBucket b, DevInputs in, int ne_tile, uint32_t slice_mask, KT* __restrict__ ke) {
-    extern __shared__ unsigned char smem_raw[];
-    CT* C = reinterpret_cast<CT*>(smem_raw);  // [ne_tile][nq][n_coefs]
+// Element matrices, lanes = elements.  Shape tables are read through the
+// constant address space (scalar loads, Gauss point major); coefficients come
+// from the coalesced table.  Per slice and (TI x TJ) tile:
+//   D_rs   = sum of the slice's term coefficients with derivative pair (r, s)
+//   h_r[j] = w_q sum_s D_rs dof_s[j][q]
+//   Ke[i][j] += sum_r tf_r[i][q] h_r[j]
+// which is the reference's sum over terms and Gauss points regrouped.
+template <typename CT, int TI, int TJ>
+__global__ void __launch_bounds__(256) elem_tiled(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, CT* __restrict__ ke) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = e < b.n_elem;
+    const int ec = live ? e : b.n_elem - 1;
     const int nq = b.q_inv ? 1 : b.n_gp;
-    const int e0 = blockIdx.x * ne_tile;
-    const int nloc = min(ne_tile, b.n_elem - e0);
-    for (int idx = threadIdx.x; idx < nloc * nq; idx += blockDim.x) {
-        int el = idx / nq, q = idx % nq;
-        Geo g;
-        geometry(b, in.xyz, e0 + el, q, g);
-        run_program<CT>(b, in, e0 + el, q, g, C + ((size_t)el * nq + q) * b.n_coefs, 1);
-    }
-    __syncthreads();
-    // phase 2
-    int items_before[SLX_MAX_SLICES + 1];
-    items_before[0] = 0;
+    const int ngp = b.n_gp;
+    const size_t cstride = (size_t)b.n_elem;
+    const SLX_CONST double* gw = (const SLX_CONST double*)b.gp_w;
+    const SLX_CONST DevSlice* dsl = (const SLX_CONST DevSlice*)b.dslices;
     for (int s = 0; s < b.n_slices; s++) {
-        int nj = (b.slices[s].n_dof + SLX_JBLK - 1) / SLX_JBLK;
-        int n = ((slice_mask >> s) & 1u) ? nloc * b.slices[s].n_tf * nj : 0;
-        items_before[s + 1] = items_before[s] + n;
-    }
-    int total = items_before[b.n_slices];
-    for (int it = threadIdx.x; it < total; it += blockDim.x) {
-        int s = 0;
-        while (it >= items_before[s + 1]) s++;
-        const DevSlice& sl = b.slices[s];
-        int nj = (sl.n_dof + SLX_JBLK - 1) / SLX_JBLK;
-        int local = it - items_before[s];
-        int jb = local % nj;
-        int rest = local / nj;
-        int i = rest % sl.n_tf;
-        int el = rest / sl.n_tf;
-        int j0 = jb * SLX_JBLK;
-        int jn = min(SLX_JBLK, sl.n_dof - j0);
-        CT acc[SLX_JBLK];
-        for (int jj = 0; jj < SLX_JBLK; jj++) acc[jj] = 0;
-        const CT* Ce = C + (size_t)el * nq * b.n_coefs;
-        for (int q = 0; q < b.n_gp; q++) {
-            const CT* Cq = Ce + (size_t)(b.q_inv ? 0 : q) * b.n_coefs;
-            CT w = (CT)b.gp_w[q];
-            for (int t = 0; t < sl.n_terms; t++) {
-                slx_term tm = sl.terms[t];
-                CT c = Cq[tm.coef] * w * tabv<CT>(sl.tf_tab, tm.tf_deriv, sl.n_tf, i, b.n_gp, q);
-                if (sl.is_rhs) {
-                    acc[0] += c;
-                } else {
-                    for (int jj = 0; jj < jn; jj++) acc[jj] += c * tabv<CT>(sl.dof_tab, tm.dof_deriv, sl.n_dof, j0 + jj, b.n_gp, q);
+        if (!((slice_mask >> s) & 1u)) continue;
+        const int n_tf = dsl[s].n_tf, n_dof = dsl[s].n_dof, is_rhs = dsl[s].is_rhs, n_terms = dsl[s].n_terms;
+        const int tf_pad = dsl[s].tf_pad, dof_pad = dsl[s].dof_pad;
+        const SLX_CONST slx_term* terms = (const SLX_CONST slx_term*)dsl[s].terms;
+        const SLX_CONST double* tfq = (const SLX_CONST double*)dsl[s].tf_tabq;
+        const SLX_CONST double* dfq = (const SLX_CONST double*)dsl[s].dof_tabq;
+        const int64_t ke_base = dsl[s].ke_base, ke_ld = dsl[s].ke_ld;
+        unsigned rmask = 0, dmask = 0;
+        for (int t = 0; t < n_terms; t++) {
+            rmask |= 1u << terms[t].tf_deriv;
+            dmask |= 1u << terms[t].dof_deriv;
+        }
+        CT D[4][4];
+        auto load_D = [&](int qq) {
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int d = 0; d < 4; d++) D[r][d] = 0;
+            for (int t = 0; t < n_terms; t++) {
+                const int tr = terms[t].tf_deriv, td = terms[t].dof_deriv;
+                const CT c = ctab[((size_t)terms[t].coef * nq + qq) * cstride + ec];
+#pragma unroll
+                for (int r = 0; r < 4; r++)
+#pragma unroll
+                    for (int d = 0; d < 4; d++)
+                        if (r == tr && d == td) D[r][d] += c;
+            }
+        };
+        if (nq == 1) load_D(0);
+        for (int i0 = 0; i0 < n_tf; i0 += TI) {
+            for (int j0 = 0; j0 < n_dof; j0 += TJ) {
+                CT acc[TI][TJ];
+#pragma unroll
+                for (int ii = 0; ii < TI; ii++)
+#pragma unroll
+                    for (int jj = 0; jj < TJ; jj++) acc[ii][jj] = 0;
+                for (int q = 0; q < ngp; q++) {
+                    if (nq != 1) load_D(q);
+                    const CT w = (CT)gw[q];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        if (!((rmask >> r) & 1u)) continue;
+                        CT h[TJ];
+                        if (is_rhs) {
+                            // right-hand side: the "dof" factor is 1
+                            CT sum = 0;
+#pragma unroll
+                            for (int d = 0; d < 4; d++) sum += D[r][d];
+#pragma unroll
+                            for (int jj = 0; jj < TJ; jj++) h[jj] = jj == 0 ? w * sum : (CT)0;
+                        } else {
+#pragma unroll
+                            for (int jj = 0; jj < TJ; jj++) h[jj] = 0;
+#pragma unroll
+                            for (int d = 0; d < 4; d++) {
+                                if (!((dmask >> d) & 1u)) continue;
+                                const SLX_CONST double* dv = dfq + ((size_t)q * 4 + d) * dof_pad + j0;
+                                const CT wd = w * D[r][d];
+#pragma unroll
+                                for (int jj = 0; jj < TJ; jj++) h[jj] += wd * (CT)dv[jj];
+                            }
+                        }
+                        const SLX_CONST double* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
+#pragma unroll
+                        for (int ii = 0; ii < TI; ii++) {
+                            const CT t = (CT)tv[ii];
+#pragma unroll
+                            for (int jj = 0; jj < TJ; jj++) acc[ii][jj] += t * h[jj];
+                        }
+                    }
+                }
+                if (live) {
+                    const int ni = min(TI, n_tf - i0), nj = min(TJ, n_dof - j0);
+                    CT* out = ke + ke_base + ((int64_t)e * n_tf + i0) * ke_ld + j0;
+#pragma unroll
+                    for (int ii = 0; ii < TI; ii++)
+#pragma unroll
+                        for (int jj = 0; jj < TJ; jj++)
+                            if (ii < ni && jj < nj) out[(int64_t)ii * ke_ld + jj] = is_rhs ? -acc[ii][jj] : acc[ii][jj];
                 }
             }
         }
-        size_t base = sl.ke_base + ((size_t)(e0 + el) * sl.n_tf + i) * sl.n_dof + j0;
-        if (sl.is_rhs)
-            ke[base] = (KT)(-acc[0]);
-        else
-            for (int jj = 0; jj < jn; jj++) ke[base + jj] = (KT)acc[jj];
     }
 }
 
-// Deterministic reduction of the element rows into the CSR A/D values.
-template <typename KT>
+// Deterministic reduction of the element-row segments into the CSR A/D
+// values.  A lane group (group >= longest segment) owns one row; lanes hold
+// consecutive entries of a segment, so within one segment every slot differs
+// and the LDS additions of a segment never collide.  Segments are consumed in
+// a fixed order (batches of four loaded ahead), so every value is summed in
+// the same order on every run.
+template <typename KT, typename ST>
 __global__ void __launch_bounds__(256) reduce_matrix(DevReduce rd, const KT* __restrict__ ke, int group, double* __restrict__ Aval,
                                                       double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
@@ -214,31 +303,69 @@ __global__ void __launch_bounds__(256) reduce_matrix(DevReduce rd, const KT* __r
     const int groups_per_block = blockDim.x / group;
     const int gid = threadIdx.x / group, lane = threadIdx.x % group;
     const int64_t r = (int64_t)blockIdx.x * groups_per_block + gid;
+    if (r >= rd.nrows) return;
     double* acc = accall + (size_t)gid * rd.max_row_len;
-    if (r < rd.nrows) {
-        int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
-        int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
-        int len = lenA + lenD;
-        for (int x = lane; x < len; x += group) acc[x] = 0.0;
-        // Lanes of a group belong to one wavefront and step through the element
-        // rows in lockstep: within an element row all slots differ, so every
-        // slot receives its contributions in element-row order.
-        __builtin_amdgcn_wave_barrier();
-        int64_t sp = rd.slot_ptr[r];
-        for (int64_t k = rd.erow_ptr[r]; k < rd.erow_ptr[r + 1]; k++) {
-            uint64_t pk = rd.erows[k];
-            int n = (int)(pk >> 48);
-            int64_t off = (int64_t)(pk & 0xFFFFFFFFFFFFULL);
-            for (int j = lane; j < n; j += group) {
-                int sl = rd.slots[sp + j];
-                if (sl >= 0) acc[sl] += (double)ke[off + j];
+    const ST* slots = reinterpret_cast<const ST*>(rd.slots);
+    const ST SKIP = (ST)~(ST)0;
+    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
+    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
+    for (int x = lane; x < lenA + lenD; x += group) acc[x] = 0.0;
+    const int64_t k0 = rd.erow_ptr[r], k1 = rd.erow_ptr[r + 1];
+    int64_t sp = rd.slot_ptr[r];
+    for (int64_t kb = k0; kb < k1; kb += group) {
+        const int nk = (int)min((int64_t)group, k1 - kb);
+        const uint64_t mine = lane < nk ? rd.erows[kb + lane] : 0ull;
+        for (int k = 0; k < nk; k += 4) {
+            KT v[4];
+            ST sl[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const uint64_t dsc = __shfl(mine, min(k + u, nk - 1), group);
+                const int n = (k + u < nk) ? (int)(dsc >> 48) : 0;
+                const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+                const bool on = lane < n;
+                v[u] = on ? ke[off + lane] : (KT)0;
+                sl[u] = on ? slots[sp + lane] : SKIP;
+                sp += n;
             }
-            sp += n;
-            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (sl[u] != SKIP) acc[sl[u]] += (double)v[u];
         }
-        for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
-        for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
     }
+    for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
+    for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
+}
+
+// Segments longer than 64 entries: lanes stride over the segment.
+template <typename KT, typename ST>
+__global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT* __restrict__ ke, double* __restrict__ Aval,
+                                                           double* __restrict__ Dval) {
+    extern __shared__ unsigned char smem_raw[];
+    double* accall = reinterpret_cast<double*>(smem_raw);
+    const int group = 64;
+    const int gid = threadIdx.x / group, lane = threadIdx.x % group;
+    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / group) + gid;
+    if (r >= rd.nrows) return;
+    double* acc = accall + (size_t)gid * rd.max_row_len;
+    const ST* slots = reinterpret_cast<const ST*>(rd.slots);
+    const ST SKIP = (ST)~(ST)0;
+    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
+    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
+    for (int x = lane; x < lenA + lenD; x += group) acc[x] = 0.0;
+    int64_t sp = rd.slot_ptr[r];
+    for (int64_t k = rd.erow_ptr[r]; k < rd.erow_ptr[r + 1]; k++) {
+        const uint64_t dsc = rd.erows[k];
+        const int n = (int)(dsc >> 48);
+        const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+        for (int j = lane; j < n; j += group) {
+            const ST sl = slots[sp + j];
+            if (sl != SKIP) acc[sl] += (double)ke[off + j];
+        }
+        sp += n;
+    }
+    for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
+    for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
 }
 
 // rhs: b[dof] = sum of the element-row values in accumulation order
@@ -255,150 +382,43 @@ __global__ void reduce_rhs(DevReduce rd, const KT* __restrict__ ke, double* __re
     rhs[rd.free_dofs[r]] = s;
 }
 
-// FP64 element kernel, factorized and register-blocked ("lanes = elements").
-// A workgroup takes 64 elements; every wave walks work combos (slice, row
-// tile, column tile) that are uniform across its lanes, so the shape-function
-// tables are read through the scalar unit while each lane owns one element:
-//   H_r[j] = w_q sum_{t: r_t = r} C_t(e,q) dof_{s_t}[j][q]
-//   Ke[i][j] += sum_r tf_r[i][q] H_r[j]
-// which is the reference's sum over terms and Gauss points regrouped
-// (B^T D B with D gathered per tf derivative).
-template <int TI, int TJ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) elem64_tiled(DevBucket b, DevInputs in, uint32_t slice_mask, double* __restrict__ ke) {
-    extern __shared__ unsigned char smem_raw[];
-    double* C = reinterpret_cast<double*>(smem_raw);  // [64][nq][n_coefs]
-    const int NE = 64;
-    const int nq = b.q_inv ? 1 : b.n_gp;
-    const int e0 = blockIdx.x * NE;
-    const int nloc = min(NE, b.n_elem - e0);
-    for (int idx = threadIdx.x; idx < nloc * nq; idx += blockDim.x) {
-        int el = idx % nloc, q = idx / nloc;
-        Geo g;
-        geometry(b, in.xyz, e0 + el, q, g);
-        run_program<double>(b, in, e0 + el, q, g, C + ((size_t)el * nq + q) * b.n_coefs, 1);
-    }
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // combos: for each masked slice, row tiles x column tiles
-    int ncombo = 0;
-    for (int s = 0; s < b.n_slices; s++)
-        if ((slice_mask >> s) & 1u) ncombo += ((b.slices[s].n_tf + TI - 1) / TI) * ((b.slices[s].n_dof + TJ - 1) / TJ);
-    for (int c = wave; c < ncombo; c += 4) {
-        // decode the combo (uniform)
-        int s = 0, rem = c;
-        for (; s < b.n_slices; s++) {
-            if (!((slice_mask >> s) & 1u)) continue;
-            int n = ((b.slices[s].n_tf + TI - 1) / TI) * ((b.slices[s].n_dof + TJ - 1) / TJ);
-            if (rem < n) break;
-            rem -= n;
-        }
-        const DevSlice& sl = b.slices[s];
-        const int njt = (sl.n_dof + TJ - 1) / TJ;
-        const int i0 = (rem / njt) * TI, j0 = (rem % njt) * TJ;
-        const int ni = min(TI, sl.n_tf - i0), nj = min(TJ, sl.n_dof - j0);
-        double acc[TI][TJ];
-#pragma unroll
-        for (int ii = 0; ii < TI; ii++)
-#pragma unroll
-            for (int jj = 0; jj < TJ; jj++) acc[ii][jj] = 0.0;
-        const int el = lane;
-        const double* Ce = C + (size_t)min(el, NE - 1) * nq * b.n_coefs;
-        for (int q = 0; q < b.n_gp; q++) {
-            const double* Cq = Ce + (size_t)(b.q_inv ? 0 : q) * b.n_coefs;
-            const double w = b.gp_w[q];
-            double h[4][TJ];
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int jj = 0; jj < TJ; jj++) h[r][jj] = 0.0;
-            unsigned rmask = 0;
-            for (int t = 0; t < sl.n_terms; t++) {
-                const slx_term tm = sl.terms[t];
-                const double cw = Cq[tm.coef] * w;
-                rmask |= 1u << tm.tf_deriv;
-                if (sl.is_rhs) {
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (r == tm.tf_deriv) h[r][0] += cw;
-                } else {
-                    const double* dt = sl.dof_tab + ((size_t)tm.dof_deriv * sl.n_dof + j0) * b.n_gp + q;
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        if (r == tm.tf_deriv)
-#pragma unroll
-                            for (int jj = 0; jj < TJ; jj++) h[r][jj] += jj < nj ? cw * dt[(size_t)jj * b.n_gp] : 0.0;
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                if (!((rmask >> r) & 1u)) continue;
-                const double* tt = sl.tf_tab + ((size_t)r * sl.n_tf + i0) * b.n_gp + q;
-#pragma unroll
-                for (int ii = 0; ii < TI; ii++) {
-                    const double tv = ii < ni ? tt[(size_t)ii * b.n_gp] : 0.0;
-#pragma unroll
-                    for (int jj = 0; jj < TJ; jj++) acc[ii][jj] += tv * h[r][jj];
-                }
-            }
-        }
-        if (el < nloc) {
-            const size_t base = sl.ke_base + ((size_t)(e0 + el) * sl.n_tf + i0) * sl.n_dof + j0;
-#pragma unroll
-            for (int ii = 0; ii < TI; ii++)
-#pragma unroll
-                for (int jj = 0; jj < TJ; jj++)
-                    if (ii < ni && jj < nj) ke[base + (size_t)ii * sl.n_dof + jj] = sl.is_rhs ? -acc[ii][jj] : acc[ii][jj];
-        }
-    }
-}
-
 // ---------------------------------------------------------------- launchers --
 
-int elem_tile_size(const DevBucket& b, size_t elem_bytes, size_t lds_budget) {
-    int nq = b.q_inv ? 1 : b.n_gp;
-    size_t per = (size_t)nq * b.n_coefs * elem_bytes;
-    int t = (int)(lds_budget / (per ? per : 1));
-    if (t > 64) t = 64;
-    if (t < 1) t = 1;
-    return t;
-}
-
-template <int T>
-static hipError_t launch_tiled64(const DevBucket& b, const DevInputs& in, uint32_t mask, double* ke, hipStream_t st, size_t lds) {
-    int grid = (b.n_elem + 63) / 64;
-    hipLaunchKernelGGL((elem64_tiled<T, T>), dim3(grid), dim3(256), lds, st, b, in, mask, ke);
+hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st) {
+    if (b.n_elem == 0 || b.n_coefs == 0) return hipSuccess;
+    const int nq = b.q_inv ? 1 : b.n_gp;
+    const int64_t n = (int64_t)b.n_elem * nq;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    if (fp64)
+        hipLaunchKernelGGL((coef_kernel<double>), dim3(grid), dim3(256), 0, st, b, in, nq, (double*)ctab);
+    else
+        hipLaunchKernelGGL((coef_kernel<float>), dim3(grid), dim3(256), 0, st, b, in, nq, (float*)ctab);
     return hipGetLastError();
 }
 
-hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, uint32_t slice_mask) {
+template <typename CT, int T>
+static void launch_tiled(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
+    const unsigned grid = (unsigned)((b.n_elem + 255) / 256);
+    hipLaunchKernelGGL((elem_tiled<CT, T, T>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, (CT*)ke);
+}
+
+template <typename CT>
+static void launch_tiled_any(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
+    int nmax = 0;
+    for (int s = 0; s < b.n_slices; s++)
+        if ((mask >> s) & 1u) nmax = max(nmax, max(b.slices[s].n_tf, b.slices[s].n_dof));
+    if (nmax <= 4) launch_tiled<CT, 4>(b, ctab, mask, ke, st);
+    else if (nmax <= 6) launch_tiled<CT, 6>(b, ctab, mask, ke, st);
+    else if (nmax <= 10) launch_tiled<CT, 10>(b, ctab, mask, ke, st);
+    else launch_tiled<CT, 8>(b, ctab, mask, ke, st);
+}
+
+hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask) {
     if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
-    if (precision == SLX_PREC_FP64) {
-        // lanes-as-elements kernel when the 64-element coefficient table fits
-        int nq = b.q_inv ? 1 : b.n_gp;
-        size_t lds64 = (size_t)64 * nq * b.n_coefs * sizeof(double);
-        int nmax = 0;
-        for (int s = 0; s < b.n_slices; s++)
-            if ((slice_mask >> s) & 1u) nmax = max(nmax, max(b.slices[s].n_tf, b.slices[s].n_dof));
-        if (lds64 <= 48 * 1024 && b.n_coefs > 0) {
-            if (nmax <= 4) return launch_tiled64<4>(b, in, slice_mask, (double*)ke, st, lds64);
-            if (nmax <= 6) return launch_tiled64<6>(b, in, slice_mask, (double*)ke, st, lds64);
-            if (nmax <= 10) return launch_tiled64<10>(b, in, slice_mask, (double*)ke, st, lds64);
-            return launch_tiled64<8>(b, in, slice_mask, (double*)ke, st, lds64);
-        }
-        int tile = elem_tile_size(b, sizeof(double), 48 * 1024);
-        size_t lds = (size_t)tile * nq * b.n_coefs * sizeof(double);
-        if (lds > 64 * 1024) return hipErrorInvalidValue;
-        int grid = (b.n_elem + tile - 1) / tile;
-        hipLaunchKernelGGL((elem_kernel<double, double>), dim3(grid), dim3(256), lds, st, b, in, tile, slice_mask, (double*)ke);
-    } else {
-        int tile = elem_tile_size(b, sizeof(float), 48 * 1024);
-        int nq = b.q_inv ? 1 : b.n_gp;
-        size_t lds = (size_t)tile * nq * b.n_coefs * sizeof(float);
-        if (lds > 64 * 1024) return hipErrorInvalidValue;
-        int grid = (b.n_elem + tile - 1) / tile;
-        hipLaunchKernelGGL((elem_kernel<float, float>), dim3(grid), dim3(256), lds, st, b, in, tile, slice_mask, (float*)ke);
-    }
+    if (precision == SLX_PREC_FP64)
+        launch_tiled_any<double>(b, ctab, slice_mask, ke, st);
+    else
+        launch_tiled_any<float>(b, ctab, slice_mask, ke, st);
     return hipGetLastError();
 }
 
@@ -408,24 +428,40 @@ int reduce_group_size(int max_erow_len) {
     return g;
 }
 
+template <typename KT, typename ST>
+static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st) {
+    const bool longseg = max_erow_len > 64;
+    const int g = longseg ? 64 : reduce_group_size(max_erow_len);
+    int threads = 256, gpb = threads / g;
+    size_t lds = (size_t)gpb * rd.max_row_len * sizeof(double);
+    while (lds > 64 * 1024 && threads > g) {
+        threads /= 2;
+        gpb = threads / g;
+        lds = (size_t)gpb * rd.max_row_len * sizeof(double);
+    }
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    const int64_t grid = (rd.nrows + gpb - 1) / gpb;
+    if (longseg)
+        hipLaunchKernelGGL((reduce_matrix_long<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
+    else
+        hipLaunchKernelGGL((reduce_matrix<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, g, Aval, Dval);
+    return hipGetLastError();
+}
+
+template <typename KT>
+static hipError_t launch_reduce_k(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st) {
+    if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t>(rd, max_erow_len, ke, Aval, Dval, st);
+    if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t>(rd, max_erow_len, ke, Aval, Dval, st);
+    return launch_reduce_t<KT, uint32_t>(rd, max_erow_len, ke, Aval, Dval, st);
+}
+
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
                          double* rhs, bool has_matrix, bool has_rhs, hipStream_t st) {
     if (rd.nrows == 0) return hipSuccess;
     if (has_matrix) {
-        int g = reduce_group_size(max_erow_len);
-        int threads = 256, gpb = threads / g;
-        size_t lds = (size_t)gpb * rd.max_row_len * sizeof(double);
-        while (lds > 64 * 1024 && threads > g) {
-            threads /= 2;
-            gpb = threads / g;
-            lds = (size_t)gpb * rd.max_row_len * sizeof(double);
-        }
-        if (lds > 64 * 1024) return hipErrorInvalidValue;
-        int64_t grid = (rd.nrows + gpb - 1) / gpb;
-        if (ke_fp64)
-            hipLaunchKernelGGL((reduce_matrix<double>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const double*)ke, g, Aval, Dval);
-        else
-            hipLaunchKernelGGL((reduce_matrix<float>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const float*)ke, g, Aval, Dval);
+        hipError_t err = ke_fp64 ? launch_reduce_k<double>(rd, max_erow_len, ke, Aval, Dval, st)
+                                 : launch_reduce_k<float>(rd, max_erow_len, ke, Aval, Dval, st);
+        if (err != hipSuccess) return err;
     }
     if (has_rhs) {
         int64_t grid = (rd.nrows + 255) / 256;

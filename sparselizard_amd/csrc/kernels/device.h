@@ -13,9 +13,11 @@ namespace slx {
 struct DevSlice {
     int32_t n_tf, n_dof, is_rhs, n_terms;
     const slx_term* terms;      // device
-    const double* tf_tab;       // [4][n_tf][n_gp]
-    const double* dof_tab;      // [4][n_dof][n_gp] or null
-    int64_t ke_base;            // Ke entry (e, i, j) at ke_base + (e*n_tf + i)*n_dof + j
+    const double* tf_tabq;      // [n_gp][4][tf_pad]  (Gauss point major, zero padded)
+    const double* dof_tabq;     // [n_gp][4][dof_pad] or null for right-hand sides
+    int32_t tf_pad, dof_pad;
+    int64_t ke_base;            // Ke entry (e, i, j) at ke_base + (e*n_tf + i)*ke_ld + j
+    int64_t ke_ld;              // element-row segment length (1 for right-hand sides)
 };
 
 struct DevField {
@@ -32,6 +34,7 @@ struct DevBucket {
     const double* gp_w;         // [n_gp]
     const double* geo_tab;      // [4][n_geo][n_gp] order-1 Lagrange
     const slx_instr* prog;
+    const DevSlice* dslices;    // device copy of the slice descriptions
     DevSlice slices[SLX_MAX_SLICES];
     DevField fields[SLX_MAX_FIELDS];
 };
@@ -50,7 +53,8 @@ struct DevReduce {
     const int64_t* erow_ptr;        // [nrows+1] into erows
     const uint64_t* erows;          // packed (len << 48) | ke offset, in accumulation order
     const int64_t* slot_ptr;        // [nrows+1] into slots
-    const int32_t* slots;           // per Ke entry of the row: A position, lenA + D position, or -1
+    const void* slots;              // per Ke entry of the row: A position, lenA + D position, or all ones
+    int32_t slot_bytes;             // 1, 2 or 4
     const int64_t* Arow;            // [nrows+1]
     const int64_t* Drow;            // [nrows+1]
     int32_t max_row_len;            // max (lenA + lenD)
@@ -58,5 +62,8 @@ struct DevReduce {
     const uint64_t* rhs_erows;      // packed (1 << 48) | ke offset
     const int64_t* free_dofs;       // [nrows] global DOF of each free row
 };
+
+// Constant-address-space views: uniform loads through them become scalar loads.
+#define SLX_CONST __attribute__((address_space(4)))
 
 }  // namespace slx

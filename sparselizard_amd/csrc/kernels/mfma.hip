@@ -1,6 +1,7 @@
 // Mixed-precision element contraction on MFMA (gfx950 v_mfma_f32_16x16x32_f16).
 //
-// For the matrix slices of a bucket the reference computes, per term t,
+// For the matrix slices of a bucket that share one tf and one dof shape
+// table the reference computes, per term t,
 //   Ke_t[(i,j), e] = sum_q tf_t[i][q] w_q dof_t[j][q] * C_t(e,q)
 // (contribution.cpp:172-211) and sums the terms (addproduct).  Stacking the
 // terms' derivative pairs p = (r, s) and the Gauss points q into one K index
@@ -28,137 +29,45 @@ namespace slx {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
+typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 
-// Geometry and coefficient program are shared with the VALU kernels.
-struct GeoM {
-    double J[3][3], inv[3][3], det;
-};
-
-__device__ inline void geometry_m(const DevBucket& b, const double* xyz, int e, int q, GeoM& g) {
-    double x[SLX_MAX_GEO][3];
-    for (int k = 0; k < b.n_geo; k++) {
-        int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-        x[k][0] = xyz[3 * (size_t)node];
-        x[k][1] = xyz[3 * (size_t)node + 1];
-        x[k][2] = xyz[3 * (size_t)node + 2];
-    }
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) g.J[r][c] = 0.0;
-    for (int r = 0; r < b.elemdim; r++)
-        for (int c = 0; c < b.meshdim; c++) {
-            double s = 0.0;
-            for (int k = 0; k < b.n_geo; k++) s += x[k][c] * b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q];
-            g.J[r][c] = s;
-        }
-    const double(*J)[3] = g.J;
-    if (b.meshdim == 1) {
-        g.det = J[0][0];
-        g.inv[0][0] = 1.0 / J[0][0];
-    } else if (b.meshdim == 2) {
-        g.det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        g.inv[0][0] = J[1][1] / g.det;
-        g.inv[0][1] = -J[0][1] / g.det;
-        g.inv[1][0] = -J[1][0] / g.det;
-        g.inv[1][1] = J[0][0] / g.det;
-    } else {
-        g.det = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[2][0] * J[1][2]) +
-                J[0][2] * (J[1][0] * J[2][1] - J[2][0] * J[1][1]);
-        double d = g.det;
-        g.inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / d;
-        g.inv[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / d;
-        g.inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / d;
-        g.inv[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / d;
-        g.inv[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / d;
-        g.inv[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / d;
-        g.inv[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / d;
-        g.inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / d;
-        g.inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / d;
-    }
+// Column preparation (per workgroup tile of NE elements): for every
+// (element, slice) column the coefficient values C[p][q] are read from the
+// coalesced coefficient table, scaled by a power of two and split.  A column
+// lives in LDS as packed 32-bit words (hi | lo' << 16):
+//   compact  (coefficients constant over the Gauss points, n_gp >= 8):
+//            word p = C[p]; B[k][e] = C[k / n_gp] is rebuilt per lane from
+//            the two words its eight consecutive k can touch;
+//   expanded otherwise: word k = C[k / n_gp][k % n_gp] for k < K, 0 up to
+//            the padded K, read as two 16-byte LDS loads per K step.
+__device__ inline uint32_t pack_split(float x) {
+    _Float16 hi = (_Float16)x;
+    _Float16 lo = (_Float16)((x - (float)hi) * 2048.f);
+    return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
 }
 
-__device__ inline double field_value_m(const DevBucket& b, const DevInputs& in, int f, int sd, int e, int q, const GeoM& g) {
-    const DevField& fl = b.fields[f];
-    const double* data = in.field_data[f];
-    double v[4] = {0, 0, 0, 0};
-    int nd = sd == 0 ? 1 : 1 + b.elemdim;
-    for (int k = 0; k < fl.nff; k++) {
-        int a = fl.addr[(size_t)k * b.n_elem + e];
-        if (a < 0) continue;
-        double c = data[a];
-        for (int d = 0; d < nd; d++) v[d] += c * fl.tab[((size_t)d * fl.nff + k) * b.n_gp + q];
-    }
-    if (sd == 0) return v[0];
-    double s = 0.0;
-    for (int i = 0; i < b.elemdim; i++) s += g.inv[sd - 1][i] * v[1 + i];
-    return s;
-}
-
-// FP32 coefficient program; OUT slots are routed through `route`
-// (slot -> index into the tile's coefficient table, -1 = not for this kernel).
-__device__ void run_program_f32(const DevBucket& b, const DevInputs& in, int e, int q, const GeoM& g, const int* route, float* Cf,
-                                int cf_stride_slot_unused) {
-    float reg[SLX_MAX_REGS];
-    float adet = (float)fabs(g.det);
-    for (int pc = 0; pc < b.n_instr; pc++) {
-        slx_instr ins = b.prog[pc];
-        float va = reg[ins.a], vb = reg[ins.b];
-        float r = 0.f;
-        switch (ins.op) {
-            case SLX_OP_CONST: r = (float)in.consts[ins.imm]; break;
-            case SLX_OP_PARAM: r = (float)in.params[(size_t)ins.imm * in.param_len + b.elem_ids[e]]; break;
-            case SLX_OP_JAC: r = (float)g.J[ins.a][ins.b]; break;
-            case SLX_OP_INVJAC: r = (float)g.inv[ins.a][ins.b]; break;
-            case SLX_OP_DETJ: r = (float)g.det; break;
-            case SLX_OP_COORD: {
-                double s = 0.0;
-                for (int k = 0; k < b.n_geo; k++) {
-                    int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-                    s += in.xyz[3 * (size_t)node + ins.a] * b.geo_tab[((size_t)0 * b.n_geo + k) * b.n_gp + q];
-                }
-                r = (float)s;
-                break;
-            }
-            case SLX_OP_FIELD: r = (float)field_value_m(b, in, ins.imm, ins.a, e, q, g); break;
-            case SLX_OP_ADD: r = va + vb; break;
-            case SLX_OP_MUL: r = va * vb; break;
-            case SLX_OP_POW: r = powf(va, vb); break;
-            case SLX_OP_INV: r = 1.f / va; break;
-            case SLX_OP_ABS: r = fabsf(va); break;
-            case SLX_OP_EXP: r = expf(va); break;
-            case SLX_OP_LOG: r = logf(va); break;
-            case SLX_OP_SIN: r = sinf(va); break;
-            case SLX_OP_COS: r = cosf(va); break;
-            case SLX_OP_TAN: r = tanf(va); break;
-            case SLX_OP_SQRT: r = sqrtf(va); break;
-            case SLX_OP_OUT: {
-                int dst = route[ins.imm];
-                if (dst >= 0) Cf[dst] = va * adet;
-                continue;
-            }
-        }
-        reg[ins.dst] = r;
-    }
-}
-
-template <int MTW, int MAXK>
-__global__ void __launch_bounds__(256) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, DevInputs in, float* __restrict__ ke) {
+template <int MTW, int MAXK, bool COMPACT>
+__global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
+                                                            float* __restrict__ ke) {
     extern __shared__ unsigned char smem_raw[];
     const int NE = mb.ne_tile;
     const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq, ngp = b.n_gp;
-    const int colvals = np * nq;  // coefficient values per (element, slice) column
-    float* Cf = reinterpret_cast<float*>(smem_raw);                         // [NE][ns][np][nq]
-    _Float16* Cs = reinterpret_cast<_Float16*>(Cf + (size_t)NE * ns * colvals);  // [NE][ns][np][nq][2]
-    float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * ns * colvals * 2);  // [ns][NE]
-    int* route = reinterpret_cast<int*>(colsc + (size_t)ns * NE);                 // [n_coefs]
-    for (int c = threadIdx.x; c < b.n_coefs; c += blockDim.x) route[c] = mb.route[c];
+    const int estride = mb.estride, cstride = mb.cstride;
+    uint32_t* Cs = reinterpret_cast<uint32_t*>(smem_raw);          // [NE][estride]
+    float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * estride);  // [ns][NE]
+    const SLX_CONST int32_t* csrc = (const SLX_CONST int32_t*)mb.csrc;
+    const SLX_CONST int64_t* kebase = (const SLX_CONST int64_t*)mb.kebase;
+    const SLX_CONST int64_t* keld = (const SLX_CONST int64_t*)mb.keld;
 
+    const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lrow = lane & 15, lgrp = lane >> 4;
-    // A fragments of this wave's M tiles (interleaved: tile = wave + 4 i)
+    // A fragments of this wave's M tiles (tile = wave + nw * i)
     half8 Ahi[MTW][MAXK], Alo[MTW][MAXK];
+    int irow[MTW][4], jrow[MTW][4];
 #pragma unroll
     for (int i = 0; i < MTW; i++) {
-        int mt = wave + 4 * i;
+        const int mt = wave + nw * i;
 #pragma unroll
         for (int ks = 0; ks < MAXK; ks++) {
             if (mt < mb.mtiles && ks < mb.ksteps) {
@@ -166,54 +75,74 @@ __global__ void __launch_bounds__(256) mfma_f16split_kernel(DevBucket b, MfmaBuc
                 Ahi[i][ks] = src[lane];
                 Alo[i][ks] = src[64 + lane];
             } else {
+#pragma unroll
                 for (int v = 0; v < 8; v++) {
                     Ahi[i][ks][v] = (_Float16)0.f;
                     Alo[i][ks][v] = (_Float16)0.f;
                 }
             }
         }
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = mt * 16 + 4 * lgrp + v;
+            irow[i][v] = row / mb.n_dof;
+            jrow[i][v] = row - irow[i][v] * mb.n_dof;
+        }
     }
+    // compact columns: the K range of this lane's eight B values per K step
+    int plo[MAXK], vsplit[MAXK], vvalid[MAXK];
+#pragma unroll
+    for (int ks = 0; ks < MAXK; ks++) {
+        const int k0 = ks * 32 + 8 * lgrp;
+        const int p0 = min(k0 / ngp, np - 1);
+        plo[ks] = p0;
+        vsplit[ks] = (p0 + 1) * ngp - k0;   // v < vsplit -> word p0, else p0 + 1
+        vvalid[ks] = mb.K - k0;              // v >= vvalid -> 0
+    }
+
     const int ntiles_e = (b.n_elem + NE - 1) / NE;
+    const int64_t nel = b.n_elem;
     for (int tile = blockIdx.x; tile < ntiles_e; tile += gridDim.x) {
         const int e0 = tile * NE;
         const int nloc = min(NE, b.n_elem - e0);
         __syncthreads();
-        for (int x = threadIdx.x; x < NE * ns * colvals; x += blockDim.x) Cf[x] = 0.f;
-        __syncthreads();
-        // phase 1a: coefficients of every (element, Gauss point) -> Cf
-        for (int idx = threadIdx.x; idx < nloc * nq; idx += blockDim.x) {
-            int el = idx / nq, q = idx % nq;
-            GeoM g;
-            geometry_m(b, in.xyz, e0 + el, q, g);
-            // route[slot] = (s * np + p) * nq; add the element / Gauss point offsets
-            float* base = Cf + (size_t)el * ns * colvals + q;
-            run_program_f32(b, in, e0 + el, q, g, route, base, 0);
-        }
-        __syncthreads();
-        // phase 1b: per (element, slice) power-of-two scaling and hi/lo split
-        for (int idx = threadIdx.x; idx < NE * ns; idx += blockDim.x) {
-            int el = idx / ns, s = idx % ns;
-            const float* src = Cf + ((size_t)el * ns + s) * colvals;
-            _Float16* dst = Cs + ((size_t)el * ns + s) * colvals * 2;
+        // columns: one thread per (element, slice), element fastest
+        for (int col = threadIdx.x; col < NE * ns; col += blockDim.x) {
+            const int el = col % NE, s = col / NE;
+            const bool valid = el < nloc;
+            const int64_t e = e0 + (valid ? el : 0);
             float mx = 0.f;
-            if (el < nloc)
-                for (int v = 0; v < colvals; v++) mx = fmaxf(mx, fabsf(src[v]));
+            for (int p = 0; p < np; p++) {
+                const int c = csrc[s * np + p];
+                if (c < 0 || !valid) continue;
+                for (int q = 0; q < nq; q++) mx = fmaxf(mx, fabsf(ctab[((int64_t)c * nq + q) * nel + e]));
+            }
             int ex = 0;
             if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
-            float sc = ldexpf(1.f, -ex);          // scaled max in [0.5, 1)
+            const float sc = ldexpf(1.f, -ex);
             colsc[s * NE + el] = ldexpf(1.f, ex);
-            for (int v = 0; v < colvals; v++) {
-                float x = el < nloc ? src[v] * sc : 0.f;
-                _Float16 hi = (_Float16)x;
-                _Float16 lo = (_Float16)((x - (float)hi) * 2048.f);
-                dst[2 * v] = hi;
-                dst[2 * v + 1] = lo;
+            uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
+            for (int p = 0; p < np; p++) {
+                const int c = csrc[s * np + p];
+                if (COMPACT) {
+                    const float x = (c >= 0 && valid) ? ctab[(int64_t)c * nel + e] * sc : 0.f;
+                    dst[p] = pack_split(x);
+                } else {
+                    for (int q = 0; q < ngp; q++) {
+                        const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
+                        dst[p * ngp + q] = pack_split(x);
+                    }
+                }
             }
+            if (COMPACT) dst[np] = 0u;
+            else
+                for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
         }
         __syncthreads();
-        // phase 2: per slice and 16-element column tile, the wave's M tiles
+        // contraction: per slice and 16-element column tile, the wave's M tiles
         const int nct = (NE + 15) / 16;
         for (int s = 0; s < ns; s++) {
+            const int64_t kb = kebase[s], ld = keld[s];
             for (int ct = 0; ct < nct; ct++) {
                 const int el = ct * 16 + lrow;  // this lane's B / D column
                 float4v acc_hh[MTW], acc_x[MTW];
@@ -224,44 +153,52 @@ __global__ void __launch_bounds__(256) mfma_f16split_kernel(DevBucket b, MfmaBuc
                         acc_hh[i][v] = 0.f;
                         acc_x[i][v] = 0.f;
                     }
-                const _Float16* colbase = Cs + ((size_t)el * ns + s) * colvals * 2;
+                const uint32_t* colbase = Cs + (size_t)el * estride + (size_t)s * cstride;
 #pragma unroll
                 for (int ks = 0; ks < MAXK; ks++) {
                     if (ks < mb.ksteps) {
-                    half8 bhi, blo;
-                    int k0 = ks * 32 + 8 * lgrp;
+                        uint32_t w[8];
+                        if (COMPACT) {
+                            const uint32_t u0 = colbase[plo[ks]], u1 = colbase[plo[ks] + 1];
 #pragma unroll
-                    for (int v = 0; v < 8; v++) {
-                        int k = k0 + v;
-                        _Float16 h = (_Float16)0.f, l = (_Float16)0.f;
-                        if (k < mb.K && el < NE) {
-                            int p = k / ngp, q = k - p * ngp;
-                            int off = (p * nq + (nq == 1 ? 0 : q)) * 2;
-                            h = colbase[off];
-                            l = colbase[off + 1];
+                            for (int v = 0; v < 8; v++) w[v] = v >= vvalid[ks] ? 0u : (v < vsplit[ks] ? u0 : u1);
+                        } else {
+                            const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
+                            const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
+#pragma unroll
+                            for (int v = 0; v < 4; v++) {
+                                w[v] = x0[v];
+                                w[4 + v] = x1[v];
+                            }
                         }
-                        bhi[v] = h;
-                        blo[v] = l;
-                    }
+                        uint4v hw, lw;
 #pragma unroll
-                    for (int i = 0; i < MTW; i++) {
-                        acc_hh[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi, acc_hh[i], 0, 0, 0);
-                        acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo, acc_x[i], 0, 0, 0);
-                        acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[i], 0, 0, 0);
-                    }
+                        for (int v = 0; v < 4; v++) {
+                            hw[v] = (w[2 * v] & 0xFFFFu) | (w[2 * v + 1] << 16);
+                            lw[v] = (w[2 * v] >> 16) | (w[2 * v + 1] & 0xFFFF0000u);
+                        }
+                        const half8 bhi = __builtin_bit_cast(half8, hw);
+                        const half8 blo = __builtin_bit_cast(half8, lw);
+#pragma unroll
+                        for (int i = 0; i < MTW; i++) {
+                            acc_hh[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi, acc_hh[i], 0, 0, 0);
+                            acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo, acc_x[i], 0, 0, 0);
+                            acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[i], 0, 0, 0);
+                        }
                     }
                 }
                 if (el < nloc) {
                     const float colscale = colsc[s * NE + el] * mb.a_unscale;
-                    const int64_t base = mb.ke_base[s] + (int64_t)(e0 + el) * mb.M;
+                    const int64_t ebase = kb + (int64_t)(e0 + el) * mb.n_tf * ld;
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
-                        int mt = wave + 4 * i;
+                        const int mt = wave + nw * i;
                         if (mt >= mb.mtiles) continue;
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            int row = mt * 16 + 4 * lgrp + v;
-                            if (row < mb.M) ke[base + row] = (acc_hh[i][v] + acc_x[i][v] * (1.f / 2048.f)) * colscale;
+                            const int row = mt * 16 + 4 * lgrp + v;
+                            if (row < mb.M)
+                                ke[ebase + (int64_t)irow[i][v] * ld + jrow[i][v]] = (acc_hh[i][v] + acc_x[i][v] * (1.f / 2048.f)) * colscale;
                         }
                     }
                 }
@@ -271,41 +208,45 @@ __global__ void __launch_bounds__(256) mfma_f16split_kernel(DevBucket b, MfmaBuc
 }
 
 template <int MTW, int MAXK>
-static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const DevInputs& in, float* ke, hipStream_t st) {
-    size_t lds = mfma_lds_bytes(mb, b.n_coefs);
-    int tiles = (b.n_elem + mb.ne_tile - 1) / mb.ne_tile;
-    int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
-    hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK>), dim3(grid), dim3(256), lds, st, b, mb, in, ke);
+static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
+    const size_t lds = mfma_lds_bytes(mb);
+    const int tiles = (b.n_elem + mb.ne_tile - 1) / mb.ne_tile;
+    const int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
+    if (mb.compact)
+        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true>), dim3(grid), dim3(64 * mb.nw), lds, st, b, mb, ctab, ke);
+    else
+        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, false>), dim3(grid), dim3(64 * mb.nw), lds, st, b, mb, ctab, ke);
     return hipGetLastError();
 }
 
-size_t mfma_lds_bytes(const MfmaBucket& mb, int n_coefs) {
-    size_t cv = (size_t)mb.ne_tile * mb.n_mslices * mb.n_pairs * mb.nq;
-    return cv * 4 + cv * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)n_coefs * 4;
+size_t mfma_lds_bytes(const MfmaBucket& mb) {
+    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4;
 }
 
-bool mfma_variant_fits(int mtiles, int ksteps, int* mtw, int* maxk) {
-    static const int variants[][2] = {{1, 2}, {1, 4}, {1, 8}, {2, 4}, {2, 7}, {2, 8}, {4, 4}, {4, 8}};
-    int need_mtw = (mtiles + 3) / 4;
-    for (auto& v : variants)
-        if (v[0] >= need_mtw && v[1] >= ksteps) {
-            *mtw = v[0];
-            *maxk = v[1];
-            return true;
-        }
-    return false;
+bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk) {
+    if (ksteps < 1 || ksteps > 8 || mtiles < 1) return false;
+    int w = mtiles <= 8 ? mtiles : 8;
+    int t = (mtiles + w - 1) / w;
+    if (t > 2) return false;
+    *nw = w;
+    *mtw = t;
+    *maxk = ksteps <= 1 ? 1 : ksteps <= 2 ? 2 : ksteps <= 4 ? 4 : 8;
+    return true;
 }
 
-hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const DevInputs& in, float* ke, hipStream_t st) {
+hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     if (b.n_elem == 0) return hipSuccess;
-    if (mb.mtw == 1 && mb.maxk == 2) return launch_variant<1, 2>(b, mb, in, ke, st);
-    if (mb.mtw == 1 && mb.maxk == 4) return launch_variant<1, 4>(b, mb, in, ke, st);
-    if (mb.mtw == 1 && mb.maxk == 8) return launch_variant<1, 8>(b, mb, in, ke, st);
-    if (mb.mtw == 2 && mb.maxk == 4) return launch_variant<2, 4>(b, mb, in, ke, st);
-    if (mb.mtw == 2 && mb.maxk == 7) return launch_variant<2, 7>(b, mb, in, ke, st);
-    if (mb.mtw == 2 && mb.maxk == 8) return launch_variant<2, 8>(b, mb, in, ke, st);
-    if (mb.mtw == 4 && mb.maxk == 4) return launch_variant<4, 4>(b, mb, in, ke, st);
-    if (mb.mtw == 4 && mb.maxk == 8) return launch_variant<4, 8>(b, mb, in, ke, st);
+    if (mb.mtw == 1) {
+        if (mb.maxk == 1) return launch_variant<1, 1>(b, mb, ctab, ke, st);
+        if (mb.maxk == 2) return launch_variant<1, 2>(b, mb, ctab, ke, st);
+        if (mb.maxk == 4) return launch_variant<1, 4>(b, mb, ctab, ke, st);
+        if (mb.maxk == 8) return launch_variant<1, 8>(b, mb, ctab, ke, st);
+    } else if (mb.mtw == 2) {
+        if (mb.maxk == 1) return launch_variant<2, 1>(b, mb, ctab, ke, st);
+        if (mb.maxk == 2) return launch_variant<2, 2>(b, mb, ctab, ke, st);
+        if (mb.maxk == 4) return launch_variant<2, 4>(b, mb, ctab, ke, st);
+        if (mb.maxk == 8) return launch_variant<2, 8>(b, mb, ctab, ke, st);
+    }
     return hipErrorInvalidValue;
 }
 

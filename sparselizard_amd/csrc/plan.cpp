@@ -18,7 +18,8 @@
 #include <cmath>
 
 namespace slx {
-hipError_t launch_elem(const DevBucket& b, const DevInputs& in, int precision, void* ke, hipStream_t st, uint32_t slice_mask);
+hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st);
+hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask);
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
                          double* rhs, bool has_matrix, bool has_rhs, hipStream_t st);
 }  // namespace slx
@@ -101,12 +102,16 @@ struct slx_plan_s {
     // host CSR pattern
     std::vector<int64_t> Arow, Drow;
     std::vector<int32_t> Acol, Dcol;
+    // pattern-only plans keep the reduction maps and the Ke layout on the host
+    std::vector<int64_t> h_erow_ptr, h_slot_ptr, h_rhs_ptr, h_ke_base, h_ke_ld;
+    std::vector<uint64_t> h_erows, h_rhs_erows;
+    std::vector<int32_t> h_slots;
     // device: per-bucket data
     std::vector<std::unique_ptr<DBuf>> bufs;
     std::vector<DevBucket> dbuckets;
     DBuf consts;
     DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_erow_ptr, d_erows, d_slot_ptr, d_slots, d_rhs_ptr, d_rhs_erows, d_free;
-    DBuf d_Aval, d_Dval, d_rhs, d_ke;
+    DBuf d_Aval, d_Dval, d_rhs, d_ke, d_ctab;
     size_t ke_capacity_bytes = 0;
     DevReduce rd{};
     // numeric inputs staged by slx_assemble
@@ -136,15 +141,104 @@ struct slx_plan_s {
 
 namespace {
 
-struct SliceRef {
-    int bucket, slice;
-    int64_t ke_base;
+// Element-row segment groups.  The matrix slices of a bucket that share the
+// tf function set (identical tf address table) are stored side by side: the
+// Ke entry (slice s, element e, tf function i, dof function j) lives at
+//   ke_base[s] + (e * n_tf + i) * ke_ld[s] + j,
+//   ke_base[s] = group base + column offset of s,  ke_ld[s] = total columns,
+// so one CSR row reads one contiguous segment per (element, tf function)
+// whatever the number of dof fields.  Slices whose dof address tables are
+// identical are kept in separate groups (their columns would coincide).
+// Right-hand-side slices get one value per (element, tf function).
+struct SegGroup {
+    int bucket;
+    std::vector<int> slices;
+    int n_tf;
+    int64_t len;      // segment length (sum of the slices' n_dof, 1 for rhs)
+    int64_t base;     // Ke offset of the group
+    int rank;         // smallest fragment rank of the group's slices
+    bool rhs;
 };
+
+struct KeLayout {
+    std::vector<SegGroup> groups;
+    std::vector<std::vector<int64_t>> ke_base, ke_ld;  // [bucket][slice]
+    int64_t n_ke = 0;
+};
+
+uint64_t hash_ints(const int32_t* a, size_t n) {
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < n; i++) h = (h ^ (uint32_t)a[i]) * 1099511628211ull;
+    return h;
+}
+
+KeLayout compute_layout(const slx_plan_desc& d) {
+    KeLayout L;
+    L.ke_base.resize(d.n_buckets);
+    L.ke_ld.resize(d.n_buckets);
+    for (int b = 0; b < d.n_buckets; b++) {
+        const slx_bucket_desc& bd = d.buckets[b];
+        L.ke_base[b].assign(bd.n_slices, 0);
+        L.ke_ld[b].assign(bd.n_slices, 0);
+        std::vector<uint64_t> htf(bd.n_slices, 0), hdof(bd.n_slices, 0);
+        for (int s = 0; s < bd.n_slices; s++) {
+            const slx_slice_desc& sd = bd.slices[s];
+            htf[s] = hash_ints(sd.tf_addr, (size_t)sd.n_tf * bd.n_elem);
+            if (!sd.is_rhs) hdof[s] = hash_ints(sd.dof_addr, (size_t)sd.n_dof * bd.n_elem);
+        }
+        auto same = [&](const int32_t* x, const int32_t* y, size_t n) { return x == y || std::memcmp(x, y, n * sizeof(int32_t)) == 0; };
+        size_t first = L.groups.size();
+        for (int s = 0; s < bd.n_slices; s++) {
+            const slx_slice_desc& sd = bd.slices[s];
+            int join = -1;
+            if (!sd.is_rhs)
+                for (size_t g = first; g < L.groups.size() && join < 0; g++) {
+                    const SegGroup& G = L.groups[g];
+                    if (G.rhs || G.n_tf != sd.n_tf) continue;
+                    const int s0 = G.slices[0];
+                    if (htf[s0] != htf[s] || !same(bd.slices[s0].tf_addr, sd.tf_addr, (size_t)sd.n_tf * bd.n_elem)) continue;
+                    bool clash = false;
+                    for (int t : G.slices)
+                        if (bd.slices[t].n_dof == sd.n_dof && hdof[t] == hdof[s] &&
+                            same(bd.slices[t].dof_addr, sd.dof_addr, (size_t)sd.n_dof * bd.n_elem))
+                            clash = true;
+                    if (!clash) join = (int)g;
+                }
+            if (join < 0) {
+                SegGroup G;
+                G.bucket = b;
+                G.n_tf = sd.n_tf;
+                G.len = 0;
+                G.base = 0;
+                G.rank = sd.frag_rank;
+                G.rhs = sd.is_rhs != 0;
+                L.groups.push_back(G);
+                join = (int)L.groups.size() - 1;
+            }
+            SegGroup& G = L.groups[join];
+            L.ke_base[b][s] = G.len;  // column offset for now
+            G.slices.push_back(s);
+            G.len += sd.is_rhs ? 1 : sd.n_dof;
+            G.rank = std::min(G.rank, sd.frag_rank);
+        }
+        for (size_t g = first; g < L.groups.size(); g++) {
+            SegGroup& G = L.groups[g];
+            SLX_CHECK(G.len < 65536, "element-row segment longer than 65535 entries");
+            G.base = L.n_ke;
+            for (int s : G.slices) {
+                L.ke_base[b][s] += G.base;
+                L.ke_ld[b][s] = G.len;
+            }
+            L.n_ke += (int64_t)bd.n_elem * G.n_tf * G.len;
+        }
+    }
+    return L;
+}
 
 // Symbolic phase on the host: element-row lists in the reference
 // accumulation order, per-row sorted unique columns, A/D split with local
 // renumbering (gentools::findtruefalse, gentools.cpp:2040-2068), slots.
-void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
+void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     const int64_t nd = d.n_dofs;
     std::vector<int64_t> freeidx(nd), localidx(nd);
     int64_t nf = 0, nc = 0;
@@ -158,33 +252,23 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
     std::vector<int64_t> free_dofs(nf);
     for (int64_t i = 0; i < nd; i++)
         if (freeidx[i] >= 0) free_dofs[freeidx[i]] = i;
-    // Ke layout and the slices in fragment (accumulation) order
-    std::vector<SliceRef> mat, rhs;
-    int64_t ke = 0;
-    std::vector<std::vector<int64_t>> kebase(d.n_buckets);
-    for (int b = 0; b < d.n_buckets; b++) {
-        const slx_bucket_desc& bd = d.buckets[b];
-        kebase[b].resize(bd.n_slices);
-        for (int s = 0; s < bd.n_slices; s++) {
-            const slx_slice_desc& sd = bd.slices[s];
-            kebase[b][s] = ke;
-            ke += (int64_t)bd.n_elem * sd.n_tf * (sd.is_rhs ? 1 : sd.n_dof);
-            (sd.is_rhs ? rhs : mat).push_back({b, s, kebase[b][s]});
-        }
-    }
-    P.n_ke = ke;
-    auto rank = [&](const SliceRef& r) { return d.buckets[r.bucket].slices[r.slice].frag_rank; };
-    std::stable_sort(mat.begin(), mat.end(), [&](const SliceRef& a, const SliceRef& b) { return rank(a) < rank(b); });
-    std::stable_sort(rhs.begin(), rhs.end(), [&](const SliceRef& a, const SliceRef& b) { return rank(a) < rank(b); });
+    P.n_ke = L.n_ke;
+    // segment groups in fragment (accumulation) order
+    std::vector<int> mat, rhs;
+    for (size_t g = 0; g < L.groups.size(); g++) (L.groups[g].rhs ? rhs : mat).push_back((int)g);
+    auto grank = [&](int g) { return L.groups[g].rank; };
+    std::stable_sort(mat.begin(), mat.end(), [&](int a, int b) { return grank(a) < grank(b); });
+    std::stable_sort(rhs.begin(), rhs.end(), [&](int a, int b) { return grank(a) < grank(b); });
 
-    // element rows per free row: counting sort keeps (fragment, i, element) order
-    auto collect = [&](const std::vector<SliceRef>& refs, std::vector<int64_t>& ptr, std::vector<uint64_t>& erows,
+    // element-row segments per free row: counting sort keeps (group, i, element) order
+    auto collect = [&](const std::vector<int>& refs, std::vector<int64_t>& ptr, std::vector<uint64_t>& erows,
                        std::vector<int32_t>* ref_idx, std::vector<int32_t>* elem_idx) {
         ptr.assign(nf + 1, 0);
-        for (const SliceRef& r : refs) {
-            const slx_bucket_desc& bd = d.buckets[r.bucket];
-            const slx_slice_desc& sd = bd.slices[r.slice];
-            for (int i = 0; i < sd.n_tf; i++)
+        for (int g : refs) {
+            const SegGroup& G = L.groups[g];
+            const slx_bucket_desc& bd = d.buckets[G.bucket];
+            const slx_slice_desc& sd = bd.slices[G.slices[0]];
+            for (int i = 0; i < G.n_tf; i++)
                 for (int e = 0; e < bd.n_elem; e++) {
                     int32_t a = sd.tf_addr[(size_t)i * bd.n_elem + e];
                     if (a < 0 || (d.constrained && d.constrained[a])) continue;
@@ -197,22 +281,22 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
         if (elem_idx) elem_idx->resize(ptr[nf]);
         std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
         for (size_t ri = 0; ri < refs.size(); ri++) {
-            const SliceRef& r = refs[ri];
-            const slx_bucket_desc& bd = d.buckets[r.bucket];
-            const slx_slice_desc& sd = bd.slices[r.slice];
-            int64_t len = sd.is_rhs ? 1 : sd.n_dof;
-            for (int i = 0; i < sd.n_tf; i++)
+            const SegGroup& G = L.groups[refs[ri]];
+            const slx_bucket_desc& bd = d.buckets[G.bucket];
+            const slx_slice_desc& sd = bd.slices[G.slices[0]];
+            for (int i = 0; i < G.n_tf; i++)
                 for (int e = 0; e < bd.n_elem; e++) {
                     int32_t a = sd.tf_addr[(size_t)i * bd.n_elem + e];
                     if (a < 0 || (d.constrained && d.constrained[a])) continue;
                     int64_t pos = fill[freeidx[a]]++;
-                    int64_t off = r.ke_base + ((int64_t)e * sd.n_tf + i) * len;
-                    erows[pos] = ((uint64_t)len << 48) | (uint64_t)off;
+                    int64_t off = G.base + ((int64_t)e * G.n_tf + i) * G.len;
+                    erows[pos] = ((uint64_t)G.len << 48) | (uint64_t)off;
                     if (ref_idx) (*ref_idx)[pos] = (int32_t)ri;
                     if (elem_idx) (*elem_idx)[pos] = e;
                 }
         }
     };
+    SLX_CHECK(L.n_ke < (int64_t(1) << 48), "element matrices exceed the 48-bit offset range");
     std::vector<int64_t> erow_ptr, rhs_ptr;
     std::vector<uint64_t> erows, rhs_erows;
     std::vector<int32_t> eref, eelem;
@@ -229,12 +313,23 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
     P.n_coo = slot_ptr[nf];
     std::vector<int32_t> slots(slot_ptr[nf]);
     std::vector<int32_t> lenA(nf), lenD(nf);
-    std::vector<std::vector<int32_t>> colsA_chunks, colsD_chunks;
     int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (nf < 20000) nthreads = 1;
     std::vector<std::vector<int32_t>> tA(nthreads), tD(nthreads);
     std::vector<int> tmax(nthreads, 1), tmaxrow(nthreads, 0);
     std::atomic<bool> dup_error{false};
+    // the columns of segment k of a row: dof addresses of every slice of its group
+    auto for_cols = [&](int64_t k, auto&& fn) {
+        const SegGroup& G = L.groups[mat[eref[k]]];
+        const slx_bucket_desc& bd = d.buckets[G.bucket];
+        int e = eelem[k];
+        int pos = 0;
+        for (int s : G.slices) {
+            const slx_slice_desc& sd = bd.slices[s];
+            for (int j = 0; j < sd.n_dof; j++) fn(pos++, sd.dof_addr[(size_t)j * bd.n_elem + e]);
+        }
+        return (int)G.len;
+    };
     auto work = [&](int t) {
         int64_t r0 = nf * t / nthreads, r1 = nf * (t + 1) / nthreads;
         std::vector<int64_t> cols;
@@ -242,15 +337,10 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
         for (int64_t r = r0; r < r1; r++) {
             cols.clear();
             for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
-                const SliceRef& ref = mat[eref[k]];
-                const slx_bucket_desc& bd = d.buckets[ref.bucket];
-                const slx_slice_desc& sd = bd.slices[ref.slice];
-                int e = eelem[k];
-                tmax[t] = std::max(tmax[t], sd.n_dof);
-                for (int j = 0; j < sd.n_dof; j++) {
-                    int32_t c = sd.dof_addr[(size_t)j * bd.n_elem + e];
+                int len = for_cols(k, [&](int, int32_t c) {
                     if (c >= 0) cols.push_back(c);
-                }
+                });
+                tmax[t] = std::max(tmax[t], len);
             }
             std::sort(cols.begin(), cols.end());
             cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
@@ -275,21 +365,16 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
             }
             int64_t sp = slot_ptr[r];
             for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
-                const SliceRef& ref = mat[eref[k]];
-                const slx_bucket_desc& bd = d.buckets[ref.bucket];
-                const slx_slice_desc& sd = bd.slices[ref.slice];
-                int e = eelem[k];
-                for (int j = 0; j < sd.n_dof; j++) {
-                    int32_t c = sd.dof_addr[(size_t)j * bd.n_elem + e];
+                int len = for_cols(k, [&](int j, int32_t c) {
                     int32_t slot = -1;
                     if (c >= 0) slot = rankv[std::lower_bound(cols.begin(), cols.end(), (int64_t)c) - cols.begin()];
                     slots[sp + j] = slot;
-                }
-                // slots of one element row must be distinct for the lockstep reduction
-                for (int j = 0; j < sd.n_dof; j++)
-                    for (int j2 = j + 1; j2 < sd.n_dof; j2++)
+                });
+                // slots of one segment must be distinct for the lockstep reduction
+                for (int j = 0; j < len; j++)
+                    for (int j2 = j + 1; j2 < len; j2++)
                         if (slots[sp + j] >= 0 && slots[sp + j] == slots[sp + j2]) dup_error = true;
-                sp += sd.n_dof;
+                sp += len;
             }
         }
     };
@@ -318,7 +403,20 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
         P.max_erow_len = std::max(P.max_erow_len, tmax[t]);
         maxrow = std::max(maxrow, tmaxrow[t]);
     }
-    if (P.host_only) return;
+    for (int b = 0; b < d.n_buckets; b++)
+        for (int s = 0; s < d.buckets[b].n_slices; s++) {
+            P.h_ke_base.push_back(L.ke_base[b][s]);
+            P.h_ke_ld.push_back(L.ke_ld[b][s]);
+        }
+    if (P.host_only) {
+        P.h_erow_ptr = std::move(erow_ptr);
+        P.h_erows = std::move(erows);
+        P.h_slot_ptr = std::move(slot_ptr);
+        P.h_slots = std::move(slots);
+        P.h_rhs_ptr = std::move(rhs_ptr);
+        P.h_rhs_erows = std::move(rhs_erows);
+        return;
+    }
     // device copies
     hipStream_t st = P.stream;
     upload(P.d_Arow, P.Arow.data(), P.Arow.size(), st);
@@ -328,19 +426,39 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d) {
     upload(P.d_erow_ptr, erow_ptr.data(), erow_ptr.size(), st);
     upload(P.d_erows, erows.data(), erows.size(), st);
     upload(P.d_slot_ptr, slot_ptr.data(), slot_ptr.size(), st);
-    upload(P.d_slots, slots.data(), slots.size(), st);
+    // slots in the narrowest type that holds every row length (all ones = skip)
+    int slot_bytes = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
+    if (slot_bytes == 1) {
+        std::vector<uint8_t> v(slots.size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = slots[i] < 0 ? 0xFF : (uint8_t)slots[i];
+        upload(P.d_slots, v.data(), v.size(), st);
+    } else if (slot_bytes == 2) {
+        std::vector<uint16_t> v(slots.size());
+        for (size_t i = 0; i < v.size(); i++) v[i] = slots[i] < 0 ? 0xFFFF : (uint16_t)slots[i];
+        upload(P.d_slots, v.data(), v.size(), st);
+    } else {
+        upload(P.d_slots, slots.data(), slots.size(), st);
+    }
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
     P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
     P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
-    P.d_rhs.alloc(std::max<int64_t>(nd, 1) * sizeof(double));
-    HIPCHECK(hipMemsetAsync(P.d_rhs.p, 0, std::max<int64_t>(nd, 1) * sizeof(double), st));
+    // rhs: constrained entries hold the Dirichlet values (formulation::b(),
+    // vec.cpp:43-61); the free entries are rewritten by every assembly
+    {
+        std::vector<double> b0(std::max<int64_t>(nd, 1), 0.0);
+        if (d.dirichlet && d.constrained)
+            for (int64_t i = 0; i < nd; i++)
+                if (d.constrained[i]) b0[i] = d.dirichlet[i];
+        upload(P.d_rhs, b0.data(), b0.size(), st);
+    }
     P.rd.nrows = nf;
     P.rd.erow_ptr = P.d_erow_ptr.as<int64_t>();
     P.rd.erows = P.d_erows.as<uint64_t>();
     P.rd.slot_ptr = P.d_slot_ptr.as<int64_t>();
-    P.rd.slots = P.d_slots.as<int32_t>();
+    P.rd.slots = P.d_slots.p;
+    P.rd.slot_bytes = slot_bytes;
     P.rd.Arow = P.d_Arow.as<int64_t>();
     P.rd.Drow = P.d_Drow.as<int64_t>();
     P.rd.max_row_len = maxrow;
@@ -393,7 +511,7 @@ float f16_to_f32(uint16_t h) {
 
 // Builds the MFMA description of a bucket's matrix slices (all sharing one
 // Phi table) or leaves M = 0 when the bucket is not eligible.
-void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const std::vector<int64_t>& kebase) {
+void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
     MfmaBucket mb;
     std::memset(&mb, 0, sizeof(mb));
     uint32_t mask = 0;
@@ -418,33 +536,45 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const std::vect
     }
     if (ms.empty() || (int)ms.size() > SLX_MAX_SLICES) return;
     mb.M = n_tf * n_dof;
+    mb.n_tf = n_tf;
+    mb.n_dof = n_dof;
     mb.n_pairs = (int)pairs.size();
     mb.K = mb.n_pairs * bd.n_gp;
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
     mb.nq = bd.q_invariant ? 1 : bd.n_gp;
     mb.n_mslices = (int)ms.size();
-    if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.mtw, &mb.maxk)) return;
-    // element tile: the largest of 64 / 32 / 16 with <= 48 KB of LDS
+    if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk)) return;
+    // LDS column layout (words): compact when constant over Gauss points and
+    // eight consecutive k touch at most two derivative pairs
+    mb.compact = (mb.nq == 1 && bd.n_gp >= 8) ? 1 : 0;
+    mb.cstride = mb.compact ? mb.n_pairs + 1 : 32 * mb.ksteps;
+    int es = mb.n_mslices * mb.cstride;
+    if (mb.compact) es |= 1;  // odd: conflict-free 32-bit reads across elements
+    else es = ((es + 59) / 64) * 64 + 4;  // = 4 (mod 64): 16-byte reads spread over the banks
+    mb.estride = es;
     mb.ne_tile = 0;
     for (int ne : {64, 32, 16}) {
         mb.ne_tile = ne;
-        if (mfma_lds_bytes(mb, bd.n_coefs) <= 48 * 1024) break;
+        if (mfma_lds_bytes(mb) <= 64 * 1024) break;
         mb.ne_tile = 0;
     }
     if (mb.ne_tile == 0) return;
-    // coefficient routing: slot -> (s * n_pairs + p) * nq
-    std::vector<int32_t> route(std::max(bd.n_coefs, 1), -1);
+    // coefficient slot of every (slice, pair)
+    std::vector<int32_t> csrc((size_t)mb.n_mslices * mb.n_pairs, -1);
+    std::vector<int64_t> kb(mb.n_mslices), kl(mb.n_mslices);
     for (size_t si = 0; si < ms.size(); si++) {
         const slx_slice_desc& sd = bd.slices[ms[si]];
         mb.slice_idx[si] = ms[si];
-        mb.ke_base[si] = kebase[ms[si]];
+        kb[si] = L.ke_base[b][ms[si]];
+        kl[si] = L.ke_ld[b][ms[si]];
         mask |= 1u << ms[si];
         for (int t = 0; t < sd.n_terms; t++) {
             std::pair<int, int> pr(sd.terms[t].tf_deriv, sd.terms[t].dof_deriv);
             int p = (int)(std::find(pairs.begin(), pairs.end(), pr) - pairs.begin());
-            SLX_CHECK(route[sd.terms[t].coef] < 0, "coefficient slot shared by two matrix terms");
-            route[sd.terms[t].coef] = ((int32_t)si * mb.n_pairs + p) * mb.nq;
+            // two terms of one slice with the same derivative pair: not a single coefficient
+            if (csrc[si * mb.n_pairs + p] >= 0) return;
+            csrc[si * mb.n_pairs + p] = sd.terms[t].coef;
         }
     }
     // Phi[m = i*n_dof + j][k = p*n_gp + q] = tf_r[i][q] w_q dof_s[j][q]
@@ -481,13 +611,29 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const std::vect
                     frag[base + 64 * 8] = lo;
                 }
     mb.afrag = P.dev(frag.data(), frag.size());
-    mb.route = P.dev(route.data(), route.size());
+    mb.csrc = P.dev(csrc.data(), csrc.size());
+    mb.kebase = P.dev(kb.data(), kb.size());
+    mb.keld = P.dev(kl.data(), kl.size());
     mb.max_blocks = P.n_cu * 2;
     P.mfma[b] = mb;
     P.mfma_mask[b] = mask;
 }
 
-void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
+// Gauss-point-major copy of a shape table, rows padded with zeros so that a
+// tile of up to 16 functions past the end stays inside the allocation:
+// out[(q * 4 + d) * pad + f] = tab[(d * nff + f) * ngp + q].
+int table_pad(int nff) { return ((nff + 7) / 8) * 8 + 16; }
+
+std::vector<double> table_qmajor(const double* tab, int nff, int ngp) {
+    const int pad = table_pad(nff);
+    std::vector<double> out((size_t)ngp * 4 * pad, 0.0);
+    for (int q = 0; q < ngp; q++)
+        for (int d = 0; d < 4; d++)
+            for (int f = 0; f < nff; f++) out[((size_t)q * 4 + d) * pad + f] = tab[((size_t)d * nff + f) * ngp + q];
+    return out;
+}
+
+void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.dbuckets.resize(d.n_buckets);
     P.mfma.assign(d.n_buckets, MfmaBucket());
     for (auto& m : P.mfma) m.M = 0;
@@ -496,8 +642,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, P.ctx->device) == hipSuccess && prop.multiProcessorCount > 0) P.n_cu = prop.multiProcessorCount;
     }
-    std::vector<std::vector<int64_t>> kebase(d.n_buckets);
-    int64_t ke = 0;
+    size_t ctab_words = 1;
     for (int b = 0; b < d.n_buckets; b++) {
         const slx_bucket_desc& bd = d.buckets[b];
         DevBucket& db = P.dbuckets[b];
@@ -532,7 +677,11 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
         db.geo_tab = P.dev(geo.data(), geo.size());
         db.prog = P.dev(bd.prog, (size_t)bd.n_instr);
         std::vector<const double*> dtabs(bd.n_tables);
-        for (int t = 0; t < bd.n_tables; t++) dtabs[t] = P.dev(bd.tables[t], (size_t)4 * bd.table_nff[t] * bd.n_gp);
+        for (int t = 0; t < bd.n_tables; t++) {
+            std::vector<double> tq = table_qmajor(bd.tables[t], bd.table_nff[t], bd.n_gp);
+            dtabs[t] = P.dev(tq.data(), tq.size());
+        }
+        ctab_words = std::max(ctab_words, (size_t)std::max(bd.n_coefs, 1) * (bd.q_invariant ? 1 : bd.n_gp) * std::max(bd.n_elem, 1));
         for (int s = 0; s < bd.n_slices; s++) {
             const slx_slice_desc& sd = bd.slices[s];
             DevSlice& ds = db.slices[s];
@@ -545,17 +694,19 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
                       "slice tf table does not match n_tf");
             SLX_CHECK(sd.is_rhs || (sd.dof_table >= 0 && sd.dof_table < bd.n_tables && bd.table_nff[sd.dof_table] == sd.n_dof),
                       "slice dof table does not match n_dof");
-            ds.tf_tab = dtabs[sd.tf_table];
-            ds.dof_tab = sd.is_rhs ? nullptr : dtabs[sd.dof_table];
-            ds.ke_base = ke;
-            kebase[b].push_back(ke);
-            ke += (int64_t)bd.n_elem * ds.n_tf * ds.n_dof;
+            ds.tf_tabq = dtabs[sd.tf_table];
+            ds.dof_tabq = sd.is_rhs ? nullptr : dtabs[sd.dof_table];
+            ds.tf_pad = table_pad(sd.n_tf);
+            ds.dof_pad = sd.is_rhs ? 0 : table_pad(sd.n_dof);
+            ds.ke_base = L.ke_base[b][s];
+            ds.ke_ld = L.ke_ld[b][s];
             for (int t = 0; t < sd.n_terms; t++) {
                 SLX_CHECK(sd.terms[t].coef >= 0 && sd.terms[t].coef < bd.n_coefs, "term coefficient slot out of range");
                 SLX_CHECK(sd.terms[t].tf_deriv >= 0 && sd.terms[t].tf_deriv <= 3 && sd.terms[t].dof_deriv >= 0 && sd.terms[t].dof_deriv <= 3,
                           "term derivative out of range");
             }
         }
+        db.dslices = P.dev(db.slices, (size_t)bd.n_slices);
         for (int f = 0; f < d.n_fields; f++) {
             size_t k = (size_t)f * d.n_buckets + b;
             db.fields[f].nff = d.field_nff[k];
@@ -582,13 +733,49 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d) {
         // element node numbers must be in range of the node array
         for (size_t k = 0; k < (size_t)bd.n_elem * bd.n_geo; k++)
             SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
-        build_mfma(P, bd, b, kebase[b]);
+        build_mfma(P, bd, b, L);
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
+    P.d_ctab.alloc(ctab_words * sizeof(double));
 }
 
+// Per-launch HIP events by kernel class (profiling only).
+enum { T_COEF = 0, T_MFMA = 1, T_VALU = 2, T_REDUCE = 3, T_COUNT = 4 };
+struct LaunchTimer {
+    hipStream_t st;
+    std::vector<hipEvent_t> pool;
+    std::vector<std::pair<int, size_t>> marks;  // (class, index of the start event)
+    size_t used = 0;
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+    void begin(int cls) {
+        marks.push_back({cls, used});
+        HIPCHECK(hipEventRecord(next(), st));
+    }
+    void end() { HIPCHECK(hipEventRecord(next(), st)); }
+    void collect(double* ms) {
+        HIPCHECK(hipEventSynchronize(pool[used - 1]));
+        for (auto& m : marks) {
+            float t = 0;
+            HIPCHECK(hipEventElapsedTime(&t, pool[m.second], pool[m.second + 1]));
+            ms[m.first] += t;
+        }
+        marks.clear();
+        used = 0;
+    }
+    ~LaunchTimer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
 void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t plen, const double* const* fields, int precision,
-             hipStream_t st, hipEvent_t mid = nullptr) {
+             hipStream_t st, LaunchTimer* tm = nullptr) {
     SLX_CHECK(precision >= 0 && precision <= 3, "unknown precision " << precision);
     if (P.max_param >= 0) SLX_CHECK(params && plen >= P.max_elem_id, "per-element parameters missing or shorter than the element count");
     if (P.uses_fields) {
@@ -606,16 +793,26 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         const DevBucket& b = P.dbuckets[bi];
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
+        if (tm) tm->begin(T_COEF);
+        HIPCHECK(launch_coef(b, in, ke64, P.d_ctab.p, st));
+        if (tm) tm->end();
         if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
-            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], in, P.d_ke.as<float>(), st));
+            if (tm) tm->begin(T_MFMA);
+            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], P.d_ctab.as<float>(), P.d_ke.as<float>(), st));
+            if (tm) tm->end();
             done = P.mfma_mask[bi];
         }
         int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
-        HIPCHECK(launch_elem(b, in, p, P.d_ke.p, st, all & ~done));
+        if ((all & ~done) != 0) {
+            if (tm) tm->begin(T_VALU);
+            HIPCHECK(launch_elem(b, P.d_ctab.p, p, P.d_ke.p, st, all & ~done));
+            if (tm) tm->end();
+        }
     }
-    if (mid) HIPCHECK(hipEventRecord(mid, st));
+    if (tm) tm->begin(T_REDUCE);
     HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
                            P.has_matrix, P.has_rhs, st));
+    if (tm) tm->end();
     P.last_precision = precision;
 }
 
@@ -674,8 +871,9 @@ slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out
                 for (size_t k = 0; k < ndf; k++) SLX_CHECK(sd.dof_addr[k] < desc->n_dofs, "dof address out of range");
             }
         }
-        if (!P->host_only) upload_buckets(*P, *desc);
-        build_symbolic(*P, *desc);
+        const KeLayout L = compute_layout(*desc);
+        if (!P->host_only) upload_buckets(*P, *desc, L);
+        build_symbolic(*P, *desc, L);
         if (!P->host_only) HIPCHECK(hipStreamSynchronize(P->stream));
         *out = P.release();
     });
@@ -739,27 +937,14 @@ slx_status slx_plan_profile(slx_plan p, const double* d_node_xyz, const double* 
     return guarded([&] {
         SLX_CHECK(p && d_node_xyz && ms && iters > 0, "bad argument");
         SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
-        hipEvent_t e0, e1, e2;
-        HIPCHECK(hipEventCreate(&e0));
-        HIPCHECK(hipEventCreate(&e1));
-        HIPCHECK(hipEventCreate(&e2));
-        double t_elem = 0, t_red = 0;
+        LaunchTimer tm;
+        tm.st = p->stream;
+        double acc[T_COUNT] = {0, 0, 0, 0};
         for (int it = 0; it < iters; it++) {
-            HIPCHECK(hipEventRecord(e0, p->stream));
-            numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, p->stream, e1);
-            HIPCHECK(hipEventRecord(e2, p->stream));
-            HIPCHECK(hipEventSynchronize(e2));
-            float a = 0, b = 0;
-            HIPCHECK(hipEventElapsedTime(&a, e0, e1));
-            HIPCHECK(hipEventElapsedTime(&b, e1, e2));
-            t_elem += a;
-            t_red += b;
+            numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, p->stream, &tm);
+            tm.collect(acc);
         }
-        ms[0] = t_elem / iters;
-        ms[1] = t_red / iters;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        (void)hipEventDestroy(e2);
+        for (int k = 0; k < T_COUNT; k++) ms[k] = acc[k] / iters;
     });
 }
 
@@ -809,6 +994,68 @@ slx_status slx_plan_kernels(slx_plan p, int32_t* n_mfma, int32_t* n_buckets) {
         for (auto& m : p->mfma) n += m.M > 0 ? 1 : 0;
         if (n_mfma) *n_mfma = n;
         if (n_buckets) *n_buckets = (int32_t)p->dbuckets.size();
+    });
+}
+
+slx_status slx_plan_reduction_maps(slx_plan p, int64_t* sizes, int64_t* erow_ptr, uint64_t* erows, int64_t* slot_ptr,
+                                   int32_t* slots, int64_t* rhs_ptr, uint64_t* rhs_erows, int64_t* ke_base, int64_t* ke_ld) {
+    return guarded([&] {
+        SLX_CHECK(p && sizes, "null argument");
+        SLX_CHECK(p->host_only || (!erow_ptr && !erows && !slot_ptr && !slots && !rhs_ptr && !rhs_erows),
+                  "reduction maps are kept only by pattern-only plans (ctx == NULL); the layout is always available");
+        sizes[0] = (int64_t)p->h_erows.size();
+        sizes[1] = (int64_t)p->h_slots.size();
+        sizes[2] = (int64_t)p->h_rhs_erows.size();
+        sizes[3] = (int64_t)p->h_ke_base.size();
+        auto cp = [](auto* dst, const auto& v) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        cp(erow_ptr, p->h_erow_ptr);
+        cp(erows, p->h_erows);
+        cp(slot_ptr, p->h_slot_ptr);
+        cp(slots, p->h_slots);
+        cp(rhs_ptr, p->h_rhs_ptr);
+        cp(rhs_erows, p->h_rhs_erows);
+        cp(ke_base, p->h_ke_base);
+        cp(ke_ld, p->h_ke_ld);
+    });
+}
+
+slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* n_out) {
+    return guarded([&] {
+        SLX_CHECK(p && n_out, "null argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
+        SLX_CHECK(p->last_precision >= 0 && !p->dbuckets.empty(), "no assembly has run on this plan");
+        const DevBucket& b = p->dbuckets.back();
+        const int64_t n = (int64_t)b.n_coefs * (b.q_inv ? 1 : b.n_gp) * b.n_elem;
+        *n_out = n;
+        if (!out) return;
+        SLX_CHECK(cap >= n, "output holds " << cap << " entries, the table has " << n);
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (p->last_precision == SLX_PREC_FP64) {
+            HIPCHECK(hipMemcpy(out, p->d_ctab.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(n);
+            HIPCHECK(hipMemcpy(tmp.data(), p->d_ctab.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
+        }
+    });
+}
+
+slx_status slx_plan_element_matrices(slx_plan p, double* out, int64_t cap) {
+    return guarded([&] {
+        SLX_CHECK(p && out, "null argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
+        SLX_CHECK(p->last_precision >= 0, "no assembly has run on this plan");
+        SLX_CHECK(cap >= p->n_ke, "output holds " << cap << " entries, the element matrices have " << p->n_ke);
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (p->last_precision == SLX_PREC_FP64) {
+            HIPCHECK(hipMemcpy(out, p->d_ke.p, (size_t)p->n_ke * sizeof(double), hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp(p->n_ke);
+            HIPCHECK(hipMemcpy(tmp.data(), p->d_ke.p, (size_t)p->n_ke * sizeof(float), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < p->n_ke; i++) out[i] = tmp[i];
+        }
     });
 }
 

@@ -18,6 +18,7 @@ import scipy.sparse.linalg as spl
 import sparselizard_amd as sa
 from sparselizard_amd import _lib as L
 from oracle import py_oracle
+from _replay import oracle_ke_layout, reduce_host
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 TRI, QUAD, TET, HEX, LINE = 2, 3, 4, 5, 1
@@ -411,3 +412,34 @@ def test_host_symbolic_pattern_equals_oracle(kind, spec):
                                     L._p(Drow, C.c_int64), L._p(Dcol, C.c_int32), None, None))
     assert np.array_equal(Arow, o.Arow) and np.array_equal(Drow, o.Drow)
     assert np.array_equal(Acol[:s["nnzA"]], o.Acol) and np.array_equal(Dcol[:s["nnzD"]], o.Dcol)
+
+
+# ------------------------------------------------- reduction replay (CPU) ---
+
+def replay_reduction(prob):
+    """Oracle element matrices through the plan's reduction maps (numpy)."""
+    info = prob.info()
+    plan = prob.plan()
+    maps = plan.reduction_maps()
+    o, ke = oracle_ke_layout(prob, maps["ke_base"], maps["ke_ld"], plan.stats()["n_ke"])
+    Aval, Dval, rhs_free = reduce_host(maps, ke, o.Arow, o.Drow, plan.shape())
+    return o, Aval, Dval, rhs_free
+
+
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=6"), ("elasticity3d", "n=2"), ("elasticity3d", "n=2;order=1"),
+                                       ("hppoisson3d", "n=3"), ("quadbatch", "instances=3;nmin=2;nmax=4")])
+def test_reduction_maps_replay_oracle(kind, spec):
+    """Device Ke layout + segment/slot maps reproduce the oracle's CSR values."""
+    p = sa.Problem(None, kind, spec)
+    o, Aval, Dval, rhs_free = replay_reduction(p)
+
+    def rel(a, b):
+        return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+    assert rel(Aval, o.Aval) < 1e-13
+    if o.Dval.size:
+        assert rel(Dval, o.Dval) < 1e-13
+    d = p.desc()
+    n = p.info()["ndofs"]
+    cons = np.ctypeslib.as_array(d.constrained, shape=(n,)).astype(bool) if d.constrained else np.zeros(n, bool)
+    if np.linalg.norm(o.rhs[~cons]) > 0:
+        assert rel(rhs_free, o.rhs[~cons]) < 1e-13

@@ -1,0 +1,137 @@
+"""Reference golden values: the validation lines of two reference examples,
+run on the meshes those examples ship (copied as data into tests/golden/).
+
+  examples/electrostatics-1d/main.cpp:44
+      21.5962 < ||x||_2 < 21.5964 for the solution of
+      integral(line, -eps grad(dof(v)) grad(tf(v))), v = 10 / 2 on the ends.
+  examples/high-order-interpolations-2d/main.cpp:68
+      relative L2 error < 1e-5 of the order-5 projection of sin(10x) sin(13y)
+      on the 81-quad mesh (integration order +20).
+
+The solve x_A = A^-1 (b_A - D x_D) (mat::eliminate + sl::solve,
+src/formulation/mat.cpp:78-103, src/expression/operation/sl.cpp:1565-1627)
+is done here with scipy as a test tool.  The projection error uses
+||v - f||^2 = int f^2 - b^T v  (M v = b), with int f^2 in closed form.
+
+CPU tests run the oracle (pinning it against the reference's own goldens);
+the GPU tests run the HIP path on the same problems.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spl
+
+import sparselizard_amd as sa
+from oracle import py_oracle
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+LINE_MSH = os.path.join(GOLDEN, "electrostatics1d_line.msh")
+QUAD_MSH = os.path.join(GOLDEN, "hointerp2d_quad.msh")
+
+
+def solve_full(csr, ndofs, nfree):
+    """x with x[free] = A^-1 (b[free] - D b[constrained]) and x[constrained] = b[constrained]."""
+    nc = ndofs - nfree
+    A = sp.csr_matrix((csr.Aval, csr.Acol, csr.Arow), shape=(nfree, nfree))
+    D = sp.csr_matrix((csr.Dval, csr.Dcol, csr.Drow), shape=(nfree, nc))
+    # free DOFs are numbered first in A, constrained ones in D, both ascending;
+    # the constrained rows of b hold the Dirichlet values
+    rhs = np.asarray(csr.rhs)
+    cons = np.ones(ndofs, bool)
+    cons[free_dofs(csr, ndofs, nfree)] = False
+    bA = rhs[~cons]
+    xD = rhs[cons]
+    xA = spl.spsolve(A.tocsc(), bA - D @ xD) if nfree else np.zeros(0)
+    x = np.empty(ndofs)
+    x[~cons] = xA
+    x[cons] = xD
+    return x, A, bA
+
+
+def free_dofs(csr, ndofs, nfree):
+    # the plan description marks the constrained DOFs; it is carried by the
+    # problem, passed in through the closure below
+    return _FREE[(ndofs, nfree)]
+
+
+_FREE = {}
+
+
+def register_free(prob):
+    d = prob.desc()
+    info = prob.info()
+    n = info["ndofs"]
+    mask = np.zeros(n, bool)
+    if d.constrained:
+        mask = np.ctypeslib.as_array(d.constrained, shape=(n,)).astype(bool)
+    _FREE[(n, info["nfree"])] = np.nonzero(~mask)[0]
+    return info
+
+
+def projection_error(csr, nfree):
+    M = sp.csr_matrix((csr.Aval, csr.Acol, csr.Arow), shape=(nfree, nfree))
+    b = np.asarray(csr.rhs)
+    v = spl.spsolve(M.tocsc(), b)
+    f2 = (0.5 - math.sin(20.0) / 40.0) * (0.5 - math.sin(26.0) / 52.0)
+    err2 = f2 - float(b @ v)
+    return math.sqrt(max(err2, 0.0) / f2)
+
+
+# ------------------------------------------------------------------- CPU ----
+
+def test_oracle_electrostatics_1d_golden():
+    p = sa.Problem(None, "electrostatics1d", "file=" + LINE_MSH)
+    info = register_free(p)
+    o = py_oracle.assemble(p.desc_ptr(), p.numeric(), info["ndofs"], info["nfree"])
+    x, _, _ = solve_full(o, info["ndofs"], info["nfree"])
+    nrm = float(np.linalg.norm(x))
+    assert info["ndofs"] == 11 and info["nfree"] == 9
+    assert 21.5962 < nrm < 21.5964, nrm
+
+
+def test_oracle_high_order_projection_2d_golden():
+    p = sa.Problem(None, "hoprojection2d", "file=%s;order=5" % QUAD_MSH)
+    info = register_free(p)
+    assert info["nelem"] == 81
+    o = py_oracle.assemble(p.desc_ptr(), p.numeric(), info["ndofs"], info["nfree"])
+    err = projection_error(o, info["nfree"])
+    assert err < 1e-5, err
+
+
+def test_oracle_projection_error_decreases_with_order():
+    # the reference example's own claim: the error falls with the order
+    errs = []
+    for order in (2, 3, 4):
+        p = sa.Problem(None, "hoprojection2d", "file=%s;order=%d" % (QUAD_MSH, order))
+        info = register_free(p)
+        o = py_oracle.assemble(p.desc_ptr(), p.numeric(), info["ndofs"], info["nfree"])
+        errs.append(projection_error(o, info["nfree"]))
+    assert errs[0] > errs[1] > errs[2], errs
+
+
+# ------------------------------------------------------------------- GPU ----
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "f16split", "fp32"])
+def test_gpu_electrostatics_1d_golden(ctx, precision):
+    print("case electrostatics1d", precision, flush=True)
+    p = sa.Problem(ctx, "electrostatics1d", "file=" + LINE_MSH)
+    info = register_free(p)
+    plan = p.generate(precision)
+    x, _, _ = solve_full(plan.csr(info["ndofs"]), info["ndofs"], info["nfree"])
+    nrm = float(np.linalg.norm(x))
+    assert 21.5962 < nrm < 21.5964, nrm
+
+
+@pytest.mark.gpu
+def test_gpu_high_order_projection_2d_golden(ctx):
+    # FP64 path: the energy identity needs the matrix to ~1e-12
+    print("case hoprojection2d fp64", flush=True)
+    p = sa.Problem(ctx, "hoprojection2d", "file=%s;order=5" % QUAD_MSH)
+    info = register_free(p)
+    plan = p.generate("fp64")
+    err = projection_error(plan.csr(info["ndofs"]), info["nfree"])
+    assert err < 1e-5, err

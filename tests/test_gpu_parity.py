@@ -9,6 +9,7 @@ import pytest
 
 import sparselizard_amd as sa
 from oracle import py_oracle
+from _replay import oracle_ke_layout, reduce_host
 
 pytestmark = pytest.mark.gpu
 
@@ -82,6 +83,19 @@ def test_f16split_mfma_matches_oracle(ctx, kind, spec):
     print("errors", errs)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32)])
+def test_hp_hex_long_segments(ctx, precision, tol):
+    # per-element orders 1..5 on hexahedra: element rows up to 216 entries
+    # (strided reduction) and rows longer than 255 columns (16-bit slots)
+    run_case(ctx, "hppoisson3d", "n=3", precision, tol)
+
+
+@pytest.mark.parametrize("elem", ["tri", "quad", "tet", "hex"])
+def test_projection_fp32_qvarying(ctx, elem):
+    # non-affine quads/hexes vary over Gauss points; FP32 VALU element kernel
+    run_case(ctx, "projection", "elem=%s;n=3;order=2" % elem, "fp32", TOL_FP32)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "f16split"])
 def test_deterministic_repeat(ctx, precision):
     prob = sa.Problem(ctx, "elasticity3d", "n=3")
@@ -89,3 +103,60 @@ def test_deterministic_repeat(ctx, precision):
     a = prob.generate(precision).csr(info["ndofs"])
     b = prob.generate(precision).csr(info["ndofs"])
     assert np.array_equal(a.Aval, b.Aval) and np.array_equal(a.Dval, b.Dval) and np.array_equal(a.rhs, b.rhs)
+
+
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2")])
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("f16split", TOL_SPLIT), ("fp32", TOL_FP32)])
+def test_element_matrices_match_oracle(ctx, kind, spec, precision, tol):
+    """Element matrices (before the reduction) against the oracle, slice by
+    slice, and the device reduction against its host replay on the device's
+    own element matrices."""
+    print("case ke", kind, spec, precision, flush=True)
+    prob = sa.Problem(ctx, kind, spec)
+    info = prob.info()
+    plan = prob.generate(precision)
+    ke = plan.element_matrices()
+    kb, kl = plan.ke_layout()
+    o, ke_ref = oracle_ke_layout(prob, kb, kl, len(ke))
+    d = prob.desc()
+    worst, k = [], 0
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            ndf = 1 if sd.is_rhs else sd.n_dof
+            e, i, j = np.meshgrid(np.arange(bd.n_elem), np.arange(sd.n_tf), np.arange(ndf), indexing="ij")
+            idx = (kb[k] + (e * sd.n_tf + i) * kl[k] + j).ravel()
+            worst.append(relerr(ke[idx], ke_ref[idx]))
+            k += 1
+    print("per-slice element-matrix errors", ["%.2e" % w for w in worst])
+    # the device reduction applied to the device's element matrices
+    host = sa.Problem(None, kind, spec)
+    hplan = host.plan()
+    maps = hplan.reduction_maps()
+    assert np.array_equal(maps["ke_base"], kb) and np.array_equal(maps["ke_ld"], kl)
+    Aval, Dval, rhs_free = reduce_host(maps, ke, o.Arow, o.Drow, hplan.shape())
+    got = plan.csr(info["ndofs"])
+    red_err = relerr(got.Aval, Aval)
+    print("device reduction vs host replay on device Ke: %.2e" % red_err)
+    assert max(worst) <= tol, "slice errors %s" % worst
+    assert red_err <= 1e-13
+
+
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2"),
+                                       ("projection", "elem=hex;n=2;order=2")])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
+def test_coefficient_kernel_matches_oracle(ctx, kind, spec, precision, tol):
+    """Gauss-point coefficients (geometry + program) of the last bucket."""
+    print("case coef", kind, spec, precision, flush=True)
+    prob = sa.Problem(ctx, kind, spec)
+    plan = prob.generate(precision)
+    got = plan.coefficients()
+    d = prob.desc()
+    ref = py_oracle.coefficients(prob.desc_ptr(), prob.numeric(), d.n_buckets - 1, len(got))
+    bd = d.buckets[d.n_buckets - 1]
+    nq = 1 if bd.q_invariant else bd.n_gp
+    per_slot = [relerr(got[c * nq * bd.n_elem:(c + 1) * nq * bd.n_elem], ref[c * nq * bd.n_elem:(c + 1) * nq * bd.n_elem])
+                for c in range(bd.n_coefs)]
+    print("per-slot coefficient errors", ["%.1e" % x for x in per_slot])
+    assert max(per_slot) <= tol
