@@ -7,7 +7,7 @@
 //   |det J| (contribution.cpp:147, 200), into the table
 //   ctab[(slot * nq + q) * n_elem + e]  (element fastest: coalesced for kernels
 //   that map lanes to elements).
-// elem_tiled: lanes = elements; per slice, tile of tf rows x dof columns:
+// elem_cols: lanes = (element, dof column); per slice, TI tf rows at a time:
 //   Ke[i][j] = sum_q sum_t tf_t[i][q] w_q C_t(e,q) dof_t[j][q]
 //   (contribution.cpp:172-246; the per-term dgemm of densemat.cpp:228-311 with
 //   the terms summed as in addproduct), right-hand-side slices write -sum
@@ -186,18 +186,24 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
     run_program<T>(b, in, e, q, g, ctab + (size_t)q * b.n_elem + e, (int64_t)nq * b.n_elem);
 }
 
-// Element matrices, lanes = elements.  Shape tables are read through the
-// constant address space (scalar loads, Gauss point major); coefficients come
-// from the coalesced table.  Per slice and (TI x TJ) tile:
-//   D_rs   = sum of the slice's term coefficients with derivative pair (r, s)
-//   h_r[j] = w_q sum_s D_rs dof_s[j][q]
-//   Ke[i][j] += sum_r tf_r[i][q] h_r[j]
+// Element matrices, one lane per (element, dof column): `jc` consecutive
+// lanes share an element and own one column j each, so the rows written to
+// the element-row segments are contiguous runs.  Shape tables of the tf
+// functions are read through the constant address space (scalar loads, Gauss
+// point major); the dof table value of the lane's column is a vector load;
+// coefficients come from the coefficient table.  Per slice:
+//   D_rs    = sum of the slice's term coefficients with derivative pair (r, s)
+//   h_r     = w_q sum_s D_rs dof_s[j][q]
+//   Ke[i][j] += sum_r tf_r[i][q] h_r        (TI rows per pass)
 // which is the reference's sum over terms and Gauss points regrouped.
-template <typename CT, int TI, int TJ>
-__global__ void __launch_bounds__(256) elem_tiled(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, CT* __restrict__ ke) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = e < b.n_elem;
-    const int ec = live ? e : b.n_elem - 1;
+template <typename CT, int TI>
+__global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc,
+                                                 CT* __restrict__ ke) {
+    const int epb = blockDim.x / jc;
+    const int el = threadIdx.x / jc, jl = threadIdx.x - (threadIdx.x / jc) * jc;
+    const int e = blockIdx.x * epb + el;
+    const bool live = el < epb && e < b.n_elem;
+    const int ec = live ? e : 0;
     const int nq = b.q_inv ? 1 : b.n_gp;
     const int ngp = b.n_gp;
     const size_t cstride = (size_t)b.n_elem;
@@ -209,7 +215,7 @@ __global__ void __launch_bounds__(256) elem_tiled(DevBucket b, const CT* __restr
         const int tf_pad = dsl[s].tf_pad, dof_pad = dsl[s].dof_pad;
         const SLX_CONST slx_term* terms = (const SLX_CONST slx_term*)dsl[s].terms;
         const SLX_CONST double* tfq = (const SLX_CONST double*)dsl[s].tf_tabq;
-        const SLX_CONST double* dfq = (const SLX_CONST double*)dsl[s].dof_tabq;
+        const double* dfq = dsl[s].dof_tabq;
         const int64_t ke_base = dsl[s].ke_base, ke_ld = dsl[s].ke_ld;
         unsigned rmask = 0, dmask = 0;
         for (int t = 0; t < n_terms; t++) {
@@ -233,56 +239,44 @@ __global__ void __launch_bounds__(256) elem_tiled(DevBucket b, const CT* __restr
             }
         };
         if (nq == 1) load_D(0);
-        for (int i0 = 0; i0 < n_tf; i0 += TI) {
-            for (int j0 = 0; j0 < n_dof; j0 += TJ) {
-                CT acc[TI][TJ];
+        for (int j0 = 0; j0 < n_dof; j0 += jc) {
+            const int j = j0 + jl;
+            const bool jon = live && j < n_dof;
+            const int jr = jon ? j : 0;  // table column read by this lane (in range)
+            for (int i0 = 0; i0 < n_tf; i0 += TI) {
+                CT acc[TI];
 #pragma unroll
-                for (int ii = 0; ii < TI; ii++)
-#pragma unroll
-                    for (int jj = 0; jj < TJ; jj++) acc[ii][jj] = 0;
+                for (int ii = 0; ii < TI; ii++) acc[ii] = 0;
                 for (int q = 0; q < ngp; q++) {
                     if (nq != 1) load_D(q);
                     const CT w = (CT)gw[q];
+                    CT dv[4];
+#pragma unroll
+                    for (int d = 0; d < 4; d++)
+                        dv[d] = (!is_rhs && ((dmask >> d) & 1u)) ? (CT)dfq[((size_t)q * 4 + d) * dof_pad + jr] : (CT)(d == 0 ? 1 : 0);
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         if (!((rmask >> r) & 1u)) continue;
-                        CT h[TJ];
+                        CT h = 0;
                         if (is_rhs) {
-                            // right-hand side: the "dof" factor is 1
-                            CT sum = 0;
 #pragma unroll
-                            for (int d = 0; d < 4; d++) sum += D[r][d];
-#pragma unroll
-                            for (int jj = 0; jj < TJ; jj++) h[jj] = jj == 0 ? w * sum : (CT)0;
+                            for (int d = 0; d < 4; d++) h += D[r][d];
                         } else {
 #pragma unroll
-                            for (int jj = 0; jj < TJ; jj++) h[jj] = 0;
-#pragma unroll
-                            for (int d = 0; d < 4; d++) {
-                                if (!((dmask >> d) & 1u)) continue;
-                                const SLX_CONST double* dv = dfq + ((size_t)q * 4 + d) * dof_pad + j0;
-                                const CT wd = w * D[r][d];
-#pragma unroll
-                                for (int jj = 0; jj < TJ; jj++) h[jj] += wd * (CT)dv[jj];
-                            }
+                            for (int d = 0; d < 4; d++) h += D[r][d] * dv[d];
                         }
+                        h *= w;
                         const SLX_CONST double* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
 #pragma unroll
-                        for (int ii = 0; ii < TI; ii++) {
-                            const CT t = (CT)tv[ii];
-#pragma unroll
-                            for (int jj = 0; jj < TJ; jj++) acc[ii][jj] += t * h[jj];
-                        }
+                        for (int ii = 0; ii < TI; ii++) acc[ii] += (CT)tv[ii] * h;
                     }
                 }
-                if (live) {
-                    const int ni = min(TI, n_tf - i0), nj = min(TJ, n_dof - j0);
-                    CT* out = ke + ke_base + ((int64_t)e * n_tf + i0) * ke_ld + j0;
+                if (jon) {
+                    const int ni = min(TI, n_tf - i0);
+                    CT* out = ke + ke_base + ((int64_t)e * n_tf + i0) * ke_ld + j;
 #pragma unroll
                     for (int ii = 0; ii < TI; ii++)
-#pragma unroll
-                        for (int jj = 0; jj < TJ; jj++)
-                            if (ii < ni && jj < nj) out[(int64_t)ii * ke_ld + jj] = is_rhs ? -acc[ii][jj] : acc[ii][jj];
+                        if (ii < ni) out[(int64_t)ii * ke_ld] = is_rhs ? -acc[ii] : acc[ii];
                 }
             }
         }
@@ -290,11 +284,33 @@ __global__ void __launch_bounds__(256) elem_tiled(DevBucket b, const CT* __restr
 }
 
 // Deterministic reduction of the element-row segments into the CSR A/D
-// values.  A lane group (group >= longest segment) owns one row; lanes hold
-// consecutive entries of a segment, so within one segment every slot differs
-// and the LDS additions of a segment never collide.  Segments are consumed in
-// a fixed order (batches of four loaded ahead), so every value is summed in
-// the same order on every run.
+// values.  A lane group (group >= longest segment) owns one row at a time;
+// lanes hold consecutive entries of a segment, so within one segment every
+// slot differs and the LDS additions of a segment never collide.  Segments
+// are consumed in a fixed order, so every value is summed in the same order
+// on every run.  The groups are persistent and software-pipelined over their
+// rows: the first eight segments of the current row are loaded together, the
+// next row's segment descriptors and the metadata of the row after it are
+// already in flight while the current row accumulates.
+struct RowMeta {
+    int64_t a0, d0, k0, sp;
+    int lenA, lenD, nk;
+};
+
+__device__ inline RowMeta row_meta(const DevReduce& rd, int64_t r) {
+    RowMeta m;
+    m.a0 = rd.Arow[r];
+    m.d0 = rd.Drow[r];
+    m.k0 = rd.erow_ptr[r];
+    m.sp = rd.slot_ptr[r];
+    m.lenA = (int)(rd.Arow[r + 1] - m.a0);
+    m.lenD = (int)(rd.Drow[r + 1] - m.d0);
+    m.nk = (int)(rd.erow_ptr[r + 1] - m.k0);
+    return m;
+}
+
+#define SLX_RED_BATCH 8
+
 template <typename KT, typename ST>
 __global__ void __launch_bounds__(256) reduce_matrix(DevReduce rd, const KT* __restrict__ ke, int group, double* __restrict__ Aval,
                                                       double* __restrict__ Dval) {
@@ -302,39 +318,72 @@ __global__ void __launch_bounds__(256) reduce_matrix(DevReduce rd, const KT* __r
     double* accall = reinterpret_cast<double*>(smem_raw);
     const int groups_per_block = blockDim.x / group;
     const int gid = threadIdx.x / group, lane = threadIdx.x % group;
-    const int64_t r = (int64_t)blockIdx.x * groups_per_block + gid;
+    const int64_t stride = (int64_t)gridDim.x * groups_per_block;
+    int64_t r = (int64_t)blockIdx.x * groups_per_block + gid;
     if (r >= rd.nrows) return;
     double* acc = accall + (size_t)gid * rd.max_row_len;
     const ST* slots = reinterpret_cast<const ST*>(rd.slots);
     const ST SKIP = (ST)~(ST)0;
-    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
-    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
-    for (int x = lane; x < lenA + lenD; x += group) acc[x] = 0.0;
-    const int64_t k0 = rd.erow_ptr[r], k1 = rd.erow_ptr[r + 1];
-    int64_t sp = rd.slot_ptr[r];
-    for (int64_t kb = k0; kb < k1; kb += group) {
-        const int nk = (int)min((int64_t)group, k1 - kb);
-        const uint64_t mine = lane < nk ? rd.erows[kb + lane] : 0ull;
-        for (int k = 0; k < nk; k += 4) {
-            KT v[4];
-            ST sl[4];
+    RowMeta cur = row_meta(rd, r);
+    uint64_t mine = lane < min(group, cur.nk) ? rd.erows[cur.k0 + lane] : 0ull;
+    RowMeta nxt = cur;
+    if (r + stride < rd.nrows) nxt = row_meta(rd, r + stride);
+    for (;;) {
+        const int64_t rn = r + stride, rn2 = r + 2 * stride;
+        int64_t sp = cur.sp;
+        // first batch of the row's segments
+        KT v[SLX_RED_BATCH];
+        ST sl[SLX_RED_BATCH];
+        const int nk0 = min(group, cur.nk);
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const uint64_t dsc = __shfl(mine, min(k + u, nk - 1), group);
-                const int n = (k + u < nk) ? (int)(dsc >> 48) : 0;
-                const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-                const bool on = lane < n;
-                v[u] = on ? ke[off + lane] : (KT)0;
-                sl[u] = on ? slots[sp + lane] : SKIP;
-                sp += n;
-            }
-#pragma unroll
-            for (int u = 0; u < 4; u++)
-                if (sl[u] != SKIP) acc[sl[u]] += (double)v[u];
+        for (int u = 0; u < SLX_RED_BATCH; u++) {
+            const uint64_t dsc = __shfl(mine, min(u, max(nk0 - 1, 0)), group);
+            const int n = (u < nk0) ? (int)(dsc >> 48) : 0;
+            const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+            const bool on = lane < n;
+            v[u] = on ? ke[off + lane] : (KT)0;
+            sl[u] = on ? slots[sp + lane] : SKIP;
+            sp += n;
         }
+        // next row's descriptors, then the metadata two rows ahead
+        uint64_t mine_next = 0ull;
+        if (rn < rd.nrows) mine_next = lane < min(group, nxt.nk) ? rd.erows[nxt.k0 + lane] : 0ull;
+        RowMeta nxt2 = nxt;
+        if (rn2 < rd.nrows) nxt2 = row_meta(rd, rn2);
+        for (int x = lane; x < cur.lenA + cur.lenD; x += group) acc[x] = 0.0;
+#pragma unroll
+        for (int u = 0; u < SLX_RED_BATCH; u++)
+            if (sl[u] != SKIP) acc[sl[u]] += (double)v[u];
+        // remaining segments of long rows
+        for (int64_t kb = 0; kb < cur.nk; kb += group) {
+            const int nk = (int)min((int64_t)group, (int64_t)cur.nk - kb);
+            const uint64_t desc = kb == 0 ? mine : (lane < nk ? rd.erows[cur.k0 + kb + lane] : 0ull);
+            for (int k = (kb == 0 ? SLX_RED_BATCH : 0); k < nk; k += 4) {
+                KT w[4];
+                ST sw[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const uint64_t dsc = __shfl(desc, min(k + u, nk - 1), group);
+                    const int n = (k + u < nk) ? (int)(dsc >> 48) : 0;
+                    const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+                    const bool on = lane < n;
+                    w[u] = on ? ke[off + lane] : (KT)0;
+                    sw[u] = on ? slots[sp + lane] : SKIP;
+                    sp += n;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (sw[u] != SKIP) acc[sw[u]] += (double)w[u];
+            }
+        }
+        for (int x = lane; x < cur.lenA; x += group) Aval[cur.a0 + x] = acc[x];
+        for (int x = lane; x < cur.lenD; x += group) Dval[cur.d0 + x] = acc[cur.lenA + x];
+        if (rn >= rd.nrows) break;
+        r = rn;
+        cur = nxt;
+        nxt = nxt2;
+        mine = mine_next;
     }
-    for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
-    for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
 }
 
 // Segments longer than 64 entries: lanes stride over the segment.
@@ -396,29 +445,34 @@ hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void*
     return hipGetLastError();
 }
 
-template <typename CT, int T>
-static void launch_tiled(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
-    const unsigned grid = (unsigned)((b.n_elem + 255) / 256);
-    hipLaunchKernelGGL((elem_tiled<CT, T, T>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, (CT*)ke);
+template <typename CT, int TI>
+static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int jc, void* ke, hipStream_t st) {
+    const int epb = 256 / jc;
+    const unsigned grid = (unsigned)((b.n_elem + epb - 1) / epb);
+    hipLaunchKernelGGL((elem_cols<CT, TI>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, (CT*)ke);
 }
 
 template <typename CT>
-static void launch_tiled_any(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
-    int nmax = 0;
+static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
+    int ntf = 1, ndof = 1;
     for (int s = 0; s < b.n_slices; s++)
-        if ((mask >> s) & 1u) nmax = max(nmax, max(b.slices[s].n_tf, b.slices[s].n_dof));
-    if (nmax <= 4) launch_tiled<CT, 4>(b, ctab, mask, ke, st);
-    else if (nmax <= 6) launch_tiled<CT, 6>(b, ctab, mask, ke, st);
-    else if (nmax <= 10) launch_tiled<CT, 10>(b, ctab, mask, ke, st);
-    else launch_tiled<CT, 8>(b, ctab, mask, ke, st);
+        if ((mask >> s) & 1u) {
+            ntf = max(ntf, b.slices[s].n_tf);
+            ndof = max(ndof, b.slices[s].n_dof);
+        }
+    const int jc = min(ndof, 64);
+    if (ntf <= 4) launch_cols<CT, 4>(b, ctab, mask, jc, ke, st);
+    else if (ntf <= 8) launch_cols<CT, 8>(b, ctab, mask, jc, ke, st);
+    else if (ntf <= 10) launch_cols<CT, 10>(b, ctab, mask, jc, ke, st);
+    else launch_cols<CT, 16>(b, ctab, mask, jc, ke, st);
 }
 
 hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask) {
     if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
     if (precision == SLX_PREC_FP64)
-        launch_tiled_any<double>(b, ctab, slice_mask, ke, st);
+        launch_cols_any<double>(b, ctab, slice_mask, ke, st);
     else
-        launch_tiled_any<float>(b, ctab, slice_mask, ke, st);
+        launch_cols_any<float>(b, ctab, slice_mask, ke, st);
     return hipGetLastError();
 }
 
@@ -440,7 +494,16 @@ static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const v
         lds = (size_t)gpb * rd.max_row_len * sizeof(double);
     }
     if (lds > 64 * 1024) return hipErrorInvalidValue;
-    const int64_t grid = (rd.nrows + gpb - 1) / gpb;
+    int64_t grid = (rd.nrows + gpb - 1) / gpb;
+    // persistent groups for the pipelined kernel: exactly the blocks that are
+    // resident at once (a larger grid would run its extra blocks as a tail)
+    if (!longseg) {
+        int per_cu = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&reduce_matrix<KT, ST>),
+                                                                    threads, lds);
+        if (e != hipSuccess || per_cu < 1) per_cu = 1;
+        grid = std::min<int64_t>(grid, (int64_t)rd.n_cu * per_cu);
+    }
     if (longseg)
         hipLaunchKernelGGL((reduce_matrix_long<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
     else

@@ -55,6 +55,7 @@ struct DevReduce {
     const int64_t* slot_ptr;        // [nrows+1] into slots
     const void* slots;              // per Ke entry of the row: A position, lenA + D position, or all ones
     int32_t slot_bytes;             // 1, 2 or 4
+    int32_t n_cu;                   // compute units (persistent grid size)
     const int64_t* Arow;            // [nrows+1]
     const int64_t* Drow;            // [nrows+1]
     int32_t max_row_len;            // max (lenA + lenD)

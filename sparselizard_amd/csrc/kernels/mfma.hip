@@ -35,11 +35,16 @@ typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 // (element, slice) column the coefficient values C[p][q] are read from the
 // coalesced coefficient table, scaled by a power of two and split.  A column
 // lives in LDS as packed 32-bit words (hi | lo' << 16):
-//   compact  (coefficients constant over the Gauss points, n_gp >= 8):
-//            word p = C[p]; B[k][e] = C[k / n_gp] is rebuilt per lane from
-//            the two words its eight consecutive k can touch;
+//   compact  (coefficients constant over the Gauss points): word p = C[p].
+//            The K index is k = p * kq + q with the Gauss points of a pair
+//            padded to kq = n_gp rounded up to 8 (zero rows of Phi), so the
+//            eight consecutive k of a lane share one pair: the B fragment is
+//            one LDS word broadcast.  Word n_pairs is 0 (K padding).
 //   expanded otherwise: word k = C[k / n_gp][k % n_gp] for k < K, 0 up to
 //            the padded K, read as two 16-byte LDS loads per K step.
+// Output: the element rows of every segment group are assembled in LDS for
+// 16 elements and written with coalesced 16-byte stores (one contiguous
+// range per group).
 __device__ inline uint32_t pack_split(float x) {
     _Float16 hi = (_Float16)x;
     _Float16 lo = (_Float16)((x - (float)hi) * 2048.f);
@@ -47,17 +52,21 @@ __device__ inline uint32_t pack_split(float x) {
 }
 
 template <int MTW, int MAXK, bool COMPACT>
-__global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW == 1 ? (MAXK >= 8 ? 3 : 4) : 2, 8))) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
                                                             float* __restrict__ ke) {
-    extern __shared__ unsigned char smem_raw[];
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     const int NE = mb.ne_tile;
     const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq, ngp = b.n_gp;
     const int estride = mb.estride, cstride = mb.cstride;
-    uint32_t* Cs = reinterpret_cast<uint32_t*>(smem_raw);          // [NE][estride]
-    float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * estride);  // [ns][NE]
+    const int n_tf = mb.n_tf;
+    uint32_t* Cs = reinterpret_cast<uint32_t*>(smem_raw);                    // [NE][estride]
+    float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * estride);        // [ns][NE]
+    float* stg = colsc + (size_t)ns * NE;                                      // [16][n_tf][ld] of one group
     const SLX_CONST int32_t* csrc = (const SLX_CONST int32_t*)mb.csrc;
-    const SLX_CONST int64_t* kebase = (const SLX_CONST int64_t*)mb.kebase;
-    const SLX_CONST int64_t* keld = (const SLX_CONST int64_t*)mb.keld;
+    const SLX_CONST int64_t* gbase = (const SLX_CONST int64_t*)mb.gbase;
+    const SLX_CONST int32_t* gld = (const SLX_CONST int32_t*)mb.gld;
+    const SLX_CONST int32_t* sgrp = (const SLX_CONST int32_t*)mb.sgrp;
+    const SLX_CONST int32_t* scol = (const SLX_CONST int32_t*)mb.scol;
 
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -70,8 +79,8 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
         const int mt = wave + nw * i;
 #pragma unroll
         for (int ks = 0; ks < MAXK; ks++) {
-            if (mt < mb.mtiles && ks < mb.ksteps) {
-                const half8* src = reinterpret_cast<const half8*>(mb.afrag) + ((size_t)(mt * mb.ksteps + ks) * 2) * 64;
+            if (mt < mb.mtiles) {
+                const half8* src = reinterpret_cast<const half8*>(mb.afrag) + ((size_t)(mt * MAXK + ks) * 2) * 64;
                 Ahi[i][ks] = src[lane];
                 Alo[i][ks] = src[64 + lane];
             } else {
@@ -89,16 +98,10 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
             jrow[i][v] = row - irow[i][v] * mb.n_dof;
         }
     }
-    // compact columns: the K range of this lane's eight B values per K step
-    int plo[MAXK], vsplit[MAXK], vvalid[MAXK];
+    // compact columns: the pair word of this lane's eight k per K step
+    int pword[MAXK];
 #pragma unroll
-    for (int ks = 0; ks < MAXK; ks++) {
-        const int k0 = ks * 32 + 8 * lgrp;
-        const int p0 = min(k0 / ngp, np - 1);
-        plo[ks] = p0;
-        vsplit[ks] = (p0 + 1) * ngp - k0;   // v < vsplit -> word p0, else p0 + 1
-        vvalid[ks] = mb.K - k0;              // v >= vvalid -> 0
-    }
+    for (int ks = 0; ks < MAXK; ks++) pword[ks] = min((ks * 32 + 8 * lgrp) / mb.kq, np);
 
     const int ntiles_e = (b.n_elem + NE - 1) / NE;
     const int64_t nel = b.n_elem;
@@ -120,7 +123,7 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
             int ex = 0;
             if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
             const float sc = ldexpf(1.f, -ex);
-            colsc[s * NE + el] = ldexpf(1.f, ex);
+            colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
             uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
             for (int p = 0; p < np; p++) {
                 const int c = csrc[s * np + p];
@@ -139,43 +142,46 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
                 for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
         }
         __syncthreads();
-        // contraction: per slice and 16-element column tile, the wave's M tiles
+        // contraction: per 16-element column tile and segment group; the
+        // wave's M tiles of every slice of the group go to the LDS block
         const int nct = (NE + 15) / 16;
-        for (int s = 0; s < ns; s++) {
-            const int64_t kb = kebase[s], ld = keld[s];
-            for (int ct = 0; ct < nct; ct++) {
-                const int el = ct * 16 + lrow;  // this lane's B / D column
-                float4v acc_hh[MTW], acc_x[MTW];
+        for (int ct = 0; ct < nct; ct++) {
+            const int el = ct * 16 + lrow;  // this lane's B / D column
+            for (int g = 0; g < mb.n_groups; g++) {
+                const int ld = gld[g];
+                const int erow = n_tf * ld;  // floats per element in the group block
+                for (int s = 0; s < ns; s++) {
+                    if (sgrp[s] != g) continue;
+                    const int col0 = scol[s];
+                    float4v acc_hh[MTW], acc_x[MTW];
 #pragma unroll
-                for (int i = 0; i < MTW; i++)
+                    for (int i = 0; i < MTW; i++)
 #pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        acc_hh[i][v] = 0.f;
-                        acc_x[i][v] = 0.f;
-                    }
-                const uint32_t* colbase = Cs + (size_t)el * estride + (size_t)s * cstride;
+                        for (int v = 0; v < 4; v++) {
+                            acc_hh[i][v] = 0.f;
+                            acc_x[i][v] = 0.f;
+                        }
+                    const uint32_t* colbase = Cs + (size_t)el * estride + (size_t)s * cstride;
 #pragma unroll
-                for (int ks = 0; ks < MAXK; ks++) {
-                    if (ks < mb.ksteps) {
-                        uint32_t w[8];
+                    for (int ks = 0; ks < MAXK; ks++) {
+                        uint4v hw, lw;
                         if (COMPACT) {
-                            const uint32_t u0 = colbase[plo[ks]], u1 = colbase[plo[ks] + 1];
-#pragma unroll
-                            for (int v = 0; v < 8; v++) w[v] = v >= vvalid[ks] ? 0u : (v < vsplit[ks] ? u0 : u1);
+                            const uint32_t u = colbase[pword[ks]];
+                            const uint32_t h = u & 0xFFFFu, l = u >> 16;
+                            const uint32_t h2 = h | (h << 16), l2 = l | (l << 16);
+                            hw = (uint4v){h2, h2, h2, h2};
+                            lw = (uint4v){l2, l2, l2, l2};
                         } else {
                             const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
                             const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
-#pragma unroll
-                            for (int v = 0; v < 4; v++) {
-                                w[v] = x0[v];
-                                w[4 + v] = x1[v];
-                            }
-                        }
-                        uint4v hw, lw;
-#pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            hw[v] = (w[2 * v] & 0xFFFFu) | (w[2 * v + 1] << 16);
-                            lw[v] = (w[2 * v] >> 16) | (w[2 * v + 1] & 0xFFFF0000u);
+                            hw[0] = (x0[0] & 0xFFFFu) | (x0[1] << 16);
+                            hw[1] = (x0[2] & 0xFFFFu) | (x0[3] << 16);
+                            hw[2] = (x1[0] & 0xFFFFu) | (x1[1] << 16);
+                            hw[3] = (x1[2] & 0xFFFFu) | (x1[3] << 16);
+                            lw[0] = (x0[0] >> 16) | (x0[1] & 0xFFFF0000u);
+                            lw[1] = (x0[2] >> 16) | (x0[3] & 0xFFFF0000u);
+                            lw[2] = (x1[0] >> 16) | (x1[1] & 0xFFFF0000u);
+                            lw[3] = (x1[2] >> 16) | (x1[3] & 0xFFFF0000u);
                         }
                         const half8 bhi = __builtin_bit_cast(half8, hw);
                         const half8 blo = __builtin_bit_cast(half8, lw);
@@ -186,10 +192,8 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
                             acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[i], 0, 0, 0);
                         }
                     }
-                }
-                if (el < nloc) {
-                    const float colscale = colsc[s * NE + el] * mb.a_unscale;
-                    const int64_t ebase = kb + (int64_t)(e0 + el) * mb.n_tf * ld;
+                    const float colscale = colsc[s * NE + el];
+                    float* out = stg + (size_t)lrow * erow + col0;
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
                         const int mt = wave + nw * i;
@@ -197,11 +201,25 @@ __global__ void __launch_bounds__(512) mfma_f16split_kernel(DevBucket b, MfmaBuc
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
                             const int row = mt * 16 + 4 * lgrp + v;
-                            if (row < mb.M)
-                                ke[ebase + (int64_t)irow[i][v] * ld + jrow[i][v]] = (acc_hh[i][v] + acc_x[i][v] * (1.f / 2048.f)) * colscale;
+                            if (row < mb.M) out[irow[i][v] * ld + jrow[i][v]] = (acc_hh[i][v] + acc_x[i][v] * (1.f / 2048.f)) * colscale;
                         }
                     }
                 }
+                __syncthreads();
+                // the group's rows of these 16 elements are one contiguous range
+                const int nel_here = min(16, nloc - ct * 16);
+                if (nel_here > 0) {
+                    float* dst = ke + gbase[g] + (int64_t)(e0 + ct * 16) * erow;
+                    const int total = nel_here * erow;
+                    if (((uintptr_t)dst & 15u) == 0 && (erow & 3) == 0) {
+                        float4v* d4 = reinterpret_cast<float4v*>(dst);
+                        const float4v* s4 = reinterpret_cast<const float4v*>(stg);
+                        for (int x = threadIdx.x; x < total / 4; x += blockDim.x) d4[x] = s4[x];
+                    } else {
+                        for (int x = threadIdx.x; x < total; x += blockDim.x) dst[x] = stg[x];
+                    }
+                }
+                __syncthreads();
             }
         }
     }
@@ -220,7 +238,7 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
 }
 
 size_t mfma_lds_bytes(const MfmaBucket& mb) {
-    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4;
+    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.n_tf * mb.max_gld * 4;
 }
 
 bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk) {
@@ -230,23 +248,29 @@ bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk) {
     if (t > 2) return false;
     *nw = w;
     *mtw = t;
-    *maxk = ksteps <= 1 ? 1 : ksteps <= 2 ? 2 : ksteps <= 4 ? 4 : 8;
+    *maxk = ksteps;
     return true;
+}
+
+template <int MTW>
+static hipError_t launch_mtw(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
+    switch (mb.maxk) {
+        case 1: return launch_variant<MTW, 1>(b, mb, ctab, ke, st);
+        case 2: return launch_variant<MTW, 2>(b, mb, ctab, ke, st);
+        case 3: return launch_variant<MTW, 3>(b, mb, ctab, ke, st);
+        case 4: return launch_variant<MTW, 4>(b, mb, ctab, ke, st);
+        case 5: return launch_variant<MTW, 5>(b, mb, ctab, ke, st);
+        case 6: return launch_variant<MTW, 6>(b, mb, ctab, ke, st);
+        case 7: return launch_variant<MTW, 7>(b, mb, ctab, ke, st);
+        case 8: return launch_variant<MTW, 8>(b, mb, ctab, ke, st);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     if (b.n_elem == 0) return hipSuccess;
-    if (mb.mtw == 1) {
-        if (mb.maxk == 1) return launch_variant<1, 1>(b, mb, ctab, ke, st);
-        if (mb.maxk == 2) return launch_variant<1, 2>(b, mb, ctab, ke, st);
-        if (mb.maxk == 4) return launch_variant<1, 4>(b, mb, ctab, ke, st);
-        if (mb.maxk == 8) return launch_variant<1, 8>(b, mb, ctab, ke, st);
-    } else if (mb.mtw == 2) {
-        if (mb.maxk == 1) return launch_variant<2, 1>(b, mb, ctab, ke, st);
-        if (mb.maxk == 2) return launch_variant<2, 2>(b, mb, ctab, ke, st);
-        if (mb.maxk == 4) return launch_variant<2, 4>(b, mb, ctab, ke, st);
-        if (mb.maxk == 8) return launch_variant<2, 8>(b, mb, ctab, ke, st);
-    }
+    if (mb.mtw == 1) return launch_mtw<1>(b, mb, ctab, ke, st);
+    if (mb.mtw == 2) return launch_mtw<2>(b, mb, ctab, ke, st);
     return hipErrorInvalidValue;
 }
 

@@ -39,6 +39,10 @@ struct DeviceError : std::runtime_error {
     using std::runtime_error::runtime_error;
 };
 
+struct UnsupportedError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
 template <typename F>
 slx_status guarded(F&& f) {
     try {
@@ -47,6 +51,9 @@ slx_status guarded(F&& f) {
     } catch (const DeviceError& e) {
         g_last_error = e.what();
         return SLX_ERR_DEVICE;
+    } catch (const UnsupportedError& e) {
+        g_last_error = e.what();
+        return SLX_ERR_UNSUPPORTED;
     } catch (const std::bad_alloc& e) {
         g_last_error = "allocation failed";
         return SLX_ERR_NOMEM;
@@ -163,6 +170,7 @@ struct SegGroup {
 struct KeLayout {
     std::vector<SegGroup> groups;
     std::vector<std::vector<int64_t>> ke_base, ke_ld;  // [bucket][slice]
+    std::vector<std::vector<int>> group_of;            // [bucket][slice]
     int64_t n_ke = 0;
 };
 
@@ -176,10 +184,12 @@ KeLayout compute_layout(const slx_plan_desc& d) {
     KeLayout L;
     L.ke_base.resize(d.n_buckets);
     L.ke_ld.resize(d.n_buckets);
+    L.group_of.resize(d.n_buckets);
     for (int b = 0; b < d.n_buckets; b++) {
         const slx_bucket_desc& bd = d.buckets[b];
         L.ke_base[b].assign(bd.n_slices, 0);
         L.ke_ld[b].assign(bd.n_slices, 0);
+        L.group_of[b].assign(bd.n_slices, -1);
         std::vector<uint64_t> htf(bd.n_slices, 0), hdof(bd.n_slices, 0);
         for (int s = 0; s < bd.n_slices; s++) {
             const slx_slice_desc& sd = bd.slices[s];
@@ -216,6 +226,7 @@ KeLayout compute_layout(const slx_plan_desc& d) {
                 join = (int)L.groups.size() - 1;
             }
             SegGroup& G = L.groups[join];
+            L.group_of[b][s] = join;
             L.ke_base[b][s] = G.len;  // column offset for now
             G.slices.push_back(s);
             G.len += sd.is_rhs ? 1 : sd.n_dof;
@@ -459,6 +470,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.rd.slot_ptr = P.d_slot_ptr.as<int64_t>();
     P.rd.slots = P.d_slots.p;
     P.rd.slot_bytes = slot_bytes;
+    P.rd.n_cu = P.n_cu;
     P.rd.Arow = P.d_Arow.as<int64_t>();
     P.rd.Drow = P.d_Drow.as<int64_t>();
     P.rd.max_row_len = maxrow;
@@ -539,20 +551,43 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.n_tf = n_tf;
     mb.n_dof = n_dof;
     mb.n_pairs = (int)pairs.size();
-    mb.K = mb.n_pairs * bd.n_gp;
+    mb.nq = bd.q_invariant ? 1 : bd.n_gp;
+    // compact columns (coefficients constant over the Gauss points): the
+    // Gauss points of a pair are padded to a multiple of 8 in K
+    mb.compact = mb.nq == 1 ? 1 : 0;
+    mb.kq = mb.compact ? ((bd.n_gp + 7) / 8) * 8 : bd.n_gp;
+    mb.K = mb.n_pairs * mb.kq;
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
-    mb.nq = bd.q_invariant ? 1 : bd.n_gp;
     mb.n_mslices = (int)ms.size();
     if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk)) return;
-    // LDS column layout (words): compact when constant over Gauss points and
-    // eight consecutive k touch at most two derivative pairs
-    mb.compact = (mb.nq == 1 && bd.n_gp >= 8) ? 1 : 0;
     mb.cstride = mb.compact ? mb.n_pairs + 1 : 32 * mb.ksteps;
     int es = mb.n_mslices * mb.cstride;
     if (mb.compact) es |= 1;  // odd: conflict-free 32-bit reads across elements
     else es = ((es + 59) / 64) * 64 + 4;  // = 4 (mod 64): 16-byte reads spread over the banks
     mb.estride = es;
+    // segment groups of the slices (output blocks)
+    std::vector<int> gl;  // layout group index per kernel group
+    std::vector<int32_t> sgrp(ms.size()), scol(ms.size());
+    std::vector<int64_t> gb;
+    std::vector<int32_t> gd;
+    for (size_t si = 0; si < ms.size(); si++) {
+        const int g = L.group_of[b][ms[si]];
+        int k = (int)(std::find(gl.begin(), gl.end(), g) - gl.begin());
+        if (k == (int)gl.size()) {
+            gl.push_back(g);
+            gb.push_back(L.groups[g].base);
+            gd.push_back((int32_t)L.groups[g].len);
+        }
+        sgrp[si] = k;
+        scol[si] = (int32_t)(L.ke_base[b][ms[si]] - L.groups[g].base);
+    }
+    // every slice of a group must be computed here (the block is written whole)
+    for (int g : gl)
+        for (int s2 : L.groups[g].slices)
+            if (std::find(ms.begin(), ms.end(), s2) == ms.end()) return;
+    mb.n_groups = (int)gl.size();
+    mb.max_gld = *std::max_element(gd.begin(), gd.end());
     mb.ne_tile = 0;
     for (int ne : {64, 32, 16}) {
         mb.ne_tile = ne;
@@ -562,12 +597,9 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     if (mb.ne_tile == 0) return;
     // coefficient slot of every (slice, pair)
     std::vector<int32_t> csrc((size_t)mb.n_mslices * mb.n_pairs, -1);
-    std::vector<int64_t> kb(mb.n_mslices), kl(mb.n_mslices);
     for (size_t si = 0; si < ms.size(); si++) {
         const slx_slice_desc& sd = bd.slices[ms[si]];
         mb.slice_idx[si] = ms[si];
-        kb[si] = L.ke_base[b][ms[si]];
-        kl[si] = L.ke_ld[b][ms[si]];
         mask |= 1u << ms[si];
         for (int t = 0; t < sd.n_terms; t++) {
             std::pair<int, int> pr(sd.terms[t].tf_deriv, sd.terms[t].dof_deriv);
@@ -589,7 +621,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
                 for (int q = 0; q < ng; q++) {
                     double v = tft[((size_t)pairs[p].first * n_tf + i) * ng + q] * bd.gp_weights[q] *
                                dft[((size_t)pairs[p].second * n_dof + j) * ng + q];
-                    phi[((size_t)i * n_dof + j) * mb.K + p * ng + q] = v;
+                    phi[((size_t)i * n_dof + j) * mb.K + p * mb.kq + q] = v;
                     mx = std::max(mx, std::fabs(v));
                 }
     int ex = 0;
@@ -612,8 +644,10 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
                 }
     mb.afrag = P.dev(frag.data(), frag.size());
     mb.csrc = P.dev(csrc.data(), csrc.size());
-    mb.kebase = P.dev(kb.data(), kb.size());
-    mb.keld = P.dev(kl.data(), kl.size());
+    mb.gbase = P.dev(gb.data(), gb.size());
+    mb.gld = P.dev(gd.data(), gd.size());
+    mb.sgrp = P.dev(sgrp.data(), sgrp.size());
+    mb.scol = P.dev(scol.data(), scol.size());
     mb.max_blocks = P.n_cu * 2;
     P.mfma[b] = mb;
     P.mfma_mask[b] = mask;
@@ -777,6 +811,7 @@ struct LaunchTimer {
 void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t plen, const double* const* fields, int precision,
              hipStream_t st, LaunchTimer* tm = nullptr) {
     SLX_CHECK(precision >= 0 && precision <= 3, "unknown precision " << precision);
+    if (precision == SLX_PREC_BF16_SPLIT) throw UnsupportedError("precision SLX_PREC_BF16_SPLIT is not implemented");
     if (P.max_param >= 0) SLX_CHECK(params && plen >= P.max_elem_id, "per-element parameters missing or shorter than the element count");
     if (P.uses_fields) {
         SLX_CHECK(fields, "field data required by the coefficient programs");
