@@ -174,6 +174,13 @@ slx_status slx_assemble_device(slx_plan plan, const double* d_node_xyz, const do
                                int64_t param_len, const double* const* d_field_data, int32_t precision,
                                void* hip_stream);
 slx_status slx_plan_sync(slx_plan plan);
+/* Captures one numeric phase on these device inputs into a HIP graph owned by
+ * the plan (for Newton loops: update the field data in place, relaunch).  The
+ * captured work is kernels only (no memset, no allocation, no host sync). */
+slx_status slx_plan_graph_capture(slx_plan plan, const double* d_node_xyz, const double* d_params, int64_t param_len,
+                                  const double* const* d_field_data, int32_t precision);
+/* Launches the captured graph on hip_stream (NULL: the plan's stream). */
+slx_status slx_plan_graph_launch(slx_plan plan, void* hip_stream);
 /* Kernel timing on the plan's stream with HIP events around every launch:
  * runs `iters` numeric phases on device-resident inputs and returns the
  * average milliseconds per phase of the Gauss-point coefficient kernels
@@ -240,7 +247,9 @@ typedef struct slx_problem_s* slx_problem;
  *       "elasticity3d" (P2 tets, h1xyz, n^3 cubes x 6 tets),
  *       "hppoisson3d" (hex, per-element order 1..5),
  *       "quadbatch" (independent quad instances, K + k^2 M),
- *       "newton2d" (P2 tris, nonlinear reluctivity), "msh" (a gmsh 2.2 file).
+ *       "newton2d" (P2 tris, nonlinear reluctivity, Newton re-assembly),
+ *       "msh" (a gmsh 2.2 file), "electrostatics1d" / "hoprojection2d" (the
+ *       reference examples of the same names on their mesh files).
  * The spec string is "key=value;key=value" (n, seed, jitter, order, ...). */
 slx_status slx_problem_create(slx_ctx ctx, const char* kind, const char* spec, slx_problem* out);
 slx_status slx_problem_destroy(slx_problem p);
@@ -256,6 +265,9 @@ slx_status slx_problem_info(slx_problem p, int64_t* ndofs, int64_t* nfree, int64
 slx_status slx_problem_desc(slx_problem p, const slx_plan_desc** desc);
 /* Newton: set the field data used by SLX_OP_FIELD (setdata). */
 slx_status slx_problem_setdata(slx_problem p, int32_t field, const double* values, int64_t n);
+/* sl::setdata(vec) (src/expression/operation/sl.cpp): the fields of the
+ * formulation take their data from a solution vector in DOF numbering. */
+slx_status slx_problem_setdata_vec(slx_problem p, const double* x, int64_t n);
 
 #ifdef __cplusplus
 }

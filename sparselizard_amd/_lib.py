@@ -112,6 +112,9 @@ def lib():
         "slx_problem_info": (i32, [vp, p64, p64, p64, p64]),
         "slx_problem_desc": (i32, [vp, C.POINTER(vp)]),
         "slx_problem_setdata": (i32, [vp, i32, pd, i64]),
+        "slx_problem_setdata_vec": (i32, [vp, pd, i64]),
+        "slx_plan_graph_capture": (i32, [vp, vp, vp, i64, vp, i32]),
+        "slx_plan_graph_launch": (i32, [vp, vp]),
         "slx_gauss_rule": (i32, [i32, i32, i32, p32, pd, pd]),
         "slx_selftest_mfma_layout": (i32, [vp, C.POINTER(C.c_float)]),
         "slx_h1_count": (i32, [i32, i32, p32]),
@@ -129,7 +132,8 @@ EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
     "slx_plan_destroy", "slx_plan_kernels", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
-    "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata",
+    "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
+    "slx_plan_graph_capture", "slx_plan_graph_launch",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout",
 ]
 
@@ -253,6 +257,21 @@ class Problem:
         v = np.ascontiguousarray(values, dtype=np.float64)
         check(lib().slx_problem_setdata(self.h, field, _p(v, C.c_double), len(v)))
 
+    def setdata_vec(self, x):
+        """sl::setdata(vec): field data from a solution vector in DOF numbering."""
+        v = np.ascontiguousarray(x, dtype=np.float64)
+        check(lib().slx_problem_setdata_vec(self.h, _p(v, C.c_double), len(v)))
+
+    def field_data(self):
+        """Host views of the data of the fields read by the coefficient programs."""
+        nd = self.numeric()
+        d = self.desc()
+        out = []
+        for k in range(d.n_fields):
+            n = int(nd.field_len[k])
+            out.append(np.ctypeslib.as_array(nd.field_data[k], shape=(n,)) if n else np.zeros(0))
+        return out
+
     def close(self):
         if self.h:
             lib().slx_problem_destroy(self.h)
@@ -331,6 +350,13 @@ class Plan:
     def assemble_device(self, d_xyz, d_params, param_len, d_fields, precision, stream=None):
         check(lib().slx_assemble_device(self.h, d_xyz, d_params, param_len, d_fields,
                                         PRECISIONS[precision] if isinstance(precision, str) else precision, stream))
+
+    def graph_capture(self, d_xyz, d_params, param_len, d_fields, precision):
+        check(lib().slx_plan_graph_capture(self.h, d_xyz, d_params, param_len, d_fields,
+                                           PRECISIONS[precision] if isinstance(precision, str) else precision))
+
+    def graph_launch(self, stream=None):
+        check(lib().slx_plan_graph_launch(self.h, stream))
 
     def profile(self, d_xyz, d_params, param_len, d_fields, precision, iters=5):
         """Average ms per numeric phase: coef, mfma, valu, reduce (HIP events per launch)."""

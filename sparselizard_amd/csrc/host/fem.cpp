@@ -249,6 +249,23 @@ std::vector<double> DofManager::constraint_values() const {
     return out;
 }
 
+void DofManager::setdata(const double* x, int64_t n) const {
+    // vec -> field data (sl::setdata, rawfield::setdata through the dof
+    // structure): same disjoint region and form function index, element offset
+    SLX_CHECK(n == ndofs_, "solution vector has " << n << " entries, the dof structure " << ndofs_);
+    for (size_t fi = 0; fi < fields_.size(); fi++) {
+        RawField& f = *fields_[fi];
+        if (f.data_size() == 0) continue;
+        for (int dr = 0; dr < (int)rangebegin_[fi].size(); dr++) {
+            const std::vector<int>& rb = rangebegin_[fi][dr];
+            const std::vector<int>& db = f.data_rangebegin[dr];
+            const int cnt = mesh_->disjreg_count(dr);
+            for (size_t ff = 0; ff < rb.size() && ff < db.size(); ff++)
+                for (int i = 0; i < cnt; i++) f.data[(size_t)db[ff] + i] = x[(size_t)rb[ff] + i];
+        }
+    }
+}
+
 std::vector<int32_t> DofManager::getaddresses(const std::shared_ptr<RawField>& f, int order, int elemtype,
                                               const std::vector<int>& elems, int fieldphysreg) const {
     // dofmanager::getaddresses (dofmanager.cpp:846-918)

@@ -133,6 +133,10 @@ class DofManager {
     std::vector<uint8_t> isconstrained() const;
     // Dirichlet values of the constrained DOFs (0 elsewhere).
     std::vector<double> constraint_values() const;
+    // sl::setdata(vec): copies the values of a solution vector (this DOF
+    // numbering) into the data of every field of the structure that holds
+    // data (the field's own numbering).
+    void setdata(const double* x, int64_t n) const;
     // dofmanager::getaddresses: [n_ff][elems.size()] row-major.
     std::vector<int32_t> getaddresses(const std::shared_ptr<RawField>& f, int order, int elemtype,
                                       const std::vector<int>& elems, int fieldphysreg) const;

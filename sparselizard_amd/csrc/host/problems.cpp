@@ -131,6 +131,47 @@ std::shared_ptr<Problem> poisson2d(const std::map<std::string, std::string>& s) 
     return p;
 }
 
+// Config: nonlinear 2D magnetostatics-type problem with Newton re-assembly.
+// P2 field a on n x n x 2 jittered triangles of the unit square (a = 0 on the
+// boundary), reluctivity nu(|grad a|) = 1 + 500 / (1 + |grad a|^2), source f.
+// The weak form is linearized at the current field a0 as in the reference's
+// saturation example (examples/magnetostatics-vector-potential-saturation-2d/
+// main.cpp:93-94):  integral((h(b0) + dh/db(b0) (grad(dof a) - b0)) . grad(tf a))
+// with b0 = grad a0, h = nu b0, dh/db = nu I + (nu'/|b0|) b0 b0^T and
+// nu'/|b| = -1000 / (1 + |b|^2)^2; plus integral(-f tf a).  Each generate()
+// re-assembles the Jacobian and the rhs at the field data last set.
+std::shared_ptr<Problem> newton2d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 708), order = geti(s, "order", 2);
+    double jit = getd(s, "jitter", 0.2), f = getd(s, "f", 500.0);
+    Rng rng((uint64_t)geti(s, "seed", 4));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    m.set_nodes(square_nodes(n, jit, rng));
+    square_boundary(m, n, 0, 2);
+    square_tris(m, n, 0, 1);
+    m.finalize();
+    field a(m, "h1", "a");
+    a.setorder(1, order);
+    a.setconstraint(2);
+    p->fields = {a};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    expression g = grad(a);
+    expression n2 = g * g;
+    expression one(1.0);
+    expression nu = one + expression(500.0) / (one + n2);
+    expression c = expression(-1000.0) / ((one + n2) * (one + n2));
+    expression gd = grad(dof(a)), gt = grad(tf(a));
+    expression dg = gd - g;
+    *p->form += integral(1, (nu * g + nu * dg + c * (g * dg) * g) * gt);
+    *p->form += integral(1, -expression(f) * tf(a));
+    std::ostringstream d;
+    d << "2D Newton magnetostatics P" << order << ", " << n << "x" << n << "x2 triangles, jitter " << jit << ", f " << f;
+    p->describe = d.str();
+    return p;
+}
+
 // Config: 3D isotropic elasticity, h1xyz order p on Kuhn tets of an n^3 grid,
 // E per element lognormal(0, sigma), nu, clamped on x = 0, body force -z.
 std::shared_ptr<Problem> elasticity3d(const std::map<std::string, std::string>& s) {
@@ -466,6 +507,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "hppoisson3d") p = hppoisson3d(s);
     else if (kind == "msh") p = from_msh(s);
     else if (kind == "electrostatics1d") p = electrostatics1d(s);
+    else if (kind == "newton2d") p = newton2d(s);
     else if (kind == "hoprojection2d") p = highorderprojection2d(s);
     else fail("unknown problem kind '" + kind + "'");
     p->kind = kind;

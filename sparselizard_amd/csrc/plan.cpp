@@ -130,6 +130,7 @@ struct slx_plan_s {
     bool uses_fields = false;
     bool host_only = false;        // symbolic phase only (no device): pattern inspection
 
+    hipGraphExec_t graph_exec = nullptr;  // captured numeric phase (slx_plan_graph_capture)
     std::vector<MfmaBucket> mfma;      // per bucket; M == 0 when not eligible
     std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
     int n_cu = 256;
@@ -1056,6 +1057,41 @@ slx_status slx_plan_reduction_maps(slx_plan p, int64_t* sizes, int64_t* erow_ptr
     });
 }
 
+slx_status slx_plan_graph_capture(slx_plan p, const double* d_node_xyz, const double* d_params, int64_t param_len,
+                                  const double* const* d_field_data, int32_t precision) {
+    return guarded([&] {
+        SLX_CHECK(p && d_node_xyz, "null argument");
+        SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
+        HIPCHECK(hipSetDevice(p->ctx->device));
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (p->graph_exec) {
+            (void)hipGraphExecDestroy(p->graph_exec);
+            p->graph_exec = nullptr;
+        }
+        hipGraph_t g = nullptr;
+        HIPCHECK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
+        try {
+            numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, p->stream);
+        } catch (...) {
+            hipGraph_t dropped = nullptr;
+            (void)hipStreamEndCapture(p->stream, &dropped);
+            if (dropped) (void)hipGraphDestroy(dropped);
+            throw;
+        }
+        HIPCHECK(hipStreamEndCapture(p->stream, &g));
+        hipError_t e = hipGraphInstantiate(&p->graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIPCHECK(e);
+    });
+}
+
+slx_status slx_plan_graph_launch(slx_plan p, void* hip_stream) {
+    return guarded([&] {
+        SLX_CHECK(p && p->graph_exec, "no captured graph on this plan");
+        HIPCHECK(hipGraphLaunch(p->graph_exec, hip_stream ? (hipStream_t)hip_stream : p->stream));
+    });
+}
+
 slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* n_out) {
     return guarded([&] {
         SLX_CHECK(p && n_out, "null argument");
@@ -1098,6 +1134,7 @@ slx_status slx_plan_destroy(slx_plan p) {
     return guarded([&] {
         if (!p) return;
         if (!p->host_only) (void)hipStreamSynchronize(p->stream);
+        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
         delete p;
     });
 }
@@ -1234,6 +1271,13 @@ slx_status slx_problem_desc(slx_problem p, const slx_plan_desc** desc) {
     return guarded([&] {
         SLX_CHECK(p && desc, "null argument");
         *desc = &p->prob->plan->desc;
+    });
+}
+
+slx_status slx_problem_setdata_vec(slx_problem p, const double* x, int64_t n) {
+    return guarded([&] {
+        SLX_CHECK(p && x, "null argument");
+        p->prob->form->dofmanager().setdata(x, n);
     });
 }
 
