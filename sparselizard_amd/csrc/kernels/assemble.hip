@@ -138,52 +138,106 @@ __device__ inline T apply_unary(int op, T x) {
     return x;
 }
 
-// Coefficient program for one (element, Gauss point); outputs go to
-// out[slot] multiplied by |det J|.
-template <typename T>
-__device__ void run_program(const DevBucket& b, const DevInputs& in, int e, int q, const Geo& g, T* out, int64_t out_stride) {
-    T reg[SLX_MAX_REGS];
-    T adet = (T)fabs(g.det);
+// Coefficient program, evaluated for V (element, Gauss point) pairs per
+// thread: every interpreted instruction (uniform across the wave) is applied
+// to V independent values, so one register-file access moves a V-vector and
+// the instruction fetch and dispatch are shared.  Outputs go to
+// ctab[(slot * nq + q) * n_elem + e] multiplied by |det J|.
+template <typename T, int V>
+__device__ void run_program(const DevBucket& b, const DevInputs& in, const int (&e)[V], const int (&q)[V], const Geo (&g)[V],
+                            const bool (&on)[V], int nq, T* __restrict__ ctab) {
+    T reg[SLX_MAX_REGS][V];
+    const int64_t stride = (int64_t)nq * b.n_elem;
     for (int pc = 0; pc < b.n_instr; pc++) {
-        slx_instr ins = b.prog[pc];
-        T va = reg[ins.a < SLX_MAX_REGS ? ins.a : 0], vb = reg[ins.b < SLX_MAX_REGS ? ins.b : 0];
-        T r = 0;
+        const slx_instr ins = b.prog[pc];
+        const int ia = ins.a < SLX_MAX_REGS ? ins.a : 0, ib = ins.b < SLX_MAX_REGS ? ins.b : 0;
+        T r[V];
         switch (ins.op) {
-            case SLX_OP_CONST: r = (T)in.consts[ins.imm]; break;
-            case SLX_OP_PARAM: r = (T)in.params[(size_t)ins.imm * in.param_len + b.elem_ids[e]]; break;
-            case SLX_OP_JAC: r = (T)pick9(g.J, 3 * ins.a + ins.b); break;
-            case SLX_OP_INVJAC: r = (T)pick9(g.inv, 3 * ins.a + ins.b); break;
-            case SLX_OP_DETJ: r = (T)g.det; break;
-            case SLX_OP_COORD: {
-                double s = 0.0;
-                for (int k = 0; k < b.n_geo; k++) {
-                    int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-                    s += in.xyz[3 * (size_t)node + ins.a] * b.geo_tab[((size_t)0 * b.n_geo + k) * b.n_gp + q];
-                }
-                r = (T)s;
+            case SLX_OP_CONST: {
+                const T c = (T)in.consts[ins.imm];
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = c;
                 break;
             }
-            case SLX_OP_FIELD: r = (T)field_value(b, in, ins.imm, ins.a, e, q, g); break;
-            case SLX_OP_ADD: r = va + vb; break;
-            case SLX_OP_MUL: r = va * vb; break;
-            case SLX_OP_POW: r = pow(va, vb); break;
-            case SLX_OP_OUT: out[(size_t)ins.imm * out_stride] = va * adet; continue;
-            default: r = apply_unary<T>(ins.op, va); break;
+            case SLX_OP_PARAM:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = (T)in.params[(size_t)ins.imm * in.param_len + b.elem_ids[e[v]]];
+                break;
+            case SLX_OP_JAC:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = (T)pick9(g[v].J, 3 * ins.a + ins.b);
+                break;
+            case SLX_OP_INVJAC:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = (T)pick9(g[v].inv, 3 * ins.a + ins.b);
+                break;
+            case SLX_OP_DETJ:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = (T)g[v].det;
+                break;
+            case SLX_OP_COORD:
+#pragma unroll
+                for (int v = 0; v < V; v++) {
+                    double s = 0.0;
+                    for (int k = 0; k < b.n_geo; k++) {
+                        const int node = b.geo_nodes[(size_t)e[v] * b.n_geo + k];
+                        s += in.xyz[3 * (size_t)node + ins.a] * b.geo_tab[(size_t)k * b.n_gp + q[v]];
+                    }
+                    r[v] = (T)s;
+                }
+                break;
+            case SLX_OP_FIELD:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = (T)field_value(b, in, ins.imm, ins.a, e[v], q[v], g[v]);
+                break;
+            case SLX_OP_ADD:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = reg[ia][v] + reg[ib][v];
+                break;
+            case SLX_OP_MUL:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = reg[ia][v] * reg[ib][v];
+                break;
+            case SLX_OP_POW:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = pow(reg[ia][v], reg[ib][v]);
+                break;
+            case SLX_OP_OUT:
+#pragma unroll
+                for (int v = 0; v < V; v++)
+                    if (on[v]) ctab[(size_t)ins.imm * stride + (size_t)q[v] * b.n_elem + e[v]] = reg[ia][v] * (T)fabs(g[v].det);
+                continue;
+            default:
+#pragma unroll
+                for (int v = 0; v < V; v++) r[v] = apply_unary<T>(ins.op, reg[ia][v]);
+                break;
         }
-        reg[ins.dst] = r;
+#pragma unroll
+        for (int v = 0; v < V; v++) reg[ins.dst][v] = r[v];
     }
 }
 
 // Coefficient table of one bucket (nq = 1 when the coefficients do not vary
-// over the Gauss points).
-template <typename T>
+// over the Gauss points); each thread takes V consecutive (element, Gauss
+// point) pairs, element fastest.
+template <typename T, int V>
 __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, int nq, T* __restrict__ ctab) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)b.n_elem * nq) return;
-    const int e = (int)(idx % b.n_elem), q = (int)(idx / b.n_elem);
-    Geo g;
-    geometry(b, in.xyz, e, q, g);
-    run_program<T>(b, in, e, q, g, ctab + (size_t)q * b.n_elem + e, (int64_t)nq * b.n_elem);
+    const int64_t total = (int64_t)b.n_elem * nq;
+    const int64_t idx0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
+    if (idx0 >= total) return;
+    int e[V], q[V];
+    bool on[V];
+    Geo g[V];
+#pragma unroll
+    for (int v = 0; v < V; v++) {
+        const int64_t idx = idx0 + v;
+        on[v] = idx < total;
+        const int64_t id = on[v] ? idx : idx0;
+        e[v] = (int)(id % b.n_elem);
+        q[v] = (int)(id / b.n_elem);
+        geometry(b, in.xyz, e[v], q[v], g[v]);
+    }
+    run_program<T, V>(b, in, e, q, g, on, nq, ctab);
 }
 
 // Element matrices, one lane per (element, dof column): `jc` consecutive
@@ -437,11 +491,13 @@ hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void*
     if (b.n_elem == 0 || b.n_coefs == 0) return hipSuccess;
     const int nq = b.q_inv ? 1 : b.n_gp;
     const int64_t n = (int64_t)b.n_elem * nq;
-    const unsigned grid = (unsigned)((n + 255) / 256);
-    if (fp64)
-        hipLaunchKernelGGL((coef_kernel<double>), dim3(grid), dim3(256), 0, st, b, in, nq, (double*)ctab);
-    else
-        hipLaunchKernelGGL((coef_kernel<float>), dim3(grid), dim3(256), 0, st, b, in, nq, (float*)ctab);
+    if (fp64) {
+        const unsigned grid = (unsigned)((n + 2 * 256 - 1) / (2 * 256));
+        hipLaunchKernelGGL((coef_kernel<double, 2>), dim3(grid), dim3(256), 0, st, b, in, nq, (double*)ctab);
+    } else {
+        const unsigned grid = (unsigned)((n + 4 * 256 - 1) / (4 * 256));
+        hipLaunchKernelGGL((coef_kernel<float, 4>), dim3(grid), dim3(256), 0, st, b, in, nq, (float*)ctab);
+    }
     return hipGetLastError();
 }
 

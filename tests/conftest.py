@@ -14,6 +14,10 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # torch first: its HIP runtime must be the one the process initialises
+    # (tests that stage device buffers through torch share it with libslx)
+    import torch
+    torch.cuda.init()
     import sparselizard_amd as sa
     c = sa.Context(0)
     yield c
