@@ -151,7 +151,7 @@ def main():
     else:
         achieved = red_bytes / (prof_mix["reduce"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
-                "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "reduce_matrix",
+                "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "reduce_rows",
                 "kernel_ms": round(prof_mix["reduce"], 4)}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):

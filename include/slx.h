@@ -185,7 +185,7 @@ slx_status slx_plan_graph_launch(slx_plan plan, void* hip_stream);
  * runs `iters` numeric phases on device-resident inputs and returns the
  * average milliseconds per phase of the Gauss-point coefficient kernels
  * (ms[0]), the MFMA element contractions (ms[1]), the VALU element kernels
- * (ms[2]) and the CSR/rhs reduction (ms[3]). */
+ * (ms[2]), the CSR value reduction (ms[3]) and the rhs reduction (ms[4]). */
 slx_status slx_plan_profile(slx_plan plan, const double* d_node_xyz, const double* d_params, int64_t param_len,
                             const double* const* d_field_data, int32_t precision, int32_t iters, double* ms);
 /* Device views of the assembled CSR (valid until slx_plan_destroy). */

@@ -359,12 +359,13 @@ class Plan:
         check(lib().slx_plan_graph_launch(self.h, stream))
 
     def profile(self, d_xyz, d_params, param_len, d_fields, precision, iters=5):
-        """Average ms per numeric phase: coef, mfma, valu, reduce (HIP events per launch)."""
-        ms = np.zeros(4, np.float64)
+        """Average ms per numeric phase: coef, mfma, valu, reduce, rhs (HIP events per launch)."""
+        ms = np.zeros(5, np.float64)
         check(lib().slx_plan_profile(self.h, d_xyz, d_params, param_len, d_fields,
                                      PRECISIONS[precision] if isinstance(precision, str) else precision, iters,
                                      _p(ms, C.c_double)))
-        return {"coef": float(ms[0]), "mfma": float(ms[1]), "valu": float(ms[2]), "reduce": float(ms[3])}
+        return {"coef": float(ms[0]), "mfma": float(ms[1]), "valu": float(ms[2]), "reduce": float(ms[3]),
+                "rhs": float(ms[4])}
 
     def sync(self):
         check(lib().slx_plan_sync(self.h))

@@ -12,13 +12,19 @@
 //   (contribution.cpp:172-246; the per-term dgemm of densemat.cpp:228-311 with
 //   the terms summed as in addproduct), right-hand-side slices write -sum
 //   (contribution.cpp:296).
-// reduce_matrix: one lane group per CSR row walks the row's element-row
-//   segments and adds each entry into its column slot in LDS (FP64), then
-//   writes the A and D blocks (rawmat.cpp:67-125, 238-289).
+// reduce_rows: a few lanes per CSR row walk the row's element-row segments
+//   (4 entries per lane, aligned vector loads) and add each entry into its
+//   column slot in LDS (FP64), then write the A and D blocks
+//   (rawmat.cpp:67-125, 238-289).
 #include <hip/hip_runtime.h>
 #include "device.h"
 
 namespace slx {
+
+typedef float float4v __attribute__((ext_vector_type(4)));
+typedef double double2v __attribute__((ext_vector_type(2)));
+typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
+typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 
 // Jacobian, inverse and determinant of one (element, Gauss point).  Every
 // index below is a compile-time constant (loops fully unrolled, dimensions
@@ -338,109 +344,113 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
 }
 
 // Deterministic reduction of the element-row segments into the CSR A/D
-// values.  A lane group (group >= longest segment) owns one row at a time;
-// lanes hold consecutive entries of a segment, so within one segment every
-// slot differs and the LDS additions of a segment never collide.  Segments
-// are consumed in a fixed order, so every value is summed in the same order
-// on every run.  The groups are persistent and software-pipelined over their
-// rows: the first eight segments of the current row are loaded together, the
-// next row's segment descriptors and the metadata of the row after it are
-// already in flight while the current row accumulates.
-struct RowMeta {
-    int64_t a0, d0, k0, sp;
-    int lenA, lenD, nk;
+// values.  LPR lanes own one row; lane l holds entries 4l..4l+3 of the
+// current segment (one aligned 4-entry vector load of values and one of
+// slots), so within a segment every slot differs and the FP64 additions into
+// the row's LDS accumulator never collide.  Segments are consumed in their
+// fixed order, eight loaded ahead, so every value is summed in the same order
+// on every run.  Rows are taken in the plan's length order (rows of one
+// launch class have similar lengths; the LDS accumulator is sized per class).
+template <typename T>
+struct Vec4 {
+    T x[4];
 };
 
-__device__ inline RowMeta row_meta(const DevReduce& rd, int64_t r) {
-    RowMeta m;
-    m.a0 = rd.Arow[r];
-    m.d0 = rd.Drow[r];
-    m.k0 = rd.erow_ptr[r];
-    m.sp = rd.slot_ptr[r];
-    m.lenA = (int)(rd.Arow[r + 1] - m.a0);
-    m.lenD = (int)(rd.Drow[r + 1] - m.d0);
-    m.nk = (int)(rd.erow_ptr[r + 1] - m.k0);
-    return m;
+template <typename KT>
+__device__ inline Vec4<KT> load_vals(const KT* p) {
+    Vec4<KT> r;
+    if (sizeof(KT) == 4) {
+        const float4v v = *reinterpret_cast<const float4v*>(p);
+#pragma unroll
+        for (int c = 0; c < 4; c++) r.x[c] = (KT)v[c];
+    } else {
+        const double2v a = *reinterpret_cast<const double2v*>(p);
+        const double2v b = *reinterpret_cast<const double2v*>(p + 2);
+        r.x[0] = (KT)a[0];
+        r.x[1] = (KT)a[1];
+        r.x[2] = (KT)b[0];
+        r.x[3] = (KT)b[1];
+    }
+    return r;
+}
+
+template <typename ST>
+__device__ inline Vec4<ST> load_slots(const ST* p) {
+    Vec4<ST> r;
+    if (sizeof(ST) == 1) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+#pragma unroll
+        for (int c = 0; c < 4; c++) r.x[c] = (ST)((w >> (8 * c)) & 0xFFu);
+    } else if (sizeof(ST) == 2) {
+        const uint2v w = *reinterpret_cast<const uint2v*>(p);
+        r.x[0] = (ST)(w[0] & 0xFFFFu);
+        r.x[1] = (ST)(w[0] >> 16);
+        r.x[2] = (ST)(w[1] & 0xFFFFu);
+        r.x[3] = (ST)(w[1] >> 16);
+    } else {
+        const uint4v w = *reinterpret_cast<const uint4v*>(p);
+#pragma unroll
+        for (int c = 0; c < 4; c++) r.x[c] = (ST)w[c];
+    }
+    return r;
 }
 
 #define SLX_RED_BATCH 8
 
-template <typename KT, typename ST>
-__global__ void __launch_bounds__(256) reduce_matrix(DevReduce rd, const KT* __restrict__ ke, int group, double* __restrict__ Aval,
-                                                      double* __restrict__ Dval) {
+template <typename KT, typename ST, int LPR>
+__global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __restrict__ ke, int64_t pos0, int64_t pos1, int acc_len,
+                                                    double* __restrict__ Aval, double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
     double* accall = reinterpret_cast<double*>(smem_raw);
-    const int groups_per_block = blockDim.x / group;
-    const int gid = threadIdx.x / group, lane = threadIdx.x % group;
-    const int64_t stride = (int64_t)gridDim.x * groups_per_block;
-    int64_t r = (int64_t)blockIdx.x * groups_per_block + gid;
-    if (r >= rd.nrows) return;
-    double* acc = accall + (size_t)gid * rd.max_row_len;
+    const int rpb = blockDim.x / LPR;
+    const int gid = threadIdx.x / LPR, l = threadIdx.x % LPR;
+    const int64_t pos = pos0 + (int64_t)blockIdx.x * rpb + gid;
+    if (pos >= pos1) return;
+    const int64_t r = rd.row_order[pos];
+    double* acc = accall + (size_t)gid * acc_len;
     const ST* slots = reinterpret_cast<const ST*>(rd.slots);
     const ST SKIP = (ST)~(ST)0;
-    RowMeta cur = row_meta(rd, r);
-    uint64_t mine = lane < min(group, cur.nk) ? rd.erows[cur.k0 + lane] : 0ull;
-    RowMeta nxt = cur;
-    if (r + stride < rd.nrows) nxt = row_meta(rd, r + stride);
-    for (;;) {
-        const int64_t rn = r + stride, rn2 = r + 2 * stride;
-        int64_t sp = cur.sp;
-        // first batch of the row's segments
-        KT v[SLX_RED_BATCH];
-        ST sl[SLX_RED_BATCH];
-        const int nk0 = min(group, cur.nk);
+    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
+    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
+    const int64_t k0 = rd.erow_ptr[r];
+    const int nk = (int)(rd.erow_ptr[r + 1] - k0);
+    int64_t sp = rd.slot_ptr[r];
+    for (int x = l; x < lenA + lenD; x += LPR) acc[x] = 0.0;
+    for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
+        Vec4<KT> v[SLX_RED_BATCH];
+        Vec4<ST> sl[SLX_RED_BATCH];
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++) {
-            const uint64_t dsc = __shfl(mine, min(u, max(nk0 - 1, 0)), group);
-            const int n = (u < nk0) ? (int)(dsc >> 48) : 0;
-            const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-            const bool on = lane < n;
-            v[u] = on ? ke[off + lane] : (KT)0;
-            sl[u] = on ? slots[sp + lane] : SKIP;
-            sp += n;
+            int n = 0;
+            int64_t off = 0;
+            if (kb + u < nk) {
+                const uint64_t dsc = rd.erows[k0 + kb + u];
+                n = (int)(dsc >> 48);
+                off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+            }
+            if (4 * l < n) {
+                v[u] = load_vals<KT>(ke + off + 4 * l);
+                sl[u] = load_slots<ST>(slots + sp + 4 * l);
+            } else {
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    v[u].x[c] = (KT)0;
+                    sl[u].x[c] = SKIP;
+                }
+            }
+            sp += (n + 3) & ~3;
         }
-        // next row's descriptors, then the metadata two rows ahead
-        uint64_t mine_next = 0ull;
-        if (rn < rd.nrows) mine_next = lane < min(group, nxt.nk) ? rd.erows[nxt.k0 + lane] : 0ull;
-        RowMeta nxt2 = nxt;
-        if (rn2 < rd.nrows) nxt2 = row_meta(rd, rn2);
-        for (int x = lane; x < cur.lenA + cur.lenD; x += group) acc[x] = 0.0;
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++)
-            if (sl[u] != SKIP) acc[sl[u]] += (double)v[u];
-        // remaining segments of long rows
-        for (int64_t kb = 0; kb < cur.nk; kb += group) {
-            const int nk = (int)min((int64_t)group, (int64_t)cur.nk - kb);
-            const uint64_t desc = kb == 0 ? mine : (lane < nk ? rd.erows[cur.k0 + kb + lane] : 0ull);
-            for (int k = (kb == 0 ? SLX_RED_BATCH : 0); k < nk; k += 4) {
-                KT w[4];
-                ST sw[4];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const uint64_t dsc = __shfl(desc, min(k + u, nk - 1), group);
-                    const int n = (k + u < nk) ? (int)(dsc >> 48) : 0;
-                    const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-                    const bool on = lane < n;
-                    w[u] = on ? ke[off + lane] : (KT)0;
-                    sw[u] = on ? slots[sp + lane] : SKIP;
-                    sp += n;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (sw[u] != SKIP) acc[sw[u]] += (double)w[u];
-            }
-        }
-        for (int x = lane; x < cur.lenA; x += group) Aval[cur.a0 + x] = acc[x];
-        for (int x = lane; x < cur.lenD; x += group) Dval[cur.d0 + x] = acc[cur.lenA + x];
-        if (rn >= rd.nrows) break;
-        r = rn;
-        cur = nxt;
-        nxt = nxt2;
-        mine = mine_next;
+            for (int c = 0; c < 4; c++)
+                if (sl[u].x[c] != SKIP) acc[sl[u].x[c]] += (double)v[u].x[c];
     }
+    for (int x = l; x < lenA; x += LPR) Aval[a0 + x] = acc[x];
+    for (int x = l; x < lenD; x += LPR) Dval[d0 + x] = acc[lenA + x];
 }
 
-// Segments longer than 64 entries: lanes stride over the segment.
+// Segments longer than 64 entries: a wave per row, lanes stride over the segment.
 template <typename KT, typename ST>
 __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT* __restrict__ ke, double* __restrict__ Aval,
                                                            double* __restrict__ Dval) {
@@ -465,7 +475,7 @@ __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT
             const ST sl = slots[sp + j];
             if (sl != SKIP) acc[sl] += (double)ke[off + j];
         }
-        sp += n;
+        sp += (n + 3) & ~3;
     }
     for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
     for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
@@ -532,54 +542,66 @@ hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void
     return hipGetLastError();
 }
 
-int reduce_group_size(int max_erow_len) {
-    int g = 1;
-    while (g < max_erow_len && g < 64) g *= 2;
-    return g;
-}
-
-template <typename KT, typename ST>
-static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st) {
-    const bool longseg = max_erow_len > 64;
-    const int g = longseg ? 64 : reduce_group_size(max_erow_len);
-    int threads = 256, gpb = threads / g;
-    size_t lds = (size_t)gpb * rd.max_row_len * sizeof(double);
-    while (lds > 64 * 1024 && threads > g) {
-        threads /= 2;
-        gpb = threads / g;
-        lds = (size_t)gpb * rd.max_row_len * sizeof(double);
+template <typename KT, typename ST, int LPR>
+static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval, double* Dval, hipStream_t st, const int64_t* cb,
+                              const int64_t* ce, const int* cl, int nc) {
+    for (int c = 0; c < nc; c++) {
+        const int64_t nrows = ce[c] - cb[c];
+        if (nrows <= 0) continue;
+        int threads = 256;
+        size_t lds = (size_t)(threads / LPR) * cl[c] * sizeof(double);
+        while (lds > 64 * 1024 && threads > 64) {
+            threads /= 2;
+            lds = (size_t)(threads / LPR) * cl[c] * sizeof(double);
+        }
+        if (lds > 64 * 1024) return hipErrorInvalidValue;
+        const int rpb = threads / LPR;
+        const int64_t grid = (nrows + rpb - 1) / rpb;
+        hipLaunchKernelGGL((reduce_rows<KT, ST, LPR>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, cb[c], ce[c], cl[c],
+                           Aval, Dval);
     }
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
-    int64_t grid = (rd.nrows + gpb - 1) / gpb;
-    // persistent groups for the pipelined kernel: exactly the blocks that are
-    // resident at once (a larger grid would run its extra blocks as a tail)
-    if (!longseg) {
-        int per_cu = 0;
-        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&reduce_matrix<KT, ST>),
-                                                                    threads, lds);
-        if (e != hipSuccess || per_cu < 1) per_cu = 1;
-        grid = std::min<int64_t>(grid, (int64_t)rd.n_cu * per_cu);
-    }
-    if (longseg)
-        hipLaunchKernelGGL((reduce_matrix_long<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
-    else
-        hipLaunchKernelGGL((reduce_matrix<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, g, Aval, Dval);
     return hipGetLastError();
 }
 
+template <typename KT, typename ST>
+static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st,
+                                  const int64_t* cb, const int64_t* ce, const int* cl, int nc) {
+    if (max_erow_len > 64) {
+        int threads = 256, gpb = threads / 64;
+        size_t lds = (size_t)gpb * rd.max_row_len * sizeof(double);
+        while (lds > 64 * 1024 && threads > 64) {
+            threads /= 2;
+            gpb = threads / 64;
+            lds = (size_t)gpb * rd.max_row_len * sizeof(double);
+        }
+        if (lds > 64 * 1024) return hipErrorInvalidValue;
+        const int64_t grid = (rd.nrows + gpb - 1) / gpb;
+        hipLaunchKernelGGL((reduce_matrix_long<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
+        return hipGetLastError();
+    }
+    // lanes per row: 4 entries per lane
+    if (max_erow_len <= 4) return launch_rows<KT, ST, 1>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 8) return launch_rows<KT, ST, 2>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 16) return launch_rows<KT, ST, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 32) return launch_rows<KT, ST, 8>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    return launch_rows<KT, ST, 16>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+}
+
 template <typename KT>
-static hipError_t launch_reduce_k(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st) {
-    if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t>(rd, max_erow_len, ke, Aval, Dval, st);
-    if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t>(rd, max_erow_len, ke, Aval, Dval, st);
-    return launch_reduce_t<KT, uint32_t>(rd, max_erow_len, ke, Aval, Dval, st);
+static hipError_t launch_reduce_k(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st,
+                                  const int64_t* cb, const int64_t* ce, const int* cl, int nc) {
+    if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    return launch_reduce_t<KT, uint32_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
 }
 
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
-                         double* rhs, bool has_matrix, bool has_rhs, hipStream_t st) {
+                         double* rhs, bool has_matrix, bool has_rhs, hipStream_t st, const int64_t* class_begin,
+                         const int64_t* class_end, const int* class_len, int n_classes) {
     if (rd.nrows == 0) return hipSuccess;
     if (has_matrix) {
-        hipError_t err = ke_fp64 ? launch_reduce_k<double>(rd, max_erow_len, ke, Aval, Dval, st)
-                                 : launch_reduce_k<float>(rd, max_erow_len, ke, Aval, Dval, st);
+        hipError_t err = ke_fp64 ? launch_reduce_k<double>(rd, max_erow_len, ke, Aval, Dval, st, class_begin, class_end, class_len, n_classes)
+                                 : launch_reduce_k<float>(rd, max_erow_len, ke, Aval, Dval, st, class_begin, class_end, class_len, n_classes);
         if (err != hipSuccess) return err;
     }
     if (has_rhs) {

@@ -53,9 +53,10 @@ struct DevReduce {
     const int64_t* erow_ptr;        // [nrows+1] into erows
     const uint64_t* erows;          // packed (len << 48) | ke offset, in accumulation order
     const int64_t* slot_ptr;        // [nrows+1] into slots
-    const void* slots;              // per Ke entry of the row: A position, lenA + D position, or all ones
+    const void* slots;              // per segment entry (blocks padded to 4): A position, lenA + D position, or all ones
     int32_t slot_bytes;             // 1, 2 or 4
-    int32_t n_cu;                   // compute units (persistent grid size)
+    int32_t n_cu;                   // compute units
+    const int32_t* row_order;       // [nrows] rows by decreasing length (reduction work order)
     const int64_t* Arow;            // [nrows+1]
     const int64_t* Drow;            // [nrows+1]
     int32_t max_row_len;            // max (lenA + lenD)

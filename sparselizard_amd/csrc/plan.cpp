@@ -21,7 +21,8 @@ namespace slx {
 hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st);
 hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask);
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
-                         double* rhs, bool has_matrix, bool has_rhs, hipStream_t st);
+                         double* rhs, bool has_matrix, bool has_rhs, hipStream_t st, const int64_t* class_begin,
+                         const int64_t* class_end, const int* class_len, int n_classes);
 }  // namespace slx
 
 using namespace slx;
@@ -97,6 +98,11 @@ struct slx_ctx_s {
     hipStream_t stream = nullptr;
 };
 
+struct RedClass {
+    int64_t begin, end;  // positions in the row order
+    int max_len;         // longest row of the class (A + D columns)
+};
+
 struct slx_plan_s {
     slx_ctx ctx = nullptr;
     hipStream_t stream = nullptr;
@@ -117,7 +123,8 @@ struct slx_plan_s {
     std::vector<std::unique_ptr<DBuf>> bufs;
     std::vector<DevBucket> dbuckets;
     DBuf consts;
-    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_erow_ptr, d_erows, d_slot_ptr, d_slots, d_rhs_ptr, d_rhs_erows, d_free;
+    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_erow_ptr, d_erows, d_slot_ptr, d_slots, d_rhs_ptr, d_rhs_erows, d_free, d_order;
+    std::vector<RedClass> red_classes;  // reduction launch classes (rows of similar length)
     DBuf d_Aval, d_Dval, d_rhs, d_ke, d_ctab;
     size_t ke_capacity_bytes = 0;
     DevReduce rd{};
@@ -155,7 +162,10 @@ namespace {
 //   ke_base[s] + (e * n_tf + i) * ke_ld[s] + j,
 //   ke_base[s] = group base + column offset of s,  ke_ld[s] = total columns,
 // so one CSR row reads one contiguous segment per (element, tf function)
-// whatever the number of dof fields.  Slices whose dof address tables are
+// whatever the number of dof fields.  Matrix segments are strided by a
+// multiple of 4 entries (16 when 16 or longer) from a 16-aligned base: the
+// reduction reads them as aligned 4-entry vectors, and a 32-entry FP32
+// segment is exactly one 128-byte line.  The padding is never read into a sum.  Slices whose dof address tables are
 // identical are kept in separate groups (their columns would coincide).
 // Right-hand-side slices get one value per (element, tf function).
 struct SegGroup {
@@ -163,7 +173,8 @@ struct SegGroup {
     std::vector<int> slices;
     int n_tf;
     int64_t len;      // segment length (sum of the slices' n_dof, 1 for rhs)
-    int64_t base;     // Ke offset of the group
+    int64_t ld;       // segment stride: len, rounded up to a multiple of 16 when len >= 16
+    int64_t base;     // Ke offset of the group (a multiple of 16)
     int rank;         // smallest fragment rank of the group's slices
     bool rhs;
 };
@@ -236,12 +247,14 @@ KeLayout compute_layout(const slx_plan_desc& d) {
         for (size_t g = first; g < L.groups.size(); g++) {
             SegGroup& G = L.groups[g];
             SLX_CHECK(G.len < 65536, "element-row segment longer than 65535 entries");
+            G.ld = G.rhs ? G.len : (G.len >= 16 ? (G.len + 15) / 16 * 16 : (G.len + 3) / 4 * 4);
+            L.n_ke = (L.n_ke + 15) / 16 * 16;
             G.base = L.n_ke;
             for (int s : G.slices) {
                 L.ke_base[b][s] += G.base;
-                L.ke_ld[b][s] = G.len;
+                L.ke_ld[b][s] = G.ld;
             }
-            L.n_ke += (int64_t)bd.n_elem * G.n_tf * G.len;
+            L.n_ke += (int64_t)bd.n_elem * G.n_tf * G.ld;
         }
     }
     return L;
@@ -301,7 +314,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
                     int32_t a = sd.tf_addr[(size_t)i * bd.n_elem + e];
                     if (a < 0 || (d.constrained && d.constrained[a])) continue;
                     int64_t pos = fill[freeidx[a]]++;
-                    int64_t off = G.base + ((int64_t)e * G.n_tf + i) * G.len;
+                    int64_t off = G.base + ((int64_t)e * G.n_tf + i) * G.ld;
                     erows[pos] = ((uint64_t)G.len << 48) | (uint64_t)off;
                     if (ref_idx) (*ref_idx)[pos] = (int32_t)ri;
                     if (elem_idx) (*elem_idx)[pos] = e;
@@ -315,15 +328,20 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     collect(mat, erow_ptr, erows, &eref, &eelem);
     collect(rhs, rhs_ptr, rhs_erows, nullptr, nullptr);
     P.n_erows = (int64_t)erows.size();
-    // slot pointers
+    // slot pointers: every segment's slot block padded to 4 entries (aligned vector loads)
     std::vector<int64_t> slot_ptr(nf + 1, 0);
+    int64_t n_coo = 0;
     for (int64_t r = 0; r < nf; r++) {
         int64_t s = 0;
-        for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) s += (int64_t)(erows[k] >> 48);
+        for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
+            const int64_t len = (int64_t)(erows[k] >> 48);
+            s += (len + 3) / 4 * 4;
+            n_coo += len;
+        }
         slot_ptr[r + 1] = slot_ptr[r] + s;
     }
-    P.n_coo = slot_ptr[nf];
-    std::vector<int32_t> slots(slot_ptr[nf]);
+    P.n_coo = n_coo;
+    std::vector<int32_t> slots(slot_ptr[nf], -1);
     std::vector<int32_t> lenA(nf), lenD(nf);
     int nthreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (nf < 20000) nthreads = 1;
@@ -386,7 +404,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
                 for (int j = 0; j < len; j++)
                     for (int j2 = j + 1; j2 < len; j2++)
                         if (slots[sp + j] >= 0 && slots[sp + j] == slots[sp + j2]) dup_error = true;
-                sp += len;
+                sp += (len + 3) / 4 * 4;
             }
         }
     };
@@ -420,6 +438,20 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             P.h_ke_base.push_back(L.ke_base[b][s]);
             P.h_ke_ld.push_back(L.ke_ld[b][s]);
         }
+    // reduction work order: rows by decreasing length (stable), in classes whose
+    // longest row is at most twice the shortest class bound (LDS sized per class)
+    std::vector<int32_t> order(nf);
+    for (int64_t r = 0; r < nf; r++) order[r] = (int32_t)r;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return lenA[a] + lenD[a] > lenA[b] + lenD[b]; });
+    P.red_classes.clear();
+    for (int64_t i = 0; i < nf;) {
+        const int top = std::max(1, lenA[order[i]] + lenD[order[i]]);
+        int64_t j = i;
+        while (j < nf && 2 * (lenA[order[j]] + lenD[order[j]]) > top) j++;
+        if (j == i) j = i + 1;
+        P.red_classes.push_back({i, j, top});
+        i = j;
+    }
     if (P.host_only) {
         P.h_erow_ptr = std::move(erow_ptr);
         P.h_erows = std::move(erows);
@@ -454,6 +486,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
+    upload(P.d_order, order.data(), order.size(), st);
     P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
     P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
     // rhs: constrained entries hold the Dirichlet values (formulation::b(),
@@ -472,6 +505,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.rd.slots = P.d_slots.p;
     P.rd.slot_bytes = slot_bytes;
     P.rd.n_cu = P.n_cu;
+    P.rd.row_order = P.d_order.as<int32_t>();
     P.rd.Arow = P.d_Arow.as<int64_t>();
     P.rd.Drow = P.d_Drow.as<int64_t>();
     P.rd.max_row_len = maxrow;
@@ -578,7 +612,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         if (k == (int)gl.size()) {
             gl.push_back(g);
             gb.push_back(L.groups[g].base);
-            gd.push_back((int32_t)L.groups[g].len);
+            gd.push_back((int32_t)L.groups[g].ld);
         }
         sgrp[si] = k;
         scol[si] = (int32_t)(L.ke_base[b][ms[si]] - L.groups[g].base);
@@ -775,7 +809,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
 }
 
 // Per-launch HIP events by kernel class (profiling only).
-enum { T_COEF = 0, T_MFMA = 1, T_VALU = 2, T_REDUCE = 3, T_COUNT = 4 };
+enum { T_COEF = 0, T_MFMA = 1, T_VALU = 2, T_REDUCE = 3, T_RHS = 4, T_COUNT = 5 };
 struct LaunchTimer {
     hipStream_t st;
     std::vector<hipEvent_t> pool;
@@ -845,10 +879,25 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
             if (tm) tm->end();
         }
     }
-    if (tm) tm->begin(T_REDUCE);
-    HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
-                           P.has_matrix, P.has_rhs, st));
-    if (tm) tm->end();
+    if (P.has_matrix) {
+        if (tm) tm->begin(T_REDUCE);
+        std::vector<int64_t> cb, ce;
+        std::vector<int> cl;
+        for (const RedClass& c : P.red_classes) {
+            cb.push_back(c.begin);
+            ce.push_back(c.end);
+            cl.push_back(c.max_len);
+        }
+        HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
+                               true, false, st, cb.data(), ce.data(), cl.data(), (int)cl.size()));
+        if (tm) tm->end();
+    }
+    if (P.has_rhs) {
+        if (tm) tm->begin(T_RHS);
+        HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
+                               false, true, st, nullptr, nullptr, nullptr, 0));
+        if (tm) tm->end();
+    }
     P.last_precision = precision;
 }
 
@@ -975,7 +1024,7 @@ slx_status slx_plan_profile(slx_plan p, const double* d_node_xyz, const double* 
         SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         LaunchTimer tm;
         tm.st = p->stream;
-        double acc[T_COUNT] = {0, 0, 0, 0};
+        double acc[T_COUNT] = {0, 0, 0, 0, 0};
         for (int it = 0; it < iters; it++) {
             numeric(*p, d_node_xyz, d_params, param_len, d_field_data, precision, p->stream, &tm);
             tm.collect(acc);

@@ -39,8 +39,11 @@ def reduce_host(maps, ke, Arow, Drow, shape):
     within = np.arange(lens.sum()) - np.repeat(np.cumsum(lens) - lens, lens)
     kidx = starts + within
     rows = np.repeat(rows_of_seg, lens)
-    slots = maps["slots"]
-    assert len(slots) == len(kidx)
+    # slot blocks are padded to 4 entries per segment
+    padded = (lens + 3) // 4 * 4
+    sstart = np.repeat(np.cumsum(padded) - padded, lens)
+    slots = maps["slots"][sstart + within]
+    assert len(maps["slots"]) == padded.sum()
     lenA = np.diff(Arow)
     on = slots >= 0
     isA = on & (slots < lenA[rows])
