@@ -25,6 +25,7 @@ typedef float float4v __attribute__((ext_vector_type(4)));
 typedef double double2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
+typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
 
 // Jacobian, inverse and determinant of one (element, Gauss point).  Every
 // index below is a compile-time constant (loops fully unrolled, dimensions
@@ -397,6 +398,26 @@ __device__ inline Vec4<ST> load_slots(const ST* p) {
 
 #define SLX_RED_BATCH 8
 
+struct RowDesc {
+    int64_t a0, d0, k0, sp;
+    int lenA, lenD, nk;
+};
+
+__device__ inline RowDesc load_rowdesc(const uint64_t* rdesc, int64_t pos) {
+    const uint64_t M48 = 0xFFFFFFFFFFFFull;
+    const u64x2 w01 = *reinterpret_cast<const u64x2*>(rdesc + 4 * pos);
+    const u64x2 w23 = *reinterpret_cast<const u64x2*>(rdesc + 4 * pos + 2);
+    RowDesc r;
+    r.a0 = (int64_t)(w01[0] & M48);
+    r.lenA = (int)(w01[0] >> 48);
+    r.d0 = (int64_t)(w01[1] & M48);
+    r.lenD = (int)(w01[1] >> 48);
+    r.k0 = (int64_t)(w23[0] & M48);
+    r.nk = (int)(w23[0] >> 48);
+    r.sp = (int64_t)w23[1];
+    return r;
+}
+
 template <typename KT, typename ST, int LPR>
 __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __restrict__ ke, int64_t pos0, int64_t pos1, int acc_len,
                                                     double* __restrict__ Aval, double* __restrict__ Dval) {
@@ -406,15 +427,13 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
     const int gid = threadIdx.x / LPR, l = threadIdx.x % LPR;
     const int64_t pos = pos0 + (int64_t)blockIdx.x * rpb + gid;
     if (pos >= pos1) return;
-    const int64_t r = rd.row_order[pos];
-    double* acc = accall + (size_t)gid * acc_len;
+    double* acc = accall + (size_t)gid * (acc_len | 1);  // odd stride: the rows' accumulators start on different banks
     const ST* slots = reinterpret_cast<const ST*>(rd.slots);
     const ST SKIP = (ST)~(ST)0;
-    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
-    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
-    const int64_t k0 = rd.erow_ptr[r];
-    const int nk = (int)(rd.erow_ptr[r + 1] - k0);
-    int64_t sp = rd.slot_ptr[r];
+    const RowDesc rw = load_rowdesc(rd.rdesc, pos);
+    const int64_t a0 = rw.a0, d0 = rw.d0, k0 = rw.k0;
+    const int lenA = rw.lenA, lenD = rw.lenD, nk = rw.nk;
+    int64_t sp = rw.sp;
     for (int x = l; x < lenA + lenD; x += LPR) acc[x] = 0.0;
     for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
         Vec4<KT> v[SLX_RED_BATCH];
@@ -440,11 +459,15 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
             }
             sp += (n + 3) & ~3;
         }
+        // LDS FP64 add without return (ds_add_f64): no read-after-write wait per
+        // update; one wave's LDS operations complete in issue order and the
+        // lanes of one segment never share a slot, so the order of the
+        // additions into every slot is the program order (deterministic).
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++)
 #pragma unroll
             for (int c = 0; c < 4; c++)
-                if (sl[u].x[c] != SKIP) acc[sl[u].x[c]] += (double)v[u].x[c];
+                if (sl[u].x[c] != SKIP) (void)atomicAdd(&acc[sl[u].x[c]], (double)v[u].x[c]);
     }
     for (int x = l; x < lenA; x += LPR) Aval[a0 + x] = acc[x];
     for (int x = l; x < lenD; x += LPR) Dval[d0 + x] = acc[lenA + x];
@@ -458,22 +481,23 @@ __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT
     double* accall = reinterpret_cast<double*>(smem_raw);
     const int group = 64;
     const int gid = threadIdx.x / group, lane = threadIdx.x % group;
-    const int64_t r = (int64_t)blockIdx.x * (blockDim.x / group) + gid;
-    if (r >= rd.nrows) return;
+    const int64_t pos = (int64_t)blockIdx.x * (blockDim.x / group) + gid;
+    if (pos >= rd.nrows) return;
     double* acc = accall + (size_t)gid * rd.max_row_len;
     const ST* slots = reinterpret_cast<const ST*>(rd.slots);
     const ST SKIP = (ST)~(ST)0;
-    const int64_t a0 = rd.Arow[r], d0 = rd.Drow[r];
-    const int lenA = (int)(rd.Arow[r + 1] - a0), lenD = (int)(rd.Drow[r + 1] - d0);
+    const RowDesc rw = load_rowdesc(rd.rdesc, pos);
+    const int64_t a0 = rw.a0, d0 = rw.d0;
+    const int lenA = rw.lenA, lenD = rw.lenD;
     for (int x = lane; x < lenA + lenD; x += group) acc[x] = 0.0;
-    int64_t sp = rd.slot_ptr[r];
-    for (int64_t k = rd.erow_ptr[r]; k < rd.erow_ptr[r + 1]; k++) {
+    int64_t sp = rw.sp;
+    for (int64_t k = rw.k0; k < rw.k0 + rw.nk; k++) {
         const uint64_t dsc = rd.erows[k];
         const int n = (int)(dsc >> 48);
         const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
         for (int j = lane; j < n; j += group) {
             const ST sl = slots[sp + j];
-            if (sl != SKIP) acc[sl] += (double)ke[off + j];
+            if (sl != SKIP) (void)atomicAdd(&acc[sl], (double)ke[off + j]);
         }
         sp += (n + 3) & ~3;
     }
@@ -549,10 +573,10 @@ static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval,
         const int64_t nrows = ce[c] - cb[c];
         if (nrows <= 0) continue;
         int threads = 256;
-        size_t lds = (size_t)(threads / LPR) * cl[c] * sizeof(double);
+        size_t lds = (size_t)(threads / LPR) * (cl[c] | 1) * sizeof(double);
         while (lds > 64 * 1024 && threads > 64) {
             threads /= 2;
-            lds = (size_t)(threads / LPR) * cl[c] * sizeof(double);
+            lds = (size_t)(threads / LPR) * (cl[c] | 1) * sizeof(double);
         }
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         const int rpb = threads / LPR;

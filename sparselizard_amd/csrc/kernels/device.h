@@ -50,15 +50,13 @@ struct DevInputs {
 // Reduction maps of the cached CSR pattern.
 struct DevReduce {
     int64_t nrows;                  // free rows
-    const int64_t* erow_ptr;        // [nrows+1] into erows
-    const uint64_t* erows;          // packed (len << 48) | ke offset, in accumulation order
-    const int64_t* slot_ptr;        // [nrows+1] into slots
+    // per work position (rows by decreasing length), 4 words:
+    //   Arow | lenA << 48,  Drow | lenD << 48,  first segment | n_segments << 48,  first slot
+    const uint64_t* rdesc;
+    const uint64_t* erows;          // packed (len << 48) | ke offset; a row's segments consecutive, rows in work order
     const void* slots;              // per segment entry (blocks padded to 4): A position, lenA + D position, or all ones
     int32_t slot_bytes;             // 1, 2 or 4
     int32_t n_cu;                   // compute units
-    const int32_t* row_order;       // [nrows] rows by decreasing length (reduction work order)
-    const int64_t* Arow;            // [nrows+1]
-    const int64_t* Drow;            // [nrows+1]
     int32_t max_row_len;            // max (lenA + lenD)
     const int64_t* rhs_erow_ptr;    // [nrows+1]
     const uint64_t* rhs_erows;      // packed (1 << 48) | ke offset

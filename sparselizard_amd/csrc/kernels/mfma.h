@@ -21,6 +21,7 @@ struct MfmaBucket {
     const void* afrag;          // [mtiles][ksteps][2 (hi, lo')][64 lanes] x 8 halfs
     const int32_t* csrc;        // [n_mslices][n_pairs] coefficient slot of (slice, pair), or -1
     int32_t n_groups, max_gld;  // segment groups of the slices, longest group segment
+    int32_t stg_words;          // LDS staging words per element of all groups: sum (n_tf * ld + 4)
     const int64_t* gbase;       // [n_groups] Ke base of each group
     const int32_t* gld;         // [n_groups] segment length of each group
     const int32_t* sgrp;        // [n_mslices] group of each slice
@@ -29,7 +30,9 @@ struct MfmaBucket {
 };
 
 size_t mfma_lds_bytes(const MfmaBucket& mb);
-bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk);
+// Waves per workgroup and M tiles per wave: one tile per wave up to 8 waves,
+// or (prefer_mtw == 2) two tiles per wave so each B fragment feeds 6 MFMAs.
+bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw);
 hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st);
 hipError_t launch_mfma_probe(float* out, hipStream_t st);
 

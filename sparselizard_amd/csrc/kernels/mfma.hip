@@ -74,6 +74,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
     // A fragments of this wave's M tiles (tile = wave + nw * i)
     half8 Ahi[MTW][MAXK], Alo[MTW][MAXK];
     int irow[MTW][4], jrow[MTW][4];
+    bool rowok[MTW][4];
 #pragma unroll
     for (int i = 0; i < MTW; i++) {
         const int mt = wave + nw * i;
@@ -96,6 +97,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
             const int row = mt * 16 + 4 * lgrp + v;
             irow[i][v] = row / mb.n_dof;
             jrow[i][v] = row - irow[i][v] * mb.n_dof;
+            rowok[i][v] = row < mb.M;
         }
     }
     // compact columns: the pair word of this lane's eight k per K step
@@ -109,7 +111,8 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
         const int e0 = tile * NE;
         const int nloc = min(NE, b.n_elem - e0);
         __syncthreads();
-        // columns: one thread per (element, slice), element fastest
+        // columns: one thread per (element, slice), element fastest: power-of-two
+        // scale and packed hi/lo' words
         for (int col = threadIdx.x; col < NE * ns; col += blockDim.x) {
             const int el = col % NE, s = col / NE;
             const bool valid = el < nloc;
@@ -128,8 +131,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
             for (int p = 0; p < np; p++) {
                 const int c = csrc[s * np + p];
                 if (COMPACT) {
-                    const float x = (c >= 0 && valid) ? ctab[(int64_t)c * nel + e] * sc : 0.f;
-                    dst[p] = pack_split(x);
+                    dst[p] = pack_split((c >= 0 && valid) ? ctab[(int64_t)c * nel + e] * sc : 0.f);
                 } else {
                     for (int q = 0; q < ngp; q++) {
                         const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
@@ -142,85 +144,119 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                 for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
         }
         __syncthreads();
-        // contraction: per 16-element column tile and segment group; the
-        // wave's M tiles of every slice of the group go to the LDS block
+        // contraction: per 16-element column tile, the wave's M tiles of every
+        // slice go to the LDS block of the slice's segment group; then the
+        // blocks of all groups are written out (two barriers per column tile)
         const int nct = (NE + 15) / 16;
         for (int ct = 0; ct < nct; ct++) {
             const int el = ct * 16 + lrow;  // this lane's B / D column
-            for (int g = 0; g < mb.n_groups; g++) {
-                const int ld = gld[g];
-                const int erow = n_tf * ld;  // floats per element in the group block
-                for (int s = 0; s < ns; s++) {
-                    if (sgrp[s] != g) continue;
-                    const int col0 = scol[s];
-                    float4v acc_hh[MTW], acc_x[MTW];
+            // slices two at a time: six independent accumulator chains per
+            // M tile (hi*hi, hi*lo', lo'*hi for each slice) share the A fragments
+            for (int s0 = 0; s0 < ns; s0 += 2) {
+                const bool two = s0 + 1 < ns;
+                float4v acc_hh[2][MTW], acc_xa[2][MTW], acc_xb[2][MTW];
+#pragma unroll
+                for (int c = 0; c < 2; c++)
 #pragma unroll
                     for (int i = 0; i < MTW; i++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            acc_hh[i][v] = 0.f;
-                            acc_x[i][v] = 0.f;
+                            acc_hh[c][i][v] = 0.f;
+                            acc_xa[c][i][v] = 0.f;
+                            acc_xb[c][i][v] = 0.f;
                         }
-                    const uint32_t* colbase = Cs + (size_t)el * estride + (size_t)s * cstride;
+                const uint32_t* colbase0 = Cs + (size_t)el * estride + (size_t)s0 * cstride;
+                const uint32_t* colbase1 = colbase0 + (two ? cstride : 0);
 #pragma unroll
-                    for (int ks = 0; ks < MAXK; ks++) {
+                for (int ks = 0; ks < MAXK; ks++) {
+                    half8 bhi[2], blo[2];
+#pragma unroll
+                    for (int c = 0; c < 2; c++) {
+                        const uint32_t* colbase = c == 0 ? colbase0 : colbase1;
                         uint4v hw, lw;
                         if (COMPACT) {
+                            // one packed word (hi | lo' << 16) -> both halves broadcast
+                            // to all eight k of the lane (byte permutes)
                             const uint32_t u = colbase[pword[ks]];
-                            const uint32_t h = u & 0xFFFFu, l = u >> 16;
-                            const uint32_t h2 = h | (h << 16), l2 = l | (l << 16);
+                            const uint32_t h2 = __builtin_amdgcn_perm(u, u, 0x01000100u);
+                            const uint32_t l2 = __builtin_amdgcn_perm(u, u, 0x03020302u);
                             hw = (uint4v){h2, h2, h2, h2};
                             lw = (uint4v){l2, l2, l2, l2};
                         } else {
                             const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
                             const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
-                            hw[0] = (x0[0] & 0xFFFFu) | (x0[1] << 16);
-                            hw[1] = (x0[2] & 0xFFFFu) | (x0[3] << 16);
-                            hw[2] = (x1[0] & 0xFFFFu) | (x1[1] << 16);
-                            hw[3] = (x1[2] & 0xFFFFu) | (x1[3] << 16);
-                            lw[0] = (x0[0] >> 16) | (x0[1] & 0xFFFF0000u);
-                            lw[1] = (x0[2] >> 16) | (x0[3] & 0xFFFF0000u);
-                            lw[2] = (x1[0] >> 16) | (x1[1] & 0xFFFF0000u);
-                            lw[3] = (x1[2] >> 16) | (x1[3] & 0xFFFF0000u);
+                            hw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x05040100u);
+                            hw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x05040100u);
+                            hw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x05040100u);
+                            hw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x05040100u);
+                            lw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x07060302u);
+                            lw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x07060302u);
+                            lw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x07060302u);
+                            lw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x07060302u);
                         }
-                        const half8 bhi = __builtin_bit_cast(half8, hw);
-                        const half8 blo = __builtin_bit_cast(half8, lw);
+                        bhi[c] = __builtin_bit_cast(half8, hw);
+                        blo[c] = __builtin_bit_cast(half8, lw);
+                    }
 #pragma unroll
-                        for (int i = 0; i < MTW; i++) {
-                            acc_hh[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi, acc_hh[i], 0, 0, 0);
-                            acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo, acc_x[i], 0, 0, 0);
-                            acc_x[i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[i], 0, 0, 0);
+                    for (int i = 0; i < MTW; i++) {
+                        if (i > 0 && wave + nw * i >= mb.mtiles) continue;  // wave-uniform: no work for an empty tile
+                        acc_hh[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi[0], acc_hh[0][i], 0, 0, 0);
+                        acc_xa[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo[0], acc_xa[0][i], 0, 0, 0);
+                        acc_xb[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi[0], acc_xb[0][i], 0, 0, 0);
+                        if (two) {
+                            acc_hh[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi[1], acc_hh[1][i], 0, 0, 0);
+                            acc_xa[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo[1], acc_xa[1][i], 0, 0, 0);
+                            acc_xb[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi[1], acc_xb[1][i], 0, 0, 0);
                         }
                     }
+                }
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (c == 1 && !two) break;
+                    const int s = s0 + c;
+                    const int g = sgrp[s];
+                    float* gstg = stg;  // LDS block of group g: [16][n_tf * ld + 4]
+                    for (int g2 = 0; g2 < g; g2++) gstg += (size_t)16 * (n_tf * gld[g2] + 4);
+                    const int ld = gld[g], srow = n_tf * ld + 4;
                     const float colscale = colsc[s * NE + el];
-                    float* out = stg + (size_t)lrow * erow + col0;
+                    const float xscale = colscale * (1.f / 2048.f);
+                    float* out = gstg + (size_t)lrow * srow + scol[s];
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
                         const int mt = wave + nw * i;
                         if (mt >= mb.mtiles) continue;
 #pragma unroll
-                        for (int v = 0; v < 4; v++) {
-                            const int row = mt * 16 + 4 * lgrp + v;
-                            if (row < mb.M) out[irow[i][v] * ld + jrow[i][v]] = (acc_hh[i][v] + acc_x[i][v] * (1.f / 2048.f)) * colscale;
+                        for (int v = 0; v < 4; v++)
+                            if (rowok[i][v])
+                                out[irow[i][v] * ld + jrow[i][v]] = acc_hh[c][i][v] * colscale + (acc_xa[c][i][v] + acc_xb[c][i][v]) * xscale;
+                    }
+                }
+            }
+            __syncthreads();
+            // each group's rows of these 16 elements are one contiguous range
+            const int nel_here = min(16, nloc - ct * 16);
+            float* gsrc = stg;
+            for (int g = 0; g < mb.n_groups; g++) {
+                const int erow = n_tf * gld[g], srow = erow + 4;
+                if (nel_here > 0) {
+                    float* dst = ke + gbase[g] + (int64_t)(e0 + ct * 16) * erow;
+                    if (((uintptr_t)dst & 15u) == 0 && (erow & 3) == 0) {
+                        const int per = erow / 4;  // float4 per element row
+                        float4v* d4 = reinterpret_cast<float4v*>(dst);
+                        for (int x = threadIdx.x; x < nel_here * per; x += blockDim.x) {
+                            const int eo = x / per, w = x - eo * per;
+                            d4[x] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
+                        }
+                    } else {
+                        for (int x = threadIdx.x; x < nel_here * erow; x += blockDim.x) {
+                            const int eo = x / erow, w = x - eo * erow;
+                            dst[x] = gsrc[(size_t)eo * srow + w];
                         }
                     }
                 }
-                __syncthreads();
-                // the group's rows of these 16 elements are one contiguous range
-                const int nel_here = min(16, nloc - ct * 16);
-                if (nel_here > 0) {
-                    float* dst = ke + gbase[g] + (int64_t)(e0 + ct * 16) * erow;
-                    const int total = nel_here * erow;
-                    if (((uintptr_t)dst & 15u) == 0 && (erow & 3) == 0) {
-                        float4v* d4 = reinterpret_cast<float4v*>(dst);
-                        const float4v* s4 = reinterpret_cast<const float4v*>(stg);
-                        for (int x = threadIdx.x; x < total / 4; x += blockDim.x) d4[x] = s4[x];
-                    } else {
-                        for (int x = threadIdx.x; x < total; x += blockDim.x) dst[x] = stg[x];
-                    }
-                }
-                __syncthreads();
+                gsrc += (size_t)16 * srow;
             }
+            __syncthreads();
         }
     }
 }
@@ -238,12 +274,13 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
 }
 
 size_t mfma_lds_bytes(const MfmaBucket& mb) {
-    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.n_tf * mb.max_gld * 4;
+    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.stg_words * 4;
 }
 
-bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk) {
+bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw) {
     if (ksteps < 1 || ksteps > 8 || mtiles < 1) return false;
     int w = mtiles <= 8 ? mtiles : 8;
+    if (prefer_mtw == 2 && mtiles > 1) w = (mtiles + 1) / 2;
     int t = (mtiles + w - 1) / w;
     if (t > 2) return false;
     *nw = w;

@@ -2,6 +2,7 @@
 // device buffers of the numeric phase, plus the C ABI of include/slx.h.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstdlib>
 #include <atomic>
 #include <cstring>
 #include <memory>
@@ -123,7 +124,7 @@ struct slx_plan_s {
     std::vector<std::unique_ptr<DBuf>> bufs;
     std::vector<DevBucket> dbuckets;
     DBuf consts;
-    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_erow_ptr, d_erows, d_slot_ptr, d_slots, d_rhs_ptr, d_rhs_erows, d_free, d_order;
+    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_rdesc, d_erows, d_slots, d_rhs_ptr, d_rhs_erows, d_free;
     std::vector<RedClass> red_classes;  // reduction launch classes (rows of similar length)
     DBuf d_Aval, d_Dval, d_rhs, d_ke, d_ctab;
     size_t ke_capacity_bytes = 0;
@@ -467,26 +468,49 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     upload(P.d_Drow, P.Drow.data(), P.Drow.size(), st);
     upload(P.d_Acol, P.Acol.data(), P.Acol.size(), st);
     upload(P.d_Dcol, P.Dcol.data(), P.Dcol.size(), st);
-    upload(P.d_erow_ptr, erow_ptr.data(), erow_ptr.size(), st);
-    upload(P.d_erows, erows.data(), erows.size(), st);
-    upload(P.d_slot_ptr, slot_ptr.data(), slot_ptr.size(), st);
-    // slots in the narrowest type that holds every row length (all ones = skip)
-    int slot_bytes = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
-    if (slot_bytes == 1) {
-        std::vector<uint8_t> v(slots.size());
-        for (size_t i = 0; i < v.size(); i++) v[i] = slots[i] < 0 ? 0xFF : (uint8_t)slots[i];
-        upload(P.d_slots, v.data(), v.size(), st);
-    } else if (slot_bytes == 2) {
-        std::vector<uint16_t> v(slots.size());
-        for (size_t i = 0; i < v.size(); i++) v[i] = slots[i] < 0 ? 0xFFFF : (uint16_t)slots[i];
-        upload(P.d_slots, v.data(), v.size(), st);
-    } else {
-        upload(P.d_slots, slots.data(), slots.size(), st);
+    // reduction streams in work order: a 4-word descriptor per position, and the
+    // segments and slot blocks of consecutive positions stored consecutively, so
+    // the rows one block reduces read contiguous metadata
+    {
+        std::vector<uint64_t> rdesc((size_t)nf * 4), erows_w(erows.size());
+        std::vector<int64_t> slot_w(nf);  // work-order start of each position's slots
+        int64_t kw = 0, sw = 0;
+        for (int64_t p = 0; p < nf; p++) {
+            const int64_t r = order[p];
+            const int64_t nk = erow_ptr[r + 1] - erow_ptr[r];
+            SLX_CHECK(nk < 65536 && lenA[r] < 65536 && lenD[r] < 65536, "a row has more than 65535 segments or columns");
+            rdesc[4 * p + 0] = (uint64_t)P.Arow[r] | ((uint64_t)lenA[r] << 48);
+            rdesc[4 * p + 1] = (uint64_t)P.Drow[r] | ((uint64_t)lenD[r] << 48);
+            rdesc[4 * p + 2] = (uint64_t)kw | ((uint64_t)nk << 48);
+            rdesc[4 * p + 3] = (uint64_t)sw;
+            std::copy(erows.begin() + erow_ptr[r], erows.begin() + erow_ptr[r + 1], erows_w.begin() + kw);
+            slot_w[p] = sw;
+            kw += nk;
+            sw += slot_ptr[r + 1] - slot_ptr[r];
+        }
+        SLX_CHECK(P.Arow[nf] < (int64_t(1) << 48) && P.Drow[nf] < (int64_t(1) << 48), "matrix exceeds the 48-bit index range");
+        upload(P.d_rdesc, rdesc.data(), rdesc.size(), st);
+        upload(P.d_erows, erows_w.data(), erows_w.size(), st);
+        // slots in the narrowest type that holds every row length (all ones = skip)
+        auto narrow = [&](auto tag) {
+            using T = decltype(tag);
+            std::vector<T> v(slots.size());
+            for (int64_t p = 0; p < nf; p++) {
+                const int64_t r = order[p];
+                T* dst = v.data() + slot_w[p];
+                for (int64_t i = slot_ptr[r]; i < slot_ptr[r + 1]; i++) *dst++ = slots[i] < 0 ? (T)~(T)0 : (T)slots[i];
+            }
+            upload(P.d_slots, v.data(), v.size(), st);
+        };
+        const int sb = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
+        if (sb == 1) narrow(uint8_t{});
+        else if (sb == 2) narrow(uint16_t{});
+        else narrow(uint32_t{});
     }
+    int slot_bytes = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
-    upload(P.d_order, order.data(), order.size(), st);
     P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
     P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
     // rhs: constrained entries hold the Dirichlet values (formulation::b(),
@@ -499,15 +523,11 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         upload(P.d_rhs, b0.data(), b0.size(), st);
     }
     P.rd.nrows = nf;
-    P.rd.erow_ptr = P.d_erow_ptr.as<int64_t>();
+    P.rd.rdesc = P.d_rdesc.as<uint64_t>();
     P.rd.erows = P.d_erows.as<uint64_t>();
-    P.rd.slot_ptr = P.d_slot_ptr.as<int64_t>();
     P.rd.slots = P.d_slots.p;
     P.rd.slot_bytes = slot_bytes;
     P.rd.n_cu = P.n_cu;
-    P.rd.row_order = P.d_order.as<int32_t>();
-    P.rd.Arow = P.d_Arow.as<int64_t>();
-    P.rd.Drow = P.d_Drow.as<int64_t>();
     P.rd.max_row_len = maxrow;
     P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();
     P.rd.rhs_erows = P.d_rhs_erows.as<uint64_t>();
@@ -595,7 +615,10 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
     mb.n_mslices = (int)ms.size();
-    if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk)) return;
+    // tile split: SLX_TUNE_MFMA_MTW=2 selects two M tiles per wave (tuning knob)
+    int prefer_mtw = 1;
+    if (const char* t = std::getenv("SLX_TUNE_MFMA_MTW")) prefer_mtw = std::atoi(t);
+    if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk, prefer_mtw)) return;
     mb.cstride = mb.compact ? mb.n_pairs + 1 : 32 * mb.ksteps;
     int es = mb.n_mslices * mb.cstride;
     if (mb.compact) es |= 1;  // odd: conflict-free 32-bit reads across elements
@@ -623,10 +646,13 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
             if (std::find(ms.begin(), ms.end(), s2) == ms.end()) return;
     mb.n_groups = (int)gl.size();
     mb.max_gld = *std::max_element(gd.begin(), gd.end());
+    mb.stg_words = 0;
+    for (int32_t l : gd) mb.stg_words += n_tf * l + 4;
+    // element tile: two workgroups per CU must fit the 160 KB of LDS
     mb.ne_tile = 0;
     for (int ne : {64, 32, 16}) {
         mb.ne_tile = ne;
-        if (mfma_lds_bytes(mb) <= 64 * 1024) break;
+        if (mfma_lds_bytes(mb) <= 80 * 1024) break;
         mb.ne_tile = 0;
     }
     if (mb.ne_tile == 0) return;
