@@ -93,6 +93,15 @@ typedef struct slx_slice_desc {
     int32_t n_terms;
     const slx_term* terms;
     int32_t tf_table, dof_table; /* indices into the bucket's shape tables */
+    /* Orientation folding (NULL = every sign +1).  A bucket may hold elements of
+     * several total orientations when each orientation's basis is a signed
+     * renumbering of orientation 0 (hexahedra, quadrangles): the tables are those
+     * of orientation 0, the address tables are already renumbered, and
+     * sign[i][e] = +1/-1 multiplies row i (tf) or column i (dof) of element e's
+     * matrix.  No reference counterpart: the reference keeps one batch per total
+     * orientation (elementselector.cpp:4-133); folding gives the same sums. */
+    const int8_t* tf_sign;    /* [n_tf][n_elem] or NULL */
+    const int8_t* dof_sign;   /* [n_dof][n_elem] or NULL */
 } slx_slice_desc;
 
 /* A bucket: elements of one type sharing Gauss rule, orders and (when the

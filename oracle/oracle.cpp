@@ -12,6 +12,10 @@
 //     - first term copied, next ones added contribution.cpp:241-245 (getproduct/addproduct)
 //     - rhs: minus, VecSetValues ADD       contribution.cpp:293-297, rawvec.cpp:216-254
 //     - Dirichlet values in b()            formulation.cpp:361-385, vec.cpp:43-61
+//   orientation folding (plan descriptions whose slices carry tf_sign /
+//   dof_sign): element matrices on the orientation-0 tables, rows/columns
+//   signed when accumulated; pinned against the unfolded description
+//   (one batch per total orientation, elementselector.cpp:4-133)
 //   rawmat::process                        src/formulation/rawmat.cpp:127-298
 //     - per-row bucketing in fragment -> row r -> element order
 //     - per-row stable sort by column (ties by position), sequential FP64 sum
@@ -295,7 +299,9 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
             for (int i = 0; i < sd.n_tf; i++)
                 for (int e = 0; e < b.n_elem; e++) {
                     int a = sd.tf_addr[(size_t)i * b.n_elem + e];
-                    if (a >= 0) rhs[a] += vals[oi][(size_t)i * b.n_elem + e];
+                    // folded orientations: signed rows (exact)
+                    double sg = sd.tf_sign ? (double)sd.tf_sign[(size_t)i * b.n_elem + e] : 1.0;
+                    if (a >= 0) rhs[a] += sg * vals[oi][(size_t)i * b.n_elem + e];
                 }
         }
         // vec::updateconstraints: constrained entries take the (zero) Dirichlet value
@@ -333,7 +339,12 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
         for (int r = 0; r < nr; r++)
             for (int c = 0; c < nc; c++) {
                 int cr = sd.tf_addr[(size_t)(r / sd.n_dof) * nc + c], cc = sd.dof_addr[(size_t)(r % sd.n_dof) * nc + c];
-                if (cr >= 0 && cc >= 0 && !isc[cr]) pairs[fill[cr]++] = {cc, vals[oi][(size_t)r * nc + c]};
+                if (cr >= 0 && cc >= 0 && !isc[cr]) {
+                    double sg = 1.0;
+                    if (sd.tf_sign) sg *= sd.tf_sign[(size_t)(r / sd.n_dof) * nc + c];
+                    if (sd.dof_sign) sg *= sd.dof_sign[(size_t)(r % sd.n_dof) * nc + c];
+                    pairs[fill[cr]++] = {cc, sg * vals[oi][(size_t)r * nc + c]};
+                }
             }
     }
     // processrows: stable per-row sort by column, sequential duplicate sum

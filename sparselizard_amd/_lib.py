@@ -43,7 +43,8 @@ class Term(C.Structure):
 class SliceDesc(C.Structure):
     _fields_ = [("frag_rank", C.c_int32), ("is_rhs", C.c_int32), ("n_tf", C.c_int32), ("n_dof", C.c_int32),
                 ("tf_addr", C.POINTER(C.c_int32)), ("dof_addr", C.POINTER(C.c_int32)), ("n_terms", C.c_int32),
-                ("terms", C.POINTER(Term)), ("tf_table", C.c_int32), ("dof_table", C.c_int32)]
+                ("terms", C.POINTER(Term)), ("tf_table", C.c_int32), ("dof_table", C.c_int32),
+                ("tf_sign", C.POINTER(C.c_int8)), ("dof_sign", C.POINTER(C.c_int8))]
 
 
 class BucketDesc(C.Structure):

@@ -313,6 +313,86 @@ void h1_tables(int type, int order, int totalorientation, const std::vector<doub
     }
 }
 
+H1Fold h1_fold(int type, int order, const std::vector<double>& pts) {
+    H1Fold F;
+    std::vector<FormFunction> ffs = h1_iterate(type, order);
+    const int nff = (int)ffs.size(), np = (int)pts.size() / 3;
+    auto values = [&](const FormFunction& ff, int orient, std::vector<double>& v) {
+        v.resize((size_t)4 * np);
+        for (int p = 0; p < np; p++) {
+            D ki = var(pts[3 * p], 0), eta = var(pts[3 * p + 1], 1), phi = var(pts[3 * p + 2], 2);
+            D x = eval_ff(type, std::max(order, 2), ff, orient, ki, eta, phi);
+            v[4 * p] = x.v;
+            v[4 * p + 1] = x.d[0];
+            v[4 * p + 2] = x.d[1];
+            v[4 * p + 3] = x.d[2];
+        }
+    };
+    auto oriented = [](const FormFunction& ff) { return ff.dim == 1 || ff.dim == 2; };
+    std::vector<std::vector<double>> v0(nff);
+    for (int f = 0; f < nff; f++)
+        if (oriented(ffs[f])) values(ffs[f], 0, v0[f]);
+    F.map.assign(nff, {});
+    F.dim.resize(nff);
+    F.num.resize(nff);
+    for (int f = 0; f < nff; f++) {
+        F.dim[f] = ffs[f].dim;
+        F.num[f] = ffs[f].num;
+    }
+    std::vector<double> a;
+    for (int f = 0; f < nff; f++) {
+        const FormFunction& ff = ffs[f];
+        const int nv = ff.dim == 1 ? 2 : (ff.dim == 2 ? (ff.assoc_type == TRI ? 6 : 8) : 1);
+        F.map[f].assign(nv, {f, 1});
+        if (!oriented(ff)) continue;
+        for (int v = 1; v < nv; v++) {
+            values(ff, v, a);
+            double amax = 0.0;
+            for (double x : a) amax = std::max(amax, std::fabs(x));
+            const double tol = 1e-10 * std::max(1.0, amax);
+            int best = -1, bsign = 0, nmatch = 0;
+            for (int g = 0; g < nff; g++) {
+                if (ffs[g].dim != ff.dim || ffs[g].num != ff.num) continue;
+                for (int sg = 1; sg >= -1; sg -= 2) {
+                    double err = 0.0;
+                    for (size_t k = 0; k < a.size() && err <= tol; k++) err = std::max(err, std::fabs(a[k] - sg * v0[g][k]));
+                    if (err <= tol) {
+                        best = g;
+                        bsign = sg;
+                        nmatch++;
+                    }
+                }
+            }
+            if (nmatch != 1) return F;  // no (or no unique) signed match: not foldable
+            F.map[f][v] = {best, bsign};
+        }
+    }
+    // every entity orientation must renumber the entity's functions one to one
+    for (int f = 0; f < nff; f++)
+        for (size_t v = 0; v < F.map[f].size(); v++)
+            for (int g = f + 1; g < nff; g++)
+                if (ffs[g].dim == ffs[f].dim && ffs[g].num == ffs[f].num && F.map[g].size() > v &&
+                    F.map[g][v].first == F.map[f][v].first)
+                    return F;
+    F.ok = true;
+    return F;
+}
+
+void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm, std::vector<int8_t>& sign) {
+    SLX_CHECK(F.ok, "orientation folding is not available for this basis");
+    std::vector<int> eo, fo;
+    orientations_from_total(type, total, eo, fo);
+    const int nff = (int)F.map.size();
+    perm.resize(nff);
+    sign.resize(nff);
+    for (int f = 0; f < nff; f++) {
+        const int v = F.dim[f] == 1 ? eo[F.num[f]] : (F.dim[f] == 2 ? fo[F.num[f]] : 0);
+        SLX_CHECK(v >= 0 && v < (int)F.map[f].size(), "orientation " << v << " out of range");
+        perm[f] = F.map[f][v].first;
+        sign[f] = (int8_t)F.map[f][v].second;
+    }
+}
+
 void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out) {
     int nn = topo::count_nodes(type), np = (int)pts.size() / 3;
     out.assign((size_t)4 * nn * np, 0.0);

@@ -47,6 +47,24 @@ void orientations_from_total(int type, int total, std::vector<int>& edges, std::
 void h1_tables(int type, int order, int totalorientation, const std::vector<double>& pts,
                std::vector<double>& out);
 
+// Orientation folding.  Where every edge/face function under every orientation
+// of its entity equals, at the given points and up to sign, one function of the
+// same entity under orientation 0 (hexahedra and quadrangles: Legendre parity
+// and exchanged face axes), an element of any total orientation can be
+// integrated with the orientation-0 tables: its functions are renumbered
+// (absorbed in the address tables) and signed.  The relation is found
+// numerically on the values and reference derivatives at `pts`.
+struct H1Fold {
+    bool ok = false;
+    // map[f][v] = (orientation-0 function, sign) that function f equals when
+    // its entity has orientation v (vertex/volume functions: v = 0 only)
+    std::vector<std::vector<std::pair<int, int>>> map;
+    std::vector<int> dim, num;  // entity of every function
+};
+H1Fold h1_fold(int type, int order, const std::vector<double>& pts);
+// perm[f], sign[f] of every function for the given total orientation.
+void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm, std::vector<int8_t>& sign);
+
 // Order-1 Lagrange geometry functions (corner nodes) and their reference
 // derivatives: out[deriv][node][pt] (src/shapefunction/lagrange/lagrange*.cpp, order 1).
 void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out);

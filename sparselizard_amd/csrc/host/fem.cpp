@@ -3,7 +3,9 @@
 #include <algorithm>
 #include <cmath>
 #include <functional>
+#include <map>
 #include <numeric>
+#include <tuple>
 #include <unordered_map>
 #include <cstring>
 #include "basis.h"
@@ -530,9 +532,30 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         const Contribution* c;
         int m, type, ptf, pdof, intorder, orient;
         bool orientdep;
+        bool folded = false;        // all orientations, orientation-0 tables, signed renumbering
         std::vector<int> elems, disjregs;
+        std::vector<int> eorient;   // folded: total orientation of every element
     };
     std::vector<Frag> frags;
+    // orientation folding maps per (type, order, integration order)
+    std::map<std::tuple<int, int, int>, H1Fold> folds;
+    auto fold_of = [&](int type, int order, int intorder) -> const H1Fold& {
+        auto key = std::make_tuple(type, order, intorder);
+        auto it = folds.find(key);
+        if (it == folds.end()) it = folds.emplace(key, h1_fold(type, order, gauss_rule(type, intorder).coords)).first;
+        return it->second;
+    };
+    auto reads_fields = [](const Contribution& c) {
+        std::function<bool(const OpP&)> walk = [&](const OpP& op) {
+            if (op->kind == OpKind::Field) return true;
+            for (const OpP& a : op->args)
+                if (walk(a)) return true;
+            return false;
+        };
+        for (const OpP& co : c.coeffs)
+            if (walk(co)) return true;
+        return false;
+    };
     // formulation::generate (formulation.cpp:211-218) loops rhs first, then K;
     // contribution::generate (contribution.cpp:44-133) builds groups and
     // orientation batches.
@@ -572,6 +595,21 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
                 std::vector<int> ro(elems.size());
                 std::iota(ro.begin(), ro.end(), 0);
                 std::stable_sort(ro.begin(), ro.end(), [&](int a, int b) { return to[a] < to[b]; });
+                // folded: the batches, in the same order, as one fragment on the
+                // orientation-0 tables (element matrices equal up to the signed
+                // renumbering, which the addresses and signs carry)
+                if (fold_ && !reads_fields(c) && fold_of(f.type, f.ptf, f.intorder).ok &&
+                    (!c.doffield || fold_of(f.type, f.pdof, f.intorder).ok)) {
+                    Frag fb = f;
+                    fb.orient = 0;
+                    fb.folded = true;
+                    for (size_t k = 0; k < ro.size(); k++) {
+                        fb.elems.push_back(elems[ro[k]]);
+                        fb.eorient.push_back(to[ro[k]]);
+                    }
+                    frags.push_back(fb);
+                    continue;
+                }
                 size_t b = 0;
                 while (b < ro.size()) {
                     size_t e = b;
@@ -593,7 +631,7 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         for (size_t b = 0; b < bucket_frags.size(); b++) {
             const Frag& g = frags[bucket_frags[b][0]];
             if (g.c->integral_id == f.c->integral_id && g.type == f.type && g.intorder == f.intorder && g.ptf == f.ptf &&
-                g.pdof == f.pdof && g.orient == f.orient && g.elems == f.elems) {
+                g.pdof == f.pdof && g.orient == f.orient && g.folded == f.folded && g.elems == f.elems) {
                 bucket_of[i] = (int)b;
                 break;
             }
@@ -616,6 +654,31 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
     int rank_rhs = 0, rank_mat = 0;
     std::vector<int> frag_rank(frags.size());
     for (size_t i = 0; i < frags.size(); i++) frag_rank[i] = frags[i].m == 0 ? rank_rhs++ : rank_mat++;
+    // folded fragment: renumber the [nff][n_elem] address table to the
+    // orientation-0 functions and return the signs (same layout)
+    auto fold_addresses = [&](const Frag& f, int order, std::vector<int32_t>& addr) -> const int8_t* {
+        const H1Fold& F = fold_of(f.type, order, f.intorder);
+        const int ne = (int)f.elems.size(), nff = (int)F.map.size();
+        std::vector<int32_t> src = addr;
+        ps->signs.emplace_back((size_t)nff * ne, (int8_t)1);
+        std::vector<int8_t>& sg = ps->signs.back();
+        std::map<int, std::pair<std::vector<int>, std::vector<int8_t>>> cache;
+        for (int e = 0; e < ne; e++) {
+            auto it = cache.find(f.eorient[e]);
+            if (it == cache.end()) {
+                std::pair<std::vector<int>, std::vector<int8_t>> ps_;
+                h1_fold_apply(F, f.type, f.eorient[e], ps_.first, ps_.second);
+                it = cache.emplace(f.eorient[e], std::move(ps_)).first;
+            }
+            const std::vector<int>& perm = it->second.first;
+            const std::vector<int8_t>& sgn = it->second.second;
+            for (int i = 0; i < nff; i++) {
+                addr[(size_t)perm[i] * ne + e] = src[(size_t)i * ne + e];
+                sg[(size_t)perm[i] * ne + e] = sgn[i];
+            }
+        }
+        return sg.data();
+    };
     for (int b = 0; b < nb; b++) {
         const Frag& f0 = frags[bucket_frags[b][0]];
         slx_bucket_desc& bd = ps->buckets[b];
@@ -665,11 +728,15 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
             sd.n_tf = h1_count(f.type, f.ptf);
             ps->ints.push_back(dofm_->getaddresses(c.tffield, f.ptf, f.type, f.elems, c.tfphysreg));
             sd.tf_addr = ps->ints.back().data();
+            sd.tf_sign = nullptr;
+            sd.dof_sign = nullptr;
+            if (f.folded && h1_orientation_dependent(f.ptf)) sd.tf_sign = fold_addresses(f, f.ptf, ps->ints.back());
             if (c.doffield) {
                 sd.dof_table = table_for(f.pdof);
                 sd.n_dof = h1_count(f.type, f.pdof);
                 ps->ints.push_back(dofm_->getaddresses(c.doffield, f.pdof, f.type, f.elems, c.dofphysreg));
                 sd.dof_addr = ps->ints.back().data();
+                if (f.folded && h1_orientation_dependent(f.pdof)) sd.dof_sign = fold_addresses(f, f.pdof, ps->ints.back());
             } else {
                 sd.dof_table = -1;
                 sd.n_dof = 1;

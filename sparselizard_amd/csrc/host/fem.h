@@ -158,6 +158,7 @@ struct PlanStorage {
     std::vector<std::vector<slx_slice_desc>> slices;
     std::vector<std::vector<slx_term>> terms;                 // per (bucket, slice) flattened
     std::vector<std::vector<int32_t>> ints;                   // owned int arrays
+    std::vector<std::vector<int8_t>> signs;                   // owned sign arrays
     std::vector<std::vector<double>> dbls;                    // owned double arrays
     std::vector<std::vector<const double*>> table_ptrs;
     std::vector<std::vector<slx_instr>> progs;
@@ -185,6 +186,9 @@ class formulation {
     // Builds the plan description in the reference's fragment order.
     std::shared_ptr<PlanStorage> buildplan();
     DofManager& dofmanager() { return *dofm_; }
+    // Orientation folding: orientation batches whose bases are signed
+    // renumberings of orientation 0 share one bucket (on by default).
+    void set_orientation_folding(bool on) { fold_ = on; }
 
    private:
     struct Contribution {
@@ -197,6 +201,7 @@ class formulation {
     std::shared_ptr<DofManager> dofm_;
     std::vector<Contribution> contribs_[2];  // [0] rhs, [1] K
     int nintegrals_ = 0;
+    bool fold_ = true;
 };
 
 }  // namespace slx

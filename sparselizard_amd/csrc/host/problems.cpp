@@ -512,6 +512,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else fail("unknown problem kind '" + kind + "'");
     p->kind = kind;
     p->spec = s;
+    p->form->set_orientation_folding(geti(s, "fold", 1) != 0);
     p->plan = p->form->buildplan();
     return p;
 }

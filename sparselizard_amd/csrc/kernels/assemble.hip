@@ -418,7 +418,20 @@ __device__ inline RowDesc load_rowdesc(const uint64_t* rdesc, int64_t pos) {
     return r;
 }
 
-template <typename KT, typename ST, int LPR>
+// Signed slots (folded orientations): the top bit negates the entry.
+template <typename ST, bool SG>
+__device__ inline void slot_add(double* acc, ST sl, double v) {
+    const ST SKIP = (ST)~(ST)0;
+    if (sl == SKIP) return;
+    if (SG) {
+        const ST top = (ST)(ST(1) << (8 * sizeof(ST) - 1));
+        if (sl & top) v = -v;
+        sl = (ST)(sl & (ST)~top);
+    }
+    (void)atomicAdd(&acc[sl], v);
+}
+
+template <typename KT, typename ST, int LPR, bool SG>
 __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __restrict__ ke, int64_t pos0, int64_t pos1, int acc_len,
                                                     double* __restrict__ Aval, double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
@@ -466,15 +479,14 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++)
 #pragma unroll
-            for (int c = 0; c < 4; c++)
-                if (sl[u].x[c] != SKIP) (void)atomicAdd(&acc[sl[u].x[c]], (double)v[u].x[c]);
+            for (int c = 0; c < 4; c++) slot_add<ST, SG>(acc, sl[u].x[c], (double)v[u].x[c]);
     }
     for (int x = l; x < lenA; x += LPR) Aval[a0 + x] = acc[x];
     for (int x = l; x < lenD; x += LPR) Dval[d0 + x] = acc[lenA + x];
 }
 
 // Segments longer than 64 entries: a wave per row, lanes stride over the segment.
-template <typename KT, typename ST>
+template <typename KT, typename ST, bool SG>
 __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT* __restrict__ ke, double* __restrict__ Aval,
                                                            double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
@@ -485,7 +497,6 @@ __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT
     if (pos >= rd.nrows) return;
     double* acc = accall + (size_t)gid * rd.max_row_len;
     const ST* slots = reinterpret_cast<const ST*>(rd.slots);
-    const ST SKIP = (ST)~(ST)0;
     const RowDesc rw = load_rowdesc(rd.rdesc, pos);
     const int64_t a0 = rw.a0, d0 = rw.d0;
     const int lenA = rw.lenA, lenD = rw.lenD;
@@ -495,10 +506,7 @@ __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT
         const uint64_t dsc = rd.erows[k];
         const int n = (int)(dsc >> 48);
         const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-        for (int j = lane; j < n; j += group) {
-            const ST sl = slots[sp + j];
-            if (sl != SKIP) (void)atomicAdd(&acc[sl], (double)ke[off + j]);
-        }
+        for (int j = lane; j < n; j += group) slot_add<ST, SG>(acc, slots[sp + j], (double)ke[off + j]);
         sp += (n + 3) & ~3;
     }
     for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
@@ -514,7 +522,8 @@ __global__ void reduce_rhs(DevReduce rd, const KT* __restrict__ ke, double* __re
     double s = 0.0;
     for (int64_t k = rd.rhs_erow_ptr[r]; k < rd.rhs_erow_ptr[r + 1]; k++) {
         uint64_t pk = rd.rhs_erows[k];
-        s += (double)ke[(int64_t)(pk & 0xFFFFFFFFFFFFULL)];
+        const double v = (double)ke[(int64_t)(pk & 0xFFFFFFFFFFFFULL)];
+        s += (pk >> 63) ? -v : v;
     }
     rhs[rd.free_dofs[r]] = s;
 }
@@ -566,7 +575,7 @@ hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void
     return hipGetLastError();
 }
 
-template <typename KT, typename ST, int LPR>
+template <typename KT, typename ST, int LPR, bool SG>
 static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval, double* Dval, hipStream_t st, const int64_t* cb,
                               const int64_t* ce, const int* cl, int nc) {
     for (int c = 0; c < nc; c++) {
@@ -581,13 +590,13 @@ static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval,
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         const int rpb = threads / LPR;
         const int64_t grid = (nrows + rpb - 1) / rpb;
-        hipLaunchKernelGGL((reduce_rows<KT, ST, LPR>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, cb[c], ce[c], cl[c],
+        hipLaunchKernelGGL((reduce_rows<KT, ST, LPR, SG>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, cb[c], ce[c], cl[c],
                            Aval, Dval);
     }
     return hipGetLastError();
 }
 
-template <typename KT, typename ST>
+template <typename KT, typename ST, bool SG>
 static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st,
                                   const int64_t* cb, const int64_t* ce, const int* cl, int nc) {
     if (max_erow_len > 64) {
@@ -600,23 +609,28 @@ static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const v
         }
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         const int64_t grid = (rd.nrows + gpb - 1) / gpb;
-        hipLaunchKernelGGL((reduce_matrix_long<KT, ST>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
+        hipLaunchKernelGGL((reduce_matrix_long<KT, ST, SG>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
         return hipGetLastError();
     }
     // lanes per row: 4 entries per lane
-    if (max_erow_len <= 4) return launch_rows<KT, ST, 1>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 8) return launch_rows<KT, ST, 2>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 16) return launch_rows<KT, ST, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 32) return launch_rows<KT, ST, 8>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    return launch_rows<KT, ST, 16>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 4) return launch_rows<KT, ST, 1, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 8) return launch_rows<KT, ST, 2, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 16) return launch_rows<KT, ST, 4, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 32) return launch_rows<KT, ST, 8, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    return launch_rows<KT, ST, 16, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
 }
 
 template <typename KT>
 static hipError_t launch_reduce_k(const DevReduce& rd, int max_erow_len, const void* ke, double* Aval, double* Dval, hipStream_t st,
                                   const int64_t* cb, const int64_t* ce, const int* cl, int nc) {
-    if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
-    return launch_reduce_t<KT, uint32_t>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (rd.slot_signed) {
+        if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t, true>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+        if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t, true>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+        return launch_reduce_t<KT, uint32_t, true>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    }
+    if (rd.slot_bytes == 1) return launch_reduce_t<KT, uint8_t, false>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (rd.slot_bytes == 2) return launch_reduce_t<KT, uint16_t, false>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
+    return launch_reduce_t<KT, uint32_t, false>(rd, max_erow_len, ke, Aval, Dval, st, cb, ce, cl, nc);
 }
 
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,

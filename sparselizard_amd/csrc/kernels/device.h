@@ -56,10 +56,11 @@ struct DevReduce {
     const uint64_t* erows;          // packed (len << 48) | ke offset; a row's segments consecutive, rows in work order
     const void* slots;              // per segment entry (blocks padded to 4): A position, lenA + D position, or all ones
     int32_t slot_bytes;             // 1, 2 or 4
+    int32_t slot_signed;            // 1: the top slot bit negates the entry (folded orientations)
     int32_t n_cu;                   // compute units
     int32_t max_row_len;            // max (lenA + lenD)
     const int64_t* rhs_erow_ptr;    // [nrows+1]
-    const uint64_t* rhs_erows;      // packed (1 << 48) | ke offset
+    const uint64_t* rhs_erows;      // packed (1 << 48) | ke offset, bit 63: negate
     const int64_t* free_dofs;       // [nrows] global DOF of each free row
 };
 

@@ -92,6 +92,17 @@ void upload(DBuf& d, const T* src, size_t count, hipStream_t st) {
     else if (count) HIPCHECK(hipMemset(d.p, 0, count * sizeof(T)));
 }
 
+// Host slots: bit 30 marks a negative entry (folded orientations); -1 = skip.
+#define SLX_SLOT_NEG 0x40000000
+
+// Narrowest device slot type for rows of up to maxrow columns (all ones = skip;
+// signed plans give up the top bit).
+int slot_bytes_for(int maxrow, bool sgn) {
+    if (maxrow < (sgn ? 127 : 255)) return 1;
+    if (maxrow < (sgn ? 32767 : 65535)) return 2;
+    return 4;
+}
+
 }  // namespace
 
 struct slx_ctx_s {
@@ -137,6 +148,7 @@ struct slx_plan_s {
     int64_t max_elem_id = 0;       // parameter arrays must hold this many entries
     bool uses_fields = false;
     bool host_only = false;        // symbolic phase only (no device): pattern inspection
+    bool signed_slots = false;     // folded orientations: signed segment entries
 
     hipGraphExec_t graph_exec = nullptr;  // captured numeric phase (slx_plan_graph_capture)
     std::vector<MfmaBucket> mfma;      // per bucket; M == 0 when not eligible
@@ -288,7 +300,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
 
     // element-row segments per free row: counting sort keeps (group, i, element) order
     auto collect = [&](const std::vector<int>& refs, std::vector<int64_t>& ptr, std::vector<uint64_t>& erows,
-                       std::vector<int32_t>* ref_idx, std::vector<int32_t>* elem_idx) {
+                       std::vector<int32_t>* ref_idx, std::vector<int32_t>* elem_idx, std::vector<int32_t>* row_idx) {
         ptr.assign(nf + 1, 0);
         for (int g : refs) {
             const SegGroup& G = L.groups[g];
@@ -305,6 +317,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         erows.resize(ptr[nf]);
         if (ref_idx) ref_idx->resize(ptr[nf]);
         if (elem_idx) elem_idx->resize(ptr[nf]);
+        if (row_idx) row_idx->resize(ptr[nf]);
         std::vector<int64_t> fill(ptr.begin(), ptr.end() - 1);
         for (size_t ri = 0; ri < refs.size(); ri++) {
             const SegGroup& G = L.groups[refs[ri]];
@@ -317,17 +330,28 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
                     int64_t pos = fill[freeidx[a]]++;
                     int64_t off = G.base + ((int64_t)e * G.n_tf + i) * G.ld;
                     erows[pos] = ((uint64_t)G.len << 48) | (uint64_t)off;
+                    // rhs segments (length 1): bit 63 = negative tf sign (folded orientations)
+                    if (G.rhs && sd.tf_sign && sd.tf_sign[(size_t)i * bd.n_elem + e] < 0) erows[pos] |= uint64_t(1) << 63;
                     if (ref_idx) (*ref_idx)[pos] = (int32_t)ri;
                     if (elem_idx) (*elem_idx)[pos] = e;
+                    if (row_idx) (*row_idx)[pos] = i;
                 }
         }
     };
     SLX_CHECK(L.n_ke < (int64_t(1) << 48), "element matrices exceed the 48-bit offset range");
     std::vector<int64_t> erow_ptr, rhs_ptr;
     std::vector<uint64_t> erows, rhs_erows;
-    std::vector<int32_t> eref, eelem;
-    collect(mat, erow_ptr, erows, &eref, &eelem);
-    collect(rhs, rhs_ptr, rhs_erows, nullptr, nullptr);
+    std::vector<int32_t> eref, eelem, erowi;
+    collect(mat, erow_ptr, erows, &eref, &eelem, &erowi);
+    collect(rhs, rhs_ptr, rhs_erows, nullptr, nullptr, nullptr);
+    // folded orientations: a sign per segment entry (tf sign of the row x dof sign of the column)
+    bool signed_slots = false;
+    for (const SegGroup& G : L.groups)
+        for (int s : G.slices) {
+            const slx_slice_desc& sd = d.buckets[G.bucket].slices[s];
+            if (!G.rhs && (sd.tf_sign || sd.dof_sign)) signed_slots = true;
+        }
+    P.signed_slots = signed_slots;
     P.n_erows = (int64_t)erows.size();
     // slot pointers: every segment's slot block padded to 4 entries (aligned vector loads)
     std::vector<int64_t> slot_ptr(nf + 1, 0);
@@ -353,11 +377,15 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     auto for_cols = [&](int64_t k, auto&& fn) {
         const SegGroup& G = L.groups[mat[eref[k]]];
         const slx_bucket_desc& bd = d.buckets[G.bucket];
-        int e = eelem[k];
+        int e = eelem[k], i = erowi[k];
         int pos = 0;
         for (int s : G.slices) {
             const slx_slice_desc& sd = bd.slices[s];
-            for (int j = 0; j < sd.n_dof; j++) fn(pos++, sd.dof_addr[(size_t)j * bd.n_elem + e]);
+            const int rs = sd.tf_sign ? sd.tf_sign[(size_t)i * bd.n_elem + e] : 1;
+            for (int j = 0; j < sd.n_dof; j++) {
+                const int cs = sd.dof_sign ? sd.dof_sign[(size_t)j * bd.n_elem + e] : 1;
+                fn(pos++, sd.dof_addr[(size_t)j * bd.n_elem + e], rs * cs < 0);
+            }
         }
         return (int)G.len;
     };
@@ -368,7 +396,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         for (int64_t r = r0; r < r1; r++) {
             cols.clear();
             for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
-                int len = for_cols(k, [&](int, int32_t c) {
+                int len = for_cols(k, [&](int, int32_t c, bool) {
                     if (c >= 0) cols.push_back(c);
                 });
                 tmax[t] = std::max(tmax[t], len);
@@ -396,15 +424,15 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             }
             int64_t sp = slot_ptr[r];
             for (int64_t k = erow_ptr[r]; k < erow_ptr[r + 1]; k++) {
-                int len = for_cols(k, [&](int j, int32_t c) {
+                int len = for_cols(k, [&](int j, int32_t c, bool neg) {
                     int32_t slot = -1;
-                    if (c >= 0) slot = rankv[std::lower_bound(cols.begin(), cols.end(), (int64_t)c) - cols.begin()];
+                    if (c >= 0) slot = rankv[std::lower_bound(cols.begin(), cols.end(), (int64_t)c) - cols.begin()] | (neg ? SLX_SLOT_NEG : 0);
                     slots[sp + j] = slot;
                 });
                 // slots of one segment must be distinct for the lockstep reduction
                 for (int j = 0; j < len; j++)
                     for (int j2 = j + 1; j2 < len; j2++)
-                        if (slots[sp + j] >= 0 && slots[sp + j] == slots[sp + j2]) dup_error = true;
+                        if (slots[sp + j] >= 0 && (slots[sp + j] & ~SLX_SLOT_NEG) == (slots[sp + j2] & ~SLX_SLOT_NEG)) dup_error = true;
                 sp += (len + 3) / 4 * 4;
             }
         }
@@ -492,22 +520,27 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         upload(P.d_rdesc, rdesc.data(), rdesc.size(), st);
         upload(P.d_erows, erows_w.data(), erows_w.size(), st);
         // slots in the narrowest type that holds every row length (all ones = skip)
+        // signed plans: the top bit of a slot is the entry's sign
         auto narrow = [&](auto tag) {
             using T = decltype(tag);
+            const T top = (T)(T(1) << (8 * sizeof(T) - 1));
             std::vector<T> v(slots.size());
             for (int64_t p = 0; p < nf; p++) {
                 const int64_t r = order[p];
                 T* dst = v.data() + slot_w[p];
-                for (int64_t i = slot_ptr[r]; i < slot_ptr[r + 1]; i++) *dst++ = slots[i] < 0 ? (T)~(T)0 : (T)slots[i];
+                for (int64_t i = slot_ptr[r]; i < slot_ptr[r + 1]; i++) {
+                    const int32_t sl = slots[i];
+                    *dst++ = sl < 0 ? (T)~(T)0 : (T)((sl & ~SLX_SLOT_NEG) | ((sl & SLX_SLOT_NEG) ? top : 0));
+                }
             }
             upload(P.d_slots, v.data(), v.size(), st);
         };
-        const int sb = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
+        const int sb = slot_bytes_for(maxrow, signed_slots);
         if (sb == 1) narrow(uint8_t{});
         else if (sb == 2) narrow(uint16_t{});
         else narrow(uint32_t{});
     }
-    int slot_bytes = maxrow < 255 ? 1 : (maxrow < 65535 ? 2 : 4);
+    int slot_bytes = slot_bytes_for(maxrow, signed_slots);
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
@@ -527,6 +560,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.rd.erows = P.d_erows.as<uint64_t>();
     P.rd.slots = P.d_slots.p;
     P.rd.slot_bytes = slot_bytes;
+    P.rd.slot_signed = signed_slots ? 1 : 0;
     P.rd.n_cu = P.n_cu;
     P.rd.max_row_len = maxrow;
     P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();

@@ -46,15 +46,20 @@ def reduce_host(maps, ke, Arow, Drow, shape):
     assert len(maps["slots"]) == padded.sum()
     lenA = np.diff(Arow)
     on = slots >= 0
+    # folded orientations: bit 30 of a slot negates the entry
+    neg = on & ((slots & (1 << 30)) != 0)
+    slots = np.where(on, slots & ~(1 << 30), slots)
+    vals = np.where(neg, -ke[kidx], ke[kidx])
     isA = on & (slots < lenA[rows])
     Aval = np.zeros(shape["nnzA"])
     Dval = np.zeros(shape["nnzD"])
-    np.add.at(Aval, Arow[rows[isA]] + slots[isA], ke[kidx[isA]])
+    np.add.at(Aval, Arow[rows[isA]] + slots[isA], vals[isA])
     isD = on & ~isA
-    np.add.at(Dval, Drow[rows[isD]] + slots[isD] - lenA[rows[isD]], ke[kidx[isD]])
+    np.add.at(Dval, Drow[rows[isD]] + slots[isD] - lenA[rows[isD]], vals[isD])
     rr = maps["rhs_erows"]
     roffs = (rr & np.uint64((1 << 48) - 1)).astype(np.int64)
+    rneg = (rr >> np.uint64(63)) != 0
     rrows = np.repeat(np.arange(nf), np.diff(maps["rhs_ptr"]))
     rhs_free = np.zeros(nf)
-    np.add.at(rhs_free, rrows, ke[roffs])
+    np.add.at(rhs_free, rrows, np.where(rneg, -ke[roffs], ke[roffs]))
     return Aval, Dval, rhs_free
