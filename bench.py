@@ -41,7 +41,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=64, help="cells per cube edge (workload: 64)")
+    ap.add_argument("--problem", default="elasticity3d",
+                    help="elasticity3d (the headline workload) or hppoisson3d (hp hexahedra, orders 1..5)")
+    ap.add_argument("--n", type=int, default=0, help="cells per cube edge (0: 64 for elasticity3d, 48 for hppoisson3d)")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--mixed", default="f16split", help="mixed precision mode")
     ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU baseline sample size (0: auto)")
@@ -63,9 +65,11 @@ def main():
         copy_probe(torch, dev, args.copy_probe_mib)
         return
     ctx = sa.Context(0)
-    spec = "n=%d;order=%d" % (args.n, args.order)
+    if args.n <= 0:
+        args.n = 64 if args.problem == "elasticity3d" else 48
+    spec = "n=%d;order=%d" % (args.n, args.order) if args.problem == "elasticity3d" else "n=%d" % args.n
     t0 = time.time()
-    prob = sa.Problem(ctx, "elasticity3d", spec)
+    prob = sa.Problem(ctx, args.problem, spec)
     info = prob.info()
     t_host = time.time() - t0
     log("problem built in %.1f s: %s" % (t_host, info))
@@ -132,10 +136,14 @@ def main():
                             args.mixed, iters=max(3, args.steps // 2))
     prof_64 = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
                            "fp64", iters=max(3, args.steps // 2))
-    n_elem = count_tets(args.n)
-    n = (args.order + 1) * (args.order + 2) * (args.order + 3) // 6
-    ngp = {1: 11, 2: 24, 3: 43}.get(args.order, 24)
-    flops = element_flops(3, n, 3, ngp) * n_elem
+    if args.problem == "elasticity3d":
+        n_elem = count_tets(args.n)
+        n = (args.order + 1) * (args.order + 2) * (args.order + 3) // 6
+        ngp = {1: 11, 2: 24, 3: 43}.get(args.order, 24)
+        flops = element_flops(3, n, 3, ngp) * n_elem
+    else:
+        n_elem = info["nelem"]
+        flops = slice_flops(prob.desc())
     ke_bytes = 4 if args.mixed in ("f16split", "bf16split", "fp32") else 8
     # reduction: every element-matrix entry read once, every CSR value and rhs entry written once
     red_bytes = stats["n_coo"] * ke_bytes + nnz * 8 + nn * 8
@@ -144,9 +152,10 @@ def main():
     t_contract = prof_mix["mfma"] if prof_mix["mfma"] > 0 else prof_mix["valu"]
     if t_contract >= prof_mix["reduce"]:
         achieved = flops / (t_contract * 1e-3) / 1e12
+        kname = "mfma_f16split_kernel" if args.problem == "elasticity3d" else "factor_f16split_kernel"
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": peak_elem, "unit": "TFLOP/s",
                 "frac": round(achieved / peak_elem, 5), "traffic": None,
-                "kernel": "mfma_f16split_kernel" if prof_mix["mfma"] > 0 else "elem_tiled",
+                "kernel": kname if prof_mix["mfma"] > 0 else "elem_cols",
                 "kernel_ms": round(t_contract, 4)}
     else:
         achieved = red_bytes / (prof_mix["reduce"] * 1e-3) / 1e9
@@ -181,8 +190,11 @@ def main():
         "dtype": {"f16split": "f16x2-split/f32acc/f64reduce", "bf16split": "bf16x3-split/f32acc/f64reduce",
                   "fp32": "f32/f64reduce"}.get(args.mixed, args.mixed),
         "data": "synthetic",
-        "config": {"workload": "3D linear elasticity h1xyz P%d, %d^3 x 6 jittered tets (%d), E lognormal(0,0.5) per "
-                               "element, nu=0.3, clamped x=0" % (args.order, args.n, n_elem),
+        "config": {"workload": ("3D linear elasticity h1xyz P%d, %d^3 x 6 jittered tets (%d), E lognormal(0,0.5) per "
+                                "element, nu=0.3, clamped x=0" % (args.order, args.n, n_elem))
+                   if args.problem == "elasticity3d" else
+                   ("3D hp-Poisson, %d^3 jittered hexahedra (%d), per-element order U{1..5}, c lognormal(0,1), "
+                    "Gauss order 2p+1" % (args.n, n_elem)),
                    "n_elem": n_elem, "ndofs": nn, "nnzA": shape["nnzA"], "nnzD": shape["nnzD"],
                    "coo_entries": stats["n_coo"], "parallelism": "single GPU"},
         "fp64_gpu": {"ms_per_step": ms64, "nnz_per_s": nnz / (ms64 * 1e-3), "kernel_ms": prof_64},
@@ -208,6 +220,22 @@ def copy_probe(torch, dev, mib):
     print(json.dumps({"copy_probe_bytes": n * 4}))
 
 
+def slice_flops(d):
+    """Canonical contraction flops of every matrix slice, sum over buckets:
+    2 n_gp n_tf n_dof n_a per element (n_a: tf derivatives of the slice's
+    terms), which is 6 (p+1)^9 for a scalar Poisson hexahedron of order p."""
+    total = 0.0
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            if sd.is_rhs:
+                continue
+            na = len({sd.terms[t].tf_deriv for t in range(sd.n_terms)})
+            total += 2.0 * bd.n_gp * sd.n_tf * sd.n_dof * na * bd.n_elem
+    return total
+
+
 def count_tets(n):
     return 6 * n ** 3
 
@@ -218,26 +246,30 @@ def cpu_baseline(sa, args):
     from oracle import py_oracle
     import numpy as np
 
+    elast = args.problem == "elasticity3d"
+    per_cell = 6 if elast else 1
+
     def run(n):
-        p = sa.Problem(None, "elasticity3d", "n=%d;order=%d" % (n, args.order))
+        spec = "n=%d;order=%d" % (n, args.order) if elast else "n=%d" % n
+        p = sa.Problem(None, args.problem, spec)
         info = p.info()
         nd = p.numeric()
         t0 = time.time()
         o = py_oracle.assemble(p.desc_ptr(), nd, info["ndofs"], info["nfree"], nthreads=1)
         dt = time.time() - t0
-        return dt, len(o.Aval) + len(o.Dval), count_tets(n)
+        return dt, len(o.Aval) + len(o.Dval), info["nelem"]
 
     n = args.cpu_sample_n
     if n <= 0:
-        dt, nnz, ne = run(4)
+        dt, nnz, ne = run(4 if elast else 3)
         per_elem = dt / ne
         # aim at ~15 s of CPU work
-        n = max(4, int(round((15.0 / max(per_elem, 1e-9) / 6) ** (1.0 / 3))))
+        n = max(3, int(round((15.0 / max(per_elem, 1e-9) / per_cell) ** (1.0 / 3))))
         n = min(n, 20)
     dt, nnz, ne = run(n)
+    what = ("elasticity P%d on %d^3 x 6 tets" % (args.order, n)) if elast else ("hp-Poisson on %d^3 hexahedra" % n)
     return {"value": nnz / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
-            "sample": "oracle generate()+A()+b() FP64, 1 thread, elasticity P%d on %d^3 x 6 tets (%d elements), %.1f s"
-                      % (args.order, n, ne, dt)}
+            "sample": "oracle generate()+A()+b() FP64, 1 thread, %s (%d elements), %.1f s" % (what, ne, dt)}
 
 
 if __name__ == "__main__":

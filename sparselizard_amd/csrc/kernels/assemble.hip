@@ -6,7 +6,8 @@
 //   (operation::interpolate of every term, contribution.cpp:155-163), times
 //   |det J| (contribution.cpp:147, 200), into the table
 //   ctab[(slot * nq + q) * n_elem + e]  (element fastest: coalesced for kernels
-//   that map lanes to elements).
+//   that map lanes to elements), or with qfast ctab[(slot * n_elem + e) * nq + q]
+//   (Gauss point fastest: contiguous per element, for the factorized kernel).
 // elem_cols: lanes = (element, dof column); per slice, TI tf rows at a time:
 //   Ke[i][j] = sum_q sum_t tf_t[i][q] w_q C_t(e,q) dof_t[j][q]
 //   (contribution.cpp:172-246; the per-term dgemm of densemat.cpp:228-311 with
@@ -152,7 +153,7 @@ __device__ inline T apply_unary(int op, T x) {
 // ctab[(slot * nq + q) * n_elem + e] multiplied by |det J|.
 template <typename T, int V>
 __device__ void run_program(const DevBucket& b, const DevInputs& in, const int (&e)[V], const int (&q)[V], const Geo (&g)[V],
-                            const bool (&on)[V], int nq, T* __restrict__ ctab) {
+                            const bool (&on)[V], int nq, int qfast, T* __restrict__ ctab) {
     T reg[SLX_MAX_REGS][V];
     const int64_t stride = (int64_t)nq * b.n_elem;
     for (int pc = 0; pc < b.n_instr; pc++) {
@@ -212,7 +213,9 @@ __device__ void run_program(const DevBucket& b, const DevInputs& in, const int (
             case SLX_OP_OUT:
 #pragma unroll
                 for (int v = 0; v < V; v++)
-                    if (on[v]) ctab[(size_t)ins.imm * stride + (size_t)q[v] * b.n_elem + e[v]] = reg[ia][v] * (T)fabs(g[v].det);
+                    if (on[v])
+                        ctab[(size_t)ins.imm * stride + (qfast ? (size_t)e[v] * nq + q[v] : (size_t)q[v] * b.n_elem + e[v])] =
+                            reg[ia][v] * (T)fabs(g[v].det);
                 continue;
             default:
 #pragma unroll
@@ -228,7 +231,7 @@ __device__ void run_program(const DevBucket& b, const DevInputs& in, const int (
 // over the Gauss points); each thread takes V consecutive (element, Gauss
 // point) pairs, element fastest.
 template <typename T, int V>
-__global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, int nq, T* __restrict__ ctab) {
+__global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, int nq, int qfast, T* __restrict__ ctab) {
     const int64_t total = (int64_t)b.n_elem * nq;
     const int64_t idx0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
     if (idx0 >= total) return;
@@ -240,11 +243,11 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
         const int64_t idx = idx0 + v;
         on[v] = idx < total;
         const int64_t id = on[v] ? idx : idx0;
-        e[v] = (int)(id % b.n_elem);
-        q[v] = (int)(id / b.n_elem);
+        e[v] = qfast ? (int)(id / nq) : (int)(id % b.n_elem);
+        q[v] = qfast ? (int)(id % nq) : (int)(id / b.n_elem);
         geometry(b, in.xyz, e[v], q[v], g[v]);
     }
-    run_program<T, V>(b, in, e, q, g, on, nq, ctab);
+    run_program<T, V>(b, in, e, q, g, on, nq, qfast, ctab);
 }
 
 // Element matrices, one lane per (element, dof column): `jc` consecutive
@@ -258,7 +261,7 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
 //   Ke[i][j] += sum_r tf_r[i][q] h_r        (TI rows per pass)
 // which is the reference's sum over terms and Gauss points regrouped.
 template <typename CT, int TI>
-__global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc,
+__global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc, int qfast,
                                                  CT* __restrict__ ke) {
     const int epb = blockDim.x / jc;
     const int el = threadIdx.x / jc, jl = threadIdx.x - (threadIdx.x / jc) * jc;
@@ -291,7 +294,7 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
                 for (int d = 0; d < 4; d++) D[r][d] = 0;
             for (int t = 0; t < n_terms; t++) {
                 const int tr = terms[t].tf_deriv, td = terms[t].dof_deriv;
-                const CT c = ctab[((size_t)terms[t].coef * nq + qq) * cstride + ec];
+                const CT c = qfast ? ctab[((size_t)terms[t].coef * cstride + ec) * nq + qq] : ctab[((size_t)terms[t].coef * nq + qq) * cstride + ec];
 #pragma unroll
                 for (int r = 0; r < 4; r++)
 #pragma unroll
@@ -530,29 +533,29 @@ __global__ void reduce_rhs(DevReduce rd, const KT* __restrict__ ke, double* __re
 
 // ---------------------------------------------------------------- launchers --
 
-hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st) {
+hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st, int qfast) {
     if (b.n_elem == 0 || b.n_coefs == 0) return hipSuccess;
     const int nq = b.q_inv ? 1 : b.n_gp;
     const int64_t n = (int64_t)b.n_elem * nq;
     if (fp64) {
         const unsigned grid = (unsigned)((n + 2 * 256 - 1) / (2 * 256));
-        hipLaunchKernelGGL((coef_kernel<double, 2>), dim3(grid), dim3(256), 0, st, b, in, nq, (double*)ctab);
+        hipLaunchKernelGGL((coef_kernel<double, 2>), dim3(grid), dim3(256), 0, st, b, in, nq, qfast, (double*)ctab);
     } else {
         const unsigned grid = (unsigned)((n + 4 * 256 - 1) / (4 * 256));
-        hipLaunchKernelGGL((coef_kernel<float, 4>), dim3(grid), dim3(256), 0, st, b, in, nq, (float*)ctab);
+        hipLaunchKernelGGL((coef_kernel<float, 4>), dim3(grid), dim3(256), 0, st, b, in, nq, qfast, (float*)ctab);
     }
     return hipGetLastError();
 }
 
 template <typename CT, int TI>
-static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int jc, void* ke, hipStream_t st) {
+static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int jc, int qfast, void* ke, hipStream_t st) {
     const int epb = 256 / jc;
     const unsigned grid = (unsigned)((b.n_elem + epb - 1) / epb);
-    hipLaunchKernelGGL((elem_cols<CT, TI>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, (CT*)ke);
+    hipLaunchKernelGGL((elem_cols<CT, TI>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
 }
 
 template <typename CT>
-static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask, void* ke, hipStream_t st) {
+static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask, int qfast, void* ke, hipStream_t st) {
     int ntf = 1, ndof = 1;
     for (int s = 0; s < b.n_slices; s++)
         if ((mask >> s) & 1u) {
@@ -560,18 +563,18 @@ static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask,
             ndof = max(ndof, b.slices[s].n_dof);
         }
     const int jc = min(ndof, 64);
-    if (ntf <= 4) launch_cols<CT, 4>(b, ctab, mask, jc, ke, st);
-    else if (ntf <= 8) launch_cols<CT, 8>(b, ctab, mask, jc, ke, st);
-    else if (ntf <= 10) launch_cols<CT, 10>(b, ctab, mask, jc, ke, st);
-    else launch_cols<CT, 16>(b, ctab, mask, jc, ke, st);
+    if (ntf <= 4) launch_cols<CT, 4>(b, ctab, mask, jc, qfast, ke, st);
+    else if (ntf <= 8) launch_cols<CT, 8>(b, ctab, mask, jc, qfast, ke, st);
+    else if (ntf <= 10) launch_cols<CT, 10>(b, ctab, mask, jc, qfast, ke, st);
+    else launch_cols<CT, 16>(b, ctab, mask, jc, qfast, ke, st);
 }
 
-hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask) {
+hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask, int qfast) {
     if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
     if (precision == SLX_PREC_FP64)
-        launch_cols_any<double>(b, ctab, slice_mask, ke, st);
+        launch_cols_any<double>(b, ctab, slice_mask, qfast, ke, st);
     else
-        launch_cols_any<float>(b, ctab, slice_mask, ke, st);
+        launch_cols_any<float>(b, ctab, slice_mask, qfast, ke, st);
     return hipGetLastError();
 }
 

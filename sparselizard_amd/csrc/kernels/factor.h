@@ -1,0 +1,41 @@
+// Factorized MFMA contraction for large element matrices (high-order
+// hexahedra and other slices with n_tf * n_dof > 256).  Per element and slice
+//   Ke = P^T Q_e,   P[k][i] = tf_a(k)[i][q(k)],
+//                   Q_e[k][j] = w_q(k) sum_{terms t with tf derivative a(k)} C_t(e, q(k)) dof_{b_t}[j][q(k)]
+// which is the reference's sum over terms and Gauss points of
+// tf_t[i][q] w_q C_t(e,q) dof_t[j][q] (contribution.cpp:172-246) regrouped by
+// tf derivative.  P is shared by the elements of a bucket (orientation-0
+// tables), Q_e is built per element; both run on FP16 MFMA with hi/lo split
+// operands and FP32 accumulation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "device.h"
+
+#define SLX_FK_WAVES 8     // consumer (MFMA) waves
+#define SLX_FK_BWAVES 4    // producer waves: 256 threads, one per padded dof column
+#define SLX_FK_RU 16       // Gauss points per k-step at most (registers of a producer thread)
+
+namespace slx {
+
+struct FactorSlice {
+    int32_t n_tf, n_dof, ngp, nq;   // nq: coefficient points per element (1 when q-invariant)
+    int32_t na, qb, nks;            // tf derivatives, Gauss points per k-step (qb * na <= 32), k-steps
+    int32_t mt, nt, ntp;            // 16-row / 16-column tiles, padded dof columns (nt * 16)
+    int32_t n_terms, bmask;         // terms; dof derivatives in use (bit b)
+    int32_t bm, bn;                 // tiles per wave (rows, columns): waves 2 x 4
+    // tinfo: [n_terms] tf-derivative index (into the na derivatives), then
+    // [n_terms] dof derivative, then [n_terms] coefficient slot
+    const int32_t* tinfo;
+    float a_unscale;                // 1 / power-of-two scale of P
+    float tqmax;                    // max |tq|
+    const uint32_t* afrag;          // [nks][mt][2][64][4] words: f16 hi/lo fragments of the scaled P
+    const float* tq;                // [ngp][ntp][4]: w_q dof_b[j][q], zero padded columns
+    int64_t ke_base, ke_ld;
+};
+
+size_t factor_lds_bytes(const FactorSlice& f);
+bool factor_variant_fits(int mt, int nt, int* bm, int* bn);
+hipError_t launch_factor_f16split(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu,
+                                  hipStream_t st);
+
+}  // namespace slx

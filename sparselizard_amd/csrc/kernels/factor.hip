@@ -1,0 +1,234 @@
+// Factorized element contraction on FP16 MFMA with hi/lo split operands.
+//
+// One workgroup per element at a time, persistent over the bucket: 8 consumer
+// waves (a 2 x 4 grid over the 16x16 output tiles) run the MFMAs while 4
+// producer waves stage the next k-step.
+// The K dimension runs over (Gauss point, tf derivative) pairs in k-steps of
+// 32: a k-step holds qb Gauss points times the na tf derivatives, so the dof
+// table values of a Gauss point are read once per element.  Per k-step:
+//   A chunk (shared P fragments, precomputed hi/lo f16) -> LDS
+//   B chunk (Q_e for the element, built from the element's coefficients in
+//            LDS and the dof table, split into hi/lo f16) -> LDS
+// both double buffered, and every wave runs its block of 16x16 output tiles:
+//   acc += Ahi*Bhi + Ahi*Blo + Alo*Bhi        (v_mfma_f32_16x16x32_f16)
+// P is scaled on the host and Q_e per element by powers of two so that the
+// largest magnitudes sit in [128, 256): the lo parts of every entry above
+// 2^-10 of the maximum stay normal f16 numbers.  The accumulators are
+// unscaled once when the element matrix is stored.
+#include "factor.h"
+
+namespace slx {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float4v __attribute__((ext_vector_type(4)));
+typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
+
+size_t factor_lds_bytes(const FactorSlice& f) {
+    return (size_t)2 * f.mt * 128 * 16 + (size_t)2 * (4 * f.bn) * 128 * 16 + (size_t)f.ngp * 16 * 4 + 16;
+}
+
+// tile blocks per wave: rows over 2 waves, columns over 4 waves; the column
+// tiles of a wider matrix are taken in passes of 4 * BN
+static const int FK_BM[3] = {2, 4, 7};
+static const int FK_BN[3] = {1, 2, 2};
+
+bool factor_variant_fits(int mt, int nt, int* bm, int* bn) {
+    for (int v = 0; v < 3; v++)
+        if (2 * FK_BM[v] >= mt && (4 * FK_BN[v] >= nt || v == 2)) {
+            *bm = FK_BM[v];
+            *bn = FK_BN[v];
+            return true;
+        }
+    return false;
+}
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
+    factor_f16split_kernel(int n_elem, FactorSlice f, const float* __restrict__ ctab, float* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb, ntp = f.ntp;
+    constexpr int NTB = 4 * BN;                                 // column tiles per pass
+    uint4v* As = reinterpret_cast<uint4v*>(smem_raw);           // [2][MT][2][64]
+    uint4v* Bs = As + 2 * MT * 128;                             // [2][NTB][2][64]
+    float* Ms = reinterpret_cast<float*>(Bs + 2 * NTB * 128);   // [ngp][4 tf derivative][4 dof derivative]
+    unsigned* smax = reinterpret_cast<unsigned*>(Ms + ngp * 16);
+    _Float16* Bh = reinterpret_cast<_Float16*>(Bs);
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool mfma_wave = w < SLX_FK_WAVES;
+    const int32_t* t_ai = f.tinfo;
+    const int32_t* t_b = f.tinfo + f.n_terms;
+    const int32_t* t_slot = f.tinfo + 2 * f.n_terms;
+    const int64_t stride = (int64_t)nq * n_elem;
+    const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
+    const int achunk = MT * 128;
+    // k slots past qb * na stay zero for the whole launch (their A rows are zero)
+    for (int x = tid; x < 2 * NTB * 128; x += blockDim.x) Bs[x] = uint4v{0u, 0u, 0u, 0u};
+    const int npass = (NT + NTB - 1) / NTB;
+
+    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
+        if (tid == 0) *smax = 0u;
+        __syncthreads();  // the previous element is done with Ms, As, Bs
+        // the element's coefficient matrices per Gauss point:
+        //   Ms[q][ai][b] = sum of C_t(e, q) over terms with derivatives (alist[ai], b)
+        float m = 0.f;
+        for (int q = tid; q < ngp; q += blockDim.x) {
+            float* mq = Ms + q * 16;  // owned by this thread
+#pragma unroll
+            for (int x = 0; x < 16; x++) mq[x] = 0.f;
+            const int qq = nq == 1 ? 0 : q;
+            for (int t = 0; t < f.n_terms; t++) mq[t_ai[t] * 4 + t_b[t]] += ctab[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq];
+#pragma unroll
+            for (int ai = 0; ai < 4; ai++) m = fmaxf(m, fabsf(mq[4 * ai]) + fabsf(mq[4 * ai + 1]) + fabsf(mq[4 * ai + 2]) + fabsf(mq[4 * ai + 3]));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        if (lane == 0) atomicMax(smax, __float_as_uint(m));
+        __syncthreads();
+        // power-of-two scale: |Q| <= max_q,a sum_b |Ms| * max |tq|  ->  [128, 256)
+        const float bound = __uint_as_float(*smax) * f.tqmax;
+        float sc = 1.f, unsc = f.a_unscale;
+        if (bound > 0.f) {
+            int ex = 0;
+            (void)frexpf(bound, &ex);  // bound in [2^(ex-1), 2^ex)
+            sc = ldexpf(1.f, 8 - ex);
+            unsc = ldexpf(f.a_unscale, ex - 8);
+        }
+        if (mfma_wave) {
+            // consumer waves: a 2 x 4 grid over the 16x16 output tiles of a pass
+            const int wm = w >> 2, wn = w & 3;
+            for (int pass = 0; pass < npass; pass++) {
+                const int cb = pass * NTB;
+                float4v acc[BM][BN];
+#pragma unroll
+                for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+                    for (int nj = 0; nj < BN; nj++) acc[mi][nj] = float4v{0.f, 0.f, 0.f, 0.f};
+                __syncthreads();  // k-step 0 staged
+                for (int ks = 0; ks < f.nks; ks++) {
+                    const int cur = ks & 1;
+                    const uint4v* Ac = As + cur * achunk;
+                    const uint4v* Bc = Bs + cur * NTB * 128;
+                    uint4v bh[BN], bl[BN];
+#pragma unroll
+                    for (int nj = 0; nj < BN; nj++) {
+                        const int jl = wn + 4 * nj;
+                        bh[nj] = Bc[jl * 128 + lane];
+                        bl[nj] = Bc[jl * 128 + 64 + lane];
+                    }
+#pragma unroll
+                    for (int mi = 0; mi < BM; mi++) {
+                        const int mt = wm + 2 * mi;
+                        if (mt < MT) {
+                            const half8 ah = __builtin_bit_cast(half8, Ac[mt * 128 + lane]);
+                            const half8 al = __builtin_bit_cast(half8, Ac[mt * 128 + 64 + lane]);
+#pragma unroll
+                            for (int nj = 0; nj < BN; nj++) {
+                                if (cb + wn + 4 * nj < NT) {
+                                    const half8 xh = __builtin_bit_cast(half8, bh[nj]);
+                                    const half8 xl = __builtin_bit_cast(half8, bl[nj]);
+                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh, acc[mi][nj], 0, 0, 0);
+                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl, acc[mi][nj], 0, 0, 0);
+                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh, acc[mi][nj], 0, 0, 0);
+                                }
+                            }
+                        }
+                    }
+                    __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
+                }
+                // element matrix rows i = mt*16 + 4*(lane/16) + r, column j = jt*16 + lane%16;
+                // 32-bit offsets from the element's (uniform) base keep the addresses cheap
+                float* eb = ke + f.ke_base + (int64_t)e * f.n_tf * f.ke_ld;
+                const int ld = (int)f.ke_ld;
+#pragma unroll
+                for (int mi = 0; mi < BM; mi++) {
+                    const int mt = wm + 2 * mi;
+#pragma unroll
+                    for (int nj = 0; nj < BN; nj++) {
+                        const int jt = cb + wn + 4 * nj;
+                        const int j = jt * 16 + (lane & 15);
+                        if (mt >= MT || jt >= NT || j >= f.n_dof) continue;
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            const int i = mt * 16 + 4 * (lane >> 4) + r;
+                            if (i < f.n_tf) eb[i * ld + j] = acc[mi][nj][r] * unsc;
+                        }
+                    }
+                }
+            }
+        } else {
+            // producer waves: stage the shared P fragments and build Q_e, one
+            // k-step ahead of the consumers.  Thread bj takes column j = bj and
+            // every Gauss point of the k-step: the dof table values (a float4 of
+            // the four derivatives per point) are loaded first, then combined
+            // with Ms[q][ai][b] into the entries k = qi * na + ai, split into hi/lo
+            // f16 in the B fragment layout (lane 16 * (k / 8) + j % 16 of column
+            // tile j / 16, element k % 8).
+            const int bt = tid - 64 * SLX_FK_WAVES;
+            const float4v* tq4 = reinterpret_cast<const float4v*>(f.tq);
+            // thread bt: column bc = bt % (16 * NTB) of the pass, Gauss points qi = bg, bg + ng, ...
+            constexpr int CW = 16 * NTB;
+            constexpr int NG = (64 * SLX_FK_BWAVES) / CW;
+            const int bc = bt % CW, bg = bt / CW;
+            auto produce = [&](int ks, int buf, int cb) {
+                for (int x = bt; x < achunk; x += 64 * SLX_FK_BWAVES) As[buf * achunk + x] = afrag[(size_t)ks * achunk + x];
+                if (bg >= NG) return;
+                const int j = cb * 16 + bc;
+                const bool jon = j < ntp;
+                _Float16* col = Bh + ((size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15)) * 8;
+                // Gauss points in groups of 8: loads first, then the products
+                for (int q0 = bg; q0 < qb; q0 += 8 * NG) {
+                    float4v h[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int qi = q0 + u * NG, q = ks * qb + qi;
+                        h[u] = (jon && qi < qb && q < ngp) ? tq4[(size_t)q * ntp + j] : float4v{0.f, 0.f, 0.f, 0.f};
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int qi = q0 + u * NG;
+                        if (qi >= qb) break;
+                        const int q = min(ks * qb + qi, ngp - 1);
+                        for (int ai = 0; ai < na; ai++) {
+                            const float4v mm = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
+                            const float val = (mm[0] * h[u][0] + mm[1] * h[u][1] + mm[2] * h[u][2] + mm[3] * h[u][3]) * sc;
+                            const _Float16 hi = (_Float16)val;
+                            const _Float16 lo = (_Float16)(val - (float)hi);
+                            const int k = qi * na + ai;
+                            const int at = 16 * 8 * (k >> 3) + (k & 7);
+                            col[at] = hi;
+                            col[at + 64 * 8] = lo;
+                        }
+                    }
+                }
+            };
+            for (int pass = 0; pass < npass; pass++) {
+                produce(0, 0, pass * NTB);
+                __syncthreads();  // k-step 0 staged
+                for (int ks = 0; ks < f.nks; ks++) {
+                    if (ks + 1 < f.nks) produce(ks + 1, (ks & 1) ^ 1, pass * NTB);
+                    __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
+                }
+            }
+        }
+    }
+}
+
+template <int BM, int BN>
+static hipError_t launch_fk(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
+    const size_t lds = factor_lds_bytes(f);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / lds));
+    int64_t grid = std::min<int64_t>(b.n_elem, (int64_t)n_cu * per_cu);
+    hipLaunchKernelGGL((factor_f16split_kernel<BM, BN>), dim3((unsigned)grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, b.n_elem,
+                       f, ctab, ke);
+    return hipGetLastError();
+}
+
+hipError_t launch_factor_f16split(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
+    if (b.n_elem == 0) return hipSuccess;
+    if (f.bm == 2 && f.bn == 1) return launch_fk<2, 1>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 4 && f.bn == 2) return launch_fk<4, 2>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 7 && f.bn == 2) return launch_fk<7, 2>(b, f, ctab, ke, n_cu, st);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace slx

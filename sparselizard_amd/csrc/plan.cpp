@@ -16,11 +16,12 @@
 #include "host/problems.h"
 #include "kernels/device.h"
 #include "kernels/mfma.h"
+#include "kernels/factor.h"
 #include <cmath>
 
 namespace slx {
-hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st);
-hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask);
+hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st, int qfast);
+hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask, int qfast);
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
                          double* rhs, bool has_matrix, bool has_rhs, hipStream_t st, const int64_t* class_begin,
                          const int64_t* class_end, const int* class_len, int n_classes);
@@ -153,6 +154,9 @@ struct slx_plan_s {
     hipGraphExec_t graph_exec = nullptr;  // captured numeric phase (slx_plan_graph_capture)
     std::vector<MfmaBucket> mfma;      // per bucket; M == 0 when not eligible
     std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
+    std::vector<std::vector<std::pair<int, FactorSlice>>> factor;  // per bucket: (slice, factorized MFMA data)
+    std::vector<uint32_t> factor_mask; // matrix slices handled by the factorized MFMA kernel
+    int last_qfast = 0;                // coefficient table layout of the last bucket assembled
     int n_cu = 256;
 
     DBuf* newbuf() {
@@ -762,11 +766,99 @@ std::vector<double> table_qmajor(const double* tab, int nff, int ngp) {
     return out;
 }
 
+// Factorized MFMA data for the large matrix slices of a bucket (those the
+// batched MFMA kernel does not take): Ke = P^T Q_e with K over (Gauss point,
+// tf derivative); see kernels/factor.h.
+void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
+    for (int s = 0; s < bd.n_slices; s++) {
+        const slx_slice_desc& sd = bd.slices[s];
+        if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
+        if ((int64_t)sd.n_tf * sd.n_dof <= 256 || sd.n_terms > 16) continue;
+        FactorSlice f;
+        std::memset(&f, 0, sizeof(f));
+        f.n_tf = sd.n_tf;
+        f.n_dof = sd.n_dof;
+        f.ngp = bd.n_gp;
+        f.nq = bd.q_invariant ? 1 : bd.n_gp;
+        f.mt = (sd.n_tf + 15) / 16;
+        f.nt = (sd.n_dof + 15) / 16;
+        f.ntp = f.nt * 16;
+        f.n_terms = sd.n_terms;
+        if (!factor_variant_fits(f.mt, f.nt, &f.bm, &f.bn)) continue;
+        // tf derivatives in use; every term's (derivative index, dof derivative, slot)
+        std::vector<int> alist;
+        for (int t = 0; t < sd.n_terms; t++)
+            if (std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) == alist.end()) alist.push_back(sd.terms[t].tf_deriv);
+        std::sort(alist.begin(), alist.end());
+        f.na = (int)alist.size();
+        std::vector<int32_t> tinfo(3 * sd.n_terms);
+        for (int t = 0; t < sd.n_terms; t++) {
+            tinfo[t] = (int32_t)(std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) - alist.begin());
+            tinfo[sd.n_terms + t] = sd.terms[t].dof_deriv;
+            tinfo[2 * sd.n_terms + t] = sd.terms[t].coef;
+            f.bmask |= 1 << sd.terms[t].dof_deriv;
+        }
+        // Gauss points per k-step: qb * na <= 32 k slots, at most SLX_FK_RU per producer thread
+        f.qb = std::min(32 / f.na, SLX_FK_RU);
+        if (f.qb < 1) continue;
+        f.nks = (f.ngp + f.qb - 1) / f.qb;
+        if (factor_lds_bytes(f) > 160 * 1024) continue;
+        // P fragments: lane l of tile (ks, mt) holds P^T[i = mt*16 + l%16][k = 8*(l/16) + v],
+        // k -> (Gauss point ks*qb + k/na, tf derivative alist[k%na]); power-of-two scale to [128, 256)
+        const double* tft = bd.tables[sd.tf_table];
+        const int nff = bd.table_nff[sd.tf_table], ngp = bd.n_gp;
+        double amax = 0.0;
+        for (int a : alist)
+            for (int i = 0; i < sd.n_tf; i++)
+                for (int q = 0; q < ngp; q++) amax = std::max(amax, std::fabs(tft[((size_t)a * nff + i) * ngp + q]));
+        int aex = 0;
+        if (amax > 0) (void)std::frexp(amax, &aex);
+        const double ascale = std::ldexp(1.0, 8 - aex);
+        f.a_unscale = (float)std::ldexp(1.0, aex - 8);
+        std::vector<uint16_t> frag((size_t)f.nks * f.mt * 2 * 64 * 8, 0);
+        for (int ks = 0; ks < f.nks; ks++)
+            for (int mt = 0; mt < f.mt; mt++)
+                for (int l = 0; l < 64; l++)
+                    for (int v = 0; v < 8; v++) {
+                        const int i = mt * 16 + l % 16, kl = 8 * (l / 16) + v, qi = kl / f.na, ai = kl % f.na, q = ks * f.qb + qi;
+                        float x = 0.f;
+                        if (i < sd.n_tf && qi < f.qb && q < ngp) x = (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+                        const uint16_t hi = f32_to_f16(x);
+                        const uint16_t lo = f32_to_f16(x - f16_to_f32(hi));
+                        const size_t base = ((((size_t)ks * f.mt + mt) * 2) * 64 + l) * 8 + v;
+                        frag[base] = hi;
+                        frag[base + 64 * 8] = lo;
+                    }
+        // dof table times the weights, Gauss point major, padded columns
+        const double* dft = bd.tables[sd.dof_table];
+        const int nfd = bd.table_nff[sd.dof_table];
+        std::vector<float> tq((size_t)ngp * 4 * f.ntp, 0.f);
+        double tqmax = 0.0;
+        for (int q = 0; q < ngp; q++)
+            for (int d = 0; d < 4; d++)
+                for (int j = 0; j < sd.n_dof; j++) {
+                    const double x = bd.gp_weights[q] * dft[((size_t)d * nfd + j) * ngp + q];
+                    tq[((size_t)q * f.ntp + j) * 4 + d] = (float)x;
+                    tqmax = std::max(tqmax, std::fabs(x));
+                }
+        f.tqmax = (float)(tqmax * 1.0001);
+        f.tinfo = P.dev(tinfo.data(), tinfo.size());
+        f.afrag = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
+        f.tq = P.dev(tq.data(), tq.size());
+        f.ke_base = L.ke_base[b][s];
+        f.ke_ld = L.ke_ld[b][s];
+        P.factor[b].push_back({s, f});
+        P.factor_mask[b] |= 1u << s;
+    }
+}
+
 void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.dbuckets.resize(d.n_buckets);
     P.mfma.assign(d.n_buckets, MfmaBucket());
     for (auto& m : P.mfma) m.M = 0;
     P.mfma_mask.assign(d.n_buckets, 0);
+    P.factor.assign(d.n_buckets, {});
+    P.factor_mask.assign(d.n_buckets, 0);
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, P.ctx->device) == hipSuccess && prop.multiProcessorCount > 0) P.n_cu = prop.multiProcessorCount;
@@ -863,6 +955,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         for (size_t k = 0; k < (size_t)bd.n_elem * bd.n_geo; k++)
             SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
         build_mfma(P, bd, b, L);
+        build_factor(P, bd, b, L);
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));
@@ -923,8 +1016,11 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         const DevBucket& b = P.dbuckets[bi];
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
+        // the factorized kernel reads the coefficients of one element contiguously
+        const int qfast = (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) ? 1 : 0;
+        P.last_qfast = qfast;
         if (tm) tm->begin(T_COEF);
-        HIPCHECK(launch_coef(b, in, ke64, P.d_ctab.p, st));
+        HIPCHECK(launch_coef(b, in, ke64, P.d_ctab.p, st, qfast));
         if (tm) tm->end();
         if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
             if (tm) tm->begin(T_MFMA);
@@ -932,10 +1028,16 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
             if (tm) tm->end();
             done = P.mfma_mask[bi];
         }
+        if (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) {
+            if (tm) tm->begin(T_MFMA);
+            for (auto& fs : P.factor[bi]) HIPCHECK(launch_factor_f16split(b, fs.second, P.d_ctab.as<float>(), P.d_ke.as<float>(), P.n_cu, st));
+            if (tm) tm->end();
+            done |= P.factor_mask[bi];
+        }
         int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
         if ((all & ~done) != 0) {
             if (tm) tm->begin(T_VALU);
-            HIPCHECK(launch_elem(b, P.d_ctab.p, p, P.d_ke.p, st, all & ~done));
+            HIPCHECK(launch_elem(b, P.d_ctab.p, p, P.d_ke.p, st, all & ~done, qfast));
             if (tm) tm->end();
         }
     }
@@ -1217,7 +1319,16 @@ slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* 
         } else {
             std::vector<float> tmp(n);
             HIPCHECK(hipMemcpy(tmp.data(), p->d_ctab.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
-            for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
+            const int64_t nq = b.q_inv ? 1 : b.n_gp, ne = b.n_elem;
+            if (p->last_qfast) {
+                // (slot, element, q) -> (slot, q, element)
+                for (int64_t i = 0; i < n; i++) {
+                    const int64_t c = i / (nq * ne), r = i % (nq * ne), e = r / nq, q = r % nq;
+                    out[(c * nq + q) * ne + e] = tmp[i];
+                }
+            } else {
+                for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
+            }
         }
     });
 }
