@@ -13,7 +13,9 @@
 
 #define SLX_FK_WAVES 8     // consumer (MFMA) waves
 #define SLX_FK_BWAVES 4    // producer waves: 256 threads, one per padded dof column
-#define SLX_FK_RU 16       // Gauss points per k-step at most (registers of a producer thread)
+#define SLX_FK_RU 16       // Gauss points per k-step at most
+#define SLX_FK_AREG 8      // P chunk words per producer thread (mt * 128 <= 8 * 256)
+#define SLX_FK_MAXT 16     // terms of a slice
 
 namespace slx {
 

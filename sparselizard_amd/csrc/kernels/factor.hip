@@ -52,7 +52,6 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
     uint4v* Bs = As + 2 * MT * 128;                             // [2][NTB][2][64]
     float* Ms = reinterpret_cast<float*>(Bs + 2 * NTB * 128);   // [ngp][4 tf derivative][4 dof derivative]
     unsigned* smax = reinterpret_cast<unsigned*>(Ms + ngp * 16);
-    _Float16* Bh = reinterpret_cast<_Float16*>(Bs);
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool mfma_wave = w < SLX_FK_WAVES;
     const int32_t* t_ai = f.tinfo;
@@ -61,8 +60,6 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
     const int64_t stride = (int64_t)nq * n_elem;
     const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
     const int achunk = MT * 128;
-    // k slots past qb * na stay zero for the whole launch (their A rows are zero)
-    for (int x = tid; x < 2 * NTB * 128; x += blockDim.x) Bs[x] = uint4v{0u, 0u, 0u, 0u};
     const int npass = (NT + NTB - 1) / NTB;
 
     for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
@@ -76,7 +73,13 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
 #pragma unroll
             for (int x = 0; x < 16; x++) mq[x] = 0.f;
             const int qq = nq == 1 ? 0 : q;
-            for (int t = 0; t < f.n_terms; t++) mq[t_ai[t] * 4 + t_b[t]] += ctab[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq];
+            float cv[SLX_FK_MAXT];
+#pragma unroll
+            for (int t = 0; t < SLX_FK_MAXT; t++)
+                cv[t] = t < f.n_terms ? ctab[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq] : 0.f;
+#pragma unroll
+            for (int t = 0; t < SLX_FK_MAXT; t++)
+                if (t < f.n_terms) mq[t_ai[t] * 4 + t_b[t]] += cv[t];
 #pragma unroll
             for (int ai = 0; ai < 4; ai++) m = fmaxf(m, fabsf(mq[4 * ai]) + fabsf(mq[4 * ai + 1]) + fabsf(mq[4 * ai + 2]) + fabsf(mq[4 * ai + 3]));
         }
@@ -149,7 +152,9 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
                             const int i = mt * 16 + 4 * (lane >> 4) + r;
-                            if (i < f.n_tf) eb[i * ld + j] = acc[mi][nj][r] * unsc;
+                            // streaming store: element matrices must not evict the shared
+                            // P fragments and dof table from L2
+                            if (i < f.n_tf) __builtin_nontemporal_store(acc[mi][nj][r] * unsc, eb + i * ld + j);
                         }
                     }
                 }
@@ -164,40 +169,56 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
             // tile j / 16, element k % 8).
             const int bt = tid - 64 * SLX_FK_WAVES;
             const float4v* tq4 = reinterpret_cast<const float4v*>(f.tq);
-            // thread bt: column bc = bt % (16 * NTB) of the pass, Gauss points qi = bg, bg + ng, ...
+            // thread bt: column bc = bt % CW of the pass and the 8-entry k groups
+            // kg = bt / CW, kg + NG, ... of the k-step (4 groups of 8 make the 32
+            // k slots); entry k is Gauss point k / na, tf derivative k % na.
+            // A group is packed in registers and written as one 16-byte hi and
+            // one 16-byte lo fragment word.
             constexpr int CW = 16 * NTB;
             constexpr int NG = (64 * SLX_FK_BWAVES) / CW;
-            const int bc = bt % CW, bg = bt / CW;
+            static_assert(NG >= 1 && CW % 64 == 0, "a pass needs whole producer waves per column group");
+            // k group and everything derived from it are wave-uniform (scalar
+            // registers); the column is the lane's
+            const int bc = bt % CW, kg0 = __builtin_amdgcn_readfirstlane(bt / CW);
+            const unsigned nmagic = (65536u + na - 1) / na;  // k / na = (k * nmagic) >> 16 for k < 32, na <= 4
             auto produce = [&](int ks, int buf, int cb) {
-                for (int x = bt; x < achunk; x += 64 * SLX_FK_BWAVES) As[buf * achunk + x] = afrag[(size_t)ks * achunk + x];
-                if (bg >= NG) return;
+                // the shared P chunk: every load issued before the first store
+                uint4v areg[SLX_FK_AREG];
+                const uint4v* asrc = afrag + (size_t)ks * achunk;
+#pragma unroll
+                for (int u = 0; u < SLX_FK_AREG; u++) {
+                    const int x = bt + u * 64 * SLX_FK_BWAVES;
+                    if (x < achunk) areg[u] = asrc[x];
+                }
                 const int j = cb * 16 + bc;
                 const bool jon = j < ntp;
-                _Float16* col = Bh + ((size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15)) * 8;
-                // Gauss points in groups of 8: loads first, then the products
-                for (int q0 = bg; q0 < qb; q0 += 8 * NG) {
+                uint4v* colw = Bs + (size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15);
+                for (int kg = kg0; kg < 4; kg += NG) {
                     float4v h[8];
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int qi = q0 + u * NG, q = ks * qb + qi;
-                        h[u] = (jon && qi < qb && q < ngp) ? tq4[(size_t)q * ntp + j] : float4v{0.f, 0.f, 0.f, 0.f};
+                    for (int v = 0; v < 8; v++) {
+                        const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), q = ks * qb + qi;
+                        const float4v* row = tq4 + (size_t)min(q, ngp - 1) * ntp;  // uniform row of the table
+                        h[v] = (jon && qi < qb && q < ngp) ? row[j] : float4v{0.f, 0.f, 0.f, 0.f};
                     }
+                    half8 hv, lv;
 #pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        const int qi = q0 + u * NG;
-                        if (qi >= qb) break;
+                    for (int v = 0; v < 8; v++) {
+                        const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), ai = k - qi * na;
                         const int q = min(ks * qb + qi, ngp - 1);
-                        for (int ai = 0; ai < na; ai++) {
-                            const float4v mm = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
-                            const float val = (mm[0] * h[u][0] + mm[1] * h[u][1] + mm[2] * h[u][2] + mm[3] * h[u][3]) * sc;
-                            const _Float16 hi = (_Float16)val;
-                            const _Float16 lo = (_Float16)(val - (float)hi);
-                            const int k = qi * na + ai;
-                            const int at = 16 * 8 * (k >> 3) + (k & 7);
-                            col[at] = hi;
-                            col[at + 64 * 8] = lo;
-                        }
+                        const float4v mm = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
+                        const float val = (mm[0] * h[v][0] + mm[1] * h[v][1] + mm[2] * h[v][2] + mm[3] * h[v][3]) * sc;
+                        const _Float16 hi = (_Float16)val;
+                        hv[v] = hi;
+                        lv[v] = (_Float16)(val - (float)hi);
                     }
+                    colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
+                    colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+                }
+#pragma unroll
+                for (int u = 0; u < SLX_FK_AREG; u++) {
+                    const int x = bt + u * 64 * SLX_FK_BWAVES;
+                    if (x < achunk) As[buf * achunk + x] = areg[u];
                 }
             };
             for (int pass = 0; pass < npass; pass++) {

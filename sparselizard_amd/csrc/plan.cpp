@@ -773,7 +773,7 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
-        if ((int64_t)sd.n_tf * sd.n_dof <= 256 || sd.n_terms > 16) continue;
+        if ((int64_t)sd.n_tf * sd.n_dof <= 256 || sd.n_terms > SLX_FK_MAXT) continue;
         FactorSlice f;
         std::memset(&f, 0, sizeof(f));
         f.n_tf = sd.n_tf;
@@ -785,6 +785,7 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         f.ntp = f.nt * 16;
         f.n_terms = sd.n_terms;
         if (!factor_variant_fits(f.mt, f.nt, &f.bm, &f.bn)) continue;
+        if (f.mt * 128 > SLX_FK_AREG * 64 * SLX_FK_BWAVES) continue;
         // tf derivatives in use; every term's (derivative index, dof derivative, slot)
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
