@@ -451,18 +451,31 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
     const int lenA = rw.lenA, lenD = rw.lenD, nk = rw.nk;
     int64_t sp = rw.sp;
     for (int x = l; x < lenA + lenD; x += LPR) acc[x] = 0.0;
+    // segment descriptors: the row's lanes load a batch together (DPL each) and
+    // pass them around with shuffles; the next batch is fetched while the
+    // current one is summed
+    constexpr int DPL = LPR >= SLX_RED_BATCH ? 1 : SLX_RED_BATCH / LPR;
+    uint64_t dnext[DPL];
+#pragma unroll
+    for (int t = 0; t < DPL; t++) {
+        const int i = l * DPL + t;
+        dnext[t] = (i < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
+    }
     for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
+        uint64_t dcur[DPL];
+#pragma unroll
+        for (int t = 0; t < DPL; t++) {
+            dcur[t] = dnext[t];
+            const int i = kb + SLX_RED_BATCH + l * DPL + t;
+            dnext[t] = (l * DPL + t < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
+        }
         Vec4<KT> v[SLX_RED_BATCH];
         Vec4<ST> sl[SLX_RED_BATCH];
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++) {
-            int n = 0;
-            int64_t off = 0;
-            if (kb + u < nk) {
-                const uint64_t dsc = rd.erows[k0 + kb + u];
-                n = (int)(dsc >> 48);
-                off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-            }
+            const uint64_t dsc = __shfl(dcur[u % DPL], u / DPL, LPR);
+            const int n = (int)(dsc >> 48);
+            const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
             if (4 * l < n) {
                 v[u] = load_vals<KT>(ke + off + 4 * l);
                 sl[u] = load_slots<ST>(slots + sp + 4 * l);
