@@ -452,28 +452,21 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
     int64_t sp = rw.sp;
     for (int x = l; x < lenA + lenD; x += LPR) acc[x] = 0.0;
     // segment descriptors: the row's lanes load a batch together (DPL each) and
-    // pass them around with shuffles; the next batch is fetched while the
-    // current one is summed
+    // pass them around with shuffles.  Software pipeline: the values and slots
+    // of batch b + 1 and the descriptors of batch b + 2 are in flight while
+    // batch b is summed.
     constexpr int DPL = LPR >= SLX_RED_BATCH ? 1 : SLX_RED_BATCH / LPR;
-    uint64_t dnext[DPL];
-#pragma unroll
-    for (int t = 0; t < DPL; t++) {
-        const int i = l * DPL + t;
-        dnext[t] = (i < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
-    }
-    for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
-        uint64_t dcur[DPL];
+    auto load_desc = [&](int kb, uint64_t (&d)[DPL]) {
 #pragma unroll
         for (int t = 0; t < DPL; t++) {
-            dcur[t] = dnext[t];
-            const int i = kb + SLX_RED_BATCH + l * DPL + t;
-            dnext[t] = (l * DPL + t < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
+            const int i = kb + l * DPL + t;
+            d[t] = (l * DPL + t < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
         }
-        Vec4<KT> v[SLX_RED_BATCH];
-        Vec4<ST> sl[SLX_RED_BATCH];
+    };
+    auto load_batch = [&](const uint64_t (&d)[DPL], Vec4<KT> (&v)[SLX_RED_BATCH], Vec4<ST> (&sl)[SLX_RED_BATCH]) {
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++) {
-            const uint64_t dsc = __shfl(dcur[u % DPL], u / DPL, LPR);
+            const uint64_t dsc = __shfl(d[u % DPL], u / DPL, LPR);
             const int n = (int)(dsc >> 48);
             const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
             if (4 * l < n) {
@@ -488,6 +481,24 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
             }
             sp += (n + 3) & ~3;
         }
+    };
+    uint64_t dsc_next[DPL];
+    Vec4<KT> v[SLX_RED_BATCH];
+    Vec4<ST> sl[SLX_RED_BATCH];
+    {
+        uint64_t d0_[DPL];
+        load_desc(0, d0_);
+        load_desc(SLX_RED_BATCH, dsc_next);
+        load_batch(d0_, v, sl);
+    }
+    for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
+        Vec4<KT> vn[SLX_RED_BATCH];
+        Vec4<ST> sn[SLX_RED_BATCH];
+        const bool more = kb + SLX_RED_BATCH < nk;
+        if (more) {
+            load_batch(dsc_next, vn, sn);
+            load_desc(kb + 2 * SLX_RED_BATCH, dsc_next);
+        }
         // LDS FP64 add without return (ds_add_f64): no read-after-write wait per
         // update; one wave's LDS operations complete in issue order and the
         // lanes of one segment never share a slot, so the order of the
@@ -496,6 +507,13 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
         for (int u = 0; u < SLX_RED_BATCH; u++)
 #pragma unroll
             for (int c = 0; c < 4; c++) slot_add<ST, SG>(acc, sl[u].x[c], (double)v[u].x[c]);
+        if (more) {
+#pragma unroll
+            for (int u = 0; u < SLX_RED_BATCH; u++) {
+                v[u] = vn[u];
+                sl[u] = sn[u];
+            }
+        }
     }
     for (int x = l; x < lenA; x += LPR) Aval[a0 + x] = acc[x];
     for (int x = l; x < lenD; x += LPR) Dval[d0 + x] = acc[lenA + x];
