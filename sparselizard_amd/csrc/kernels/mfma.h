@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include "device.h"
 
+#define SLX_MFMA_MAXP 16   // derivative pairs of a batched-MFMA bucket
+
 namespace slx {
 
 struct MfmaBucket {

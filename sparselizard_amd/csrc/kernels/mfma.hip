@@ -62,12 +62,14 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
     uint32_t* Cs = reinterpret_cast<uint32_t*>(smem_raw);                    // [NE][estride]
     float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * estride);        // [ns][NE]
     float* stg = colsc + (size_t)ns * NE;                                      // [16][n_tf][ld] of one group
+    int* csl = reinterpret_cast<int*>(stg + (size_t)16 * mb.stg_words);        // [ns][np] coefficient slots
     const SLX_CONST int32_t* csrc = (const SLX_CONST int32_t*)mb.csrc;
     const SLX_CONST int64_t* gbase = (const SLX_CONST int64_t*)mb.gbase;
     const SLX_CONST int32_t* gld = (const SLX_CONST int32_t*)mb.gld;
     const SLX_CONST int32_t* sgrp = (const SLX_CONST int32_t*)mb.sgrp;
     const SLX_CONST int32_t* scol = (const SLX_CONST int32_t*)mb.scol;
 
+    for (int x = threadIdx.x; x < ns * np; x += blockDim.x) csl[x] = csrc[x];  // read after the first barrier
     const int nw = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int lrow = lane & 15, lgrp = lane >> 4;
@@ -117,31 +119,54 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
             const int el = col % NE, s = col / NE;
             const bool valid = el < nloc;
             const int64_t e = e0 + (valid ? el : 0);
-            float mx = 0.f;
-            for (int p = 0; p < np; p++) {
-                const int c = csrc[s * np + p];
-                if (c < 0 || !valid) continue;
-                for (int q = 0; q < nq; q++) mx = fmaxf(mx, fabsf(ctab[((int64_t)c * nq + q) * nel + e]));
-            }
-            int ex = 0;
-            if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
-            const float sc = ldexpf(1.f, -ex);
-            colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
             uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
-            for (int p = 0; p < np; p++) {
-                const int c = csrc[s * np + p];
-                if (COMPACT) {
-                    dst[p] = pack_split((c >= 0 && valid) ? ctab[(int64_t)c * nel + e] * sc : 0.f);
-                } else {
-                    for (int q = 0; q < ngp; q++) {
-                        const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
-                        dst[p * ngp + q] = pack_split(x);
+            float mx = 0.f;
+            if (COMPACT) {
+                // every coefficient of the column loaded at once (independent loads)
+                float cv[SLX_MFMA_MAXP];
+#pragma unroll
+                for (int p = 0; p < SLX_MFMA_MAXP; p++) {
+                    const int c = p < np ? csl[s * np + p] : -1;
+                    cv[p] = (c >= 0 && valid) ? ctab[(int64_t)c * nel + e] : 0.f;
+                    mx = fmaxf(mx, fabsf(cv[p]));
+                }
+                int ex = 0;
+                if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+                const float sc = ldexpf(1.f, -ex);
+                colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+#pragma unroll
+                for (int p = 0; p < SLX_MFMA_MAXP; p++)
+                    if (p < np) dst[p] = pack_split(cv[p] * sc);
+                dst[np] = 0u;
+            } else {
+                // Gauss-point values in batches of 8 independent loads
+                for (int p = 0; p < np; p++) {
+                    const int c = csl[s * np + p];
+                    if (c < 0 || !valid) continue;
+                    for (int q0 = 0; q0 < nq; q0 += 8) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++)
+                            if (q0 + u < nq) mx = fmaxf(mx, fabsf(ctab[((int64_t)c * nq + q0 + u) * nel + e]));
                     }
                 }
-            }
-            if (COMPACT) dst[np] = 0u;
-            else
+                int ex = 0;
+                if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+                const float sc = ldexpf(1.f, -ex);
+                colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+                for (int p = 0; p < np; p++) {
+                    const int c = csl[s * np + p];
+                    for (int q0 = 0; q0 < ngp; q0 += 8) {
+#pragma unroll
+                        for (int u = 0; u < 8; u++) {
+                            const int q = q0 + u;
+                            if (q >= ngp) break;
+                            const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
+                            dst[p * ngp + q] = pack_split(x);
+                        }
+                    }
+                }
                 for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
+            }
         }
         __syncthreads();
         // contraction: per 16-element column tile, the wave's M tiles of every
@@ -274,7 +299,8 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
 }
 
 size_t mfma_lds_bytes(const MfmaBucket& mb) {
-    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.stg_words * 4;
+    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.stg_words * 4 +
+           (size_t)mb.n_mslices * mb.n_pairs * 4;
 }
 
 bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw) {

@@ -644,6 +644,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.n_tf = n_tf;
     mb.n_dof = n_dof;
     mb.n_pairs = (int)pairs.size();
+    if (mb.n_pairs > SLX_MFMA_MAXP) return;
     mb.nq = bd.q_invariant ? 1 : bd.n_gp;
     // compact columns (coefficients constant over the Gauss points): the
     // Gauss points of a pair are padded to a multiple of 8 in K
