@@ -17,6 +17,7 @@ struct MfmaBucket {
     int32_t n_mslices;          // matrix slices handled
     int32_t nw, mtw, maxk;      // waves per workgroup, M tiles per wave, K steps (template, = ksteps)
     int32_t compact;            // compact (constant-coefficient) column layout
+    int32_t sgroup;             // slices per contraction group (divides n_mslices; 1 unless compact, one M tile per wave)
     int32_t cstride, estride;   // LDS words per column / per element (all slices)
     int32_t max_blocks;         // persistent grid size
     float a_unscale;            // 2^-kA of the scaled Phi

@@ -51,7 +51,7 @@ __device__ inline uint32_t pack_split(float x) {
     return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
 }
 
-template <int MTW, int MAXK, bool COMPACT>
+template <int MTW, int MAXK, bool COMPACT, int G>
 __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW == 1 ? (MAXK >= 8 ? 3 : 4) : 2, 8))) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
                                                             float* __restrict__ ke) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -175,39 +175,43 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
         const int nct = (NE + 15) / 16;
         for (int ct = 0; ct < nct; ct++) {
             const int el = ct * 16 + lrow;  // this lane's B / D column
-            // slices two at a time: six independent accumulator chains per
-            // M tile (hi*hi, hi*lo', lo'*hi for each slice) share the A fragments
-            for (int s0 = 0; s0 < ns; s0 += 2) {
-                const bool two = s0 + 1 < ns;
-                float4v acc_hh[2][MTW], acc_xa[2][MTW], acc_xb[2][MTW];
+            // slices G at a time (G divides the slice count): 2 G independent
+            // accumulator chains per M tile (hi*hi, and hi*lo' + lo'*hi) share the
+            // A fragments; every LDS word of the group is read before the MFMAs
+            for (int s0 = 0; s0 < ns; s0 += G) {
+                float4v acc_hh[G][MTW], acc_x[G][MTW];
 #pragma unroll
-                for (int c = 0; c < 2; c++)
+                for (int c = 0; c < G; c++)
 #pragma unroll
                     for (int i = 0; i < MTW; i++)
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
                             acc_hh[c][i][v] = 0.f;
-                            acc_xa[c][i][v] = 0.f;
-                            acc_xb[c][i][v] = 0.f;
+                            acc_x[c][i][v] = 0.f;
                         }
                 const uint32_t* colbase0 = Cs + (size_t)el * estride + (size_t)s0 * cstride;
-                const uint32_t* colbase1 = colbase0 + (two ? cstride : 0);
+                uint32_t cw[G][COMPACT ? MAXK : 1];
+                if (COMPACT) {
+#pragma unroll
+                    for (int c = 0; c < G; c++)
+#pragma unroll
+                        for (int ks = 0; ks < MAXK; ks++) cw[c][ks] = colbase0[c * cstride + pword[ks]];
+                }
 #pragma unroll
                 for (int ks = 0; ks < MAXK; ks++) {
-                    half8 bhi[2], blo[2];
 #pragma unroll
-                    for (int c = 0; c < 2; c++) {
-                        const uint32_t* colbase = c == 0 ? colbase0 : colbase1;
+                    for (int c = 0; c < G; c++) {
                         uint4v hw, lw;
                         if (COMPACT) {
                             // one packed word (hi | lo' << 16) -> both halves broadcast
                             // to all eight k of the lane (byte permutes)
-                            const uint32_t u = colbase[pword[ks]];
+                            const uint32_t u = cw[c][COMPACT ? ks : 0];
                             const uint32_t h2 = __builtin_amdgcn_perm(u, u, 0x01000100u);
                             const uint32_t l2 = __builtin_amdgcn_perm(u, u, 0x03020302u);
                             hw = (uint4v){h2, h2, h2, h2};
                             lw = (uint4v){l2, l2, l2, l2};
                         } else {
+                            const uint32_t* colbase = colbase0 + c * cstride;
                             const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
                             const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
                             hw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x05040100u);
@@ -219,25 +223,19 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                             lw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x07060302u);
                             lw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x07060302u);
                         }
-                        bhi[c] = __builtin_bit_cast(half8, hw);
-                        blo[c] = __builtin_bit_cast(half8, lw);
-                    }
+                        const half8 bhi = __builtin_bit_cast(half8, hw);
+                        const half8 blo = __builtin_bit_cast(half8, lw);
 #pragma unroll
-                    for (int i = 0; i < MTW; i++) {
-                        if (i > 0 && wave + nw * i >= mb.mtiles) continue;  // wave-uniform: no work for an empty tile
-                        acc_hh[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi[0], acc_hh[0][i], 0, 0, 0);
-                        acc_xa[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo[0], acc_xa[0][i], 0, 0, 0);
-                        acc_xb[0][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi[0], acc_xb[0][i], 0, 0, 0);
-                        if (two) {
-                            acc_hh[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi[1], acc_hh[1][i], 0, 0, 0);
-                            acc_xa[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo[1], acc_xa[1][i], 0, 0, 0);
-                            acc_xb[1][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi[1], acc_xb[1][i], 0, 0, 0);
+                        for (int i = 0; i < MTW; i++) {
+                            if (i > 0 && wave + nw * i >= mb.mtiles) continue;  // wave-uniform: no work for an empty tile
+                            acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi, acc_hh[c][i], 0, 0, 0);
+                            acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo, acc_x[c][i], 0, 0, 0);
+                            acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[c][i], 0, 0, 0);
                         }
                     }
                 }
 #pragma unroll
-                for (int c = 0; c < 2; c++) {
-                    if (c == 1 && !two) break;
+                for (int c = 0; c < G; c++) {
                     const int s = s0 + c;
                     const int g = sgrp[s];
                     float* gstg = stg;  // LDS block of group g: [16][n_tf * ld + 4]
@@ -252,8 +250,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                         if (mt >= mb.mtiles) continue;
 #pragma unroll
                         for (int v = 0; v < 4; v++)
-                            if (rowok[i][v])
-                                out[irow[i][v] * ld + jrow[i][v]] = acc_hh[c][i][v] * colscale + (acc_xa[c][i][v] + acc_xb[c][i][v]) * xscale;
+                            if (rowok[i][v]) out[irow[i][v] * ld + jrow[i][v]] = acc_hh[c][i][v] * colscale + acc_x[c][i][v] * xscale;
                     }
                 }
             }
@@ -291,10 +288,17 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
     const size_t lds = mfma_lds_bytes(mb);
     const int tiles = (b.n_elem + mb.ne_tile - 1) / mb.ne_tile;
     const int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
-    if (mb.compact)
-        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true>), dim3(grid), dim3(64 * mb.nw), lds, st, b, mb, ctab, ke);
-    else
-        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, false>), dim3(grid), dim3(64 * mb.nw), lds, st, b, mb, ctab, ke);
+    const dim3 gd(grid), bd(64 * mb.nw);
+    if (!mb.compact || MTW != 1 || mb.sgroup == 1) {
+        if (mb.compact)
+            hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+        else
+            hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, false, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+    } else if (mb.sgroup == 2) {
+        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, (MTW == 1 ? 2 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+    } else {
+        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, (MTW == 1 ? 3 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+    }
     return hipGetLastError();
 }
 

@@ -190,7 +190,7 @@ struct SegGroup {
     std::vector<int> slices;
     int n_tf;
     int64_t len;      // segment length (sum of the slices' n_dof, 1 for rhs)
-    int64_t ld;       // segment stride: len, rounded up to a multiple of 16 when len >= 16
+    int64_t ld = 0;       // segment stride: len, rounded up to a multiple of 16 when len >= 16
     int64_t base;     // Ke offset of the group (a multiple of 16)
     int rank;         // smallest fragment rank of the group's slices
     bool rhs;
@@ -654,10 +654,13 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
     mb.n_mslices = (int)ms.size();
+    mb.sgroup = 1;
+    if (mb.compact) mb.sgroup = mb.n_mslices % 3 == 0 ? 3 : (mb.n_mslices % 2 == 0 ? 2 : 1);
     // tile split: SLX_TUNE_MFMA_MTW=2 selects two M tiles per wave (tuning knob)
     int prefer_mtw = 1;
     if (const char* t = std::getenv("SLX_TUNE_MFMA_MTW")) prefer_mtw = std::atoi(t);
     if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk, prefer_mtw)) return;
+    if (mb.mtw != 1) mb.sgroup = 1;
     mb.cstride = mb.compact ? mb.n_pairs + 1 : 32 * mb.ksteps;
     int es = mb.n_mslices * mb.cstride;
     if (mb.compact) es |= 1;  // odd: conflict-free 32-bit reads across elements
