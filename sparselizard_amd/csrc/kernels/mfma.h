@@ -4,6 +4,7 @@
 #include "device.h"
 
 #define SLX_MFMA_MAXP 16   // derivative pairs of a batched-MFMA bucket
+#define SLX_MFMA_PWAVES 4  // producer waves of the batched-MFMA workgroup
 
 namespace slx {
 
@@ -13,7 +14,7 @@ struct MfmaBucket {
     int32_t n_tf, n_dof;
     int32_t mtiles, ksteps;     // ceil(M / 16), ceil(K / 32)
     int32_t n_pairs, nq;        // derivative pairs; 1 when the coefficients do not vary over Gauss points
-    int32_t ne_tile;            // elements per workgroup tile
+    int32_t ne_tile;            // elements per workgroup tile (16)
     int32_t n_mslices;          // matrix slices handled
     int32_t nw, mtw, maxk;      // waves per workgroup, M tiles per wave, K steps (template, = ksteps)
     int32_t compact;            // compact (constant-coefficient) column layout

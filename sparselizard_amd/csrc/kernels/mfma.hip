@@ -51,130 +51,80 @@ __device__ inline uint32_t pack_split(float x) {
     return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
 }
 
+// Work split (one workgroup per CU, persistent over 16-element tiles):
+//   consumer waves (one or two M tiles each) run the contraction of tile s
+//     from the column buffer Cs[s & 1] into the staging buffer stg[s & 1];
+//   producer waves (SLX_MFMA_PWAVES) meanwhile prepare the columns of tile
+//     s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from stg[(s - 1) & 1] to
+//     the element matrices with coalesced 16-byte stores.
+// One barrier per tile; every buffer is written one tile after its last read.
 template <int MTW, int MAXK, bool COMPACT, int G>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW == 1 ? (MAXK >= 8 ? 3 : 4) : 2, 8))) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
-                                                            float* __restrict__ ke) {
+__global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
+                                                                                  float* __restrict__ ke) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-    const int NE = mb.ne_tile;
+    constexpr int NE = 16;
     const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq, ngp = b.n_gp;
     const int estride = mb.estride, cstride = mb.cstride;
     const int n_tf = mb.n_tf;
-    uint32_t* Cs = reinterpret_cast<uint32_t*>(smem_raw);                    // [NE][estride]
-    float* colsc = reinterpret_cast<float*>(Cs + (size_t)NE * estride);        // [ns][NE]
-    float* stg = colsc + (size_t)ns * NE;                                      // [16][n_tf][ld] of one group
-    int* csl = reinterpret_cast<int*>(stg + (size_t)16 * mb.stg_words);        // [ns][np] coefficient slots
+    uint32_t* Csb = reinterpret_cast<uint32_t*>(smem_raw);                     // [2][NE][estride]
+    float* colscb = reinterpret_cast<float*>(Csb + (size_t)2 * NE * estride);   // [2][ns][NE]
+    float* stgb = colscb + (size_t)2 * ns * NE;                                 // [2][16 * stg_words]
+    int* csl = reinterpret_cast<int*>(stgb + (size_t)2 * 16 * mb.stg_words);    // [ns][np] coefficient slots
     const SLX_CONST int32_t* csrc = (const SLX_CONST int32_t*)mb.csrc;
     const SLX_CONST int64_t* gbase = (const SLX_CONST int64_t*)mb.gbase;
     const SLX_CONST int32_t* gld = (const SLX_CONST int32_t*)mb.gld;
     const SLX_CONST int32_t* sgrp = (const SLX_CONST int32_t*)mb.sgrp;
     const SLX_CONST int32_t* scol = (const SLX_CONST int32_t*)mb.scol;
+    for (int x = threadIdx.x; x < ns * np; x += blockDim.x) csl[x] = csrc[x];
 
-    for (int x = threadIdx.x; x < ns * np; x += blockDim.x) csl[x] = csrc[x];  // read after the first barrier
-    const int nw = blockDim.x >> 6;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nw = mb.nw;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int lrow = lane & 15, lgrp = lane >> 4;
-    // A fragments of this wave's M tiles (tile = wave + nw * i)
-    half8 Ahi[MTW][MAXK], Alo[MTW][MAXK];
-    int irow[MTW][4], jrow[MTW][4];
-    bool rowok[MTW][4];
-#pragma unroll
-    for (int i = 0; i < MTW; i++) {
-        const int mt = wave + nw * i;
-#pragma unroll
-        for (int ks = 0; ks < MAXK; ks++) {
-            if (mt < mb.mtiles) {
-                const half8* src = reinterpret_cast<const half8*>(mb.afrag) + ((size_t)(mt * MAXK + ks) * 2) * 64;
-                Ahi[i][ks] = src[lane];
-                Alo[i][ks] = src[64 + lane];
-            } else {
-#pragma unroll
-                for (int v = 0; v < 8; v++) {
-                    Ahi[i][ks][v] = (_Float16)0.f;
-                    Alo[i][ks][v] = (_Float16)0.f;
-                }
-            }
-        }
-#pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = mt * 16 + 4 * lgrp + v;
-            irow[i][v] = row / mb.n_dof;
-            jrow[i][v] = row - irow[i][v] * mb.n_dof;
-            rowok[i][v] = row < mb.M;
-        }
-    }
-    // compact columns: the pair word of this lane's eight k per K step
-    int pword[MAXK];
-#pragma unroll
-    for (int ks = 0; ks < MAXK; ks++) pword[ks] = min((ks * 32 + 8 * lgrp) / mb.kq, np);
-
-    const int ntiles_e = (b.n_elem + NE - 1) / NE;
+    const int ntiles = (b.n_elem + NE - 1) / NE;
+    const int nsteps = ntiles > (int)blockIdx.x ? (ntiles - 1 - (int)blockIdx.x) / (int)gridDim.x + 1 : 0;
     const int64_t nel = b.n_elem;
-    for (int tile = blockIdx.x; tile < ntiles_e; tile += gridDim.x) {
-        const int e0 = tile * NE;
-        const int nloc = min(NE, b.n_elem - e0);
-        __syncthreads();
-        // columns: one thread per (element, slice), element fastest: power-of-two
-        // scale and packed hi/lo' words
-        for (int col = threadIdx.x; col < NE * ns; col += blockDim.x) {
-            const int el = col % NE, s = col / NE;
-            const bool valid = el < nloc;
-            const int64_t e = e0 + (valid ? el : 0);
-            uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
-            float mx = 0.f;
-            if (COMPACT) {
-                // every coefficient of the column loaded at once (independent loads)
-                float cv[SLX_MFMA_MAXP];
+    __syncthreads();  // csl
+
+    if (wave < nw) {
+        // ------------------------------------------------------- consumers --
+        // A fragments of this wave's M tiles (tile = wave + nw * i)
+        half8 Ahi[MTW][MAXK], Alo[MTW][MAXK];
+        int irow[MTW][4], jrow[MTW][4];
+        bool rowok[MTW][4];
 #pragma unroll
-                for (int p = 0; p < SLX_MFMA_MAXP; p++) {
-                    const int c = p < np ? csl[s * np + p] : -1;
-                    cv[p] = (c >= 0 && valid) ? ctab[(int64_t)c * nel + e] : 0.f;
-                    mx = fmaxf(mx, fabsf(cv[p]));
-                }
-                int ex = 0;
-                if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
-                const float sc = ldexpf(1.f, -ex);
-                colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+        for (int i = 0; i < MTW; i++) {
+            const int mt = wave + nw * i;
 #pragma unroll
-                for (int p = 0; p < SLX_MFMA_MAXP; p++)
-                    if (p < np) dst[p] = pack_split(cv[p] * sc);
-                dst[np] = 0u;
-            } else {
-                // Gauss-point values in batches of 8 independent loads
-                for (int p = 0; p < np; p++) {
-                    const int c = csl[s * np + p];
-                    if (c < 0 || !valid) continue;
-                    for (int q0 = 0; q0 < nq; q0 += 8) {
+            for (int ks = 0; ks < MAXK; ks++) {
+                if (mt < mb.mtiles) {
+                    const half8* src = reinterpret_cast<const half8*>(mb.afrag) + ((size_t)(mt * MAXK + ks) * 2) * 64;
+                    Ahi[i][ks] = src[lane];
+                    Alo[i][ks] = src[64 + lane];
+                } else {
 #pragma unroll
-                        for (int u = 0; u < 8; u++)
-                            if (q0 + u < nq) mx = fmaxf(mx, fabsf(ctab[((int64_t)c * nq + q0 + u) * nel + e]));
+                    for (int v = 0; v < 8; v++) {
+                        Ahi[i][ks][v] = (_Float16)0.f;
+                        Alo[i][ks][v] = (_Float16)0.f;
                     }
                 }
-                int ex = 0;
-                if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
-                const float sc = ldexpf(1.f, -ex);
-                colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
-                for (int p = 0; p < np; p++) {
-                    const int c = csl[s * np + p];
-                    for (int q0 = 0; q0 < ngp; q0 += 8) {
+            }
 #pragma unroll
-                        for (int u = 0; u < 8; u++) {
-                            const int q = q0 + u;
-                            if (q >= ngp) break;
-                            const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
-                            dst[p * ngp + q] = pack_split(x);
-                        }
-                    }
-                }
-                for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
+            for (int v = 0; v < 4; v++) {
+                const int row = mt * 16 + 4 * lgrp + v;
+                irow[i][v] = row / mb.n_dof;
+                jrow[i][v] = row - irow[i][v] * mb.n_dof;
+                rowok[i][v] = row < mb.M;
             }
         }
-        __syncthreads();
-        // contraction: per 16-element column tile, the wave's M tiles of every
-        // slice go to the LDS block of the slice's segment group; then the
-        // blocks of all groups are written out (two barriers per column tile)
-        const int nct = (NE + 15) / 16;
-        for (int ct = 0; ct < nct; ct++) {
-            const int el = ct * 16 + lrow;  // this lane's B / D column
+        // compact columns: the pair word of this lane's eight k per K step
+        int pword[MAXK];
+#pragma unroll
+        for (int ks = 0; ks < MAXK; ks++) pword[ks] = min((ks * 32 + 8 * lgrp) / mb.kq, np);
+        __syncthreads();  // tile 0 prepared
+        for (int st = 0; st < nsteps; st++) {
+            const uint32_t* Cs = Csb + (size_t)(st & 1) * NE * estride;
+            const float* colsc = colscb + (size_t)(st & 1) * ns * NE;
+            float* stg = stgb + (size_t)(st & 1) * 16 * mb.stg_words;
             // slices G at a time (G divides the slice count): 2 G independent
             // accumulator chains per M tile (hi*hi, and hi*lo' + lo'*hi) share the
             // A fragments; every LDS word of the group is read before the MFMAs
@@ -189,7 +139,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                             acc_hh[c][i][v] = 0.f;
                             acc_x[c][i][v] = 0.f;
                         }
-                const uint32_t* colbase0 = Cs + (size_t)el * estride + (size_t)s0 * cstride;
+                const uint32_t* colbase0 = Cs + (size_t)lrow * estride + (size_t)s0 * cstride;
                 uint32_t cw[G][COMPACT ? MAXK : 1];
                 if (COMPACT) {
 #pragma unroll
@@ -241,7 +191,7 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                     float* gstg = stg;  // LDS block of group g: [16][n_tf * ld + 4]
                     for (int g2 = 0; g2 < g; g2++) gstg += (size_t)16 * (n_tf * gld[g2] + 4);
                     const int ld = gld[g], srow = n_tf * ld + 4;
-                    const float colscale = colsc[s * NE + el];
+                    const float colscale = colsc[s * NE + lrow];
                     const float xscale = colscale * (1.f / 2048.f);
                     float* out = gstg + (size_t)lrow * srow + scol[s];
 #pragma unroll
@@ -254,41 +204,120 @@ __global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MTW ==
                     }
                 }
             }
-            __syncthreads();
-            // each group's rows of these 16 elements are one contiguous range
-            const int nel_here = min(16, nloc - ct * 16);
-            float* gsrc = stg;
+            __syncthreads();  // tile st staged; tile st + 1 prepared
+        }
+    } else {
+        // ------------------------------------------------------- producers --
+        const int pt = threadIdx.x - 64 * nw, np_threads = blockDim.x - 64 * nw;
+        // columns of a tile: one thread per (element, slice), element fastest:
+        // power-of-two scale and packed hi/lo' words
+        auto prepare = [&](int tile, int buf) {
+            const int e0 = tile * NE;
+            const int nloc = min(NE, b.n_elem - e0);
+            uint32_t* Cs = Csb + (size_t)buf * NE * estride;
+            float* colsc = colscb + (size_t)buf * ns * NE;
+            for (int col = pt; col < NE * ns; col += np_threads) {
+                const int el = col % NE, s = col / NE;
+                const bool valid = el < nloc;
+                const int64_t e = e0 + (valid ? el : 0);
+                uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
+                float mx = 0.f;
+                if (COMPACT) {
+                    // every coefficient of the column loaded at once (independent loads)
+                    float cv[SLX_MFMA_MAXP];
+#pragma unroll
+                    for (int p = 0; p < SLX_MFMA_MAXP; p++) {
+                        const int c = p < np ? csl[s * np + p] : -1;
+                        cv[p] = (c >= 0 && valid) ? ctab[(int64_t)c * nel + e] : 0.f;
+                        mx = fmaxf(mx, fabsf(cv[p]));
+                    }
+                    int ex = 0;
+                    if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+                    const float sc = ldexpf(1.f, -ex);
+                    colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+#pragma unroll
+                    for (int p = 0; p < SLX_MFMA_MAXP; p++)
+                        if (p < np) dst[p] = pack_split(cv[p] * sc);
+                    dst[np] = 0u;
+                } else {
+                    // Gauss-point values in batches of 8 independent loads
+                    for (int p = 0; p < np; p++) {
+                        const int c = csl[s * np + p];
+                        if (c < 0 || !valid) continue;
+                        for (int q0 = 0; q0 < nq; q0 += 8) {
+#pragma unroll
+                            for (int u = 0; u < 8; u++)
+                                if (q0 + u < nq) mx = fmaxf(mx, fabsf(ctab[((int64_t)c * nq + q0 + u) * nel + e]));
+                        }
+                    }
+                    int ex = 0;
+                    if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+                    const float sc = ldexpf(1.f, -ex);
+                    colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+                    for (int p = 0; p < np; p++) {
+                        const int c = csl[s * np + p];
+                        for (int q0 = 0; q0 < ngp; q0 += 8) {
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int q = q0 + u;
+                                if (q >= ngp) break;
+                                const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
+                                dst[p * ngp + q] = pack_split(x);
+                            }
+                        }
+                    }
+                    for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
+                }
+            }
+        };
+        // each group's rows of the tile's elements are one contiguous range
+        auto write_out = [&](int tile, int buf) {
+            const int e0 = tile * NE;
+            const int nel_here = min(NE, b.n_elem - e0);
+            const float* gsrc = stgb + (size_t)buf * 16 * mb.stg_words;
             for (int g = 0; g < mb.n_groups; g++) {
                 const int erow = n_tf * gld[g], srow = erow + 4;
-                if (nel_here > 0) {
-                    float* dst = ke + gbase[g] + (int64_t)(e0 + ct * 16) * erow;
-                    if (((uintptr_t)dst & 15u) == 0 && (erow & 3) == 0) {
-                        const int per = erow / 4;  // float4 per element row
-                        float4v* d4 = reinterpret_cast<float4v*>(dst);
-                        for (int x = threadIdx.x; x < nel_here * per; x += blockDim.x) {
-                            const int eo = x / per, w = x - eo * per;
-                            d4[x] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
+                float* dst = ke + gbase[g] + (int64_t)e0 * erow;
+                if (((uintptr_t)dst & 15u) == 0 && (erow & 3) == 0) {
+                    const int per = erow / 4;  // float4 per element row
+                    float4v* d4 = reinterpret_cast<float4v*>(dst);
+                    int eo = pt / per, w = pt - eo * per;
+                    const int step_e = np_threads / per, step_w = np_threads - step_e * per;
+                    for (int x = pt; x < nel_here * per; x += np_threads) {
+                        d4[x] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
+                        eo += step_e;
+                        w += step_w;
+                        if (w >= per) {
+                            w -= per;
+                            eo++;
                         }
-                    } else {
-                        for (int x = threadIdx.x; x < nel_here * erow; x += blockDim.x) {
-                            const int eo = x / erow, w = x - eo * erow;
-                            dst[x] = gsrc[(size_t)eo * srow + w];
-                        }
+                    }
+                } else {
+                    for (int x = pt; x < nel_here * erow; x += np_threads) {
+                        const int eo = x / erow, w = x - eo * erow;
+                        dst[x] = gsrc[(size_t)eo * srow + w];
                     }
                 }
                 gsrc += (size_t)16 * srow;
             }
-            __syncthreads();
+        };
+        if (nsteps > 0) prepare(blockIdx.x, 0);
+        __syncthreads();  // tile 0 prepared
+        for (int st = 0; st < nsteps; st++) {
+            if (st + 1 < nsteps) prepare(blockIdx.x + (st + 1) * gridDim.x, (st + 1) & 1);
+            if (st > 0) write_out(blockIdx.x + (st - 1) * gridDim.x, (st - 1) & 1);
+            __syncthreads();  // tile st staged; tile st + 1 prepared
         }
+        if (nsteps > 0) write_out(blockIdx.x + (nsteps - 1) * gridDim.x, (nsteps - 1) & 1);
     }
 }
 
 template <int MTW, int MAXK>
 static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(mb);
-    const int tiles = (b.n_elem + mb.ne_tile - 1) / mb.ne_tile;
+    const int tiles = (b.n_elem + 15) / 16;
     const int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
-    const dim3 gd(grid), bd(64 * mb.nw);
+    const dim3 gd(grid), bd(64 * (mb.nw + SLX_MFMA_PWAVES));
     if (!mb.compact || MTW != 1 || mb.sgroup == 1) {
         if (mb.compact)
             hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
@@ -303,7 +332,7 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
 }
 
 size_t mfma_lds_bytes(const MfmaBucket& mb) {
-    return (size_t)mb.ne_tile * mb.estride * 4 + (size_t)mb.n_mslices * mb.ne_tile * 4 + (size_t)16 * mb.stg_words * 4 +
+    return (size_t)2 * 16 * mb.estride * 4 + (size_t)2 * mb.n_mslices * 16 * 4 + (size_t)2 * 16 * mb.stg_words * 4 +
            (size_t)mb.n_mslices * mb.n_pairs * 4;
 }
 

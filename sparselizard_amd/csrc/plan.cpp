@@ -690,14 +690,9 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.max_gld = *std::max_element(gd.begin(), gd.end());
     mb.stg_words = 0;
     for (int32_t l : gd) mb.stg_words += n_tf * l + 4;
-    // element tile: two workgroups per CU must fit the 160 KB of LDS
-    mb.ne_tile = 0;
-    for (int ne : {64, 32, 16}) {
-        mb.ne_tile = ne;
-        if (mfma_lds_bytes(mb) <= 80 * 1024) break;
-        mb.ne_tile = 0;
-    }
-    if (mb.ne_tile == 0) return;
+    // 16-element tiles; double-buffered columns and staging of one workgroup per CU
+    mb.ne_tile = 16;
+    if (mfma_lds_bytes(mb) > 160 * 1024) return;
     // coefficient slot of every (slice, pair)
     std::vector<int32_t> csrc((size_t)mb.n_mslices * mb.n_pairs, -1);
     for (size_t si = 0; si < ms.size(); si++) {
@@ -751,7 +746,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.gld = P.dev(gd.data(), gd.size());
     mb.sgrp = P.dev(sgrp.data(), sgrp.size());
     mb.scol = P.dev(scol.data(), scol.size());
-    mb.max_blocks = P.n_cu * 2;
+    mb.max_blocks = P.n_cu;
     P.mfma[b] = mb;
     P.mfma_mask[b] = mask;
 }
