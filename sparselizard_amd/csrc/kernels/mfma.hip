@@ -301,11 +301,60 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                 gsrc += (size_t)16 * srow;
             }
         };
-        if (nsteps > 0) prepare(blockIdx.x, 0);
+        // compact columns: one column per producer thread, its coefficients
+        // loaded two tiles ahead (the loads of tile s + 2 are in flight while
+        // tile s + 1 is packed and tile s - 1 written)
+        float cvn[SLX_MFMA_MAXP];
+        auto load_cols = [&](int tile) {
+            const int e0 = tile * NE, nloc = min(NE, b.n_elem - e0);
+            const int el = pt % NE, s = pt / NE;
+            const bool valid = pt < NE * ns && el < nloc;
+            const int64_t e = e0 + (valid ? el : 0);
+#pragma unroll
+            for (int p = 0; p < SLX_MFMA_MAXP; p++) {
+                const int c = (valid && p < np) ? csl[s * np + p] : -1;
+                cvn[p] = c >= 0 ? ctab[(int64_t)c * nel + e] : 0.f;
+            }
+        };
+        auto finish_cols = [&](int buf) {
+            if (pt >= NE * ns) return;
+            const int el = pt % NE, s = pt / NE;
+            uint32_t* dst = Csb + (size_t)buf * NE * estride + (size_t)el * estride + (size_t)s * cstride;
+            float* colsc = colscb + (size_t)buf * ns * NE;
+            float mx = 0.f;
+#pragma unroll
+            for (int p = 0; p < SLX_MFMA_MAXP; p++) mx = fmaxf(mx, fabsf(cvn[p]));
+            int ex = 0;
+            if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
+            const float sc = ldexpf(1.f, -ex);
+            colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+#pragma unroll
+            for (int p = 0; p < SLX_MFMA_MAXP; p++)
+                if (p < np) dst[p] = pack_split(cvn[p] * sc);
+            dst[np] = 0u;
+        };
+        const bool pipelined = COMPACT && NE * ns <= np_threads;
+        auto tile_of = [&](int st) { return (int)blockIdx.x + st * (int)gridDim.x; };
+        if (pipelined) {
+            if (nsteps > 0) {
+                load_cols(tile_of(0));
+                finish_cols(0);
+            }
+            if (nsteps > 1) load_cols(tile_of(1));
+        } else if (nsteps > 0) {
+            prepare(tile_of(0), 0);
+        }
         __syncthreads();  // tile 0 prepared
         for (int st = 0; st < nsteps; st++) {
-            if (st + 1 < nsteps) prepare(blockIdx.x + (st + 1) * gridDim.x, (st + 1) & 1);
-            if (st > 0) write_out(blockIdx.x + (st - 1) * gridDim.x, (st - 1) & 1);
+            if (st + 1 < nsteps) {
+                if (pipelined) {
+                    finish_cols((st + 1) & 1);
+                    if (st + 2 < nsteps) load_cols(tile_of(st + 2));
+                } else {
+                    prepare(tile_of(st + 1), (st + 1) & 1);
+                }
+            }
+            if (st > 0) write_out(tile_of(st - 1), (st - 1) & 1);
             __syncthreads();  // tile st staged; tile st + 1 prepared
         }
         if (nsteps > 0) write_out(blockIdx.x + (nsteps - 1) * gridDim.x, (nsteps - 1) & 1);
