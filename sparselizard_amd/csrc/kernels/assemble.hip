@@ -24,6 +24,7 @@ namespace slx {
 
 typedef float float4v __attribute__((ext_vector_type(4)));
 typedef double double2v __attribute__((ext_vector_type(2)));
+typedef float float2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
@@ -355,46 +356,60 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
 // fixed order, eight loaded ahead, so every value is summed in the same order
 // on every run.  Rows are taken in the plan's length order (rows of one
 // launch class have similar lengths; the LDS accumulator is sized per class).
-template <typename T>
-struct Vec4 {
-    T x[4];
+template <typename T, int N>
+struct VecN {
+    T x[N];
 };
 
-template <typename KT>
-__device__ inline Vec4<KT> load_vals(const KT* p) {
-    Vec4<KT> r;
-    if (sizeof(KT) == 4) {
+// N consecutive entries (N = 4 or 2) with one aligned vector load
+template <typename KT, int N>
+__device__ inline VecN<KT, N> load_vals(const KT* p) {
+    VecN<KT, N> r;
+    if (sizeof(KT) == 4 && N == 4) {
         const float4v v = *reinterpret_cast<const float4v*>(p);
 #pragma unroll
-        for (int c = 0; c < 4; c++) r.x[c] = (KT)v[c];
+        for (int c = 0; c < N; c++) r.x[c] = (KT)v[c];
+    } else if (sizeof(KT) == 4) {
+        const float2v v = *reinterpret_cast<const float2v*>(p);
+#pragma unroll
+        for (int c = 0; c < N; c++) r.x[c] = (KT)v[c];
     } else {
-        const double2v a = *reinterpret_cast<const double2v*>(p);
-        const double2v b = *reinterpret_cast<const double2v*>(p + 2);
-        r.x[0] = (KT)a[0];
-        r.x[1] = (KT)a[1];
-        r.x[2] = (KT)b[0];
-        r.x[3] = (KT)b[1];
+#pragma unroll
+        for (int c = 0; c < N; c += 2) {
+            const double2v a = *reinterpret_cast<const double2v*>(p + c);
+            r.x[c] = (KT)a[0];
+            r.x[c + 1] = (KT)a[1];
+        }
     }
     return r;
 }
 
-template <typename ST>
-__device__ inline Vec4<ST> load_slots(const ST* p) {
-    Vec4<ST> r;
-    if (sizeof(ST) == 1) {
+template <typename ST, int N>
+__device__ inline VecN<ST, N> load_slots(const ST* p) {
+    VecN<ST, N> r;
+    if (sizeof(ST) * N == 4) {
         const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
 #pragma unroll
-        for (int c = 0; c < 4; c++) r.x[c] = (ST)((w >> (8 * c)) & 0xFFu);
-    } else if (sizeof(ST) == 2) {
+        for (int c = 0; c < N; c++) r.x[c] = (ST)((w >> (8 * sizeof(ST) * c)) & (sizeof(ST) == 1 ? 0xFFu : 0xFFFFu));
+    } else if (sizeof(ST) * N == 2) {
+        const uint16_t w = *reinterpret_cast<const uint16_t*>(p);
+        r.x[0] = (ST)(w & 0xFFu);
+        r.x[1] = (ST)(w >> 8);
+    } else if (sizeof(ST) * N == 8) {
         const uint2v w = *reinterpret_cast<const uint2v*>(p);
-        r.x[0] = (ST)(w[0] & 0xFFFFu);
-        r.x[1] = (ST)(w[0] >> 16);
-        r.x[2] = (ST)(w[1] & 0xFFFFu);
-        r.x[3] = (ST)(w[1] >> 16);
+        if (sizeof(ST) == 2) {
+            r.x[0] = (ST)(w[0] & 0xFFFFu);
+            r.x[1] = (ST)(w[0] >> 16);
+            r.x[2] = (ST)(w[1] & 0xFFFFu);
+            r.x[3] = (ST)(w[1] >> 16);
+        } else {
+            r.x[0] = (ST)w[0];
+            r.x[1] = (ST)w[1];
+        }
     } else {
         const uint4v w = *reinterpret_cast<const uint4v*>(p);
 #pragma unroll
-        for (int c = 0; c < 4; c++) r.x[c] = (ST)w[c];
+        for (int c = 0; c < N; c++) r.x[c] = (ST)w[c];
     }
     return r;
 }
@@ -434,7 +449,7 @@ __device__ inline void slot_add(double* acc, ST sl, double v) {
     (void)atomicAdd(&acc[sl], v);
 }
 
-template <typename KT, typename ST, int LPR, bool SG>
+template <typename KT, typename ST, int LPR, bool SG, int EPL>
 __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __restrict__ ke, int64_t pos0, int64_t pos1, int acc_len,
                                                     double* __restrict__ Aval, double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
@@ -463,18 +478,18 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
             d[t] = (l * DPL + t < SLX_RED_BATCH && i < nk) ? rd.erows[k0 + i] : 0ull;
         }
     };
-    auto load_batch = [&](const uint64_t (&d)[DPL], Vec4<KT> (&v)[SLX_RED_BATCH], Vec4<ST> (&sl)[SLX_RED_BATCH]) {
+    auto load_batch = [&](const uint64_t (&d)[DPL], VecN<KT, EPL> (&v)[SLX_RED_BATCH], VecN<ST, EPL> (&sl)[SLX_RED_BATCH]) {
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++) {
             const uint64_t dsc = __shfl(d[u % DPL], u / DPL, LPR);
             const int n = (int)(dsc >> 48);
             const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-            if (4 * l < n) {
-                v[u] = load_vals<KT>(ke + off + 4 * l);
-                sl[u] = load_slots<ST>(slots + sp + 4 * l);
+            if (EPL * l < n) {
+                v[u] = load_vals<KT, EPL>(ke + off + EPL * l);
+                sl[u] = load_slots<ST, EPL>(slots + sp + EPL * l);
             } else {
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int c = 0; c < EPL; c++) {
                     v[u].x[c] = (KT)0;
                     sl[u].x[c] = SKIP;
                 }
@@ -483,8 +498,8 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
         }
     };
     uint64_t dsc_next[DPL];
-    Vec4<KT> v[SLX_RED_BATCH];
-    Vec4<ST> sl[SLX_RED_BATCH];
+    VecN<KT, EPL> v[SLX_RED_BATCH];
+    VecN<ST, EPL> sl[SLX_RED_BATCH];
     {
         uint64_t d0_[DPL];
         load_desc(0, d0_);
@@ -492,8 +507,8 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
         load_batch(d0_, v, sl);
     }
     for (int kb = 0; kb < nk; kb += SLX_RED_BATCH) {
-        Vec4<KT> vn[SLX_RED_BATCH];
-        Vec4<ST> sn[SLX_RED_BATCH];
+        VecN<KT, EPL> vn[SLX_RED_BATCH];
+        VecN<ST, EPL> sn[SLX_RED_BATCH];
         const bool more = kb + SLX_RED_BATCH < nk;
         if (more) {
             load_batch(dsc_next, vn, sn);
@@ -506,7 +521,7 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
 #pragma unroll
         for (int u = 0; u < SLX_RED_BATCH; u++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) slot_add<ST, SG>(acc, sl[u].x[c], (double)v[u].x[c]);
+            for (int c = 0; c < EPL; c++) slot_add<ST, SG>(acc, sl[u].x[c], (double)v[u].x[c]);
         if (more) {
 #pragma unroll
             for (int u = 0; u < SLX_RED_BATCH; u++) {
@@ -609,7 +624,7 @@ hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void
     return hipGetLastError();
 }
 
-template <typename KT, typename ST, int LPR, bool SG>
+template <typename KT, typename ST, int LPR, bool SG, int EPL>
 static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval, double* Dval, hipStream_t st, const int64_t* cb,
                               const int64_t* ce, const int* cl, int nc) {
     for (int c = 0; c < nc; c++) {
@@ -624,7 +639,7 @@ static hipError_t launch_rows(const DevReduce& rd, const void* ke, double* Aval,
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         const int rpb = threads / LPR;
         const int64_t grid = (nrows + rpb - 1) / rpb;
-        hipLaunchKernelGGL((reduce_rows<KT, ST, LPR, SG>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, cb[c], ce[c], cl[c],
+        hipLaunchKernelGGL((reduce_rows<KT, ST, LPR, SG, EPL>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, cb[c], ce[c], cl[c],
                            Aval, Dval);
     }
     return hipGetLastError();
@@ -646,12 +661,13 @@ static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const v
         hipLaunchKernelGGL((reduce_matrix_long<KT, ST, SG>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
         return hipGetLastError();
     }
-    // lanes per row: 4 entries per lane
-    if (max_erow_len <= 4) return launch_rows<KT, ST, 1, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 8) return launch_rows<KT, ST, 2, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 16) return launch_rows<KT, ST, 4, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    if (max_erow_len <= 32) return launch_rows<KT, ST, 8, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
-    return launch_rows<KT, ST, 16, SG>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    // lanes per row: 4 entries per lane (2 entries per lane with twice the
+    // lanes measured slower on the elasticity rows)
+    if (max_erow_len <= 4) return launch_rows<KT, ST, 1, SG, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 8) return launch_rows<KT, ST, 2, SG, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 16) return launch_rows<KT, ST, 4, SG, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    if (max_erow_len <= 32) return launch_rows<KT, ST, 8, SG, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
+    return launch_rows<KT, ST, 16, SG, 4>(rd, ke, Aval, Dval, st, cb, ce, cl, nc);
 }
 
 template <typename KT>
