@@ -4,7 +4,7 @@
 #include "device.h"
 
 #define SLX_MFMA_MAXP 16   // derivative pairs of a batched-MFMA bucket
-#define SLX_MFMA_PWAVES 4  // producer waves of the batched-MFMA workgroup
+#define SLX_MFMA_PWAVES 8  // producer waves of the batched-MFMA workgroup (half prepare, half write)
 
 namespace slx {
 

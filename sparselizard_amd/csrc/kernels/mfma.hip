@@ -54,9 +54,9 @@ __device__ inline uint32_t pack_split(float x) {
 // Work split (one workgroup per CU, persistent over 16-element tiles):
 //   consumer waves (one or two M tiles each) run the contraction of tile s
 //     from the column buffer Cs[s & 1] into the staging buffer stg[s & 1];
-//   producer waves (SLX_MFMA_PWAVES) meanwhile prepare the columns of tile
-//     s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from stg[(s - 1) & 1] to
-//     the element matrices with coalesced 16-byte stores.
+//   producer waves (SLX_MFMA_PWAVES, in two groups) meanwhile prepare the
+//     columns of tile s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from
+//     stg[(s - 1) & 1] to the element matrices with coalesced 16-byte stores.
 // One barrier per tile; every buffer is written one tile after its last read.
 template <int MTW, int MAXK, bool COMPACT, int G>
 __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
@@ -208,7 +208,11 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
         }
     } else {
         // ------------------------------------------------------- producers --
-        const int pt = threadIdx.x - 64 * nw, np_threads = blockDim.x - 64 * nw;
+        // two producer groups: the first prepares columns, the second writes out
+        const int np_threads = 32 * SLX_MFMA_PWAVES;
+        const int pt0 = threadIdx.x - 64 * nw;
+        const bool is_prep = pt0 < np_threads;  // wave-uniform
+        const int pt = is_prep ? pt0 : pt0 - np_threads;
         // columns of a tile: one thread per (element, slice), element fastest:
         // power-of-two scale and packed hi/lo' words
         auto prepare = [&](int tile, int buf) {
@@ -335,29 +339,34 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
         };
         const bool pipelined = COMPACT && NE * ns <= np_threads;
         auto tile_of = [&](int st) { return (int)blockIdx.x + st * (int)gridDim.x; };
-        if (pipelined) {
-            if (nsteps > 0) {
-                load_cols(tile_of(0));
-                finish_cols(0);
+        if (is_prep) {
+            if (pipelined) {
+                if (nsteps > 0) {
+                    load_cols(tile_of(0));
+                    finish_cols(0);
+                }
+                if (nsteps > 1) load_cols(tile_of(1));
+            } else if (nsteps > 0) {
+                prepare(tile_of(0), 0);
             }
-            if (nsteps > 1) load_cols(tile_of(1));
-        } else if (nsteps > 0) {
-            prepare(tile_of(0), 0);
         }
         __syncthreads();  // tile 0 prepared
         for (int st = 0; st < nsteps; st++) {
-            if (st + 1 < nsteps) {
-                if (pipelined) {
-                    finish_cols((st + 1) & 1);
-                    if (st + 2 < nsteps) load_cols(tile_of(st + 2));
-                } else {
-                    prepare(tile_of(st + 1), (st + 1) & 1);
+            if (is_prep) {
+                if (st + 1 < nsteps) {
+                    if (pipelined) {
+                        finish_cols((st + 1) & 1);
+                        if (st + 2 < nsteps) load_cols(tile_of(st + 2));
+                    } else {
+                        prepare(tile_of(st + 1), (st + 1) & 1);
+                    }
                 }
+            } else if (st > 0) {
+                write_out(tile_of(st - 1), (st - 1) & 1);
             }
-            if (st > 0) write_out(tile_of(st - 1), (st - 1) & 1);
             __syncthreads();  // tile st staged; tile st + 1 prepared
         }
-        if (nsteps > 0) write_out(blockIdx.x + (nsteps - 1) * gridDim.x, (nsteps - 1) & 1);
+        if (!is_prep && nsteps > 0) write_out(tile_of(nsteps - 1), (nsteps - 1) & 1);
     }
 }
 
