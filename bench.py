@@ -150,8 +150,11 @@ def main():
         ngp = {1: 11, 2: 24, 3: 43}.get(args.order, 24)
         flops = element_flops(3, n, 3, ngp) * n_elem
     else:
-        n_elem = info["nelem"]
-        flops = slice_flops(prob.desc())
+        # elements of the matrix buckets (every element once; rhs buckets repeat them)
+        dd = prob.desc()
+        n_elem = sum(dd.buckets[b].n_elem for b in range(dd.n_buckets)
+                     if any(not dd.buckets[b].slices[s].is_rhs for s in range(dd.buckets[b].n_slices)))
+        flops = slice_flops(dd)
     ke_bytes = 4 if args.mixed in ("f16split", "bf16split", "fp32") else 8
     # reduction: every element-matrix entry read once, every CSR value and rhs entry written once
     red_bytes = stats["n_coo"] * ke_bytes + nnz * 8 + nn * 8
