@@ -615,12 +615,58 @@ static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask,
     else launch_cols<CT, 16>(b, ctab, mask, jc, qfast, ke, st);
 }
 
+// Right-hand-side slices: one lane per (element, tf function),
+//   b_e[i] = -sum_q w_q sum_t C_t(e, q) tf_{r_t}[i][q]
+// (contribution.cpp:293-297; the element's lanes write one contiguous run).
+template <typename CT>
+__global__ void __launch_bounds__(256) rhs_rows(DevBucket b, const CT* __restrict__ ctab, int s, int qfast, CT* __restrict__ ke) {
+    const SLX_CONST DevSlice* dsl = (const SLX_CONST DevSlice*)b.dslices;
+    const int n_tf = dsl[s].n_tf, n_terms = dsl[s].n_terms, tf_pad = dsl[s].tf_pad;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)b.n_elem * n_tf) return;
+    const int e = (int)(idx / n_tf), i = (int)(idx - (int64_t)e * n_tf);
+    const int nq = b.q_inv ? 1 : b.n_gp, ngp = b.n_gp;
+    const size_t ne = (size_t)b.n_elem;
+    const SLX_CONST slx_term* terms = (const SLX_CONST slx_term*)dsl[s].terms;
+    const SLX_CONST double* gw = (const SLX_CONST double*)b.gp_w;
+    const double* tfq = dsl[s].tf_tabq;
+    CT acc = 0;
+    for (int q = 0; q < ngp; q++) {
+        const int qq = nq == 1 ? 0 : q;
+        CT h = 0;
+        for (int t = 0; t < n_terms; t++) {
+            const int c = terms[t].coef, r = terms[t].tf_deriv;
+            const CT cv = qfast ? ctab[((size_t)c * ne + e) * nq + qq] : ctab[((size_t)c * nq + qq) * ne + e];
+            h += cv * (CT)tfq[((size_t)q * 4 + r) * tf_pad + i];
+        }
+        acc += h * (CT)gw[q];
+    }
+    ke[dsl[s].ke_base + ((int64_t)e * n_tf + i) * dsl[s].ke_ld] = -acc;
+}
+
 hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask, int qfast) {
     if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
-    if (precision == SLX_PREC_FP64)
-        launch_cols_any<double>(b, ctab, slice_mask, qfast, ke, st);
-    else
-        launch_cols_any<float>(b, ctab, slice_mask, qfast, ke, st);
+    // right-hand sides with more tf functions than one column pass of elem_cols
+    // covers go to the row-per-lane kernel
+    uint32_t rhs_mask = 0;
+    for (int s = 0; s < b.n_slices; s++)
+        if (((slice_mask >> s) & 1u) && b.slices[s].is_rhs && b.slices[s].n_tf > 16) rhs_mask |= 1u << s;
+    const uint32_t mat_mask = slice_mask & ~rhs_mask;
+    if (mat_mask) {
+        if (precision == SLX_PREC_FP64)
+            launch_cols_any<double>(b, ctab, mat_mask, qfast, ke, st);
+        else
+            launch_cols_any<float>(b, ctab, mat_mask, qfast, ke, st);
+    }
+    for (int s = 0; s < b.n_slices; s++) {
+        if (!((rhs_mask >> s) & 1u)) continue;
+        const int64_t n = (int64_t)b.n_elem * b.slices[s].n_tf;
+        const unsigned grid = (unsigned)((n + 255) / 256);
+        if (precision == SLX_PREC_FP64)
+            hipLaunchKernelGGL((rhs_rows<double>), dim3(grid), dim3(256), 0, st, b, (const double*)ctab, s, qfast, (double*)ke);
+        else
+            hipLaunchKernelGGL((rhs_rows<float>), dim3(grid), dim3(256), 0, st, b, (const float*)ctab, s, qfast, (float*)ke);
+    }
     return hipGetLastError();
 }
 
