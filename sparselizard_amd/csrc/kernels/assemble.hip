@@ -535,7 +535,10 @@ __global__ void __launch_bounds__(256) reduce_rows(DevReduce rd, const KT* __res
 }
 
 // Segments longer than 64 entries: a wave per row, lanes stride over the segment.
-template <typename KT, typename ST, bool SG>
+// U > 0 (segments of at most 64 U entries): lane l takes entries l + 64 u, the
+// next segment's values and slots are in flight while the current one is
+// summed, and the descriptors come 64 at a time (one per lane, shuffled).
+template <typename KT, typename ST, bool SG, int U>
 __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT* __restrict__ ke, double* __restrict__ Aval,
                                                            double* __restrict__ Dval) {
     extern __shared__ unsigned char smem_raw[];
@@ -551,12 +554,48 @@ __global__ void __launch_bounds__(256) reduce_matrix_long(DevReduce rd, const KT
     const int lenA = rw.lenA, lenD = rw.lenD;
     for (int x = lane; x < lenA + lenD; x += group) acc[x] = 0.0;
     int64_t sp = rw.sp;
-    for (int64_t k = rw.k0; k < rw.k0 + rw.nk; k++) {
-        const uint64_t dsc = rd.erows[k];
-        const int n = (int)(dsc >> 48);
-        const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
-        for (int j = lane; j < n; j += group) slot_add<ST, SG>(acc, slots[sp + j], (double)ke[off + j]);
-        sp += (n + 3) & ~3;
+    if (U == 0) {
+        for (int64_t k = rw.k0; k < rw.k0 + rw.nk; k++) {
+            const uint64_t dsc = rd.erows[k];
+            const int n = (int)(dsc >> 48);
+            const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+            for (int j = lane; j < n; j += group) slot_add<ST, SG>(acc, slots[sp + j], (double)ke[off + j]);
+            sp += (n + 3) & ~3;
+        }
+    } else {
+        constexpr int UU = U > 0 ? U : 1;
+        const ST SKIP = (ST)~(ST)0;
+        const int nk = rw.nk;
+        uint64_t dl = lane < nk ? rd.erows[rw.k0 + lane] : 0ull;
+        auto load_seg = [&](int k, KT (&v)[UU], ST (&sl)[UU]) {
+            if ((k & 63) == 0 && k > 0) dl = (k + lane < nk) ? rd.erows[rw.k0 + k + lane] : 0ull;
+            const uint64_t dsc = __shfl(dl, k & 63, 64);
+            const int n = (int)(dsc >> 48);
+            const int64_t off = (int64_t)(dsc & 0xFFFFFFFFFFFFull);
+#pragma unroll
+            for (int u = 0; u < UU; u++) {
+                const int j = lane + 64 * u;
+                v[u] = j < n ? ke[off + j] : (KT)0;
+                sl[u] = j < n ? slots[sp + j] : SKIP;
+            }
+            sp += (n + 3) & ~3;
+        };
+        KT v[UU], vn[UU];
+        ST sl[UU], sn[UU];
+        if (nk > 0) load_seg(0, v, sl);
+        for (int k = 0; k < nk; k++) {
+            const bool more = k + 1 < nk;
+            if (more) load_seg(k + 1, vn, sn);
+#pragma unroll
+            for (int u = 0; u < UU; u++) slot_add<ST, SG>(acc, sl[u], (double)v[u]);
+            if (more) {
+#pragma unroll
+                for (int u = 0; u < UU; u++) {
+                    v[u] = vn[u];
+                    sl[u] = sn[u];
+                }
+            }
+        }
     }
     for (int x = lane; x < lenA; x += group) Aval[a0 + x] = acc[x];
     for (int x = lane; x < lenD; x += group) Dval[d0 + x] = acc[lenA + x];
@@ -704,7 +743,10 @@ static hipError_t launch_reduce_t(const DevReduce& rd, int max_erow_len, const v
         }
         if (lds > 64 * 1024) return hipErrorInvalidValue;
         const int64_t grid = (rd.nrows + gpb - 1) / gpb;
-        hipLaunchKernelGGL((reduce_matrix_long<KT, ST, SG>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
+        if (max_erow_len <= 256)
+            hipLaunchKernelGGL((reduce_matrix_long<KT, ST, SG, 4>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
+        else
+            hipLaunchKernelGGL((reduce_matrix_long<KT, ST, SG, 0>), dim3((unsigned)grid), dim3(threads), lds, st, rd, (const KT*)ke, Aval, Dval);
         return hipGetLastError();
     }
     // lanes per row: 4 entries per lane (2 entries per lane with twice the
