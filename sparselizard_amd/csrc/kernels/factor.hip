@@ -181,33 +181,58 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
             // registers); the column is the lane's
             const int bc = bt % CW, kg0 = __builtin_amdgcn_readfirstlane(bt / CW);
             const unsigned nmagic = (65536u + na - 1) / na;  // k / na = (k * nmagic) >> 16 for k < 32, na <= 4
-            auto produce = [&](int ks, int buf, int cb) {
-                // the shared P chunk: every load issued before the first store
-                uint4v areg[SLX_FK_AREG];
-                const uint4v* asrc = afrag + (size_t)ks * achunk;
-#pragma unroll
-                for (int u = 0; u < SLX_FK_AREG; u++) {
-                    const int x = bt + u * 64 * SLX_FK_BWAVES;
-                    if (x < achunk) areg[u] = asrc[x];
-                }
+            // the dof-table values of a k-step (one float4 of the four derivatives
+            // per entry) are loaded one k-step ahead into hq, so their latency
+            // overlaps the barrier and the consumers' MFMAs
+            constexpr int GP = 4 / NG;  // k groups per thread
+            float4v hq[GP][8];
+            auto load_h = [&](int ks, int cb) {
                 const int j = cb * 16 + bc;
                 const bool jon = j < ntp;
-                uint4v* colw = Bs + (size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15);
-                for (int kg = kg0; kg < 4; kg += NG) {
-                    float4v h[8];
+#pragma unroll
+                for (int g = 0; g < GP; g++) {
+                    const int kg = kg0 + g * NG;
 #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), q = ks * qb + qi;
                         const float4v* row = tq4 + (size_t)min(q, ngp - 1) * ntp;  // uniform row of the table
-                        h[v] = (jon && qi < qb && q < ngp) ? row[j] : float4v{0.f, 0.f, 0.f, 0.f};
+                        hq[g][v] = (jon && qi < qb && q < ngp && ks < f.nks) ? row[j] : float4v{0.f, 0.f, 0.f, 0.f};
                     }
-                    half8 hv, lv;
+                }
+            };
+            auto produce = [&](int ks, int buf, int cb) {
+                // the shared P chunk goes straight from global memory to LDS
+                // (global_load_lds: lane i of a wave lands at the wave's base + 16 i),
+                // in flight while Q_e is built
+                const int wp = __builtin_amdgcn_readfirstlane(bt >> 6);
+                const int lane = bt & 63;
+#pragma unroll
+                for (int u = 0; u < SLX_FK_AREG; u++) {
+                    const int x0 = 64 * wp + u * 64 * SLX_FK_BWAVES;  // wave-uniform first entry
+                    if (x0 + lane < achunk) {
+                        const uint4v* src = afrag + ((size_t)ks * achunk + x0);  // uniform base, lane offset below
+                        __builtin_amdgcn_global_load_lds((const void*)(src + lane),
+                                                         (__attribute__((address_space(3))) void*)(As + buf * achunk + x0), 16, 0, 0);
+                    }
+                }
+                uint4v* colw = Bs + (size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15);
+#pragma unroll
+                for (int g = 0; g < GP; g++) {
+                    const int kg = kg0 + g * NG;
+                    // the group's coefficient rows first (one LDS wait), then the
+                    // products (unused derivatives have zero coefficients)
+                    float4v mm[8];
 #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), ai = k - qi * na;
                         const int q = min(ks * qb + qi, ngp - 1);
-                        const float4v mm = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
-                        const float val = (mm[0] * h[v][0] + mm[1] * h[v][1] + mm[2] * h[v][2] + mm[3] * h[v][3]) * sc;
+                        mm[v] = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
+                    }
+                    half8 hv, lv;
+#pragma unroll
+                    for (int v = 0; v < 8; v++) {
+                        const float4v hh = hq[g][v];
+                        const float val = (mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3]) * sc;
                         const _Float16 hi = (_Float16)val;
                         hv[v] = hi;
                         lv[v] = (_Float16)(val - (float)hi);
@@ -215,13 +240,13 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
                     colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
                     colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
                 }
-#pragma unroll
-                for (int u = 0; u < SLX_FK_AREG; u++) {
-                    const int x = bt + u * 64 * SLX_FK_BWAVES;
-                    if (x < achunk) As[buf * achunk + x] = areg[u];
-                }
+                // the P chunk is in LDS before this wave reaches the barrier; the
+                // next k-step's table values are requested after that wait
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                load_h(ks + 1, cb);  // zeros past the last k-step
             };
             for (int pass = 0; pass < npass; pass++) {
+                load_h(0, pass * NTB);
                 produce(0, 0, pass * NTB);
                 __syncthreads();  // k-step 0 staged
                 for (int ks = 0; ks < f.nks; ks++) {
