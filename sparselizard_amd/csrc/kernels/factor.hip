@@ -95,6 +95,13 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
             sc = ldexpf(1.f, 8 - ex);
             unsc = ldexpf(f.a_unscale, ex - 8);
         }
+        // the element scale folded into the coefficient matrices (exact: a power of two)
+        for (int q = tid; q < ngp; q += blockDim.x) {
+            float* mq = Ms + q * 16;
+#pragma unroll
+            for (int x = 0; x < 16; x++) mq[x] *= sc;
+        }
+        __syncthreads();
         if (mfma_wave) {
             // consumer waves: a 2 x 4 grid over the 16x16 output tiles of a pass
             const int wm = w >> 2, wn = w & 3;
@@ -232,7 +239,7 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
 #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const float4v hh = hq[g][v];
-                        const float val = (mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3]) * sc;
+                        const float val = mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3];
                         const _Float16 hi = (_Float16)val;
                         hv[v] = hi;
                         lv[v] = (_Float16)(val - (float)hi);
