@@ -209,6 +209,22 @@ slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, in
 /* Buckets whose matrix slices run on the MFMA split kernel (the others and
  * every right-hand-side slice use the VALU element kernel). */
 slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_buckets);
+/* Solve of the last assembled system on the device, replacing
+ * sl::solve(A, b) (src/expression/operation/sl.cpp:1565-1627): the
+ * Dirichlet elimination b_A - D x_D of mat::eliminate
+ * (src/formulation/mat.cpp:78-103), a solve with A, and the merge of the
+ * free and constrained values of mat::xbmerge (x_out: n_dofs entries, the
+ * constrained ones equal to the Dirichlet values).  The reference factorizes
+ * A (MUMPS LU / Cholesky); here A must be symmetric positive definite:
+ * FP64 iterative refinement (FP64 residuals r = b_A - D x_D - A x) around a
+ * Jacobi-preconditioned FP32 conjugate-gradient inner solve.  Stops when
+ * ||r|| <= rel_tol ||b_A - D x_D|| or after max_outer refinements of at most
+ * max_inner inner iterations each.  Returns the counts and the final relative
+ * FP64 residual; not reaching rel_tol is reported through rel_residual, not
+ * as an error.  Allocates its device workspace per call.  x_out: host
+ * (d_x_out NULL) or device (x_out NULL) memory. */
+slx_status slx_plan_solve(slx_plan plan, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
+                          double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual);
 slx_status slx_plan_destroy(slx_plan plan);
 /* Element matrices of the last assembly (n_ke entries, as FP64) in the
  * layout given by slx_plan_reduction_maps: for the element-matrix error of

@@ -102,6 +102,7 @@ def lib():
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
+        "slx_plan_solve": (i32, [vp, C.c_double, i32, i32, pd, vp, p32, p32, pd]),
         "slx_plan_element_matrices": (i32, [vp, pd, i64]),
         "slx_plan_coefficients": (i32, [vp, pd, i64, p64]),
         "slx_plan_reduction_maps": (i32, [vp, p64, p64, C.POINTER(C.c_uint64), p64, p32, p64, C.POINTER(C.c_uint64), p64, p64]),
@@ -132,7 +133,7 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout",
@@ -385,6 +386,18 @@ class Plan:
                                     _p(Drow, C.c_int64), _p(Dcol, C.c_int32), _p(Dval, C.c_double),
                                     _p(rhs, C.c_double)))
         return CSR(Arow, Acol[:s["nnzA"]], Aval[:s["nnzA"]], Drow, Dcol[:s["nnzD"]], Dval[:s["nnzD"]], rhs[:ndofs])
+
+    def solve(self, ndofs, rel_tol=1e-10, max_outer=20, max_inner=2000):
+        """sl::solve(A, b) on the last assembled system (sl.cpp:1565-1627):
+        solution in DOF numbering, constrained entries = Dirichlet values.
+        Returns (x, info) with the refinement count, the inner iteration count
+        and the final relative FP64 residual of b_A - D x_D - A x."""
+        x = np.zeros(max(ndofs, 1), np.float64)
+        o, i = C.c_int32(), C.c_int32()
+        r = C.c_double()
+        check(lib().slx_plan_solve(self.h, rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
+                                   C.byref(o), C.byref(i), C.byref(r)))
+        return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value)
 
     def device_ptrs(self):
         ptrs = [C.c_void_p() for _ in range(7)]

@@ -17,6 +17,7 @@
 #include "kernels/device.h"
 #include "kernels/mfma.h"
 #include "kernels/factor.h"
+#include "kernels/solve.h"
 #include <cmath>
 
 namespace slx {
@@ -136,7 +137,7 @@ struct slx_plan_s {
     std::vector<std::unique_ptr<DBuf>> bufs;
     std::vector<DevBucket> dbuckets;
     DBuf consts;
-    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_rdesc, d_erows, d_slots, d_rhs_ptr, d_rhs_erows, d_free;
+    DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_rdesc, d_erows, d_slots, d_rhs_ptr, d_rhs_erows, d_free, d_cons;
     std::vector<RedClass> red_classes;  // reduction launch classes (rows of similar length)
     DBuf d_Aval, d_Dval, d_rhs, d_ke, d_ctab;
     size_t ke_capacity_bytes = 0;
@@ -548,6 +549,13 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
+    {
+        std::vector<int64_t> cons_dofs;
+        cons_dofs.reserve(nc);
+        for (int64_t i = 0; i < nd; i++)
+            if (d.constrained && d.constrained[i]) cons_dofs.push_back(i);
+        upload(P.d_cons, cons_dofs.data(), cons_dofs.size(), st);
+    }
     P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
     P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
     // rhs: constrained entries hold the Dirichlet values (formulation::b(),
@@ -1231,6 +1239,53 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
         if (Aval && p->nnzA) HIPCHECK(hipMemcpy(Aval, p->d_Aval.p, p->nnzA * sizeof(double), hipMemcpyDeviceToHost));
         if (Dval && p->nnzD) HIPCHECK(hipMemcpy(Dval, p->d_Dval.p, p->nnzD * sizeof(double), hipMemcpyDeviceToHost));
         if (rhs && p->n_dofs) HIPCHECK(hipMemcpy(rhs, p->d_rhs.p, p->n_dofs * sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out, double* d_x_out,
+                          int32_t* outer_its, int32_t* inner_its, double* rel_residual) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        SLX_CHECK(!p->host_only, "plan was created without a device context");
+        SLX_CHECK(p->has_matrix, "the formulation has no matrix terms");
+        SLX_CHECK(p->last_precision >= 0, "no assembly has run on this plan");
+        SLX_CHECK(rel_tol > 0 && max_outer > 0 && max_inner > 0, "solve: rel_tol, max_outer and max_inner must be positive");
+        SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
+        HIPCHECK(hipSetDevice(p->ctx->device));
+        const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
+        // one workspace block: doubles first, then floats (8-byte aligned)
+        const int64_t nD = 3 * nf + nc + nd + 512 + 8, nF = 6 * nf + p->nnzA;
+        DBuf ws;
+        ws.alloc((size_t)(nD * sizeof(double) + nF * sizeof(float) + 64));
+        SolveWork w;
+        double* dp = ws.as<double>();
+        w.f = dp; dp += nf;
+        w.r = dp; dp += nf;
+        w.x = dp; dp += nf;
+        w.xd = dp; dp += nc;
+        w.xfull = dp; dp += nd;
+        w.part = dp; dp += 512;
+        w.sc = dp; dp += 8;
+        float* fp = (float*)dp;
+        w.r32 = fp; fp += nf;
+        w.z = fp; fp += nf;
+        w.p = fp; fp += nf;
+        w.q = fp; fp += nf;
+        w.d = fp; fp += nf;
+        w.dinv = fp; fp += nf;
+        w.val32 = fp;
+        SolveSystem sys{nd, nf, nc, p->d_Arow.as<int64_t>(), p->d_Drow.as<int64_t>(), p->d_Acol.as<int32_t>(), p->d_Dcol.as<int32_t>(),
+                        p->d_Aval.as<double>(), p->d_Dval.as<double>(), p->d_rhs.as<double>(), p->d_free.as<int64_t>(),
+                        p->d_cons.as<int64_t>()};
+        SolveParams prm{rel_tol, 1e-3, max_outer, max_inner};
+        SolveStats stats;
+        HIPCHECK(solve_system(sys, prm, w, stats, p->stream));
+        if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+        if (d_x_out && nd) HIPCHECK(hipMemcpyAsync(d_x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (outer_its) *outer_its = stats.outer;
+        if (inner_its) *inner_its = stats.inner;
+        if (rel_residual) *rel_residual = stats.rel_residual;
     });
 }
 

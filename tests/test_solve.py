@@ -1,0 +1,95 @@
+"""Device solve of the assembled system (slx_plan_solve) against a direct
+solve of the same system.
+
+sl::solve(A, b) (src/expression/operation/sl.cpp:1565-1627) eliminates the
+Dirichlet DOFs (mat::eliminate, src/formulation/mat.cpp:78-103), factorizes A
+and merges the free and constrained values (mat::xbmerge).  The checker here
+is scipy's sparse LU on the CSR the device assembled, so the comparison
+isolates the solve; the reference examples' validation lines pin the whole
+assemble + solve chain.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import sparselizard_amd as sa
+from test_golden_examples import _FREE, LINE_MSH, QUAD_MSH, register_free, solve_full
+
+pytestmark = pytest.mark.gpu
+
+
+def relerr(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "f16split", "fp32"])
+def test_solve_electrostatics_1d_golden(ctx, precision):
+    # examples/electrostatics-1d/main.cpp:44: 21.5962 < ||x||_2 < 21.5964
+    print("case solve electrostatics1d", precision, flush=True)
+    p = sa.Problem(ctx, "electrostatics1d", "file=" + LINE_MSH)
+    info = register_free(p)
+    plan = p.generate(precision)
+    x, it = plan.solve(info["ndofs"], rel_tol=1e-12)
+    assert it["rel_residual"] <= 1e-12, it
+    nrm = float(np.linalg.norm(x))
+    assert 21.5962 < nrm < 21.5964, nrm
+
+
+@pytest.mark.parametrize("kind,spec,precision", [("poisson2d", "n=40", "fp64"), ("poisson2d", "n=12;order=3", "fp64"),
+                                                 ("elasticity3d", "n=4", "fp64"), ("elasticity3d", "n=4", "f16split"),
+                                                 ("hppoisson3d", "n=3", "fp64")])
+def test_solve_matches_direct(ctx, kind, spec, precision):
+    print("case solve", kind, spec, precision, flush=True)
+    p = sa.Problem(ctx, kind, spec)
+    info = register_free(p)
+    plan = p.generate(precision)
+    ref, A, bA = solve_full(plan.csr(info["ndofs"]), info["ndofs"], info["nfree"])
+    x, it = plan.solve(info["ndofs"], rel_tol=1e-12)
+    print("solve", it, flush=True)
+    assert it["rel_residual"] <= 1e-12, it
+    assert it["outer"] >= 1 and it["inner"] >= 1
+    # residual bound 1e-12 times the condition number of these small systems
+    assert relerr(x, ref) <= 1e-8, relerr(x, ref)
+    # constrained entries are the Dirichlet values, exactly
+    cons = np.ones(info["ndofs"], bool)
+    cons[_FREE[(info["ndofs"], info["nfree"])]] = False
+    rhs = plan.csr(info["ndofs"]).rhs
+    assert np.array_equal(x[cons], rhs[cons])
+
+
+def test_solve_projection_no_constraints(ctx):
+    # order-5 projection on quads (mass matrix, no Dirichlet DOFs):
+    # examples/high-order-interpolations-2d/main.cpp:68, relative L2 error < 1e-5
+    print("case solve hoprojection2d", flush=True)
+    p = sa.Problem(ctx, "hoprojection2d", "file=%s;order=5" % QUAD_MSH)
+    info = register_free(p)
+    plan = p.generate("fp64")
+    v, it = plan.solve(info["ndofs"], rel_tol=1e-13, max_outer=30)
+    print("solve", it, flush=True)
+    assert it["rel_residual"] <= 1e-13, it
+    b = plan.csr(info["ndofs"]).rhs
+    f2 = (0.5 - math.sin(20.0) / 40.0) * (0.5 - math.sin(26.0) / 52.0)
+    err = math.sqrt(max(f2 - float(b @ v), 0.0) / f2)
+    assert err < 1e-5, err
+
+
+def test_solve_deterministic(ctx):
+    print("case solve deterministic", flush=True)
+    p = sa.Problem(ctx, "elasticity3d", "n=4")
+    info = register_free(p)
+    plan = p.generate("fp64")
+    x1, i1 = plan.solve(info["ndofs"], rel_tol=1e-11)
+    x2, i2 = plan.solve(info["ndofs"], rel_tol=1e-11)
+    assert i1 == i2 and np.array_equal(x1, x2)
+
+
+def test_solve_argument_errors(ctx):
+    p = sa.Problem(ctx, "poisson2d", "n=4")
+    info = p.info()
+    plan = p.plan()
+    with pytest.raises(sa.SlxError, match="no assembly"):
+        plan.solve(info["ndofs"])
+    p.generate("fp64")
+    with pytest.raises(sa.SlxError, match="must be positive"):
+        plan.solve(info["ndofs"], rel_tol=0.0)
