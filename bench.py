@@ -50,6 +50,8 @@ def main():
     ap.add_argument("--mixed", default="f16split", help="mixed precision mode")
     ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-solve", action="store_true",
+                    help="skip the device solve of the assembled system (slx_plan_solve) reported under 'solve'")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -205,6 +207,21 @@ def main():
         except (OSError, ValueError):
             pass
 
+    solve = None
+    if not args.no_solve:
+        # sl::solve on the last assembled system: FP64 refinement around FP32 Jacobi-PCG
+        torch.cuda.synchronize()
+        t0 = time.time()
+        x, it = plan.solve(nn, rel_tol=1e-10, max_outer=20, max_inner=20000)
+        t_solve = time.time() - t0
+        # per inner iteration: FP32 values + column indices once, p gathered
+        # (at least once), q, d, r, z, p, diag streamed
+        it_bytes = shape["nnzA"] * 8 + shape["nfree"] * 4 * 11
+        solve = {"s": t_solve, "outer": it["outer"], "inner": it["inner"], "rel_residual": it["rel_residual"],
+                 "ms_per_inner": t_solve * 1e3 / max(it["inner"], 1), "bytes_per_inner": it_bytes,
+                 "GBps_per_inner": it_bytes / (t_solve / max(it["inner"], 1)) / 1e9, "x_norm": float(np.linalg.norm(x))}
+        log("solve: %s" % solve)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(sa, args)
@@ -238,6 +255,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if solve is not None:
+        out["solve"] = solve
     print(json.dumps(out))
 
 

@@ -225,6 +225,15 @@ slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_b
  * (d_x_out NULL) or device (x_out NULL) memory. */
 slx_status slx_plan_solve(slx_plan plan, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
                           double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual);
+/* mat::eliminate(b) (src/formulation/mat.cpp:78-103): b_A - D b_D of a
+ * vector in DOF numbering (n_dofs entries; b NULL: the rhs of the last
+ * assembly, whose constrained entries are the Dirichlet values), on the
+ * device.  out: nfree entries in the free-DOF order of A.  Host memory. */
+slx_status slx_plan_eliminate(slx_plan plan, const double* b, double* out);
+/* mat * vec (src/formulation/mat.cpp:182-205): y_A = A x_A + D x_D on the
+ * device, and the constrained entries of y are those of x (mat::xbmerge).
+ * x, y: n_dofs entries in DOF numbering, host memory. */
+slx_status slx_plan_matvec(slx_plan plan, const double* x, double* y);
 slx_status slx_plan_destroy(slx_plan plan);
 /* Element matrices of the last assembly (n_ke entries, as FP64) in the
  * layout given by slx_plan_reduction_maps: for the element-matrix error of
@@ -251,6 +260,11 @@ slx_status slx_plan_reduction_maps(slx_plan plan, int64_t* sizes, int64_t* erow_
  * the 16x16 result (row-major) so the operand lane layout the kernels assume
  * can be checked against a host product. */
 slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256);
+/* Test hook: the streaming FP32 CSR product of slx_plan_solve's inner
+ * iteration on a host CSR (square, nrows rows; values and x rounded to FP32):
+ * y = A x and xy = x . y (FP64 sum of the FP32 products). */
+slx_status slx_selftest_spmv_stream(slx_ctx ctx, int64_t nrows, const int64_t* rowp, const int32_t* col,
+                                    const double* val, const double* x, double* y, double* xy);
 
 /* ---- Reference tables ------------------------------------------------------
  * gausspoints(type, order) (src/gausspoint/gausspoints.cpp:4-54): writes up to

@@ -103,6 +103,9 @@ def lib():
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
         "slx_plan_solve": (i32, [vp, C.c_double, i32, i32, pd, vp, p32, p32, pd]),
+        "slx_plan_eliminate": (i32, [vp, pd, pd]),
+        "slx_plan_matvec": (i32, [vp, pd, pd]),
+        "slx_selftest_spmv_stream": (i32, [vp, i64, p64, p32, pd, pd, pd, pd]),
         "slx_plan_element_matrices": (i32, [vp, pd, i64]),
         "slx_plan_coefficients": (i32, [vp, pd, i64, p64]),
         "slx_plan_reduction_maps": (i32, [vp, p64, p64, C.POINTER(C.c_uint64), p64, p32, p64, C.POINTER(C.c_uint64), p64, p64]),
@@ -133,10 +136,10 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
-    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_spmv_stream",
 ]
 
 
@@ -145,6 +148,20 @@ def mfma_layout_probe(ctx):
     out = np.zeros(256, np.float32)
     check(lib().slx_selftest_mfma_layout(ctx.h, out.ctypes.data_as(C.POINTER(C.c_float))))
     return out.reshape(16, 16)
+
+
+def spmv_stream_probe(ctx, rowp, col, val, x):
+    """The solve's streaming FP32 CSR product on a host CSR: (y, x.y)."""
+    rowp = np.ascontiguousarray(rowp, np.int64)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    n = len(rowp) - 1
+    y = np.zeros(n, np.float64)
+    xy = C.c_double()
+    check(lib().slx_selftest_spmv_stream(ctx.h, n, _p(rowp, C.c_int64), _p(col, C.c_int32), _p(val, C.c_double),
+                                         _p(x, C.c_double), _p(y, C.c_double), C.byref(xy)))
+    return y, xy.value
 
 
 def gauss_rule(elem_type, order):
@@ -398,6 +415,20 @@ class Plan:
         check(lib().slx_plan_solve(self.h, rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
                                    C.byref(o), C.byref(i), C.byref(r)))
         return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value)
+
+    def eliminate(self, nfree, b=None):
+        """mat::eliminate(b) (mat.cpp:78-103): b_A - D b_D (b None: the assembled rhs)."""
+        out = np.zeros(max(nfree, 1), np.float64)
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        check(lib().slx_plan_eliminate(self.h, None if bb is None else _p(bb, C.c_double), _p(out, C.c_double)))
+        return out[:nfree]
+
+    def matvec(self, x):
+        """mat * vec (mat.cpp:182-205): y_A = A x_A + D x_D, constrained entries of y = those of x."""
+        xx = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(max(len(xx), 1), np.float64)
+        check(lib().slx_plan_matvec(self.h, _p(xx, C.c_double), _p(y, C.c_double)))
+        return y[:len(xx)]
 
     def device_ptrs(self):
         ptrs = [C.c_void_p() for _ in range(7)]

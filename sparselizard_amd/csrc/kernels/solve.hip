@@ -6,38 +6,163 @@
 // refinement (FP64 CSR residuals) around a Jacobi-preconditioned FP32
 // conjugate-gradient inner solve, for symmetric positive definite A.
 //
-// Every reduction is deterministic: fixed lanes per row for the products, a
-// fixed grid of partial sums and one ordered final sum for the dots.
+// One inner iteration is three streaming kernels and two one-block sums:
+//   K1  q = A p (FP32 values),             partials of p.q
+//   F   pq = ordered sum of the partials
+//   K2  d += a p, r -= a q, z = r / diag,  partials of r.z and r.r  (a = rz / pq)
+//   F   rz' and rr, beta = rz' / rz
+//   K3  p = z + beta p
+// Every reduction is deterministic: a fixed grid of partial sums, summed in a
+// fixed order by one block, and fixed lanes per row for the products.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "solve.h"
 
 namespace slx {
 
-#define SLX_SPMV_LPR 8
+enum { SC_RZ = 0, SC_PQ = 1, SC_RZN = 2, SC_RR = 3, SC_BETA = 4, SC_DOT = 5 };
 
-// y[r] = c0 * f[r] - c1 * (A x)[r] for the rows of one CSR matrix (FP64);
-// f may be null (then y = -c1 A x, with c1 = -1 giving y = A x)
-template <typename VT, typename XT, typename YT>
-__global__ void __launch_bounds__(256) spmv_kernel(int64_t nrows, const int64_t* __restrict__ rowp, const int32_t* __restrict__ col,
-                                                   const VT* __restrict__ val, const XT* __restrict__ x, const YT* f,
-                                                   YT c0, YT c1, YT* y) {
-    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / SLX_SPMV_LPR;
-    const int l = threadIdx.x % SLX_SPMV_LPR;
-    YT s = 0;
-    if (r < nrows)
-        for (int64_t k = rowp[r] + l; k < rowp[r + 1]; k += SLX_SPMV_LPR) s += (YT)val[k] * (YT)x[col[k]];
+// block sum of one double per thread (256 threads); result valid in thread 0
+__device__ inline double block_sum(double v, double* sh) {
 #pragma unroll
-    for (int o = SLX_SPMV_LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, SLX_SPMV_LPR);
-    if (r < nrows && l == 0) y[r] = (f ? c0 * f[r] : (YT)0) - c1 * s;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    __syncthreads();
+    return s;
+}
+
+// y[r] = c0 f[r] - c1 (A x)[r] over one CSR matrix, LPR lanes per row (the
+// lanes of a row group share r, so the width-LPR shuffles stay inside
+// active lanes); with part != null the block also writes its partial sum of
+// x[r] y[r] (square A)
+template <int LPR, typename VT, typename XT, typename YT>
+__global__ void __launch_bounds__(256) spmv_kernel(int64_t nrows, const int64_t* __restrict__ rowp, const int32_t* __restrict__ col,
+                                                   const VT* __restrict__ val, const XT* __restrict__ x, const YT* f, YT c0, YT c1,
+                                                   YT* y, double* __restrict__ part) {
+    __shared__ double sh[4];
+    const int l = threadIdx.x % LPR;
+    const int64_t rstride = (int64_t)gridDim.x * (256 / LPR);
+    double acc = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * (256 / LPR) + threadIdx.x / LPR; r < nrows; r += rstride) {
+        YT s = 0;
+        const int64_t e = rowp[r + 1];
+        for (int64_t k = rowp[r] + l; k < e; k += LPR) s += (YT)val[k] * (YT)x[col[k]];
+#pragma unroll
+        for (int o = LPR / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, LPR);
+        if (l == 0) {
+            const YT v = (f ? c0 * f[r] : (YT)0) - c1 * s;
+            y[r] = v;
+            if (part) acc += (double)x[r] * (double)v;
+        }
+    }
+    if (part) {
+        const double t = block_sum(acc, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
+// lanes per row from the mean row length (a few products per lane)
+static int spmv_lpr(int64_t nrows, int64_t nnz) {
+    const double avg = nrows ? (double)nnz / (double)nrows : 0.0;
+    if (avg >= 96) return 32;
+    if (avg >= 40) return 16;
+    if (avg >= 16) return 8;
+    if (avg >= 6) return 4;
+    return 2;
 }
 
 template <typename VT, typename XT, typename YT>
-static void spmv(int64_t nrows, const int64_t* rowp, const int32_t* col, const VT* val, const XT* x, const YT* f, YT c0, YT c1, YT* y,
-                 hipStream_t st) {
+static void spmv(int64_t nrows, int64_t nnz, const int64_t* rowp, const int32_t* col, const VT* val, const XT* x, const YT* f, YT c0,
+                 YT c1, YT* y, double* part, hipStream_t st) {
     if (nrows <= 0) return;
-    const int64_t threads = nrows * SLX_SPMV_LPR;
-    hipLaunchKernelGGL((spmv_kernel<VT, XT, YT>), dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, nrows, rowp, col, val, x, f,
-                       c0, c1, y);
+    const int lpr = spmv_lpr(nrows, nnz);
+    // the fixed grid when partial sums are wanted, else one row group per thread group
+    const int64_t want = (nrows * lpr + 255) / 256;
+    const unsigned grid = part ? SLX_SOLVE_BLOCKS : (unsigned)std::min<int64_t>(want, 1 << 20);
+#define SLX_SPMV_CASE(L)                                                                                                          \
+    case L:                                                                                                                       \
+        hipLaunchKernelGGL((spmv_kernel<L, VT, XT, YT>), dim3(grid), dim3(256), 0, st, nrows, rowp, col, val, x, f, c0, c1, y, part); \
+        break;
+    switch (lpr) {
+        SLX_SPMV_CASE(2)
+        SLX_SPMV_CASE(4)
+        SLX_SPMV_CASE(8)
+        SLX_SPMV_CASE(16)
+        SLX_SPMV_CASE(32)
+    }
+#undef SLX_SPMV_CASE
+}
+
+// q = A p over FP32 values, streamed by row blocks: a block loads the
+// nonzeros of its rows with coalesced loads, keeps the products in LDS
+// and reduces each row from there (lanes per row from the rows in the block);
+// a row longer than SLX_STREAM_NNZ is summed by the whole block.  The fixed
+// grid walks the row blocks; each block writes its partial sum of p.q.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w on XCD
+// w % 8): each XCD takes one contiguous eighth of the row blocks, so the
+// entries of x its rows gather stay within a range its own L2 can hold.
+__global__ void __launch_bounds__(256) spmv_stream_f32(int64_t nblk, const int64_t* __restrict__ rblk, const int64_t* __restrict__ rowp,
+                                                       const int32_t* __restrict__ col, const float* __restrict__ val, int64_t nnz,
+                                                       const float* __restrict__ x, float* __restrict__ y, double* __restrict__ part) {
+    __shared__ float prod[SLX_STREAM_NNZ];
+    __shared__ double sh[4];
+    double acc = 0.0;
+    const int xcd = blockIdx.x % SLX_XCDS;
+    const int64_t b_end = nblk * (xcd + 1) / SLX_XCDS;
+    for (int64_t b = nblk * xcd / SLX_XCDS + blockIdx.x / SLX_XCDS; b < b_end; b += gridDim.x / SLX_XCDS) {
+        const int64_t r0 = rblk[b], r1 = rblk[b + 1];
+        const int64_t k0 = rowp[r0], k1 = rowp[r1];
+        if (k1 - k0 > SLX_STREAM_NNZ) {
+            // one long row (block-uniform branch)
+            double s = 0.0;
+            for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) s += (double)(val[k] * x[col[k]]);
+            const double t = block_sum(s, sh);
+            if (threadIdx.x == 0) {
+                y[r0] = (float)t;
+                acc += (double)x[r0] * (double)(float)t;
+            }
+            continue;
+        }
+        // consecutive lanes take consecutive nonzeros: the x entries one
+        // wave gathers then share cache lines more often than with 16-byte
+        // loads per lane (measured: 1.54 vs 1.75 ms per product, 64^3 P2 elasticity)
+        const int64_t kb = k0;
+        for (int64_t k = k0 + threadIdx.x; k < k1; k += 256) prod[k - k0] = val[k] * x[col[k]];
+        __syncthreads();
+        const int rows = (int)(r1 - r0);
+        int lpr = 32;
+        while (lpr > 1 && rows * lpr > 256) lpr >>= 1;
+        const int g = threadIdx.x / lpr, l = threadIdx.x % lpr;
+        for (int rr = g; rr < rows; rr += 256 / lpr) {
+            const int a = (int)(rowp[r0 + rr] - kb), e = (int)(rowp[r0 + rr + 1] - kb);
+            float s = 0.f;
+            for (int k = a + l; k < e; k += lpr) s += prod[k];
+            for (int o = lpr / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, lpr);
+            if (l == 0) {
+                y[r0 + rr] = s;
+                acc += (double)x[r0 + rr] * (double)s;
+            }
+        }
+        __syncthreads();
+    }
+    const double t = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+void stream_row_blocks(const int64_t* rowp, int64_t nrows, std::vector<int64_t>& out) {
+    out.clear();
+    out.push_back(0);
+    int64_t r = 0;
+    while (r < nrows) {
+        int64_t e = r;
+        while (e < nrows && e - r < 256 && rowp[e + 1] - rowp[r] <= SLX_STREAM_NNZ) e++;
+        if (e == r) e = r + 1;
+        out.push_back(e);
+        r = e;
+    }
 }
 
 __global__ void gather_kernel(int64_t n, const int64_t* __restrict__ idx, const double* __restrict__ src, double* __restrict__ dst) {
@@ -63,69 +188,90 @@ __global__ void prep_kernel(int64_t nrows, const int64_t* __restrict__ rowp, con
     dinv[r] = d != 0.0 ? (float)(1.0 / d) : 1.0f;
 }
 
-// partial sums of a dot product over a fixed grid, then one ordered sum
-#define SLX_DOT_BLOCKS 512
+// partials of a.b (FP64 accumulation) over the fixed grid
 template <typename T>
 __global__ void __launch_bounds__(256) dot_partial(int64_t n, const T* __restrict__ a, const T* __restrict__ b, double* __restrict__ part) {
-    __shared__ double sh[256];
+    __shared__ double sh[4];
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        s += (double)a[i] * (double)b[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += (double)a[i] * (double)b[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// sc[out0] (and sc[out1]) = ordered sums of one (two) partial arrays; mode 1
+// also sets beta = sc[SC_RZN] / sc[SC_RZ] and then sc[SC_RZ] = sc[SC_RZN]
+__global__ void __launch_bounds__(256) sum_partials(const double* __restrict__ part, int nsum, int out0, int out1, int mode,
+                                                    double* __restrict__ sc) {
+    __shared__ double sh[4];
+    double t[2] = {0.0, 0.0};
+    for (int j = 0; j < nsum; j++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < SLX_SOLVE_BLOCKS; i += 256) s += part[j * SLX_SOLVE_BLOCKS + i];
+        t[j] = block_sum(s, sh);
     }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-__global__ void __launch_bounds__(256) dot_final(const double* __restrict__ part, double* __restrict__ out) {
-    __shared__ double sh[256];
-    double s = 0.0;
-    for (int i = threadIdx.x; i < SLX_DOT_BLOCKS; i += blockDim.x) s += part[i];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
+    if (threadIdx.x == 0) {
+        sc[out0] = t[0];
+        if (nsum > 1) sc[out1] = t[1];
+        if (mode == 1) {
+            sc[SC_BETA] = sc[SC_RZ] != 0.0 ? sc[SC_RZN] / sc[SC_RZ] : 0.0;
+            sc[SC_RZ] = sc[SC_RZN];
+        }
     }
-    if (threadIdx.x == 0) *out = sh[0];
 }
 
-template <typename T>
-static void dot(int64_t n, const T* a, const T* b, double* part, double* out, hipStream_t st) {
-    hipLaunchKernelGGL((dot_partial<T>), dim3(SLX_DOT_BLOCKS), dim3(256), 0, st, n, a, b, part);
-    hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, st, part, out);
+// inner start: r32 = r, z = r / diag, p = z, d = 0; partials of r.z and r.r
+__global__ void __launch_bounds__(256) pcg_start(int64_t n, const double* __restrict__ r, const float* __restrict__ dinv,
+                                                 float* __restrict__ r32, float* __restrict__ z, float* __restrict__ p,
+                                                 float* __restrict__ d, double* __restrict__ part) {
+    __shared__ double sh[4];
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        const float rv = (float)r[i];
+        const float zv = dinv[i] * rv;
+        r32[i] = rv;
+        z[i] = zv;
+        p[i] = zv;
+        d[i] = 0.f;
+        rz += (double)rv * (double)zv;
+        rr += (double)rv * (double)rv;
+    }
+    const double a = block_sum(rz, sh);
+    const double b = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[SLX_SOLVE_BLOCKS + blockIdx.x] = b;
+    }
 }
 
-// z = dinv .* r ; with first: p = z
-__global__ void precond_kernel(int64_t n, const float* __restrict__ dinv, const float* __restrict__ r, float* __restrict__ z) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) z[i] = dinv[i] * r[i];
-}
-
-// d += alpha p ; r -= alpha q   (alpha = rz / pq from device scalars)
-__global__ void cg_update(int64_t n, const double* __restrict__ sc, float* __restrict__ d, float* __restrict__ r, const float* __restrict__ p,
-                          const float* __restrict__ q) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const float alpha = sc[1] != 0.0 ? (float)(sc[0] / sc[1]) : 0.f;
-    if (i < n) {
+// d += a p ; r -= a q ; z = r / diag   (a = rz / pq); partials of r.z and r.r
+__global__ void __launch_bounds__(256) pcg_update(int64_t n, const double* __restrict__ sc, float* __restrict__ d, float* __restrict__ r,
+                                                  const float* __restrict__ p, const float* __restrict__ q, const float* __restrict__ dinv,
+                                                  float* __restrict__ z, double* __restrict__ part) {
+    __shared__ double sh[4];
+    const float alpha = sc[SC_PQ] != 0.0 ? (float)(sc[SC_RZ] / sc[SC_PQ]) : 0.f;
+    double rz = 0.0, rr = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
         d[i] += alpha * p[i];
-        r[i] -= alpha * q[i];
+        const float rv = r[i] - alpha * q[i];
+        const float zv = dinv[i] * rv;
+        r[i] = rv;
+        z[i] = zv;
+        rz += (double)rv * (double)zv;
+        rr += (double)rv * (double)rv;
+    }
+    const double a = block_sum(rz, sh);
+    const double b = block_sum(rr, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = a;
+        part[SLX_SOLVE_BLOCKS + blockIdx.x] = b;
     }
 }
 
-// p = z + beta p   (beta = rz_new / rz)
-__global__ void cg_direction(int64_t n, const double* __restrict__ sc, const float* __restrict__ z, float* __restrict__ p) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const float beta = sc[0] != 0.0 ? (float)(sc[2] / sc[0]) : 0.f;
-    if (i < n) p[i] = z[i] + beta * p[i];
-}
-
-__global__ void to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+// p = z + beta p
+__global__ void __launch_bounds__(256) pcg_direction(int64_t n, const double* __restrict__ sc, const float* __restrict__ z,
+                                                     float* __restrict__ p) {
+    const float beta = (float)sc[SC_BETA];
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = z[i] + beta * p[i];
 }
 
 __global__ void add_f32_to_f64(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
@@ -135,63 +281,92 @@ __global__ void add_f32_to_f64(int64_t n, const float* __restrict__ d, double* _
 
 static unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
 
+static hipError_t read_scalar(const double* dsc, int idx, double* h, hipStream_t st) {
+    HIPERR(hipMemcpyAsync(h, dsc + idx, sizeof(double), hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+hipError_t eliminate_system(const SolveSystem& s, const double* b, double* out, double* xd, hipStream_t st) {
+    const int64_t n = s.nfree;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks(n)), dim3(256), 0, st, n, s.free_dofs, b, out);
+    if (s.ncons > 0) {
+        hipLaunchKernelGGL(gather_kernel, dim3(blocks(s.ncons)), dim3(256), 0, st, s.ncons, s.cons_dofs, b, xd);
+        spmv<double, double, double>(n, s.nnzD, s.Drow, s.Dcol, s.Dval, xd, out, 1.0, 1.0, out, nullptr, st);
+    }
+    return hipGetLastError();
+}
+
+hipError_t matvec_system(const SolveSystem& s, const double* x, double* y, double* xa, double* xd, double* ya, hipStream_t st) {
+    const int64_t n = s.nfree;
+    HIPERR(hipMemcpyAsync(y, x, s.n_dofs * sizeof(double), hipMemcpyDeviceToDevice, st));
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_kernel, dim3(blocks(n)), dim3(256), 0, st, n, s.free_dofs, x, xa);
+    spmv<double, double, double>(n, s.nnzA, s.Arow, s.Acol, s.Aval, xa, (const double*)nullptr, 0.0, -1.0, ya, nullptr, st);
+    if (s.ncons > 0) {
+        hipLaunchKernelGGL(gather_kernel, dim3(blocks(s.ncons)), dim3(256), 0, st, s.ncons, s.cons_dofs, x, xd);
+        spmv<double, double, double>(n, s.nnzD, s.Drow, s.Dcol, s.Dval, xd, ya, 1.0, -1.0, ya, nullptr, st);
+    }
+    hipLaunchKernelGGL(scatter_kernel, dim3(blocks(n)), dim3(256), 0, st, n, s.free_dofs, ya, y);
+    return hipGetLastError();
+}
+
+hipError_t spmv_stream_probe(int64_t nrows, const int64_t* rowp, const int32_t* col, const float* val, int64_t nnz, const int64_t* blk,
+                             int64_t nblk, const float* x, float* y, double* part, double* sc, hipStream_t st) {
+    hipLaunchKernelGGL(spmv_stream_f32, dim3(SLX_SOLVE_BLOCKS), dim3(256), 0, st, nblk, blk, rowp, col, val, nnz, x, y, part);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, part, 1, (int)SC_PQ, 0, 0, sc);
+    return hipGetLastError();
+}
+
 hipError_t solve_system(const SolveSystem& s, const SolveParams& prm, SolveWork& w, SolveStats& out, hipStream_t st) {
     const int64_t n = s.nfree;
     out = SolveStats{};
     if (n == 0) return hipSuccess;
+    const unsigned G = SLX_SOLVE_BLOCKS;
     // f = b_A - D x_D ; x_D = Dirichlet values from the rhs
     hipLaunchKernelGGL(gather_kernel, dim3(blocks(n)), dim3(256), 0, st, n, s.free_dofs, s.rhs, w.f);
     if (s.ncons > 0) {
         hipLaunchKernelGGL(gather_kernel, dim3(blocks(s.ncons)), dim3(256), 0, st, s.ncons, s.cons_dofs, s.rhs, w.xd);
-        spmv<double, double, double>(n, s.Drow, s.Dcol, s.Dval, w.xd, w.f, 1.0, 1.0, w.f, st);
+        spmv<double, double, double>(n, s.nnzD, s.Drow, s.Dcol, s.Dval, w.xd, w.f, 1.0, 1.0, w.f, nullptr, st);
     }
     hipLaunchKernelGGL(prep_kernel, dim3(blocks(n)), dim3(256), 0, st, n, s.Arow, s.Acol, s.Aval, w.val32, w.dinv);
     HIPERR(hipMemsetAsync(w.x, 0, n * sizeof(double), st));
-    double h[3];
-    dot<double>(n, w.f, w.f, w.part, w.sc + 3, st);
-    HIPERR(hipMemcpyAsync(h, w.sc + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIPERR(hipStreamSynchronize(st));
-    const double fnorm = sqrt(h[0]);
+    double h = 0.0;
+    hipLaunchKernelGGL((dot_partial<double>), dim3(G), dim3(256), 0, st, n, w.f, w.f, w.part);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 1, (int)SC_DOT, 0, 0, w.sc);
+    HIPERR(read_scalar(w.sc, SC_DOT, &h, st));
+    const double fnorm = sqrt(h);
     // r = f (x = 0)
     HIPERR(hipMemcpyAsync(w.r, w.f, n * sizeof(double), hipMemcpyDeviceToDevice, st));
     double rnorm = fnorm;
     for (int it = 0; it < prm.max_outer; it++) {
-        out.rel_residual = fnorm > 0 ? rnorm / fnorm : 0.0;
         if (rnorm <= prm.rel_tol * fnorm) break;
         out.outer++;
         // inner FP32 PCG on A d = r
-        hipLaunchKernelGGL(to_f32, dim3(blocks(n)), dim3(256), 0, st, n, w.r, w.r32);
-        HIPERR(hipMemsetAsync(w.d, 0, n * sizeof(float), st));
-        hipLaunchKernelGGL(precond_kernel, dim3(blocks(n)), dim3(256), 0, st, n, w.dinv, w.r32, w.z);
-        HIPERR(hipMemcpyAsync(w.p, w.z, n * sizeof(float), hipMemcpyDeviceToDevice, st));
-        dot<float>(n, w.r32, w.z, w.part, w.sc + 0, st);  // sc[0] = rz
-        dot<float>(n, w.r32, w.r32, w.part, w.sc + 3, st);
-        HIPERR(hipMemcpyAsync(h, w.sc + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPERR(hipStreamSynchronize(st));
-        const double r0 = sqrt(h[0]);
+        hipLaunchKernelGGL(pcg_start, dim3(G), dim3(256), 0, st, n, w.r, w.dinv, w.r32, w.z, w.p, w.d, w.part);
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 2, (int)SC_RZ, (int)SC_RR, 0, w.sc);
+        HIPERR(read_scalar(w.sc, SC_RR, &h, st));
+        const double r0 = sqrt(h);
         for (int k = 0; k < prm.max_inner; k++) {
-            spmv<float, float, float>(n, s.Arow, s.Acol, w.val32, w.p, (const float*)nullptr, 0.f, -1.f, w.q, st);  // q = A p
-            dot<float>(n, w.p, w.q, w.part, w.sc + 1, st);                                                           // sc[1] = pq
-            hipLaunchKernelGGL(cg_update, dim3(blocks(n)), dim3(256), 0, st, n, w.sc, w.d, w.r32, w.p, w.q);
+            hipLaunchKernelGGL(spmv_stream_f32, dim3(G), dim3(256), 0, st, s.nblk, s.Ablk, s.Arow, s.Acol, w.val32, s.nnzA, w.p, w.q,
+                               w.part);
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 1, (int)SC_PQ, 0, 0, w.sc);
+            hipLaunchKernelGGL(pcg_update, dim3(G), dim3(256), 0, st, n, w.sc, w.d, w.r32, w.p, w.q, w.dinv, w.z, w.part);
+            hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 2, (int)SC_RZN, (int)SC_RR, 1, w.sc);
             out.inner++;
             if ((k + 1) % 8 == 0 || k + 1 == prm.max_inner) {
-                dot<float>(n, w.r32, w.r32, w.part, w.sc + 3, st);
-                HIPERR(hipMemcpyAsync(h, w.sc + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-                HIPERR(hipStreamSynchronize(st));
-                if (sqrt(h[0]) <= prm.inner_tol * r0) break;
+                HIPERR(read_scalar(w.sc, SC_RR, &h, st));
+                if (sqrt(h) <= prm.inner_tol * r0) break;
             }
-            hipLaunchKernelGGL(precond_kernel, dim3(blocks(n)), dim3(256), 0, st, n, w.dinv, w.r32, w.z);
-            dot<float>(n, w.r32, w.z, w.part, w.sc + 2, st);  // sc[2] = rz_new
-            hipLaunchKernelGGL(cg_direction, dim3(blocks(n)), dim3(256), 0, st, n, w.sc, w.z, w.p);
-            HIPERR(hipMemcpyAsync(w.sc + 0, w.sc + 2, sizeof(double), hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(pcg_direction, dim3(G), dim3(256), 0, st, n, w.sc, w.z, w.p);
         }
         hipLaunchKernelGGL(add_f32_to_f64, dim3(blocks(n)), dim3(256), 0, st, n, w.d, w.x);
         // FP64 residual r = f - A x
-        spmv<double, double, double>(n, s.Arow, s.Acol, s.Aval, w.x, w.f, 1.0, 1.0, w.r, st);
-        dot<double>(n, w.r, w.r, w.part, w.sc + 3, st);
-        HIPERR(hipMemcpyAsync(h, w.sc + 3, sizeof(double), hipMemcpyDeviceToHost, st));
-        HIPERR(hipStreamSynchronize(st));
-        rnorm = sqrt(h[0]);
+        spmv<double, double, double>(n, s.nnzA, s.Arow, s.Acol, s.Aval, w.x, w.f, 1.0, 1.0, w.r, nullptr, st);
+        hipLaunchKernelGGL((dot_partial<double>), dim3(G), dim3(256), 0, st, n, w.r, w.r, w.part);
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 1, (int)SC_DOT, 0, 0, w.sc);
+        HIPERR(read_scalar(w.sc, SC_DOT, &h, st));
+        rnorm = sqrt(h);
     }
     out.rel_residual = fnorm > 0 ? rnorm / fnorm : 0.0;
     // full solution: free entries from x, constrained entries the Dirichlet values

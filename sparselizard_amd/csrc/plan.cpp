@@ -138,6 +138,8 @@ struct slx_plan_s {
     std::vector<DevBucket> dbuckets;
     DBuf consts;
     DBuf d_Arow, d_Drow, d_Acol, d_Dcol, d_rdesc, d_erows, d_slots, d_rhs_ptr, d_rhs_erows, d_free, d_cons;
+    DBuf d_ablk;          // row blocks of the streaming FP32 product (solve)
+    int64_t n_ablk = 0;
     std::vector<RedClass> red_classes;  // reduction launch classes (rows of similar length)
     DBuf d_Aval, d_Dval, d_rhs, d_ke, d_ctab;
     size_t ke_capacity_bytes = 0;
@@ -1242,19 +1244,79 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
     });
 }
 
+namespace {
+// device view of the assembled system for the solve kernels (row blocks of
+// the streaming product built on first use)
+SolveSystem solve_view(slx_plan p) {
+    if (p->n_ablk == 0 && p->nfree > 0) {
+        std::vector<int64_t> blk;
+        stream_row_blocks(p->Arow.data(), p->nfree, blk);
+        p->n_ablk = (int64_t)blk.size() - 1;
+        upload(p->d_ablk, blk.data(), blk.size(), p->stream);
+    }
+    return SolveSystem{p->n_dofs, p->nfree, p->ncons, p->nnzA, p->nnzD, p->d_Arow.as<int64_t>(), p->d_Drow.as<int64_t>(),
+                       p->d_Acol.as<int32_t>(), p->d_Dcol.as<int32_t>(), p->d_Aval.as<double>(), p->d_Dval.as<double>(),
+                       p->d_rhs.as<double>(), p->d_free.as<int64_t>(), p->d_cons.as<int64_t>(), p->d_ablk.as<int64_t>(),
+                       p->n_ablk};
+}
+
+void check_assembled(slx_plan p) {
+    SLX_CHECK(p, "null plan");
+    SLX_CHECK(!p->host_only, "plan was created without a device context");
+    SLX_CHECK(p->has_matrix, "the formulation has no matrix terms");
+    SLX_CHECK(p->last_precision >= 0, "no assembly has run on this plan");
+}
+}  // namespace
+
+slx_status slx_plan_eliminate(slx_plan p, const double* b, double* out) {
+    return guarded([&] {
+        check_assembled(p);
+        SLX_CHECK(out || p->nfree == 0, "null output");
+        HIPCHECK(hipSetDevice(p->ctx->device));
+        const SolveSystem sys = solve_view(p);
+        DBuf ws;
+        ws.alloc((size_t)(p->n_dofs + p->nfree + p->ncons + 1) * sizeof(double));
+        double* db = ws.as<double>();
+        double* dout = db + p->n_dofs;
+        double* dxd = dout + p->nfree;
+        if (b) HIPCHECK(hipMemcpyAsync(db, b, p->n_dofs * sizeof(double), hipMemcpyHostToDevice, p->stream));
+        else HIPCHECK(hipMemcpyAsync(db, p->d_rhs.p, p->n_dofs * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+        HIPCHECK(eliminate_system(sys, db, dout, dxd, p->stream));
+        if (p->nfree) HIPCHECK(hipMemcpyAsync(out, dout, p->nfree * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+        HIPCHECK(hipStreamSynchronize(p->stream));
+    });
+}
+
+slx_status slx_plan_matvec(slx_plan p, const double* x, double* y) {
+    return guarded([&] {
+        check_assembled(p);
+        SLX_CHECK((x && y) || p->n_dofs == 0, "null argument");
+        HIPCHECK(hipSetDevice(p->ctx->device));
+        const SolveSystem sys = solve_view(p);
+        DBuf ws;
+        ws.alloc((size_t)(2 * p->n_dofs + 2 * p->nfree + p->ncons + 1) * sizeof(double));
+        double* dx = ws.as<double>();
+        double* dy = dx + p->n_dofs;
+        double* xa = dy + p->n_dofs;
+        double* ya = xa + p->nfree;
+        double* xd = ya + p->nfree;
+        if (p->n_dofs) HIPCHECK(hipMemcpyAsync(dx, x, p->n_dofs * sizeof(double), hipMemcpyHostToDevice, p->stream));
+        HIPCHECK(matvec_system(sys, dx, dy, xa, xd, ya, p->stream));
+        if (p->n_dofs) HIPCHECK(hipMemcpyAsync(y, dy, p->n_dofs * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+        HIPCHECK(hipStreamSynchronize(p->stream));
+    });
+}
+
 slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out, double* d_x_out,
                           int32_t* outer_its, int32_t* inner_its, double* rel_residual) {
     return guarded([&] {
-        SLX_CHECK(p, "null plan");
-        SLX_CHECK(!p->host_only, "plan was created without a device context");
-        SLX_CHECK(p->has_matrix, "the formulation has no matrix terms");
-        SLX_CHECK(p->last_precision >= 0, "no assembly has run on this plan");
+        check_assembled(p);
         SLX_CHECK(rel_tol > 0 && max_outer > 0 && max_inner > 0, "solve: rel_tol, max_outer and max_inner must be positive");
         SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
         HIPCHECK(hipSetDevice(p->ctx->device));
         const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
         // one workspace block: doubles first, then floats (8-byte aligned)
-        const int64_t nD = 3 * nf + nc + nd + 512 + 8, nF = 6 * nf + p->nnzA;
+        const int64_t nD = 3 * nf + nc + nd + 2 * SLX_SOLVE_BLOCKS + 8, nF = 6 * nf + p->nnzA + 4;
         DBuf ws;
         ws.alloc((size_t)(nD * sizeof(double) + nF * sizeof(float) + 64));
         SolveWork w;
@@ -1264,7 +1326,7 @@ slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t
         w.x = dp; dp += nf;
         w.xd = dp; dp += nc;
         w.xfull = dp; dp += nd;
-        w.part = dp; dp += 512;
+        w.part = dp; dp += 2 * SLX_SOLVE_BLOCKS;
         w.sc = dp; dp += 8;
         float* fp = (float*)dp;
         w.r32 = fp; fp += nf;
@@ -1273,10 +1335,8 @@ slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t
         w.q = fp; fp += nf;
         w.d = fp; fp += nf;
         w.dinv = fp; fp += nf;
-        w.val32 = fp;
-        SolveSystem sys{nd, nf, nc, p->d_Arow.as<int64_t>(), p->d_Drow.as<int64_t>(), p->d_Acol.as<int32_t>(), p->d_Dcol.as<int32_t>(),
-                        p->d_Aval.as<double>(), p->d_Dval.as<double>(), p->d_rhs.as<double>(), p->d_free.as<int64_t>(),
-                        p->d_cons.as<int64_t>()};
+        w.val32 = (float*)(((uintptr_t)fp + 15) & ~(uintptr_t)15);
+        const SolveSystem sys = solve_view(p);
         SolveParams prm{rel_tol, 1e-3, max_outer, max_inner};
         SolveStats stats;
         HIPCHECK(solve_system(sys, prm, w, stats, p->stream));
@@ -1423,6 +1483,37 @@ slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256) {
         HIPCHECK(launch_mfma_probe(d.as<float>(), ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         HIPCHECK(hipMemcpy(out256, d.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+slx_status slx_selftest_spmv_stream(slx_ctx ctx, int64_t nrows, const int64_t* rowp, const int32_t* col, const double* val,
+                                    const double* x, double* y, double* xy) {
+    return guarded([&] {
+        SLX_CHECK(ctx && rowp && y && xy && nrows > 0, "null argument");
+        const int64_t nnz = rowp[nrows];
+        SLX_CHECK(rowp[0] == 0 && nnz >= 0, "row pointer must start at 0");
+        for (int64_t r = 0; r < nrows; r++) SLX_CHECK(rowp[r + 1] >= rowp[r], "row pointer decreases at row " << r);
+        for (int64_t k = 0; k < nnz; k++) SLX_CHECK(col[k] >= 0 && col[k] < nrows, "column index out of range at " << k);
+        HIPCHECK(hipSetDevice(ctx->device));
+        std::vector<int64_t> blk;
+        stream_row_blocks(rowp, nrows, blk);
+        std::vector<float> v32(val, val + nnz), x32(x, x + nrows);
+        DBuf drow, dcol, dval, dblk, dx, dy, dpart;
+        upload(drow, rowp, nrows + 1, ctx->stream);
+        upload(dcol, nnz ? col : nullptr, std::max<int64_t>(nnz, 1), ctx->stream);
+        upload(dval, nnz ? v32.data() : nullptr, std::max<int64_t>(nnz, 1), ctx->stream);
+        upload(dblk, blk.data(), blk.size(), ctx->stream);
+        upload(dx, x32.data(), nrows, ctx->stream);
+        dy.alloc(nrows * sizeof(float));
+        dpart.alloc((SLX_SOLVE_BLOCKS + 8) * sizeof(double));
+        HIPCHECK(spmv_stream_probe(nrows, drow.as<int64_t>(), dcol.as<int32_t>(), dval.as<float>(), nnz, dblk.as<int64_t>(),
+                                   (int64_t)blk.size() - 1, dx.as<float>(), dy.as<float>(), dpart.as<double>(),
+                                   dpart.as<double>() + SLX_SOLVE_BLOCKS, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        std::vector<float> y32(nrows);
+        HIPCHECK(hipMemcpy(y32.data(), dy.p, nrows * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(xy, dpart.as<double>() + SLX_SOLVE_BLOCKS + 1, sizeof(double), hipMemcpyDeviceToHost));
+        std::copy(y32.begin(), y32.end(), y);
     });
 }
 

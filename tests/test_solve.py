@@ -93,3 +93,75 @@ def test_solve_argument_errors(ctx):
     p.generate("fp64")
     with pytest.raises(sa.SlxError, match="must be positive"):
         plan.solve(info["ndofs"], rel_tol=0.0)
+
+
+def _random_csr(rng, lens):
+    n = len(lens)
+    rowp = np.zeros(n + 1, np.int64)
+    rowp[1:] = np.cumsum(lens)
+    # repeated columns in a row are allowed (the product sums them, as scipy does)
+    col = rng.integers(0, n, int(rowp[-1])).astype(np.int32)
+    val = rng.standard_normal(int(rowp[-1]))
+    return rowp, col, val
+
+
+@pytest.mark.parametrize("case", ["short", "mixed_long", "edges"])
+def test_spmv_stream_matches_numpy(ctx, case):
+    # the inner-iteration product: row blocks of at most 4096 nonzeros and
+    # 256 rows, 16-byte loads from aligned-down starts, long rows by a whole block
+    from sparselizard_amd import _lib as L
+    print("case spmv_stream", case, flush=True)
+    rng = np.random.default_rng(7)
+    n = 20000
+    if case == "short":
+        lens = rng.integers(0, 12, n)
+    elif case == "mixed_long":
+        lens = rng.integers(20, 120, n)
+        lens[[5, 777, n - 1]] = [4097, 9000, 12001]
+    else:
+        lens = np.zeros(n, np.int64)
+        lens[:300] = 1                 # more than 256 rows in one block's nonzero budget
+        lens[300] = 4096               # exactly one block's budget
+        lens[301] = 4095
+        lens[302:] = rng.integers(0, 7, n - 302)
+        lens[-1] = 3                   # nnz not a multiple of 4 at the very end
+    lens = np.asarray(lens, np.int64)
+    rowp, col, val = _random_csr(rng, lens)
+    x = rng.standard_normal(n)
+    y, xy = L.spmv_stream_probe(ctx, rowp, col, val, x)
+    v32 = val.astype(np.float32).astype(np.float64)
+    x32 = x.astype(np.float32).astype(np.float64)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((v32, col, rowp), shape=(n, n))
+    ref = A @ x32
+    mag = abs(A) @ abs(x32)
+    assert np.all(np.abs(y - ref) <= 2e-6 * np.maximum(mag, 1e-30) + 1e-30), float(np.max(np.abs(y - ref) / np.maximum(mag, 1e-30)))
+    assert abs(xy - float(x32 @ y)) <= 1e-10 * float(abs(x32) @ abs(y))
+
+
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=20"), ("elasticity3d", "n=3")])
+def test_matvec_and_eliminate(ctx, kind, spec):
+    import scipy.sparse as sp
+    print("case matvec/eliminate", kind, spec, flush=True)
+    p = sa.Problem(ctx, kind, spec)
+    info = register_free(p)
+    plan = p.generate("fp64")
+    nd, nf = info["ndofs"], info["nfree"]
+    csr = plan.csr(nd)
+    A = sp.csr_matrix((csr.Aval, csr.Acol, csr.Arow), shape=(nf, nf))
+    D = sp.csr_matrix((csr.Dval, csr.Dcol, csr.Drow), shape=(nf, nd - nf))
+    free = _FREE[(nd, nf)]
+    cons = np.ones(nd, bool)
+    cons[free] = False
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(nd)
+    # mat * vec (mat.cpp:182-205)
+    y = plan.matvec(x)
+    yA = A @ x[free] + D @ x[cons]
+    assert relerr(y[free], yA) <= 1e-13
+    assert np.array_equal(y[cons], x[cons])
+    # mat::eliminate (mat.cpp:78-103), of a given vector and of the assembled rhs
+    e = plan.eliminate(nf, x)
+    assert relerr(e, x[free] - D @ x[cons]) <= 1e-13
+    e0 = plan.eliminate(nf)
+    assert relerr(e0, csr.rhs[free] - D @ csr.rhs[cons]) <= 1e-13
