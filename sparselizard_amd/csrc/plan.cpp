@@ -155,6 +155,14 @@ struct slx_plan_s {
     bool signed_slots = false;     // folded orientations: signed segment entries
 
     hipGraphExec_t graph_exec = nullptr;  // captured numeric phase (slx_plan_graph_capture)
+    // right-hand-side-only buckets run, with the rhs reduction, on a side
+    // stream forked from and joined back into the caller's stream; they keep
+    // their coefficients in their own table
+    std::vector<uint8_t> rhs_only;     // per bucket: every slice is a rhs slice
+    bool side_ok = false;              // every rhs slice lives in a rhs-only bucket
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    DBuf d_ctab_side;
     std::vector<MfmaBucket> mfma;      // per bucket; M == 0 when not eligible
     std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
     std::vector<std::vector<std::pair<int, FactorSlice>>> factor;  // per bucket: (slice, factorized MFMA data)
@@ -873,10 +881,19 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, P.ctx->device) == hipSuccess && prop.multiProcessorCount > 0) P.n_cu = prop.multiProcessorCount;
     }
-    size_t ctab_words = 1;
+    size_t ctab_words = 1, ctab_side_words = 1;
+    P.rhs_only.assign(d.n_buckets, 0);
+    bool rhs_in_mixed = false, any_rhs_only = false;
     for (int b = 0; b < d.n_buckets; b++) {
         const slx_bucket_desc& bd = d.buckets[b];
         DevBucket& db = P.dbuckets[b];
+        {
+            int nr = 0;
+            for (int s = 0; s < bd.n_slices; s++) nr += bd.slices[s].is_rhs ? 1 : 0;
+            P.rhs_only[b] = (bd.n_slices > 0 && nr == bd.n_slices) ? 1 : 0;
+            if (nr > 0 && !P.rhs_only[b]) rhs_in_mixed = true;
+            any_rhs_only = any_rhs_only || P.rhs_only[b];
+        }
         std::memset(&db, 0, sizeof(db));
         SLX_CHECK(bd.n_slices <= SLX_MAX_SLICES, "bucket has " << bd.n_slices << " slices (max " << SLX_MAX_SLICES << ")");
         SLX_CHECK(bd.n_regs <= SLX_MAX_REGS, "coefficient program needs " << bd.n_regs << " registers (max " << SLX_MAX_REGS << ")");
@@ -912,7 +929,9 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             std::vector<double> tq = table_qmajor(bd.tables[t], bd.table_nff[t], bd.n_gp);
             dtabs[t] = P.dev(tq.data(), tq.size());
         }
-        ctab_words = std::max(ctab_words, (size_t)std::max(bd.n_coefs, 1) * (bd.q_invariant ? 1 : bd.n_gp) * std::max(bd.n_elem, 1));
+        const size_t words = (size_t)std::max(bd.n_coefs, 1) * (bd.q_invariant ? 1 : bd.n_gp) * std::max(bd.n_elem, 1);
+        if (P.rhs_only[b]) ctab_side_words = std::max(ctab_side_words, words);
+        else ctab_words = std::max(ctab_words, words);
         for (int s = 0; s < bd.n_slices; s++) {
             const slx_slice_desc& sd = bd.slices[s];
             DevSlice& ds = db.slices[s];
@@ -969,6 +988,13 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));
+    P.d_ctab_side.alloc(ctab_side_words * sizeof(double));
+    P.side_ok = d.has_matrix && d.has_rhs && any_rhs_only && !rhs_in_mixed;
+    if (P.side_ok) {
+        HIPCHECK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
+    }
 }
 
 // Per-launch HIP events by kernel class (profiling only).
@@ -1022,32 +1048,41 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
     in.consts = P.consts.as<double>();
     for (int f = 0; f < P.n_fields; f++) in.field_data[f] = fields ? fields[f] : nullptr;
     bool ke64 = precision == SLX_PREC_FP64;
+    // fork: rhs-only buckets and the rhs reduction overlap the matrix work
+    // (serial when every launch is timed)
+    const bool overlap = P.side_ok && !tm;
+    if (overlap) {
+        HIPCHECK(hipEventRecord(P.ev_fork, st));
+        HIPCHECK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
+    }
     for (size_t bi = 0; bi < P.dbuckets.size(); bi++) {
         const DevBucket& b = P.dbuckets[bi];
+        const hipStream_t bst = (overlap && P.rhs_only[bi]) ? P.side : st;
+        void* ctab = P.rhs_only[bi] ? P.d_ctab_side.p : P.d_ctab.p;
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
         // the factorized kernel reads the coefficients of one element contiguously
         const int qfast = (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) ? 1 : 0;
         P.last_qfast = qfast;
         if (tm) tm->begin(T_COEF);
-        HIPCHECK(launch_coef(b, in, ke64, P.d_ctab.p, st, qfast));
+        HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
         if (tm) tm->end();
         if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
             if (tm) tm->begin(T_MFMA);
-            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], P.d_ctab.as<float>(), P.d_ke.as<float>(), st));
+            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], (const float*)ctab, P.d_ke.as<float>(), bst));
             if (tm) tm->end();
             done = P.mfma_mask[bi];
         }
         if (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) {
             if (tm) tm->begin(T_MFMA);
-            for (auto& fs : P.factor[bi]) HIPCHECK(launch_factor_f16split(b, fs.second, P.d_ctab.as<float>(), P.d_ke.as<float>(), P.n_cu, st));
+            for (auto& fs : P.factor[bi]) HIPCHECK(launch_factor_f16split(b, fs.second, (const float*)ctab, P.d_ke.as<float>(), P.n_cu, bst));
             if (tm) tm->end();
             done |= P.factor_mask[bi];
         }
         int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
         if ((all & ~done) != 0) {
             if (tm) tm->begin(T_VALU);
-            HIPCHECK(launch_elem(b, P.d_ctab.p, p, P.d_ke.p, st, all & ~done, qfast));
+            HIPCHECK(launch_elem(b, ctab, p, P.d_ke.p, bst, all & ~done, qfast));
             if (tm) tm->end();
         }
     }
@@ -1067,8 +1102,13 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
     if (P.has_rhs) {
         if (tm) tm->begin(T_RHS);
         HIPCHECK(launch_reduce(P.rd, P.max_erow_len, ke64, P.d_ke.p, P.d_Aval.as<double>(), P.d_Dval.as<double>(), P.d_rhs.as<double>(),
-                               false, true, st, nullptr, nullptr, nullptr, 0));
+                               false, true, overlap ? P.side : st, nullptr, nullptr, nullptr, 0));
         if (tm) tm->end();
+    }
+    if (overlap) {
+        // join
+        HIPCHECK(hipEventRecord(P.ev_join, P.side));
+        HIPCHECK(hipStreamWaitEvent(st, P.ev_join, 0));
     }
     P.last_precision = precision;
 }
@@ -1424,16 +1464,17 @@ slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* 
         SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         SLX_CHECK(p->last_precision >= 0 && !p->dbuckets.empty(), "no assembly has run on this plan");
         const DevBucket& b = p->dbuckets.back();
+        const void* ctab = p->rhs_only.back() ? p->d_ctab_side.p : p->d_ctab.p;
         const int64_t n = (int64_t)b.n_coefs * (b.q_inv ? 1 : b.n_gp) * b.n_elem;
         *n_out = n;
         if (!out) return;
         SLX_CHECK(cap >= n, "output holds " << cap << " entries, the table has " << n);
         HIPCHECK(hipStreamSynchronize(p->stream));
         if (p->last_precision == SLX_PREC_FP64) {
-            HIPCHECK(hipMemcpy(out, p->d_ctab.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(out, ctab, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
         } else {
             std::vector<float> tmp(n);
-            HIPCHECK(hipMemcpy(tmp.data(), p->d_ctab.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(tmp.data(), ctab, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
             const int64_t nq = b.q_inv ? 1 : b.n_gp, ne = b.n_elem;
             if (p->last_qfast) {
                 // (slot, element, q) -> (slot, q, element)
@@ -1469,7 +1510,11 @@ slx_status slx_plan_destroy(slx_plan p) {
     return guarded([&] {
         if (!p) return;
         if (!p->host_only) (void)hipStreamSynchronize(p->stream);
+        if (p->side) (void)hipStreamSynchronize(p->side);
         if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+        if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+        if (p->ev_join) (void)hipEventDestroy(p->ev_join);
+        if (p->side) (void)hipStreamDestroy(p->side);
         delete p;
     });
 }
