@@ -482,8 +482,54 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             P.h_ke_base.push_back(L.ke_base[b][s]);
             P.h_ke_ld.push_back(L.ke_ld[b][s]);
         }
-    // reduction work order: rows by decreasing length (stable), in classes whose
-    // longest row is at most twice the shortest class bound (LDS sized per class)
+    // rows whose slot blocks are identical share one copy: the components of a
+    // vector field at one node have the same columns and the same segments, and
+    // on meshes with a repeating topology most rows of one kind have the same
+    // block (a jittered 16^3 tet grid at P2: 141 distinct blocks for 104,544
+    // rows); the few distinct blocks then stay in cache.
+    // rep[r] = lowest row with the same block
+    std::vector<int64_t> rep(nf);
+    {
+        std::vector<uint64_t> h(nf);
+        auto hwork = [&](int t) {
+            for (int64_t r = nf * t / nthreads; r < nf * (t + 1) / nthreads; r++) {
+                uint64_t x = 1469598103934665603ull ^ (uint64_t)(slot_ptr[r + 1] - slot_ptr[r]);
+                for (int64_t i = slot_ptr[r]; i < slot_ptr[r + 1]; i++) x = (x ^ (uint32_t)slots[i]) * 1099511628211ull;
+                h[r] = x;
+            }
+        };
+        std::vector<std::thread> hth;
+        for (int t = 0; t < nthreads; t++) hth.emplace_back(hwork, t);
+        for (auto& x : hth) x.join();
+        std::vector<int64_t> byh(nf);
+        for (int64_t r = 0; r < nf; r++) byh[r] = r;
+        std::sort(byh.begin(), byh.end(), [&](int64_t a, int64_t b) { return h[a] != h[b] ? h[a] < h[b] : a < b; });
+        auto same = [&](int64_t a, int64_t b) {
+            const int64_t la = slot_ptr[a + 1] - slot_ptr[a];
+            return la == slot_ptr[b + 1] - slot_ptr[b] &&
+                   std::equal(slots.begin() + slot_ptr[a], slots.begin() + slot_ptr[a + 1], slots.begin() + slot_ptr[b]);
+        };
+        for (int64_t i = 0; i < nf;) {
+            int64_t j = i;
+            while (j < nf && h[byh[j]] == h[byh[i]]) j++;
+            // rows i..j-1 share a hash (ascending row order): match each to the first equal one
+            for (int64_t a = i; a < j; a++) {
+                rep[byh[a]] = byh[a];
+                for (int64_t b = i; b < a; b++)
+                    if (rep[byh[b]] == byh[b] && same(byh[b], byh[a])) {
+                        rep[byh[a]] = byh[b];
+                        break;
+                    }
+            }
+            i = j;
+        }
+    }
+    // reduction work order: rows by decreasing length (stable: a row's slot-block
+    // representative, an earlier row of the same length, comes first), in
+    // classes whose longest row is at most twice the shortest class bound (LDS
+    // sized per class).  Row order is kept within a length so that rows sharing
+    // elements, whose segments sit next to each other in the element matrices,
+    // are reduced close together in time.
     std::vector<int32_t> order(nf);
     for (int64_t r = 0; r < nf; r++) order[r] = (int32_t)r;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return lenA[a] + lenD[a] > lenA[b] + lenD[b]; });
@@ -516,20 +562,29 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     // the rows one block reduces read contiguous metadata
     {
         std::vector<uint64_t> rdesc((size_t)nf * 4), erows_w(erows.size());
-        std::vector<int64_t> slot_w(nf);  // work-order start of each position's slots
+        std::vector<int64_t> slot_w(nf, -1);  // work-order start of each position's slots (-1: shares its rep's block)
+        std::vector<int64_t> rep_sw(nf, -1);  // slot block start of a representative row
         int64_t kw = 0, sw = 0;
         for (int64_t p = 0; p < nf; p++) {
             const int64_t r = order[p];
             const int64_t nk = erow_ptr[r + 1] - erow_ptr[r];
             SLX_CHECK(nk < 65536 && lenA[r] < 65536 && lenD[r] < 65536, "a row has more than 65535 segments or columns");
+            int64_t sp;
+            if (rep[r] == r) {
+                sp = sw;
+                rep_sw[r] = sw;
+                slot_w[p] = sw;
+                sw += slot_ptr[r + 1] - slot_ptr[r];
+            } else {
+                sp = rep_sw[rep[r]];  // the representative comes first in the work order
+                SLX_CHECK(sp >= 0, "internal: shared slot block used before its representative");
+            }
             rdesc[4 * p + 0] = (uint64_t)P.Arow[r] | ((uint64_t)lenA[r] << 48);
             rdesc[4 * p + 1] = (uint64_t)P.Drow[r] | ((uint64_t)lenD[r] << 48);
             rdesc[4 * p + 2] = (uint64_t)kw | ((uint64_t)nk << 48);
-            rdesc[4 * p + 3] = (uint64_t)sw;
+            rdesc[4 * p + 3] = (uint64_t)sp;
             std::copy(erows.begin() + erow_ptr[r], erows.begin() + erow_ptr[r + 1], erows_w.begin() + kw);
-            slot_w[p] = sw;
             kw += nk;
-            sw += slot_ptr[r + 1] - slot_ptr[r];
         }
         SLX_CHECK(P.Arow[nf] < (int64_t(1) << 48) && P.Drow[nf] < (int64_t(1) << 48), "matrix exceeds the 48-bit index range");
         upload(P.d_rdesc, rdesc.data(), rdesc.size(), st);
@@ -539,8 +594,9 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         auto narrow = [&](auto tag) {
             using T = decltype(tag);
             const T top = (T)(T(1) << (8 * sizeof(T) - 1));
-            std::vector<T> v(slots.size());
+            std::vector<T> v(std::max<int64_t>(sw, 1));
             for (int64_t p = 0; p < nf; p++) {
+                if (slot_w[p] < 0) continue;
                 const int64_t r = order[p];
                 T* dst = v.data() + slot_w[p];
                 for (int64_t i = slot_ptr[r]; i < slot_ptr[r + 1]; i++) {
