@@ -215,16 +215,25 @@ slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_b
  * (src/formulation/mat.cpp:78-103), a solve with A, and the merge of the
  * free and constrained values of mat::xbmerge (x_out: n_dofs entries, the
  * constrained ones equal to the Dirichlet values).  The reference factorizes
- * A (MUMPS LU / Cholesky); here A must be symmetric positive definite:
- * FP64 iterative refinement (FP64 residuals r = b_A - D x_D - A x) around a
- * Jacobi-preconditioned FP32 conjugate-gradient inner solve.  Stops when
- * ||r|| <= rel_tol ||b_A - D x_D|| or after max_outer refinements of at most
- * max_inner inner iterations each.  Returns the counts and the final relative
- * FP64 residual; not reaching rel_tol is reported through rel_residual, not
- * as an error.  Allocates its device workspace per call.  x_out: host
+ * A (MUMPS LU / Cholesky); here the solve is iterative: FP64 iterative
+ * refinement (FP64 residuals r = b_A - D x_D - A x) around an FP32 inner
+ * solve with a Jacobi preconditioner, either conjugate gradients
+ * (SLX_SOLVE_CG: symmetric positive definite A) or one GMRES cycle of at most
+ * min(max_inner, 100) Krylov vectors per refinement (SLX_SOLVE_GMRES: any
+ * nonsingular A).  SLX_SOLVE_AUTO checks A for symmetry (per entry: 1e-5
+ * relative plus 1e-6 of the two diagonal entries, so assembly rounding does
+ * not count) and picks CG or GMRES.  Stops when ||r|| <= rel_tol ||b_A - D x_D||
+ * or after max_outer refinements; max_inner bounds the inner iterations of
+ * one refinement.  Returns the counts, the final relative FP64 residual and
+ * the method used; not reaching rel_tol is reported through rel_residual,
+ * not as an error.  Allocates its device workspace per call.  x_out: host
  * (d_x_out NULL) or device (x_out NULL) memory. */
-slx_status slx_plan_solve(slx_plan plan, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
-                          double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual);
+#define SLX_SOLVE_AUTO 0
+#define SLX_SOLVE_CG 1
+#define SLX_SOLVE_GMRES 2
+slx_status slx_plan_solve(slx_plan plan, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
+                          double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual,
+                          int32_t* method_used);
 /* mat::eliminate(b) (src/formulation/mat.cpp:78-103): b_A - D b_D of a
  * vector in DOF numbering (n_dofs entries; b NULL: the rhs of the last
  * assembly, whose constrained entries are the Dirichlet values), on the

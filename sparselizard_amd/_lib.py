@@ -102,7 +102,7 @@ def lib():
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
-        "slx_plan_solve": (i32, [vp, C.c_double, i32, i32, pd, vp, p32, p32, pd]),
+        "slx_plan_solve": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
         "slx_plan_eliminate": (i32, [vp, pd, pd]),
         "slx_plan_matvec": (i32, [vp, pd, pd]),
         "slx_selftest_spmv_stream": (i32, [vp, i64, p64, p32, pd, pd, pd, pd]),
@@ -404,17 +404,22 @@ class Plan:
                                     _p(rhs, C.c_double)))
         return CSR(Arow, Acol[:s["nnzA"]], Aval[:s["nnzA"]], Drow, Dcol[:s["nnzD"]], Dval[:s["nnzD"]], rhs[:ndofs])
 
-    def solve(self, ndofs, rel_tol=1e-10, max_outer=20, max_inner=2000):
+    SOLVE_METHODS = {"auto": 0, "cg": 1, "gmres": 2}
+
+    def solve(self, ndofs, rel_tol=1e-10, max_outer=20, max_inner=2000, method="auto"):
         """sl::solve(A, b) on the last assembled system (sl.cpp:1565-1627):
         solution in DOF numbering, constrained entries = Dirichlet values.
-        Returns (x, info) with the refinement count, the inner iteration count
-        and the final relative FP64 residual of b_A - D x_D - A x."""
+        method: "cg" (SPD A), "gmres" (any nonsingular A) or "auto" (symmetry
+        check).  Returns (x, info) with the refinement count, the inner
+        iteration count, the final relative FP64 residual of b_A - D x_D - A x
+        and the method used."""
         x = np.zeros(max(ndofs, 1), np.float64)
-        o, i = C.c_int32(), C.c_int32()
+        o, i, mu = C.c_int32(), C.c_int32(), C.c_int32()
         r = C.c_double()
-        check(lib().slx_plan_solve(self.h, rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
-                                   C.byref(o), C.byref(i), C.byref(r)))
-        return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value)
+        check(lib().slx_plan_solve(self.h, self.SOLVE_METHODS[method], rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
+                                   C.byref(o), C.byref(i), C.byref(r), C.byref(mu)))
+        names = {v: k for k, v in self.SOLVE_METHODS.items()}
+        return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value, method=names.get(mu.value, "?"))
 
     def eliminate(self, nfree, b=None):
         """mat::eliminate(b) (mat.cpp:78-103): b_A - D b_D (b None: the assembled rhs)."""

@@ -131,6 +131,36 @@ std::shared_ptr<Problem> poisson2d(const std::map<std::string, std::string>& s) 
     return p;
 }
 
+// Steady convection-diffusion on the unit square (a non-symmetric system):
+//   integral(grad(dof(u)) . grad(tf(u)) + (b . grad(dof(u))) tf(u) - f tf(u)),
+// b = (bx, by) constant, u = 0 on the boundary; the geometry and numbering of
+// poisson2d.
+std::shared_ptr<Problem> convdiff2d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 64), order = geti(s, "order", 1);
+    double jit = getd(s, "jitter", 0.2), f = getd(s, "f", 1.0), bx = getd(s, "bx", 20.0), by = getd(s, "by", 10.0);
+    Rng rng((uint64_t)geti(s, "seed", 0));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    m.set_nodes(square_nodes(n, jit, rng));
+    square_boundary(m, n, 0, 2);
+    square_tris(m, n, 0, 1);
+    m.finalize();
+    field u(m, "h1", "u");
+    if (order != 1) u.setorder(1, order);
+    u.setconstraint(2);
+    p->fields = {u};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    const expression b(1, 2, {expression(bx), expression(by)});
+    *p->form += integral(1, grad(dof(u)) * grad(tf(u)) + (b * grad(dof(u))) * tf(u));
+    *p->form += integral(1, -expression(f) * tf(u));
+    std::ostringstream d;
+    d << "2D convection-diffusion P" << order << ", b = (" << bx << ", " << by << "), " << n << "x" << n << "x2 triangles";
+    p->describe = d.str();
+    return p;
+}
+
 // Config: nonlinear 2D magnetostatics-type problem with Newton re-assembly.
 // P2 field a on n x n x 2 jittered triangles of the unit square (a = 0 on the
 // boundary), reluctivity nu(|grad a|) = 1 + 500 / (1 + |grad a|^2), source f.
@@ -509,6 +539,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "electrostatics1d") p = electrostatics1d(s);
     else if (kind == "newton2d") p = newton2d(s);
     else if (kind == "hoprojection2d") p = highorderprojection2d(s);
+    else if (kind == "convdiff2d") p = convdiff2d(s);
     else fail("unknown problem kind '" + kind + "'");
     p->kind = kind;
     p->spec = s;

@@ -13,6 +13,7 @@
 
 #define SLX_SOLVE_BLOCKS 2048  // fixed grid of the partial-sum kernels
 #define SLX_STREAM_NNZ 4096    // nonzeros of one row block of the streaming FP32 product
+#define SLX_GMRES_MAX 100        // Krylov vectors per GMRES cycle at most
 #define SLX_XCDS 8              // XCDs of the device (SLX_SOLVE_BLOCKS is a multiple)
 
 namespace slx {
@@ -31,8 +32,10 @@ struct SolveSystem {
 
 struct SolveParams {
     double rel_tol;    // stop when ||b_A - D x_D - A x|| <= rel_tol ||b_A - D x_D||
-    double inner_tol;  // inner PCG: relative reduction of its own FP32 residual
+    double inner_tol;  // inner solve: relative reduction of its own FP32 residual
     int max_outer, max_inner;
+    int gmres;         // 0: Jacobi-PCG inner solve (SPD A); 1: Jacobi right-preconditioned GMRES
+    int restart;       // GMRES: Krylov vectors per refinement (<= SLX_GMRES_MAX)
 };
 
 struct SolveStats {
@@ -46,7 +49,16 @@ struct SolveStats {
 struct SolveWork {
     double *f, *r, *x, *xd, *xfull, *part, *sc;
     float *r32, *z, *p, *q, *d, *dinv, *val32;
+    // GMRES only: Krylov basis [restart + 1][nfree] floats, partial sums
+    // [restart + 1][SLX_SOLVE_BLOCKS] and restart + 1 coefficients (doubles)
+    float* V;
+    double *mpart, *hv;
 };
+
+// 1 when A differs from its transpose structurally, or numerically by more
+// than 1e-5 relative plus 1e-6 of the two diagonal entries in any entry;
+// flag: one int of device memory
+hipError_t check_symmetric(const SolveSystem& s, int* flag_dev, int* nonsym, hipStream_t st);
 
 // Row blocks of the streaming product: consecutive rows whose nonzeros fit
 // SLX_STREAM_NNZ (at most 256 rows); a longer row is a block of its own.

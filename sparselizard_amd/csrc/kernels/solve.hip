@@ -16,6 +16,8 @@
 // fixed order by one block, and fixed lanes per row for the products.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cmath>
+#include <vector>
 #include "solve.h"
 
 namespace slx {
@@ -274,12 +276,191 @@ __global__ void __launch_bounds__(256) pcg_direction(int64_t n, const double* __
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) p[i] = z[i] + beta * p[i];
 }
 
+__global__ void to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 __global__ void add_f32_to_f64(int64_t n, const float* __restrict__ d, double* __restrict__ x) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += (double)d[i];
 }
 
 static unsigned blocks(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// ---- GMRES (FP32 vectors, FP64 sums) ----
+
+// z = dinv .* v
+__global__ void precond_kernel(int64_t n, const float* __restrict__ dinv, const float* __restrict__ v, float* __restrict__ z) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) z[i] = dinv[i] * v[i];
+}
+
+// dst = alpha * src
+__global__ void scale_kernel(int64_t n, float alpha, const float* __restrict__ src, float* __restrict__ dst) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = alpha * src[i];
+}
+
+// partials of V_i . w for i < nv (basis rows of length n)
+__global__ void __launch_bounds__(256) multi_dot_partial(int64_t n, const float* __restrict__ V, int nv, const float* __restrict__ w,
+                                                         double* __restrict__ part) {
+    __shared__ double sh[4];
+    for (int i = 0; i < nv; i++) {
+        const float* v = V + (size_t)i * n;
+        double s = 0.0;
+        for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) s += (double)v[k] * (double)w[k];
+        const double t = block_sum(s, sh);
+        if (threadIdx.x == 0) part[(size_t)i * SLX_SOLVE_BLOCKS + blockIdx.x] = t;
+    }
+}
+
+// out[i] = ordered sum of partial row i (one block per row)
+__global__ void __launch_bounds__(256) sum_rows(const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < SLX_SOLVE_BLOCKS; i += 256) s += part[(size_t)blockIdx.x * SLX_SOLVE_BLOCKS + i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// w -= sum_i h_i V_i
+__global__ void basis_update(int64_t n, const float* __restrict__ V, int nv, const double* __restrict__ h, float* __restrict__ w) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float acc = w[k];
+    for (int i = 0; i < nv; i++) acc -= (float)h[i] * V[(size_t)i * n + k];
+    w[k] = acc;
+}
+
+// d = dinv .* sum_i y_i V_i   (right preconditioning)
+__global__ void basis_combine(int64_t n, const float* __restrict__ V, int nv, const double* __restrict__ y, const float* __restrict__ dinv,
+                              float* __restrict__ d) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    float acc = 0.f;
+    for (int i = 0; i < nv; i++) acc += (float)y[i] * V[(size_t)i * n + k];
+    d[k] = dinv[k] * acc;
+}
+
+// position of column c in CSR row r (sorted columns), or -1
+__device__ inline int64_t find_col(const int64_t* __restrict__ rowp, const int32_t* __restrict__ col, int64_t r, int64_t c) {
+    int64_t lo = rowp[r], hi = rowp[r + 1];
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return (lo < rowp[r + 1] && col[lo] == c) ? lo : -1;
+}
+
+// any entry a_rc whose transpose partner a_cr is missing or differs by more
+// than 1e-5 (|a_rc| + |a_cr|) + 1e-6 (|a_rr| + |a_cc|) sets *flag (every
+// writer stores 1); the diagonal term keeps rounding noise (FP64, or the
+// 1e-7 of the mixed-precision element matrices) on entries that cancel to
+// almost zero from counting as asymmetry
+__global__ void sym_kernel(int64_t n, const int64_t* __restrict__ rowp, const int32_t* __restrict__ col, const double* __restrict__ val,
+                           int* __restrict__ flag) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int64_t kd = find_col(rowp, col, r, r);
+    const double drr = kd >= 0 ? fabs(val[kd]) : 0.0;
+    bool bad = false;
+    for (int64_t k = rowp[r]; k < rowp[r + 1] && !bad; k++) {
+        const int32_t c = col[k];
+        const int64_t kt = find_col(rowp, col, c, r);
+        if (kt < 0) {
+            bad = true;
+        } else {
+            const int64_t kc = find_col(rowp, col, c, c);
+            const double dcc = kc >= 0 ? fabs(val[kc]) : 0.0;
+            bad = fabs(val[k] - val[kt]) > 1e-5 * (fabs(val[k]) + fabs(val[kt])) + 1e-6 * (drr + dcc);
+        }
+    }
+    if (bad) flag[0] = 1;
+}
+
+hipError_t check_symmetric(const SolveSystem& s, int* flag_dev, int* nonsym, hipStream_t st) {
+    HIPERR(hipMemsetAsync(flag_dev, 0, sizeof(int), st));
+    if (s.nfree > 0) hipLaunchKernelGGL(sym_kernel, dim3(blocks(s.nfree)), dim3(256), 0, st, s.nfree, s.Arow, s.Acol, s.Aval, flag_dev);
+    HIPERR(hipMemcpyAsync(nonsym, flag_dev, sizeof(int), hipMemcpyDeviceToHost, st));
+    return hipStreamSynchronize(st);
+}
+
+static hipError_t read_scalar(const double* dsc, int idx, double* h, hipStream_t st);
+
+// One cycle of right-preconditioned GMRES on A d = r32 (FP32 vectors, FP64
+// sums, classical Gram-Schmidt applied twice), at most min(restart,
+// max_inner) iterations or until the FP32 residual estimate falls below
+// inner_tol of its start.  d = 0 when r32 = 0.
+static hipError_t gmres_cycle(const SolveSystem& s, const SolveParams& prm, SolveWork& w, int& iters, hipStream_t st) {
+    const int64_t n = s.nfree;
+    const unsigned G = SLX_SOLVE_BLOCKS;
+    const int m = std::max(1, std::min(prm.restart, prm.max_inner));
+    double beta2 = 0.0;
+    hipLaunchKernelGGL((dot_partial<float>), dim3(G), dim3(256), 0, st, n, w.r32, w.r32, w.part);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 1, (int)SC_DOT, 0, 0, w.sc);
+    HIPERR(read_scalar(w.sc, SC_DOT, &beta2, st));
+    const double beta = sqrt(beta2);
+    if (!(beta > 0.0)) {
+        HIPERR(hipMemsetAsync(w.d, 0, n * sizeof(float), st));
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(scale_kernel, dim3(blocks(n)), dim3(256), 0, st, n, (float)(1.0 / beta), w.r32, w.V);
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), g(m + 1, 0.0), hc(m + 1);
+    g[0] = beta;
+    int k = 0;
+    for (int j = 0; j < m; j++) {
+        const float* vj = w.V + (size_t)j * n;
+        hipLaunchKernelGGL(precond_kernel, dim3(blocks(n)), dim3(256), 0, st, n, w.dinv, vj, w.z);
+        hipLaunchKernelGGL(spmv_stream_f32, dim3(G), dim3(256), 0, st, s.nblk, s.Ablk, s.Arow, s.Acol, w.val32, s.nnzA, w.z, w.q,
+                           w.part);
+        // two passes of classical Gram-Schmidt against v_0 .. v_j
+        for (int pass = 0; pass < 2; pass++) {
+            hipLaunchKernelGGL(multi_dot_partial, dim3(G), dim3(256), 0, st, n, w.V, j + 1, w.q, w.mpart);
+            hipLaunchKernelGGL(sum_rows, dim3(j + 1), dim3(256), 0, st, w.mpart, w.hv);
+            hipLaunchKernelGGL(basis_update, dim3(blocks(n)), dim3(256), 0, st, n, w.V, j + 1, w.hv, w.q);
+            HIPERR(hipMemcpyAsync(hc.data(), w.hv, (size_t)(j + 1) * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPERR(hipStreamSynchronize(st));
+            for (int i = 0; i <= j; i++) H[(size_t)i * m + j] += hc[i];
+        }
+        double hn2 = 0.0;
+        hipLaunchKernelGGL((dot_partial<float>), dim3(G), dim3(256), 0, st, n, w.q, w.q, w.part);
+        hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 1, (int)SC_DOT, 0, 0, w.sc);
+        HIPERR(read_scalar(w.sc, SC_DOT, &hn2, st));
+        const double hn = sqrt(hn2);
+        H[(size_t)(j + 1) * m + j] = hn;
+        if (hn > 0.0 && j + 1 <= m)
+            hipLaunchKernelGGL(scale_kernel, dim3(blocks(n)), dim3(256), 0, st, n, (float)(1.0 / hn), w.q, w.V + (size_t)(j + 1) * n);
+        // Givens rotations of column j
+        for (int i = 0; i < j; i++) {
+            const double a = H[(size_t)i * m + j], b = H[(size_t)(i + 1) * m + j];
+            H[(size_t)i * m + j] = cs[i] * a + sn[i] * b;
+            H[(size_t)(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+        }
+        const double a = H[(size_t)j * m + j], b = H[(size_t)(j + 1) * m + j];
+        const double rr = std::hypot(a, b);
+        cs[j] = rr > 0.0 ? a / rr : 1.0;
+        sn[j] = rr > 0.0 ? b / rr : 0.0;
+        H[(size_t)j * m + j] = rr;
+        H[(size_t)(j + 1) * m + j] = 0.0;
+        g[j + 1] = -sn[j] * g[j];
+        g[j] = cs[j] * g[j];
+        k = j + 1;
+        iters++;
+        if (fabs(g[j + 1]) <= prm.inner_tol * beta || !(hn > 0.0)) break;
+    }
+    // y = H(0:k, 0:k)^-1 g(0:k)
+    std::vector<double> y(k, 0.0);
+    for (int i = k - 1; i >= 0; i--) {
+        double t = g[i];
+        for (int c = i + 1; c < k; c++) t -= H[(size_t)i * m + c] * y[c];
+        y[i] = H[(size_t)i * m + i] != 0.0 ? t / H[(size_t)i * m + i] : 0.0;
+    }
+    HIPERR(hipMemcpyAsync(w.hv, y.data(), (size_t)k * sizeof(double), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(basis_combine, dim3(blocks(n)), dim3(256), 0, st, n, w.V, k, w.hv, w.dinv, w.d);
+    return hipStreamSynchronize(st);
+}
 
 static hipError_t read_scalar(const double* dsc, int idx, double* h, hipStream_t st) {
     HIPERR(hipMemcpyAsync(h, dsc + idx, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -342,6 +523,10 @@ hipError_t solve_system(const SolveSystem& s, const SolveParams& prm, SolveWork&
     for (int it = 0; it < prm.max_outer; it++) {
         if (rnorm <= prm.rel_tol * fnorm) break;
         out.outer++;
+        if (prm.gmres) {
+            hipLaunchKernelGGL(to_f32, dim3(blocks(n)), dim3(256), 0, st, n, w.r, w.r32);
+            HIPERR(gmres_cycle(s, prm, w, out.inner, st));
+        } else {
         // inner FP32 PCG on A d = r
         hipLaunchKernelGGL(pcg_start, dim3(G), dim3(256), 0, st, n, w.r, w.dinv, w.r32, w.z, w.p, w.d, w.part);
         hipLaunchKernelGGL(sum_partials, dim3(1), dim3(256), 0, st, w.part, 2, (int)SC_RZ, (int)SC_RR, 0, w.sc);
@@ -359,6 +544,7 @@ hipError_t solve_system(const SolveSystem& s, const SolveParams& prm, SolveWork&
                 if (sqrt(h) <= prm.inner_tol * r0) break;
             }
             hipLaunchKernelGGL(pcg_direction, dim3(G), dim3(256), 0, st, n, w.sc, w.z, w.p);
+        }
         }
         hipLaunchKernelGGL(add_f32_to_f64, dim3(blocks(n)), dim3(256), 0, st, n, w.d, w.x);
         // FP64 residual r = f - A x

@@ -1403,18 +1403,31 @@ slx_status slx_plan_matvec(slx_plan p, const double* x, double* y) {
     });
 }
 
-slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out, double* d_x_out,
-                          int32_t* outer_its, int32_t* inner_its, double* rel_residual) {
+slx_status slx_plan_solve(slx_plan p, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
+                          double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual, int32_t* method_used) {
     return guarded([&] {
         check_assembled(p);
+        SLX_CHECK(method == SLX_SOLVE_AUTO || method == SLX_SOLVE_CG || method == SLX_SOLVE_GMRES, "solve: unknown method " << method);
         SLX_CHECK(rel_tol > 0 && max_outer > 0 && max_inner > 0, "solve: rel_tol, max_outer and max_inner must be positive");
         SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
         HIPCHECK(hipSetDevice(p->ctx->device));
         const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
+        const SolveSystem sys = solve_view(p);
+        if (method == SLX_SOLVE_AUTO) {
+            DBuf flag;
+            flag.alloc(sizeof(int));
+            int nonsym = 0;
+            HIPCHECK(check_symmetric(sys, flag.as<int>(), &nonsym, p->stream));
+            method = nonsym ? SLX_SOLVE_GMRES : SLX_SOLVE_CG;
+        }
+        const bool gm = method == SLX_SOLVE_GMRES;
+        const int restart = std::min<int>(max_inner, SLX_GMRES_MAX);
         // one workspace block: doubles first, then floats (8-byte aligned)
-        const int64_t nD = 3 * nf + nc + nd + 2 * SLX_SOLVE_BLOCKS + 8, nF = 6 * nf + p->nnzA + 4;
+        const int64_t nDg = gm ? (int64_t)(restart + 1) * (SLX_SOLVE_BLOCKS + 1) : 0;
+        const int64_t nD = 3 * nf + nc + nd + 2 * SLX_SOLVE_BLOCKS + 8 + nDg, nF = 6 * nf + p->nnzA + 4;
+        const int64_t nV = gm ? (int64_t)(restart + 1) * nf : 0;
         DBuf ws;
-        ws.alloc((size_t)(nD * sizeof(double) + nF * sizeof(float) + 64));
+        ws.alloc((size_t)(nD * sizeof(double) + (nF + nV) * sizeof(float) + 64));
         SolveWork w;
         double* dp = ws.as<double>();
         w.f = dp; dp += nf;
@@ -1424,6 +1437,8 @@ slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t
         w.xfull = dp; dp += nd;
         w.part = dp; dp += 2 * SLX_SOLVE_BLOCKS;
         w.sc = dp; dp += 8;
+        w.mpart = gm ? dp : nullptr; dp += gm ? (int64_t)(restart + 1) * SLX_SOLVE_BLOCKS : 0;
+        w.hv = gm ? dp : nullptr; dp += gm ? restart + 1 : 0;
         float* fp = (float*)dp;
         w.r32 = fp; fp += nf;
         w.z = fp; fp += nf;
@@ -1432,8 +1447,8 @@ slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t
         w.d = fp; fp += nf;
         w.dinv = fp; fp += nf;
         w.val32 = (float*)(((uintptr_t)fp + 15) & ~(uintptr_t)15);
-        const SolveSystem sys = solve_view(p);
-        SolveParams prm{rel_tol, 1e-3, max_outer, max_inner};
+        w.V = gm ? w.val32 + p->nnzA + 4 : nullptr;
+        SolveParams prm{rel_tol, 1e-3, max_outer, max_inner, gm ? 1 : 0, restart};
         SolveStats stats;
         HIPCHECK(solve_system(sys, prm, w, stats, p->stream));
         if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
@@ -1442,6 +1457,7 @@ slx_status slx_plan_solve(slx_plan p, double rel_tol, int32_t max_outer, int32_t
         if (outer_its) *outer_its = stats.outer;
         if (inner_its) *inner_its = stats.inner;
         if (rel_residual) *rel_residual = stats.rel_residual;
+        if (method_used) *method_used = method;
     });
 }
 

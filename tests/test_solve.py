@@ -49,6 +49,7 @@ def test_solve_matches_direct(ctx, kind, spec, precision):
     print("solve", it, flush=True)
     assert it["rel_residual"] <= 1e-12, it
     assert it["outer"] >= 1 and it["inner"] >= 1
+    assert it["method"] == "cg"  # symmetric systems (the symmetry check tolerates rounding)
     # residual bound 1e-12 times the condition number of these small systems
     assert relerr(x, ref) <= 1e-8, relerr(x, ref)
     # constrained entries are the Dirichlet values, exactly
@@ -67,6 +68,7 @@ def test_solve_projection_no_constraints(ctx):
     plan = p.generate("fp64")
     v, it = plan.solve(info["ndofs"], rel_tol=1e-13, max_outer=30)
     print("solve", it, flush=True)
+    assert it["method"] == "cg"
     assert it["rel_residual"] <= 1e-13, it
     b = plan.csr(info["ndofs"]).rhs
     f2 = (0.5 - math.sin(20.0) / 40.0) * (0.5 - math.sin(26.0) / 52.0)
@@ -165,3 +167,35 @@ def test_matvec_and_eliminate(ctx, kind, spec):
     assert relerr(e, x[free] - D @ x[cons]) <= 1e-13
     e0 = plan.eliminate(nf)
     assert relerr(e0, csr.rhs[free] - D @ csr.rhs[cons]) <= 1e-13
+
+
+@pytest.mark.parametrize("spec,precision", [("n=24", "fp64"), ("n=16;order=2", "fp64"), ("n=24", "f16split"),
+                                            ("n=24;bx=200;by=-80", "fp64")])
+def test_gmres_convection_diffusion_matches_direct(ctx, spec, precision):
+    # non-symmetric A: the automatic choice is GMRES; the result matches a
+    # direct solve of the same assembled system
+    print("case solve convdiff2d", spec, precision, flush=True)
+    p = sa.Problem(ctx, "convdiff2d", spec)
+    info = register_free(p)
+    plan = p.generate(precision)
+    ref, A, bA = solve_full(plan.csr(info["ndofs"]), info["ndofs"], info["nfree"])
+    assert abs(A - A.T).max() > 1e-3 * abs(A).max()
+    x, it = plan.solve(info["ndofs"], rel_tol=1e-12, max_outer=40, max_inner=100)
+    print("solve", it, flush=True)
+    assert it["method"] == "gmres"
+    assert it["rel_residual"] <= 1e-12, it
+    assert relerr(x, ref) <= 1e-8, relerr(x, ref)
+
+
+def test_solve_method_choice(ctx):
+    p = sa.Problem(ctx, "poisson2d", "n=12")
+    info = register_free(p)
+    plan = p.generate("fp64")
+    x_cg, i_cg = plan.solve(info["ndofs"], rel_tol=1e-12)
+    assert i_cg["method"] == "cg"
+    # GMRES on the symmetric system gives the same solution
+    x_gm, i_gm = plan.solve(info["ndofs"], rel_tol=1e-12, method="gmres", max_outer=40, max_inner=60)
+    assert i_gm["method"] == "gmres" and i_gm["rel_residual"] <= 1e-12
+    assert relerr(x_gm, x_cg) <= 1e-9
+    with pytest.raises(KeyError):
+        plan.solve(info["ndofs"], method="lu")
