@@ -217,7 +217,7 @@ def main():
         # per inner iteration: FP32 values + column indices once, p gathered
         # (at least once), q, d, r, z, p, diag streamed
         it_bytes = shape["nnzA"] * 8 + shape["nfree"] * 4 * 11
-        solve = {"s": t_solve, "outer": it["outer"], "inner": it["inner"], "rel_residual": it["rel_residual"],
+        solve = {"s": t_solve, "method": it["method"], "outer": it["outer"], "inner": it["inner"], "rel_residual": it["rel_residual"],
                  "ms_per_inner": t_solve * 1e3 / max(it["inner"], 1), "bytes_per_inner": it_bytes,
                  "GBps_per_inner": it_bytes / (t_solve / max(it["inner"], 1)) / 1e9, "x_norm": float(np.linalg.norm(x))}
         log("solve: %s" % solve)
