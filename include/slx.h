@@ -165,7 +165,9 @@ const char* slx_last_error(void);
 slx_status slx_create(int device, slx_ctx* out);
 slx_status slx_destroy(slx_ctx ctx);
 
-/* Symbolic phase: copies the description, builds the element-row lists, the
+/* Symbolic phase: reads the description (which must stay valid for the life
+ * of the plan: the numeric phase reads its node count and programs), builds
+ * the element-row lists, the
  * CSR pattern (per-row sorted unique columns = sorted COO keys), the A/D
  * split with local renumbering and the reduction maps.  Pattern equals
  * rawmat::process (rawmat.cpp:127-298) on the same address tables.
@@ -243,6 +245,16 @@ slx_status slx_plan_eliminate(slx_plan plan, const double* b, double* out);
  * device, and the constrained entries of y are those of x (mat::xbmerge).
  * x, y: n_dofs entries in DOF numbering, host memory. */
 slx_status slx_plan_matvec(slx_plan plan, const double* x, double* y);
+/* Coefficient programs: 0 (auto, the default) compiles the program of every
+ * bucket of at least 4096 elements at plan creation into its own kernel
+ * (hipRTC; values in registers instead of the interpreter's scratch register
+ * file; the same arithmetic, results equal to rounding), 1 uses the
+ * interpreter everywhere, 2
+ * compiles every bucket's program.  Returns the number of compiled
+ * (bucket, precision) kernels in use; mode 2 fails if any compilation fails.
+ * No reference counterpart (the reference evaluates expression trees on the
+ * host, operation::interpolate). */
+slx_status slx_plan_coef_mode(slx_plan plan, int32_t mode, int32_t* n_compiled);
 slx_status slx_plan_destroy(slx_plan plan);
 /* Element matrices of the last assembly (n_ke entries, as FP64) in the
  * layout given by slx_plan_reduction_maps: for the element-matrix error of

@@ -104,6 +104,7 @@ def lib():
         "slx_plan_kernels": (i32, [vp, p32, p32]),
         "slx_plan_solve": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
         "slx_plan_eliminate": (i32, [vp, pd, pd]),
+        "slx_plan_coef_mode": (i32, [vp, i32, p32]),
         "slx_plan_matvec": (i32, [vp, pd, pd]),
         "slx_selftest_spmv_stream": (i32, [vp, i64, p64, p32, pd, pd, pd, pd]),
         "slx_plan_element_matrices": (i32, [vp, pd, i64]),
@@ -136,7 +137,7 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_spmv_stream",
@@ -420,6 +421,15 @@ class Plan:
                                    C.byref(o), C.byref(i), C.byref(r), C.byref(mu)))
         names = {v: k for k, v in self.SOLVE_METHODS.items()}
         return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value, method=names.get(mu.value, "?"))
+
+    COEF_MODES = {"auto": 0, "interpreter": 1, "compiled": 2}
+
+    def coef_mode(self, mode):
+        """Coefficient programs: "auto" (compiled for buckets of >= 4096 elements),
+        "interpreter" or "compiled"; returns the compiled kernels in use."""
+        n = C.c_int32()
+        check(lib().slx_plan_coef_mode(self.h, self.COEF_MODES[mode], C.byref(n)))
+        return n.value
 
     def eliminate(self, nfree, b=None):
         """mat::eliminate(b) (mat.cpp:78-103): b_A - D b_D (b None: the assembled rhs)."""

@@ -19,6 +19,7 @@
 //   (rawmat.cpp:67-125, 238-289).
 #include <hip/hip_runtime.h>
 #include "device.h"
+#include "coef_device.h"
 
 namespace slx {
 
@@ -28,124 +29,6 @@ typedef float float2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-
-// Jacobian, inverse and determinant of one (element, Gauss point).  Every
-// index below is a compile-time constant (loops fully unrolled, dimensions
-// guarded), so the values stay in registers: the program reads them through
-// a constant-index switch instead of indexing a private array.
-struct Geo {
-    double J[9], inv[9], det;
-};
-
-__device__ inline double pick9(const double* v, int i) {
-    switch (i) {
-        case 0: return v[0];
-        case 1: return v[1];
-        case 2: return v[2];
-        case 3: return v[3];
-        case 4: return v[4];
-        case 5: return v[5];
-        case 6: return v[6];
-        case 7: return v[7];
-        case 8: return v[8];
-    }
-    return 0.0;
-}
-
-// J[r][c] = sum_k x_k^c dN_k/dki_r (rawfield.cpp:2442-2502); det and inverse
-// as jacobian.cpp:308-349, 396-505.
-__device__ inline void geometry(const DevBucket& b, const double* xyz, int e, int q, Geo& g) {
-    double x[SLX_MAX_GEO][3];
-#pragma unroll
-    for (int k = 0; k < SLX_MAX_GEO; k++) {
-        if (k < b.n_geo) {
-            const int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-            x[k][0] = xyz[3 * (size_t)node];
-            x[k][1] = xyz[3 * (size_t)node + 1];
-            x[k][2] = xyz[3 * (size_t)node + 2];
-        } else {
-            x[k][0] = x[k][1] = x[k][2] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        double dn[SLX_MAX_GEO];
-#pragma unroll
-        for (int k = 0; k < SLX_MAX_GEO; k++)
-            dn[k] = (r < b.elemdim && k < b.n_geo) ? b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q] : 0.0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double s = 0.0;
-#pragma unroll
-            for (int k = 0; k < SLX_MAX_GEO; k++) s += x[k][c] * dn[k];
-            g.J[3 * r + c] = (c < b.meshdim) ? s : 0.0;
-        }
-    }
-    const double* J = g.J;
-#pragma unroll
-    for (int i = 0; i < 9; i++) g.inv[i] = 0.0;
-    if (b.meshdim == 1) {
-        g.det = J[0];
-        g.inv[0] = 1.0 / J[0];
-    } else if (b.meshdim == 2) {
-        g.det = J[0] * J[4] - J[1] * J[3];
-        g.inv[0] = J[4] / g.det;
-        g.inv[1] = -J[1] / g.det;
-        g.inv[3] = -J[3] / g.det;
-        g.inv[4] = J[0] / g.det;
-    } else {
-        g.det = J[0] * (J[4] * J[8] - J[7] * J[5]) - J[1] * (J[3] * J[8] - J[6] * J[5]) + J[2] * (J[3] * J[7] - J[6] * J[4]);
-        const double d = g.det;
-        g.inv[0] = (J[4] * J[8] - J[5] * J[7]) / d;
-        g.inv[1] = (J[2] * J[7] - J[1] * J[8]) / d;
-        g.inv[2] = (J[1] * J[5] - J[2] * J[4]) / d;
-        g.inv[3] = (J[5] * J[6] - J[3] * J[8]) / d;
-        g.inv[4] = (J[0] * J[8] - J[2] * J[6]) / d;
-        g.inv[5] = (J[2] * J[3] - J[0] * J[5]) / d;
-        g.inv[6] = (J[3] * J[7] - J[4] * J[6]) / d;
-        g.inv[7] = (J[1] * J[6] - J[0] * J[7]) / d;
-        g.inv[8] = (J[0] * J[4] - J[1] * J[3]) / d;
-    }
-}
-
-// Value of field f (space derivative sd) at Gauss point q of element e
-// (opfield / rawfield::interpolate: coefficients x form functions, space
-// derivatives through the inverse Jacobian).
-__device__ inline double field_value(const DevBucket& b, const DevInputs& in, int f, int sd, int e, int q, const Geo& g) {
-    const DevField& fl = b.fields[f];
-    const double* data = in.field_data[f];
-    double v[4] = {0, 0, 0, 0};
-    const int nd = sd == 0 ? 1 : 1 + b.elemdim;
-    for (int k = 0; k < fl.nff; k++) {
-        const int a = fl.addr[(size_t)k * b.n_elem + e];
-        if (a < 0) continue;
-        const double c = data[a];
-#pragma unroll
-        for (int d = 0; d < 4; d++)
-            if (d < nd) v[d] += c * fl.tab[((size_t)d * fl.nff + k) * b.n_gp + q];
-    }
-    if (sd == 0) return v[0];
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-        if (i < b.elemdim) s += pick9(g.inv, 3 * (sd - 1) + i) * v[1 + i];
-    return s;
-}
-
-template <typename T>
-__device__ inline T apply_unary(int op, T x) {
-    switch (op) {
-        case SLX_OP_INV: return (T)1 / x;
-        case SLX_OP_ABS: return fabs(x);
-        case SLX_OP_EXP: return exp(x);
-        case SLX_OP_LOG: return log(x);
-        case SLX_OP_SIN: return sin(x);
-        case SLX_OP_COS: return cos(x);
-        case SLX_OP_TAN: return tan(x);
-        case SLX_OP_SQRT: return sqrt(x);
-    }
-    return x;
-}
 
 // Coefficient program, evaluated for V (element, Gauss point) pairs per
 // thread: every interpreted instruction (uniform across the wave) is applied

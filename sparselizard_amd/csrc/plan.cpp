@@ -18,6 +18,9 @@
 #include "kernels/mfma.h"
 #include "kernels/factor.h"
 #include "kernels/solve.h"
+#include "coefjit.h"
+#include <array>
+#include <mutex>
 #include <cmath>
 
 namespace slx {
@@ -160,6 +163,10 @@ struct slx_plan_s {
     // their coefficients in their own table
     std::vector<uint8_t> rhs_only;     // per bucket: every slice is a rhs slice
     bool side_ok = false;              // every rhs slice lives in a rhs-only bucket
+    // compiled coefficient programs per bucket ([0] FP32, [1] FP64; null: interpreter)
+    std::vector<std::array<hipFunction_t, 2>> coef_fn;
+    int coef_mode = 0;                 // 0 auto, 1 interpreter only, 2 compiled everywhere
+    std::string coef_jit_error;        // last compiler message (diagnostics)
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     DBuf d_ctab_side;
@@ -926,6 +933,8 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     }
 }
 
+void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d);
+
 void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.dbuckets.resize(d.n_buckets);
     P.mfma.assign(d.n_buckets, MfmaBucket());
@@ -1045,12 +1054,48 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));
     P.d_ctab_side.alloc(ctab_side_words * sizeof(double));
+    compile_coef_programs(P, d);
     P.side_ok = d.has_matrix && d.has_rhs && any_rhs_only && !rhs_in_mixed;
     if (P.side_ok) {
         HIPCHECK(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
         HIPCHECK(hipEventCreateWithFlags(&P.ev_fork, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&P.ev_join, hipEventDisableTiming));
     }
+}
+
+// Compiled coefficient programs for the buckets the mode selects (auto: buckets
+// of at least SLX_COEF_JIT_MIN elements); unique programs compiled in parallel.
+#define SLX_COEF_JIT_MIN 4096
+void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d) {
+    P.coef_fn.assign(d.n_buckets, {nullptr, nullptr});
+    if (P.coef_mode == 1) return;
+    std::vector<std::pair<int, int>> todo;  // (bucket, fp64)
+    for (int b = 0; b < d.n_buckets; b++) {
+        if (d.buckets[b].n_coefs == 0 || d.buckets[b].n_elem == 0) continue;
+        if (P.coef_mode == 0 && d.buckets[b].n_elem < SLX_COEF_JIT_MIN) continue;
+        todo.push_back({b, 0});
+        todo.push_back({b, 1});
+    }
+    std::mutex mu;
+    std::vector<std::thread> th;
+    const int device = P.ctx->device;
+    const size_t nth = std::min<size_t>(todo.size(), 8);
+    for (size_t t = 0; t < nth; t++)
+        th.emplace_back([&, t]() {
+            (void)hipSetDevice(device);
+            for (size_t i = t; i < todo.size(); i += nth) {
+                const slx_bucket_desc& bd = d.buckets[todo[i].first];
+                hipFunction_t fn = nullptr;
+                std::string err;
+                if (coef_compile(bd.prog, bd.n_instr, todo[i].second != 0, &fn, &err)) {
+                    P.coef_fn[todo[i].first][todo[i].second] = fn;
+                } else {
+                    std::lock_guard<std::mutex> lk(mu);
+                    P.coef_jit_error = err;
+                }
+            }
+        });
+    for (auto& x : th) x.join();
 }
 
 // Per-launch HIP events by kernel class (profiling only).
@@ -1121,7 +1166,9 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         const int qfast = (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) ? 1 : 0;
         P.last_qfast = qfast;
         if (tm) tm->begin(T_COEF);
-        HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
+        const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
+        if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, qfast, ctab, bst));
+        else HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
         if (tm) tm->end();
         if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
             if (tm) tm->begin(T_MFMA);
@@ -1574,6 +1621,29 @@ slx_status slx_plan_element_matrices(slx_plan p, double* out, int64_t cap) {
             std::vector<float> tmp(p->n_ke);
             HIPCHECK(hipMemcpy(tmp.data(), p->d_ke.p, (size_t)p->n_ke * sizeof(float), hipMemcpyDeviceToHost));
             for (int64_t i = 0; i < p->n_ke; i++) out[i] = tmp[i];
+        }
+    });
+}
+
+slx_status slx_plan_coef_mode(slx_plan p, int32_t mode, int32_t* n_compiled) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        SLX_CHECK(mode >= 0 && mode <= 2, "coefficient mode must be 0 (auto), 1 (interpreter) or 2 (compiled)");
+        SLX_CHECK(!p->host_only, "plan was created without a device context");
+        HIPCHECK(hipSetDevice(p->ctx->device));
+        if (mode != p->coef_mode) {
+            HIPCHECK(hipStreamSynchronize(p->stream));
+            p->coef_mode = mode;
+            compile_coef_programs(*p, *p->src);
+        }
+        int n = 0;
+        for (auto& f : p->coef_fn) n += (f[0] ? 1 : 0) + (f[1] ? 1 : 0);
+        if (n_compiled) *n_compiled = n;
+        if (mode == 2) {
+            int want = 0;
+            for (int b = 0; b < p->src->n_buckets; b++)
+                if (p->src->buckets[b].n_coefs > 0 && p->src->buckets[b].n_elem > 0) want += 2;
+            SLX_CHECK(n == want, "coefficient program compilation failed: " << p->coef_jit_error);
         }
     });
 }
