@@ -358,15 +358,18 @@ __device__ inline int64_t find_col(const int64_t* __restrict__ rowp, const int32
 // than 1e-5 (|a_rc| + |a_cr|) + 1e-6 (|a_rr| + |a_cc|) sets *flag (every
 // writer stores 1); the diagonal term keeps rounding noise (FP64, or the
 // 1e-7 of the mixed-precision element matrices) on entries that cancel to
-// almost zero from counting as asymmetry
-__global__ void sym_kernel(int64_t n, const int64_t* __restrict__ rowp, const int32_t* __restrict__ col, const double* __restrict__ val,
-                           int* __restrict__ flag) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// almost zero from counting as asymmetry.  SLX_SYM_LPR lanes share a row
+// (one entry each per step), so the binary searches of a row run side by side.
+#define SLX_SYM_LPR 16
+__global__ void __launch_bounds__(256) sym_kernel(int64_t n, const int64_t* __restrict__ rowp, const int32_t* __restrict__ col,
+                                                  const double* __restrict__ val, int* __restrict__ flag) {
+    const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / SLX_SYM_LPR;
+    const int l = threadIdx.x % SLX_SYM_LPR;
     if (r >= n) return;
     const int64_t kd = find_col(rowp, col, r, r);
     const double drr = kd >= 0 ? fabs(val[kd]) : 0.0;
     bool bad = false;
-    for (int64_t k = rowp[r]; k < rowp[r + 1] && !bad; k++) {
+    for (int64_t k = rowp[r] + l; k < rowp[r + 1] && !bad; k += SLX_SYM_LPR) {
         const int32_t c = col[k];
         const int64_t kt = find_col(rowp, col, c, r);
         if (kt < 0) {
@@ -382,7 +385,8 @@ __global__ void sym_kernel(int64_t n, const int64_t* __restrict__ rowp, const in
 
 hipError_t check_symmetric(const SolveSystem& s, int* flag_dev, int* nonsym, hipStream_t st) {
     HIPERR(hipMemsetAsync(flag_dev, 0, sizeof(int), st));
-    if (s.nfree > 0) hipLaunchKernelGGL(sym_kernel, dim3(blocks(s.nfree)), dim3(256), 0, st, s.nfree, s.Arow, s.Acol, s.Aval, flag_dev);
+    if (s.nfree > 0)
+        hipLaunchKernelGGL(sym_kernel, dim3(blocks(s.nfree * SLX_SYM_LPR)), dim3(256), 0, st, s.nfree, s.Arow, s.Acol, s.Aval, flag_dev);
     HIPERR(hipMemcpyAsync(nonsym, flag_dev, sizeof(int), hipMemcpyDeviceToHost, st));
     return hipStreamSynchronize(st);
 }
