@@ -42,8 +42,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--problem", default="elasticity3d",
-                    help="elasticity3d (the headline workload) or hppoisson3d (hp hexahedra, orders 1..5)")
-    ap.add_argument("--n", type=int, default=0, help="cells per cube edge (0: 64 for elasticity3d, 48 for hppoisson3d)")
+                    help="elasticity3d (the headline workload), hppoisson3d (hp hexahedra, orders 1..5), poisson2d, "
+                         "quadbatch (512 independent problems) or newton2d (Newton re-assembly with a field)")
+    ap.add_argument("--n", type=int, default=0,
+                    help="cells per edge (0: the workload's full size: 64 elasticity3d, 48 hppoisson3d, 200 poisson2d, "
+                         "708 newton2d; quadbatch takes its sizes from --extra)")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--extra", default="", help="extra problem options, e.g. ';pmin=5;pmax=5'")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run (timing sweeps)")
@@ -70,8 +73,14 @@ def main():
         return
     ctx = sa.Context(0)
     if args.n <= 0:
-        args.n = 64 if args.problem == "elasticity3d" else 48
-    spec = ("n=%d;order=%d" % (args.n, args.order) if args.problem == "elasticity3d" else "n=%d" % args.n) + args.extra
+        args.n = {"elasticity3d": 64, "hppoisson3d": 48, "poisson2d": 200, "newton2d": 708}.get(args.problem, 0)
+    if args.problem == "elasticity3d":
+        spec = "n=%d;order=%d" % (args.n, args.order)
+    elif args.problem == "quadbatch":
+        spec = ""
+    else:
+        spec = "n=%d" % args.n
+    spec += args.extra
     t0 = time.time()
     prob = sa.Problem(ctx, args.problem, spec)
     info = prob.info()
@@ -90,14 +99,18 @@ def main():
     # device-resident inputs
     import ctypes as C
     xyz_host = np.ctypeslib.as_array(nd.node_xyz, shape=(3 * prob.n_nodes(),)).copy()
-    par_host = np.ctypeslib.as_array(nd.params, shape=(nd.n_params * nd.param_len,)).copy()
+    par_host = (np.ctypeslib.as_array(nd.params, shape=(nd.n_params * nd.param_len,)).copy()
+                if nd.n_params > 0 and nd.param_len > 0 else np.zeros(1))
     d_xyz = torch.from_numpy(xyz_host).to(dev)
     d_par = torch.from_numpy(par_host).to(dev)
+    # field data read by the coefficient programs (Newton workloads), device-resident
+    d_fdata = [torch.from_numpy(f.copy()).to(dev) for f in prob.field_data()]
+    d_fields = (C.c_void_p * len(d_fdata))(*[f.data_ptr() for f in d_fdata]) if d_fdata else None
     stream = torch.cuda.current_stream()
     sptr = C.c_void_p(stream.cuda_stream)
 
     def step(prec):
-        plan.assemble_device(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None, prec, sptr)
+        plan.assemble_device(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, d_fields, prec, sptr)
 
     def timed(prec):
         for _ in range(args.warmup):
@@ -142,10 +155,10 @@ def main():
                      np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2)))
 
     # per-kernel timing: HIP events around every launch on the plan's stream
-    prof_mix = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, None,
+    prof_mix = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, d_fields,
                             args.mixed, iters=max(3, args.steps // 2))
     prof_64 = None if args.no_fp64 else plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len,
-                                                      None, "fp64", iters=max(3, args.steps // 2))
+                                                      d_fields, "fp64", iters=max(3, args.steps // 2))
     if args.problem == "elasticity3d":
         n_elem = count_tets(args.n)
         n = (args.order + 1) * (args.order + 2) * (args.order + 3) // 6
@@ -175,7 +188,7 @@ def main():
             sd = bd.slices[s_]
             if not sd.is_rhs:
                 con_bytes += bd.n_elem * sd.n_tf * sd.n_dof * 4.0
-    kname = ("mfma_f16split_kernel" if args.problem == "elasticity3d" else "factor_f16split_kernel") \
+    kname = ("factor_f16split_kernel" if args.problem == "hppoisson3d" else "mfma_f16split_kernel") \
         if prof_mix["mfma"] > 0 else "elem_cols"
     if t_contract >= prof_mix["reduce"]:
         # the roofline that bounds the kernel: the larger of the two ideal times
@@ -240,11 +253,7 @@ def main():
         "dtype": {"f16split": "f16x2-split/f32acc/f64reduce", "bf16split": "bf16x3-split/f32acc/f64reduce",
                   "fp32": "f32/f64reduce"}.get(args.mixed, args.mixed),
         "data": "synthetic",
-        "config": {"workload": ("3D linear elasticity h1xyz P%d, %d^3 x 6 jittered tets (%d), E lognormal(0,0.5) per "
-                                "element, nu=0.3, clamped x=0" % (args.order, args.n, n_elem))
-                   if args.problem == "elasticity3d" else
-                   ("3D hp-Poisson, %d^3 jittered hexahedra (%d), per-element order U{1..5}, c lognormal(0,1), "
-                    "Gauss order 2p+1" % (args.n, n_elem)),
+        "config": {"workload": workload_name(args, n_elem),
                    "n_elem": n_elem, "ndofs": nn, "nnzA": shape["nnzA"], "nnzD": shape["nnzD"],
                    "coo_entries": stats["n_coo"], "parallelism": "single GPU"},
         "fp64_gpu": {"ms_per_step": ms64, "nnz_per_s": nnz / (ms64 * 1e-3), "kernel_ms": prof_64},
@@ -258,6 +267,24 @@ def main():
     if solve is not None:
         out["solve"] = solve
     print(json.dumps(out))
+
+
+def workload_name(args, n_elem):
+    if args.problem == "elasticity3d":
+        return ("3D linear elasticity h1xyz P%d, %d^3 x 6 jittered tets (%d), E lognormal(0,0.5) per element, nu=0.3, "
+                "clamped x=0" % (args.order, args.n, n_elem))
+    if args.problem == "hppoisson3d":
+        return ("3D hp-Poisson, %d^3 jittered hexahedra (%d), per-element order U{1..5}, c lognormal(0,1), Gauss order 2p+1"
+                % (args.n, n_elem))
+    if args.problem == "poisson2d":
+        return "2D Poisson P1, %dx%dx2 jittered triangles (%d), f=1, u=0 on the boundary" % (args.n, args.n, n_elem)
+    if args.problem == "quadbatch":
+        return ("512 independent K + k^2 M problems, n x n quads of order 3, n in [16,160], k^2 in [1,50] (%d elements)%s"
+                % (n_elem, args.extra))
+    if args.problem == "newton2d":
+        return ("2D nonlinear magnetostatics Newton re-assembly, P2, %dx%dx2 jittered triangles (%d), "
+                "nu(|grad a|) = 1 + 500/(1+|grad a|^2)" % (args.n, args.n, n_elem))
+    return "%s %s" % (args.problem, args.extra)
 
 
 def copy_probe(torch, dev, mib):
@@ -296,14 +323,18 @@ def cpu_baseline(sa, args):
     """The CPU oracle (restated generate()+A()+b(), FP64, 1 thread) on a bounded
     sample of the same workload; nnz/s is reported for that sample."""
     from oracle import py_oracle
-    import numpy as np
 
-    elast = args.problem == "elasticity3d"
-    per_cell = 6 if elast else 1
+    # sample size knob, its start, cap and how the element count grows with it
+    knob = {"elasticity3d": ("n", 4, 20, 3), "hppoisson3d": ("n", 3, 20, 3), "poisson2d": ("n", 20, 1200, 2),
+            "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 64, 1)}[args.problem]
 
-    def run(n):
-        spec = "n=%d;order=%d" % (n, args.order) if elast else "n=%d" % n
-        p = sa.Problem(None, args.problem, spec)
+    def spec_of(v):
+        if args.problem == "elasticity3d":
+            return "n=%d;order=%d" % (v, args.order)
+        return "%s=%d" % (knob[0], v)
+
+    def run(v):
+        p = sa.Problem(None, args.problem, spec_of(v))
         info = p.info()
         nd = p.numeric()
         t0 = time.time()
@@ -311,17 +342,17 @@ def cpu_baseline(sa, args):
         dt = time.time() - t0
         return dt, len(o.Aval) + len(o.Dval), info["nelem"]
 
-    n = args.cpu_sample_n
-    if n <= 0:
-        dt, nnz, ne = run(4 if elast else 3)
-        per_elem = dt / ne
+    v = args.cpu_sample_n
+    if v <= 0:
+        dt, nnz, ne = run(knob[1])
         # aim at ~15 s of CPU work
-        n = max(3, int(round((15.0 / max(per_elem, 1e-9) / per_cell) ** (1.0 / 3))))
-        n = min(n, 20)
-    dt, nnz, ne = run(n)
-    what = ("elasticity P%d on %d^3 x 6 tets" % (args.order, n)) if elast else ("hp-Poisson on %d^3 hexahedra" % n)
+        grow = 15.0 / max(dt, 1e-9)
+        v = max(knob[1], int(round(knob[1] * grow ** (1.0 / knob[3]))))
+        v = min(v, knob[2])
+    dt, nnz, ne = run(v)
     return {"value": nnz / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
-            "sample": "oracle generate()+A()+b() FP64, 1 thread, %s (%d elements), %.1f s" % (what, ne, dt)}
+            "sample": "oracle generate()+A()+b() FP64, 1 thread, %s with %s (%d elements), %.1f s"
+                      % (args.problem, spec_of(v), ne, dt)}
 
 
 if __name__ == "__main__":
