@@ -113,7 +113,19 @@ int slot_bytes_for(int maxrow, bool sgn) {
 struct slx_ctx_s {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the context and every plan created on it hold one reference: the stream
+    // lives until the last of slx_destroy and slx_plan_destroy
+    std::atomic<int> refs{1};
 };
+
+namespace {
+void ctx_release(slx_ctx c) {
+    if (c && c->refs.fetch_sub(1) == 1) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        delete c;
+    }
+}
+}  // namespace
 
 struct RedClass {
     int64_t begin, end;  // positions in the row order
@@ -697,6 +709,17 @@ float f16_to_f32(uint16_t h) {
 
 // Builds the MFMA description of a bucket's matrix slices (all sharing one
 // Phi table) or leaves M = 0 when the bucket is not eligible.
+// Largest element matrix (n_tf * n_dof entries) of the batched MFMA kernel;
+// larger slices go to the factorized kernel.  SLX_TUNE_BATCHED_MAXM overrides
+// it (tuning knob).
+int batched_max_m() {
+    static const int v = [] {
+        const char* t = std::getenv("SLX_TUNE_BATCHED_MAXM");
+        return t ? std::atoi(t) : 256;
+    }();
+    return v;
+}
+
 void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
     MfmaBucket mb;
     std::memset(&mb, 0, sizeof(mb));
@@ -721,12 +744,22 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         }
     }
     if (ms.empty() || (int)ms.size() > SLX_MAX_SLICES) return;
+    if (n_tf * n_dof > batched_max_m()) return;  // the factorized kernel takes larger slices
     mb.M = n_tf * n_dof;
     mb.n_tf = n_tf;
     mb.n_dof = n_dof;
     mb.n_pairs = (int)pairs.size();
     if (mb.n_pairs > SLX_MFMA_MAXP) return;
     mb.nq = bd.q_invariant ? 1 : bd.n_gp;
+    // coefficients that vary over the Gauss points: the per-point columns make
+    // the batched kernel producer-bound; the FP32 VALU element kernel is faster
+    // there (measured: 2D P2 Newton 6.9 vs 2.1 ms, 512 order-3 quad problems
+    // 78 vs 33 ms) at the same 1e-7 error.  SLX_TUNE_MFMA_QVARYING=1 keeps them
+    // on MFMA (tuning knob; the tests use it to cover that path).
+    if (mb.nq != 1) {
+        const char* t = std::getenv("SLX_TUNE_MFMA_QVARYING");
+        if (!(t && std::atoi(t) != 0)) return;
+    }
     // compact columns (coefficients constant over the Gauss points): the
     // Gauss points of a pair are padded to a multiple of 8 in K
     mb.compact = mb.nq == 1 ? 1 : 0;
@@ -853,7 +886,7 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
-        if ((int64_t)sd.n_tf * sd.n_dof <= 256 || sd.n_terms > SLX_FK_MAXT) continue;
+        if ((int64_t)sd.n_tf * sd.n_dof <= batched_max_m() || sd.n_terms > SLX_FK_MAXT) continue;
         FactorSlice f;
         std::memset(&f, 0, sizeof(f));
         f.n_tf = sd.n_tf;
@@ -1240,11 +1273,7 @@ slx_status slx_create(int device, slx_ctx* out) {
 }
 
 slx_status slx_destroy(slx_ctx ctx) {
-    return guarded([&] {
-        if (!ctx) return;
-        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-        delete ctx;
-    });
+    return guarded([&] { ctx_release(ctx); });
 }
 
 slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out) {
@@ -1275,6 +1304,7 @@ slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out
         if (!P->host_only) upload_buckets(*P, *desc, L);
         build_symbolic(*P, *desc, L);
         if (!P->host_only) HIPCHECK(hipStreamSynchronize(P->stream));
+        if (ctx) ctx->refs.fetch_add(1);
         *out = P.release();
     });
 }
@@ -1657,7 +1687,9 @@ slx_status slx_plan_destroy(slx_plan p) {
         if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
         if (p->ev_join) (void)hipEventDestroy(p->ev_join);
         if (p->side) (void)hipStreamDestroy(p->side);
+        slx_ctx c = p->ctx;
         delete p;
+        ctx_release(c);
     });
 }
 

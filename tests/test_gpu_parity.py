@@ -77,9 +77,21 @@ def test_mfma_operand_layout(ctx):
 
 @pytest.mark.parametrize("kind,spec", [("poisson2d", "n=9"), ("poisson2d", "n=7;order=2"), ("elasticity3d", "n=3"),
                                        ("elasticity3d", "n=4;order=1"), ("quadbatch", "instances=3;nmin=2;nmax=5")])
-def test_f16split_mfma_matches_oracle(ctx, kind, spec):
+def test_f16split_mfma_matches_oracle(ctx, kind, spec, monkeypatch):
+    # quadbatch has coefficients that vary over the Gauss points: by default its
+    # buckets go to the FP32 VALU kernel; the knob keeps them on the batched
+    # MFMA kernel so its per-Gauss-point column path stays covered
+    monkeypatch.setenv("SLX_TUNE_MFMA_QVARYING", "1")
     prob, plan, got, errs = run_case(ctx, kind, spec, "f16split", TOL_SPLIT)
     assert plan.kernels()["mfma_buckets"] >= 1, "MFMA kernel not used"
+    print("errors", errs)
+
+
+def test_f16split_qvarying_buckets_on_valu(ctx):
+    # default kernel choice: Gauss-point-varying coefficients run on the FP32
+    # VALU element kernel, still within the split path's 1e-6 gate
+    prob, plan, got, errs = run_case(ctx, "quadbatch", "instances=3;nmin=2;nmax=5", "f16split", TOL_SPLIT)
+    assert plan.kernels()["mfma_buckets"] == 0
     print("errors", errs)
 
 
@@ -160,3 +172,18 @@ def test_coefficient_kernel_matches_oracle(ctx, kind, spec, precision, tol):
                 for c in range(bd.n_coefs)]
     print("per-slot coefficient errors", ["%.1e" % x for x in per_slot])
     assert max(per_slot) <= tol
+
+
+def test_plan_keeps_its_context_alive():
+    # slx_destroy on a context with live plans only drops its reference: the
+    # plans keep assembling on the context's stream until they are destroyed
+    print("case context lifetime", flush=True)
+    c = sa.Context(0)
+    prob = sa.Problem(c, "poisson2d", "n=6")
+    info = prob.info()
+    prob.generate("fp64")
+    first = prob.plan().csr(info["ndofs"])
+    c.close()
+    again = prob.generate("fp64").csr(info["ndofs"])
+    assert np.array_equal(first.Aval, again.Aval)
+    prob.close()

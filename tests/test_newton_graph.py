@@ -60,8 +60,11 @@ def test_oracle_newton_converges_quadratically():
 
 
 @pytest.mark.gpu
-def test_newton_steps_fp64_and_split(ctx):
+def test_newton_steps_fp64_and_split(ctx, monkeypatch):
     print("case newton2d per-step", flush=True)
+    # Gauss-point-varying coefficients: the knob keeps the split path on the
+    # batched MFMA kernel (the default runs them on the FP32 VALU kernel)
+    monkeypatch.setenv("SLX_TUNE_MFMA_QVARYING", "1")
     p = sa.Problem(ctx, "newton2d", SPEC)
     info = p.info()
     cons = mask_of(p)
