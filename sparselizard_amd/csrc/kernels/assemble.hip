@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
 //   h_r     = w_q sum_s D_rs dof_s[j][q]
 //   Ke[i][j] += sum_r tf_r[i][q] h_r        (TI rows per pass)
 // which is the reference's sum over terms and Gauss points regrouped.
-template <typename CT, int TI>
+template <typename CT, int TI, bool QV>
 __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc, int qfast,
                                                  CT* __restrict__ ke) {
     const int epb = blockDim.x / jc;
@@ -196,12 +196,40 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
 #pragma unroll
                 for (int ii = 0; ii < TI; ii++) acc[ii] = 0;
                 for (int q = 0; q < ngp; q++) {
-                    if (nq != 1) load_D(q);
+                    if (!QV && nq != 1) load_D(q);
                     const CT w = (CT)gw[q];
                     CT dv[4];
 #pragma unroll
                     for (int d = 0; d < 4; d++)
                         dv[d] = (!is_rhs && ((dmask >> d) & 1u)) ? (CT)dfq[((size_t)q * 4 + d) * dof_pad + jr] : (CT)(d == 0 ? 1 : 0);
+                    if (QV) {
+                        // Gauss-point-varying coefficients: h_r = sum over the terms with
+                        // tf derivative r of C_t(e, q) dof_{s_t}[j][q], term by term (no
+                        // per-point 4 x 4 D assembly)
+                        CT h[4] = {0, 0, 0, 0};
+                        for (int t = 0; t < n_terms; t++) {
+                            const int tr = terms[t].tf_deriv, td = terms[t].dof_deriv;
+                            const CT c = qfast ? ctab[((size_t)terms[t].coef * cstride + ec) * nq + q]
+                                               : ctab[((size_t)terms[t].coef * nq + q) * cstride + ec];
+                            CT dsel = dv[0];
+#pragma unroll
+                            for (int d = 1; d < 4; d++)
+                                if (d == td) dsel = dv[d];
+                            const CT x = is_rhs ? c : c * dsel;
+#pragma unroll
+                            for (int r = 0; r < 4; r++)
+                                if (r == tr) h[r] += x;
+                        }
+#pragma unroll
+                        for (int r = 0; r < 4; r++) {
+                            if (!((rmask >> r) & 1u)) continue;
+                            const CT hw = h[r] * w;
+                            const SLX_CONST double* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
+#pragma unroll
+                            for (int ii = 0; ii < TI; ii++) acc[ii] += (CT)tv[ii] * hw;
+                        }
+                        continue;
+                    }
 #pragma unroll
                     for (int r = 0; r < 4; r++) {
                         if (!((rmask >> r) & 1u)) continue;
@@ -519,7 +547,10 @@ template <typename CT, int TI>
 static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int jc, int qfast, void* ke, hipStream_t st) {
     const int epb = 256 / jc;
     const unsigned grid = (unsigned)((b.n_elem + epb - 1) / epb);
-    hipLaunchKernelGGL((elem_cols<CT, TI>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
+    if (b.q_inv)
+        hipLaunchKernelGGL((elem_cols<CT, TI, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
+    else
+        hipLaunchKernelGGL((elem_cols<CT, TI, true>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
 }
 
 template <typename CT>
