@@ -397,7 +397,7 @@ size_t mfma_lds_bytes(const MfmaBucket& mb) {
 bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw) {
     if (ksteps < 1 || ksteps > 8 || mtiles < 1) return false;
     int w = mtiles <= 8 ? mtiles : 8;
-    if (prefer_mtw == 2 && mtiles > 1) w = (mtiles + 1) / 2;
+    if (prefer_mtw == 2 && mtiles > 1) w = (mtiles + 1) / 2 < 8 ? (mtiles + 1) / 2 : 8;
     int t = (mtiles + w - 1) / w;
     if (t > 2) return false;
     *nw = w;

@@ -22,6 +22,8 @@
 #include <array>
 #include <mutex>
 #include <cmath>
+#include <cstdio>
+#include <dlfcn.h>
 
 namespace slx {
 hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st, int qfast);
@@ -709,15 +711,74 @@ float f16_to_f32(uint16_t h) {
 
 // Builds the MFMA description of a bucket's matrix slices (all sharing one
 // Phi table) or leaves M = 0 when the bucket is not eligible.
-// Largest element matrix (n_tf * n_dof entries) of the batched MFMA kernel;
-// larger slices go to the factorized kernel.  SLX_TUNE_BATCHED_MAXM overrides
-// it (tuning knob).
-int batched_max_m() {
-    static const int v = [] {
-        const char* t = std::getenv("SLX_TUNE_BATCHED_MAXM");
-        return t ? std::atoi(t) : 256;
+// Kernel choice per bucket class, from the table the autotuner writes
+// (sparselizard_amd/tune/kernel_table.txt, found next to the library's
+// directory, or the file SLX_TUNE_TABLE names; SLX_TUNE_TABLE=none disables
+// it).  A line "elem_type n_tf n_dof q_invariant kernel mtw" picks the kernel
+// of the matrix slices of buckets of that class: mfma (batched, mtw M tiles
+// per wave), valu (FP32 element kernel) or factor (factorized MFMA).
+// Classes the table does not list use the built-in rules below.
+enum { TK_NONE = 0, TK_MFMA = 1, TK_VALU = 2, TK_FACTOR = 3 };
+struct TuneEntry {
+    int type, n_tf, n_dof, q_inv, kernel, mtw;
+};
+
+// The table is read again when SLX_TUNE_TABLE changes (plan creation reads
+// it from one thread at a time).
+const std::vector<TuneEntry>& tune_table() {
+    static std::string loaded_key = "\x01unset";
+    static std::vector<TuneEntry> table;
+    const char* env = std::getenv("SLX_TUNE_TABLE");
+    const std::string key = env ? env : "";
+    if (key == loaded_key) return table;
+    loaded_key = key;
+    table = [&] {
+        std::vector<TuneEntry> t;
+        std::string path;
+        if (env) {
+            path = env;
+            if (path == "none") return t;
+        } else {
+            Dl_info info;
+            if (dladdr((void*)&tune_table, &info) && info.dli_fname) {
+                std::string lib = info.dli_fname;
+                const size_t k = lib.rfind('/');
+                path = (k == std::string::npos ? std::string(".") : lib.substr(0, k)) + "/../tune/kernel_table.txt";
+            }
+        }
+        FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+        if (!f) return t;
+        char line[512];
+        while (std::fgets(line, sizeof(line), f)) {
+            if (line[0] == '#') continue;
+            TuneEntry x{};
+            char kern[32] = {0};
+            if (std::sscanf(line, "%d %d %d %d %31s %d", &x.type, &x.n_tf, &x.n_dof, &x.q_inv, kern, &x.mtw) != 6) continue;
+            const std::string k = kern;
+            x.kernel = k == "mfma" ? TK_MFMA : k == "valu" ? TK_VALU : k == "factor" ? TK_FACTOR : TK_NONE;
+            if (x.kernel != TK_NONE) t.push_back(x);
+        }
+        std::fclose(f);
+        return t;
     }();
-    return v;
+    return table;
+}
+
+int tune_lookup(int type, int n_tf, int n_dof, int q_inv, int* mtw) {
+    for (const TuneEntry& x : tune_table())
+        if (x.type == type && x.n_tf == n_tf && x.n_dof == n_dof && x.q_inv == q_inv) {
+            if (mtw) *mtw = x.mtw;
+            return x.kernel;
+        }
+    return TK_NONE;
+}
+
+// Largest element matrix (n_tf * n_dof entries) of the batched MFMA kernel
+// without a table entry; larger slices go to the factorized kernel.
+// SLX_TUNE_BATCHED_MAXM overrides it (tuning knob).
+int batched_max_m() {
+    const char* t = std::getenv("SLX_TUNE_BATCHED_MAXM");
+    return t ? std::atoi(t) : 256;
 }
 
 void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
@@ -744,7 +805,10 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         }
     }
     if (ms.empty() || (int)ms.size() > SLX_MAX_SLICES) return;
-    if (n_tf * n_dof > batched_max_m()) return;  // the factorized kernel takes larger slices
+    int tuned_mtw = 1;
+    const int tuned = tune_lookup(bd.elem_type, n_tf, n_dof, bd.q_invariant ? 1 : 0, &tuned_mtw);
+    if (tuned == TK_VALU || tuned == TK_FACTOR) return;
+    if (tuned == TK_NONE && n_tf * n_dof > batched_max_m()) return;  // the factorized kernel takes larger slices
     mb.M = n_tf * n_dof;
     mb.n_tf = n_tf;
     mb.n_dof = n_dof;
@@ -756,7 +820,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     // there (measured: 2D P2 Newton 6.9 vs 2.1 ms, 512 order-3 quad problems
     // 78 vs 33 ms) at the same 1e-7 error.  SLX_TUNE_MFMA_QVARYING=1 keeps them
     // on MFMA (tuning knob; the tests use it to cover that path).
-    if (mb.nq != 1) {
+    if (mb.nq != 1 && tuned != TK_MFMA) {
         const char* t = std::getenv("SLX_TUNE_MFMA_QVARYING");
         if (!(t && std::atoi(t) != 0)) return;
     }
@@ -771,7 +835,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.sgroup = 1;
     if (mb.compact) mb.sgroup = mb.n_mslices % 3 == 0 ? 3 : (mb.n_mslices % 2 == 0 ? 2 : 1);
     // tile split: SLX_TUNE_MFMA_MTW=2 selects two M tiles per wave (tuning knob)
-    int prefer_mtw = 1;
+    int prefer_mtw = tuned == TK_MFMA ? tuned_mtw : 1;
     if (const char* t = std::getenv("SLX_TUNE_MFMA_MTW")) prefer_mtw = std::atoi(t);
     if (!mfma_variant_fits(mb.mtiles, mb.ksteps, &mb.nw, &mb.mtw, &mb.maxk, prefer_mtw)) return;
     if (mb.mtw != 1) mb.sgroup = 1;
@@ -886,7 +950,9 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
-        if ((int64_t)sd.n_tf * sd.n_dof <= batched_max_m() || sd.n_terms > SLX_FK_MAXT) continue;
+        const int tuned = tune_lookup(bd.elem_type, sd.n_tf, sd.n_dof, bd.q_invariant ? 1 : 0, nullptr);
+        if (tuned == TK_VALU || sd.n_terms > SLX_FK_MAXT) continue;
+        if (tuned != TK_FACTOR && (int64_t)sd.n_tf * sd.n_dof <= batched_max_m()) continue;
         FactorSlice f;
         std::memset(&f, 0, sizeof(f));
         f.n_tf = sd.n_tf;

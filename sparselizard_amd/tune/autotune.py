@@ -1,0 +1,151 @@
+"""Autotuner of the element-kernel choice per bucket class.
+
+For every representative workload below it times the split-precision
+assembly (HIP events per launch, slx_plan_profile) with each candidate
+kernel for the matrix slices of the workload's buckets -- the batched MFMA
+kernel with one or two M tiles per wave, the factorized MFMA kernel, or the
+FP32 VALU element kernel -- keeps the candidates whose global-matrix error
+against the FP64 assembly is at most 1e-6, and writes the fastest per bucket
+class to kernel_table.txt (next to this script), which the library reads at
+plan creation.  Every (workload, candidate) runs in a fresh process with
+SLX_TUNE_TABLE pointing at a one-candidate table.  Needs one GPU.
+
+usage: python3 sparselizard_amd/tune/autotune.py [--out FILE] [--iters N]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+# (kind, spec): mid-size versions of the benchmark workloads and a few classes around them
+WORKLOADS = [
+    ("elasticity3d", "n=20;order=2"),
+    ("elasticity3d", "n=24;order=1"),
+    ("poisson2d", "n=400"),
+    ("poisson2d", "n=200;order=2"),
+    ("quadbatch", "instances=48"),
+    ("newton2d", "n=300"),
+    ("hppoisson3d", "n=16"),
+]
+CANDIDATES = [("mfma", 1), ("mfma", 2), ("factor", 1), ("valu", 1)]
+
+
+def feasible(kern, c):
+    """Whether a kernel can take the matrix slices of class c = (type, n_tf, n_dof, q_inv):
+    the batched kernel holds at most 16 M tiles of 16 rows, the factorized one is
+    for larger element matrices, the VALU kernel takes everything."""
+    m = c[1] * c[2]
+    if kern == "mfma":
+        return m <= 256
+    if kern == "factor":
+        return m > 256
+    return True
+
+
+def classes_of(kind, spec):
+    sys.path.insert(0, ROOT)
+    import sparselizard_amd as sa
+    p = sa.Problem(None, kind, spec)
+    d = p.desc()
+    out = set()
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            if not sd.is_rhs:
+                out.add((bd.elem_type, sd.n_tf, sd.n_dof, 1 if bd.q_invariant else 0))
+    return sorted(out)
+
+
+def child(kind, spec, iters):
+    """Runs in the tuning subprocess: ms per split assembly and its error."""
+    sys.path.insert(0, ROOT)
+    import ctypes as C
+    import numpy as np
+    import torch
+    import sparselizard_amd as sa
+    torch.cuda.init()
+    ctx = sa.Context(0)
+    prob = sa.Problem(ctx, kind, spec)
+    info = prob.info()
+    nd = prob.numeric()
+    xyz = torch.from_numpy(np.ctypeslib.as_array(nd.node_xyz, shape=(3 * prob.n_nodes(),)).copy()).to("cuda:0")
+    par = torch.from_numpy(np.ctypeslib.as_array(nd.params, shape=(nd.n_params * nd.param_len,)).copy()
+                           if nd.n_params > 0 and nd.param_len > 0 else np.zeros(1)).to("cuda:0")
+    fdata = [torch.from_numpy(f.copy()).to("cuda:0") for f in prob.field_data()]
+    fptr = (C.c_void_p * len(fdata))(*[f.data_ptr() for f in fdata]) if fdata else None
+    plan = prob.generate("fp64")
+    ref = plan.csr(info["ndofs"])
+    prob.generate("f16split")
+    got = plan.csr(info["ndofs"])
+    num = np.sqrt(np.sum((got.Aval - ref.Aval) ** 2) + np.sum((got.Dval - ref.Dval) ** 2))
+    den = np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2))
+    ms = plan.profile(C.c_void_p(xyz.data_ptr()), C.c_void_p(par.data_ptr()), nd.param_len, fptr, "f16split", iters=iters)
+    print(json.dumps({"ms": sum(ms.values()), "kernels": ms, "err": float(num / max(den, 1e-300)),
+                      "kernels_used": plan.kernels()}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "kernel_table.txt"))
+    ap.add_argument("--iters", type=int, default=5)
+    ap.add_argument("--child", nargs=2)
+    args = ap.parse_args()
+    if args.child:
+        child(args.child[0], args.child[1], args.iters)
+        return
+    best = {}
+    log = []
+    for kind, spec in WORKLOADS:
+        cls = classes_of(kind, spec)
+        results = []
+        for kern, mtw in CANDIDATES:
+            if not any(feasible(kern, c) for c in cls):
+                continue
+            # classes the candidate cannot take keep the built-in choice
+            with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+                for c in cls:
+                    if feasible(kern, c):
+                        f.write("%d %d %d %d %s %d\n" % (c + (kern, mtw)))
+                tab = f.name
+            env = dict(os.environ, SLX_TUNE_TABLE=tab)
+            print("tuning %s %s: %s mtw %d" % (kind, spec, kern, mtw), flush=True)
+            r = subprocess.run([sys.executable, __file__, "--child", kind, spec, "--iters", str(args.iters)],
+                               env=env, capture_output=True, text=True, timeout=600)
+            os.unlink(tab)
+            if r.returncode != 0:
+                # a candidate the kernels reject (launch limits) is skipped
+                print("   failed: %s" % (r.stderr.strip().splitlines() or ["?"])[-1][:300], flush=True)
+                continue
+            res = json.loads(r.stdout.strip().splitlines()[-1])
+            print("   %.3f ms, error %.2e" % (res["ms"], res["err"]), flush=True)
+            results.append((kern, mtw, res))
+        ok = [x for x in results if x[2]["err"] <= 1e-6]
+        # the first candidate in CANDIDATES order unless another is at least 3 % faster
+        kern, mtw, res = ok[0]
+        for x in ok[1:]:
+            if x[2]["ms"] < 0.97 * res["ms"]:
+                kern, mtw, res = x
+        for c in cls:
+            if feasible(kern, c):
+                best[c] = (kern, mtw)
+        log.append("# %s %s: %s" % (kind, spec, ", ".join("%s/%d %.3f ms" % (k, m, x["ms"]) for k, m, x in results)))
+    with open(args.out, "w") as f:
+        f.write("# Kernel choice per bucket class, written by autotune.py on one MI355X (split precision,\n")
+        f.write("# HIP-event time of one assembly; candidates above 1e-6 global error excluded; a candidate\n")
+        f.write("# replaces an earlier one in the list only when at least 3 % faster).\n")
+        f.write("# columns: elem_type n_tf n_dof q_invariant kernel mtw\n")
+        for line in log:
+            f.write(line + "\n")
+        for c in sorted(best):
+            f.write("%d %d %d %d %s %d\n" % (c + best[c]))
+    print("wrote", args.out)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,7 @@ def test_newton_steps_fp64_and_split(ctx, monkeypatch):
     # Gauss-point-varying coefficients: the knob keeps the split path on the
     # batched MFMA kernel (the default runs them on the FP32 VALU kernel)
     monkeypatch.setenv("SLX_TUNE_MFMA_QVARYING", "1")
+    monkeypatch.setenv("SLX_TUNE_TABLE", "none")
     p = sa.Problem(ctx, "newton2d", SPEC)
     info = p.info()
     cons = mask_of(p)
