@@ -31,6 +31,8 @@ WORKLOADS = [
     ("quadbatch", "instances=48"),
     ("newton2d", "n=300"),
     ("hppoisson3d", "n=16"),
+    # isolates the order-1 hexahedra of the hp workload (a later workload's choice wins)
+    ("hppoisson3d", "n=40;pmin=1;pmax=1"),
 ]
 CANDIDATES = [("mfma", 1), ("mfma", 2), ("factor", 1), ("valu", 1)]
 
