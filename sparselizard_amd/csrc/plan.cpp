@@ -723,8 +723,9 @@ struct TuneEntry {
     int type, n_tf, n_dof, q_inv, kernel, mtw;
 };
 
-// The table is read again when SLX_TUNE_TABLE changes (plan creation reads
-// it from one thread at a time).
+// The table is read again when SLX_TUNE_TABLE changes; callers copy what they
+// need under the lock (tune_lookup).
+std::mutex g_tune_mu;
 const std::vector<TuneEntry>& tune_table() {
     static std::string loaded_key = "\x01unset";
     static std::vector<TuneEntry> table;
@@ -765,6 +766,7 @@ const std::vector<TuneEntry>& tune_table() {
 }
 
 int tune_lookup(int type, int n_tf, int n_dof, int q_inv, int* mtw) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const TuneEntry& x : tune_table())
         if (x.type == type && x.n_tf == n_tf && x.n_dof == n_dof && x.q_inv == q_inv) {
             if (mtw) *mtw = x.mtw;
