@@ -162,8 +162,9 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
         const int n_tf = dsl[s].n_tf, n_dof = dsl[s].n_dof, is_rhs = dsl[s].is_rhs, n_terms = dsl[s].n_terms;
         const int tf_pad = dsl[s].tf_pad, dof_pad = dsl[s].dof_pad;
         const SLX_CONST slx_term* terms = (const SLX_CONST slx_term*)dsl[s].terms;
-        const SLX_CONST double* tfq = (const SLX_CONST double*)dsl[s].tf_tabq;
-        const double* dfq = dsl[s].dof_tabq;
+        // shape tables in the kernel's precision (FP32 copies for CT = float)
+        const SLX_CONST CT* tfq = sizeof(CT) == 4 ? (const SLX_CONST CT*)dsl[s].tf_tabq32 : (const SLX_CONST CT*)dsl[s].tf_tabq;
+        const CT* dfq = sizeof(CT) == 4 ? (const CT*)dsl[s].dof_tabq32 : (const CT*)dsl[s].dof_tabq;
         const int64_t ke_base = dsl[s].ke_base, ke_ld = dsl[s].ke_ld;
         unsigned rmask = 0, dmask = 0;
         for (int t = 0; t < n_terms; t++) {
@@ -224,9 +225,9 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
                         for (int r = 0; r < 4; r++) {
                             if (!((rmask >> r) & 1u)) continue;
                             const CT hw = h[r] * w;
-                            const SLX_CONST double* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
+                            const SLX_CONST CT* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
 #pragma unroll
-                            for (int ii = 0; ii < TI; ii++) acc[ii] += (CT)tv[ii] * hw;
+                            for (int ii = 0; ii < TI; ii++) acc[ii] += tv[ii] * hw;
                         }
                         continue;
                     }
@@ -242,9 +243,9 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
                             for (int d = 0; d < 4; d++) h += D[r][d] * dv[d];
                         }
                         h *= w;
-                        const SLX_CONST double* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
+                        const SLX_CONST CT* tv = tfq + ((size_t)q * 4 + r) * tf_pad + i0;
 #pragma unroll
-                        for (int ii = 0; ii < TI; ii++) acc[ii] += (CT)tv[ii] * h;
+                        for (int ii = 0; ii < TI; ii++) acc[ii] += tv[ii] * h;
                     }
                 }
                 if (jon) {

@@ -15,6 +15,8 @@ struct DevSlice {
     const slx_term* terms;      // device
     const double* tf_tabq;      // [n_gp][4][tf_pad]  (Gauss point major, zero padded)
     const double* dof_tabq;     // [n_gp][4][dof_pad] or null for right-hand sides
+    const float* tf_tabq32;     // FP32 copies of the two tables (FP32 element kernels)
+    const float* dof_tabq32;
     int32_t tf_pad, dof_pad;
     int64_t ke_base;            // Ke entry (e, i, j) at ke_base + (e*n_tf + i)*ke_ld + j
     int64_t ke_ld;              // element-row segment length (1 for right-hand sides)

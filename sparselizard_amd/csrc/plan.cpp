@@ -1091,9 +1091,13 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         db.geo_tab = P.dev(geo.data(), geo.size());
         db.prog = P.dev(bd.prog, (size_t)bd.n_instr);
         std::vector<const double*> dtabs(bd.n_tables);
+        std::vector<const float*> dtabs32(bd.n_tables);
         for (int t = 0; t < bd.n_tables; t++) {
             std::vector<double> tq = table_qmajor(bd.tables[t], bd.table_nff[t], bd.n_gp);
             dtabs[t] = P.dev(tq.data(), tq.size());
+            // FP32 copy for the FP32 element kernels (rounded once here, not per use)
+            std::vector<float> tq32(tq.begin(), tq.end());
+            dtabs32[t] = P.dev(tq32.data(), tq32.size());
         }
         const size_t words = (size_t)std::max(bd.n_coefs, 1) * (bd.q_invariant ? 1 : bd.n_gp) * std::max(bd.n_elem, 1);
         if (P.rhs_only[b]) ctab_side_words = std::max(ctab_side_words, words);
@@ -1112,6 +1116,8 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
                       "slice dof table does not match n_dof");
             ds.tf_tabq = dtabs[sd.tf_table];
             ds.dof_tabq = sd.is_rhs ? nullptr : dtabs[sd.dof_table];
+            ds.tf_tabq32 = dtabs32[sd.tf_table];
+            ds.dof_tabq32 = sd.is_rhs ? nullptr : dtabs32[sd.dof_table];
             ds.tf_pad = table_pad(sd.n_tf);
             ds.dof_pad = sd.is_rhs ? 0 : table_pad(sd.n_dof);
             ds.ke_base = L.ke_base[b][s];
