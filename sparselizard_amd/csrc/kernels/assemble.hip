@@ -146,7 +146,12 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
 // which is the reference's sum over terms and Gauss points regrouped.
 template <typename CT, int TI, bool QV>
 __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc, int qfast,
-                                                 CT* __restrict__ ke) {
+                                                 int coop, CT* __restrict__ ke) {
+    // QV with coop: the block's coefficients of the current slice, staged once
+    // in LDS ([element][term][q]) and read from there by the element's lanes
+    // (dynamic shared memory sized by the launcher)
+    extern __shared__ __attribute__((aligned(16))) unsigned char elem_lds[];
+    CT* cl = reinterpret_cast<CT*>(elem_lds);
     const int epb = blockDim.x / jc;
     const int el = threadIdx.x / jc, jl = threadIdx.x - (threadIdx.x / jc) * jc;
     const int e = blockIdx.x * epb + el;
@@ -166,6 +171,18 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
         const SLX_CONST CT* tfq = sizeof(CT) == 4 ? (const SLX_CONST CT*)dsl[s].tf_tabq32 : (const SLX_CONST CT*)dsl[s].tf_tabq;
         const CT* dfq = sizeof(CT) == 4 ? (const CT*)dsl[s].dof_tabq32 : (const CT*)dsl[s].dof_tabq;
         const int64_t ke_base = dsl[s].ke_base, ke_ld = dsl[s].ke_ld;
+        if (QV && coop) {
+            const int per = n_terms * nq;
+            __syncthreads();  // the previous slice's reads are done
+            for (int x = threadIdx.x; x < epb * per; x += blockDim.x) {
+                const int ee = x / per, rem = x - ee * per, t = rem / nq, qq = rem - t * nq;
+                const int eg = blockIdx.x * epb + ee;
+                const int eu = eg < b.n_elem ? eg : 0;
+                const int cs = terms[t].coef;
+                cl[x] = qfast ? ctab[((size_t)cs * cstride + eu) * nq + qq] : ctab[((size_t)cs * nq + qq) * cstride + eu];
+            }
+            __syncthreads();
+        }
         unsigned rmask = 0, dmask = 0;
         for (int t = 0; t < n_terms; t++) {
             rmask |= 1u << terms[t].tf_deriv;
@@ -210,8 +227,9 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
                         CT h[4] = {0, 0, 0, 0};
                         for (int t = 0; t < n_terms; t++) {
                             const int tr = terms[t].tf_deriv, td = terms[t].dof_deriv;
-                            const CT c = qfast ? ctab[((size_t)terms[t].coef * cstride + ec) * nq + q]
-                                               : ctab[((size_t)terms[t].coef * nq + q) * cstride + ec];
+                            const CT c = coop ? cl[((size_t)(live ? el : 0) * n_terms + t) * nq + q]
+                                              : qfast ? ctab[((size_t)terms[t].coef * cstride + ec) * nq + q]
+                                                      : ctab[((size_t)terms[t].coef * nq + q) * cstride + ec];
                             CT dsel = dv[0];
 #pragma unroll
                             for (int d = 1; d < 4; d++)
@@ -548,10 +566,18 @@ template <typename CT, int TI>
 static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int jc, int qfast, void* ke, hipStream_t st) {
     const int epb = 256 / jc;
     const unsigned grid = (unsigned)((b.n_elem + epb - 1) / epb);
-    if (b.q_inv)
-        hipLaunchKernelGGL((elem_cols<CT, TI, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
-    else
-        hipLaunchKernelGGL((elem_cols<CT, TI, true>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
+    if (b.q_inv) {
+        hipLaunchKernelGGL((elem_cols<CT, TI, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, 0, (CT*)ke);
+    } else {
+        // stage the block's coefficients in LDS when they fit
+        int maxt = 0;
+        for (int s = 0; s < b.n_slices; s++)
+            if ((mask >> s) & 1u) maxt = max(maxt, b.slices[s].n_terms);
+        const size_t bytes = (size_t)epb * maxt * b.n_gp * sizeof(CT);
+        const int coop = bytes <= 48 * 1024 ? 1 : 0;
+        hipLaunchKernelGGL((elem_cols<CT, TI, true>), dim3(grid), dim3(256), coop ? bytes : 0, st, b, (const CT*)ctab, mask, jc, qfast,
+                           coop, (CT*)ke);
+    }
 }
 
 template <typename CT>
