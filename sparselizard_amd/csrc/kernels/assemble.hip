@@ -18,6 +18,8 @@
 //   column slot in LDS (FP64), then write the A and D blocks
 //   (rawmat.cpp:67-125, 238-289).
 #include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstdlib>
 #include "device.h"
 #include "coef_device.h"
 
@@ -144,10 +146,10 @@ __global__ void __launch_bounds__(256) coef_kernel(DevBucket b, DevInputs in, in
 //   h_r     = w_q sum_s D_rs dof_s[j][q]
 //   Ke[i][j] += sum_r tf_r[i][q] h_r        (TI rows per pass)
 // which is the reference's sum over terms and Gauss points regrouped.
-template <typename CT, int TI, bool QV>
+template <typename CT, int TI, bool QV, bool COOP>
 __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restrict__ ctab, uint32_t slice_mask, int jc, int qfast,
-                                                 int coop, CT* __restrict__ ke) {
-    // QV with coop: the block's coefficients of the current slice, staged once
+                                                 CT* __restrict__ ke) {
+    // QV with COOP: the block's coefficients of the current slice, staged once
     // in LDS ([element][term][q]) and read from there by the element's lanes
     // (dynamic shared memory sized by the launcher)
     extern __shared__ __attribute__((aligned(16))) unsigned char elem_lds[];
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
         const SLX_CONST CT* tfq = sizeof(CT) == 4 ? (const SLX_CONST CT*)dsl[s].tf_tabq32 : (const SLX_CONST CT*)dsl[s].tf_tabq;
         const CT* dfq = sizeof(CT) == 4 ? (const CT*)dsl[s].dof_tabq32 : (const CT*)dsl[s].dof_tabq;
         const int64_t ke_base = dsl[s].ke_base, ke_ld = dsl[s].ke_ld;
-        if (QV && coop) {
+        if (QV && COOP) {
             const int per = n_terms * nq;
             __syncthreads();  // the previous slice's reads are done
             for (int x = threadIdx.x; x < epb * per; x += blockDim.x) {
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(256) elem_cols(DevBucket b, const CT* __restri
                         CT h[4] = {0, 0, 0, 0};
                         for (int t = 0; t < n_terms; t++) {
                             const int tr = terms[t].tf_deriv, td = terms[t].dof_deriv;
-                            const CT c = coop ? cl[((size_t)(live ? el : 0) * n_terms + t) * nq + q]
+                            const CT c = COOP ? cl[((size_t)(live ? el : 0) * n_terms + t) * nq + q]
                                               : qfast ? ctab[((size_t)terms[t].coef * cstride + ec) * nq + q]
                                                       : ctab[((size_t)terms[t].coef * nq + q) * cstride + ec];
                             CT dsel = dv[0];
@@ -567,16 +569,21 @@ static void launch_cols(const DevBucket& b, const void* ctab, uint32_t mask, int
     const int epb = 256 / jc;
     const unsigned grid = (unsigned)((b.n_elem + epb - 1) / epb);
     if (b.q_inv) {
-        hipLaunchKernelGGL((elem_cols<CT, TI, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, 0, (CT*)ke);
+        hipLaunchKernelGGL((elem_cols<CT, TI, false, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
     } else {
-        // stage the block's coefficients in LDS when they fit
+        // stage the block's coefficients in LDS when they fit under the limit
+        // (SLX_TUNE_ELEM_LDS_MAX bytes): a larger stage lowers the blocks a CU
+        // holds at once
         int maxt = 0;
         for (int s = 0; s < b.n_slices; s++)
             if ((mask >> s) & 1u) maxt = max(maxt, b.slices[s].n_terms);
         const size_t bytes = (size_t)epb * maxt * b.n_gp * sizeof(CT);
-        const int coop = bytes <= 48 * 1024 ? 1 : 0;
-        hipLaunchKernelGGL((elem_cols<CT, TI, true>), dim3(grid), dim3(256), coop ? bytes : 0, st, b, (const CT*)ctab, mask, jc, qfast,
-                           coop, (CT*)ke);
+        size_t lim = 48 * 1024;
+        if (const char* t = std::getenv("SLX_TUNE_ELEM_LDS_MAX")) lim = (size_t)std::min(48L * 1024, std::max(0L, std::atol(t)));
+        if (bytes <= lim)
+            hipLaunchKernelGGL((elem_cols<CT, TI, true, true>), dim3(grid), dim3(256), bytes, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
+        else
+            hipLaunchKernelGGL((elem_cols<CT, TI, true, false>), dim3(grid), dim3(256), 0, st, b, (const CT*)ctab, mask, jc, qfast, (CT*)ke);
     }
 }
 
