@@ -43,10 +43,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--problem", default="elasticity3d",
                     help="elasticity3d (the headline workload), hppoisson3d (hp hexahedra, orders 1..5), poisson2d, "
-                         "quadbatch (512 independent problems) or newton2d (Newton re-assembly with a field)")
+                         "quadbatch (512 independent problems), newton2d (Newton re-assembly with a field) or prism3d "
+                         "(order-1 prisms)")
     ap.add_argument("--n", type=int, default=0,
                     help="cells per edge (0: the workload's full size: 64 elasticity3d, 48 hppoisson3d, 200 poisson2d, "
-                         "708 newton2d; quadbatch takes its sizes from --extra)")
+                         "708 newton2d, 96 prism3d; quadbatch takes its sizes from --extra)")
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--extra", default="", help="extra problem options, e.g. ';pmin=5;pmax=5'")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run (timing sweeps)")
@@ -73,7 +74,7 @@ def main():
         return
     ctx = sa.Context(0)
     if args.n <= 0:
-        args.n = {"elasticity3d": 64, "hppoisson3d": 48, "poisson2d": 200, "newton2d": 708}.get(args.problem, 0)
+        args.n = {"elasticity3d": 64, "hppoisson3d": 48, "poisson2d": 200, "newton2d": 708, "prism3d": 96}.get(args.problem, 0)
     if args.problem == "elasticity3d":
         spec = "n=%d;order=%d" % (args.n, args.order)
     elif args.problem == "quadbatch":
@@ -284,6 +285,9 @@ def workload_name(args, n_elem):
     if args.problem == "newton2d":
         return ("2D nonlinear magnetostatics Newton re-assembly, P2, %dx%dx2 jittered triangles (%d), "
                 "nu(|grad a|) = 1 + 500/(1+|grad a|^2)" % (args.n, args.n, n_elem))
+    if args.problem == "prism3d":
+        return ("3D Poisson + mass, order-1 prisms, %d^3 x 2 jittered prisms (%d), c lognormal(0,1), u=0 on the side faces"
+                % (args.n, 2 * args.n ** 3))
     return "%s %s" % (args.problem, args.extra)
 
 
@@ -326,7 +330,8 @@ def cpu_baseline(sa, args):
 
     # sample size knob, its start, cap and how the element count grows with it
     knob = {"elasticity3d": ("n", 4, 20, 3), "hppoisson3d": ("n", 3, 20, 3), "poisson2d": ("n", 20, 1200, 2),
-            "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 64, 1)}[args.problem]
+            "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 64, 1),
+            "prism3d": ("n", 4, 60, 3)}[args.problem]
 
     def spec_of(v):
         if args.problem == "elasticity3d":

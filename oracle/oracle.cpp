@@ -84,7 +84,7 @@ void jacobian(const slx_plan_desc& d, const slx_bucket_desc& b, const double* xy
 
 // Order-1 Lagrange functions and reference derivatives: out[deriv][node][pt].
 void lagrange1(int type, const double* pts, int np, std::vector<double>& out) {
-    int nn = type == 1 ? 2 : type == 2 ? 3 : type == 3 ? 4 : type == 4 ? 4 : 8;
+    int nn = type == 1 ? 2 : type == 2 ? 3 : type == 3 ? 4 : type == 4 ? 4 : type == 6 ? 6 : 8;
     out.assign((size_t)4 * nn * np, 0.0);
     for (int p = 0; p < np; p++) {
         double k = pts[3 * p], e = pts[3 * p + 1], f = pts[3 * p + 2];
@@ -106,6 +106,18 @@ void lagrange1(int type, const double* pts, int np, std::vector<double>& out) {
                 v[n][1] = 0.25 * sx[n] * (1 + sy[n] * e);
                 v[n][2] = 0.25 * sy[n] * (1 + sx[n] * k);
                 v[n][3] = 0;
+            }
+        } else if (type == 6) {
+            // prism: triangle barycentrics (ki, eta) times (1 -+ phi) / 2
+            // (lagrangeprism.cpp node order: 0-2 at phi = -1, 3-5 at phi = +1)
+            double t[3][3] = {{1 - k - e, -1, -1}, {k, 1, 0}, {e, 0, 1}};
+            for (int n = 0; n < 6; n++) {
+                const double* tn = t[n % 3];
+                const double s = n < 3 ? -1.0 : 1.0, mu = 0.5 * (1 + s * f);
+                v[n][0] = tn[0] * mu;
+                v[n][1] = tn[1] * mu;
+                v[n][2] = tn[2] * mu;
+                v[n][3] = 0.5 * s * tn[0];
             }
         } else {
             double sx[8] = {-1, 1, 1, -1, -1, 1, 1, -1}, sy[8] = {-1, -1, 1, 1, -1, -1, 1, 1}, sz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};

@@ -90,8 +90,17 @@ void lambdas(int type, const D& ki, const D& eta, const D& phi, std::vector<D>& 
         sig[3] = x + y + (1.0 - z); sig[4] = (1.0 - x) + y + (1.0 - z);
         sig[5] = (1.0 - x) + (1.0 - y) + z; sig[6] = x + (1.0 - y) + z;
         sig[7] = x + y + z; sig[8] = (1.0 - x) + y + z;
+    } else if (type == PRISM) {
+        // triangle barycentrics times the linear functions of the prism axis,
+        // phi mapped to [0, 1] (h1prism.cpp: lambda and mu, vertex functions)
+        D z = 0.5 * (phi + 1.0);
+        D t[3] = {1.0 - ki - eta, ki, eta};
+        for (int n = 0; n < 3; n++) {
+            lam[1 + n] = t[n] * (1.0 - z);
+            lam[4 + n] = t[n] * z;
+        }
     } else {
-        fail("H1 functions are provided for line, triangle, quadrangle, tetrahedron and hexahedron elements");
+        fail("H1 functions are provided for line, triangle, quadrangle, tetrahedron, hexahedron and (order 1) prism elements");
     }
 }
 
@@ -207,11 +216,17 @@ int h1_count(int type, int order) {
         case QUAD: return (order + 1) * (order + 1);
         case TET: return (order + 1) * (order + 2) * (order + 3) / 6;
         case HEX: return (order + 1) * (order + 1) * (order + 1);
+        case PRISM:
+            // vertex functions only (h1prism.cpp count(): 6 at order 1)
+            SLX_CHECK(order == 1, "prism H1 functions are provided at order 1 (got order " << order << ")");
+            return 6;
     }
     fail("H1 count: unsupported element type " + std::to_string(type));
 }
 
 std::vector<FormFunction> h1_iterate(int type, int order) {
+    SLX_CHECK(type != PRISM || order <= 1, "prism H1 functions are provided at order 1 (got order " << order << ")");
+    SLX_CHECK(type != PYRAMID, "pyramid H1 functions are not provided");
     std::vector<FormFunction> out;
     for (int dim = 0; dim <= 3; dim++) {
         int nent = topo::count_dim(type, dim);

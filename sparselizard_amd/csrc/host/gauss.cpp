@@ -64,7 +64,18 @@ GaussRule gauss_rule(int type, int order) {
             return r;
         }
     }
-    fail("Gauss rules are provided for point, line, triangle, quadrangle, tetrahedron and hexahedron elements");
+    if (type == PRISM) {
+        // line points outer, triangle points inner; (ki, eta) from the
+        // triangle, phi from the line (gpprism.cpp:13-49)
+        GaussRule l = gauss_rule(LINE, order), t = gauss_rule(TRI, order);
+        for (size_t i = 0; i < l.weights.size(); i++)
+            for (size_t j = 0; j < t.weights.size(); j++) {
+                r.coords.insert(r.coords.end(), {t.coords[3 * j], t.coords[3 * j + 1], l.coords[3 * i]});
+                r.weights.push_back(l.weights[i] * t.weights[j]);
+            }
+        return r;
+    }
+    fail("Gauss rules are provided for point, line, triangle, quadrangle, tetrahedron, hexahedron and prism elements");
 }
 
 }  // namespace slx

@@ -43,7 +43,7 @@ int Mesh::physreg_dim(int physnum) const {
 
 int Mesh::add_element(int type, const std::vector<int>& nodes, int physreg) {
     SLX_CHECK(!finalized_, "mesh already finalized");
-    SLX_CHECK(type >= 0 && type <= 5 && type != PRISM, "unsupported element type " << type);
+    SLX_CHECK(type >= 0 && type <= PRISM, "unsupported element type " << type);
     SLX_CHECK((int)nodes.size() == topo::count_nodes(type), "element of type " << type << " needs " << topo::count_nodes(type) << " nodes");
     for (int n : nodes) SLX_CHECK(n >= 0 && n < count(0), "element node " << n << " out of range");
     // physicalregions::get: regions are created on first use (physicalregions.cpp:112-127).

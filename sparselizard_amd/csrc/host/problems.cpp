@@ -319,7 +319,7 @@ std::shared_ptr<Problem> quadbatch(const std::map<std::string, std::string>& s) 
 
 // Structured unit-square / unit-cube grids of one element type.
 void grid_mesh(Mesh& m, const std::string& elem, int n, double jit, Rng& rng, int volreg, int bndreg) {
-    bool is3d = elem == "tet" || elem == "hex";
+    bool is3d = elem == "tet" || elem == "hex" || elem == "prism";
     int n1 = n + 1;
     double h = 1.0 / n;
     std::vector<double> xyz;
@@ -365,6 +365,31 @@ void grid_mesh(Mesh& m, const std::string& elem, int n, double jit, Rng& rng, in
                                   volreg);
         return;
     }
+    if (elem == "prism") {
+        if (bndreg >= 0) {
+            // the four side faces of the cube as quadrangles (prism side faces)
+            for (int a = 0; a < n; a++)
+                for (int b = 0; b < n; b++) {
+                    m.add_element(QUAD, {v3(0, a, b), v3(0, a, b + 1), v3(0, a + 1, b + 1), v3(0, a + 1, b)}, bndreg);
+                    m.add_element(QUAD, {v3(n, a, b), v3(n, a + 1, b), v3(n, a + 1, b + 1), v3(n, a, b + 1)}, bndreg);
+                    m.add_element(QUAD, {v3(a, 0, b), v3(a + 1, 0, b), v3(a + 1, 0, b + 1), v3(a, 0, b + 1)}, bndreg);
+                    m.add_element(QUAD, {v3(a, n, b), v3(a, n, b + 1), v3(a + 1, n, b + 1), v3(a + 1, n, b)}, bndreg);
+                }
+        }
+        // every cube cut along its (i, j) -> (i+1, j+1) diagonal into two
+        // prisms, bottom triangle counter-clockwise seen from +z
+        for (int k = 0; k < n; k++)
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    m.add_element(PRISM, {v3(i, j, k), v3(i + 1, j, k), v3(i + 1, j + 1, k), v3(i, j, k + 1), v3(i + 1, j, k + 1),
+                                          v3(i + 1, j + 1, k + 1)},
+                                  volreg);
+                    m.add_element(PRISM, {v3(i, j, k), v3(i + 1, j + 1, k), v3(i, j + 1, k), v3(i, j, k + 1), v3(i + 1, j + 1, k + 1),
+                                          v3(i, j + 1, k + 1)},
+                                  volreg);
+                }
+        return;
+    }
     static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
     for (int k = 0; k < n; k++)
         for (int j = 0; j < n; j++)
@@ -400,7 +425,7 @@ std::shared_ptr<Problem> projection(const std::map<std::string, std::string>& s)
     p->fields = {u};
     p->form = std::make_shared<formulation>(m);
     using namespace sl;
-    bool is3d = elem == "tet" || elem == "hex";
+    bool is3d = elem == "tet" || elem == "hex" || elem == "prism";
     expression f(0.0);
     bool first = true;
     for (int a = 0; a <= deg; a++)
@@ -454,6 +479,40 @@ std::shared_ptr<Problem> hppoisson3d(const std::map<std::string, std::string>& s
     *p->form += integral(1, -expression(1.0) * tf(u), -1);
     std::ostringstream d;
     d << "3D hp-Poisson, " << n << "^3 hexahedra, orders " << pmin << ".." << pmax << ", jitter " << jit;
+    p->describe = d.str();
+    return p;
+}
+
+// 3D Poisson with a mass term on order-1 prisms (n^3 cubes, two prisms each,
+// jittered interior nodes): integral(c grad(dof(u)) grad(tf(u)) + dof(u) tf(u))
+// and -f tf(u), u = 0 on the four side faces (mass=0 drops the mass term).  The prism H1 vertex functions
+// follow h1prism.cpp; the Gauss rule gpprism.cpp.
+std::shared_ptr<Problem> prism3d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    int n = geti(s, "n", 32);
+    double jit = getd(s, "jitter", 0.1), sigma = getd(s, "sigma", 1.0);
+    Rng rng((uint64_t)geti(s, "seed", 3));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    grid_mesh(m, "prism", n, jit, rng, 1, 2);
+    m.finalize();
+    int np = m.count(PRISM);
+    parameter c;
+    std::vector<double> cv(np);
+    for (int e = 0; e < np; e++) cv[e] = std::exp(sigma * rng.normal());
+    c.setelementvalues(cv);
+    field u(m, "h1", "u");
+    u.setorder(1, 1);
+    u.setconstraint(2);
+    p->fields = {u};
+    p->params = {c};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    *p->form += integral(1, expression(c) * grad(dof(u)) * grad(tf(u)), geti(s, "delta", 0));
+    if (geti(s, "mass", 1)) *p->form += integral(1, dof(u) * tf(u), geti(s, "delta", 0));
+    *p->form += integral(1, -expression(getd(s, "f", 1.0)) * tf(u), geti(s, "delta", 0));
+    std::ostringstream d;
+    d << "3D Poisson + mass, " << n << "^3 x 2 order-1 prisms, jitter " << jit;
     p->describe = d.str();
     return p;
 }
@@ -535,6 +594,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "quadbatch") p = quadbatch(s);
     else if (kind == "projection") p = projection(s);
     else if (kind == "hppoisson3d") p = hppoisson3d(s);
+    else if (kind == "prism3d") p = prism3d(s);
     else if (kind == "msh") p = from_msh(s);
     else if (kind == "electrostatics1d") p = electrostatics1d(s);
     else if (kind == "newton2d") p = newton2d(s);

@@ -97,6 +97,19 @@ def test_f16split_qvarying_buckets_on_valu(ctx):
     print("errors", errs)
 
 
+@pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32), ("f16split", TOL_SPLIT)])
+def test_prism3d(ctx, precision, tol):
+    # order-1 prisms: Gauss-point-varying Jacobians, stiffness + mass + rhs
+    run_case(ctx, "prism3d", "n=5;jitter=0.15", precision, tol)
+
+
+def test_prism3d_f16split_mfma(ctx, monkeypatch):
+    monkeypatch.setenv("SLX_TUNE_MFMA_QVARYING", "1")
+    monkeypatch.setenv("SLX_TUNE_TABLE", "none")
+    prob, plan, got, errs = run_case(ctx, "prism3d", "n=4;jitter=0.15", "f16split", TOL_SPLIT)
+    assert plan.kernels()["mfma_buckets"] >= 1, "MFMA kernel not used"
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32)])
 def test_hp_hex_long_segments(ctx, precision, tol):
     # per-element orders 1..5 on hexahedra: element rows up to 216 entries
@@ -119,7 +132,8 @@ def test_deterministic_repeat(ctx, precision):
     assert np.array_equal(a.Aval, b.Aval) and np.array_equal(a.Dval, b.Dval) and np.array_equal(a.rhs, b.rhs)
 
 
-@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2")])
+@pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2"),
+                                       ("prism3d", "n=3")])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("f16split", TOL_SPLIT), ("fp32", TOL_FP32)])
 def test_element_matrices_match_oracle(ctx, kind, spec, precision, tol):
     """Element matrices (before the reduction) against the oracle, slice by
@@ -158,7 +172,7 @@ def test_element_matrices_match_oracle(ctx, kind, spec, precision, tol):
 
 
 @pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2"),
-                                       ("projection", "elem=hex;n=2;order=2")])
+                                       ("projection", "elem=hex;n=2;order=2"), ("prism3d", "n=3")])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-6)])
 def test_coefficient_kernel_matches_oracle(ctx, kind, spec, precision, tol):
     """Gauss-point coefficients (geometry + program) of the last bucket."""
