@@ -286,8 +286,12 @@ def workload_name(args, n_elem):
         return ("2D nonlinear magnetostatics Newton re-assembly, P2, %dx%dx2 jittered triangles (%d), "
                 "nu(|grad a|) = 1 + 500/(1+|grad a|^2)" % (args.n, args.n, n_elem))
     if args.problem == "prism3d":
-        return ("3D Poisson + mass, order-1 prisms, %d^3 x 2 jittered prisms (%d), c lognormal(0,1), u=0 on the side faces"
-                % (args.n, 2 * args.n ** 3))
+        order = 1
+        for kv in args.extra.split(";"):
+            if kv.startswith("order="):
+                order = int(kv[6:])
+        return ("3D Poisson + mass, order-%d prisms, %d^3 x 2 jittered prisms (%d), c lognormal(0,1), u=0 on the side faces"
+                % (order, args.n, 2 * args.n ** 3))
     return "%s %s" % (args.problem, args.extra)
 
 
@@ -336,7 +340,7 @@ def cpu_baseline(sa, args):
     def spec_of(v):
         if args.problem == "elasticity3d":
             return "n=%d;order=%d" % (v, args.order)
-        return "%s=%d" % (knob[0], v)
+        return "%s=%d%s" % (knob[0], v, args.extra)
 
     def run(v):
         p = sa.Problem(None, args.problem, spec_of(v))

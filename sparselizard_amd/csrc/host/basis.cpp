@@ -100,14 +100,81 @@ void lambdas(int type, const D& ki, const D& eta, const D& phi, std::vector<D>& 
             lam[4 + n] = t[n] * z;
         }
     } else {
-        fail("H1 functions are provided for line, triangle, quadrangle, tetrahedron, hexahedron and (order 1) prism elements");
+        fail("H1 functions are provided for line, triangle, quadrangle, tetrahedron, hexahedron and prism elements");
     }
 }
 
 bool simplex(int type) { return type == TRI || type == TET; }
 
 // Evaluate form function `f` (for entity orientation `orient`) at one point.
+// Prism functions (h1prism.cpp): triangle barycentrics lambda_n and axis
+// functions mu_n per node (1-based, nodes 1-3 at phi = -1, 4-6 at phi = +1).
+D eval_prism(int maxorder, const FormFunction& f, int orient, const D& ki, const D& eta, const D& phi) {
+    const D z = 0.5 * (phi + 1.0);
+    D lam[7], mu[7];
+    const D t[3] = {1.0 - ki - eta, ki, eta};
+    for (int n = 0; n < 3; n++) {
+        lam[1 + n] = t[n];
+        lam[4 + n] = t[n];
+        mu[1 + n] = 1.0 - z;
+        mu[4 + n] = z;
+    }
+    auto same_layer = [](int a, int b) { return (a <= 3) == (b <= 3); };
+    const int h = f.order;
+    if (f.dim == 0) return lam[f.num + 1] * mu[f.num + 1];
+    if (f.dim == 1) {
+        const std::vector<int>& en = topo::edge_nodes(PRISM);
+        int e1 = en[2 * f.num + edge_reorder[orient][0]] + 1;
+        int e2 = en[2 * f.num + edge_reorder[orient][1]] + 1;
+        if (same_layer(e1, e2)) return leg_Ls(maxorder, lam[e1] - lam[e2], lam[e1] + lam[e2])[h] * mu[e1];
+        return leg_L(maxorder, 2.0 * mu[e1] - 1.0)[h] * lam[e1];
+    }
+    const std::vector<int>& fn = topo::face_nodes(PRISM);
+    if (f.dim == 2 && f.num < 2) {
+        int f1 = fn[3 * f.num + tri_reorder[orient][0]] + 1;
+        int f2 = fn[3 * f.num + tri_reorder[orient][1]] + 1;
+        int f3 = fn[3 * f.num + tri_reorder[orient][2]] + 1;
+        std::vector<D> Ls = leg_Ls(maxorder, lam[f1] - lam[f2], lam[f1] + lam[f2]);
+        std::vector<D> l = leg_l(maxorder, 2.0 * lam[f3] - 1.0);
+        // i + j = h - 3, i outer
+        int i = f.l, j = h - 3 - f.l;
+        return Ls[i + 2] * lam[f3] * l[j] * mu[f1];
+    }
+    if (f.dim == 2) {
+        const int off = 6 + 4 * (f.num - 2);
+        int f1 = fn[off + quad_reorder[orient][0]] + 1;
+        int f2 = fn[off + quad_reorder[orient][1]] + 1;
+        int f4 = fn[off + quad_reorder[orient][3]] + 1;
+        // the face direction along the triangle side starts at f1 and goes to
+        // f2 or f4, whichever lies in f1's layer
+        int f2s = same_layer(f1, f2) ? f2 : f4;
+        std::vector<D> Ls = leg_Ls(maxorder, lam[f1] - lam[f2s], lam[f1] + lam[f2s]);
+        std::vector<D> L = leg_L(maxorder, 2.0 * mu[f1] - 1.0);
+        int cnt = 0;
+        for (int i = 0; i <= h - 2; i++)
+            for (int j = 0; j <= h - 2; j++) {
+                if (i != h - 2 && j != h - 2) continue;
+                if (cnt == f.l) return f2 == f2s ? Ls[i + 2] * L[j + 2] : L[i + 2] * Ls[j + 2];
+                cnt++;
+            }
+        fail("prism quad face function index out of range");
+    }
+    std::vector<D> Ls = leg_Ls(maxorder, lam[1] - lam[2], lam[1] + lam[2]);
+    std::vector<D> l = leg_l(maxorder, 2.0 * lam[3] - 1.0);
+    std::vector<D> L = leg_L(maxorder, 2.0 * mu[1] - 1.0);
+    int cnt = 0;
+    for (int i = 0; i <= h - 3; i++)
+        for (int j = 0; j <= h - 3 - i; j++)
+            for (int k = 0; k <= h - 2; k++) {
+                if (i + j != h - 3 && k != h - 2) continue;
+                if (cnt == f.l) return Ls[i + 2] * lam[3] * l[j] * L[k + 2];
+                cnt++;
+            }
+    fail("prism volume function index out of range");
+}
+
 D eval_ff(int type, int maxorder, const FormFunction& f, int orient, const D& ki, const D& eta, const D& phi) {
+    if (type == PRISM) return eval_prism(maxorder, f, orient, ki, eta, phi);
     std::vector<D> lam, sig;
     lambdas(type, ki, eta, phi, lam, sig);
     const int h = f.order;
@@ -202,6 +269,7 @@ int h1_count(int type, int order, int dim) {
         case 3:
             if (type == TET) return (order - 3) * (order - 2) * (order - 1) / 6;
             if (type == HEX) return (order - 1) * (order - 1) * (order - 1);
+            if (type == PRISM) return (order - 2) * (order - 1) * (order - 1) / 2;
             return 0;
     }
     return 0;
@@ -216,22 +284,21 @@ int h1_count(int type, int order) {
         case QUAD: return (order + 1) * (order + 1);
         case TET: return (order + 1) * (order + 2) * (order + 3) / 6;
         case HEX: return (order + 1) * (order + 1) * (order + 1);
-        case PRISM:
-            // vertex functions only (h1prism.cpp count(): 6 at order 1)
-            SLX_CHECK(order == 1, "prism H1 functions are provided at order 1 (got order " << order << ")");
-            return 6;
+        case PRISM: return (order + 1) * (order + 1) * (order + 2) / 2;  // line x triangle
     }
     fail("H1 count: unsupported element type " + std::to_string(type));
 }
 
 std::vector<FormFunction> h1_iterate(int type, int order) {
-    SLX_CHECK(type != PRISM || order <= 1, "prism H1 functions are provided at order 1 (got order " << order << ")");
     SLX_CHECK(type != PYRAMID, "pyramid H1 functions are not provided");
     std::vector<FormFunction> out;
     for (int dim = 0; dim <= 3; dim++) {
         int nent = topo::count_dim(type, dim);
         for (int num = 0; num < nent; num++) {
-            int nff = h1_count(type, order, dim);
+            // counts follow the entity's own type (a prism has triangular and
+            // quadrangular faces, h1prism.cpp count(order, dim, num))
+            int assoc = dim == 0 ? POINT : dim == 1 ? LINE : dim == 2 ? (topo::is_triangular_face(type, num) ? TRI : QUAD) : type;
+            int nff = h1_count(assoc, order, dim);
             // entity counts are only defined up to the element dimension
             if (dim > topo::dimension(type)) nff = 0;
             for (int idx = 0; idx < nff; idx++) {
@@ -240,13 +307,11 @@ std::vector<FormFunction> h1_iterate(int type, int order) {
                 f.num = num;
                 f.index = idx;
                 int h = 0;
-                while (idx >= h1_count(type, h, dim)) h++;
+                while (idx >= h1_count(assoc, h, dim)) h++;
                 f.order = h;
-                f.l = idx - h1_count(type, h - 1, dim);
-                if (dim == 0) f.assoc_type = POINT;
-                else if (dim == 1) f.assoc_type = LINE;
-                else if (dim == 2) f.assoc_type = topo::is_triangular_face(type, num) ? TRI : QUAD;
-                else f.assoc_type = type;
+                f.l = idx - h1_count(assoc, h - 1, dim);
+                f.assoc_type = assoc;
+                f.sub = (dim == 2 && assoc == QUAD) ? num - topo::count_tri_faces(type) : num;
                 out.push_back(f);
             }
         }
@@ -309,6 +374,9 @@ void h1_tables(int type, int order, int totalorientation, const std::vector<doub
     int nff = (int)ffs.size(), np = (int)pts.size() / 3;
     std::vector<int> eo, fo;
     orientations_from_total(type, totalorientation, eo, fo);
+    for (size_t f = 0; f < fo.size(); f++)
+        SLX_CHECK(fo[f] < (topo::is_triangular_face(type, (int)f) ? 6 : 8),
+                  "total orientation " << totalorientation << " gives face " << f << " orientation " << fo[f]);
     out.assign((size_t)4 * nff * np, 0.0);
     for (int p = 0; p < np; p++) {
         D ki = var(pts[3 * p], 0), eta = var(pts[3 * p + 1], 1), phi = var(pts[3 * p + 2], 2);

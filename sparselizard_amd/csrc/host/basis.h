@@ -28,6 +28,8 @@ struct FormFunction {
     int order;          // polynomial order of the function
     int l;              // index among the functions of that order in the entity
     int assoc_type;     // element type of the sub-entity carrying the function
+    int sub;            // index of that sub-entity among the element's sub-entities of
+                        // type assoc_type (a prism's quadrangular faces 2-4 are quadrangles 0-2)
 };
 
 // Form functions in the reference iterator order.

@@ -282,9 +282,9 @@ std::vector<int32_t> DofManager::getaddresses(const std::shared_ptr<RawField>& f
         const FormFunction& ff = ffs[k];
         int prevdr = -1, cnt = 0;
         for (int i = 0; i < ne; i++) {
-            int sub = mesh_->subelement(ff.assoc_type, elemtype, elems[i], ff.num);
+            int sub = mesh_->subelement(ff.assoc_type, elemtype, elems[i], ff.sub);
             int dr = mesh_->disjreg_of(ff.assoc_type, sub);
-            if (i == 0 || dr != prevdr) cnt = h1_count(elemtype, f->order(dr), ff.dim);
+            if (i == 0 || dr != prevdr) cnt = h1_count(ff.assoc_type, f->order(dr), ff.dim);
             prevdr = dr;
             int32_t a = -1;
             if (defined[dr] && ff.index < cnt) a = rangebegin_[fi][dr][ff.index] + (sub - mesh_->disjreg_begin(dr));
@@ -805,9 +805,9 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
             for (size_t q = 0; q < ffl.size(); q++)
                 for (int i = 0; i < bd.n_elem; i++) {
                     int e = f0.elems[i];
-                    int sub = mesh_->subelement(ffl[q].assoc_type, f0.type, e, ffl[q].num);
+                    int sub = mesh_->subelement(ffl[q].assoc_type, f0.type, e, ffl[q].sub);
                     int dr = mesh_->disjreg_of(ffl[q].assoc_type, sub);
-                    int cnt = h1_count(f0.type, fl->order(dr), ffl[q].dim);
+                    int cnt = h1_count(ffl[q].assoc_type, fl->order(dr), ffl[q].dim);
                     addr[q * bd.n_elem + i] = ffl[q].index < cnt ? fl->data_rangebegin[dr][ffl[q].index] + (sub - mesh_->disjreg_begin(dr)) : -1;
                 }
             ps->ints.push_back(addr);

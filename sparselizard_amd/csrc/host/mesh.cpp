@@ -24,6 +24,7 @@ int Mesh::count(int type) const {
 int Mesh::subelement(int subtype, int type, int elem, int idx) const {
     if (subtype == type) return elem;
     int n = subtype == 0 ? topo::count_nodes(type) : topo::count_type(type, subtype);
+    SLX_CHECK(idx >= 0 && idx < n, "sub-entity " << idx << " of type " << subtype << " out of range for element type " << type);
     return sub_[type][subtype][(size_t)elem * n + idx];
 }
 

@@ -483,10 +483,11 @@ std::shared_ptr<Problem> hppoisson3d(const std::map<std::string, std::string>& s
     return p;
 }
 
-// 3D Poisson with a mass term on order-1 prisms (n^3 cubes, two prisms each,
-// jittered interior nodes): integral(c grad(dof(u)) grad(tf(u)) + dof(u) tf(u))
-// and -f tf(u), u = 0 on the four side faces (mass=0 drops the mass term).  The prism H1 vertex functions
-// follow h1prism.cpp; the Gauss rule gpprism.cpp.
+// 3D Poisson with a mass term on prisms (n^3 cubes, two prisms each,
+// jittered interior nodes, field order `order`): integral(c grad(dof(u))
+// grad(tf(u)) + dof(u) tf(u)) and -f tf(u), u = 0 on the four side faces
+// (closed=1 adds the bottom and top; mass=0 drops the mass term).  The prism
+// H1 functions follow h1prism.cpp; the Gauss rule gpprism.cpp.
 std::shared_ptr<Problem> prism3d(const std::map<std::string, std::string>& s) {
     auto p = std::make_shared<Problem>();
     int n = geti(s, "n", 32);
@@ -495,6 +496,17 @@ std::shared_ptr<Problem> prism3d(const std::map<std::string, std::string>& s) {
     p->mesh = std::make_shared<Mesh>();
     Mesh& m = *p->mesh;
     grid_mesh(m, "prism", n, jit, rng, 1, 2);
+    if (geti(s, "closed", 0)) {
+        // closed=1: the bottom and top triangles join the Dirichlet region too
+        const int n1 = n + 1;
+        auto v3 = [&](int i, int j, int k) { return i + n1 * (j + n1 * k); };
+        for (int k = 0; k <= n; k += n)
+            for (int j = 0; j < n; j++)
+                for (int i = 0; i < n; i++) {
+                    m.add_element(TRI, {v3(i, j, k), v3(i + 1, j, k), v3(i + 1, j + 1, k)}, 2);
+                    m.add_element(TRI, {v3(i, j, k), v3(i + 1, j + 1, k), v3(i, j + 1, k)}, 2);
+                }
+    }
     m.finalize();
     int np = m.count(PRISM);
     parameter c;
@@ -502,7 +514,7 @@ std::shared_ptr<Problem> prism3d(const std::map<std::string, std::string>& s) {
     for (int e = 0; e < np; e++) cv[e] = std::exp(sigma * rng.normal());
     c.setelementvalues(cv);
     field u(m, "h1", "u");
-    u.setorder(1, 1);
+    u.setorder(1, geti(s, "order", 1));
     u.setconstraint(2);
     p->fields = {u};
     p->params = {c};
@@ -512,7 +524,7 @@ std::shared_ptr<Problem> prism3d(const std::map<std::string, std::string>& s) {
     if (geti(s, "mass", 1)) *p->form += integral(1, dof(u) * tf(u), geti(s, "delta", 0));
     *p->form += integral(1, -expression(getd(s, "f", 1.0)) * tf(u), geti(s, "delta", 0));
     std::ostringstream d;
-    d << "3D Poisson + mass, " << n << "^3 x 2 order-1 prisms, jitter " << jit;
+    d << "3D Poisson + mass, " << n << "^3 x 2 prisms, order " << geti(s, "order", 1) << ", jitter " << jit;
     p->describe = d.str();
     return p;
 }

@@ -97,10 +97,13 @@ def test_f16split_qvarying_buckets_on_valu(ctx):
     print("errors", errs)
 
 
+@pytest.mark.parametrize("spec", ["n=5;jitter=0.15", "n=3;order=3", "n=2;order=5"])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32), ("f16split", TOL_SPLIT)])
-def test_prism3d(ctx, precision, tol):
-    # order-1 prisms: Gauss-point-varying Jacobians, stiffness + mass + rhs
-    run_case(ctx, "prism3d", "n=5;jitter=0.15", precision, tol)
+def test_prism3d(ctx, spec, precision, tol):
+    # prisms: Gauss-point-varying Jacobians, stiffness + mass + rhs; order 3 has
+    # orientation-dependent face functions, order 5 (126 x 126 element matrices)
+    # takes the factorized kernel in the split mode
+    run_case(ctx, "prism3d", spec, precision, tol)
 
 
 def test_prism3d_f16split_mfma(ctx, monkeypatch):
@@ -133,7 +136,7 @@ def test_deterministic_repeat(ctx, precision):
 
 
 @pytest.mark.parametrize("kind,spec", [("poisson2d", "n=5"), ("elasticity3d", "n=2"), ("hppoisson3d", "n=2"),
-                                       ("prism3d", "n=3")])
+                                       ("prism3d", "n=3"), ("prism3d", "n=2;order=3")])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("f16split", TOL_SPLIT), ("fp32", TOL_FP32)])
 def test_element_matrices_match_oracle(ctx, kind, spec, precision, tol):
     """Element matrices (before the reduction) against the oracle, slice by

@@ -1,17 +1,21 @@
-"""Order-1 prisms (element type 6): Gauss rule, H1 vertex functions and the
-oracle's assembly on a jittered prism mesh, pinned by closed forms.
+"""Prisms (element type 6): Gauss rule, hierarchical H1 functions and the
+oracle's assembly on jittered prism meshes, pinned by closed forms.
 
 - the prism rule is the line rule (outer) times the triangle rule (inner),
   coordinates (ki, eta) from the triangle and phi from the line
   (gpprism.cpp:13-49): exact for ki^a eta^b phi^c with a + b <= k, c <= k;
 - the vertex functions lambda_n mu_n (h1prism.cpp) interpolate the reference
   nodes (lagrangeprism.cpp:9) and sum to one;
-- isoparametric patch test: with c = 1 and no mass term, K x = 0 on rows of
-  interior nodes for x, y and z (the gradients of the coordinate functions
-  are the unit vectors at every Gauss point, and the rule integrates the
-  cofactor products exactly at order 5);
-- with the mass term, the row sums of [A D] equal the rhs (K 1 = 0, so the
-  row sum is int phi_i = b_i for f = 1)."""
+- at order p the (p+1)^2 (p+2)/2 functions (vertex, edge, face, volume;
+  h1prism.cpp) are linearly independent and span P_p(triangle) x P_p(line)
+  exactly, in every total orientation;
+- isoparametric patch test: with c = 1 and no mass term, K x = 0 on every
+  free row of a mesh whose whole boundary is constrained, for x, y and z
+  (each row is a sum of boundary integrals of phi_i n over the elements of
+  its support, which cancel only if phi_i is continuous across the shared
+  faces: a mis-oriented edge or face function breaks it);
+- with the mass term, A 1 + D 1 (ones on the vertex DOFs) equals the rhs
+  (K 1 = 0, so the row sum is int phi_i = b_i for f = 1)."""
 import ctypes as C
 from math import factorial
 
@@ -58,9 +62,33 @@ def test_prism_vertex_functions():
         assert np.allclose(v[1 + d], fd, atol=1e-8)
 
 
-def test_prism_order2_rejected():
-    with pytest.raises(sa.SlxError, match="order 1"):
-        L.h1_count(PRISM, 2)
+def _monomials(pts, p):
+    cols = []
+    for a in range(p + 1):
+        for b in range(p + 1 - a):
+            for c in range(p + 1):
+                cols.append(pts[:, 0] ** a * pts[:, 1] ** b * pts[:, 2] ** c)
+    return np.column_stack(cols)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5])
+def test_prism_functions_span_the_space(order):
+    nff = L.h1_count(PRISM, order)
+    assert nff == (order + 1) ** 2 * (order + 2) // 2
+    rng = np.random.default_rng(order)
+    u = rng.uniform(0, 1, (3 * nff, 2))
+    u = np.where(u.sum(1, keepdims=True) > 1, 1 - u, u)
+    pts = np.column_stack([u, rng.uniform(-1, 1, 3 * nff)])
+    M = _monomials(pts, order)
+    assert M.shape[1] == nff
+    # edge orientations in bits 0-8, then face orientations in base 8
+    # (triangles 0-5, quadrangles 0-7)
+    mixed = 511 + 512 * (5 + 8 * 3 + 64 * 7 + 512 * 2 + 4096 * 6)
+    for total in (0, 1, 37, 4097, mixed):
+        B = L.h1_tables(PRISM, order, pts, total_orientation=total)[0].T
+        assert np.linalg.matrix_rank(B) == nff, total
+        coef = np.linalg.lstsq(B, M, rcond=None)[0]
+        assert abs(B @ coef - M).max() <= 1e-9 * abs(M).max(), total
 
 
 def _assemble(spec):
@@ -72,10 +100,12 @@ def _assemble(spec):
     D = sp.csr_matrix((o.Dval, o.Dcol, o.Drow), shape=(nf, nd - nf))
     d = p.desc()
     cons = np.ctypeslib.as_array(d.constrained, shape=(nd,)).astype(bool)
-    # DOF -> node through the vertex functions of every element
+    # vertex DOF -> node through the first six functions of every element;
+    # the higher-order DOFs of a linear function are zero
     num = p.numeric()
     xyz = np.ctypeslib.as_array(num.node_xyz, shape=(3 * p.n_nodes(),)).reshape(-1, 3)
-    X = np.full((nd, 3), np.nan)
+    X = np.zeros((nd, 3))
+    vertex = np.zeros(nd, bool)
     for b in range(d.n_buckets):
         bk = d.buckets[b]
         assert bk.elem_type == PRISM
@@ -84,28 +114,32 @@ def _assemble(spec):
         tfa = np.ctypeslib.as_array(sl.tf_addr, shape=(sl.n_tf, bk.n_elem))
         for n in range(6):
             X[tfa[n], :] = xyz[geo[:, n], :]
-    assert not np.isnan(X).any()
-    return A, D, o.rhs, cons, X
+            vertex[tfa[n]] = True
+    return A, D, o.rhs, cons, X, vertex
 
 
-def test_prism_patch_test():
-    A, D, b, cons, X = _assemble("n=4;jitter=0.15;sigma=0;mass=0;delta=3")
-    assert abs(A - A.T).max() <= 1e-14 * abs(A).max()
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_prism_patch_test(order):
+    A, D, b, cons, X, vertex = _assemble("n=3;jitter=0.15;sigma=0;mass=0;closed=1;order=%d;delta=3" % order)
+    assert abs(A - A.T).max() <= 1e-13 * abs(A).max()
     Xf, Xc = X[~cons], X[cons]
-    interior = np.all((Xf > 1e-12) & (Xf < 1 - 1e-12), axis=1)
-    assert interior.sum() >= 20
+    assert (~vertex[~cons]).sum() > 0 or order == 1
     for c in range(3):
         r = A @ Xf[:, c] + D @ Xc[:, c]
-        assert abs(r[interior]).max() <= 1e-12 * abs(A).max(), c
-    # the rows of boundary-touching nodes are not zero for z (planted check)
-    r = A @ Xf[:, 2] + D @ Xc[:, 2]
-    assert abs(r[~interior]).max() > 1e-3 * abs(A).max()
+        assert abs(r).max() <= 1e-11 * abs(A).max(), (order, c, abs(r).max())
+    # planted check: open top and bottom faces leave non-zero rows for z
+    A2, D2, b2, cons2, X2, v2 = _assemble("n=3;jitter=0.15;sigma=0;mass=0;order=%d;delta=3" % order)
+    r = A2 @ X2[~cons2, 2] + D2 @ X2[cons2, 2]
+    assert abs(r).max() > 1e-3 * abs(A2).max()
 
 
-def test_prism_row_sums_are_rhs():
-    A, D, b, cons, X = _assemble("n=5;jitter=0.1")
-    rs = np.asarray(A.sum(1)).ravel() + np.asarray(D.sum(1)).ravel()
+@pytest.mark.parametrize("order", [1, 3])
+def test_prism_row_sums_are_rhs(order):
+    A, D, b, cons, X, vertex = _assemble("n=4;jitter=0.1;order=%d" % order)
+    one = vertex.astype(np.float64)
+    rs = A @ one[~cons] + D @ one[cons]
     bf = b[~cons]
-    assert np.allclose(rs, bf, rtol=1e-12, atol=1e-15)
-    # every free DOF's function integrates to a positive volume share
-    assert (bf > 0).all()
+    assert np.allclose(rs, bf, rtol=1e-11, atol=1e-13 * abs(bf).max())
+    # order 1: every free DOF's function integrates to a positive volume share
+    if order == 1:
+        assert (bf > 0).all()
