@@ -107,7 +107,7 @@ typedef struct slx_slice_desc {
 /* A bucket: elements of one type sharing Gauss rule, orders and (when the
  * basis depends on it) total orientation (elementselector.cpp:4-133). */
 typedef struct slx_bucket_desc {
-    int32_t elem_type;          /* 1 line, 2 tri, 3 quad, 4 tet, 5 hex, 6 prism (order-1 H1) */
+    int32_t elem_type;          /* 1 line, 2 tri, 3 quad, 4 tet, 5 hex, 6 prism */
     int32_t n_elem;
     int32_t n_geo;              /* corner (order-1 geometry) nodes per element */
     const int32_t* geo_nodes;   /* [n_elem][n_geo] node numbers */
