@@ -1563,6 +1563,19 @@ slx_status slx_plan_solve(slx_plan p, int32_t method, double rel_tol, int32_t ma
         SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
         HIPCHECK(hipSetDevice(p->ctx->device));
         const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
+        if (nf == 0) {
+            // every DOF constrained: the solution is the Dirichlet values, which
+            // the assembled rhs holds at the constrained entries
+            if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+            if (d_x_out && nd)
+                HIPCHECK(hipMemcpyAsync(d_x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+            HIPCHECK(hipStreamSynchronize(p->stream));
+            if (outer_its) *outer_its = 0;
+            if (inner_its) *inner_its = 0;
+            if (rel_residual) *rel_residual = 0.0;
+            if (method_used) *method_used = method == SLX_SOLVE_AUTO ? SLX_SOLVE_CG : method;
+            return;
+        }
         const SolveSystem sys = solve_view(p);
         if (method == SLX_SOLVE_AUTO) {
             DBuf flag;

@@ -113,6 +113,21 @@ def test_prism3d_f16split_mfma(ctx, monkeypatch):
     assert plan.kernels()["mfma_buckets"] >= 1, "MFMA kernel not used"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "f16split"])
+def test_every_dof_constrained(ctx, precision):
+    # one square, two triangles, all four nodes on the Dirichlet boundary: no
+    # free DOF, empty A and D; the element kernels still run, the reduction has
+    # no rows, and the solve returns the Dirichlet values
+    prob, plan, got, errs = run_case(ctx, "poisson2d", "n=1", precision, TOL_FP64)
+    info = prob.info()
+    assert info["nfree"] == 0 and len(got.Aval) == 0 and len(got.Dval) == 0
+    assert np.array_equal(got.Arow, [0]) and np.array_equal(got.Drow, [0])
+    x, si = plan.solve(info["ndofs"])
+    assert np.array_equal(x, got.rhs) and si["inner"] == 0
+    y = plan.matvec(np.arange(info["ndofs"], dtype=np.float64))
+    assert np.array_equal(y, np.arange(info["ndofs"], dtype=np.float64))
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32)])
 def test_hp_hex_long_segments(ctx, precision, tol):
     # per-element orders 1..5 on hexahedra: element rows up to 216 entries
