@@ -335,7 +335,7 @@ def cpu_baseline(sa, args):
     # sample size knob, its start, cap and how the element count grows with it
     knob = {"elasticity3d": ("n", 4, 20, 3), "hppoisson3d": ("n", 3, 20, 3), "poisson2d": ("n", 20, 1200, 2),
             "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 64, 1),
-            "prism3d": ("n", 4, 60, 3)}[args.problem]
+            "prism3d": ("n", 4, 110, 3)}[args.problem]
 
     def spec_of(v):
         if args.problem == "elasticity3d":
