@@ -33,6 +33,9 @@ WORKLOADS = [
     ("hppoisson3d", "n=16"),
     # isolates the order-1 hexahedra of the hp workload (a later workload's choice wins)
     ("hppoisson3d", "n=40;pmin=1;pmax=1"),
+    ("prism3d", "n=40"),
+    ("prism3d", "n=20;order=2"),
+    ("prism3d", "n=16;order=3"),
 ]
 CANDIDATES = [("mfma", 1), ("mfma", 2), ("factor", 1), ("valu", 1)]
 
@@ -97,6 +100,7 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "kernel_table.txt"))
     ap.add_argument("--iters", type=int, default=5)
     ap.add_argument("--child", nargs=2)
+    ap.add_argument("--only", default="", help="tune only the workloads of this kind")
     args = ap.parse_args()
     if args.child:
         child(args.child[0], args.child[1], args.iters)
@@ -104,6 +108,8 @@ def main():
     best = {}
     log = []
     for kind, spec in WORKLOADS:
+        if args.only and kind != args.only:
+            continue
         cls = classes_of(kind, spec)
         results = []
         for kern, mtw in CANDIDATES:

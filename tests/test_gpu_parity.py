@@ -97,7 +97,7 @@ def test_f16split_qvarying_buckets_on_valu(ctx):
     print("errors", errs)
 
 
-@pytest.mark.parametrize("spec", ["n=5;jitter=0.15", "n=3;order=3", "n=2;order=5"])
+@pytest.mark.parametrize("spec", ["n=5;jitter=0.15", "n=3;order=2", "n=3;order=3", "n=2;order=5"])
 @pytest.mark.parametrize("precision,tol", [("fp64", TOL_FP64), ("fp32", TOL_FP32), ("f16split", TOL_SPLIT)])
 def test_prism3d(ctx, spec, precision, tol):
     # prisms: Gauss-point-varying Jacobians, stiffness + mass + rhs; order 3 has
