@@ -14,6 +14,7 @@
 #include "coefjit.h"
 #include <hip/hiprtc.h>
 #include <cstdio>
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -29,19 +30,41 @@ std::map<std::string, hipFunction_t> g_cache;  // key: device id + source
 
 }  // namespace
 
-std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64) {
+std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool runs) {
     std::ostringstream o;
     o << "namespace slx {\ntypedef " << (fp64 ? "double" : "float") << " T;\n";
-    o << "extern \"C\" __global__ void __launch_bounds__(256) slx_coef(DevBucket b, DevInputs in, int nq, int qfast, T* __restrict__ ctab) {\n"
-         "    const int64_t total = (int64_t)b.n_elem * nq;\n"
-         "    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
-         "    if (idx >= total) return;\n"
-         "    const int e = qfast ? (int)(idx / nq) : (int)(idx % b.n_elem);\n"
-         "    const int q = qfast ? (int)(idx % nq) : (int)(idx / b.n_elem);\n"
-         "    Geo g;\n"
-         "    geometry(b, in.xyz, e, q, g);\n"
-         "    const int64_t stride = (int64_t)nq * b.n_elem;\n"
-         "    const size_t oi = qfast ? (size_t)e * nq + q : (size_t)q * b.n_elem + e;\n"
+    // runs: one thread per (element, run of qc Gauss points).  Gauss point
+    // fastest layout (qfast): qc = 1 and consecutive threads take consecutive
+    // points of one element.  Element fastest layout: consecutive threads take
+    // consecutive elements at the same points, each thread loads its element's
+    // nodes once and walks its qc points.  Without runs (one point per element
+    // or per thread): the (element, Gauss point) of the thread, qc unused.
+    o << "extern \"C\" __global__ void __launch_bounds__(256) slx_coef(DevBucket b, DevInputs in, int nq, int qfast, int qc, T* __restrict__ ctab) {\n";
+    if (runs)
+        o << "    const int nch = qfast ? nq : (nq + qc - 1) / qc;\n"
+             "    const int64_t total = (int64_t)b.n_elem * nch;\n"
+             "    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
+             "    if (idx >= total) return;\n"
+             "    const int e = qfast ? (int)(idx / nq) : (int)(idx % b.n_elem);\n"
+             "    const int q0 = qfast ? (int)(idx % nq) : (int)(idx / b.n_elem) * qc;\n"
+             "    const int q1 = qfast ? q0 + 1 : min(nq, q0 + qc);\n"
+             "    GeoX gx;\n"
+             "    geo_load(b, in.xyz, e, gx);\n"
+             "    const int64_t stride = (int64_t)nq * b.n_elem;\n"
+             "    for (int q = q0; q < q1; q++) {\n"
+             "    Geo g;\n"
+             "    geo_eval(b, gx, q, g);\n";
+    else
+        o << "    const int64_t total = (int64_t)b.n_elem * nq;\n"
+             "    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;\n"
+             "    if (idx >= total) return;\n"
+             "    const int e = qfast ? (int)(idx / nq) : (int)(idx % b.n_elem);\n"
+             "    const int q = qfast ? (int)(idx % nq) : (int)(idx / b.n_elem);\n"
+             "    const int64_t stride = (int64_t)nq * b.n_elem;\n"
+             "    {\n"
+             "    Geo g;\n"
+             "    geometry(b, in.xyz, e, q, g);\n";
+    o << "    const size_t oi = qfast ? (size_t)e * nq + q : (size_t)q * b.n_elem + e;\n"
          "    const T adet = (T)fabs(g.det);\n"
          "    {\n"
          "#pragma clang fp contract(off)\n";
@@ -82,18 +105,18 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64) {
         o << "        const T " << v << " = " << rhs << ";\n";
         if (x.dst >= 0 && x.dst < SLX_MAX_REGS) cur[x.dst] = v;
     }
-    o << "    }\n}\n}  // namespace slx\n";
+    o << "    }\n    }\n}\n}  // namespace slx\n";
     return std::string(kJitPrelude) + o.str();
 }
 
-bool coef_compile(const slx_instr* prog, int n_instr, bool fp64, hipFunction_t* fn, std::string* err) {
+bool coef_compile(const slx_instr* prog, int n_instr, bool fp64, bool runs, hipFunction_t* fn, std::string* err) {
     *fn = nullptr;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
         if (err) *err = "no current device";
         return false;
     }
-    const std::string src = coef_kernel_source(prog, n_instr, fp64);
+    const std::string src = coef_kernel_source(prog, n_instr, fp64, runs);
     const std::string key = std::to_string(dev) + "\n" + src;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -150,11 +173,19 @@ bool coef_compile(const slx_instr* prog, int n_instr, bool fp64, hipFunction_t* 
 }
 
 hipError_t coef_launch(hipFunction_t fn, const DevBucket& b, const DevInputs& in, int nq, int qfast, void* ctab, hipStream_t st) {
-    const int64_t total = (int64_t)b.n_elem * nq;
-    if (total == 0 || b.n_coefs == 0) return hipSuccess;
+    const int64_t pairs = (int64_t)b.n_elem * nq;
+    if (pairs == 0 || b.n_coefs == 0) return hipSuccess;
+    // element-fastest layout: runs of qc points per thread, as long as about
+    // 256K threads remain to fill the GPU
+    // (a kernel compiled without runs has nq = 1 here, so qc = 1 and the grid
+    // is one thread per element)
+    int qc = 1;
+    if (!qfast) qc = (int)std::max<int64_t>(1, std::min<int64_t>(nq, pairs / (256 * 1024)));
+    const int64_t nch = qfast ? nq : (nq + qc - 1) / qc;
+    const int64_t total = (int64_t)b.n_elem * nch;
     DevBucket bb = b;
     DevInputs ii = in;
-    void* args[] = {&bb, &ii, &nq, &qfast, &ctab};
+    void* args[] = {&bb, &ii, &nq, &qfast, &qc, &ctab};
     return hipModuleLaunchKernel(fn, (unsigned)((total + 255) / 256), 1, 1, 256, 1, 1, 0, st, args, nullptr);
 }
 

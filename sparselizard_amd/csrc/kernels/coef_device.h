@@ -29,21 +29,30 @@ __device__ inline double pick9(const double* v, int i) {
     return 0.0;
 }
 
-// J[r][c] = sum_k x_k^c dN_k/dki_r (rawfield.cpp:2442-2502); det and inverse
-// as jacobian.cpp:308-349, 396-505.
-__device__ inline void geometry(const DevBucket& b, const double* xyz, int e, int q, Geo& g) {
+// Corner-node coordinates of one element, loaded once and reused for every
+// Gauss point the thread evaluates.
+struct GeoX {
     double x[SLX_MAX_GEO][3];
+};
+
+__device__ inline void geo_load(const DevBucket& b, const double* xyz, int e, GeoX& gx) {
 #pragma unroll
     for (int k = 0; k < SLX_MAX_GEO; k++) {
         if (k < b.n_geo) {
             const int node = b.geo_nodes[(size_t)e * b.n_geo + k];
-            x[k][0] = xyz[3 * (size_t)node];
-            x[k][1] = xyz[3 * (size_t)node + 1];
-            x[k][2] = xyz[3 * (size_t)node + 2];
+            gx.x[k][0] = xyz[3 * (size_t)node];
+            gx.x[k][1] = xyz[3 * (size_t)node + 1];
+            gx.x[k][2] = xyz[3 * (size_t)node + 2];
         } else {
-            x[k][0] = x[k][1] = x[k][2] = 0.0;
+            gx.x[k][0] = gx.x[k][1] = gx.x[k][2] = 0.0;
         }
     }
+}
+
+// J[r][c] = sum_k x_k^c dN_k/dki_r (rawfield.cpp:2442-2502); det and inverse
+// as jacobian.cpp:308-349, 396-505.
+__device__ inline void geo_eval(const DevBucket& b, const GeoX& gx, int q, Geo& g) {
+    const auto& x = gx.x;
 #pragma unroll
     for (int r = 0; r < 3; r++) {
         double dn[SLX_MAX_GEO];
@@ -83,6 +92,12 @@ __device__ inline void geometry(const DevBucket& b, const double* xyz, int e, in
         g.inv[7] = (J[1] * J[6] - J[0] * J[7]) / d;
         g.inv[8] = (J[0] * J[4] - J[1] * J[3]) / d;
     }
+}
+
+__device__ inline void geometry(const DevBucket& b, const double* xyz, int e, int q, Geo& g) {
+    GeoX gx;
+    geo_load(b, xyz, e, gx);
+    geo_eval(b, gx, q, g);
 }
 
 // Value of field f (space derivative sd) at Gauss point q of element e

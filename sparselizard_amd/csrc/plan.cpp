@@ -1194,7 +1194,7 @@ void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d) {
                 const slx_bucket_desc& bd = d.buckets[todo[i].first];
                 hipFunction_t fn = nullptr;
                 std::string err;
-                if (coef_compile(bd.prog, bd.n_instr, todo[i].second != 0, &fn, &err)) {
+                if (coef_compile(bd.prog, bd.n_instr, todo[i].second != 0, bd.q_invariant == 0, &fn, &err)) {
                     P.coef_fn[todo[i].first][todo[i].second] = fn;
                 } else {
                     std::lock_guard<std::mutex> lk(mu);
