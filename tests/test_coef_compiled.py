@@ -15,7 +15,8 @@ from oracle import py_oracle
 pytestmark = pytest.mark.gpu
 
 CASES = [("elasticity3d", "n=3"), ("newton2d", "n=6"), ("projection", "elem=hex;n=3;order=2"),
-         ("projection", "elem=quad;n=4;order=3"), ("hppoisson3d", "n=2"), ("quadbatch", "instances=3;nmin=2;nmax=5")]
+         ("projection", "elem=quad;n=4;order=3"), ("hppoisson3d", "n=2"), ("quadbatch", "instances=3;nmin=2;nmax=5"),
+         ("prism3d", "n=3;order=2")]
 
 
 def relerr(a, b):
