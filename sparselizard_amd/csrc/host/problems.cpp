@@ -410,18 +410,28 @@ void grid_mesh(Mesh& m, const std::string& elem, int n, double jit, Rng& rng, in
 // deg <= order the projection reproduces f exactly, which pins bases,
 // orientations, numbering and assembly together (the validation of
 // examples/high-order-interpolations-2d/main.cpp:68 in exact form).
+// pmax > order: per-element orders uniform in [order, pmax] (one physical
+// region per order, set in descending order so that interfaces take the
+// lower order, as in the hp workload); f of degree <= order stays exact.
 std::shared_ptr<Problem> projection(const std::map<std::string, std::string>& s) {
     auto p = std::make_shared<Problem>();
     std::string elem = s.count("elem") ? s.at("elem") : "tri";
     int n = geti(s, "n", 4), order = geti(s, "order", 3), deg = geti(s, "deg", order);
+    const int pmax = geti(s, "pmax", order);
     double jit = getd(s, "jitter", (elem == "tri" || elem == "tet") ? 0.2 : 0.0);
     Rng rng((uint64_t)geti(s, "seed", 7));
     p->mesh = std::make_shared<Mesh>();
     Mesh& m = *p->mesh;
     grid_mesh(m, elem, n, jit, rng, 1, -1);
+    const int vtype = elem == "tri" ? TRI : elem == "quad" ? QUAD : elem == "tet" ? TET : elem == "hex" ? HEX : PRISM;
+    if (pmax > order)
+        for (int e = 0; e < m.count(vtype); e++) m.add_to_physreg(vtype, e, 100 + rng.uniform_int(order, pmax));
     m.finalize();
     field u(m, "h1", "u");
     u.setorder(1, order);
+    for (int q = pmax; q > order; q--)
+        if (m.has_physreg(100 + q)) u.setorder(100 + q, q);
+    if (pmax > order && m.has_physreg(100 + order)) u.setorder(100 + order, order);
     p->fields = {u};
     p->form = std::make_shared<formulation>(m);
     using namespace sl;
