@@ -39,11 +39,19 @@ typedef int32_t slx_status;
 #define SLX_ERR_UNSUPPORTED -3 /* feature outside the implemented subset */
 #define SLX_ERR_NOMEM -4       /* device or host allocation failed */
 
-/* Precision of the numeric phase. */
+/* Precision of the numeric phase.  The three MFMA modes run the matrix
+ * slices of every bucket that has an MFMA kernel variant (batched kernel for
+ * element matrices up to 256 entries with coefficients constant over the
+ * Gauss points, factorized kernel for larger ones; the checked-in kernel
+ * table may send a class to the FP32 VALU kernel instead) on MFMA with FP32
+ * accumulation, and everything else on the FP32 VALU element kernel; the
+ * CSR reduction is FP64 in every mode.  The FP16 split also takes the batched
+ * kernel for Gauss-point-varying coefficients when the table says so. */
 #define SLX_PREC_FP64 0        /* plain FP64 element kernel (FP64 everywhere) */
-#define SLX_PREC_F16_SPLIT 1   /* FP16 hi/lo split MFMA, FP32 accumulation, FP64 reduction */
-#define SLX_PREC_BF16_SPLIT 2  /* BF16 hi/mid/lo split MFMA, FP32 accumulation, FP64 reduction */
+#define SLX_PREC_F16_SPLIT 1   /* FP16 hi/lo split MFMA (3 products), FP32 accumulation, FP64 reduction */
+#define SLX_PREC_BF16_SPLIT 2  /* BF16 hi/mid/lo split MFMA (6 products), FP32 accumulation, FP64 reduction */
 #define SLX_PREC_FP32 3        /* FP32 VALU element kernel, FP64 reduction */
+#define SLX_PREC_FP32_MFMA 4   /* FP32 MFMA (v_mfma_f32_16x16x4_f32), FP64 reduction */
 
 /* Coefficient program opcodes (one instruction = {op, dst, a, b, imm}). */
 enum slx_opcode {
@@ -211,6 +219,10 @@ slx_status slx_plan_stats(slx_plan plan, int64_t* n_elem_rows, int64_t* n_ke, in
 /* Buckets whose matrix slices run on the MFMA split kernel (the others and
  * every right-hand-side slice use the VALU element kernel). */
 slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_buckets);
+/* For one precision mode: buckets whose matrix slices run on the batched
+ * MFMA kernel, and slices on the factorized MFMA kernel (0 and 0 for the
+ * VALU modes). */
+slx_status slx_plan_kernel_counts(slx_plan plan, int32_t precision, int32_t* n_batched, int32_t* n_factorized);
 /* Solve of the last assembled system on the device, replacing
  * sl::solve(A, b) (src/expression/operation/sl.cpp:1565-1627): the
  * Dirichlet elimination b_A - D x_D of mat::eliminate
@@ -281,6 +293,10 @@ slx_status slx_plan_reduction_maps(slx_plan plan, int64_t* sizes, int64_t* erow_
  * the 16x16 result (row-major) so the operand lane layout the kernels assume
  * can be checked against a host product. */
 slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256);
+/* The same for kind 1 (v_mfma_f32_16x16x32_bf16), 2 (v_mfma_f32_16x16x4_f32)
+ * and 3 (v_mfma_f64_16x16x4_f64) on A[m][k] = (m + 2k) % 7 - 3,
+ * B[k][n] = (3k + n) % 5 - 2 (kind 0 is slx_selftest_mfma_layout). */
+slx_status slx_selftest_mfma_layout_kind(slx_ctx ctx, int32_t kind, float* out256);
 /* Test hook: the streaming FP32 CSR product of slx_plan_solve's inner
  * iteration on a host CSR (square, nrows rows; values and x rounded to FP32):
  * y = A x and xy = x . y (FP64 sum of the FP32 products). */

@@ -4,8 +4,10 @@ The library is built in-tree (sparselizard_amd/lib/libslx.so).  Loading
 fails loudly when it is missing: there is no CPU fallback for the assembly
 path.
 """
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -17,7 +19,9 @@ PREC_FP64 = 0
 PREC_F16_SPLIT = 1
 PREC_BF16_SPLIT = 2
 PREC_FP32 = 3
-PRECISIONS = {"fp64": PREC_FP64, "f16split": PREC_F16_SPLIT, "bf16split": PREC_BF16_SPLIT, "fp32": PREC_FP32}
+PREC_FP32_MFMA = 4
+PRECISIONS = {"fp64": PREC_FP64, "f16split": PREC_F16_SPLIT, "bf16split": PREC_BF16_SPLIT, "fp32": PREC_FP32,
+              "fp32mfma": PREC_FP32_MFMA}
 
 
 class NumericDesc(C.Structure):
@@ -102,6 +106,7 @@ def lib():
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
+        "slx_plan_kernel_counts": (i32, [vp, i32, p32, p32]),
         "slx_plan_solve": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
         "slx_plan_eliminate": (i32, [vp, pd, pd]),
         "slx_plan_coef_mode": (i32, [vp, i32, p32]),
@@ -123,6 +128,7 @@ def lib():
         "slx_plan_graph_launch": (i32, [vp, vp]),
         "slx_gauss_rule": (i32, [i32, i32, i32, p32, pd, pd]),
         "slx_selftest_mfma_layout": (i32, [vp, C.POINTER(C.c_float)]),
+        "slx_selftest_mfma_layout_kind": (i32, [vp, i32, C.POINTER(C.c_float)]),
         "slx_h1_count": (i32, [i32, i32, p32]),
         "slx_h1_tables": (i32, [i32, i32, i32, i32, pd, pd]),
     }
@@ -137,17 +143,23 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
-    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_spmv_stream",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
+    "slx_selftest_spmv_stream",
 ]
 
 
-def mfma_layout_probe(ctx):
-    """16x16 result of the MFMA layout probe (see slx_selftest_mfma_layout)."""
+def mfma_layout_probe(ctx, kind=0):
+    """16x16 result of the MFMA layout probe: kind 0 f16 16x16x32
+    (slx_selftest_mfma_layout), 1 bf16 16x16x32, 2 f32 16x16x4, 3 f64 16x16x4
+    (slx_selftest_mfma_layout_kind)."""
     out = np.zeros(256, np.float32)
-    check(lib().slx_selftest_mfma_layout(ctx.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+    if kind == 0:
+        check(lib().slx_selftest_mfma_layout(ctx.h, out.ctypes.data_as(C.POINTER(C.c_float))))
+    else:
+        check(lib().slx_selftest_mfma_layout_kind(ctx.h, kind, out.ctypes.data_as(C.POINTER(C.c_float))))
     return out.reshape(16, 16)
 
 
@@ -199,12 +211,40 @@ def _p(arr, ctype):
     return arr.ctypes.data_as(C.POINTER(ctype))
 
 
+# Live handles, destroyed by _shutdown while the interpreter and the HIP
+# runtime are both still intact: Python runs atexit handlers before it tears
+# down modules and before the process runs the runtime's own exit handlers.
+# Handles that are still alive after that (objects freed late in interpreter
+# finalization, e.g. through a failed test's traceback) are left to the
+# process exit instead of being destroyed into a runtime that may be gone.
+_live = {"plan": weakref.WeakSet(), "problem": weakref.WeakSet(), "ctx": weakref.WeakSet()}
+_shut_down = False
+
+
+def _shutdown():
+    global _shut_down
+    if _shut_down:
+        return
+    # plans and problems hold device memory and a context reference: first
+    for kind in ("plan", "problem", "ctx"):
+        for obj in list(_live[kind]):
+            try:
+                obj.close()
+            except Exception:
+                pass
+    _shut_down = True
+
+
+atexit.register(_shutdown)
+
+
 class Context:
     """One device + stream (slx_create)."""
 
     def __init__(self, device=0):
         self.h = C.c_void_p()
         check(lib().slx_create(device, C.byref(self.h)))
+        _live["ctx"].add(self)
 
     def close(self):
         if self.h:
@@ -212,6 +252,8 @@ class Context:
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _shut_down:
+            return
         try:
             self.close()
         except Exception:
@@ -239,6 +281,7 @@ class Problem:
         self.h = C.c_void_p()
         check(lib().slx_problem_create(ctx.h if ctx else None, kind.encode(), spec.encode(), C.byref(self.h)))
         self._plan = None
+        _live["problem"].add(self)
 
     def info(self):
         v = [C.c_int64() for _ in range(4)]
@@ -294,10 +337,14 @@ class Problem:
 
     def close(self):
         if self.h:
+            if self._plan is not None:
+                self._plan.h = None  # owned by the problem: destroyed with it
             lib().slx_problem_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
+        if _shut_down:
+            return
         try:
             self.close()
         except Exception:
@@ -308,6 +355,8 @@ class Plan:
     def __init__(self, h, owned=True):
         self.h = h
         self.owned = owned
+        if owned:
+            _live["plan"].add(self)
 
     def shape(self):
         v = [C.c_int64() for _ in range(4)]
@@ -323,6 +372,13 @@ class Plan:
         a, b = C.c_int32(), C.c_int32()
         check(lib().slx_plan_kernels(self.h, C.byref(a), C.byref(b)))
         return dict(mfma_buckets=a.value, buckets=b.value)
+
+    def kernel_counts(self, precision):
+        """(batched MFMA buckets, factorized MFMA slices) of a precision mode."""
+        a, b = C.c_int32(), C.c_int32()
+        check(lib().slx_plan_kernel_counts(self.h, PRECISIONS[precision] if isinstance(precision, str) else precision,
+                                           C.byref(a), C.byref(b)))
+        return dict(batched=a.value, factorized=b.value)
 
     def element_matrices(self):
         n_ke = self.stats()["n_ke"]
@@ -454,3 +510,11 @@ class Plan:
         if self.owned and self.h:
             lib().slx_plan_destroy(self.h)
         self.h = None
+
+    def __del__(self):
+        if _shut_down:
+            return
+        try:
+            self.close()
+        except Exception:
+            pass

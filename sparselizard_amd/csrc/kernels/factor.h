@@ -5,8 +5,9 @@
 // which is the reference's sum over terms and Gauss points of
 // tf_t[i][q] w_q C_t(e,q) dof_t[j][q] (contribution.cpp:172-246) regrouped by
 // tf derivative.  P is shared by the elements of a bucket (orientation-0
-// tables), Q_e is built per element; both run on FP16 MFMA with hi/lo split
-// operands and FP32 accumulation.
+// tables), Q_e is built per element; both run on MFMA with FP32 accumulation
+// in one of the arithmetics of kernels/mfma.h (FP16 hi/lo split, BF16
+// hi/mid/lo split, plain FP32).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device.h"
@@ -14,7 +15,7 @@
 #define SLX_FK_WAVES 8     // consumer (MFMA) waves
 #define SLX_FK_BWAVES 4    // producer waves: 256 threads, one per padded dof column
 #define SLX_FK_RU 16       // Gauss points per k-step at most
-#define SLX_FK_AREG 8      // P chunk words per producer thread (mt * 128 <= 8 * 256)
+#define SLX_FK_AREG 11     // P chunk words per producer thread (mt * planes * 64 <= 11 * 256)
 #define SLX_FK_MAXT 16     // terms of a slice
 
 namespace slx {
@@ -30,14 +31,18 @@ struct FactorSlice {
     const int32_t* tinfo;
     float a_unscale;                // 1 / power-of-two scale of P
     float tqmax;                    // max |tq|
-    const uint32_t* afrag;          // [nks][mt][2][64][4] words: f16 hi/lo fragments of the scaled P
+    int32_t arith;                  // MfmaArith of this launch
+    const uint32_t* afrag;          // [nks][mt][planes][64][4] words: fragments of the scaled P in this launch's arithmetic
     const float* tq;                // [ngp][ntp][4]: w_q dof_b[j][q], zero padded columns
     int64_t ke_base, ke_ld;
+    const uint32_t* afrag_x[3];     // host side: the fragments of every arithmetic
+    uint32_t arith_ok;              // bit a: arithmetic a fits (LDS, P chunk staging)
 };
 
-size_t factor_lds_bytes(const FactorSlice& f);
+size_t factor_lds_bytes(const FactorSlice& f, int arith);
 bool factor_variant_fits(int mt, int nt, int* bm, int* bn);
-hipError_t launch_factor_f16split(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu,
-                                  hipStream_t st);
+// Contraction of one slice in arithmetic `arith` (MfmaArith).
+hipError_t launch_factor(const DevBucket& b, const FactorSlice& f, int arith, const float* ctab, float* ke, int n_cu,
+                         hipStream_t st);
 
 }  // namespace slx

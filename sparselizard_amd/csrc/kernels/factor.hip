@@ -1,4 +1,4 @@
-// Factorized element contraction on FP16 MFMA with hi/lo split operands.
+// Factorized element contraction on MFMA with FP32 accumulation.
 //
 // One workgroup per element at a time, persistent over the bucket: 8 consumer
 // waves (a 2 x 4 grid over the 16x16 output tiles) run the MFMAs while 4
@@ -6,25 +6,32 @@
 // The K dimension runs over (Gauss point, tf derivative) pairs in k-steps of
 // 32: a k-step holds qb Gauss points times the na tf derivatives, so the dof
 // table values of a Gauss point are read once per element.  Per k-step:
-//   A chunk (shared P fragments, precomputed hi/lo f16) -> LDS
+//   A chunk (shared P fragments, precomputed on the host) -> LDS
 //   B chunk (Q_e for the element, built from the element's coefficients in
-//            LDS and the dof table, split into hi/lo f16) -> LDS
-// both double buffered, and every wave runs its block of 16x16 output tiles:
-//   acc += Ahi*Bhi + Ahi*Blo + Alo*Bhi        (v_mfma_f32_16x16x32_f16)
+//            LDS and the dof table) -> LDS
+// both double buffered, and every wave runs its block of 16x16 output tiles
+// in the arithmetic of the launch (kernels/mfma.h):
+//   AR_F16   acc += Ahi*Bhi + Ahi*Blo + Alo*Bhi           (v_mfma_f32_16x16x32_f16)
+//   AR_BF16  acc += Ah*Bh + Ah*Bm + Am*Bh + Ah*Bl + Am*Bm + Al*Bh
+//                                                      (v_mfma_f32_16x16x32_bf16)
+//   AR_F32   acc += A*B, eight 16x16x4 steps per k-step    (v_mfma_f32_16x16x4_f32)
 // P is scaled on the host and Q_e per element by powers of two so that the
-// largest magnitudes sit in [128, 256): the lo parts of every entry above
-// 2^-10 of the maximum stay normal f16 numbers.  The accumulators are
-// unscaled once when the element matrix is stored.
+// largest magnitudes sit in [128, 256): in the FP16 split the lo parts of
+// every entry above 2^-10 of the maximum stay normal f16 numbers.  The
+// accumulators are unscaled once when the element matrix is stored.
 #include "factor.h"
+#include "mfma.h"
 
 namespace slx {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 
-size_t factor_lds_bytes(const FactorSlice& f) {
-    return (size_t)2 * f.mt * 128 * 16 + (size_t)2 * (4 * f.bn) * 128 * 16 + (size_t)f.ngp * 16 * 4 + 16;
+size_t factor_lds_bytes(const FactorSlice& f, int arith) {
+    const size_t npl = (size_t)mfma_planes(arith);
+    return (size_t)2 * f.mt * npl * 64 * 16 + (size_t)2 * (4 * f.bn) * npl * 64 * 16 + (size_t)f.ngp * 16 * 4 + 16;
 }
 
 // tile blocks per wave: rows over 2 waves, columns over 4 waves; the column
@@ -42,15 +49,21 @@ bool factor_variant_fits(int mt, int nt, int* bm, int* bn) {
     return false;
 }
 
-template <int BM, int BN>
+// LDS per k-step and buffer: A chunk [MT][NPL][64] 16-byte words; B chunk
+// [NTB][NPL][64] words (F16 / BF16: lane l of column tile t holds the eight
+// k = 8 (l / 16) + 0..7 of column l % 16, one plane per piece; F32: the tile
+// as floats Q[k][column], [32][16]).
+template <int AR, int BM, int BN>
 __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
-    factor_f16split_kernel(int n_elem, FactorSlice f, const float* __restrict__ ctab, float* __restrict__ ke) {
+    factor_contract_kernel(int n_elem, FactorSlice f, const float* __restrict__ ctab, float* __restrict__ ke) {
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    constexpr int NPL = AR == AR_BF16 ? 3 : 2;                  // 16-byte planes per lane
     const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb, ntp = f.ntp;
     constexpr int NTB = 4 * BN;                                 // column tiles per pass
-    uint4v* As = reinterpret_cast<uint4v*>(smem_raw);           // [2][MT][2][64]
-    uint4v* Bs = As + 2 * MT * 128;                             // [2][NTB][2][64]
-    float* Ms = reinterpret_cast<float*>(Bs + 2 * NTB * 128);   // [ngp][4 tf derivative][4 dof derivative]
+    constexpr int BCH = NTB * NPL * 64;                         // B chunk words
+    uint4v* As = reinterpret_cast<uint4v*>(smem_raw);           // [2][MT][NPL][64]
+    uint4v* Bs = As + 2 * MT * NPL * 64;                        // [2][NTB][NPL][64]
+    float* Ms = reinterpret_cast<float*>(Bs + 2 * BCH);         // [ngp][4 tf derivative][4 dof derivative]
     unsigned* smax = reinterpret_cast<unsigned*>(Ms + ngp * 16);
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool mfma_wave = w < SLX_FK_WAVES;
@@ -59,7 +72,7 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
     const int32_t* t_slot = f.tinfo + 2 * f.n_terms;
     const int64_t stride = (int64_t)nq * n_elem;
     const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
-    const int achunk = MT * 128;
+    const int achunk = MT * NPL * 64;
     const int npass = (NT + NTB - 1) / NTB;
 
     for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
@@ -116,28 +129,73 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
                 for (int ks = 0; ks < f.nks; ks++) {
                     const int cur = ks & 1;
                     const uint4v* Ac = As + cur * achunk;
-                    const uint4v* Bc = Bs + cur * NTB * 128;
-                    uint4v bh[BN], bl[BN];
+                    const uint4v* Bc = Bs + cur * BCH;
+                    if constexpr (AR == AR_F32) {
+                        // step j (k = 4 j + lane / 16): B lane value Q[4 j + lane / 16][lane % 16]
+                        const float* Bf = reinterpret_cast<const float*>(Bc);
+                        float bq[BN][8];
 #pragma unroll
-                    for (int nj = 0; nj < BN; nj++) {
-                        const int jl = wn + 4 * nj;
-                        bh[nj] = Bc[jl * 128 + lane];
-                        bl[nj] = Bc[jl * 128 + 64 + lane];
-                    }
+                        for (int nj = 0; nj < BN; nj++) {
+                            const int jl = wn + 4 * nj;
 #pragma unroll
-                    for (int mi = 0; mi < BM; mi++) {
-                        const int mt = wm + 2 * mi;
-                        if (mt < MT) {
-                            const half8 ah = __builtin_bit_cast(half8, Ac[mt * 128 + lane]);
-                            const half8 al = __builtin_bit_cast(half8, Ac[mt * 128 + 64 + lane]);
+                            for (int j = 0; j < 8; j++) bq[nj][j] = Bf[jl * 512 + 64 * j + lane];
+                        }
 #pragma unroll
-                            for (int nj = 0; nj < BN; nj++) {
-                                if (cb + wn + 4 * nj < NT) {
-                                    const half8 xh = __builtin_bit_cast(half8, bh[nj]);
-                                    const half8 xl = __builtin_bit_cast(half8, bl[nj]);
-                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh, acc[mi][nj], 0, 0, 0);
-                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl, acc[mi][nj], 0, 0, 0);
-                                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh, acc[mi][nj], 0, 0, 0);
+                        for (int mi = 0; mi < BM; mi++) {
+                            const int mt = wm + 2 * mi;
+                            if (mt < MT) {
+                                const float4v a0 = __builtin_bit_cast(float4v, Ac[(mt * NPL) * 64 + lane]);
+                                const float4v a1 = __builtin_bit_cast(float4v, Ac[(mt * NPL + 1) * 64 + lane]);
+#pragma unroll
+                                for (int nj = 0; nj < BN; nj++) {
+                                    if (cb + wn + 4 * nj < NT) {
+#pragma unroll
+                                        for (int j = 0; j < 4; j++)
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[j], bq[nj][j], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+                                        for (int j = 0; j < 4; j++)
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[j], bq[nj][4 + j], acc[mi][nj], 0, 0, 0);
+                                    }
+                                }
+                            }
+                        }
+                    } else {
+                        uint4v bp[BN][NPL];
+#pragma unroll
+                        for (int nj = 0; nj < BN; nj++) {
+                            const int jl = wn + 4 * nj;
+#pragma unroll
+                            for (int pl = 0; pl < NPL; pl++) bp[nj][pl] = Bc[(jl * NPL + pl) * 64 + lane];
+                        }
+#pragma unroll
+                        for (int mi = 0; mi < BM; mi++) {
+                            const int mt = wm + 2 * mi;
+                            if (mt < MT) {
+                                uint4v ap[NPL];
+#pragma unroll
+                                for (int pl = 0; pl < NPL; pl++) ap[pl] = Ac[(mt * NPL + pl) * 64 + lane];
+#pragma unroll
+                                for (int nj = 0; nj < BN; nj++) {
+                                    if (cb + wn + 4 * nj < NT) {
+                                        if constexpr (AR == AR_F16) {
+                                            const half8 ah = __builtin_bit_cast(half8, ap[0]), al = __builtin_bit_cast(half8, ap[1]);
+                                            const half8 xh = __builtin_bit_cast(half8, bp[nj][0]), xl = __builtin_bit_cast(half8, bp[nj][1]);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh, acc[mi][nj], 0, 0, 0);
+                                        } else {
+                                            const bf16x8 ah = __builtin_bit_cast(bf16x8, ap[0]), am = __builtin_bit_cast(bf16x8, ap[1]),
+                                                         al = __builtin_bit_cast(bf16x8, ap[2]);
+                                            const bf16x8 xh = __builtin_bit_cast(bf16x8, bp[nj][0]), xm = __builtin_bit_cast(bf16x8, bp[nj][1]),
+                                                         xl = __builtin_bit_cast(bf16x8, bp[nj][2]);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xh, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xm, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xh, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xl, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xm, acc[mi][nj], 0, 0, 0);
+                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, xh, acc[mi][nj], 0, 0, 0);
+                                        }
+                                    }
                                 }
                             }
                         }
@@ -222,7 +280,8 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
                                                          (__attribute__((address_space(3))) void*)(As + buf * achunk + x0), 16, 0, 0);
                     }
                 }
-                uint4v* colw = Bs + (size_t)(buf * NTB + (bc >> 4)) * 128 + (bc & 15);
+                uint4v* colw = Bs + (size_t)buf * BCH + (size_t)(bc >> 4) * NPL * 64 + (bc & 15);
+                float* colf = reinterpret_cast<float*>(Bs + (size_t)buf * BCH) + (size_t)(bc >> 4) * 512 + (bc & 15);
 #pragma unroll
                 for (int g = 0; g < GP; g++) {
                     const int kg = kg0 + g * NG;
@@ -235,17 +294,40 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
                         const int q = min(ks * qb + qi, ngp - 1);
                         mm[v] = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
                     }
-                    half8 hv, lv;
+                    float vals[8];
 #pragma unroll
                     for (int v = 0; v < 8; v++) {
                         const float4v hh = hq[g][v];
-                        const float val = mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3];
-                        const _Float16 hi = (_Float16)val;
-                        hv[v] = hi;
-                        lv[v] = (_Float16)(val - (float)hi);
+                        vals[v] = mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3];
                     }
-                    colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
-                    colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+                    if constexpr (AR == AR_F16) {
+                        half8 hv, lv;
+#pragma unroll
+                        for (int v = 0; v < 8; v++) {
+                            const _Float16 hi = (_Float16)vals[v];
+                            hv[v] = hi;
+                            lv[v] = (_Float16)(vals[v] - (float)hi);
+                        }
+                        colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
+                        colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+                    } else if constexpr (AR == AR_BF16) {
+                        bf16x8 hv, mv, lv;
+#pragma unroll
+                        for (int v = 0; v < 8; v++) {
+                            const __bf16 hi = (__bf16)vals[v];
+                            const float r1 = vals[v] - (float)hi;
+                            const __bf16 mid = (__bf16)r1;
+                            hv[v] = hi;
+                            mv[v] = mid;
+                            lv[v] = (__bf16)(r1 - (float)mid);
+                        }
+                        colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
+                        colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, mv);
+                        colw[128 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 8; v++) colf[(8 * kg + v) * 16] = vals[v];
+                    }
                 }
                 // the P chunk is in LDS before this wave reaches the barrier; the
                 // next k-step's table values are requested after that wait
@@ -265,23 +347,34 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
     }
 }
 
-template <int BM, int BN>
+template <int AR, int BM, int BN>
 static hipError_t launch_fk(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
-    const size_t lds = factor_lds_bytes(f);
+    const size_t lds = factor_lds_bytes(f, AR);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / lds));
     int64_t grid = std::min<int64_t>(b.n_elem, (int64_t)n_cu * per_cu);
-    hipLaunchKernelGGL((factor_f16split_kernel<BM, BN>), dim3((unsigned)grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, b.n_elem,
+    hipLaunchKernelGGL((factor_contract_kernel<AR, BM, BN>), dim3((unsigned)grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, b.n_elem,
                        f, ctab, ke);
     return hipGetLastError();
 }
 
-hipError_t launch_factor_f16split(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
-    if (b.n_elem == 0) return hipSuccess;
-    if (f.bm == 2 && f.bn == 1) return launch_fk<2, 1>(b, f, ctab, ke, n_cu, st);
-    if (f.bm == 4 && f.bn == 2) return launch_fk<4, 2>(b, f, ctab, ke, n_cu, st);
-    if (f.bm == 7 && f.bn == 2) return launch_fk<7, 2>(b, f, ctab, ke, n_cu, st);
+template <int AR>
+static hipError_t launch_fk_ar(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
+    if (f.bm == 2 && f.bn == 1) return launch_fk<AR, 2, 1>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 4 && f.bn == 2) return launch_fk<AR, 4, 2>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 7 && f.bn == 2) return launch_fk<AR, 7, 2>(b, f, ctab, ke, n_cu, st);
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_factor(const DevBucket& b, const FactorSlice& f0, int arith, const float* ctab, float* ke, int n_cu, hipStream_t st) {
+    if (b.n_elem == 0) return hipSuccess;
+    if (arith < 0 || arith >= AR_COUNT || !((f0.arith_ok >> arith) & 1u) || !f0.afrag_x[arith]) return hipErrorInvalidValue;
+    FactorSlice f = f0;
+    f.arith = arith;
+    f.afrag = f0.afrag_x[arith];
+    if (arith == AR_BF16) return launch_fk_ar<AR_BF16>(b, f, ctab, ke, n_cu, st);
+    if (arith == AR_F32) return launch_fk_ar<AR_F32>(b, f, ctab, ke, n_cu, st);
+    return launch_fk_ar<AR_F16>(b, f, ctab, ke, n_cu, st);
 }
 
 }  // namespace slx

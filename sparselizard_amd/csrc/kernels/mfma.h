@@ -8,6 +8,19 @@
 
 namespace slx {
 
+// MFMA arithmetic of the contraction kernels (FP32 accumulation in all three):
+//   AR_F16   v_mfma_f32_16x16x32_f16, operands split x = hi + 2^-11 lo' (3 products)
+//   AR_BF16  v_mfma_f32_16x16x32_bf16, operands split x = hi + mid + lo (6 products)
+//   AR_F32   v_mfma_f32_16x16x4_f32, plain FP32 operands (1 product)
+enum MfmaArith { AR_F16 = 0, AR_BF16 = 1, AR_F32 = 2, AR_COUNT = 3 };
+
+// 16-byte fragment planes per (M tile, K step of 32, lane) of the A operand:
+// F16 hi, lo'; BF16 hi, mid, lo; F32 k = 4 j + lane / 16 for j = 0..3, 4..7.
+inline int mfma_planes(int arith) { return arith == AR_BF16 ? 3 : 2; }
+// LDS words per derivative pair of a compact column: F16 (hi | lo' << 16),
+// BF16 (hi | mid << 16, lo), F32 the value.
+inline int mfma_words_per_pair(int arith) { return arith == AR_BF16 ? 2 : 1; }
+
 struct MfmaBucket {
     int32_t M, K;               // rows (n_tf * n_dof) and K = n_pairs * kq
     int32_t kq;                 // K entries per derivative pair (n_gp, padded to 8 when compact)
@@ -21,8 +34,9 @@ struct MfmaBucket {
     int32_t sgroup;             // slices per contraction group (divides n_mslices; 1 unless compact, one M tile per wave)
     int32_t cstride, estride;   // LDS words per column / per element (all slices)
     int32_t max_blocks;         // persistent grid size
+    int32_t arith;              // MfmaArith of this launch (mfma_for)
     float a_unscale;            // 2^-kA of the scaled Phi
-    const void* afrag;          // [mtiles][ksteps][2 (hi, lo')][64 lanes] x 8 halfs
+    const void* afrag;          // [mtiles][ksteps][planes][64 lanes] x 16 bytes, of this launch's arithmetic
     const int32_t* csrc;        // [n_mslices][n_pairs] coefficient slot of (slice, pair), or -1
     int32_t n_groups, max_gld;  // segment groups of the slices, longest group segment
     int32_t stg_words;          // LDS staging words per element of all groups: sum (n_tf * ld + 4)
@@ -31,13 +45,21 @@ struct MfmaBucket {
     const int32_t* sgrp;        // [n_mslices] group of each slice
     const int32_t* scol;        // [n_mslices] column offset of each slice in its group
     int32_t slice_idx[SLX_MAX_SLICES];
+    // host side: the fragments of every arithmetic and which arithmetics fit
+    const void* afrag_x[AR_COUNT];
+    uint32_t arith_ok;          // bit a: arithmetic a has a kernel variant for this bucket
 };
 
+// The launch description of one arithmetic (fragments, column strides, waves).
+MfmaBucket mfma_for(const MfmaBucket& mb, int arith);
 size_t mfma_lds_bytes(const MfmaBucket& mb);
 // Waves per workgroup and M tiles per wave: one tile per wave up to 8 waves,
 // or (prefer_mtw == 2) two tiles per wave so each B fragment feeds 6 MFMAs.
 bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw);
-hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st);
-hipError_t launch_mfma_probe(float* out, hipStream_t st);
+// Contraction of the bucket's matrix slices in the arithmetic of mb (from mfma_for).
+hipError_t launch_mfma(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st);
+// One MFMA of the given arithmetic (AR_*, or 3 for v_mfma_f64_16x16x4_f64) on
+// fixed operands: out = the 16x16 result, row-major (operand layout check).
+hipError_t launch_mfma_probe(int kind, float* out, hipStream_t st);
 
 }  // namespace slx

@@ -1,4 +1,4 @@
-// Mixed-precision element contraction on MFMA (gfx950 v_mfma_f32_16x16x32_f16).
+// Mixed-precision element contraction on MFMA (gfx950).
 //
 // For the matrix slices of a bucket that share one tf and one dof shape
 // table the reference computes, per term t,
@@ -9,18 +9,30 @@
 //   Ke[m = i*n_dof + j, e] = sum_k Phi[m][k] * C[k][e]
 // with Phi[m][(p,q)] = tf_r[i][q] w_q dof_s[j][q] constant per bucket.
 //
-// Precision: Phi and C are split into FP16 hi + lo parts with
-//   x = hi + 2^-11 lo',   lo' = fp16((x - hi) * 2^11),
-// Phi scaled once by a power of two, C scaled per (slice, element) column by
-// a power of two so that both parts stay in the FP16 normal range.  Three
-// MFMA products (hi*hi into one accumulator, hi*lo' + lo'*hi into a second)
-// recover ~22 bits; accumulation is FP32; the reduction into the CSR is FP64.
-// The Jacobian, its inverse and determinant stay FP64.
+// Arithmetic (MfmaArith), FP32 accumulation in every case, FP64 reduction
+// into the CSR, Jacobian / inverse / determinant in FP64:
+//   AR_F16: Phi and C split into FP16 hi + lo parts,
+//       x = hi + 2^-11 lo',   lo' = fp16((x - hi) * 2^11),
+//     Phi scaled once by a power of two, C scaled per (slice, element) column
+//     by a power of two so both parts stay in the FP16 normal range.  Three
+//     products (hi*hi into one accumulator, hi*lo' + lo'*hi into a second)
+//     recover ~22 bits (v_mfma_f32_16x16x32_f16).
+//   AR_BF16: three BF16 pieces x = hi + mid + lo (each the BF16 rounding of
+//     what the previous pieces leave; BF16 has the FP32 exponent range, so the
+//     pieces need no scaling beyond the shared one).  Six products (hi*hi into
+//     one accumulator; hi*mid, mid*hi, hi*lo, mid*mid, lo*hi into a second)
+//     keep every term down to 2^-16 of the product: ~22 bits
+//     (v_mfma_f32_16x16x32_bf16).
+//   AR_F32: plain FP32 operands, one product (v_mfma_f32_16x16x4_f32).
 //
-// Layout of v_mfma_f32_16x16x32_f16 operands (wave64, l = lane):
-//   A (16 x 32): lane l holds A[l % 16][8 (l / 16) + 0..7]
-//   B (32 x 16): lane l holds B[8 (l / 16) + 0..7][l % 16]
-//   D (16 x 16): lane l holds D[4 (l / 16) + 0..3][l % 16]
+// Operand layouts (wave64, l = lane):
+//   16x16x32 f16/bf16:  A lane l holds A[l % 16][8 (l / 16) + 0..7]
+//                       B lane l holds B[8 (l / 16) + 0..7][l % 16]
+//   16x16x4 f32:        A lane l holds A[l % 16][l / 16]
+//                       B lane l holds B[l / 16][l % 16]
+//   D (16 x 16):        lane l holds D[4 (l / 16) + 0..3][l % 16]
+//   (v_mfma_f64_16x16x4_f64: lane l holds D[l / 16 + 4 v][l % 16], v = 0..3)
+// (checked on the GPU by the layout probe at the end of this file).
 #include <hip/hip_runtime.h>
 #include "device.h"
 #include "mfma.h"
@@ -28,29 +40,68 @@
 namespace slx {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 
-// Column preparation (per workgroup tile of NE elements): for every
-// (element, slice) column the coefficient values C[p][q] are read from the
-// coalesced coefficient table, scaled by a power of two and split.  A column
-// lives in LDS as packed 32-bit words (hi | lo' << 16):
-//   compact  (coefficients constant over the Gauss points): word p = C[p].
-//            The K index is k = p * kq + q with the Gauss points of a pair
-//            padded to kq = n_gp rounded up to 8 (zero rows of Phi), so the
-//            eight consecutive k of a lane share one pair: the B fragment is
-//            one LDS word broadcast.  Word n_pairs is 0 (K padding).
-//   expanded otherwise: word k = C[k / n_gp][k % n_gp] for k < K, 0 up to
-//            the padded K, read as two 16-byte LDS loads per K step.
-// Output: the element rows of every segment group are assembled in LDS for
-// 16 elements and written with coalesced 16-byte stores (one contiguous
-// range per group).
 __device__ inline uint32_t pack_split(float x) {
     _Float16 hi = (_Float16)x;
     _Float16 lo = (_Float16)((x - (float)hi) * 2048.f);
     return (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, lo) << 16);
 }
 
+// BF16 pieces of x: hi = bf16(x), mid = bf16(x - hi), lo = bf16(x - hi - mid)
+// (each difference is exact in FP32).  Words (hi | mid << 16, lo).
+__device__ inline void pack_bf3(float x, uint32_t& w0, uint32_t& w1) {
+    const __bf16 hi = (__bf16)x;
+    const float r1 = x - (float)hi;
+    const __bf16 mid = (__bf16)r1;
+    const __bf16 lo = (__bf16)(r1 - (float)mid);
+    w0 = (uint32_t)__builtin_bit_cast(uint16_t, hi) | ((uint32_t)__builtin_bit_cast(uint16_t, mid) << 16);
+    w1 = (uint32_t)__builtin_bit_cast(uint16_t, lo);
+}
+
+// Compact column words of pair p (value x already scaled), and the zero pad pair.
+template <int AR>
+__device__ inline void store_pair(uint32_t* dst, int p, float x) {
+    if constexpr (AR == AR_F16) {
+        dst[p] = pack_split(x);
+    } else if constexpr (AR == AR_BF16) {
+        uint32_t w0, w1;
+        pack_bf3(x, w0, w1);
+        dst[2 * p] = w0;
+        dst[2 * p + 1] = w1;
+    } else {
+        dst[p] = __float_as_uint(x);
+    }
+}
+template <int AR>
+__device__ inline void store_pad(uint32_t* dst, int np) {
+    if constexpr (AR == AR_BF16) {
+        dst[2 * np] = 0u;
+        dst[2 * np + 1] = 0u;
+    } else {
+        dst[np] = 0u;
+    }
+}
+
+// Column preparation (per workgroup tile of NE elements): for every
+// (element, slice) column the coefficient values C[p][q] are read from the
+// coalesced coefficient table, scaled by a power of two and split.  A column
+// lives in LDS as 32-bit words:
+//   compact  (coefficients constant over the Gauss points): the words of
+//            pair p (F16 one word hi | lo' << 16, BF16 two words, F32 the
+//            value).  The K index is k = p * kq + q with the Gauss points of
+//            a pair padded to kq = n_gp rounded up to 8 (zero rows of Phi), so
+//            the eight consecutive k of a lane (16x16x32) or the four k of
+//            one 16x16x4 step share one pair: the B fragment is a broadcast of
+//            one pair's words.  Pair n_pairs is 0 (K padding).
+//   expanded otherwise (F16 only): word k = C[k / n_gp][k % n_gp] for k < K,
+//            0 up to the padded K, read as two 16-byte LDS loads per K step.
+// Output: the element rows of every segment group are assembled in LDS for
+// 16 elements and written with coalesced 16-byte stores (one contiguous
+// range per group).
+//
 // Work split (one workgroup per CU, persistent over 16-element tiles):
 //   consumer waves (one or two M tiles each) run the contraction of tile s
 //     from the column buffer Cs[s & 1] into the staging buffer stg[s & 1];
@@ -58,9 +109,11 @@ __device__ inline uint32_t pack_split(float x) {
 //     columns of tile s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from
 //     stg[(s - 1) & 1] to the element matrices with coalesced 16-byte stores.
 // One barrier per tile; every buffer is written one tile after its last read.
-template <int MTW, int MAXK, bool COMPACT, int G>
-__global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
+template <int AR, int MTW, int MAXK, bool COMPACT, int G>
+__global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
                                                                                   float* __restrict__ ke) {
+    static_assert(COMPACT || AR == AR_F16, "per-Gauss-point columns are built for the FP16 split only");
+    constexpr int NPL = AR == AR_BF16 ? 3 : 2;  // A fragment planes
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     constexpr int NE = 16;
     const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq, ngp = b.n_gp;
@@ -88,7 +141,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
     if (wave < nw) {
         // ------------------------------------------------------- consumers --
         // A fragments of this wave's M tiles (tile = wave + nw * i)
-        half8 Ahi[MTW][MAXK], Alo[MTW][MAXK];
+        uint4v Af[MTW][MAXK][NPL];
         int irow[MTW][4], jrow[MTW][4];
         bool rowok[MTW][4];
 #pragma unroll
@@ -96,15 +149,13 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
             const int mt = wave + nw * i;
 #pragma unroll
             for (int ks = 0; ks < MAXK; ks++) {
-                if (mt < mb.mtiles) {
-                    const half8* src = reinterpret_cast<const half8*>(mb.afrag) + ((size_t)(mt * MAXK + ks) * 2) * 64;
-                    Ahi[i][ks] = src[lane];
-                    Alo[i][ks] = src[64 + lane];
-                } else {
 #pragma unroll
-                    for (int v = 0; v < 8; v++) {
-                        Ahi[i][ks][v] = (_Float16)0.f;
-                        Alo[i][ks][v] = (_Float16)0.f;
+                for (int pl = 0; pl < NPL; pl++) {
+                    if (mt < mb.mtiles) {
+                        const uint4v* src = reinterpret_cast<const uint4v*>(mb.afrag) + ((size_t)(mt * MAXK + ks) * NPL + pl) * 64;
+                        Af[i][ks][pl] = src[lane];
+                    } else {
+                        Af[i][ks][pl] = uint4v{0u, 0u, 0u, 0u};
                     }
                 }
             }
@@ -116,18 +167,26 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                 rowok[i][v] = row < mb.M;
             }
         }
-        // compact columns: the pair word of this lane's eight k per K step
+        // compact columns: the pair of this lane's eight k per K step (16x16x32)
+        // or of the four k of each 16x16x4 step (F32: eight-k block kb of the
+        // K step, wave-uniform)
+        constexpr int WPP = AR == AR_BF16 ? 2 : 1;
         int pword[MAXK];
+        int pw32[MAXK][4];
 #pragma unroll
-        for (int ks = 0; ks < MAXK; ks++) pword[ks] = min((ks * 32 + 8 * lgrp) / mb.kq, np);
+        for (int ks = 0; ks < MAXK; ks++) {
+            pword[ks] = WPP * min((ks * 32 + 8 * lgrp) / mb.kq, np);
+#pragma unroll
+            for (int kb = 0; kb < 4; kb++) pw32[ks][kb] = min((ks * 32 + 8 * kb) / mb.kq, np);
+        }
         __syncthreads();  // tile 0 prepared
         for (int st = 0; st < nsteps; st++) {
             const uint32_t* Cs = Csb + (size_t)(st & 1) * NE * estride;
             const float* colsc = colscb + (size_t)(st & 1) * ns * NE;
             float* stg = stgb + (size_t)(st & 1) * 16 * mb.stg_words;
             // slices G at a time (G divides the slice count): 2 G independent
-            // accumulator chains per M tile (hi*hi, and hi*lo' + lo'*hi) share the
-            // A fragments; every LDS word of the group is read before the MFMAs
+            // accumulator chains per M tile (main product, and correction
+            // products) share the A fragments
             for (int s0 = 0; s0 < ns; s0 += G) {
                 float4v acc_hh[G][MTW], acc_x[G][MTW];
 #pragma unroll
@@ -140,8 +199,8 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                             acc_x[c][i][v] = 0.f;
                         }
                 const uint32_t* colbase0 = Cs + (size_t)lrow * estride + (size_t)s0 * cstride;
-                uint32_t cw[G][COMPACT ? MAXK : 1];
-                if (COMPACT) {
+                uint32_t cw[G][(COMPACT && AR == AR_F16) ? MAXK : 1];
+                if constexpr (COMPACT && AR == AR_F16) {
 #pragma unroll
                     for (int c = 0; c < G; c++)
 #pragma unroll
@@ -151,36 +210,80 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                 for (int ks = 0; ks < MAXK; ks++) {
 #pragma unroll
                     for (int c = 0; c < G; c++) {
-                        uint4v hw, lw;
-                        if (COMPACT) {
-                            // one packed word (hi | lo' << 16) -> both halves broadcast
-                            // to all eight k of the lane (byte permutes)
-                            const uint32_t u = cw[c][COMPACT ? ks : 0];
-                            const uint32_t h2 = __builtin_amdgcn_perm(u, u, 0x01000100u);
-                            const uint32_t l2 = __builtin_amdgcn_perm(u, u, 0x03020302u);
-                            hw = (uint4v){h2, h2, h2, h2};
-                            lw = (uint4v){l2, l2, l2, l2};
-                        } else {
-                            const uint32_t* colbase = colbase0 + c * cstride;
-                            const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
-                            const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
-                            hw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x05040100u);
-                            hw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x05040100u);
-                            hw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x05040100u);
-                            hw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x05040100u);
-                            lw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x07060302u);
-                            lw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x07060302u);
-                            lw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x07060302u);
-                            lw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x07060302u);
-                        }
-                        const half8 bhi = __builtin_bit_cast(half8, hw);
-                        const half8 blo = __builtin_bit_cast(half8, lw);
+                        const uint32_t* colbase = colbase0 + c * cstride;
+                        if constexpr (AR == AR_F16) {
+                            uint4v hw, lw;
+                            if constexpr (COMPACT) {
+                                // one packed word (hi | lo' << 16) -> both halves broadcast
+                                // to all eight k of the lane (byte permutes)
+                                const uint32_t u = cw[c][ks];
+                                const uint32_t h2 = __builtin_amdgcn_perm(u, u, 0x01000100u);
+                                const uint32_t l2 = __builtin_amdgcn_perm(u, u, 0x03020302u);
+                                hw = (uint4v){h2, h2, h2, h2};
+                                lw = (uint4v){l2, l2, l2, l2};
+                            } else {
+                                const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp);
+                                const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + ks * 32 + 8 * lgrp + 4);
+                                hw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x05040100u);
+                                hw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x05040100u);
+                                hw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x05040100u);
+                                hw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x05040100u);
+                                lw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x07060302u);
+                                lw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x07060302u);
+                                lw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x07060302u);
+                                lw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x07060302u);
+                            }
+                            const half8 bhi = __builtin_bit_cast(half8, hw);
+                            const half8 blo = __builtin_bit_cast(half8, lw);
 #pragma unroll
-                        for (int i = 0; i < MTW; i++) {
-                            if (i > 0 && wave + nw * i >= mb.mtiles) continue;  // wave-uniform: no work for an empty tile
-                            acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], bhi, acc_hh[c][i], 0, 0, 0);
-                            acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Ahi[i][ks], blo, acc_x[c][i], 0, 0, 0);
-                            acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(Alo[i][ks], bhi, acc_x[c][i], 0, 0, 0);
+                            for (int i = 0; i < MTW; i++) {
+                                if (i > 0 && wave + nw * i >= mb.mtiles) continue;  // wave-uniform: no work for an empty tile
+                                const half8 ahi = __builtin_bit_cast(half8, Af[i][ks][0]);
+                                const half8 alo = __builtin_bit_cast(half8, Af[i][ks][1]);
+                                acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, acc_hh[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, acc_x[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, acc_x[c][i], 0, 0, 0);
+                            }
+                        } else if constexpr (AR == AR_BF16) {
+                            // two words (hi | mid << 16, lo) -> the three pieces broadcast
+                            const uint32_t u0 = colbase[pword[ks]], u1 = colbase[pword[ks] + 1];
+                            const uint32_t h2 = __builtin_amdgcn_perm(u0, u0, 0x01000100u);
+                            const uint32_t m2 = __builtin_amdgcn_perm(u0, u0, 0x03020302u);
+                            const uint32_t l2 = __builtin_amdgcn_perm(u1, u1, 0x01000100u);
+                            const bf16x8 bh = __builtin_bit_cast(bf16x8, (uint4v){h2, h2, h2, h2});
+                            const bf16x8 bm = __builtin_bit_cast(bf16x8, (uint4v){m2, m2, m2, m2});
+                            const bf16x8 bl = __builtin_bit_cast(bf16x8, (uint4v){l2, l2, l2, l2});
+#pragma unroll
+                            for (int i = 0; i < MTW; i++) {
+                                if (i > 0 && wave + nw * i >= mb.mtiles) continue;
+                                const bf16x8 ah = __builtin_bit_cast(bf16x8, Af[i][ks][0]);
+                                const bf16x8 am = __builtin_bit_cast(bf16x8, Af[i][ks][1]);
+                                const bf16x8 al = __builtin_bit_cast(bf16x8, Af[i][ks][2]);
+                                acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc_hh[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc_x[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc_x[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc_x[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc_x[c][i], 0, 0, 0);
+                                acc_x[c][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc_x[c][i], 0, 0, 0);
+                            }
+                        } else {
+                            // F32: step j (k = 32 ks + 4 j + lane / 16) reads the pair of
+                            // eight-k block j / 2
+                            float bv[4];
+#pragma unroll
+                            for (int kb = 0; kb < 4; kb++) bv[kb] = __uint_as_float(colbase[pw32[ks][kb]]);
+#pragma unroll
+                            for (int i = 0; i < MTW; i++) {
+                                if (i > 0 && wave + nw * i >= mb.mtiles) continue;
+                                const float4v a0 = __builtin_bit_cast(float4v, Af[i][ks][0]);
+                                const float4v a1 = __builtin_bit_cast(float4v, Af[i][ks][1]);
+#pragma unroll
+                                for (int j = 0; j < 4; j++)
+                                    acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[j], bv[j >> 1], acc_hh[c][i], 0, 0, 0);
+#pragma unroll
+                                for (int j = 0; j < 4; j++)
+                                    acc_hh[c][i] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[j], bv[2 + (j >> 1)], acc_hh[c][i], 0, 0, 0);
+                            }
                         }
                     }
                 }
@@ -192,15 +295,19 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                     for (int g2 = 0; g2 < g; g2++) gstg += (size_t)16 * (n_tf * gld[g2] + 4);
                     const int ld = gld[g], srow = n_tf * ld + 4;
                     const float colscale = colsc[s * NE + lrow];
-                    const float xscale = colscale * (1.f / 2048.f);
+                    const float xscale = AR == AR_F16 ? colscale * (1.f / 2048.f) : colscale;
                     float* out = gstg + (size_t)lrow * srow + scol[s];
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
                         const int mt = wave + nw * i;
                         if (mt >= mb.mtiles) continue;
 #pragma unroll
-                        for (int v = 0; v < 4; v++)
-                            if (rowok[i][v]) out[irow[i][v] * ld + jrow[i][v]] = acc_hh[c][i][v] * colscale + acc_x[c][i][v] * xscale;
+                        for (int v = 0; v < 4; v++) {
+                            if (!rowok[i][v]) continue;
+                            const float val = AR == AR_F32 ? acc_hh[c][i][v] * colscale
+                                                           : acc_hh[c][i][v] * colscale + acc_x[c][i][v] * xscale;
+                            out[irow[i][v] * ld + jrow[i][v]] = val;
+                        }
                     }
                 }
             }
@@ -214,7 +321,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
         const bool is_prep = pt0 < np_threads;  // wave-uniform
         const int pt = is_prep ? pt0 : pt0 - np_threads;
         // columns of a tile: one thread per (element, slice), element fastest:
-        // power-of-two scale and packed hi/lo' words
+        // power-of-two scale and the split words
         auto prepare = [&](int tile, int buf) {
             const int e0 = tile * NE;
             const int nloc = min(NE, b.n_elem - e0);
@@ -226,7 +333,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                 const int64_t e = e0 + (valid ? el : 0);
                 uint32_t* dst = Cs + (size_t)el * estride + (size_t)s * cstride;
                 float mx = 0.f;
-                if (COMPACT) {
+                if constexpr (COMPACT) {
                     // every coefficient of the column loaded at once (independent loads)
                     float cv[SLX_MFMA_MAXP];
 #pragma unroll
@@ -241,8 +348,8 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
                     colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
 #pragma unroll
                     for (int p = 0; p < SLX_MFMA_MAXP; p++)
-                        if (p < np) dst[p] = pack_split(cv[p] * sc);
-                    dst[np] = 0u;
+                        if (p < np) store_pair<AR>(dst, p, cv[p] * sc);
+                    store_pad<AR>(dst, np);
                 } else {
                     // Gauss-point values in batches of 8 independent loads
                     for (int p = 0; p < np; p++) {
@@ -334,8 +441,8 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
             colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
 #pragma unroll
             for (int p = 0; p < SLX_MFMA_MAXP; p++)
-                if (p < np) dst[p] = pack_split(cvn[p] * sc);
-            dst[np] = 0u;
+                if (p < np) store_pair<AR>(dst, p, cvn[p] * sc);
+            store_pad<AR>(dst, np);
         };
         const bool pipelined = COMPACT && NE * ns <= np_threads;
         auto tile_of = [&](int st) { return (int)blockIdx.x + st * (int)gridDim.x; };
@@ -370,21 +477,25 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_f16split_kern
     }
 }
 
-template <int MTW, int MAXK>
+template <int AR, int MTW, int MAXK>
 static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(mb);
     const int tiles = (b.n_elem + 15) / 16;
     const int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
     const dim3 gd(grid), bd(64 * (mb.nw + SLX_MFMA_PWAVES));
-    if (!mb.compact || MTW != 1 || mb.sgroup == 1) {
+    if constexpr (AR != AR_F16) {
+        // compact columns, one M tile per wave, one slice at a time
+        if (!mb.compact || MTW != 1) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((mfma_contract_kernel<AR, 1, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+    } else if (!mb.compact || MTW != 1 || mb.sgroup == 1) {
         if (mb.compact)
-            hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+            hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
         else
-            hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, false, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+            hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, false, 1>), gd, bd, lds, st, b, mb, ctab, ke);
     } else if (mb.sgroup == 2) {
-        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, (MTW == 1 ? 2 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, (MTW == 1 ? 2 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
     } else {
-        hipLaunchKernelGGL((mfma_f16split_kernel<MTW, MAXK, true, (MTW == 1 ? 3 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, (MTW == 1 ? 3 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
     }
     return hipGetLastError();
 }
@@ -406,50 +517,92 @@ bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int
     return true;
 }
 
-template <int MTW>
+MfmaBucket mfma_for(const MfmaBucket& mb0, int arith) {
+    MfmaBucket mb = mb0;
+    mb.arith = arith;
+    mb.afrag = mb0.afrag_x[arith];
+    if (arith != AR_F16) {
+        // BF16 / F32: compact columns only, one M tile per wave, one slice per group
+        mb.nw = mb.mtiles;
+        mb.mtw = 1;
+        mb.sgroup = 1;
+        mb.cstride = mfma_words_per_pair(arith) * (mb.n_pairs + 1);
+        mb.estride = (mb.n_mslices * mb.cstride) | 1;  // odd: conflict-free 32-bit reads across elements
+    }
+    return mb;
+}
+
+template <int AR, int MTW>
 static hipError_t launch_mtw(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     switch (mb.maxk) {
-        case 1: return launch_variant<MTW, 1>(b, mb, ctab, ke, st);
-        case 2: return launch_variant<MTW, 2>(b, mb, ctab, ke, st);
-        case 3: return launch_variant<MTW, 3>(b, mb, ctab, ke, st);
-        case 4: return launch_variant<MTW, 4>(b, mb, ctab, ke, st);
-        case 5: return launch_variant<MTW, 5>(b, mb, ctab, ke, st);
-        case 6: return launch_variant<MTW, 6>(b, mb, ctab, ke, st);
-        case 7: return launch_variant<MTW, 7>(b, mb, ctab, ke, st);
-        case 8: return launch_variant<MTW, 8>(b, mb, ctab, ke, st);
+        case 1: return launch_variant<AR, MTW, 1>(b, mb, ctab, ke, st);
+        case 2: return launch_variant<AR, MTW, 2>(b, mb, ctab, ke, st);
+        case 3: return launch_variant<AR, MTW, 3>(b, mb, ctab, ke, st);
+        case 4: return launch_variant<AR, MTW, 4>(b, mb, ctab, ke, st);
+        case 5: return launch_variant<AR, MTW, 5>(b, mb, ctab, ke, st);
+        case 6: return launch_variant<AR, MTW, 6>(b, mb, ctab, ke, st);
+        case 7: return launch_variant<AR, MTW, 7>(b, mb, ctab, ke, st);
+        case 8: return launch_variant<AR, MTW, 8>(b, mb, ctab, ke, st);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_mfma_f16split(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
+hipError_t launch_mfma(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     if (b.n_elem == 0) return hipSuccess;
-    if (mb.mtw == 1) return launch_mtw<1>(b, mb, ctab, ke, st);
-    if (mb.mtw == 2) return launch_mtw<2>(b, mb, ctab, ke, st);
+    if (!((mb.arith_ok >> mb.arith) & 1u) || !mb.afrag) return hipErrorInvalidValue;
+    if (mb.arith == AR_BF16) return mb.mtw == 1 ? launch_mtw<AR_BF16, 1>(b, mb, ctab, ke, st) : hipErrorInvalidValue;
+    if (mb.arith == AR_F32) return mb.mtw == 1 ? launch_mtw<AR_F32, 1>(b, mb, ctab, ke, st) : hipErrorInvalidValue;
+    if (mb.mtw == 1) return launch_mtw<AR_F16, 1>(b, mb, ctab, ke, st);
+    if (mb.mtw == 2) return launch_mtw<AR_F16, 2>(b, mb, ctab, ke, st);
     return hipErrorInvalidValue;
 }
 
-// In-bounds probe of the 16x16x32 f16 operand layout: D = A * B for
-// A[m][k] = m + 16 k / 64, B[k][n] = (k == n) + (k == n + 16), written as
-// out[m * 16 + n] (256 floats).  Used by the layout self-test.
-__global__ void mfma_layout_probe(float* out) {
+// In-bounds probes of the operand layouts: one wave, one MFMA, out[m * 16 + n]
+// (256 floats) = D[m][n] written with the assumed D layout.
+//   kind 0 (f16 16x16x32): A[m][k] = m + k / 64, B[k][n] = (k == n) + (k == n + 16)
+//   kind 1 (bf16 16x16x32), 2 (f32 16x16x4), 3 (f64 16x16x4):
+//     A[m][k] = (m + 2 k) % 7 - 3, B[k][n] = (3 k + n) % 5 - 2 (small integers: exact)
+__global__ void mfma_layout_probe(int kind, float* out) {
     const int lane = threadIdx.x & 63;
-    half8 a, bb;
-    for (int v = 0; v < 8; v++) {
-        int m = lane % 16, k = 8 * (lane / 16) + v;
-        a[v] = (_Float16)(m + k / 64.f);
-        int n = lane % 16;
-        bb[v] = (_Float16)((k == n) || (k == n + 16) ? 1.f : 0.f);
-    }
+    const int lrow = lane % 16, lgrp = lane / 16;
+    auto ia = [](int m, int k) { return (float)((m + 2 * k) % 7 - 3); };
+    auto ib = [](int k, int n) { return (float)((3 * k + n) % 5 - 2); };
     float4v c = {0.f, 0.f, 0.f, 0.f};
-    c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bb, c, 0, 0, 0);
+    if (kind == 0) {
+        half8 a, bb;
+        for (int v = 0; v < 8; v++) {
+            const int k = 8 * lgrp + v;
+            a[v] = (_Float16)(lrow + k / 64.f);
+            bb[v] = (_Float16)((k == lrow) || (k == lrow + 16) ? 1.f : 0.f);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(a, bb, c, 0, 0, 0);
+    } else if (kind == 1) {
+        bf16x8 a, bb;
+        for (int v = 0; v < 8; v++) {
+            const int k = 8 * lgrp + v;
+            a[v] = (__bf16)ia(lrow, k);
+            bb[v] = (__bf16)ib(k, lrow);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bb, c, 0, 0, 0);
+    } else if (kind == 2) {
+        c = __builtin_amdgcn_mfma_f32_16x16x4f32(ia(lrow, lgrp), ib(lgrp, lrow), c, 0, 0, 0);
+    } else {
+        typedef double double4v __attribute__((ext_vector_type(4)));
+        double4v d = {0.0, 0.0, 0.0, 0.0};
+        d = __builtin_amdgcn_mfma_f64_16x16x4f64((double)ia(lrow, lgrp), (double)ib(lgrp, lrow), d, 0, 0, 0);
+        for (int v = 0; v < 4; v++) c[v] = (float)d[v];
+    }
+    // D layout: lane l holds D[4 (l / 16) + v][l % 16] for the f32-accumulating
+    // kinds, D[l / 16 + 4 v][l % 16] for v_mfma_f64_16x16x4_f64
     for (int v = 0; v < 4; v++) {
-        int m = 4 * (lane / 16) + v, n = lane % 16;
+        const int m = kind == 3 ? lgrp + 4 * v : 4 * lgrp + v, n = lrow;
         out[m * 16 + n] = c[v];
     }
 }
 
-hipError_t launch_mfma_probe(float* out, hipStream_t st) {
-    hipLaunchKernelGGL(mfma_layout_probe, dim3(1), dim3(64), 0, st, out);
+hipError_t launch_mfma_probe(int kind, float* out, hipStream_t st) {
+    if (kind < 0 || kind > 3) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mfma_layout_probe, dim3(1), dim3(64), 0, st, kind, out);
     return hipGetLastError();
 }
 

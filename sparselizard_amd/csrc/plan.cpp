@@ -709,6 +709,34 @@ float f16_to_f32(uint16_t h) {
     return f;
 }
 
+// BF16 round-to-nearest-even (host side of the BF16 split; finite inputs).
+uint16_t f32_to_bf16(float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    const uint32_t r = x + 0x7fffu + ((x >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+float bf16_to_f32(uint16_t h) {
+    const uint32_t x = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &x, 4);
+    return f;
+}
+
+// The 16-byte fragment plane entries of one A operand value x in every
+// arithmetic: F16 (hi, lo' = fp16((x - hi) 2^11)), BF16 (hi, mid, lo).
+void split_f16(float x, uint16_t& hi, uint16_t& lo) {
+    hi = f32_to_f16(x);
+    lo = f32_to_f16((x - f16_to_f32(hi)) * 2048.f);
+}
+void split_bf16(float x, uint16_t& hi, uint16_t& mid, uint16_t& lo) {
+    hi = f32_to_bf16(x);
+    const float r1 = x - bf16_to_f32(hi);
+    mid = f32_to_bf16(r1);
+    lo = f32_to_bf16(r1 - bf16_to_f32(mid));
+}
+
 // Builds the MFMA description of a bucket's matrix slices (all sharing one
 // Phi table) or leaves M = 0 when the bucket is not eligible.
 // Kernel choice per bucket class, from the table the autotuner writes
@@ -906,21 +934,43 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     if (mx > 0) std::frexp(mx, &ex);
     double sc = std::ldexp(1.0, -ex);
     mb.a_unscale = (float)std::ldexp(1.0, ex);
-    // fragments [mt][ks][part][lane][8]
-    std::vector<uint16_t> frag((size_t)mb.mtiles * mb.ksteps * 2 * 64 * 8, 0);
+    auto phix = [&](int m, int k) { return (m < mb.M && k < mb.K) ? (float)(phi[(size_t)m * mb.K + k] * sc) : 0.f; };
+    // fragments [mt][ks][plane][lane] x 16 bytes.  16x16x32 (F16, BF16): lane
+    // l holds k = 8 (l / 16) + v of row l % 16; 16x16x4 (F32): plane pl, entry
+    // v is step j = 4 pl + v, k = 4 j + l / 16
+    const size_t nfr = (size_t)mb.mtiles * mb.ksteps * 64;
+    std::vector<uint16_t> frag(nfr * 2 * 8, 0), fragb(nfr * 3 * 8, 0);
+    std::vector<float> fragf(nfr * 2 * 4, 0.f);
     for (int mt = 0; mt < mb.mtiles; mt++)
         for (int ks = 0; ks < mb.ksteps; ks++)
-            for (int lane = 0; lane < 64; lane++)
+            for (int lane = 0; lane < 64; lane++) {
+                const int m = mt * 16 + lane % 16;
+                const size_t w = (size_t)(mt * mb.ksteps + ks);
                 for (int v = 0; v < 8; v++) {
-                    int m = mt * 16 + lane % 16, k = ks * 32 + 8 * (lane / 16) + v;
-                    float x = (m < mb.M && k < mb.K) ? (float)(phi[(size_t)m * mb.K + k] * sc) : 0.f;
-                    uint16_t hi = f32_to_f16(x);
-                    uint16_t lo = f32_to_f16((x - f16_to_f32(hi)) * 2048.f);
-                    size_t base = (((size_t)(mt * mb.ksteps + ks) * 2) * 64 + lane) * 8 + v;
-                    frag[base] = hi;
-                    frag[base + 64 * 8] = lo;
+                    const float x = phix(m, ks * 32 + 8 * (lane / 16) + v);
+                    uint16_t hi, mid, lo;
+                    split_f16(x, hi, lo);
+                    frag[((w * 2 + 0) * 64 + lane) * 8 + v] = hi;
+                    frag[((w * 2 + 1) * 64 + lane) * 8 + v] = lo;
+                    split_bf16(x, hi, mid, lo);
+                    fragb[((w * 3 + 0) * 64 + lane) * 8 + v] = hi;
+                    fragb[((w * 3 + 1) * 64 + lane) * 8 + v] = mid;
+                    fragb[((w * 3 + 2) * 64 + lane) * 8 + v] = lo;
                 }
-    mb.afrag = P.dev(frag.data(), frag.size());
+                for (int pl = 0; pl < 2; pl++)
+                    for (int v = 0; v < 4; v++) fragf[((w * 2 + pl) * 64 + lane) * 4 + v] = phix(m, ks * 32 + 4 * (4 * pl + v) + lane / 16);
+            }
+    mb.afrag_x[AR_F16] = P.dev(frag.data(), frag.size());
+    mb.afrag_x[AR_BF16] = P.dev(fragb.data(), fragb.size());
+    mb.afrag_x[AR_F32] = P.dev(fragf.data(), fragf.size());
+    mb.afrag = mb.afrag_x[AR_F16];
+    mb.arith = AR_F16;
+    // arithmetics with a kernel variant: F16 always; BF16 and F32 take compact
+    // columns with one M tile per wave, within the LDS of one CU
+    mb.arith_ok = 1u << AR_F16;
+    if (mb.compact && mb.mtiles <= 8)
+        for (int a : {AR_BF16, AR_F32})
+            if (mfma_lds_bytes(mfma_for(mb, a)) <= 160 * 1024) mb.arith_ok |= 1u << a;
     mb.csrc = P.dev(csrc.data(), csrc.size());
     mb.gbase = P.dev(gb.data(), gb.size());
     mb.gld = P.dev(gd.data(), gd.size());
@@ -966,7 +1016,6 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         f.ntp = f.nt * 16;
         f.n_terms = sd.n_terms;
         if (!factor_variant_fits(f.mt, f.nt, &f.bm, &f.bn)) continue;
-        if (f.mt * 128 > SLX_FK_AREG * 64 * SLX_FK_BWAVES) continue;
         // tf derivatives in use; every term's (derivative index, dof derivative, slot)
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
@@ -984,7 +1033,12 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         f.qb = std::min(32 / f.na, SLX_FK_RU);
         if (f.qb < 1) continue;
         f.nks = (f.ngp + f.qb - 1) / f.qb;
-        if (factor_lds_bytes(f) > 160 * 1024) continue;
+        // arithmetics whose LDS and P chunk staging fit; the FP16 split must
+        f.arith_ok = 0;
+        for (int a = 0; a < AR_COUNT; a++)
+            if (factor_lds_bytes(f, a) <= 160 * 1024 && f.mt * mfma_planes(a) * 64 <= SLX_FK_AREG * 64 * SLX_FK_BWAVES)
+                f.arith_ok |= 1u << a;
+        if (!(f.arith_ok & (1u << AR_F16))) continue;
         // P fragments: lane l of tile (ks, mt) holds P^T[i = mt*16 + l%16][k = 8*(l/16) + v],
         // k -> (Gauss point ks*qb + k/na, tf derivative alist[k%na]); power-of-two scale to [128, 256)
         const double* tft = bd.tables[sd.tf_table];
@@ -997,20 +1051,37 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         if (amax > 0) (void)std::frexp(amax, &aex);
         const double ascale = std::ldexp(1.0, 8 - aex);
         f.a_unscale = (float)std::ldexp(1.0, aex - 8);
-        std::vector<uint16_t> frag((size_t)f.nks * f.mt * 2 * 64 * 8, 0);
+        // P^T[i][kl] of k-step ks (scaled), kl = local k: Gauss point kl / na, tf derivative kl % na
+        auto px = [&](int ks, int i, int kl) {
+            const int qi = kl / f.na, ai = kl % f.na, q = ks * f.qb + qi;
+            if (i < sd.n_tf && qi < f.qb && q < ngp) return (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+            return 0.f;
+        };
+        // fragments [ks][mt][plane][lane] x 16 bytes: F16 (hi, lo), BF16 (hi, mid, lo)
+        // with k = 8 (l / 16) + v; F32 plane pl entry v: step j = 4 pl + v, k = 4 j + l / 16
+        // (F16 lo unscaled here: P sits in [128, 256))
+        const size_t nfr = (size_t)f.nks * f.mt * 64;
+        std::vector<uint16_t> frag(nfr * 2 * 8, 0), fragb(nfr * 3 * 8, 0);
+        std::vector<float> fragf(nfr * 2 * 4, 0.f);
         for (int ks = 0; ks < f.nks; ks++)
             for (int mt = 0; mt < f.mt; mt++)
-                for (int l = 0; l < 64; l++)
+                for (int l = 0; l < 64; l++) {
+                    const int i = mt * 16 + l % 16;
+                    const size_t w = (size_t)ks * f.mt + mt;
                     for (int v = 0; v < 8; v++) {
-                        const int i = mt * 16 + l % 16, kl = 8 * (l / 16) + v, qi = kl / f.na, ai = kl % f.na, q = ks * f.qb + qi;
-                        float x = 0.f;
-                        if (i < sd.n_tf && qi < f.qb && q < ngp) x = (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+                        const float x = px(ks, i, 8 * (l / 16) + v);
                         const uint16_t hi = f32_to_f16(x);
-                        const uint16_t lo = f32_to_f16(x - f16_to_f32(hi));
-                        const size_t base = ((((size_t)ks * f.mt + mt) * 2) * 64 + l) * 8 + v;
-                        frag[base] = hi;
-                        frag[base + 64 * 8] = lo;
+                        frag[((w * 2 + 0) * 64 + l) * 8 + v] = hi;
+                        frag[((w * 2 + 1) * 64 + l) * 8 + v] = f32_to_f16(x - f16_to_f32(hi));
+                        uint16_t bh, bm, bl;
+                        split_bf16(x, bh, bm, bl);
+                        fragb[((w * 3 + 0) * 64 + l) * 8 + v] = bh;
+                        fragb[((w * 3 + 1) * 64 + l) * 8 + v] = bm;
+                        fragb[((w * 3 + 2) * 64 + l) * 8 + v] = bl;
                     }
+                    for (int pl = 0; pl < 2; pl++)
+                        for (int v = 0; v < 4; v++) fragf[((w * 2 + pl) * 64 + l) * 4 + v] = px(ks, i, 4 * (4 * pl + v) + l / 16);
+                }
         // dof table times the weights, Gauss point major, padded columns
         const double* dft = bd.tables[sd.dof_table];
         const int nfd = bd.table_nff[sd.dof_table];
@@ -1025,7 +1096,11 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
                 }
         f.tqmax = (float)(tqmax * 1.0001);
         f.tinfo = P.dev(tinfo.data(), tinfo.size());
-        f.afrag = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
+        f.afrag_x[AR_F16] = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
+        f.afrag_x[AR_BF16] = reinterpret_cast<const uint32_t*>(P.dev(fragb.data(), fragb.size()));
+        f.afrag_x[AR_F32] = reinterpret_cast<const uint32_t*>(P.dev(fragf.data(), fragf.size()));
+        f.afrag = f.afrag_x[AR_F16];
+        f.arith = AR_F16;
         f.tq = P.dev(tq.data(), tq.size());
         f.ke_base = L.ke_base[b][s];
         f.ke_ld = L.ke_ld[b][s];
@@ -1242,8 +1317,7 @@ struct LaunchTimer {
 
 void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t plen, const double* const* fields, int precision,
              hipStream_t st, LaunchTimer* tm = nullptr) {
-    SLX_CHECK(precision >= 0 && precision <= 3, "unknown precision " << precision);
-    if (precision == SLX_PREC_BF16_SPLIT) throw UnsupportedError("precision SLX_PREC_BF16_SPLIT is not implemented");
+    SLX_CHECK(precision >= 0 && precision <= 4, "unknown precision " << precision);
     if (P.max_param >= 0) SLX_CHECK(params && plen >= P.max_elem_id, "per-element parameters missing or shorter than the element count");
     if (P.uses_fields) {
         SLX_CHECK(fields, "field data required by the coefficient programs");
@@ -1256,6 +1330,10 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
     in.consts = P.consts.as<double>();
     for (int f = 0; f < P.n_fields; f++) in.field_data[f] = fields ? fields[f] : nullptr;
     bool ke64 = precision == SLX_PREC_FP64;
+    // MFMA arithmetic of the mode (-1: VALU element kernels only)
+    const int arith = precision == SLX_PREC_F16_SPLIT ? AR_F16
+                      : precision == SLX_PREC_BF16_SPLIT ? AR_BF16
+                      : precision == SLX_PREC_FP32_MFMA ? AR_F32 : -1;
     // fork: rhs-only buckets and the rhs reduction overlap the matrix work
     // (serial when every launch is timed)
     const bool overlap = P.side_ok && !tm;
@@ -1270,24 +1348,30 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
         // the factorized kernel reads the coefficients of one element contiguously
-        const int qfast = (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) ? 1 : 0;
+        uint32_t fmask = 0;  // slices on the factorized kernel in this arithmetic
+        if (arith >= 0)
+            for (auto& fs : P.factor[bi])
+                if ((fs.second.arith_ok >> arith) & 1u) fmask |= 1u << fs.first;
+        const int qfast = fmask != 0 ? 1 : 0;
         P.last_qfast = qfast;
         if (tm) tm->begin(T_COEF);
         const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
         if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, qfast, ctab, bst));
         else HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
         if (tm) tm->end();
-        if (precision == SLX_PREC_F16_SPLIT && P.mfma[bi].M > 0) {
+        if (arith >= 0 && P.mfma[bi].M > 0 && ((P.mfma[bi].arith_ok >> arith) & 1u)) {
             if (tm) tm->begin(T_MFMA);
-            HIPCHECK(launch_mfma_f16split(b, P.mfma[bi], (const float*)ctab, P.d_ke.as<float>(), bst));
+            HIPCHECK(launch_mfma(b, mfma_for(P.mfma[bi], arith), (const float*)ctab, P.d_ke.as<float>(), bst));
             if (tm) tm->end();
             done = P.mfma_mask[bi];
         }
-        if (precision == SLX_PREC_F16_SPLIT && P.factor_mask[bi] != 0) {
+        if (fmask != 0) {
             if (tm) tm->begin(T_MFMA);
-            for (auto& fs : P.factor[bi]) HIPCHECK(launch_factor_f16split(b, fs.second, (const float*)ctab, P.d_ke.as<float>(), P.n_cu, bst));
+            for (auto& fs : P.factor[bi])
+                if ((fmask >> fs.first) & 1u)
+                    HIPCHECK(launch_factor(b, fs.second, arith, (const float*)ctab, P.d_ke.as<float>(), P.n_cu, bst));
             if (tm) tm->end();
-            done |= P.factor_mask[bi];
+            done |= fmask;
         }
         int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
         if ((all & ~done) != 0) {
@@ -1635,6 +1719,24 @@ slx_status slx_plan_kernels(slx_plan p, int32_t* n_mfma, int32_t* n_buckets) {
     });
 }
 
+slx_status slx_plan_kernel_counts(slx_plan p, int32_t precision, int32_t* n_batched, int32_t* n_factorized) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        SLX_CHECK(precision >= 0 && precision <= 4, "unknown precision " << precision);
+        const int arith = precision == SLX_PREC_F16_SPLIT ? AR_F16
+                          : precision == SLX_PREC_BF16_SPLIT ? AR_BF16
+                          : precision == SLX_PREC_FP32_MFMA ? AR_F32 : -1;
+        int nb = 0, nf = 0;
+        if (arith >= 0) {
+            for (auto& m : p->mfma) nb += (m.M > 0 && ((m.arith_ok >> arith) & 1u)) ? 1 : 0;
+            for (auto& v : p->factor)
+                for (auto& fs : v) nf += ((fs.second.arith_ok >> arith) & 1u) ? 1 : 0;
+        }
+        if (n_batched) *n_batched = nb;
+        if (n_factorized) *n_factorized = nf;
+    });
+}
+
 slx_status slx_plan_reduction_maps(slx_plan p, int64_t* sizes, int64_t* erow_ptr, uint64_t* erows, int64_t* slot_ptr,
                                    int32_t* slots, int64_t* rhs_ptr, uint64_t* rhs_erows, int64_t* ke_base, int64_t* ke_ld) {
     return guarded([&] {
@@ -1786,7 +1888,20 @@ slx_status slx_selftest_mfma_layout(slx_ctx ctx, float* out256) {
         HIPCHECK(hipSetDevice(ctx->device));
         DBuf d;
         d.alloc(256 * sizeof(float));
-        HIPCHECK(launch_mfma_probe(d.as<float>(), ctx->stream));
+        HIPCHECK(launch_mfma_probe(0, d.as<float>(), ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        HIPCHECK(hipMemcpy(out256, d.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
+    });
+}
+
+slx_status slx_selftest_mfma_layout_kind(slx_ctx ctx, int32_t kind, float* out256) {
+    return guarded([&] {
+        SLX_CHECK(ctx && out256, "null argument");
+        SLX_CHECK(kind >= 0 && kind <= 3, "probe kind must be 0..3");
+        HIPCHECK(hipSetDevice(ctx->device));
+        DBuf d;
+        d.alloc(256 * sizeof(float));
+        HIPCHECK(launch_mfma_probe(kind, d.as<float>(), ctx->stream));
         HIPCHECK(hipStreamSynchronize(ctx->stream));
         HIPCHECK(hipMemcpy(out256, d.p, 256 * sizeof(float), hipMemcpyDeviceToHost));
     });
