@@ -51,7 +51,9 @@ def main():
     ap.add_argument("--order", type=int, default=2)
     ap.add_argument("--extra", default="", help="extra problem options, e.g. ';pmin=5;pmax=5'")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run (timing sweeps)")
-    ap.add_argument("--mixed", default="f16split", help="mixed precision mode")
+    ap.add_argument("--mixed", default="f16split", help="mixed precision mode of the headline value")
+    ap.add_argument("--variants", default="bf16split,fp32mfma,fp32",
+                    help="further precision modes timed in the same run and reported under 'variants' ('' for none)")
     ap.add_argument("--cpu-sample-n", type=int, default=0, help="CPU baseline sample size (0: auto)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-solve", action="store_true",
@@ -146,14 +148,37 @@ def main():
     def rel(a, b):
         return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
+    def glob_err(m):
+        return float(np.sqrt(np.sum((m.Aval - ref.Aval) ** 2) + np.sum((m.Dval - ref.Dval) ** 2)) /
+                     np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2)))
+
     if ref is None:
         err_A = err_D = err_b = glob = float("nan")
     else:
         err_A = rel(mix.Aval, ref.Aval)
         err_D = rel(mix.Dval, ref.Dval) if ref.Dval.size else 0.0
         err_b = rel(mix.rhs, ref.rhs)
-        glob = float(np.sqrt(np.sum((mix.Aval - ref.Aval) ** 2) + np.sum((mix.Dval - ref.Dval) ** 2)) /
-                     np.sqrt(np.sum(ref.Aval ** 2) + np.sum(ref.Dval ** 2)))
+        glob = glob_err(mix)
+    del mix
+
+    # the other precision modes on the same plan and inputs
+    variants = {}
+    for mode in [v for v in args.variants.split(",") if v and v != args.mixed]:
+        ms_v, _ = timed(mode)
+        entry = {"ms_per_step": ms_v, "nnz_per_s": nnz / (ms_v * 1e-3), "ratio_vs_fp64": ms64 / ms_v,
+                 "kernels": plan.kernel_counts(mode)}
+        if ref is not None:
+            vcsr = csr_now()
+            entry["rel_err_vs_fp64_gpu"] = {"global_frobenius": glob_err(vcsr), "rhs": rel(vcsr.rhs, ref.rhs)}
+            del vcsr
+        entry["kernel_ms"] = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len,
+                                          d_fields, mode, iters=3)
+        log("%s: %.3f ms/step %s" % (mode, ms_v, entry.get("rel_err_vs_fp64_gpu", "")))
+        variants[mode] = entry
+    if variants:
+        # leave the headline mode's matrix in the plan (the solve below uses it)
+        step(args.mixed)
+        torch.cuda.synchronize()
 
     # per-kernel timing: HIP events around every launch on the plan's stream
     prof_mix = plan.profile(C.c_void_p(d_xyz.data_ptr()), C.c_void_p(d_par.data_ptr()), nd.param_len, d_fields,
@@ -171,10 +196,14 @@ def main():
         n_elem = sum(dd.buckets[b].n_elem for b in range(dd.n_buckets)
                      if any(not dd.buckets[b].slices[s].is_rhs for s in range(dd.buckets[b].n_slices)))
         flops = slice_flops(dd)
-    ke_bytes = 4 if args.mixed in ("f16split", "bf16split", "fp32") else 8
-    # reduction: every element-matrix entry read once, every CSR value and rhs entry written once
-    red_bytes = stats["n_coo"] * ke_bytes + nnz * 8 + nn * 8
-    mixed_kind = {"f16split": "fp16", "bf16split": "bf16", "fp32": "fp32"}.get(args.mixed, "fp32")
+    ke_bytes = 8 if args.mixed == "fp64" else 4
+    # reduction, compulsory bytes (the per-unit figure of the bytes formula):
+    # every CSR value written once; the element-matrix round trip through HBM
+    # (written by the contraction, read here) is not compulsory and is
+    # reported apart, as is the slot metadata
+    red_bytes = nnz * 8
+    ke_roundtrip = stats["n_coo"] * ke_bytes
+    mixed_kind = {"f16split": "fp16", "bf16split": "bf16", "fp32mfma": "fp32", "fp32": "fp32"}.get(args.mixed, "fp32")
     peak_elem = PEAKS[mixed_kind + "_TFLOPs"]
     t_contract = prof_mix["mfma"] if prof_mix["mfma"] > 0 else prof_mix["valu"]
     # contraction kernel: canonical flops, and its compulsory HBM bytes (the
@@ -210,7 +239,8 @@ def main():
         achieved = red_bytes / (prof_mix["reduce"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAKS["hbm_GBps"], "unit": "GB/s",
                 "frac": round(achieved / PEAKS["hbm_GBps"], 5), "traffic": None, "kernel": "reduce_rows",
-                "kernel_ms": round(prof_mix["reduce"], 4), "algorithmic": red_bytes}
+                "kernel_ms": round(prof_mix["reduce"], 4), "algorithmic": red_bytes,
+                "ke_roundtrip_bytes": ke_roundtrip}
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         try:
@@ -264,6 +294,7 @@ def main():
         "symbolic_s": t_sym,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "variants": variants,
     }
     if solve is not None:
         out["solve"] = solve
@@ -327,14 +358,28 @@ def count_tets(n):
     return 6 * n ** 3
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(sa, args):
-    """The CPU oracle (restated generate()+A()+b(), FP64, 1 thread) on a bounded
-    sample of the same workload; nnz/s is reported for that sample."""
+    """The CPU oracle (restated generate()+A()+b(), FP64) on bounded samples of
+    the same workload, timed on this host: one thread (the reference's generate
+    loop is single-threaded), and all the host cores this process may use (16
+    at most: the share of one GPU), where the oracle threads the element loop
+    and, like rawmat::process (rawmat.cpp:221), the per-row sort.  nnz/s is
+    reported for each sample; 'value' is the all-cores figure."""
     from oracle import py_oracle
 
     # sample size knob, its start, cap and how the element count grows with it
-    knob = {"elasticity3d": ("n", 4, 20, 3), "hppoisson3d": ("n", 3, 20, 3), "poisson2d": ("n", 20, 1200, 2),
-            "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 64, 1),
+    knob = {"elasticity3d": ("n", 4, 28, 3), "hppoisson3d": ("n", 3, 24, 3), "poisson2d": ("n", 20, 1200, 2),
+            "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 96, 1),
             "prism3d": ("n", 4, 110, 3)}[args.problem]
 
     def spec_of(v):
@@ -342,26 +387,36 @@ def cpu_baseline(sa, args):
             return "n=%d;order=%d" % (v, args.order)
         return "%s=%d%s" % (knob[0], v, args.extra)
 
-    def run(v):
+    def run(v, nthreads):
         p = sa.Problem(None, args.problem, spec_of(v))
         info = p.info()
         nd = p.numeric()
         t0 = time.time()
-        o = py_oracle.assemble(p.desc_ptr(), nd, info["ndofs"], info["nfree"], nthreads=1)
+        o = py_oracle.assemble(p.desc_ptr(), nd, info["ndofs"], info["nfree"], nthreads=nthreads)
         dt = time.time() - t0
         return dt, len(o.Aval) + len(o.Dval), info["nelem"]
 
-    v = args.cpu_sample_n
-    if v <= 0:
-        dt, nnz, ne = run(knob[1])
-        # aim at ~15 s of CPU work
-        grow = 15.0 / max(dt, 1e-9)
-        v = max(knob[1], int(round(knob[1] * grow ** (1.0 / knob[3]))))
-        v = min(v, knob[2])
-    dt, nnz, ne = run(v)
-    return {"value": nnz / dt, "unit": "nnz/s", "cores": 1, "kind": "port",
-            "sample": "oracle generate()+A()+b() FP64, 1 thread, %s with %s (%d elements), %.1f s"
-                      % (args.problem, spec_of(v), ne, dt)}
+    def sized(nthreads, seconds):
+        v = args.cpu_sample_n
+        if v <= 0:
+            dt, nnz, ne = run(knob[1], nthreads)
+            grow = seconds / max(dt, 1e-9)
+            v = max(knob[1], int(round(knob[1] * grow ** (1.0 / knob[3]))))
+            v = min(v, knob[2])
+        dt, nnz, ne = run(v, nthreads)
+        return {"value": nnz / dt, "s": dt, "sample": "%s with %s (%d elements)" % (args.problem, spec_of(v), ne)}
+
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    cores = max(1, min(16, avail))
+    one = sized(1, 10.0)
+    allc = sized(cores, 10.0)
+    return {"value": allc["value"], "unit": "nnz/s", "cores": cores, "kind": "port",
+            "sample": "oracle generate()+A()+b() FP64, %d threads, %s, %.1f s" % (cores, allc["sample"], allc["s"]),
+            "single_thread": {"value": one["value"], "cores": 1, "sample": "%s, %.1f s" % (one["sample"], one["s"])},
+            "cpu_model": cpu_model()}
 
 
 if __name__ == "__main__":

@@ -247,29 +247,52 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
         }
     }
     // element matrices of each fragment: vals[(i*n_dof + j) * n_elem + e] (rawmat row-major)
+    // nthreads > 1: contiguous element ranges per thread (every entry is computed
+    // by the same operations in the same order, so the result does not depend
+    // on the thread count)
     std::vector<std::vector<double>> vals(frags.size());
+    const int nth = std::max(1, nthreads);
+    auto par = [&](int64_t n, auto&& fn) {
+        if (nth <= 1 || n < 64) {
+            fn((int64_t)0, n);
+            return;
+        }
+        std::vector<std::thread> th;
+        for (int t = 0; t < nth; t++) th.emplace_back(fn, n * t / nth, n * (t + 1) / nth);
+        for (auto& x : th) x.join();
+    };
+    // coefficient table of the bucket whose fragments are being generated:
+    // every slot at every (q, e), times |det J|, evaluated once per bucket
+    // (the reference evaluates each coefficient once per batch and reuses it,
+    // universe.cpp:223-314); cslot[c][q][e]
+    int cached_bucket = -1;
+    std::vector<std::vector<double>> cslot;
     for (size_t f = 0; f < frags.size(); f++) {
         const slx_bucket_desc& b = d.buckets[frags[f].bucket];
         const slx_slice_desc& sd = b.slices[frags[f].slice];
         int ne = b.n_elem, ng = b.n_gp, nt = sd.n_tf, ndf = sd.is_rhs ? 1 : sd.n_dof;
-        std::vector<double> geo;
-        lagrange1(b.elem_type, b.gp_coords, ng, geo);
-        // coefficient of every term at every (e, q): coefs[t][e][q]
-        std::vector<std::vector<double>> C(sd.n_terms, std::vector<double>((size_t)ne * ng));
-        std::vector<double> cbuf(std::max(b.n_coefs, 1));
-        for (int e = 0; e < ne; e++)
-            for (int q = 0; q < ng; q++) {
-                double J[3][3], inv[3][3], det;
-                jacobian(d, b, in.node_xyz, geo, e, q, J, inv, det);
-                coefficients(d, b, in, geo, e, q, J, inv, det, cbuf);
-                for (int t = 0; t < sd.n_terms; t++) C[t][(size_t)e * ng + q] = cbuf[sd.terms[t].coef] * std::fabs(det);
-            }
+        if (cached_bucket != frags[f].bucket) {
+            std::vector<double> geo;
+            lagrange1(b.elem_type, b.gp_coords, ng, geo);
+            cslot.assign(b.n_coefs, std::vector<double>((size_t)ne * ng));
+            par(ne, [&](int64_t e0, int64_t e1) {
+                std::vector<double> cbuf(std::max(b.n_coefs, 1));
+                for (int64_t e = e0; e < e1; e++)
+                    for (int q = 0; q < ng; q++) {
+                        double J[3][3], inv[3][3], det;
+                        jacobian(d, b, in.node_xyz, geo, (int)e, q, J, inv, det);
+                        coefficients(d, b, in, geo, (int)e, q, J, inv, det, cbuf);
+                        for (int c = 0; c < b.n_coefs; c++) cslot[c][(size_t)q * ne + e] = cbuf[c] * std::fabs(det);
+                    }
+            });
+            cached_bucket = frags[f].bucket;
+        }
         std::vector<double>& V = vals[f];
         V.assign((size_t)nt * ndf * ne, 0.0);
         const double* tft = b.tables[sd.tf_table];
         const double* dft = sd.is_rhs ? nullptr : b.tables[sd.dof_table];
         int nff_tf = b.table_nff[sd.tf_table], nff_dof = sd.is_rhs ? 1 : b.table_nff[sd.dof_table];
-        std::vector<double> phi((size_t)nt * ndf * ng), term((size_t)nt * ndf * ne);
+        std::vector<double> phi((size_t)nt * ndf * ng);
         for (int t = 0; t < sd.n_terms; t++) {
             const slx_term& tm = sd.terms[t];
             // Phi[(i*ndf + j)][q] = tf[i][q] * w[q] * dof[j][q]
@@ -280,14 +303,29 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
                         double dv = sd.is_rhs ? 1.0 : dft[((size_t)tm.dof_deriv * nff_dof + j) * ng + q];
                         phi[((size_t)i * ndf + j) * ng + q] = tv * dv;
                     }
-            // term = Phi * C^T  (M = nt*ndf, K = ng, N = ne)
-            for (int r = 0; r < nt * ndf; r++)
-                for (int e = 0; e < ne; e++) {
-                    double s = 0.0;
-                    for (int q = 0; q < ng; q++) s += phi[(size_t)r * ng + q] * C[t][(size_t)e * ng + q];
-                    if (t == 0) V[(size_t)r * ne + e] = s;
-                    else V[(size_t)r * ne + e] += s;
+            // term = Phi * C^T  (M = nt*ndf, K = ng, N = ne): per entry a sum over
+            // the Gauss points in increasing order; elements innermost (vectorized)
+            const double* Ct = cslot[tm.coef].data();
+            par(ne, [&](int64_t e0, int64_t e1) {
+                constexpr int EB = 64;
+                double acc[EB];
+                for (int64_t eb = e0; eb < e1; eb += EB) {
+                    const int n = (int)std::min<int64_t>(EB, e1 - eb);
+                    for (int r = 0; r < nt * ndf; r++) {
+                        for (int x = 0; x < n; x++) acc[x] = 0.0;
+                        for (int q = 0; q < ng; q++) {
+                            const double ph = phi[(size_t)r * ng + q];
+                            const double* cq = Ct + (size_t)q * ne + eb;
+                            for (int x = 0; x < n; x++) acc[x] += ph * cq[x];
+                        }
+                        double* vr = V.data() + (size_t)r * ne + eb;
+                        if (t == 0)
+                            for (int x = 0; x < n; x++) vr[x] = acc[x];
+                        else
+                            for (int x = 0; x < n; x++) vr[x] += acc[x];
+                    }
                 }
+            });
         }
         if (sd.is_rhs)
             for (double& v : V) v = -v;
