@@ -248,6 +248,20 @@ slx_status slx_plan_kernel_counts(slx_plan plan, int32_t precision, int32_t* n_b
 slx_status slx_plan_solve(slx_plan plan, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
                           double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual,
                           int32_t* method_used);
+/* slx_plan_solve without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid allocation).  Same arguments, method choice,
+ * arithmetic and results as slx_plan_solve; the workspace (256 bytes, then
+ * 8 (3 nfree + ncons + n_dofs + 2056) bytes, then 4 (6 nfree + nnzA + 4)
+ * bytes, plus for GMRES 4 (restart + 1) nfree + 8 (restart + 1) 1025 bytes)
+ * lives in one fixed device array of SLX_SOLVE_SCRATCH_BYTES per GPU, held
+ * until the process ends; a system whose workspace exceeds it is rejected
+ * with SLX_ERR_INVALID before any device work.  Every call on a GPU shares
+ * that array: calls that overlap on one GPU race. */
+#define SLX_SOLVE_SCRATCH_BYTES ((size_t)128 << 20)
+slx_status slx_plan_solve_noalloc(slx_plan plan, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner,
+                                  double* x_out, double* d_x_out, int32_t* outer_its, int32_t* inner_its,
+                                  double* rel_residual, int32_t* method_used);
 /* mat::eliminate(b) (src/formulation/mat.cpp:78-103): b_A - D b_D of a
  * vector in DOF numbering (n_dofs entries; b NULL: the rhs of the last
  * assembly, whose constrained entries are the Dirichlet values), on the

@@ -108,6 +108,7 @@ def lib():
         "slx_plan_kernels": (i32, [vp, p32, p32]),
         "slx_plan_kernel_counts": (i32, [vp, i32, p32, p32]),
         "slx_plan_solve": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
+        "slx_plan_solve_noalloc": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
         "slx_plan_eliminate": (i32, [vp, pd, pd]),
         "slx_plan_coef_mode": (i32, [vp, i32, p32]),
         "slx_plan_matvec": (i32, [vp, pd, pd]),
@@ -143,7 +144,7 @@ def lib():
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_solve", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
@@ -470,11 +471,19 @@ class Plan:
         check).  Returns (x, info) with the refinement count, the inner
         iteration count, the final relative FP64 residual of b_A - D x_D - A x
         and the method used."""
+        return self._solve(lib().slx_plan_solve, ndofs, rel_tol, max_outer, max_inner, method)
+
+    def solve_noalloc(self, ndofs, rel_tol=1e-10, max_outer=20, max_inner=2000, method="auto"):
+        """The same solve with its workspace in the library's fixed per-GPU
+        array (slx_plan_solve_noalloc): no device allocation."""
+        return self._solve(lib().slx_plan_solve_noalloc, ndofs, rel_tol, max_outer, max_inner, method)
+
+    def _solve(self, fn, ndofs, rel_tol, max_outer, max_inner, method):
         x = np.zeros(max(ndofs, 1), np.float64)
         o, i, mu = C.c_int32(), C.c_int32(), C.c_int32()
         r = C.c_double()
-        check(lib().slx_plan_solve(self.h, self.SOLVE_METHODS[method], rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
-                                   C.byref(o), C.byref(i), C.byref(r), C.byref(mu)))
+        check(fn(self.h, self.SOLVE_METHODS[method], rel_tol, max_outer, max_inner, _p(x, C.c_double), None,
+                 C.byref(o), C.byref(i), C.byref(r), C.byref(mu)))
         names = {v: k for k, v in self.SOLVE_METHODS.items()}
         return x[:ndofs], dict(outer=o.value, inner=i.value, rel_residual=r.value, method=names.get(mu.value, "?"))
 

@@ -28,6 +28,7 @@
 namespace slx {
 hipError_t launch_coef(const DevBucket& b, const DevInputs& in, bool fp64, void* ctab, hipStream_t st, int qfast);
 hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask, int qfast);
+hipError_t scratch_init(int device);
 hipError_t launch_reduce(const DevReduce& rd, int max_erow_len, bool ke_fp64, const void* ke, double* Aval, double* Dval,
                          double* rhs, bool has_matrix, bool has_rhs, hipStream_t st, const int64_t* class_begin,
                          const int64_t* class_end, const int* class_len, int n_classes);
@@ -665,6 +666,14 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();
     P.rd.rhs_erows = P.d_rhs_erows.as<uint64_t>();
     P.rd.free_dofs = P.d_free.as<int64_t>();
+    // row blocks of the solve's streaming product (kept with the pattern, so
+    // that no solve allocates beyond its workspace)
+    if (nf > 0) {
+        std::vector<int64_t> blk;
+        stream_row_blocks(P.Arow.data(), nf, blk);
+        P.n_ablk = (int64_t)blk.size() - 1;
+        upload(P.d_ablk, blk.data(), blk.size(), st);
+    }
     // Ke buffer (FP64 capacity serves every precision)
     P.ke_capacity_bytes = std::max<int64_t>(P.n_ke, 1) * sizeof(double);
     P.d_ke.alloc(P.ke_capacity_bytes);
@@ -1420,6 +1429,8 @@ slx_status slx_create(int device, slx_ctx* out) {
         HIPCHECK(hipGetDeviceCount(&n));
         SLX_CHECK(device >= 0 && device < n, "device " << device << " not available (" << n << " devices)");
         HIPCHECK(hipSetDevice(device));
+        // device address of the no-allocation solve's workspace array (once per device)
+        HIPCHECK(scratch_init(device));
         auto c = new slx_ctx_s();
         c->device = device;
         if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -1576,15 +1587,9 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
 }
 
 namespace {
-// device view of the assembled system for the solve kernels (row blocks of
-// the streaming product built on first use)
+// device view of the assembled system for the solve kernels (the row blocks
+// of the streaming product are built with the pattern)
 SolveSystem solve_view(slx_plan p) {
-    if (p->n_ablk == 0 && p->nfree > 0) {
-        std::vector<int64_t> blk;
-        stream_row_blocks(p->Arow.data(), p->nfree, blk);
-        p->n_ablk = (int64_t)blk.size() - 1;
-        upload(p->d_ablk, blk.data(), blk.size(), p->stream);
-    }
     return SolveSystem{p->n_dofs, p->nfree, p->ncons, p->nnzA, p->nnzD, p->d_Arow.as<int64_t>(), p->d_Drow.as<int64_t>(),
                        p->d_Acol.as<int32_t>(), p->d_Dcol.as<int32_t>(), p->d_Aval.as<double>(), p->d_Dval.as<double>(),
                        p->d_rhs.as<double>(), p->d_free.as<int64_t>(), p->d_cons.as<int64_t>(), p->d_ablk.as<int64_t>(),
@@ -1638,74 +1643,119 @@ slx_status slx_plan_matvec(slx_plan p, const double* x, double* y) {
     });
 }
 
+namespace {
+// Device workspace of one solve, in bytes: a 256-byte header (the symmetry
+// check's flag), then the doubles, then the floats (16-byte aligned).
+struct SolveLayout {
+    int64_t nD, nF, nV;
+    size_t bytes;
+};
+SolveLayout solve_layout(slx_plan p, bool gm, int restart) {
+    const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
+    SolveLayout L;
+    const int64_t nDg = gm ? (int64_t)(restart + 1) * (SLX_SOLVE_BLOCKS + 1) : 0;
+    L.nD = 3 * nf + nc + nd + 2 * SLX_SOLVE_BLOCKS + 8 + nDg;
+    L.nF = 6 * nf + p->nnzA + 4;
+    L.nV = gm ? (int64_t)(restart + 1) * nf : 0;
+    L.bytes = 256 + (size_t)L.nD * sizeof(double) + (size_t)(L.nF + L.nV) * sizeof(float) + 64;
+    return L;
+}
+
+void check_solve_args(slx_plan p, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out, double* d_x_out) {
+    check_assembled(p);
+    SLX_CHECK(method == SLX_SOLVE_AUTO || method == SLX_SOLVE_CG || method == SLX_SOLVE_GMRES, "solve: unknown method " << method);
+    SLX_CHECK(rel_tol > 0 && max_outer > 0 && max_inner > 0, "solve: rel_tol, max_outer and max_inner must be positive");
+    SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
+}
+
+// The solve on caller workspace `ws` (device memory of `ws_bytes`, 256-byte
+// aligned); ws == nullptr: the workspace is allocated here for this call.
+// Allocates no device memory when ws is given.
+void solve_on(slx_plan p, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out, double* d_x_out,
+              int32_t* outer_its, int32_t* inner_its, double* rel_residual, int32_t* method_used, void* ws, size_t ws_bytes) {
+    HIPCHECK(hipSetDevice(p->ctx->device));
+    const int64_t nf = p->nfree, nd = p->n_dofs;
+    if (nf == 0) {
+        // every DOF constrained: the solution is the Dirichlet values, which
+        // the assembled rhs holds at the constrained entries
+        if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+        if (d_x_out && nd)
+            HIPCHECK(hipMemcpyAsync(d_x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+        HIPCHECK(hipStreamSynchronize(p->stream));
+        if (outer_its) *outer_its = 0;
+        if (inner_its) *inner_its = 0;
+        if (rel_residual) *rel_residual = 0.0;
+        if (method_used) *method_used = method == SLX_SOLVE_AUTO ? SLX_SOLVE_CG : method;
+        return;
+    }
+    const SolveSystem sys = solve_view(p);
+    DBuf own;
+    const int restart = std::min<int>(max_inner, SLX_GMRES_MAX);
+    if (method == SLX_SOLVE_AUTO) {
+        // the check's flag: the first word of the workspace (or of a small
+        // allocation of its own when the solve allocates)
+        int* flag = nullptr;
+        if (ws) {
+            SLX_CHECK(ws_bytes >= 256, "solve: workspace smaller than its header");
+            flag = static_cast<int*>(ws);
+        } else {
+            own.alloc(256);
+            flag = own.as<int>();
+        }
+        int nonsym = 0;
+        HIPCHECK(check_symmetric(sys, flag, &nonsym, p->stream));
+        method = nonsym ? SLX_SOLVE_GMRES : SLX_SOLVE_CG;
+    }
+    const bool gm = method == SLX_SOLVE_GMRES;
+    const SolveLayout L = solve_layout(p, gm, restart);
+    unsigned char* base = static_cast<unsigned char*>(ws);
+    if (ws) {
+        SLX_CHECK(L.bytes <= ws_bytes, "solve: the workspace of this system (" << L.bytes << " bytes: " << nf << " free rows, "
+                                          << p->nnzA << " nonzeros" << (gm ? ", GMRES basis" : "") << ") exceeds the "
+                                          << ws_bytes << " bytes available");
+    } else {
+        own.alloc(L.bytes);
+        base = own.as<unsigned char>();
+    }
+    SolveWork w;
+    double* dp = reinterpret_cast<double*>(base + 256);
+    w.f = dp; dp += nf;
+    w.r = dp; dp += nf;
+    w.x = dp; dp += nf;
+    w.xd = dp; dp += p->ncons;
+    w.xfull = dp; dp += nd;
+    w.part = dp; dp += 2 * SLX_SOLVE_BLOCKS;
+    w.sc = dp; dp += 8;
+    w.mpart = gm ? dp : nullptr; dp += gm ? (int64_t)(restart + 1) * SLX_SOLVE_BLOCKS : 0;
+    w.hv = gm ? dp : nullptr; dp += gm ? restart + 1 : 0;
+    float* fp = (float*)dp;
+    w.r32 = fp; fp += nf;
+    w.z = fp; fp += nf;
+    w.p = fp; fp += nf;
+    w.q = fp; fp += nf;
+    w.d = fp; fp += nf;
+    w.dinv = fp; fp += nf;
+    w.val32 = (float*)(((uintptr_t)fp + 15) & ~(uintptr_t)15);
+    w.V = gm ? w.val32 + p->nnzA + 4 : nullptr;
+    SolveParams prm{rel_tol, 1e-3, max_outer, max_inner, gm ? 1 : 0, restart};
+    SolveStats stats;
+    HIPCHECK(solve_system(sys, prm, w, stats, p->stream));
+    if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    if (d_x_out && nd) HIPCHECK(hipMemcpyAsync(d_x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
+    HIPCHECK(hipStreamSynchronize(p->stream));
+    if (outer_its) *outer_its = stats.outer;
+    if (inner_its) *inner_its = stats.inner;
+    if (rel_residual) *rel_residual = stats.rel_residual;
+    if (method_used) *method_used = method;
+}
+}  // namespace
+
 slx_status slx_plan_solve(slx_plan p, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
                           double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual, int32_t* method_used) {
     return guarded([&] {
-        check_assembled(p);
-        SLX_CHECK(method == SLX_SOLVE_AUTO || method == SLX_SOLVE_CG || method == SLX_SOLVE_GMRES, "solve: unknown method " << method);
-        SLX_CHECK(rel_tol > 0 && max_outer > 0 && max_inner > 0, "solve: rel_tol, max_outer and max_inner must be positive");
-        SLX_CHECK((x_out != nullptr) != (d_x_out != nullptr), "solve: give exactly one of x_out and d_x_out");
-        HIPCHECK(hipSetDevice(p->ctx->device));
-        const int64_t nf = p->nfree, nc = p->ncons, nd = p->n_dofs;
-        if (nf == 0) {
-            // every DOF constrained: the solution is the Dirichlet values, which
-            // the assembled rhs holds at the constrained entries
-            if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-            if (d_x_out && nd)
-                HIPCHECK(hipMemcpyAsync(d_x_out, p->d_rhs.as<double>(), nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
-            HIPCHECK(hipStreamSynchronize(p->stream));
-            if (outer_its) *outer_its = 0;
-            if (inner_its) *inner_its = 0;
-            if (rel_residual) *rel_residual = 0.0;
-            if (method_used) *method_used = method == SLX_SOLVE_AUTO ? SLX_SOLVE_CG : method;
-            return;
-        }
-        const SolveSystem sys = solve_view(p);
-        if (method == SLX_SOLVE_AUTO) {
-            DBuf flag;
-            flag.alloc(sizeof(int));
-            int nonsym = 0;
-            HIPCHECK(check_symmetric(sys, flag.as<int>(), &nonsym, p->stream));
-            method = nonsym ? SLX_SOLVE_GMRES : SLX_SOLVE_CG;
-        }
-        const bool gm = method == SLX_SOLVE_GMRES;
-        const int restart = std::min<int>(max_inner, SLX_GMRES_MAX);
-        // one workspace block: doubles first, then floats (8-byte aligned)
-        const int64_t nDg = gm ? (int64_t)(restart + 1) * (SLX_SOLVE_BLOCKS + 1) : 0;
-        const int64_t nD = 3 * nf + nc + nd + 2 * SLX_SOLVE_BLOCKS + 8 + nDg, nF = 6 * nf + p->nnzA + 4;
-        const int64_t nV = gm ? (int64_t)(restart + 1) * nf : 0;
-        DBuf ws;
-        ws.alloc((size_t)(nD * sizeof(double) + (nF + nV) * sizeof(float) + 64));
-        SolveWork w;
-        double* dp = ws.as<double>();
-        w.f = dp; dp += nf;
-        w.r = dp; dp += nf;
-        w.x = dp; dp += nf;
-        w.xd = dp; dp += nc;
-        w.xfull = dp; dp += nd;
-        w.part = dp; dp += 2 * SLX_SOLVE_BLOCKS;
-        w.sc = dp; dp += 8;
-        w.mpart = gm ? dp : nullptr; dp += gm ? (int64_t)(restart + 1) * SLX_SOLVE_BLOCKS : 0;
-        w.hv = gm ? dp : nullptr; dp += gm ? restart + 1 : 0;
-        float* fp = (float*)dp;
-        w.r32 = fp; fp += nf;
-        w.z = fp; fp += nf;
-        w.p = fp; fp += nf;
-        w.q = fp; fp += nf;
-        w.d = fp; fp += nf;
-        w.dinv = fp; fp += nf;
-        w.val32 = (float*)(((uintptr_t)fp + 15) & ~(uintptr_t)15);
-        w.V = gm ? w.val32 + p->nnzA + 4 : nullptr;
-        SolveParams prm{rel_tol, 1e-3, max_outer, max_inner, gm ? 1 : 0, restart};
-        SolveStats stats;
-        HIPCHECK(solve_system(sys, prm, w, stats, p->stream));
-        if (x_out && nd) HIPCHECK(hipMemcpyAsync(x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToHost, p->stream));
-        if (d_x_out && nd) HIPCHECK(hipMemcpyAsync(d_x_out, w.xfull, nd * sizeof(double), hipMemcpyDeviceToDevice, p->stream));
-        HIPCHECK(hipStreamSynchronize(p->stream));
-        if (outer_its) *outer_its = stats.outer;
-        if (inner_its) *inner_its = stats.inner;
-        if (rel_residual) *rel_residual = stats.rel_residual;
-        if (method_used) *method_used = method;
+        check_solve_args(p, method, rel_tol, max_outer, max_inner, x_out, d_x_out);
+        solve_on(p, method, rel_tol, max_outer, max_inner, x_out, d_x_out, outer_its, inner_its, rel_residual, method_used,
+                 nullptr, 0);
     });
 }
 
@@ -2080,3 +2130,20 @@ slx_status slx_problem_setdata(slx_problem p, int32_t field, const double* value
 }
 
 }  // extern "C"
+
+namespace slx {
+// Entry of the no-allocation solve (kernels/noalloc.hip): the same solve on
+// the caller's device workspace.
+slx_status solve_on_workspace(slx_plan p, int32_t method, double rel_tol, int32_t max_outer, int32_t max_inner, double* x_out,
+                              double* d_x_out, int32_t* outer_its, int32_t* inner_its, double* rel_residual, int32_t* method_used,
+                              void* ws, size_t ws_bytes) {
+    return guarded([&] {
+        check_solve_args(p, method, rel_tol, max_outer, max_inner, x_out, d_x_out);
+        SLX_CHECK(ws && ((uintptr_t)ws & 255u) == 0, "solve: workspace must be 256-byte aligned device memory");
+        solve_on(p, method, rel_tol, max_outer, max_inner, x_out, d_x_out, outer_its, inner_its, rel_residual, method_used, ws,
+                 ws_bytes);
+    });
+}
+int plan_device(slx_plan p) { return p && p->ctx ? p->ctx->device : -1; }
+void set_last_error(const char* msg) { g_last_error = msg; }
+}  // namespace slx
