@@ -192,7 +192,17 @@ slx_status slx_assemble(slx_plan plan, const slx_numeric_desc* in);
 slx_status slx_assemble_device(slx_plan plan, const double* d_node_xyz, const double* d_params,
                                int64_t param_len, const double* const* d_field_data, int32_t precision,
                                void* hip_stream);
+/* slx_assemble on a caller stream (SURVEY's slx_assemble(plan, desc,
+ * stream)): the host inputs are copied to the device before it returns, the
+ * numeric phase is enqueued on hip_stream (NULL: the plan's stream) and runs
+ * asynchronously to the host. */
+slx_status slx_assemble_async(slx_plan plan, const slx_numeric_desc* in, void* hip_stream);
 slx_status slx_plan_sync(slx_plan plan);
+/* mat::getainds / mat::getdinds (src/formulation/mat.cpp:105-106): the DOF
+ * numbers of the rows and columns of A (the free DOFs, ascending: nfree
+ * entries) and of the columns of D (the constrained DOFs, ascending: ncons
+ * entries).  Host memory; either pointer may be NULL.  Pattern-only plans too. */
+slx_status slx_plan_dof_maps(slx_plan plan, int64_t* ainds, int64_t* dinds);
 /* Captures one numeric phase on these device inputs into a HIP graph owned by
  * the plan (for Newton loops: update the field data in place, relaunch).  The
  * captured work is kernels only (no memset, no allocation, no host sync). */

@@ -101,6 +101,8 @@ def lib():
         "slx_assemble": (i32, [vp, C.POINTER(NumericDesc)]),
         "slx_assemble_device": (i32, [vp, vp, vp, i64, vp, i32, vp]),
         "slx_plan_sync": (i32, [vp]),
+        "slx_assemble_async": (i32, [vp, C.POINTER(NumericDesc), vp]),
+        "slx_plan_dof_maps": (i32, [vp, p64, p64]),
         "slx_plan_profile": (i32, [vp, vp, vp, i64, vp, i32, i32, pd]),
         "slx_csr_device": (i32, [vp] + [C.POINTER(vp)] * 7),
         "slx_csr_copyout": (i32, [vp, p64, p32, pd, p64, p32, pd, pd]),
@@ -143,7 +145,7 @@ def lib():
 
 EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
-    "slx_assemble_device", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
+    "slx_assemble_device", "slx_assemble_async", "slx_plan_dof_maps", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
     "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
@@ -423,6 +425,18 @@ class Plan:
     def assemble(self, numeric, precision):
         numeric.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
         check(lib().slx_assemble(self.h, C.byref(numeric)))
+
+    def dof_maps(self):
+        """mat::getainds / getdinds: DOF numbers of A's rows (free) and D's columns (constrained)."""
+        s = self.shape()
+        a = np.zeros(max(s["nfree"], 1), np.int64)
+        d = np.zeros(max(s["ncons"], 1), np.int64)
+        check(lib().slx_plan_dof_maps(self.h, _p(a, C.c_int64), _p(d, C.c_int64)))
+        return a[:s["nfree"]], d[:s["ncons"]]
+
+    def assemble_async(self, numeric, precision, stream=None):
+        numeric.precision = PRECISIONS[precision] if isinstance(precision, str) else precision
+        check(lib().slx_assemble_async(self.h, C.byref(numeric), stream))
 
     def assemble_device(self, d_xyz, d_params, param_len, d_fields, precision, stream=None):
         check(lib().slx_assemble_device(self.h, d_xyz, d_params, param_len, d_fields,

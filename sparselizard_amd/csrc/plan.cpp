@@ -1497,10 +1497,49 @@ slx_status slx_plan_stats(slx_plan p, int64_t* n_elem_rows, int64_t* n_ke, int64
     });
 }
 
+namespace {
+void assemble_host_inputs(slx_plan p, const slx_numeric_desc* in, hipStream_t st);
+}
+
 slx_status slx_assemble(slx_plan p, const slx_numeric_desc* in) {
     return guarded([&] {
         SLX_CHECK(p && in, "null argument");
+        assemble_host_inputs(p, in, p->stream);
+    });
+}
+
+slx_status slx_assemble_async(slx_plan p, const slx_numeric_desc* in, void* hip_stream) {
+    return guarded([&] {
+        SLX_CHECK(p && in, "null argument");
+        assemble_host_inputs(p, in, hip_stream ? (hipStream_t)hip_stream : p->stream);
+    });
+}
+
+slx_status slx_plan_dof_maps(slx_plan p, int64_t* ainds, int64_t* dinds) {
+    return guarded([&] {
+        SLX_CHECK(p && p->src, "null plan");
+        const slx_plan_desc& d = *p->src;
+        int64_t na = 0, nc = 0;
+        for (int64_t i = 0; i < d.n_dofs; i++) {
+            const bool c = d.constrained && d.constrained[i];
+            if (c) {
+                if (dinds) dinds[nc] = i;
+                nc++;
+            } else {
+                if (ainds) ainds[na] = i;
+                na++;
+            }
+        }
+    });
+}
+
+namespace {
+void assemble_host_inputs(slx_plan p, const slx_numeric_desc* in, hipStream_t st) {
+    {
         SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
+        // the input buffers are replaced below: earlier work that reads them must be done
+        HIPCHECK(hipStreamSynchronize(st));
+        if (st != p->stream) HIPCHECK(hipStreamSynchronize(p->stream));
         const slx_plan_desc& d = *p->src;
         upload(p->d_xyz, in->node_xyz, (size_t)3 * d.n_nodes, p->stream);
         SLX_CHECK(in->n_params > p->max_param, "numeric description has " << in->n_params << " parameters, programs read index " << p->max_param);
@@ -1517,9 +1556,10 @@ slx_status slx_assemble(slx_plan p, const slx_numeric_desc* in) {
             upload(*p->d_fields[f], in->field_data[f], (size_t)std::max<int64_t>(in->field_len[f], 1), p->stream);
             fptr[f] = p->d_fields[f]->as<double>();
         }
-        numeric(*p, p->d_xyz.as<double>(), p->d_params.as<double>(), in->param_len, fptr.data(), in->precision, p->stream);
-    });
+        numeric(*p, p->d_xyz.as<double>(), p->d_params.as<double>(), in->param_len, fptr.data(), in->precision, st);
+    }
 }
+}  // namespace
 
 slx_status slx_assemble_device(slx_plan p, const double* d_node_xyz, const double* d_params, int64_t param_len,
                                const double* const* d_field_data, int32_t precision, void* hip_stream) {

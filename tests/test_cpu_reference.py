@@ -443,3 +443,15 @@ def test_reduction_maps_replay_oracle(kind, spec):
     cons = np.ctypeslib.as_array(d.constrained, shape=(n,)).astype(bool) if d.constrained else np.zeros(n, bool)
     if np.linalg.norm(o.rhs[~cons]) > 0:
         assert rel(rhs_free, o.rhs[~cons]) < 1e-13
+
+
+def test_dof_maps_free_and_constrained():
+    # mat::getainds / getdinds (mat.cpp:105-106): A's rows are the free DOFs in
+    # ascending order, D's columns the constrained ones
+    p = sa.Problem(None, "poisson2d", "n=5")
+    info = p.info()
+    plan = p.plan()
+    a, d = plan.dof_maps()
+    cons = np.ctypeslib.as_array(p.desc().constrained, shape=(info["ndofs"],)).astype(bool)
+    assert np.array_equal(a, np.flatnonzero(~cons)) and np.array_equal(d, np.flatnonzero(cons))
+    assert len(a) == info["nfree"]

@@ -4,6 +4,8 @@ Pattern (rowptr / column indices of A and D) must be bit-exact; values match
 to the stated relative Frobenius tolerance.  Every case runs through the C ABI
 (slx_problem_generate -> slx_plan_create / slx_assemble / slx_csr_copyout).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -221,3 +223,18 @@ def test_plan_keeps_its_context_alive():
     again = prob.generate("fp64").csr(info["ndofs"])
     assert np.array_equal(first.Aval, again.Aval)
     prob.close()
+
+
+def test_assemble_async_on_caller_stream(ctx):
+    # slx_assemble_async on a torch stream gives the matrix of slx_assemble
+    import torch
+    print("case assemble_async", flush=True)
+    prob = sa.Problem(ctx, "elasticity3d", "n=3")
+    info = prob.info()
+    ref = prob.generate("f16split").csr(info["ndofs"])
+    plan = prob.plan()
+    s = torch.cuda.Stream()
+    plan.assemble_async(prob.numeric(), "f16split", C.c_void_p(s.cuda_stream))
+    s.synchronize()
+    got = plan.csr(info["ndofs"])
+    assert np.array_equal(got.Aval, ref.Aval) and np.array_equal(got.rhs, ref.rhs)
