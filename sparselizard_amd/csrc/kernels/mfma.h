@@ -4,7 +4,9 @@
 #include "device.h"
 
 #define SLX_MFMA_MAXP 16   // derivative pairs of a batched-MFMA bucket
-#define SLX_MFMA_PWAVES 8  // producer waves of the batched-MFMA workgroup (half prepare, half write)
+#define SLX_MFMA_PREP_WAVES 3   // producer waves of the batched-MFMA workgroup that prepare columns
+#define SLX_MFMA_WRITE_WAVES 2  // producer waves that write element matrices out
+#define SLX_MFMA_PWAVES (SLX_MFMA_PREP_WAVES + SLX_MFMA_WRITE_WAVES)
 
 namespace slx {
 
@@ -45,6 +47,10 @@ struct MfmaBucket {
     const int32_t* sgrp;        // [n_mslices] group of each slice
     const int32_t* scol;        // [n_mslices] column offset of each slice in its group
     int32_t slice_idx[SLX_MAX_SLICES];
+    // [n_mslices][3] per slice, for the consumers' output stores into the LDS
+    // staging of a tile: word offset of the slice's first column in its
+    // group's block, the group's segment length, the staging words per element
+    const int32_t* sprm;
     // host side: the fragments of every arithmetic and which arithmetics fit
     const void* afrag_x[AR_COUNT];
     uint32_t arith_ok;          // bit a: arithmetic a has a kernel variant for this bucket

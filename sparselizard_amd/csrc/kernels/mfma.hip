@@ -43,6 +43,7 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
+typedef unsigned int uint2v __attribute__((ext_vector_type(2)));
 
 __device__ inline uint32_t pack_split(float x) {
     _Float16 hi = (_Float16)x;
@@ -109,8 +110,10 @@ __device__ inline void store_pad(uint32_t* dst, int np) {
 //     columns of tile s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from
 //     stg[(s - 1) & 1] to the element matrices with coalesced 16-byte stores.
 // One barrier per tile; every buffer is written one tile after its last read.
-template <int AR, int MTW, int MAXK, bool COMPACT, int G>
-__global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
+// WAVES: the largest workgroup of the variant, in waves (12 leaves 168 VGPRs
+// per wave: up to 7 consumer waves; 16 leaves 128).
+template <int AR, int MTW, int MAXK, bool COMPACT, int G, int WAVES>
+__global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, MfmaBucket mb, const float* __restrict__ ctab,
                                                                                   float* __restrict__ ke) {
     static_assert(COMPACT || AR == AR_F16, "per-Gauss-point columns are built for the FP16 split only");
     constexpr int NPL = AR == AR_BF16 ? 3 : 2;  // A fragment planes
@@ -123,11 +126,11 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
     float* colscb = reinterpret_cast<float*>(Csb + (size_t)2 * NE * estride);   // [2][ns][NE]
     float* stgb = colscb + (size_t)2 * ns * NE;                                 // [2][16 * stg_words]
     int* csl = reinterpret_cast<int*>(stgb + (size_t)2 * 16 * mb.stg_words);    // [ns][np] coefficient slots
+    float* trash = reinterpret_cast<float*>(csl + ns * np);                      // [64] sink of padding-row stores
     const SLX_CONST int32_t* csrc = (const SLX_CONST int32_t*)mb.csrc;
+    const SLX_CONST int32_t* sprmc = (const SLX_CONST int32_t*)mb.sprm;
     const SLX_CONST int64_t* gbase = (const SLX_CONST int64_t*)mb.gbase;
     const SLX_CONST int32_t* gld = (const SLX_CONST int32_t*)mb.gld;
-    const SLX_CONST int32_t* sgrp = (const SLX_CONST int32_t*)mb.sgrp;
-    const SLX_CONST int32_t* scol = (const SLX_CONST int32_t*)mb.scol;
     for (int x = threadIdx.x; x < ns * np; x += blockDim.x) csl[x] = csrc[x];
 
     const int nw = mb.nw;
@@ -180,6 +183,98 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
             for (int kb = 0; kb < 4; kb++) pw32[ks][kb] = min((ks * 32 + 8 * kb) / mb.kq, np);
         }
         __syncthreads();  // tile 0 prepared
+        if constexpr (COMPACT && G == 1 && MTW == 1 && AR != AR_F32) {
+            // One M tile per wave (wave < mtiles), one slice at a time, software
+            // pipelined: the MFMAs of slice s and the output stores of slice
+            // s - 1 form one basic block (branch-free stores: rows past M go to
+            // the sink), so the stores issue while the MFMA pipe is busy.
+            auto contract = [&](const uint32_t* colbase, float4v& hh, float4v& xx) {
+                uint32_t cw[MAXK][AR == AR_BF16 ? 2 : (AR == AR_F32 ? 4 : 1)];
+#pragma unroll
+                for (int ks = 0; ks < MAXK; ks++) {
+                    if constexpr (AR == AR_F16) {
+                        cw[ks][0] = colbase[pword[ks]];
+                    } else if constexpr (AR == AR_BF16) {
+                        cw[ks][0] = colbase[pword[ks]];
+                        cw[ks][1] = colbase[pword[ks] + 1];
+                    } else {
+#pragma unroll
+                        for (int kb = 0; kb < 4; kb++) cw[ks][kb] = colbase[pw32[ks][kb]];
+                    }
+                }
+                hh = float4v{0.f, 0.f, 0.f, 0.f};
+                xx = float4v{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int ks = 0; ks < MAXK; ks++) {
+                    if constexpr (AR == AR_F16) {
+                        const uint32_t u = cw[ks][0];
+                        const uint32_t h2 = __builtin_amdgcn_perm(u, u, 0x01000100u);
+                        const uint32_t l2 = __builtin_amdgcn_perm(u, u, 0x03020302u);
+                        const half8 bhi = __builtin_bit_cast(half8, (uint4v){h2, h2, h2, h2});
+                        const half8 blo = __builtin_bit_cast(half8, (uint4v){l2, l2, l2, l2});
+                        const half8 ahi = __builtin_bit_cast(half8, Af[0][ks][0]);
+                        const half8 alo = __builtin_bit_cast(half8, Af[0][ks][1]);
+                        hh = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, hh, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, xx, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, xx, 0, 0, 0);
+                    } else if constexpr (AR == AR_BF16) {
+                        const uint32_t u0 = cw[ks][0], u1 = cw[ks][1];
+                        const uint32_t h2 = __builtin_amdgcn_perm(u0, u0, 0x01000100u);
+                        const uint32_t m2 = __builtin_amdgcn_perm(u0, u0, 0x03020302u);
+                        const uint32_t l2 = __builtin_amdgcn_perm(u1, u1, 0x01000100u);
+                        const bf16x8 bh = __builtin_bit_cast(bf16x8, (uint4v){h2, h2, h2, h2});
+                        const bf16x8 bm = __builtin_bit_cast(bf16x8, (uint4v){m2, m2, m2, m2});
+                        const bf16x8 bl = __builtin_bit_cast(bf16x8, (uint4v){l2, l2, l2, l2});
+                        const bf16x8 ah = __builtin_bit_cast(bf16x8, Af[0][ks][0]);
+                        const bf16x8 am = __builtin_bit_cast(bf16x8, Af[0][ks][1]);
+                        const bf16x8 al = __builtin_bit_cast(bf16x8, Af[0][ks][2]);
+                        hh = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, hh, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, xx, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, xx, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, xx, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, xx, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, xx, 0, 0, 0);
+                    } else {
+                        const float4v a0 = __builtin_bit_cast(float4v, Af[0][ks][0]);
+                        const float4v a1 = __builtin_bit_cast(float4v, Af[0][ks][1]);
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            hh = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[j], __uint_as_float(cw[ks][j >> 1]), hh, 0, 0, 0);
+#pragma unroll
+                        for (int j = 0; j < 4; j++)
+                            hh = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[j], __uint_as_float(cw[ks][2 + (j >> 1)]), hh, 0, 0, 0);
+                    }
+                }
+            };
+            auto store = [&](int s, const float* colsc, float* stg, const float4v& hh, const float4v& xx) {
+                const int soff = sprmc[3 * s], ld = sprmc[3 * s + 1], ssrow = sprmc[3 * s + 2];
+                const float colscale = colsc[s * NE + lrow];
+                const float xscale = AR == AR_F16 ? colscale * (1.f / 2048.f) : colscale;
+                float* out = stg + soff + lrow * ssrow;
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const float val = AR == AR_F32 ? hh[v] * colscale : hh[v] * colscale + xx[v] * xscale;
+                    float* dst = rowok[0][v] ? out + irow[0][v] * ld + jrow[0][v] : trash + lane;
+                    *dst = val;
+                }
+            };
+            for (int st = 0; st < nsteps; st++) {
+                const uint32_t* colb = Csb + (size_t)(st & 1) * NE * estride + (size_t)lrow * estride;
+                const float* colsc = colscb + (size_t)(st & 1) * ns * NE;
+                float* stg = stgb + (size_t)(st & 1) * 16 * mb.stg_words;
+                float4v hh0, xx0;
+                contract(colb, hh0, xx0);
+                for (int s = 1; s < ns; s++) {
+                    float4v hh1, xx1;
+                    contract(colb + (size_t)s * cstride, hh1, xx1);
+                    store(s - 1, colsc, stg, hh0, xx0);
+                    hh0 = hh1;
+                    xx0 = xx1;
+                }
+                store(ns - 1, colsc, stg, hh0, xx0);
+                __syncthreads();  // tile st staged; tile st + 1 prepared
+            }
+        } else
         for (int st = 0; st < nsteps; st++) {
             const uint32_t* Cs = Csb + (size_t)(st & 1) * NE * estride;
             const float* colsc = colscb + (size_t)(st & 1) * ns * NE;
@@ -287,23 +382,27 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
                         }
                     }
                 }
+                // the output stores stay below the MFMAs (hoisting their loads and
+                // addresses above them costs registers)
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int c = 0; c < G; c++) {
                     const int s = s0 + c;
-                    const int g = sgrp[s];
-                    float* gstg = stg;  // LDS block of group g: [16][n_tf * ld + 4]
-                    for (int g2 = 0; g2 < g; g2++) gstg += (size_t)16 * (n_tf * gld[g2] + 4);
-                    const int ld = gld[g], srow = n_tf * ld + 4;
+                    // the group's LDS block: [16][n_tf * ld + 4] words per tile;
+                    // the slice's three constants: independent scalar loads
+                    const int soff = sprmc[3 * s], ld = sprmc[3 * s + 1], ssrow = sprmc[3 * s + 2];
                     const float colscale = colsc[s * NE + lrow];
                     const float xscale = AR == AR_F16 ? colscale * (1.f / 2048.f) : colscale;
-                    float* out = gstg + (size_t)lrow * srow + scol[s];
+                    float* out = stg + soff + lrow * ssrow;
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
                         const int mt = wave + nw * i;
                         if (mt >= mb.mtiles) continue;
+                        // wave-uniform: every row of a full tile is a matrix row
+                        const bool full = (mt + 1) * 16 <= mb.M;
 #pragma unroll
                         for (int v = 0; v < 4; v++) {
-                            if (!rowok[i][v]) continue;
+                            if (!full && !rowok[i][v]) continue;
                             const float val = AR == AR_F32 ? acc_hh[c][i][v] * colscale
                                                            : acc_hh[c][i][v] * colscale + acc_x[c][i][v] * xscale;
                             out[irow[i][v] * ld + jrow[i][v]] = val;
@@ -316,10 +415,10 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
     } else {
         // ------------------------------------------------------- producers --
         // two producer groups: the first prepares columns, the second writes out
-        const int np_threads = 32 * SLX_MFMA_PWAVES;
+        constexpr int prep_threads = 64 * SLX_MFMA_PREP_WAVES, write_threads = 64 * SLX_MFMA_WRITE_WAVES;
         const int pt0 = threadIdx.x - 64 * nw;
-        const bool is_prep = pt0 < np_threads;  // wave-uniform
-        const int pt = is_prep ? pt0 : pt0 - np_threads;
+        const bool is_prep = pt0 < prep_threads;  // wave-uniform
+        const int pt = is_prep ? pt0 : pt0 - prep_threads;
         // columns of a tile: one thread per (element, slice), element fastest:
         // power-of-two scale and the split words
         auto prepare = [&](int tile, int buf) {
@@ -327,7 +426,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
             const int nloc = min(NE, b.n_elem - e0);
             uint32_t* Cs = Csb + (size_t)buf * NE * estride;
             float* colsc = colscb + (size_t)buf * ns * NE;
-            for (int col = pt; col < NE * ns; col += np_threads) {
+            for (int col = pt; col < NE * ns; col += prep_threads) {
                 const int el = col % NE, s = col / NE;
                 const bool valid = el < nloc;
                 const int64_t e = e0 + (valid ? el : 0);
@@ -393,8 +492,8 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
                     const int per = erow / 4;  // float4 per element row
                     float4v* d4 = reinterpret_cast<float4v*>(dst);
                     int eo = pt / per, w = pt - eo * per;
-                    const int step_e = np_threads / per, step_w = np_threads - step_e * per;
-                    for (int x = pt; x < nel_here * per; x += np_threads) {
+                    const int step_e = write_threads / per, step_w = write_threads - step_e * per;
+                    for (int x = pt; x < nel_here * per; x += write_threads) {
                         d4[x] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
                         eo += step_e;
                         w += step_w;
@@ -404,7 +503,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
                         }
                     }
                 } else {
-                    for (int x = pt; x < nel_here * erow; x += np_threads) {
+                    for (int x = pt; x < nel_here * erow; x += write_threads) {
                         const int eo = x / erow, w = x - eo * erow;
                         dst[x] = gsrc[(size_t)eo * srow + w];
                     }
@@ -444,7 +543,7 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
                 if (p < np) store_pair<AR>(dst, p, cvn[p] * sc);
             store_pad<AR>(dst, np);
         };
-        const bool pipelined = COMPACT && NE * ns <= np_threads;
+        const bool pipelined = COMPACT && NE * ns <= prep_threads;
         auto tile_of = [&](int st) { return (int)blockIdx.x + st * (int)gridDim.x; };
         if (is_prep) {
             if (pipelined) {
@@ -477,6 +576,15 @@ __global__ void __launch_bounds__(64 * (8 + SLX_MFMA_PWAVES)) mfma_contract_kern
     }
 }
 
+template <int AR, int MTW, int MAXK, bool COMPACT, int G>
+static void launch_waves(const dim3& gd, const dim3& bd, size_t lds, hipStream_t st, const DevBucket& b, const MfmaBucket& mb,
+                         const float* ctab, float* ke) {
+    if (bd.x <= 64 * 12)
+        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, COMPACT, G, 12>), gd, bd, lds, st, b, mb, ctab, ke);
+    else
+        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, COMPACT, G, 16>), gd, bd, lds, st, b, mb, ctab, ke);
+}
+
 template <int AR, int MTW, int MAXK>
 static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const float* ctab, float* ke, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(mb);
@@ -486,23 +594,23 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
     if constexpr (AR != AR_F16) {
         // compact columns, one M tile per wave, one slice at a time
         if (!mb.compact || MTW != 1) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((mfma_contract_kernel<AR, 1, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+        launch_waves<AR, 1, MAXK, true, 1>(gd, bd, lds, st, b, mb, ctab, ke);
     } else if (!mb.compact || MTW != 1 || mb.sgroup == 1) {
         if (mb.compact)
-            hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+            launch_waves<AR, MTW, MAXK, true, 1>(gd, bd, lds, st, b, mb, ctab, ke);
         else
-            hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, false, 1>), gd, bd, lds, st, b, mb, ctab, ke);
+            launch_waves<AR, MTW, MAXK, false, 1>(gd, bd, lds, st, b, mb, ctab, ke);
     } else if (mb.sgroup == 2) {
-        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, (MTW == 1 ? 2 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+        launch_waves<AR, MTW, MAXK, true, (MTW == 1 ? 2 : 1)>(gd, bd, lds, st, b, mb, ctab, ke);
     } else {
-        hipLaunchKernelGGL((mfma_contract_kernel<AR, MTW, MAXK, true, (MTW == 1 ? 3 : 1)>), gd, bd, lds, st, b, mb, ctab, ke);
+        launch_waves<AR, MTW, MAXK, true, (MTW == 1 ? 3 : 1)>(gd, bd, lds, st, b, mb, ctab, ke);
     }
     return hipGetLastError();
 }
 
 size_t mfma_lds_bytes(const MfmaBucket& mb) {
     return (size_t)2 * 16 * mb.estride * 4 + (size_t)2 * mb.n_mslices * 16 * 4 + (size_t)2 * 16 * mb.stg_words * 4 +
-           (size_t)mb.n_mslices * mb.n_pairs * 4;
+           (size_t)mb.n_mslices * mb.n_pairs * 4 + 64 * 4;
 }
 
 bool mfma_variant_fits(int mtiles, int ksteps, int* nw, int* mtw, int* maxk, int prefer_mtw) {

@@ -872,7 +872,13 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.ksteps = (mb.K + 31) / 32;
     mb.n_mslices = (int)ms.size();
     mb.sgroup = 1;
-    if (mb.compact) mb.sgroup = mb.n_mslices % 3 == 0 ? 3 : (mb.n_mslices % 2 == 0 ? 2 : 1);
+    // one slice at a time (the software-pipelined consumer loop); SLX_TUNE_MFMA_SGROUP
+    // = 2 or 3 takes compact slices in groups of that many (tuning knob)
+    mb.sgroup = 1;
+    if (const char* t = std::getenv("SLX_TUNE_MFMA_SGROUP")) {
+        const int g = std::atoi(t);
+        if (mb.compact && (g == 2 || g == 3) && mb.n_mslices % g == 0) mb.sgroup = g;
+    }
     // tile split: SLX_TUNE_MFMA_MTW=2 selects two M tiles per wave (tuning knob)
     int prefer_mtw = tuned == TK_MFMA ? tuned_mtw : 1;
     if (const char* t = std::getenv("SLX_TUNE_MFMA_MTW")) prefer_mtw = std::atoi(t);
@@ -907,6 +913,15 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.max_gld = *std::max_element(gd.begin(), gd.end());
     mb.stg_words = 0;
     for (int32_t l : gd) mb.stg_words += n_tf * l + 4;
+    // per slice: staging offset (group block + column), segment length, staging row
+    std::vector<int32_t> sprm(3 * ms.size());
+    for (size_t si = 0; si < ms.size(); si++) {
+        int off = 0;
+        for (int g2 = 0; g2 < sgrp[si]; g2++) off += 16 * (n_tf * gd[g2] + 4);
+        sprm[3 * si] = off + scol[si];
+        sprm[3 * si + 1] = gd[sgrp[si]];
+        sprm[3 * si + 2] = n_tf * gd[sgrp[si]] + 4;
+    }
     // 16-element tiles; double-buffered columns and staging of one workgroup per CU
     mb.ne_tile = 16;
     if (mfma_lds_bytes(mb) > 160 * 1024) return;
@@ -985,6 +1000,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     mb.gld = P.dev(gd.data(), gd.size());
     mb.sgrp = P.dev(sgrp.data(), sgrp.size());
     mb.scol = P.dev(scol.data(), scol.size());
+    mb.sprm = P.dev(sprm.data(), sprm.size());
     mb.max_blocks = P.n_cu;
     P.mfma[b] = mb;
     P.mfma_mask[b] = mask;
