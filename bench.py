@@ -218,7 +218,7 @@ def main():
             sd = bd.slices[s_]
             if not sd.is_rhs:
                 con_bytes += bd.n_elem * sd.n_tf * sd.n_dof * 4.0
-    kname = ("factor_f16split_kernel" if args.problem == "hppoisson3d" else "mfma_f16split_kernel") \
+    kname = ("factor_contract_kernel" if args.problem == "hppoisson3d" else "mfma_contract_kernel") \
         if prof_mix["mfma"] > 0 else "elem_cols"
     if t_contract >= prof_mix["reduce"]:
         # the roofline that bounds the kernel: the larger of the two ideal times
