@@ -180,8 +180,19 @@ slx_status slx_destroy(slx_ctx ctx);
  * split with local renumbering and the reduction maps.  Pattern equals
  * rawmat::process (rawmat.cpp:127-298) on the same address tables.
  * With ctx == NULL only the symbolic phase runs (host memory, no device):
- * the pattern can be read with slx_csr_copyout (value pointers NULL). */
+ * the pattern can be read with slx_csr_copyout (value pointers NULL).
+ * With a context the phase runs on the device (sorts and per-row column
+ * kernels; the same arrays as the host phase, bit for bit); a row of more
+ * than 8192 element-matrix entries or 1024 element rows makes it fall back
+ * to the host phase. */
 slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out);
+/* slx_plan_create with flags: SLX_PLAN_HOST_SYMBOLIC runs the symbolic phase
+ * on the host even with a context (the reference path for the device phase). */
+#define SLX_PLAN_HOST_SYMBOLIC 1
+slx_status slx_plan_create_ex(slx_ctx ctx, const slx_plan_desc* desc, int32_t flags, slx_plan* out);
+/* Where the plan's symbolic phase ran (1 device, 0 host) and its wall time
+ * in seconds (layout, uploads of the bucket data and the phase itself). */
+slx_status slx_plan_symbolic_info(slx_plan plan, int32_t* on_device, double* seconds);
 slx_status slx_plan_shape(slx_plan plan, int64_t* nfree, int64_t* ncons, int64_t* nnzA, int64_t* nnzD);
 /* Numeric phase: uploads the numeric inputs and assembles A, D and b on the
  * plan's stream (asynchronous; slx_plan_sync waits). */
@@ -312,6 +323,15 @@ slx_status slx_plan_coefficients(slx_plan plan, double* out, int64_t cap, int64_
  * Any output pointer may be NULL. */
 slx_status slx_plan_reduction_maps(slx_plan plan, int64_t* sizes, int64_t* erow_ptr, uint64_t* erows, int64_t* slot_ptr,
                                    int32_t* slots, int64_t* rhs_ptr, uint64_t* rhs_erows, int64_t* ke_base, int64_t* ke_ld);
+/* Diagnostic: a copy of one device array of the symbolic phase, to compare
+ * the device and host phases.  which: 0 reduction descriptors (4 words per
+ * work position), 1 element-row segments in work order, 2 slots in work
+ * order (slot_bytes each), 3 rhs segment pointers, 4 rhs segments, 5 free
+ * DOFs, 6 constrained DOFs, 7 row blocks of the solve's product, 8 reduction
+ * classes (begin, end, longest row: int64 triples), 9 scalars (int64: slot
+ * bytes, longest row, longest segment, segments, COO entries).  *n_bytes
+ * receives the size; out may be NULL to query it. */
+slx_status slx_plan_symbolic_array(slx_plan plan, int32_t which, void* out, int64_t cap_bytes, int64_t* n_bytes);
 
 /* Diagnostic: runs one v_mfma_f32_16x16x32_f16 on fixed operands and returns
  * the 16x16 result (row-major) so the operand lane layout the kernels assume

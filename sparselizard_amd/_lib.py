@@ -96,6 +96,9 @@ def lib():
         "slx_create": (i32, [C.c_int, C.POINTER(vp)]),
         "slx_destroy": (i32, [vp]),
         "slx_plan_create": (i32, [vp, vp, C.POINTER(vp)]),
+        "slx_plan_create_ex": (i32, [vp, vp, i32, C.POINTER(vp)]),
+        "slx_plan_symbolic_info": (i32, [vp, p32, pd]),
+        "slx_plan_symbolic_array": (i32, [vp, i32, vp, i64, p64]),
         "slx_plan_shape": (i32, [vp, p64, p64, p64, p64]),
         "slx_plan_stats": (i32, [vp, p64, p64, p64]),
         "slx_assemble": (i32, [vp, C.POINTER(NumericDesc)]),
@@ -144,7 +147,8 @@ def lib():
 
 
 EXPORTED = [
-    "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_shape", "slx_assemble",
+    "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_create_ex", "slx_plan_symbolic_info",
+    "slx_plan_symbolic_array", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_assemble_async", "slx_plan_dof_maps", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
     "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
@@ -354,12 +358,40 @@ class Problem:
             pass
 
 
+PLAN_HOST_SYMBOLIC = 1
+SYMBOLIC_ARRAYS = ("rdesc", "erows", "slots", "rhs_ptr", "rhs_erows", "free_dofs", "cons_dofs", "row_blocks", "classes",
+                   "scalars")
+
+
 class Plan:
     def __init__(self, h, owned=True):
         self.h = h
         self.owned = owned
         if owned:
             _live["plan"].add(self)
+
+    @classmethod
+    def create(cls, ctx, problem, flags=0):
+        """A plan of the problem's description (slx_plan_create_ex); the
+        problem must outlive it.  flags: PLAN_HOST_SYMBOLIC."""
+        p = C.c_void_p()
+        check(lib().slx_plan_create_ex(ctx.h if ctx else None, problem.desc_ptr(), flags, C.byref(p)))
+        return cls(p, owned=True)
+
+    def symbolic_info(self):
+        """(symbolic phase ran on the device, its wall time in seconds)."""
+        a, t = C.c_int32(), C.c_double()
+        check(lib().slx_plan_symbolic_info(self.h, C.byref(a), C.byref(t)))
+        return bool(a.value), t.value
+
+    def symbolic_array(self, name):
+        """Raw bytes (numpy uint8) of one device array of the symbolic phase."""
+        which = SYMBOLIC_ARRAYS.index(name)
+        n = C.c_int64()
+        check(lib().slx_plan_symbolic_array(self.h, which, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), np.uint8)
+        check(lib().slx_plan_symbolic_array(self.h, which, out.ctypes.data_as(C.c_void_p), out.size, C.byref(n)))
+        return out[:n.value]
 
     def shape(self):
         v = [C.c_int64() for _ in range(4)]

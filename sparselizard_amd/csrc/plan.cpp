@@ -2,6 +2,7 @@
 // device buffers of the numeric phase, plus the C ABI of include/slx.h.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <chrono>
 #include <cstdlib>
 #include <atomic>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include "kernels/mfma.h"
 #include "kernels/factor.h"
 #include "kernels/solve.h"
+#include "kernels/symbolic.h"
 #include "coefjit.h"
 #include <array>
 #include <mutex>
@@ -85,6 +87,12 @@ struct DBuf {
         p = nullptr;
         n = bytes;
         if (bytes) HIPCHECK(hipMalloc(&p, bytes));
+    }
+    // take over a hipMalloc'd allocation
+    void adopt(void* ptr, size_t bytes) {
+        if (p) (void)hipFree(p);
+        p = ptr;
+        n = bytes;
     }
     template <typename T>
     T* as() const { return (T*)p; }
@@ -170,6 +178,9 @@ struct slx_plan_s {
     int64_t max_elem_id = 0;       // parameter arrays must hold this many entries
     bool uses_fields = false;
     bool host_only = false;        // symbolic phase only (no device): pattern inspection
+    bool symbolic_on_device = false;  // the pattern and maps were built by the device phase
+    int64_t n_slots_w = 0, n_rhs_erows = 0;  // work-order slot entries, rhs segments
+    double symbolic_seconds = 0.0;
     bool signed_slots = false;     // folded orientations: signed segment entries
 
     hipGraphExec_t graph_exec = nullptr;  // captured numeric phase (slx_plan_graph_capture)
@@ -308,6 +319,63 @@ KeLayout compute_layout(const slx_plan_desc& d) {
         }
     }
     return L;
+}
+
+// Reduction launch classes over the work order (rows by decreasing length):
+// each class's longest row is at most twice its shortest row (LDS sized per
+// class).  `len` = A + D columns of the row at each work position.
+void reduction_classes(slx_plan_s& P, const std::vector<int32_t>& len) {
+    const int64_t nf = (int64_t)len.size();
+    P.red_classes.clear();
+    for (int64_t i = 0; i < nf;) {
+        const int top = std::max(1, len[i]);
+        int64_t j = i;
+        while (j < nf && 2 * len[j] > top) j++;
+        if (j == i) j = i + 1;
+        P.red_classes.push_back({i, j, top});
+        i = j;
+    }
+}
+
+// Device buffers every plan needs once its pattern and reduction maps are on
+// the device: values, the rhs with its Dirichlet entries, the reduction
+// descriptor, the solve's row blocks and the element-matrix buffer.
+void finish_plan(slx_plan_s& P, const slx_plan_desc& d, int slot_bytes, int maxrow) {
+    hipStream_t st = P.stream;
+    const int64_t nd = d.n_dofs, nf = P.nfree;
+    P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
+    P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
+    // rhs: constrained entries hold the Dirichlet values (formulation::b(),
+    // vec.cpp:43-61); the free entries are rewritten by every assembly
+    {
+        std::vector<double> b0(std::max<int64_t>(nd, 1), 0.0);
+        if (d.dirichlet && d.constrained)
+            for (int64_t i = 0; i < nd; i++)
+                if (d.constrained[i]) b0[i] = d.dirichlet[i];
+        upload(P.d_rhs, b0.data(), b0.size(), st);
+    }
+    P.rd.nrows = nf;
+    P.rd.rdesc = P.d_rdesc.as<uint64_t>();
+    P.rd.erows = P.d_erows.as<uint64_t>();
+    P.rd.slots = P.d_slots.p;
+    P.rd.slot_bytes = slot_bytes;
+    P.rd.slot_signed = P.signed_slots ? 1 : 0;
+    P.rd.n_cu = P.n_cu;
+    P.rd.max_row_len = maxrow;
+    P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();
+    P.rd.rhs_erows = P.d_rhs_erows.as<uint64_t>();
+    P.rd.free_dofs = P.d_free.as<int64_t>();
+    // row blocks of the solve's streaming product (kept with the pattern, so
+    // that no solve allocates beyond its workspace)
+    if (nf > 0) {
+        std::vector<int64_t> blk;
+        stream_row_blocks(P.Arow.data(), nf, blk);
+        P.n_ablk = (int64_t)blk.size() - 1;
+        upload(P.d_ablk, blk.data(), blk.size(), st);
+    }
+    // Ke buffer (FP64 capacity serves every precision)
+    P.ke_capacity_bytes = std::max<int64_t>(P.n_ke, 1) * sizeof(double);
+    P.d_ke.alloc(P.ke_capacity_bytes);
 }
 
 // Symbolic phase on the host: element-row lists in the reference
@@ -555,14 +623,10 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     std::vector<int32_t> order(nf);
     for (int64_t r = 0; r < nf; r++) order[r] = (int32_t)r;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) { return lenA[a] + lenD[a] > lenA[b] + lenD[b]; });
-    P.red_classes.clear();
-    for (int64_t i = 0; i < nf;) {
-        const int top = std::max(1, lenA[order[i]] + lenD[order[i]]);
-        int64_t j = i;
-        while (j < nf && 2 * (lenA[order[j]] + lenD[order[j]]) > top) j++;
-        if (j == i) j = i + 1;
-        P.red_classes.push_back({i, j, top});
-        i = j;
+    {
+        std::vector<int32_t> wlen(nf);
+        for (int64_t i = 0; i < nf; i++) wlen[i] = lenA[order[i]] + lenD[order[i]];
+        reduction_classes(P, wlen);
     }
     if (P.host_only) {
         P.h_erow_ptr = std::move(erow_ptr);
@@ -611,6 +675,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         SLX_CHECK(P.Arow[nf] < (int64_t(1) << 48) && P.Drow[nf] < (int64_t(1) << 48), "matrix exceeds the 48-bit index range");
         upload(P.d_rdesc, rdesc.data(), rdesc.size(), st);
         upload(P.d_erows, erows_w.data(), erows_w.size(), st);
+        P.n_slots_w = sw;
         // slots in the narrowest type that holds every row length (all ones = skip)
         // signed plans: the top bit of a slot is the entry's sign
         auto narrow = [&](auto tag) {
@@ -636,6 +701,7 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     int slot_bytes = slot_bytes_for(maxrow, signed_slots);
     upload(P.d_rhs_ptr, rhs_ptr.data(), rhs_ptr.size(), st);
     upload(P.d_rhs_erows, rhs_erows.data(), rhs_erows.size(), st);
+    P.n_rhs_erows = (int64_t)rhs_erows.size();
     upload(P.d_free, free_dofs.data(), free_dofs.size(), st);
     {
         std::vector<int64_t> cons_dofs;
@@ -644,39 +710,113 @@ void build_symbolic(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             if (d.constrained && d.constrained[i]) cons_dofs.push_back(i);
         upload(P.d_cons, cons_dofs.data(), cons_dofs.size(), st);
     }
-    P.d_Aval.alloc(std::max<int64_t>(P.nnzA, 1) * sizeof(double));
-    P.d_Dval.alloc(std::max<int64_t>(P.nnzD, 1) * sizeof(double));
-    // rhs: constrained entries hold the Dirichlet values (formulation::b(),
-    // vec.cpp:43-61); the free entries are rewritten by every assembly
-    {
-        std::vector<double> b0(std::max<int64_t>(nd, 1), 0.0);
-        if (d.dirichlet && d.constrained)
-            for (int64_t i = 0; i < nd; i++)
-                if (d.constrained[i]) b0[i] = d.dirichlet[i];
-        upload(P.d_rhs, b0.data(), b0.size(), st);
+    finish_plan(P, d, slot_bytes, maxrow);
+    // the column arrays live on the device from here on
+    std::vector<int32_t>().swap(P.Acol);
+    std::vector<int32_t>().swap(P.Dcol);
+}
+
+// The same phase on the device (kernels/symbolic.hip).  Returns false, having
+// changed nothing, when a row exceeds the device kernels' capacity (more than
+// 8192 entries or 1024 element rows): the host phase then runs.
+bool build_symbolic_device(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
+    SymInput in;
+    in.n_dofs = d.n_dofs;
+    in.constrained = d.constrained;
+    in.n_cu = P.n_cu;
+    for (size_t g = 0; g < L.groups.size(); g++) {
+        const SegGroup& G = L.groups[g];
+        const slx_bucket_desc& bd = d.buckets[G.bucket];
+        const slx_slice_desc& sd0 = bd.slices[G.slices[0]];
+        SymGroup sg{};
+        sg.tf_addr = sd0.tf_addr;
+        sg.tf_sign = sd0.tf_sign;
+        sg.base = G.base;
+        sg.ld = (int32_t)G.ld;
+        sg.len = (int32_t)G.len;
+        sg.n_tf = G.n_tf;
+        sg.n_elem = bd.n_elem;
+        sg.slice0 = (int32_t)in.slices.size();
+        sg.n_slices = G.rhs ? 0 : (int32_t)G.slices.size();
+        sg.rhs = G.rhs ? 1 : 0;
+        if (!G.rhs)
+            for (int s : G.slices) {
+                const slx_slice_desc& sd = bd.slices[s];
+                SymSlice ss{};
+                ss.dof_addr = sd.dof_addr;
+                ss.tf_sign = sd.tf_sign;
+                ss.dof_sign = sd.dof_sign;
+                ss.n_dof = sd.n_dof;
+                ss.col_off = (int32_t)(L.ke_base[G.bucket][s] - G.base);
+                ss.n_tf = sd.n_tf;
+                ss.n_elem = bd.n_elem;
+                in.slices.push_back(ss);
+                if (sd.tf_sign || sd.dof_sign) in.signed_slots = true;
+            }
+        in.groups.push_back(sg);
     }
-    P.rd.nrows = nf;
-    P.rd.rdesc = P.d_rdesc.as<uint64_t>();
-    P.rd.erows = P.d_erows.as<uint64_t>();
-    P.rd.slots = P.d_slots.p;
-    P.rd.slot_bytes = slot_bytes;
-    P.rd.slot_signed = signed_slots ? 1 : 0;
-    P.rd.n_cu = P.n_cu;
-    P.rd.max_row_len = maxrow;
-    P.rd.rhs_erow_ptr = P.d_rhs_ptr.as<int64_t>();
-    P.rd.rhs_erows = P.d_rhs_erows.as<uint64_t>();
-    P.rd.free_dofs = P.d_free.as<int64_t>();
-    // row blocks of the solve's streaming product (kept with the pattern, so
-    // that no solve allocates beyond its workspace)
-    if (nf > 0) {
-        std::vector<int64_t> blk;
-        stream_row_blocks(P.Arow.data(), nf, blk);
-        P.n_ablk = (int64_t)blk.size() - 1;
-        upload(P.d_ablk, blk.data(), blk.size(), st);
+    for (size_t g = 0; g < L.groups.size(); g++) (L.groups[g].rhs ? in.rhs : in.mat).push_back((int)g);
+    auto grank = [&](int g) { return L.groups[g].rank; };
+    std::stable_sort(in.mat.begin(), in.mat.end(), [&](int a, int b) { return grank(a) < grank(b); });
+    std::stable_sort(in.rhs.begin(), in.rhs.end(), [&](int a, int b) { return grank(a) < grank(b); });
+    SLX_CHECK(L.n_ke < (int64_t(1) << 48), "element matrices exceed the 48-bit offset range");
+
+    SymOutput o;
+    std::string err;
+    const hipError_t e = symbolic_device(in, o, err, P.stream);
+    if (e != hipSuccess) {
+        if (e == hipErrorInvalidValue && !err.empty()) fail(err);
+        throw DeviceError("device symbolic phase: " + (err.empty() ? std::string(hipGetErrorString(e)) : err));
     }
-    // Ke buffer (FP64 capacity serves every precision)
-    P.ke_capacity_bytes = std::max<int64_t>(P.n_ke, 1) * sizeof(double);
-    P.d_ke.alloc(P.ke_capacity_bytes);
+    if (o.fallback) return false;
+    P.nfree = o.nf;
+    P.ncons = o.nc;
+    P.n_ke = L.n_ke;
+    P.signed_slots = in.signed_slots;
+    P.n_erows = o.n_erows;
+    P.n_coo = o.n_coo;
+    P.max_erow_len = std::max(1, o.max_erow_len);
+    P.nnzA = o.nnzA;
+    P.nnzD = o.nnzD;
+    P.Arow = std::move(o.h_Arow);
+    P.Drow = std::move(o.h_Drow);
+    P.Acol.clear();
+    P.Dcol.clear();
+    for (int b = 0; b < d.n_buckets; b++)
+        for (int s = 0; s < d.buckets[b].n_slices; s++) {
+            P.h_ke_base.push_back(L.ke_base[b][s]);
+            P.h_ke_ld.push_back(L.ke_ld[b][s]);
+        }
+    reduction_classes(P, o.h_work_len);
+    P.d_Arow.adopt(o.Arow.p, o.Arow.bytes);
+    P.d_Drow.adopt(o.Drow.p, o.Drow.bytes);
+    P.d_Acol.adopt(o.Acol.p, o.Acol.bytes);
+    P.d_Dcol.adopt(o.Dcol.p, o.Dcol.bytes);
+    P.d_rdesc.adopt(o.rdesc.p, o.rdesc.bytes);
+    P.d_erows.adopt(o.erows_w.p, o.erows_w.bytes);
+    P.d_slots.adopt(o.slots_w.p, o.slots_w.bytes);
+    P.d_rhs_ptr.adopt(o.rhs_ptr.p, o.rhs_ptr.bytes);
+    P.d_rhs_erows.adopt(o.rhs_erows.p, o.rhs_erows.bytes);
+    P.d_free.adopt(o.free_dofs.p, o.free_dofs.bytes);
+    P.d_cons.adopt(o.cons_dofs.p, o.cons_dofs.bytes);
+    P.symbolic_on_device = true;
+    P.n_slots_w = o.n_slots_w;
+    P.n_rhs_erows = o.n_rhs_erows;
+    finish_plan(P, d, o.slot_bytes, o.maxrow);
+    return true;
+}
+
+// Address range check of the description (the device phase checks on the device).
+void check_addresses(const slx_plan_desc* desc) {
+    for (int b = 0; b < desc->n_buckets; b++) {
+        const slx_bucket_desc& bd = desc->buckets[b];
+        for (int s = 0; s < bd.n_slices; s++) {
+            const slx_slice_desc& sd = bd.slices[s];
+            size_t ntf = (size_t)sd.n_tf * bd.n_elem, ndf = sd.is_rhs ? 0 : (size_t)sd.n_dof * bd.n_elem;
+            for (size_t k = 0; k < ntf; k++) SLX_CHECK(sd.tf_addr[k] < desc->n_dofs, "tf address out of range");
+            for (size_t k = 0; k < ndf; k++) SLX_CHECK(sd.dof_addr[k] < desc->n_dofs, "dof address out of range");
+        }
+    }
 }
 
 // IEEE binary16 round-to-nearest-even conversion (host side of the split).
@@ -1462,8 +1602,13 @@ slx_status slx_destroy(slx_ctx ctx) {
 }
 
 slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out) {
+    return slx_plan_create_ex(ctx, desc, 0, out);
+}
+
+slx_status slx_plan_create_ex(slx_ctx ctx, const slx_plan_desc* desc, int32_t flags, slx_plan* out) {
     return guarded([&] {
         SLX_CHECK(desc && out, "null argument");
+        SLX_CHECK((flags & ~SLX_PLAN_HOST_SYMBOLIC) == 0, "unknown plan flags");
         auto P = std::make_unique<slx_plan_s>();
         P->host_only = (ctx == nullptr);
         if (!P->host_only) HIPCHECK(hipSetDevice(ctx->device));
@@ -1476,21 +1621,70 @@ slx_status slx_plan_create(slx_ctx ctx, const slx_plan_desc* desc, slx_plan* out
         P->has_matrix = desc->has_matrix != 0;
         P->has_rhs = desc->has_rhs != 0;
         SLX_CHECK(desc->n_dofs >= 0 && desc->n_nodes >= 0 && desc->n_buckets >= 0, "negative sizes in plan description");
-        for (int b = 0; b < desc->n_buckets; b++) {
-            const slx_bucket_desc& bd = desc->buckets[b];
-            for (int s = 0; s < bd.n_slices; s++) {
-                const slx_slice_desc& sd = bd.slices[s];
-                size_t ntf = (size_t)sd.n_tf * bd.n_elem, ndf = sd.is_rhs ? 0 : (size_t)sd.n_dof * bd.n_elem;
-                for (size_t k = 0; k < ntf; k++) SLX_CHECK(sd.tf_addr[k] < desc->n_dofs, "tf address out of range");
-                for (size_t k = 0; k < ndf; k++) SLX_CHECK(sd.dof_addr[k] < desc->n_dofs, "dof address out of range");
-            }
-        }
+        const auto t0 = std::chrono::steady_clock::now();
         const KeLayout L = compute_layout(*desc);
         if (!P->host_only) upload_buckets(*P, *desc, L);
-        build_symbolic(*P, *desc, L);
+        bool done = false;
+        if (!P->host_only && !(flags & SLX_PLAN_HOST_SYMBOLIC)) done = build_symbolic_device(*P, *desc, L);
+        if (!done) {
+            check_addresses(desc);
+            build_symbolic(*P, *desc, L);
+        }
+        P->symbolic_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         if (!P->host_only) HIPCHECK(hipStreamSynchronize(P->stream));
         if (ctx) ctx->refs.fetch_add(1);
         *out = P.release();
+    });
+}
+
+slx_status slx_plan_symbolic_info(slx_plan p, int32_t* on_device, double* seconds) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        if (on_device) *on_device = p->symbolic_on_device ? 1 : 0;
+        if (seconds) *seconds = p->symbolic_seconds;
+    });
+}
+
+slx_status slx_plan_symbolic_array(slx_plan p, int32_t which, void* out, int64_t cap_bytes, int64_t* n_bytes) {
+    return guarded([&] {
+        SLX_CHECK(p && n_bytes, "null argument");
+        SLX_CHECK(!p->host_only, "pattern-only plans keep their maps on the host (slx_plan_reduction_maps)");
+        const int64_t nf = p->nfree;
+        const void* src = nullptr;
+        int64_t nb = 0;
+        std::vector<int64_t> host;
+        switch (which) {
+            case 0: src = p->d_rdesc.p; nb = nf * 4 * 8; break;
+            case 1: src = p->d_erows.p; nb = p->n_erows * 8; break;
+            case 2: src = p->d_slots.p; nb = p->n_slots_w * p->rd.slot_bytes; break;
+            case 3: src = p->d_rhs_ptr.p; nb = (nf + 1) * 8; break;
+            case 4: src = p->d_rhs_erows.p; nb = p->n_rhs_erows * 8; break;
+            case 5: src = p->d_free.p; nb = nf * 8; break;
+            case 6: src = p->d_cons.p; nb = p->ncons * 8; break;
+            case 7: src = p->d_ablk.p; nb = nf > 0 ? (p->n_ablk + 1) * 8 : 0; break;
+            case 8:
+                for (const RedClass& c : p->red_classes) {
+                    host.push_back(c.begin);
+                    host.push_back(c.end);
+                    host.push_back(c.max_len);
+                }
+                nb = (int64_t)host.size() * 8;
+                break;
+            case 9:
+                host = {p->rd.slot_bytes, p->rd.max_row_len, p->max_erow_len, p->n_erows, p->n_coo};
+                nb = (int64_t)host.size() * 8;
+                break;
+            default: fail("unknown symbolic array");
+        }
+        *n_bytes = nb;
+        if (!out) return;
+        SLX_CHECK(cap_bytes >= nb, "output buffer too small");
+        if (which >= 8) {
+            std::memcpy(out, host.data(), nb);
+        } else if (nb > 0) {
+            HIPCHECK(hipStreamSynchronize(p->stream));
+            HIPCHECK(hipMemcpy(out, src, nb, hipMemcpyDeviceToHost));
+        }
     });
 }
 
@@ -1634,8 +1828,14 @@ slx_status slx_csr_copyout(slx_plan p, int64_t* Arow, int32_t* Acol, double* Ava
         SLX_CHECK(!p->host_only || (!Aval && !Dval && !rhs), "plan was created without a device context: only the pattern is available");
         if (Arow) std::copy(p->Arow.begin(), p->Arow.end(), Arow);
         if (Drow) std::copy(p->Drow.begin(), p->Drow.end(), Drow);
-        if (Acol) std::copy(p->Acol.begin(), p->Acol.end(), Acol);
-        if (Dcol) std::copy(p->Dcol.begin(), p->Dcol.end(), Dcol);
+        if (p->host_only) {
+            if (Acol) std::copy(p->Acol.begin(), p->Acol.end(), Acol);
+            if (Dcol) std::copy(p->Dcol.begin(), p->Dcol.end(), Dcol);
+        } else {
+            // device plans keep the columns on the device only
+            if (Acol && p->nnzA) HIPCHECK(hipMemcpy(Acol, p->d_Acol.p, p->nnzA * sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (Dcol && p->nnzD) HIPCHECK(hipMemcpy(Dcol, p->d_Dcol.p, p->nnzD * sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
         if (Aval && p->nnzA) HIPCHECK(hipMemcpy(Aval, p->d_Aval.p, p->nnzA * sizeof(double), hipMemcpyDeviceToHost));
         if (Dval && p->nnzD) HIPCHECK(hipMemcpy(Dval, p->d_Dval.p, p->nnzD * sizeof(double), hipMemcpyDeviceToHost));
         if (rhs && p->n_dofs) HIPCHECK(hipMemcpy(rhs, p->d_rhs.p, p->n_dofs * sizeof(double), hipMemcpyDeviceToHost));
