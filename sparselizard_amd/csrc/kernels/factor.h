@@ -37,12 +37,22 @@ struct FactorSlice {
     int64_t ke_base, ke_ld;
     const uint32_t* afrag_x[3];     // host side: the fragments of every arithmetic
     uint32_t arith_ok;              // bit a: arithmetic a fits (LDS, P chunk staging)
+    // FP64 (v_mfma_f64_16x16x4_f64): k-steps of 16 (qb64 Gauss points x na
+    // derivatives), P fragments [nks64][mt][4 steps][64] doubles (lane l of
+    // step j: P^T[mt*16 + l%16][4j + l/16]), w_q dof_b[j][q] as [ngp][ntp][4]
+    int32_t qb64, nks64, f64_ok;
+    const double* afrag64;
+    const double* tq64;
 };
 
 size_t factor_lds_bytes(const FactorSlice& f, int arith);
+size_t factor64_lds_bytes(const FactorSlice& f);
 bool factor_variant_fits(int mt, int nt, int* bm, int* bn);
 // Contraction of one slice in arithmetic `arith` (MfmaArith).
 hipError_t launch_factor(const DevBucket& b, const FactorSlice& f, int arith, const float* ctab, float* ke, int n_cu,
                          hipStream_t st);
+// The same contraction in FP64 on v_mfma_f64_16x16x4_f64 (FP64 coefficients
+// and element matrices).
+hipError_t launch_factor64(const DevBucket& b, const FactorSlice& f, const double* ctab, double* ke, int n_cu, hipStream_t st);
 
 }  // namespace slx

@@ -358,6 +358,138 @@ static hipError_t launch_fk(const DevBucket& b, const FactorSlice& f, const floa
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ FP64
+//
+// One workgroup (8 waves, a 2 x 4 grid over the 16x16 output tiles of a pass)
+// per element at a time.  Per k-step of 16 (Gauss point, tf derivative) pairs
+// every wave builds its share of the next Q_e chunk in LDS (FP64, the element's
+// coefficient matrices times w_q dof_b[j][q]) while the MFMAs of the current
+// chunk run; the P fragments come straight from global memory (shared by the
+// bucket's elements, L2-resident).  No scaling: FP64 needs none.
+typedef double double4v __attribute__((ext_vector_type(4)));
+
+size_t factor64_lds_bytes(const FactorSlice& f) {
+    return (size_t)f.ngp * 16 * 8 + (size_t)2 * 16 * (4 * f.bn * 16) * 8;
+}
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(512) factor64_kernel(int n_elem, FactorSlice f, const double* __restrict__ ctab, double* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    constexpr int NTB = 4 * BN;   // column tiles per pass
+    constexpr int PC = 16 * NTB;  // columns per pass
+    const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb64, ntp = f.ntp, nks = f.nks64;
+    double* Ms = reinterpret_cast<double*>(smem_raw);  // [ngp][4 tf derivative][4 dof derivative]
+    double* Bq = Ms + (size_t)ngp * 16;                // [2][16][PC]
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = w >> 2, wn = w & 3;
+    const int32_t* t_ai = f.tinfo;
+    const int32_t* t_b = f.tinfo + f.n_terms;
+    const int32_t* t_slot = f.tinfo + 2 * f.n_terms;
+    const int64_t stride = (int64_t)nq * n_elem;
+    const double4v* tq4 = reinterpret_cast<const double4v*>(f.tq64);
+    const int npass = (NT + NTB - 1) / NTB;
+    // Q_e chunk of k-step ks for the pass's columns: entry (kl, c) = k slot kl
+    // (Gauss point ks*qb + kl/na, tf derivative kl%na), column cb*16 + c
+    auto build = [&](int ks, int buf, int cb) {
+        double* dst = Bq + (size_t)buf * 16 * PC;
+        for (int x = tid; x < 16 * PC; x += 512) {
+            const int kl = x / PC, c = x - kl * PC;
+            const int qi = kl / na, ai = kl - qi * na, q = ks * qb + qi, j = cb * 16 + c;
+            double v = 0.0;
+            if (qi < qb && q < ngp && j < ntp) {
+                const double4v t = tq4[(size_t)q * ntp + j];
+                const double* m = Ms + (q * 4 + ai) * 4;
+                v = m[0] * t[0] + m[1] * t[1] + m[2] * t[2] + m[3] * t[3];
+            }
+            dst[x] = v;
+        }
+    };
+    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
+        __syncthreads();  // the previous element is done with Ms and Bq
+        for (int q = tid; q < ngp; q += 512) {
+            double* mq = Ms + q * 16;  // owned by this thread
+#pragma unroll
+            for (int x = 0; x < 16; x++) mq[x] = 0.0;
+            const int qq = nq == 1 ? 0 : q;
+#pragma unroll
+            for (int t = 0; t < SLX_FK_MAXT; t++)
+                if (t < f.n_terms) mq[t_ai[t] * 4 + t_b[t]] += ctab[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq];
+        }
+        __syncthreads();
+        for (int pass = 0; pass < npass; pass++) {
+            const int cb = pass * NTB;
+            double4v acc[BM][BN];
+#pragma unroll
+            for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+                for (int nj = 0; nj < BN; nj++) acc[mi][nj] = double4v{0.0, 0.0, 0.0, 0.0};
+            build(0, 0, cb);
+            __syncthreads();
+            for (int ks = 0; ks < nks; ks++) {
+                if (ks + 1 < nks) build(ks + 1, (ks + 1) & 1, cb);
+                const double* Bc = Bq + (size_t)(ks & 1) * 16 * PC;
+                double bq[BN][4];
+#pragma unroll
+                for (int nj = 0; nj < BN; nj++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) bq[nj][j] = Bc[(4 * j + (lane >> 4)) * PC + (wn + 4 * nj) * 16 + (lane & 15)];
+#pragma unroll
+                for (int mi = 0; mi < BM; mi++) {
+                    const int mt = wm + 2 * mi;
+                    if (mt < MT) {
+                        const double* af = f.afrag64 + ((size_t)(ks * MT + mt) * 4) * 64 + lane;
+                        double a[4];
+#pragma unroll
+                        for (int j = 0; j < 4; j++) a[j] = af[64 * j];
+#pragma unroll
+                        for (int nj = 0; nj < BN; nj++)
+                            if (cb + wn + 4 * nj < NT)
+#pragma unroll
+                                for (int j = 0; j < 4; j++) acc[mi][nj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[j], bq[nj][j], acc[mi][nj], 0, 0, 0);
+                    }
+                }
+                __syncthreads();  // chunk ks consumed, chunk ks + 1 built
+            }
+            // lane l holds rows mt*16 + l/16 + 4v, column jt*16 + l%16
+            double* eb = ke + f.ke_base + (int64_t)e * f.n_tf * f.ke_ld;
+            const int ld = (int)f.ke_ld;
+#pragma unroll
+            for (int mi = 0; mi < BM; mi++) {
+                const int mt = wm + 2 * mi;
+#pragma unroll
+                for (int nj = 0; nj < BN; nj++) {
+                    const int jt = cb + wn + 4 * nj;
+                    const int j = jt * 16 + (lane & 15);
+                    if (mt >= MT || jt >= NT || j >= f.n_dof) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; v++) {
+                        const int i = mt * 16 + (lane >> 4) + 4 * v;
+                        if (i < f.n_tf) __builtin_nontemporal_store(acc[mi][nj][v], eb + i * ld + j);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int BM, int BN>
+static hipError_t launch_f64(const DevBucket& b, const FactorSlice& f, const double* ctab, double* ke, int n_cu, hipStream_t st) {
+    const size_t lds = factor64_lds_bytes(f);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const int64_t grid = std::min<int64_t>(b.n_elem, (int64_t)n_cu);
+    hipLaunchKernelGGL((factor64_kernel<BM, BN>), dim3((unsigned)grid), dim3(512), lds, st, b.n_elem, f, ctab, ke);
+    return hipGetLastError();
+}
+
+hipError_t launch_factor64(const DevBucket& b, const FactorSlice& f, const double* ctab, double* ke, int n_cu, hipStream_t st) {
+    if (b.n_elem == 0) return hipSuccess;
+    if (!f.f64_ok || !f.afrag64 || !f.tq64) return hipErrorInvalidValue;
+    if (f.bm == 2 && f.bn == 1) return launch_f64<2, 1>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 4 && f.bn == 2) return launch_f64<4, 2>(b, f, ctab, ke, n_cu, st);
+    if (f.bm == 7 && f.bn == 2) return launch_f64<7, 2>(b, f, ctab, ke, n_cu, st);
+    return hipErrorInvalidValue;
+}
+
 template <int AR>
 static hipError_t launch_fk_ar(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
     if (f.bm == 2 && f.bn == 1) return launch_fk<AR, 2, 1>(b, f, ctab, ke, n_cu, st);

@@ -201,6 +201,7 @@ struct slx_plan_s {
     std::vector<std::vector<std::pair<int, FactorSlice>>> factor;  // per bucket: (slice, factorized MFMA data)
     std::vector<uint32_t> factor_mask; // matrix slices handled by the factorized MFMA kernel
     int last_qfast = 0;                // coefficient table layout of the last bucket assembled
+    bool factor64 = true;              // FP64 mode: factorized slices on v_mfma_f64 (SLX_TUNE_FP64_FACTOR=0: VALU)
     int n_cu = 256;
 
     DBuf* newbuf() {
@@ -1260,6 +1261,28 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
                     tqmax = std::max(tqmax, std::fabs(x));
                 }
         f.tqmax = (float)(tqmax * 1.0001);
+        // FP64 variant: k-steps of 16, unscaled P, FP64 w_q dof_b[j][q]
+        f.qb64 = std::min(16 / f.na, SLX_FK_RU);
+        f.nks64 = (ngp + f.qb64 - 1) / f.qb64;
+        f.f64_ok = f.qb64 >= 1 && factor64_lds_bytes(f) <= 160 * 1024;
+        if (f.f64_ok) {
+            std::vector<double> a64((size_t)f.nks64 * f.mt * 4 * 64, 0.0), t64((size_t)ngp * 4 * f.ntp, 0.0);
+            for (int ks = 0; ks < f.nks64; ks++)
+                for (int mt = 0; mt < f.mt; mt++)
+                    for (int j = 0; j < 4; j++)
+                        for (int l = 0; l < 64; l++) {
+                            const int i = mt * 16 + l % 16, kl = 4 * j + l / 16;
+                            const int qi = kl / f.na, ai = kl % f.na, q = ks * f.qb64 + qi;
+                            if (i < sd.n_tf && qi < f.qb64 && q < ngp)
+                                a64[(((size_t)ks * f.mt + mt) * 4 + j) * 64 + l] = tft[((size_t)alist[ai] * nff + i) * ngp + q];
+                        }
+            for (int q = 0; q < ngp; q++)
+                for (int dd = 0; dd < 4; dd++)
+                    for (int j = 0; j < sd.n_dof; j++)
+                        t64[((size_t)q * f.ntp + j) * 4 + dd] = bd.gp_weights[q] * dft[((size_t)dd * nfd + j) * ngp + q];
+            f.afrag64 = P.dev(a64.data(), a64.size());
+            f.tq64 = P.dev(t64.data(), t64.size());
+        }
         f.tinfo = P.dev(tinfo.data(), tinfo.size());
         f.afrag_x[AR_F16] = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
         f.afrag_x[AR_BF16] = reinterpret_cast<const uint32_t*>(P.dev(fragb.data(), fragb.size()));
@@ -1513,10 +1536,14 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
         // the factorized kernel reads the coefficients of one element contiguously
-        uint32_t fmask = 0;  // slices on the factorized kernel in this arithmetic
-        if (arith >= 0)
+        uint32_t fmask = 0;  // slices on the factorized kernel in this arithmetic (or in FP64)
+        if (arith >= 0) {
             for (auto& fs : P.factor[bi])
                 if ((fs.second.arith_ok >> arith) & 1u) fmask |= 1u << fs.first;
+        } else if (ke64 && P.factor64) {
+            for (auto& fs : P.factor[bi])
+                if (fs.second.f64_ok) fmask |= 1u << fs.first;
+        }
         const int qfast = fmask != 0 ? 1 : 0;
         P.last_qfast = qfast;
         if (tm) tm->begin(T_COEF);
@@ -1533,8 +1560,10 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         if (fmask != 0) {
             if (tm) tm->begin(T_MFMA);
             for (auto& fs : P.factor[bi])
-                if ((fmask >> fs.first) & 1u)
-                    HIPCHECK(launch_factor(b, fs.second, arith, (const float*)ctab, P.d_ke.as<float>(), P.n_cu, bst));
+                if ((fmask >> fs.first) & 1u) {
+                    if (ke64) HIPCHECK(launch_factor64(b, fs.second, (const double*)ctab, P.d_ke.as<double>(), P.n_cu, bst));
+                    else HIPCHECK(launch_factor(b, fs.second, arith, (const float*)ctab, P.d_ke.as<float>(), P.n_cu, bst));
+                }
             if (tm) tm->end();
             done |= fmask;
         }
@@ -1611,6 +1640,7 @@ slx_status slx_plan_create_ex(slx_ctx ctx, const slx_plan_desc* desc, int32_t fl
         SLX_CHECK((flags & ~SLX_PLAN_HOST_SYMBOLIC) == 0, "unknown plan flags");
         auto P = std::make_unique<slx_plan_s>();
         P->host_only = (ctx == nullptr);
+        if (const char* t = std::getenv("SLX_TUNE_FP64_FACTOR")) P->factor64 = std::atoi(t) != 0;
         if (!P->host_only) HIPCHECK(hipSetDevice(ctx->device));
         P->ctx = ctx;
         P->stream = ctx ? ctx->stream : nullptr;
@@ -2037,6 +2067,9 @@ slx_status slx_plan_kernel_counts(slx_plan p, int32_t precision, int32_t* n_batc
             for (auto& m : p->mfma) nb += (m.M > 0 && ((m.arith_ok >> arith) & 1u)) ? 1 : 0;
             for (auto& v : p->factor)
                 for (auto& fs : v) nf += ((fs.second.arith_ok >> arith) & 1u) ? 1 : 0;
+        } else if (precision == SLX_PREC_FP64 && p->factor64) {
+            for (auto& v : p->factor)
+                for (auto& fs : v) nf += fs.second.f64_ok ? 1 : 0;
         }
         if (n_batched) *n_batched = nb;
         if (n_factorized) *n_factorized = nf;

@@ -131,3 +131,47 @@ def test_mfma_modes_deterministic(ctx, precision):
     a = prob.generate(precision).csr(info["ndofs"])
     b = prob.generate(precision).csr(info["ndofs"])
     assert np.array_equal(a.Aval, b.Aval) and np.array_equal(a.Dval, b.Dval) and np.array_equal(a.rhs, b.rhs)
+
+
+@pytest.mark.parametrize("kind,spec", [
+    ("hppoisson3d", "n=3"),                   # orders 1..5: factorized FP64 for p >= 2
+    ("hppoisson3d", "n=2;pmin=5;pmax=5"),     # p = 5 only, folded orientations
+    ("prism3d", "n=2;order=5"),               # Gauss-point-varying coefficients
+])
+def test_fp64_factorized_matches_oracle(ctx, kind, spec):
+    # the factorized contraction on v_mfma_f64_16x16x4_f64: FP64 gate
+    prob, plan, got, errs = check_case(ctx, kind, spec, "fp64", 1e-12)
+    assert plan.kernel_counts("fp64")["factorized"] >= 1
+
+
+def test_fp64_factorized_element_matrices_and_valu(ctx, monkeypatch):
+    """FP64 factorized element matrices slice by slice against the oracle's
+    (1e-12), bitwise repeatable, and equal to the FP64 VALU kernel's to
+    rounding (a different summation order)."""
+    print("case ke fp64 factorized", flush=True)
+    prob = sa.Problem(ctx, "hppoisson3d", "n=2")
+    plan = prob.generate("fp64")
+    assert plan.kernel_counts("fp64")["factorized"] >= 1
+    ke = plan.element_matrices()
+    kb, kl = plan.ke_layout()
+    o, ke_ref = oracle_ke_layout(prob, kb, kl, len(ke))
+    d = prob.desc()
+    idx_all, k = [], 0
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            ndf = 1 if sd.is_rhs else sd.n_dof
+            e, i, j = np.meshgrid(np.arange(bd.n_elem), np.arange(sd.n_tf), np.arange(ndf), indexing="ij")
+            idx = (kb[k] + (e * sd.n_tf + i) * kl[k] + j).ravel()
+            assert relerr(ke[idx], ke_ref[idx]) <= 1e-12
+            idx_all.append(idx)
+            k += 1
+    idx = np.concatenate(idx_all)
+    again = prob.generate("fp64").element_matrices()
+    assert np.array_equal(ke[idx], again[idx])
+    monkeypatch.setenv("SLX_TUNE_FP64_FACTOR", "0")
+    prob2 = sa.Problem(ctx, "hppoisson3d", "n=2")
+    plan2 = prob2.generate("fp64")
+    assert plan2.kernel_counts("fp64")["factorized"] == 0
+    assert relerr(ke[idx], plan2.element_matrices()[idx]) <= 1e-13
