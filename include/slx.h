@@ -244,6 +244,11 @@ slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_b
  * MFMA kernel, and slices on the factorized MFMA kernel (0 and 0 for the
  * VALU modes). */
 slx_status slx_plan_kernel_counts(slx_plan plan, int32_t precision, int32_t* n_batched, int32_t* n_factorized);
+/* The fused launch of a precision mode: factorized slices of every bucket in
+ * one persistent launch (contribution.cpp:61-133 loops over the element
+ * groups one after the other; here their elements share one work queue,
+ * most expensive first).  n_slices = 0 when the mode runs per-slice launches. */
+slx_status slx_plan_fused_info(slx_plan plan, int32_t precision, int32_t* n_slices, int64_t* n_items);
 /* Solve of the last assembled system on the device, replacing
  * sl::solve(A, b) (src/expression/operation/sl.cpp:1565-1627): the
  * Dirichlet elimination b_A - D x_D of mat::eliminate

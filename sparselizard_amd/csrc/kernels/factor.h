@@ -55,4 +55,21 @@ hipError_t launch_factor(const DevBucket& b, const FactorSlice& f, int arith, co
 // and element matrices).
 hipError_t launch_factor64(const DevBucket& b, const FactorSlice& f, const double* ctab, double* ke, int n_cu, hipStream_t st);
 
+// One slice of a fused launch: every (slice, element) of the launch is a work
+// item; items are numbered slice by slice (slices in decreasing per-element
+// cost) and taken by the workgroups from a shared counter.
+struct FusedSlice {
+    FactorSlice f;         // arith / afrag set for the launch's arithmetic
+    const void* ctab;      // the bucket's coefficient table (FP32, or FP64 for the FP64 launch)
+    int64_t first;         // item number of the slice's first element
+    int32_t n_elem;
+    int32_t variant;       // 0: (2, 1), 1: (4, 2), 2: (7, 2) tiles per wave
+};
+int factor_variant_index(int bm, int bn);
+// One persistent launch over every slice of `fs` (device array of n_fs
+// entries, `total` items): arithmetic `arith` (MfmaArith) or 3 for FP64.
+// `counter` (device, 8 bytes) is cleared by the launch's first kernel.
+hipError_t launch_factor_fused(const FusedSlice* fs, int n_fs, int64_t total, unsigned long long* counter, int arith, size_t lds,
+                               void* ke, int n_cu, hipStream_t st);
+
 }  // namespace slx

@@ -53,10 +53,10 @@ bool factor_variant_fits(int mt, int nt, int* bm, int* bn) {
 // [NTB][NPL][64] words (F16 / BF16: lane l of column tile t holds the eight
 // k = 8 (l / 16) + 0..7 of column l % 16, one plane per piece; F32: the tile
 // as floats Q[k][column], [32][16]).
+// One element of a slice (the whole workgroup; every thread must call it).
 template <int AR, int BM, int BN>
-__global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
-    factor_contract_kernel(int n_elem, FactorSlice f, const float* __restrict__ ctab, float* __restrict__ ke) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+__device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem, int e, const float* __restrict__ ctab,
+                                               float* __restrict__ ke, unsigned char* smem_raw) {
     constexpr int NPL = AR == AR_BF16 ? 3 : 2;                  // 16-byte planes per lane
     const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb, ntp = f.ntp;
     constexpr int NTB = 4 * BN;                                 // column tiles per pass
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
     const int achunk = MT * NPL * 64;
     const int npass = (NT + NTB - 1) / NTB;
 
-    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
+    {
         if (tid == 0) *smax = 0u;
         __syncthreads();  // the previous element is done with Ms, As, Bs
         // the element's coefficient matrices per Gauss point:
@@ -348,6 +348,13 @@ __global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
 }
 
 template <int AR, int BM, int BN>
+__global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
+    factor_contract_kernel(int n_elem, FactorSlice f, const float* __restrict__ ctab, float* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) factor_element<AR, BM, BN>(f, n_elem, e, ctab, ke, smem_raw);
+}
+
+template <int AR, int BM, int BN>
 static hipError_t launch_fk(const DevBucket& b, const FactorSlice& f, const float* ctab, float* ke, int n_cu, hipStream_t st) {
     const size_t lds = factor_lds_bytes(f, AR);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -373,8 +380,8 @@ size_t factor64_lds_bytes(const FactorSlice& f) {
 }
 
 template <int BM, int BN>
-__global__ void __launch_bounds__(512) factor64_kernel(int n_elem, FactorSlice f, const double* __restrict__ ctab, double* __restrict__ ke) {
-    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+__device__ __forceinline__ void factor64_element(const FactorSlice& f, int n_elem, int e, const double* __restrict__ ctab,
+                                                 double* __restrict__ ke, unsigned char* smem_raw) {
     constexpr int NTB = 4 * BN;   // column tiles per pass
     constexpr int PC = 16 * NTB;  // columns per pass
     const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb64, ntp = f.ntp, nks = f.nks64;
@@ -404,7 +411,7 @@ __global__ void __launch_bounds__(512) factor64_kernel(int n_elem, FactorSlice f
             dst[x] = v;
         }
     };
-    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) {
+    {
         __syncthreads();  // the previous element is done with Ms and Bq
         for (int q = tid; q < ngp; q += 512) {
             double* mq = Ms + q * 16;  // owned by this thread
@@ -473,6 +480,12 @@ __global__ void __launch_bounds__(512) factor64_kernel(int n_elem, FactorSlice f
 }
 
 template <int BM, int BN>
+__global__ void __launch_bounds__(512) factor64_kernel(int n_elem, FactorSlice f, const double* __restrict__ ctab, double* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    for (int e = blockIdx.x; e < n_elem; e += gridDim.x) factor64_element<BM, BN>(f, n_elem, e, ctab, ke, smem_raw);
+}
+
+template <int BM, int BN>
 static hipError_t launch_f64(const DevBucket& b, const FactorSlice& f, const double* ctab, double* ke, int n_cu, hipStream_t st) {
     const size_t lds = factor64_lds_bytes(f);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
@@ -488,6 +501,98 @@ hipError_t launch_factor64(const DevBucket& b, const FactorSlice& f, const doubl
     if (f.bm == 4 && f.bn == 2) return launch_f64<4, 2>(b, f, ctab, ke, n_cu, st);
     if (f.bm == 7 && f.bn == 2) return launch_f64<7, 2>(b, f, ctab, ke, n_cu, st);
     return hipErrorInvalidValue;
+}
+
+// ------------------------------------------------------------------ fused
+//
+// One persistent launch over the factorized slices of every bucket: each
+// workgroup takes the next (slice, element) item from a shared counter, so the
+// expensive elements (highest order, numbered first) start first and the
+// cheap ones fill the tail.  Every item is computed by the same per-element
+// code as the per-slice launches, so the element matrices are identical.
+int factor_variant_index(int bm, int bn) {
+    for (int v = 0; v < 3; v++)
+        if (FK_BM[v] == bm && FK_BN[v] == bn) return v;
+    return -1;
+}
+
+__global__ void factor_fused_clear(unsigned long long* counter) { *counter = 0ull; }
+
+// the item's slice: the last slice whose first item is <= it
+__device__ __forceinline__ int fused_slice_of(const FusedSlice* fs, int n_fs, int64_t it) {
+    int lo = 0, hi = n_fs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (fs[mid].first <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int AR>
+__global__ void __launch_bounds__(64 * (SLX_FK_WAVES + SLX_FK_BWAVES))
+    factor_fused_kernel(const FusedSlice* __restrict__ fs, int n_fs, int64_t total, unsigned long long* __restrict__ counter,
+                        float* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    __shared__ int64_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t it = s_item;
+        __syncthreads();  // every thread has the item before thread 0 takes the next
+        if (it >= total) break;
+        const int s = fused_slice_of(fs, n_fs, it);
+        const FusedSlice& F = fs[s];
+        const int e = (int)(it - F.first);
+        const float* ctab = static_cast<const float*>(F.ctab);
+        if (F.variant == 0) factor_element<AR, 2, 1>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+        else if (F.variant == 1) factor_element<AR, 4, 2>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+        else factor_element<AR, 7, 2>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+    }
+}
+
+__global__ void __launch_bounds__(512)
+    factor64_fused_kernel(const FusedSlice* __restrict__ fs, int n_fs, int64_t total, unsigned long long* __restrict__ counter,
+                          double* __restrict__ ke) {
+    extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+    __shared__ int64_t s_item;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = (int64_t)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const int64_t it = s_item;
+        __syncthreads();
+        if (it >= total) break;
+        const int s = fused_slice_of(fs, n_fs, it);
+        const FusedSlice& F = fs[s];
+        const int e = (int)(it - F.first);
+        const double* ctab = static_cast<const double*>(F.ctab);
+        if (F.variant == 0) factor64_element<2, 1>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+        else if (F.variant == 1) factor64_element<4, 2>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+        else factor64_element<7, 2>(F.f, F.n_elem, e, ctab, ke, smem_raw);
+    }
+}
+
+hipError_t launch_factor_fused(const FusedSlice* fs, int n_fs, int64_t total, unsigned long long* counter, int arith, size_t lds,
+                               void* ke, int n_cu, hipStream_t st) {
+    if (total <= 0 || n_fs <= 0) return hipSuccess;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(factor_fused_clear, dim3(1), dim3(1), 0, st, counter);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / std::max<size_t>(lds, 1)));
+    const unsigned grid = (unsigned)std::min<int64_t>(total, (int64_t)n_cu * per_cu);
+    if (arith == 3)
+        hipLaunchKernelGGL(factor64_fused_kernel, dim3(grid), dim3(512), lds, st, fs, n_fs, total, counter, (double*)ke);
+    else if (arith == AR_BF16)
+        hipLaunchKernelGGL(factor_fused_kernel<AR_BF16>, dim3(grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, fs, n_fs, total, counter,
+                           (float*)ke);
+    else if (arith == AR_F32)
+        hipLaunchKernelGGL(factor_fused_kernel<AR_F32>, dim3(grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, fs, n_fs, total, counter,
+                           (float*)ke);
+    else if (arith == AR_F16)
+        hipLaunchKernelGGL(factor_fused_kernel<AR_F16>, dim3(grid), dim3(64 * (SLX_FK_WAVES + SLX_FK_BWAVES)), lds, st, fs, n_fs, total, counter,
+                           (float*)ke);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template <int AR>

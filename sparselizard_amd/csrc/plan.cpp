@@ -202,6 +202,20 @@ struct slx_plan_s {
     std::vector<uint32_t> factor_mask; // matrix slices handled by the factorized MFMA kernel
     int last_qfast = 0;                // coefficient table layout of the last bucket assembled
     bool factor64 = true;              // FP64 mode: factorized slices on v_mfma_f64 (SLX_TUNE_FP64_FACTOR=0: VALU)
+    // fused launch: the factorized slices of every bucket in one persistent
+    // launch (per arithmetic: F16, BF16, F32, and 3 = FP64); each fused bucket
+    // keeps its coefficients in its own region of d_ctab_fused
+    bool fused = false;
+    uint32_t fused_arith = 0;              // bit a: arithmetic a (3 = FP64) uses the fused launch
+    std::vector<uint8_t> fuse_bucket;      // per bucket: its factorized slices run in the fused launch
+    std::vector<size_t> fuse_off;          // per bucket: byte offset of its coefficient region
+    DBuf d_ctab_fused, d_fused_counter;
+    DBuf d_fused[4];
+    int n_fused[4] = {0, 0, 0, 0};
+    int64_t fused_total[4] = {0, 0, 0, 0};
+    size_t fused_lds[4] = {0, 0, 0, 0};
+    bool last_fused = false;
+    const void* last_ctab = nullptr;   // coefficient table of the last bucket assembled
     int n_cu = 256;
 
     DBuf* newbuf() {
@@ -1299,6 +1313,81 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
 
 void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d);
 
+// The fused launch (kernels/factor.hip, launch_factor_fused): when two or more
+// buckets have factorized slices, their coefficient tables get regions of
+// their own and the slice tables of the launch are built for every
+// arithmetic, slices in decreasing per-element cost.  By default the FP16 and
+// BF16 splits use it (measured faster than per-slice launches at 48^3 hp
+// Poisson: 13.1 vs 13.7 ms and 15.6 vs 17.1 ms); FP32 MFMA and FP64 keep
+// per-slice launches (25.0 vs 24.7 ms, 50.9 vs 47.7 ms: the fused kernel
+// takes the register budget of the largest tile variant for every slice).
+// SLX_TUNE_FUSED=0 turns it off, 1 uses it in every arithmetic (also for a
+// single bucket).
+void build_fused(slx_plan_s& P, const slx_plan_desc& d) {
+    int mode = -1;
+    if (const char* t = std::getenv("SLX_TUNE_FUSED")) mode = std::atoi(t);
+    P.fused_arith = mode == 1 ? 0xFu : ((1u << AR_F16) | (1u << AR_BF16));
+    P.fuse_bucket.assign(d.n_buckets, 0);
+    P.fuse_off.assign(d.n_buckets, 0);
+    int nb = 0;
+    for (int b = 0; b < d.n_buckets; b++)
+        if (!P.factor[b].empty() && !P.rhs_only[b]) nb++;
+    P.fused = mode != 0 && (nb >= 2 || (mode == 1 && nb >= 1));
+    if (!P.fused) return;
+    size_t bytes = 0;
+    for (int b = 0; b < d.n_buckets; b++) {
+        if (P.factor[b].empty() || P.rhs_only[b]) continue;
+        const slx_bucket_desc& bd = d.buckets[b];
+        P.fuse_bucket[b] = 1;
+        P.fuse_off[b] = bytes;
+        const size_t words = (size_t)std::max(bd.n_coefs, 1) * (bd.q_invariant ? 1 : bd.n_gp) * std::max(bd.n_elem, 1);
+        bytes += (words * sizeof(double) + 255) / 256 * 256;
+    }
+    P.d_ctab_fused.alloc(std::max<size_t>(bytes, 256));
+    P.d_fused_counter.alloc(sizeof(unsigned long long));
+    for (int a = 0; a < 4; a++) {
+        struct Entry {
+            FusedSlice fs;
+            double cost;
+        };
+        std::vector<Entry> v;
+        size_t lds = 0;
+        for (int b = 0; b < d.n_buckets; b++) {
+            if (!P.fuse_bucket[b]) continue;
+            for (auto& sf : P.factor[b]) {
+                const FactorSlice& f0 = sf.second;
+                if (a < 3 ? !((f0.arith_ok >> a) & 1u) : !f0.f64_ok) continue;
+                Entry en;
+                std::memset(&en, 0, sizeof(en));
+                en.fs.f = f0;
+                if (a < 3) {
+                    en.fs.f.arith = a;
+                    en.fs.f.afrag = f0.afrag_x[a];
+                }
+                en.fs.ctab = (const char*)P.d_ctab_fused.p + P.fuse_off[b];
+                en.fs.n_elem = d.buckets[b].n_elem;
+                en.fs.variant = factor_variant_index(f0.bm, f0.bn);
+                SLX_CHECK(en.fs.variant >= 0, "internal: factorized slice without a kernel variant");
+                en.cost = (double)f0.mt * f0.nt * (a < 3 ? f0.nks : f0.nks64);
+                lds = std::max(lds, a < 3 ? factor_lds_bytes(f0, a) : factor64_lds_bytes(f0));
+                v.push_back(en);
+            }
+        }
+        std::stable_sort(v.begin(), v.end(), [](const Entry& x, const Entry& y) { return x.cost > y.cost; });
+        std::vector<FusedSlice> tab;
+        int64_t first = 0;
+        for (Entry& en : v) {
+            en.fs.first = first;
+            first += en.fs.n_elem;
+            tab.push_back(en.fs);
+        }
+        P.n_fused[a] = (int)tab.size();
+        P.fused_total[a] = first;
+        P.fused_lds[a] = lds;
+        if (!tab.empty()) upload(P.d_fused[a], tab.data(), tab.size(), P.stream);
+    }
+}
+
 void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.dbuckets.resize(d.n_buckets);
     P.mfma.assign(d.n_buckets, MfmaBucket());
@@ -1424,6 +1513,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));
     P.d_ctab_side.alloc(ctab_side_words * sizeof(double));
+    build_fused(P, d);
     compile_coef_programs(P, d);
     P.side_ok = d.has_matrix && d.has_rhs && any_rhs_only && !rhs_in_mixed;
     if (P.side_ok) {
@@ -1529,35 +1619,67 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         HIPCHECK(hipEventRecord(P.ev_fork, st));
         HIPCHECK(hipStreamWaitEvent(P.side, P.ev_fork, 0));
     }
+    // fused launch: coefficients of every fused bucket into its region, then one
+    // persistent launch over all their factorized slices
+    // slices of bucket bi on the factorized kernel in this arithmetic (or in FP64)
+    auto factor_mask = [&](size_t bi) {
+        uint32_t m = 0;
+        if (arith >= 0) {
+            for (auto& fs : P.factor[bi])
+                if ((fs.second.arith_ok >> arith) & 1u) m |= 1u << fs.first;
+        } else if (ke64 && P.factor64) {
+            for (auto& fs : P.factor[bi])
+                if (fs.second.f64_ok) m |= 1u << fs.first;
+        }
+        return m;
+    };
+    const int fa = arith >= 0 ? arith : ((ke64 && P.factor64) ? 3 : -1);
+    const bool fused = P.fused && fa >= 0 && ((P.fused_arith >> fa) & 1u) && P.n_fused[fa] > 0;
+    P.last_fused = fused;
+    if (fused) {
+        for (size_t bi = 0; bi < P.dbuckets.size(); bi++) {
+            if (!P.fuse_bucket[bi] || factor_mask(bi) == 0) continue;
+            const DevBucket& b = P.dbuckets[bi];
+            void* ctab = (char*)P.d_ctab_fused.p + P.fuse_off[bi];
+            if (tm) tm->begin(T_COEF);
+            const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
+            if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, 1, ctab, st));
+            else HIPCHECK(launch_coef(b, in, ke64, ctab, st, 1));
+            if (tm) tm->end();
+        }
+        if (tm) tm->begin(T_MFMA);
+        HIPCHECK(launch_factor_fused(P.d_fused[fa].as<FusedSlice>(), P.n_fused[fa], P.fused_total[fa],
+                                     P.d_fused_counter.as<unsigned long long>(), fa, P.fused_lds[fa], P.d_ke.p, P.n_cu, st));
+        if (tm) tm->end();
+    }
     for (size_t bi = 0; bi < P.dbuckets.size(); bi++) {
         const DevBucket& b = P.dbuckets[bi];
         const hipStream_t bst = (overlap && P.rhs_only[bi]) ? P.side : st;
-        void* ctab = P.rhs_only[bi] ? P.d_ctab_side.p : P.d_ctab.p;
+        const uint32_t fmask = factor_mask(bi);  // slices on the factorized kernel
+        const bool in_fused = fused && P.fuse_bucket[bi] && fmask != 0;
+        void* ctab = in_fused ? (void*)((char*)P.d_ctab_fused.p + P.fuse_off[bi]) : (P.rhs_only[bi] ? P.d_ctab_side.p : P.d_ctab.p);
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
         uint32_t done = 0;
         // the factorized kernel reads the coefficients of one element contiguously
-        uint32_t fmask = 0;  // slices on the factorized kernel in this arithmetic (or in FP64)
-        if (arith >= 0) {
-            for (auto& fs : P.factor[bi])
-                if ((fs.second.arith_ok >> arith) & 1u) fmask |= 1u << fs.first;
-        } else if (ke64 && P.factor64) {
-            for (auto& fs : P.factor[bi])
-                if (fs.second.f64_ok) fmask |= 1u << fs.first;
-        }
         const int qfast = fmask != 0 ? 1 : 0;
         P.last_qfast = qfast;
-        if (tm) tm->begin(T_COEF);
-        const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
-        if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, qfast, ctab, bst));
-        else HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
-        if (tm) tm->end();
+        P.last_ctab = ctab;
+        if (!in_fused) {
+            if (tm) tm->begin(T_COEF);
+            const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
+            if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, qfast, ctab, bst));
+            else HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
+            if (tm) tm->end();
+        }
         if (arith >= 0 && P.mfma[bi].M > 0 && ((P.mfma[bi].arith_ok >> arith) & 1u)) {
             if (tm) tm->begin(T_MFMA);
             HIPCHECK(launch_mfma(b, mfma_for(P.mfma[bi], arith), (const float*)ctab, P.d_ke.as<float>(), bst));
             if (tm) tm->end();
             done = P.mfma_mask[bi];
         }
-        if (fmask != 0) {
+        if (fmask != 0 && in_fused) {
+            done |= fmask;  // computed by the fused launch
+        } else if (fmask != 0) {
             if (tm) tm->begin(T_MFMA);
             for (auto& fs : P.factor[bi])
                 if ((fmask >> fs.first) & 1u) {
@@ -2055,6 +2177,20 @@ slx_status slx_plan_kernels(slx_plan p, int32_t* n_mfma, int32_t* n_buckets) {
     });
 }
 
+slx_status slx_plan_fused_info(slx_plan p, int32_t precision, int32_t* n_slices, int64_t* n_items) {
+    return guarded([&] {
+        SLX_CHECK(p, "null plan");
+        SLX_CHECK(precision >= 0 && precision <= 4, "unknown precision " << precision);
+        const int fa = precision == SLX_PREC_F16_SPLIT ? AR_F16
+                       : precision == SLX_PREC_BF16_SPLIT ? AR_BF16
+                       : precision == SLX_PREC_FP32_MFMA ? AR_F32
+                       : (precision == SLX_PREC_FP64 && p->factor64) ? 3 : -1;
+        const bool on = p->fused && fa >= 0 && ((p->fused_arith >> fa) & 1u);
+        if (n_slices) *n_slices = on ? p->n_fused[fa] : 0;
+        if (n_items) *n_items = on ? p->fused_total[fa] : 0;
+    });
+}
+
 slx_status slx_plan_kernel_counts(slx_plan p, int32_t precision, int32_t* n_batched, int32_t* n_factorized) {
     return guarded([&] {
         SLX_CHECK(p, "null plan");
@@ -2141,7 +2277,7 @@ slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* 
         SLX_CHECK(!p->host_only, "plan was created without a device context (pattern only)");
         SLX_CHECK(p->last_precision >= 0 && !p->dbuckets.empty(), "no assembly has run on this plan");
         const DevBucket& b = p->dbuckets.back();
-        const void* ctab = p->rhs_only.back() ? p->d_ctab_side.p : p->d_ctab.p;
+        const void* ctab = p->last_ctab;
         const int64_t n = (int64_t)b.n_coefs * (b.q_inv ? 1 : b.n_gp) * b.n_elem;
         *n_out = n;
         if (!out) return;
