@@ -309,6 +309,13 @@ void Mesh::orient() {
     }
 }
 
+namespace {
+thread_local void* g_mesh_stream = nullptr;
+}
+
+void set_mesh_pipeline_stream(void* hip_stream) { g_mesh_stream = hip_stream; }
+void* mesh_pipeline_stream() { return g_mesh_stream; }
+
 void Mesh::finalize() {
     SLX_CHECK(!finalized_, "mesh already finalized");
     SLX_CHECK(count(0) > 0, "mesh has no nodes");
@@ -316,6 +323,21 @@ void Mesh::finalize() {
     for (int t = 1; t <= 7; t++)
         if (count(t) > 0) maxt = t;
     meshdim_ = topo::dimension(maxt);
+    if (g_mesh_stream && phys_number_.size() <= 16) {
+        // duplicate nodes first (they only rename node numbers, which commutes
+        // with the explosion), then every other step on the device
+        std::vector<int> renum = remove_duplicates(0);
+        for (auto& pe : phys_elems_)
+            for (int& e : pe[0]) e = renum[e];
+        finalize_on_device(*this, g_mesh_stream);
+        define_disjoint_region_ranges();
+        int np = (int)phys_number_.size(), nd = (int)dr_type_.size();
+        phys_includes_.assign(np, std::vector<char>(nd, 0));
+        for (int pi = 0; pi < np; pi++)
+            for (int d = 0; d < nd; d++) phys_includes_[pi][d] = dr_def_[d][pi];
+        finalized_ = true;
+        return;
+    }
     explode();
     // rawmesh::removeduplicates (rawmesh.cpp:160-182)
     for (int t = 0; t <= 7; t++) {

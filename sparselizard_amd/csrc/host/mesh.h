@@ -18,6 +18,16 @@
 
 namespace slx {
 
+class Mesh;
+// The pipeline's steps after the duplicate nodes on the device
+// (kernels/meshpipe.hip): the mesh's arrays end exactly as the host steps
+// leave them.
+void finalize_on_device(Mesh& m, void* hip_stream);
+// Stream of the device pipeline for meshes finalized on this thread (null:
+// the host pipeline); set around a problem's construction.
+void set_mesh_pipeline_stream(void* hip_stream);
+void* mesh_pipeline_stream();
+
 class Mesh {
    public:
     // ---- loading ---------------------------------------------------------
@@ -61,6 +71,7 @@ class Mesh {
     std::vector<int> physreg_elements(int physnum, int type) const;
 
    private:
+    friend void finalize_on_device(Mesh& m, void* hip_stream);
     int physreg_index(int physnum) const;
     std::vector<int> remove_duplicates(int type);   // returns the renumbering
     void renumber(int type, const std::vector<int>& renum);

@@ -608,9 +608,13 @@ std::shared_ptr<Problem> highorderprojection2d(const std::map<std::string, std::
 
 }  // namespace
 
-std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec) {
+std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec, void* hip_stream) {
     auto s = parse_spec(spec);
     std::shared_ptr<Problem> p;
+    struct StreamScope {
+        explicit StreamScope(void* st) { set_mesh_pipeline_stream(st); }
+        ~StreamScope() { set_mesh_pipeline_stream(nullptr); }
+    } scope(s.count("meshpipe") && s.at("meshpipe") == "host" ? nullptr : hip_stream);
     if (kind == "poisson2d") p = poisson2d(s);
     else if (kind == "elasticity3d") p = elasticity3d(s);
     else if (kind == "quadbatch") p = quadbatch(s);

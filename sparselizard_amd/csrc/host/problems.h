@@ -37,6 +37,8 @@ struct Problem {
 };
 
 std::map<std::string, std::string> parse_spec(const std::string& spec);
-std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec);
+// hip_stream: the meshes are finalized on the device on that stream (null or
+// the spec option meshpipe=host: on the host).
+std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string& spec, void* hip_stream = nullptr);
 
 }  // namespace slx

@@ -2456,7 +2456,8 @@ slx_status slx_problem_create(slx_ctx ctx, const char* kind, const char* spec, s
         auto p = new slx_problem_s();
         p->ctx = ctx;
         try {
-            p->prob = make_problem(kind, spec ? spec : "");
+            if (ctx) HIPCHECK(hipSetDevice(ctx->device));
+            p->prob = make_problem(kind, spec ? spec : "", ctx ? (void*)ctx->stream : nullptr);
         } catch (...) {
             delete p;
             throw;

@@ -1,0 +1,84 @@
+"""The mesh loading pipeline on the device (kernels/meshpipe.hip: explode,
+duplicates, disjoint regions, reordering by region, orientation; rawmesh.cpp:
+254-325) against the host pipeline (Mesh::finalize, the restated steps of
+elements.cpp:947-1441 and gentools.cpp:89-246) on the same input: the plan
+descriptions built on the two meshes must be identical, array by array (node
+coordinates in their final numbering, element node lists, element numbers,
+every DOF address and sign table, the constraint mask and Dirichlet values).
+A problem built with a device context uses the device pipeline; the spec
+option meshpipe=host forces the host one.  The host pipeline is pinned by the
+CPU suite (numbering, orientation, goldens)."""
+import os
+
+import numpy as np
+import pytest
+
+import sparselizard_amd as sa
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = [
+    ("poisson2d", "n=12;order=3"),
+    ("poisson2d", "n=1"),
+    ("elasticity3d", "n=4"),
+    ("elasticity3d", "n=2;order=1;clamp=0"),
+    ("hppoisson3d", "n=4;pmin=2;pmax=4"),
+    ("hppoisson3d", "n=3;pmin=1;pmax=5;fold=0"),
+    ("quadbatch", "instances=6"),
+    ("newton2d", "n=10"),
+    ("prism3d", "n=3;order=3"),
+    ("convdiff2d", "n=12;order=2;bx=20;by=10"),
+    ("projection", "elem=tet;n=3;order=3"),
+    ("projection", "elem=quad;n=4;order=3"),
+    ("electrostatics1d", "file=" + os.path.join(GOLDEN, "electrostatics1d_line.msh")),
+    ("hoprojection2d", "file=%s;order=5" % os.path.join(GOLDEN, "hointerp2d_quad.msh")),
+]
+
+
+def arr(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype)
+    assert ptr, "null table"
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype).copy()
+
+
+def snapshot(prob):
+    d = prob.desc()
+    info = prob.info()
+    out = {"info": info, "n_nodes": d.n_nodes, "mesh_dim": d.mesh_dim}
+    num = prob.numeric()
+    out["xyz"] = arr(num.node_xyz, 3 * d.n_nodes, np.float64)
+    out["constrained"] = arr(d.constrained, d.n_dofs, np.uint8)
+    out["dirichlet"] = arr(d.dirichlet, d.n_dofs, np.float64) if d.dirichlet else None
+    if num.n_params:
+        out["params"] = arr(num.params, num.n_params * num.param_len, np.float64)
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        out["b%d" % b] = (bd.elem_type, bd.n_elem, bd.n_geo, bd.n_slices, bd.n_gp)
+        out["b%d.geo" % b] = arr(bd.geo_nodes, bd.n_elem * bd.n_geo, np.int32)
+        out["b%d.ids" % b] = arr(bd.elem_ids, bd.n_elem, np.int32)
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            out["b%d.s%d.tf" % (b, s)] = arr(sd.tf_addr, sd.n_tf * bd.n_elem, np.int32)
+            if not sd.is_rhs:
+                out["b%d.s%d.dof" % (b, s)] = arr(sd.dof_addr, sd.n_dof * bd.n_elem, np.int32)
+            if sd.tf_sign:
+                out["b%d.s%d.tfs" % (b, s)] = arr(sd.tf_sign, sd.n_tf * bd.n_elem, np.int8)
+            if sd.dof_sign:
+                out["b%d.s%d.dofs" % (b, s)] = arr(sd.dof_sign, sd.n_dof * bd.n_elem, np.int8)
+    return out
+
+
+@pytest.mark.parametrize("kind,spec", CASES)
+def test_device_mesh_pipeline_matches_host(ctx, kind, spec):
+    print("case meshpipe", kind, spec, flush=True)
+    dev = snapshot(sa.Problem(ctx, kind, spec))
+    host = snapshot(sa.Problem(ctx, kind, spec + ";meshpipe=host"))
+    assert dev.keys() == host.keys()
+    for k in dev:
+        a, b = dev[k], host[k]
+        if isinstance(a, np.ndarray):
+            assert a.shape == b.shape and np.array_equal(a, b), (kind, spec, k)
+        else:
+            assert a == b, (kind, spec, k, a, b)
