@@ -47,7 +47,10 @@ struct FactorSlice {
 
 size_t factor_lds_bytes(const FactorSlice& f, int arith);
 size_t factor64_lds_bytes(const FactorSlice& f);
-bool factor_variant_fits(int mt, int nt, int* bm, int* bn);
+// Tile blocks per wave (bm row tiles, bn column tiles per pass) for an
+// mt x nt tile matrix: `variant` (0..2) when it covers the rows, else the
+// smallest variant that does.
+bool factor_variant_fits(int mt, int nt, int* bm, int* bn, int variant = -1);
 // Contraction of one slice in arithmetic `arith` (MfmaArith).
 hipError_t launch_factor(const DevBucket& b, const FactorSlice& f, int arith, const float* ctab, float* ke, int n_cu,
                          hipStream_t st);

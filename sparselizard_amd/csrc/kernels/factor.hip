@@ -39,7 +39,12 @@ size_t factor_lds_bytes(const FactorSlice& f, int arith) {
 static const int FK_BM[3] = {2, 4, 7};
 static const int FK_BN[3] = {1, 2, 2};
 
-bool factor_variant_fits(int mt, int nt, int* bm, int* bn) {
+bool factor_variant_fits(int mt, int nt, int* bm, int* bn, int variant) {
+    if (variant >= 0 && variant < 3 && 2 * FK_BM[variant] >= mt) {
+        *bm = FK_BM[variant];
+        *bn = FK_BN[variant];
+        return true;
+    }
     for (int v = 0; v < 3; v++)
         if (2 * FK_BM[v] >= mt && (4 * FK_BN[v] >= nt || v == 2)) {
             *bm = FK_BM[v];

@@ -911,8 +911,11 @@ void split_bf16(float x, uint16_t& hi, uint16_t& mid, uint16_t& lo) {
 // per wave), valu (FP32 element kernel) or factor (factorized MFMA).
 // Classes the table does not list use the built-in rules below.
 enum { TK_NONE = 0, TK_MFMA = 1, TK_VALU = 2, TK_FACTOR = 3 };
+// kernel_table.txt rows: elem_type n_tf n_dof q_invariant kernel mtw [variant]
+// (variant: tile blocks per wave of the factorized kernel, 0 = (2, 1),
+// 1 = (4, 2), 2 = (7, 2); absent or -1: the smallest that covers the matrix)
 struct TuneEntry {
-    int type, n_tf, n_dof, q_inv, kernel, mtw;
+    int type, n_tf, n_dof, q_inv, kernel, mtw, variant;
 };
 
 // The table is read again when SLX_TUNE_TABLE changes; callers copy what they
@@ -945,8 +948,9 @@ const std::vector<TuneEntry>& tune_table() {
         while (std::fgets(line, sizeof(line), f)) {
             if (line[0] == '#') continue;
             TuneEntry x{};
+            x.variant = -1;
             char kern[32] = {0};
-            if (std::sscanf(line, "%d %d %d %d %31s %d", &x.type, &x.n_tf, &x.n_dof, &x.q_inv, kern, &x.mtw) != 6) continue;
+            if (std::sscanf(line, "%d %d %d %d %31s %d %d", &x.type, &x.n_tf, &x.n_dof, &x.q_inv, kern, &x.mtw, &x.variant) < 6) continue;
             const std::string k = kern;
             x.kernel = k == "mfma" ? TK_MFMA : k == "valu" ? TK_VALU : k == "factor" ? TK_FACTOR : TK_NONE;
             if (x.kernel != TK_NONE) t.push_back(x);
@@ -957,11 +961,12 @@ const std::vector<TuneEntry>& tune_table() {
     return table;
 }
 
-int tune_lookup(int type, int n_tf, int n_dof, int q_inv, int* mtw) {
+int tune_lookup(int type, int n_tf, int n_dof, int q_inv, int* mtw, int* variant = nullptr) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     for (const TuneEntry& x : tune_table())
         if (x.type == type && x.n_tf == n_tf && x.n_dof == n_dof && x.q_inv == q_inv) {
             if (mtw) *mtw = x.mtw;
+            if (variant) *variant = x.variant;
             return x.kernel;
         }
     return TK_NONE;
@@ -1182,7 +1187,8 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
-        const int tuned = tune_lookup(bd.elem_type, sd.n_tf, sd.n_dof, bd.q_invariant ? 1 : 0, nullptr);
+        int tuned_variant = -1;
+        const int tuned = tune_lookup(bd.elem_type, sd.n_tf, sd.n_dof, bd.q_invariant ? 1 : 0, nullptr, &tuned_variant);
         if (tuned == TK_VALU || sd.n_terms > SLX_FK_MAXT) continue;
         if (tuned != TK_FACTOR && (int64_t)sd.n_tf * sd.n_dof <= batched_max_m()) continue;
         FactorSlice f;
@@ -1195,7 +1201,7 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         f.nt = (sd.n_dof + 15) / 16;
         f.ntp = f.nt * 16;
         f.n_terms = sd.n_terms;
-        if (!factor_variant_fits(f.mt, f.nt, &f.bm, &f.bn)) continue;
+        if (!factor_variant_fits(f.mt, f.nt, &f.bm, &f.bn, tuned == TK_FACTOR ? tuned_variant : -1)) continue;
         // tf derivatives in use; every term's (derivative index, dof derivative, slot)
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
@@ -2283,21 +2289,23 @@ slx_status slx_plan_coefficients(slx_plan p, double* out, int64_t cap, int64_t* 
         if (!out) return;
         SLX_CHECK(cap >= n, "output holds " << cap << " entries, the table has " << n);
         HIPCHECK(hipStreamSynchronize(p->stream));
+        std::vector<double> tmp(n);
         if (p->last_precision == SLX_PREC_FP64) {
-            HIPCHECK(hipMemcpy(out, ctab, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(tmp.data(), ctab, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
         } else {
-            std::vector<float> tmp(n);
-            HIPCHECK(hipMemcpy(tmp.data(), ctab, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
-            const int64_t nq = b.q_inv ? 1 : b.n_gp, ne = b.n_elem;
-            if (p->last_qfast) {
-                // (slot, element, q) -> (slot, q, element)
-                for (int64_t i = 0; i < n; i++) {
-                    const int64_t c = i / (nq * ne), r = i % (nq * ne), e = r / nq, q = r % nq;
-                    out[(c * nq + q) * ne + e] = tmp[i];
-                }
-            } else {
-                for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
+            std::vector<float> t32(n);
+            HIPCHECK(hipMemcpy(t32.data(), ctab, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
+            for (int64_t i = 0; i < n; i++) tmp[i] = t32[i];
+        }
+        const int64_t nq = b.q_inv ? 1 : b.n_gp, ne = b.n_elem;
+        if (p->last_qfast) {
+            // the factorized kernels' layout (slot, element, q) -> (slot, q, element)
+            for (int64_t i = 0; i < n; i++) {
+                const int64_t c = i / (nq * ne), r = i % (nq * ne), e = r / nq, q = r % nq;
+                out[(c * nq + q) * ne + e] = tmp[i];
             }
+        } else {
+            for (int64_t i = 0; i < n; i++) out[i] = tmp[i];
         }
     });
 }

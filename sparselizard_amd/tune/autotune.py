@@ -1,14 +1,17 @@
-"""Autotuner of the element-kernel choice per bucket class.
+"""Autotuner of the element-kernel choice and tile shape per bucket class.
 
-For every representative workload below it times the split-precision
-assembly (HIP events per launch, slx_plan_profile) with each candidate
-kernel for the matrix slices of the workload's buckets -- the batched MFMA
-kernel with one or two M tiles per wave, the factorized MFMA kernel, or the
-FP32 VALU element kernel -- keeps the candidates whose global-matrix error
-against the FP64 assembly is at most 1e-6, and writes the fastest per bucket
-class to kernel_table.txt (next to this script), which the library reads at
-plan creation.  Every (workload, candidate) runs in a fresh process with
-SLX_TUNE_TABLE pointing at a one-candidate table.  Needs one GPU.
+A bucket class is (element type, n_tf, n_dof, coefficients constant over the
+Gauss points).  For every class of every representative workload below it
+times the split-precision assembly (HIP events per launch, slx_plan_profile)
+with each candidate for that class alone -- the batched MFMA kernel with one
+or two M tiles per wave (16 elements per workgroup tile), the factorized MFMA
+kernel with each tile shape that covers the matrix ((2, 1), (4, 2) or (7, 2)
+16x16 tiles per wave in rows and columns), or the FP32 VALU element kernel --
+while the workload's other classes keep the built-in choice.  Candidates whose
+global-matrix error against the FP64 assembly exceeds 1e-6 are dropped, and
+the fastest per class goes to kernel_table.txt (next to this script), which
+the library reads at plan creation.  Every (class, candidate) runs in a fresh
+process with SLX_TUNE_TABLE pointing at a one-line table.  Needs one GPU.
 
 usage: python3 sparselizard_amd/tune/autotune.py [--out FILE] [--iters N]
 """
@@ -37,19 +40,21 @@ WORKLOADS = [
     ("prism3d", "n=20;order=2"),
     ("prism3d", "n=16;order=3"),
 ]
-CANDIDATES = [("mfma", 1), ("mfma", 2), ("factor", 1), ("valu", 1)]
+# factorized tile shapes (row tiles, column tiles per wave), the variant index of the table
+FACTOR_SHAPES = [(2, 1), (4, 2), (7, 2)]
 
 
-def feasible(kern, c):
-    """Whether a kernel can take the matrix slices of class c = (type, n_tf, n_dof, q_inv):
-    the batched kernel holds at most 16 M tiles of 16 rows, the factorized one is
-    for larger element matrices, the VALU kernel takes everything."""
+def candidates(c):
+    """(kernel, mtw, variant) candidates for class c = (type, n_tf, n_dof, q_inv): the
+    batched kernel holds at most 16 M tiles of 16 rows, the factorized one is for
+    larger element matrices (a shape must cover the rows with its 2 row waves), the
+    VALU kernel takes everything.  The first is the reference of the 3 % rule."""
     m = c[1] * c[2]
-    if kern == "mfma":
-        return m <= 256
-    if kern == "factor":
-        return m > 256
-    return True
+    if m <= 256:
+        return [("mfma", 1, -1), ("mfma", 2, -1), ("valu", 1, -1)]
+    mt = (c[1] + 15) // 16
+    out = [("factor", 1, v) for v, (bm, bn) in enumerate(FACTOR_SHAPES) if 2 * bm >= mt]
+    return out + [("valu", 1, -1)]
 
 
 def classes_of(kind, spec):
@@ -110,48 +115,48 @@ def main():
     for kind, spec in WORKLOADS:
         if args.only and kind != args.only:
             continue
-        cls = classes_of(kind, spec)
-        results = []
-        for kern, mtw in CANDIDATES:
-            if not any(feasible(kern, c) for c in cls):
+        for c in classes_of(kind, spec):
+            results = []
+            for kern, mtw, var in candidates(c):
+                # this class alone in the table: the workload's other classes keep the built-in choice
+                with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
+                    f.write("%d %d %d %d %s %d %d\n" % (c + (kern, mtw, var)))
+                    tab = f.name
+                env = dict(os.environ, SLX_TUNE_TABLE=tab)
+                print("tuning %s %s class %s: %s mtw %d variant %d" % (kind, spec, c, kern, mtw, var), flush=True)
+                r = subprocess.run([sys.executable, __file__, "--child", kind, spec, "--iters", str(args.iters)],
+                                   env=env, capture_output=True, text=True, timeout=600)
+                os.unlink(tab)
+                if r.returncode != 0:
+                    # a candidate the kernels reject (launch limits) is skipped
+                    print("   failed: %s" % (r.stderr.strip().splitlines() or ["?"])[-1][:300], flush=True)
+                    continue
+                res = json.loads(r.stdout.strip().splitlines()[-1])
+                print("   %.3f ms, error %.2e" % (res["ms"], res["err"]), flush=True)
+                results.append((kern, mtw, var, res))
+            ok = [x for x in results if x[3]["err"] <= 1e-6]
+            if not ok:
                 continue
-            # classes the candidate cannot take keep the built-in choice
-            with tempfile.NamedTemporaryFile("w", suffix=".txt", delete=False) as f:
-                for c in cls:
-                    if feasible(kern, c):
-                        f.write("%d %d %d %d %s %d\n" % (c + (kern, mtw)))
-                tab = f.name
-            env = dict(os.environ, SLX_TUNE_TABLE=tab)
-            print("tuning %s %s: %s mtw %d" % (kind, spec, kern, mtw), flush=True)
-            r = subprocess.run([sys.executable, __file__, "--child", kind, spec, "--iters", str(args.iters)],
-                               env=env, capture_output=True, text=True, timeout=600)
-            os.unlink(tab)
-            if r.returncode != 0:
-                # a candidate the kernels reject (launch limits) is skipped
-                print("   failed: %s" % (r.stderr.strip().splitlines() or ["?"])[-1][:300], flush=True)
-                continue
-            res = json.loads(r.stdout.strip().splitlines()[-1])
-            print("   %.3f ms, error %.2e" % (res["ms"], res["err"]), flush=True)
-            results.append((kern, mtw, res))
-        ok = [x for x in results if x[2]["err"] <= 1e-6]
-        # the first candidate in CANDIDATES order unless another is at least 3 % faster
-        kern, mtw, res = ok[0]
-        for x in ok[1:]:
-            if x[2]["ms"] < 0.97 * res["ms"]:
-                kern, mtw, res = x
-        for c in cls:
-            if feasible(kern, c):
-                best[c] = (kern, mtw)
-        log.append("# %s %s: %s" % (kind, spec, ", ".join("%s/%d %.3f ms" % (k, m, x["ms"]) for k, m, x in results)))
+            # the first candidate unless another is at least 3 % faster
+            best_x = ok[0]
+            for x in ok[1:]:
+                if x[3]["ms"] < 0.97 * best_x[3]["ms"]:
+                    best_x = x
+            best[c] = best_x[:3]  # a later workload's choice for the same class wins
+            log.append("# %s %s class %s: %s" % (kind, spec, c, ", ".join(
+                "%s/%d%s %.3f ms" % (k, m, "/v%d" % v if v >= 0 else "", x["ms"]) for k, m, v, x in results)))
     with open(args.out, "w") as f:
-        f.write("# Kernel choice per bucket class, written by autotune.py on one MI355X (split precision,\n")
-        f.write("# HIP-event time of one assembly; candidates above 1e-6 global error excluded; a candidate\n")
-        f.write("# replaces an earlier one in the list only when at least 3 % faster).\n")
-        f.write("# columns: elem_type n_tf n_dof q_invariant kernel mtw\n")
+        f.write("# Kernel choice and tile shape per bucket class, written by autotune.py on one MI355X\n")
+        f.write("# (split precision, HIP-event time of one assembly with the class on the candidate and the\n")
+        f.write("# workload's other classes on the built-in choice; candidates above 1e-6 global error\n")
+        f.write("# excluded; a candidate replaces the first one only when at least 3 % faster).\n")
+        f.write("# columns: elem_type n_tf n_dof q_invariant kernel mtw variant\n")
+        f.write("# (mtw: M tiles per wave of the batched kernel; variant: tile shape of the factorized\n")
+        f.write("# kernel, 0 = (2, 1), 1 = (4, 2), 2 = (7, 2) row and column tiles per wave)\n")
         for line in log:
             f.write(line + "\n")
         for c in sorted(best):
-            f.write("%d %d %d %d %s %d\n" % (c + best[c]))
+            f.write("%d %d %d %d %s %d %d\n" % (c + best[c]))
     print("wrote", args.out)
 
 

@@ -175,3 +175,16 @@ def test_fp64_factorized_element_matrices_and_valu(ctx, monkeypatch):
     plan2 = prob2.generate("fp64")
     assert plan2.kernel_counts("fp64")["factorized"] == 0
     assert relerr(ke[idx], plan2.element_matrices()[idx]) <= 1e-13
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("precision,tol", [("f16split", 1e-6), ("fp64", 1e-12)])
+def test_factor_tile_shapes(ctx, variant, precision, tol, tmp_path, monkeypatch):
+    # every tile shape of the factorized kernel the kernel table can name for
+    # order-3 hexahedra (64 x 64 element matrices): (2, 1), (4, 2), (7, 2)
+    # tiles per wave, same results to the mode's gate
+    table = tmp_path / "table.txt"
+    table.write_text("5 64 64 0 factor 1 %d\n" % variant)
+    monkeypatch.setenv("SLX_TUNE_TABLE", str(table))
+    prob, plan, got, errs = check_case(ctx, "hppoisson3d", "n=2;pmin=3;pmax=3", precision, tol)
+    assert plan.kernel_counts(precision)["factorized"] >= 1
