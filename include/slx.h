@@ -363,6 +363,12 @@ slx_status slx_h1_count(int32_t elem_type, int32_t order, int32_t* nff);
  * for all four derivative slots: out[deriv][ff][pt], deriv 0 value, 1..3 d/dki,eta,phi. */
 slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t npts, const double* pts,
                          double* out);
+/* Geometry (Lagrange) functions of an element type with n_geo nodes per
+ * element: the corner nodes (order 1) or every node of a curved order-2
+ * element in gmsh order (3 line, 6 triangle, 9 quadrangle, 10 tetrahedron,
+ * 27 hexahedron, 18 prism; lagrange*.cpp getnodecoordinates / evalat):
+ * out[deriv][node][pt], deriv 0 value, 1..3 d/dki,eta,phi. */
+slx_status slx_geometry_tables(int32_t elem_type, int32_t n_geo, int32_t npts, const double* pts, double* out);
 
 /* ---- Formulation layer ---------------------------------------------------- */
 typedef struct slx_problem_s* slx_problem;

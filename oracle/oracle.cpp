@@ -28,6 +28,8 @@
 // OpenBLAS's summation order for larger shapes is not pinned (parity unpinned
 // at the dgemm boundary: values are compared with a relative tolerance).
 #include <algorithm>
+#include <array>
+#include <stdexcept>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -42,6 +44,21 @@ namespace {
 struct Frag {
     int bucket, slice, rank;
 };
+
+bool lagrange2(int type, const double* pts, int np, std::vector<double>& out);
+void lagrange1(int type, const double* pts, int np, std::vector<double>& out);
+
+// Geometry functions of a bucket: order 1 on the corner nodes, or order 2 when
+// the bucket lists every node of curved elements.
+void geometry(const slx_bucket_desc& b, std::vector<double>& out) {
+    const int corners = b.elem_type == 1 ? 2 : b.elem_type == 2 ? 3 : b.elem_type == 3 ? 4 : b.elem_type == 4 ? 4 : b.elem_type == 6 ? 6 : 8;
+    if (b.n_geo != corners) {
+        if (!lagrange2(b.elem_type, b.gp_coords, b.n_gp, out) || (int)out.size() != 4 * b.n_geo * b.n_gp)
+            throw std::runtime_error("oracle: unsupported geometry");
+        return;
+    }
+    lagrange1(b.elem_type, b.gp_coords, b.n_gp, out);
+}
 
 void jacobian(const slx_plan_desc& d, const slx_bucket_desc& b, const double* xyz, const std::vector<double>& geo, int e, int q,
               double J[3][3], double inv[3][3], double& det) {
@@ -80,6 +97,97 @@ void jacobian(const slx_plan_desc& d, const slx_bucket_desc& b, const double* xy
         inv[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
         inv[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
     }
+}
+
+// Order-2 (curved) geometry functions: the Lagrange interpolants of the nodes
+// of lagrange*.cpp getnodecoordinates(2) (gmsh node order), built here by
+// inverting the Vandermonde matrix of each type's polynomial space (tensor
+// quadratics for lines, quadrangles and hexahedra, total degree 2 for
+// triangles and tetrahedra, their product for prisms) -- not from closed forms.
+// Returns false for a type without order-2 geometry.
+bool lagrange2(int type, const double* pts, int np, std::vector<double>& out) {
+    static const double line[] = {-1, 0, 0, 1, 0, 0, 0, 0, 0};
+    static const double tri[] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0.5, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0};
+    static const double quad[] = {-1, -1, 0, 1, -1, 0, 1, 1, 0, -1, 1, 0, 0, -1, 0, 1, 0, 0, 0, 1, 0, -1, 0, 0, 0, 0, 0};
+    static const double tet[] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0.5, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0, 0, 0.5, 0, 0.5, 0.5, 0.5, 0, 0.5};
+    static const double hex[] = {-1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1, 0, -1, -1,
+                                 -1, 0, -1, -1, -1, 0, 1, 0, -1, 1, -1, 0, 0, 1, -1, 1, 1, 0, -1, 1, 0, 0, -1, 1, -1, 0, 1,
+                                 1, 0, 1, 0, 1, 1, 0, 0, -1, 0, -1, 0, -1, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    static const double prism[] = {0, 0, -1, 1, 0, -1, 0, 1, -1, 0, 0, 1, 1, 0, 1, 0, 1, 1, 0.5, 0, -1, 0, 0.5, -1, 0, 0, 0,
+                                   0.5, 0.5, -1, 1, 0, 0, 0, 1, 0, 0.5, 0, 1, 0, 0.5, 1, 0.5, 0.5, 1, 0.5, 0, 0, 0, 0.5, 0,
+                                   0.5, 0.5, 0};
+    const double* c;
+    int nn;
+    switch (type) {
+        case 1: c = line; nn = 3; break;
+        case 2: c = tri; nn = 6; break;
+        case 3: c = quad; nn = 9; break;
+        case 4: c = tet; nn = 10; break;
+        case 5: c = hex; nn = 27; break;
+        case 6: c = prism; nn = 18; break;
+        default: return false;
+    }
+    // exponents (a, b, g) of the polynomial space
+    std::vector<std::array<int, 3>> mono;
+    for (int a = 0; a <= 2; a++)
+        for (int b2 = 0; b2 <= 2; b2++)
+            for (int g = 0; g <= 2; g++) {
+                bool in = false;
+                if (type == 1) in = b2 == 0 && g == 0;
+                else if (type == 3) in = g == 0;
+                else if (type == 5) in = true;
+                else if (type == 2) in = g == 0 && a + b2 <= 2;
+                else if (type == 4) in = a + b2 + g <= 2;
+                else in = a + b2 <= 2;  // prism: triangle quadratics times line quadratics
+                if (in) mono.push_back({a, b2, g});
+            }
+    if ((int)mono.size() != nn) return false;
+    // V[k][m] = monomial m at node k; invert (Gauss-Jordan, partial pivoting)
+    std::vector<double> V((size_t)nn * nn), I((size_t)nn * nn, 0.0);
+    auto pw = [](double x, int e) { return e == 0 ? 1.0 : e == 1 ? x : x * x; };
+    for (int k = 0; k < nn; k++) {
+        for (int m = 0; m < nn; m++) V[(size_t)k * nn + m] = pw(c[3 * k], mono[m][0]) * pw(c[3 * k + 1], mono[m][1]) * pw(c[3 * k + 2], mono[m][2]);
+        I[(size_t)k * nn + k] = 1.0;
+    }
+    for (int col = 0; col < nn; col++) {
+        int piv = col;
+        for (int r = col + 1; r < nn; r++)
+            if (std::fabs(V[(size_t)r * nn + col]) > std::fabs(V[(size_t)piv * nn + col])) piv = r;
+        for (int j = 0; j < nn; j++) {
+            std::swap(V[(size_t)col * nn + j], V[(size_t)piv * nn + j]);
+            std::swap(I[(size_t)col * nn + j], I[(size_t)piv * nn + j]);
+        }
+        const double d = V[(size_t)col * nn + col];
+        for (int j = 0; j < nn; j++) {
+            V[(size_t)col * nn + j] /= d;
+            I[(size_t)col * nn + j] /= d;
+        }
+        for (int r = 0; r < nn; r++)
+            if (r != col) {
+                const double f = V[(size_t)r * nn + col];
+                if (f == 0.0) continue;
+                for (int j = 0; j < nn; j++) {
+                    V[(size_t)r * nn + j] -= f * V[(size_t)col * nn + j];
+                    I[(size_t)r * nn + j] -= f * I[(size_t)col * nn + j];
+                }
+            }
+    }
+    // N_k(x) = sum_m mono_m(x) Vinv[m][k]
+    auto dpw = [](double x, int e) { return e == 0 ? 0.0 : e == 1 ? 1.0 : 2.0 * x; };
+    out.assign((size_t)4 * nn * np, 0.0);
+    for (int p = 0; p < np; p++) {
+        const double x = pts[3 * p], y = pts[3 * p + 1], z = pts[3 * p + 2];
+        for (int m = 0; m < nn; m++) {
+            const int a = mono[m][0], b2 = mono[m][1], g = mono[m][2];
+            const double v[4] = {pw(x, a) * pw(y, b2) * pw(z, g), dpw(x, a) * pw(y, b2) * pw(z, g), pw(x, a) * dpw(y, b2) * pw(z, g),
+                                 pw(x, a) * pw(y, b2) * dpw(z, g)};
+            for (int k = 0; k < nn; k++) {
+                const double w = I[(size_t)m * nn + k];
+                for (int dd = 0; dd < 4; dd++) out[((size_t)dd * nn + k) * np + p] += v[dd] * w;
+            }
+        }
+    }
+    return true;
 }
 
 // Order-1 Lagrange functions and reference derivatives: out[deriv][node][pt].
@@ -199,13 +307,23 @@ extern "C" {
 // Coefficient table of one bucket, as the device lays it out:
 // out[(slot * nq + q) * n_elem + e] = coefficient * |det J|, nq = 1 for
 // buckets whose coefficients do not vary over the Gauss points.
+// Order-2 geometry functions of `type` at np points (reference coordinates,
+// 3 per point): out[deriv][node][pt]; returns the node count (0: no order-2
+// geometry for the type).
+int oracle_lagrange2(int type, const double* pts, int np, double* out) {
+    std::vector<double> v;
+    if (!lagrange2(type, pts, np, v)) return 0;
+    std::copy(v.begin(), v.end(), out);
+    return (int)(v.size() / (4 * (size_t)np));
+}
+
 int oracle_coefficients(const slx_plan_desc* dp, const slx_numeric_desc* inp, int bucket, double* out) {
     const slx_plan_desc& d = *dp;
     if (bucket < 0 || bucket >= d.n_buckets) return -1;
     const slx_bucket_desc& b = d.buckets[bucket];
     const int nq = b.q_invariant ? 1 : b.n_gp;
     std::vector<double> geo;
-    lagrange1(b.elem_type, b.gp_coords, b.n_gp, geo);
+    geometry(b, geo);
     std::vector<double> cbuf(std::max(b.n_coefs, 1));
     for (int e = 0; e < b.n_elem; e++)
         for (int q = 0; q < nq; q++) {
@@ -273,7 +391,7 @@ int oracle_assemble(const slx_plan_desc* dp, const slx_numeric_desc* inp, int nt
         int ne = b.n_elem, ng = b.n_gp, nt = sd.n_tf, ndf = sd.is_rhs ? 1 : sd.n_dof;
         if (cached_bucket != frags[f].bucket) {
             std::vector<double> geo;
-            lagrange1(b.elem_type, b.gp_coords, ng, geo);
+            geometry(b, geo);
             cslot.assign(b.n_coefs, std::vector<double>((size_t)ne * ng));
             par(ne, [&](int64_t e0, int64_t e1) {
                 std::vector<double> cbuf(std::max(b.n_coefs, 1));

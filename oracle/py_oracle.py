@@ -32,8 +32,19 @@ def lib():
         L.oracle_free.argtypes = [vp]
         L.oracle_coefficients.restype = C.c_int
         L.oracle_coefficients.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_double)]
+        L.oracle_lagrange2.restype = C.c_int
+        L.oracle_lagrange2.argtypes = [C.c_int, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def lagrange2(elem_type, pts):
+    """The oracle's order-2 geometry functions: array [4, nodes, npts] (None: no such type)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(4 * 27 * len(pts), np.float64)
+    n = lib().oracle_lagrange2(elem_type, pts.ctypes.data_as(C.POINTER(C.c_double)), len(pts),
+                               out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out[:4 * n * len(pts)].reshape(4, n, len(pts)) if n else None
 
 
 def coefficients(desc_ptr, numeric, bucket, n):

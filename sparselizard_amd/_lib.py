@@ -138,6 +138,7 @@ def lib():
         "slx_selftest_mfma_layout_kind": (i32, [vp, i32, C.POINTER(C.c_float)]),
         "slx_h1_count": (i32, [i32, i32, p32]),
         "slx_h1_tables": (i32, [i32, i32, i32, i32, pd, pd]),
+        "slx_geometry_tables": (i32, [i32, i32, i32, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -154,7 +155,7 @@ EXPORTED = [
     "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_fused_info", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
-    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_geometry_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
     "slx_selftest_spmv_stream",
 ]
 
@@ -199,6 +200,14 @@ def h1_count(elem_type, order):
     n = C.c_int32()
     check(lib().slx_h1_count(elem_type, order, C.byref(n)))
     return n.value
+
+
+def geometry_tables(elem_type, n_geo, pts):
+    """Geometry functions and reference derivatives: array [4, n_geo, npts]."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros((4, n_geo, len(pts)), np.float64)
+    check(lib().slx_geometry_tables(elem_type, n_geo, len(pts), _p(pts, C.c_double), _p(out, C.c_double)))
+    return out
 
 
 def h1_tables(elem_type, order, pts, total_orientation=0):

@@ -68,11 +68,11 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool run
          "    const T adet = (T)fabs(g.det);\n"
          "    {\n"
          "#pragma clang fp contract(off)\n";
-    std::vector<std::string> cur(SLX_MAX_REGS, "(T)0");
+    std::vector<std::string> cur(SLX_MAX_PROG_REGS, "(T)0");
     for (int pc = 0; pc < n; pc++) {
         const slx_instr& x = prog[pc];
-        const std::string a = (x.a >= 0 && x.a < SLX_MAX_REGS) ? cur[x.a] : "(T)0";
-        const std::string bb = (x.b >= 0 && x.b < SLX_MAX_REGS) ? cur[x.b] : "(T)0";
+        const std::string a = (x.a >= 0 && x.a < SLX_MAX_PROG_REGS) ? cur[x.a] : "(T)0";
+        const std::string bb = (x.b >= 0 && x.b < SLX_MAX_PROG_REGS) ? cur[x.b] : "(T)0";
         const std::string v = "v" + std::to_string(pc);
         std::string rhs;
         switch (x.op) {
@@ -103,7 +103,7 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool run
             default: rhs = "apply_unary<T>(" + std::to_string(x.op) + ", " + a + ")"; break;
         }
         o << "        const T " << v << " = " << rhs << ";\n";
-        if (x.dst >= 0 && x.dst < SLX_MAX_REGS) cur[x.dst] = v;
+        if (x.dst >= 0 && x.dst < SLX_MAX_PROG_REGS) cur[x.dst] = v;
     }
     o << "    }\n    }\n}\n}  // namespace slx\n";
     return std::string(kJitPrelude) + o.str();

@@ -476,6 +476,87 @@ void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm,
     }
 }
 
+// Reference coordinates of the nodes of order-2 elements, gmsh node order
+// (lagrange*.cpp getnodecoordinates(2); the mesh files list nodes in this order).
+const std::vector<double>& lagrange2_nodes(int type) {
+    static const std::vector<double> line = {-1, 0, 0, 1, 0, 0, 0, 0, 0};
+    static const std::vector<double> tri = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0.5, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0};
+    static const std::vector<double> quad = {-1, -1, 0, 1, -1, 0, 1, 1, 0, -1, 1, 0, 0, -1, 0, 1, 0, 0, 0, 1, 0, -1, 0, 0, 0, 0, 0};
+    static const std::vector<double> tet = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1, 0.5, 0, 0, 0.5, 0.5, 0, 0, 0.5, 0, 0, 0, 0.5, 0, 0.5, 0.5, 0.5, 0, 0.5};
+    static const std::vector<double> hex = {-1, -1, -1, 1, -1, -1, 1, 1, -1, -1, 1, -1, -1, -1, 1, 1, -1, 1, 1, 1, 1, -1, 1, 1,
+                                            0,  -1, -1, -1, 0, -1, -1, -1, 0, 1, 0, -1, 1, -1, 0, 0, 1, -1, 1, 1, 0, -1, 1, 0,
+                                            0,  -1, 1, -1, 0, 1, 1, 0, 1, 0, 1, 1, 0, 0, -1, 0, -1, 0, -1, 0, 0, 1, 0, 0,
+                                            0,  1, 0, 0, 0, 1, 0, 0, 0};
+    static const std::vector<double> prism = {0, 0, -1, 1, 0, -1, 0, 1, -1, 0, 0, 1, 1, 0, 1, 0, 1, 1, 0.5, 0, -1, 0, 0.5, -1,
+                                              0, 0, 0, 0.5, 0.5, -1, 1, 0, 0, 0, 1, 0, 0.5, 0, 1, 0, 0.5, 1, 0.5, 0.5, 1,
+                                              0.5, 0, 0, 0, 0.5, 0, 0.5, 0.5, 0};
+    switch (type) {
+        case LINE: return line;
+        case TRI: return tri;
+        case QUAD: return quad;
+        case TET: return tet;
+        case HEX: return hex;
+        case PRISM: return prism;
+    }
+    fail("order-2 geometry is not provided for element type " + std::to_string(type));
+}
+
+int lagrange2_count(int type) { return (int)lagrange2_nodes(type).size() / 3; }
+
+namespace {
+// quadratic 1D Lagrange function of the node at c in {-1, 0, 1} on [-1, 1]
+D quad1(const D& x, double c) {
+    if (c < -0.5) return 0.5 * (x * (x - 1.0));
+    if (c > 0.5) return 0.5 * (x * (x + 1.0));
+    return 1.0 - x * x;
+}
+// quadratic simplex function of the node with barycentric coordinates b
+// (corner: lambda (2 lambda - 1); edge midpoint: 4 lambda_i lambda_j)
+D quad_simplex(const std::vector<D>& lam, const double* b, int n) {
+    int one = -1, h1 = -1, h2 = -1;
+    for (int i = 0; i < n; i++) {
+        if (std::fabs(b[i] - 1.0) < 1e-12) one = i;
+        else if (std::fabs(b[i] - 0.5) < 1e-12) (h1 < 0 ? h1 : h2) = i;
+    }
+    if (one >= 0) return lam[one] * (2.0 * lam[one] - 1.0);
+    SLX_CHECK(h1 >= 0 && h2 >= 0, "internal: order-2 simplex node is neither a corner nor an edge midpoint");
+    return 4.0 * (lam[h1] * lam[h2]);
+}
+}  // namespace
+
+void lagrange2_tables(int type, const std::vector<double>& pts, std::vector<double>& out) {
+    const std::vector<double>& c = lagrange2_nodes(type);
+    const int nn = (int)c.size() / 3, np = (int)pts.size() / 3;
+    out.assign((size_t)4 * nn * np, 0.0);
+    for (int p = 0; p < np; p++) {
+        const D ki = var(pts[3 * p], 0), eta = var(pts[3 * p + 1], 1), phi = var(pts[3 * p + 2], 2);
+        for (int n = 0; n < nn; n++) {
+            const double* cn = &c[3 * n];
+            D v(0.0);
+            if (type == LINE) {
+                v = quad1(ki, cn[0]);
+            } else if (type == QUAD) {
+                v = quad1(ki, cn[0]) * quad1(eta, cn[1]);
+            } else if (type == HEX) {
+                v = quad1(ki, cn[0]) * quad1(eta, cn[1]) * quad1(phi, cn[2]);
+            } else if (type == TRI || type == PRISM) {
+                const std::vector<D> lam = {1.0 - ki - eta, ki, eta};
+                const double b[3] = {1.0 - cn[0] - cn[1], cn[0], cn[1]};
+                v = quad_simplex(lam, b, 3);
+                if (type == PRISM) v = v * quad1(phi, cn[2]);
+            } else {  // TET
+                const std::vector<D> lam = {1.0 - ki - eta - phi, ki, eta, phi};
+                const double b[4] = {1.0 - cn[0] - cn[1] - cn[2], cn[0], cn[1], cn[2]};
+                v = quad_simplex(lam, b, 4);
+            }
+            out[((size_t)0 * nn + n) * np + p] = v.v;
+            out[((size_t)1 * nn + n) * np + p] = v.d[0];
+            out[((size_t)2 * nn + n) * np + p] = v.d[1];
+            out[((size_t)3 * nn + n) * np + p] = v.d[2];
+        }
+    }
+}
+
 void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out) {
     int nn = topo::count_nodes(type), np = (int)pts.size() / 3;
     out.assign((size_t)4 * nn * np, 0.0);
@@ -495,6 +576,13 @@ void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<doub
             out[((size_t)3 * nn + n) * np + p] = v.d[2];
         }
     }
+}
+
+void geometry_tables(int type, int n_geo, const std::vector<double>& pts, std::vector<double>& out) {
+    if (n_geo == topo::count_nodes(type)) return lagrange1_tables(type, pts, out);
+    SLX_CHECK(type != POINT && type != PYRAMID && n_geo == lagrange2_count(type),
+              "element type " << type << " with " << n_geo << " geometry nodes is not supported (order 1 or 2 geometry)");
+    lagrange2_tables(type, pts, out);
 }
 
 }  // namespace slx

@@ -70,5 +70,15 @@ void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm,
 // Order-1 Lagrange geometry functions (corner nodes) and their reference
 // derivatives: out[deriv][node][pt] (src/shapefunction/lagrange/lagrange*.cpp, order 1).
 void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out);
+// Order-2 (curved) geometry: node reference coordinates in gmsh order
+// (lagrange*.cpp getnodecoordinates(2)), node count (3 line, 6 triangle,
+// 9 quadrangle, 10 tetrahedron, 27 hexahedron, 18 prism) and the Lagrange
+// functions with their reference derivatives, out[deriv][node][pt].
+const std::vector<double>& lagrange2_nodes(int type);
+int lagrange2_count(int type);
+void lagrange2_tables(int type, const std::vector<double>& pts, std::vector<double>& out);
+// Geometry functions of an element type with n_geo nodes per element: order 1
+// (corner nodes) or order 2 (every node of a curved element).
+void geometry_tables(int type, int n_geo, const std::vector<double>& pts, std::vector<double>& out);
 
 }  // namespace slx

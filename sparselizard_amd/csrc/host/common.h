@@ -50,6 +50,7 @@ const std::vector<int>& face_edges(int type);
 bool is_triangular_face(int type, int facenum);
 // Gmsh element type → (type, curvature order) for the straight elements handled here.
 int from_gmsh(int gmshtype);
+int gmsh_order(int gmshtype);  // geometry (curvature) order of a gmsh element type: 1 or 2
 int to_gmsh(int type);
 
 }  // namespace topo

@@ -685,10 +685,10 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         int ne = (int)f0.elems.size();
         bd.elem_type = f0.type;
         bd.n_elem = ne;
-        bd.n_geo = topo::count_nodes(f0.type);
+        bd.n_geo = mesh_->geometry_nodes(f0.type);  // corners, or every node of a curved element
         ps->ints.emplace_back((size_t)ne * bd.n_geo);
         for (int e = 0; e < ne; e++)
-            for (int k = 0; k < bd.n_geo; k++) ps->ints.back()[(size_t)e * bd.n_geo + k] = mesh_->subelement(0, f0.type, f0.elems[e], k);
+            for (int k = 0; k < bd.n_geo; k++) ps->ints.back()[(size_t)e * bd.n_geo + k] = mesh_->geometry_node(f0.type, f0.elems[e], k);
         bd.geo_nodes = ps->ints.back().data();
         ps->ints.emplace_back(f0.elems.begin(), f0.elems.end());
         bd.elem_ids = ps->ints.back().data();

@@ -69,6 +69,31 @@ int Mesh::add_element(int type, const std::vector<int>& nodes, int physreg) {
     return num;
 }
 
+int Mesh::add_curved_element(int type, const std::vector<int>& nodes, int physreg) {
+    const int nn = topo::count_nodes(type);
+    SLX_CHECK(type != POINT && (int)nodes.size() > nn, "not a curved element");
+    SLX_CHECK(curv_n_[type] == 0 || curv_n_[type] == (int)nodes.size(), "elements of one type with different curvature");
+    SLX_CHECK(curv_[type].size() == (size_t)count(type) * curv_n_[type], "a type mixes straight and curved elements");
+    for (int n : nodes) SLX_CHECK(n >= 0 && n < count(0), "element node " << n << " out of range");
+    const int num = add_element(type, std::vector<int>(nodes.begin(), nodes.begin() + nn), physreg);
+    curv_n_[type] = (int)nodes.size();
+    curv_[type].insert(curv_[type].end(), nodes.begin(), nodes.end());
+    return num;
+}
+
+bool Mesh::curved() const {
+    for (int t = 1; t < 8; t++)
+        if (!curv_[t].empty()) return true;
+    return false;
+}
+
+int Mesh::geometry_nodes(int type) const { return curv_[type].empty() ? topo::count_nodes(type) : curv_n_[type]; }
+
+int Mesh::geometry_node(int type, int elem, int k) const {
+    if (curv_[type].empty()) return subelement(0, type, elem, k);
+    return curv_[type][(size_t)elem * curv_n_[type] + k];
+}
+
 void Mesh::add_to_physreg(int type, int elem, int physreg) {
     SLX_CHECK(!finalized_, "mesh already finalized");
     SLX_CHECK(elem >= 0 && elem < count(type), "element out of range");
@@ -90,6 +115,9 @@ void Mesh::renumber(int type, const std::vector<int>& renum) {
     if (type > 3) return;
     for (int t = 0; t <= 7; t++)
         for (int& v : sub_[t][type]) v = renum[v];
+    if (type == 0)
+        for (int t = 0; t <= 7; t++)
+            for (int& v : curv_[t]) v = renum[v];
 }
 
 std::vector<int> Mesh::remove_duplicates(int type) {
@@ -154,6 +182,15 @@ std::vector<int> Mesh::remove_duplicates(int type) {
         std::vector<int>& v = sub_[type][st];
         if (v.empty()) continue;
         int per = st == 0 ? nn : topo::count_type(type, st);
+        for (int i = 0; i < n; i++)
+            if (renum[i] != i)
+                for (int j = 0; j < per; j++) v[(size_t)renum[i] * per + j] = v[(size_t)i * per + j];
+        v.resize((size_t)nunique * per);
+    }
+    if (!curv_[type].empty()) {
+        // the geometry nodes follow the corner nodes (same rows)
+        std::vector<int>& v = curv_[type];
+        const int per = curv_n_[type];
         for (int i = 0; i < n; i++)
             if (renum[i] != i)
                 for (int j = 0; j < per; j++) v[(size_t)renum[i] * per + j] = v[(size_t)i * per + j];
@@ -253,6 +290,13 @@ void Mesh::reorder(int type, const std::vector<int>& ro) {
             for (int i = 0; i < n; i++)
                 for (int j = 0; j < per; j++) v[(size_t)i * per + j] = c[(size_t)ro[i] * per + j];
         }
+        if (!curv_[type].empty()) {
+            std::vector<int>& v = curv_[type];
+            const int per = curv_n_[type];
+            std::vector<int> c = v;
+            for (int i = 0; i < n; i++)
+                for (int j = 0; j < per; j++) v[(size_t)i * per + j] = c[(size_t)ro[i] * per + j];
+        }
     }
     std::vector<int> d = indisjreg_[type];
     for (int i = 0; i < n; i++) indisjreg_[type][i] = d[ro[i]];
@@ -323,7 +367,14 @@ void Mesh::finalize() {
     for (int t = 1; t <= 7; t++)
         if (count(t) > 0) maxt = t;
     meshdim_ = topo::dimension(maxt);
-    if (g_mesh_stream && phys_number_.size() <= 16) {
+    // curvature is kept for the elements of the mesh dimension only: integrals
+    // run over them, lower-dimensional elements only carry topology here
+    for (int t = 1; t <= 7; t++)
+        if (!curv_[t].empty() && topo::dimension(t) < meshdim_) {
+            curv_[t].clear();
+            curv_n_[t] = 0;
+        }
+    if (g_mesh_stream && phys_number_.size() <= 16 && !curved()) {
         // duplicate nodes first (they only rename node numbers, which commutes
         // with the explosion), then every other step on the device
         std::vector<int> renum = remove_duplicates(0);
@@ -425,7 +476,8 @@ Mesh Mesh::read_msh(const std::string& filename) {
             else is >> dummy;
         }
         int type = topo::from_gmsh(gtype);
-        std::vector<int> nodes(topo::count_nodes(type));
+        const int order = topo::gmsh_order(gtype);
+        std::vector<int> nodes(order == 2 ? lagrange2_count(type) : topo::count_nodes(type));
         for (int& n : nodes) {
             is >> n;
             n -= 1;
@@ -435,7 +487,7 @@ Mesh Mesh::read_msh(const std::string& filename) {
             m.add_to_physreg(type, previdx, phys);
             continue;
         }
-        previdx = m.add_element(type, nodes, phys);
+        previdx = order == 2 ? m.add_curved_element(type, nodes, phys) : m.add_element(type, nodes, phys);
         prev = nodes;
         prevtype = type;
     }

@@ -36,6 +36,10 @@ class Mesh {
     // elements::add + physicalregion::addelement for one element read from a
     // file (gmshinterface.cpp:239-288): returns the element number.
     int add_element(int type, const std::vector<int>& nodes, int physreg);
+    // An element of a curved (order-2) mesh: every node in gmsh order, the
+    // corner nodes first.  The corners define the topology; every node
+    // defines the geometry (rawmesh curvature, lagrange*.cpp order 2).
+    int add_curved_element(int type, const std::vector<int>& nodes, int physreg);
     // A repeated element line of a file: the previous element also joins
     // another physical region (gmshinterface.cpp:274-281).
     void add_to_physreg(int type, int elem, int physreg);
@@ -54,6 +58,11 @@ class Mesh {
     const std::vector<int>& subelements(int type, int subtype) const { return sub_[type][subtype]; }
     int disjreg_of(int type, int elem) const { return indisjreg_[type][elem]; }
     int total_orientation(int type, int elem) const { return type == 0 ? 0 : totalorient_[type][elem]; }
+    // Geometry nodes of the elements of a type: every node of a curved element
+    // (elements of the mesh dimension read with order-2 geometry), else the corners.
+    int geometry_nodes(int type) const;
+    int geometry_node(int type, int elem, int k) const;
+    bool curved() const;
 
     int count_disjregs() const { return (int)dr_type_.size(); }
     int disjreg_type(int dr) const { return dr_type_[dr]; }
@@ -86,6 +95,8 @@ class Mesh {
     bool finalized_ = false;
     std::vector<double> coords_;
     std::vector<int> sub_[8][4];
+    std::vector<int> curv_[8];   // every node of the curved elements of a type (gmsh order)
+    int curv_n_[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<int> indisjreg_[8];
     std::vector<int> totalorient_[8];
     // physical regions in creation order

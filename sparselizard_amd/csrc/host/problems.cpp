@@ -562,6 +562,41 @@ std::shared_ptr<Problem> from_msh(const std::map<std::string, std::string>& s) {
     return p;
 }
 
+// examples/elasticity-membrane-3d/main.cpp:10-40 on its mesh file (a disk of
+// 27-node curved hexahedra): h1xyz displacement of order 3 on volume 1,
+// clamped on surface 2, E = 150e9, nu = 0.3, volume force (0, 0, -10).
+std::shared_ptr<Problem> membrane3d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    auto it = s.find("file");
+    SLX_CHECK(it != s.end(), "membrane3d needs file=<path>");
+    p->mesh = std::make_shared<Mesh>(Mesh::read_msh(it->second));
+    Mesh& m = *p->mesh;
+    const int vol = 1, sur = 2;
+    field u(m, "h1xyz", "u");
+    u.setorder(vol, geti(s, "order", 3));
+    u.setconstraint(sur);
+    p->fields = {u};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    const expression E(getd(s, "E", 150e9)), nu(getd(s, "nu", 0.3));
+    *p->form += integral(vol, predefinedelasticity(dof(u), tf(u), E, nu));
+    if (geti(s, "measure", 0)) {
+        // the validation integral compz(u).integrate(vol, u, 5) (main.cpp:50) as a
+        // vector: rhs[i] = integral of the z component of form function i at
+        // integration order 5 (delta -3 from the default 3 + 3 + 2), so that the
+        // integral of u_z is rhs . x; the same contributions' fields as the
+        // problem itself, hence the same DOF numbering
+        expression ez(3, 1, {expression(0.0), expression(0.0), expression(-1.0)});
+        *p->form += integral(vol, ez * tf(u), -3);
+        p->describe = "elasticity membrane 3D: integral of u_z at order 5";
+        return p;
+    }
+    expression f(3, 1, {expression(0.0), expression(0.0), expression(-10.0)});
+    *p->form += integral(vol, f * tf(u));
+    p->describe = "elasticity membrane 3D (curved hexahedra), mesh file " + it->second;
+    return p;
+}
+
 // examples/electrostatics-1d/main.cpp:11-36 on its mesh file: P1 potential
 // on line region 1, 10 V on point region 2 and 2 V on point region 3,
 // integral(line, -epsilon grad(dof(v)) grad(tf(v))).
@@ -623,6 +658,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "prism3d") p = prism3d(s);
     else if (kind == "msh") p = from_msh(s);
     else if (kind == "electrostatics1d") p = electrostatics1d(s);
+    else if (kind == "membrane3d") p = membrane3d(s);
     else if (kind == "newton2d") p = newton2d(s);
     else if (kind == "hoprojection2d") p = highorderprojection2d(s);
     else if (kind == "convdiff2d") p = convdiff2d(s);

@@ -102,16 +102,18 @@ bool is_triangular_face(int type, int facenum) {
 int from_gmsh(int g) {
     switch (g) {
         case 15: return POINT;
-        case 1: return LINE;
-        case 2: return TRI;
-        case 3: return QUAD;
-        case 4: return TET;
-        case 5: return HEX;
-        case 6: return PRISM;
+        case 1: case 8: return LINE;
+        case 2: case 9: return TRI;
+        case 3: case 10: return QUAD;
+        case 4: case 11: return TET;
+        case 5: case 12: return HEX;
+        case 6: case 13: return PRISM;
         case 7: return PYRAMID;
     }
-    fail("unsupported gmsh element type " + std::to_string(g) + " (only straight, order-1 geometry)");
+    fail("unsupported gmsh element type " + std::to_string(g) + " (order-1 geometry, or order 2 without pyramids)");
 }
+
+int gmsh_order(int g) { return (g >= 8 && g <= 13) ? 2 : 1; }
 
 int to_gmsh(int type) {
     static const int g[8] = {15, 1, 2, 3, 4, 5, 6, 7};

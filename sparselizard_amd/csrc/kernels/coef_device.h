@@ -31,11 +31,22 @@ __device__ inline double pick9(const double* v, int i) {
 
 // Corner-node coordinates of one element, loaded once and reused for every
 // Gauss point the thread evaluates.
+// Curved elements (more than SLX_MAX_GEO geometry nodes) keep only the
+// element's node list and read the coordinates per Gauss point.
 struct GeoX {
     double x[SLX_MAX_GEO][3];
+    const int* nodes;
+    const double* xyz;
 };
 
 __device__ inline void geo_load(const DevBucket& b, const double* xyz, int e, GeoX& gx) {
+    gx.nodes = b.geo_nodes + (size_t)e * b.n_geo;
+    gx.xyz = xyz;
+    if (b.n_geo > SLX_MAX_GEO) {
+#pragma unroll
+        for (int k = 0; k < SLX_MAX_GEO; k++) gx.x[k][0] = gx.x[k][1] = gx.x[k][2] = 0.0;
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < SLX_MAX_GEO; k++) {
         if (k < b.n_geo) {
@@ -53,18 +64,35 @@ __device__ inline void geo_load(const DevBucket& b, const double* xyz, int e, Ge
 // as jacobian.cpp:308-349, 396-505.
 __device__ inline void geo_eval(const DevBucket& b, const GeoX& gx, int q, Geo& g) {
     const auto& x = gx.x;
+    if (b.n_geo > SLX_MAX_GEO) {
+        // curved: J[r][c] summed node by node (same order over k as below)
 #pragma unroll
-    for (int r = 0; r < 3; r++) {
-        double dn[SLX_MAX_GEO];
+        for (int i = 0; i < 9; i++) g.J[i] = 0.0;
+        for (int k = 0; k < b.n_geo; k++) {
+            const int node = gx.nodes[k];
+            const double xc[3] = {gx.xyz[3 * (size_t)node], gx.xyz[3 * (size_t)node + 1], gx.xyz[3 * (size_t)node + 2]};
 #pragma unroll
-        for (int k = 0; k < SLX_MAX_GEO; k++)
-            dn[k] = (r < b.elemdim && k < b.n_geo) ? b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q] : 0.0;
+            for (int r = 0; r < 3; r++) {
+                const double dn = r < b.elemdim ? b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q] : 0.0;
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            double s = 0.0;
+                for (int c = 0; c < 3; c++)
+                    if (c < b.meshdim) g.J[3 * r + c] += xc[c] * dn;
+            }
+        }
+    } else {
 #pragma unroll
-            for (int k = 0; k < SLX_MAX_GEO; k++) s += x[k][c] * dn[k];
-            g.J[3 * r + c] = (c < b.meshdim) ? s : 0.0;
+        for (int r = 0; r < 3; r++) {
+            double dn[SLX_MAX_GEO];
+#pragma unroll
+            for (int k = 0; k < SLX_MAX_GEO; k++)
+                dn[k] = (r < b.elemdim && k < b.n_geo) ? b.geo_tab[((size_t)(1 + r) * b.n_geo + k) * b.n_gp + q] : 0.0;
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                double s = 0.0;
+#pragma unroll
+                for (int k = 0; k < SLX_MAX_GEO; k++) s += x[k][c] * dn[k];
+                g.J[3 * r + c] = (c < b.meshdim) ? s : 0.0;
+            }
         }
     }
     const double* J = g.J;

@@ -5,8 +5,10 @@
 
 #define SLX_MAX_SLICES 12
 #define SLX_MAX_FIELDS 4
-#define SLX_MAX_REGS 64
-#define SLX_MAX_GEO 8
+#define SLX_MAX_REGS 64         // registers of the coefficient interpreter (larger programs run compiled)
+#define SLX_MAX_PROG_REGS 256   // registers of a coefficient program
+#define SLX_MAX_GEO 8          // geometry nodes held in registers (order-1 geometry)
+#define SLX_MAX_GEO_CURVED 27  // order-2 geometry (27-node hexahedra): streamed from memory
 
 namespace slx {
 

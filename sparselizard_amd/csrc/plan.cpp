@@ -1420,8 +1420,8 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         }
         std::memset(&db, 0, sizeof(db));
         SLX_CHECK(bd.n_slices <= SLX_MAX_SLICES, "bucket has " << bd.n_slices << " slices (max " << SLX_MAX_SLICES << ")");
-        SLX_CHECK(bd.n_regs <= SLX_MAX_REGS, "coefficient program needs " << bd.n_regs << " registers (max " << SLX_MAX_REGS << ")");
-        SLX_CHECK(bd.n_geo <= SLX_MAX_GEO, "too many geometry nodes");
+        SLX_CHECK(bd.n_regs <= SLX_MAX_PROG_REGS, "coefficient program needs " << bd.n_regs << " registers (max " << SLX_MAX_PROG_REGS << ")");
+        SLX_CHECK(bd.n_geo <= SLX_MAX_GEO_CURVED, "too many geometry nodes");
         SLX_CHECK(d.n_fields <= SLX_MAX_FIELDS, "too many fields in coefficient programs");
         int edim = bd.elem_type == 1 ? 1 : (bd.elem_type <= 3 ? 2 : 3);
         SLX_CHECK(edim == d.mesh_dim, "only integrals over elements of the mesh dimension are provided");
@@ -1445,7 +1445,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         }
         db.gp_w = P.dev(bd.gp_weights, (size_t)bd.n_gp);
         std::vector<double> pts(bd.gp_coords, bd.gp_coords + 3 * bd.n_gp), geo;
-        lagrange1_tables(bd.elem_type, pts, geo);
+        geometry_tables(bd.elem_type, bd.n_geo, pts, geo);  // order 1, or order 2 (curved)
         db.geo_tab = P.dev(geo.data(), geo.size());
         db.prog = P.dev(bd.prog, (size_t)bd.n_instr);
         std::vector<const double*> dtabs(bd.n_tables);
@@ -1496,7 +1496,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         // program sanity: registers and opcodes in range
         for (int i = 0; i < bd.n_instr; i++) {
             const slx_instr& in = bd.prog[i];
-            SLX_CHECK(in.a >= 0 && in.b >= 0 && in.a < SLX_MAX_REGS && in.b < SLX_MAX_REGS, "bad operand register");
+            SLX_CHECK(in.a >= 0 && in.b >= 0 && in.a < SLX_MAX_PROG_REGS && in.b < SLX_MAX_PROG_REGS, "bad operand register");
             if (in.op == SLX_OP_PARAM) {
                 SLX_CHECK(in.imm >= 0, "bad parameter index");
                 P.max_param = std::max(P.max_param, in.imm);
@@ -1538,7 +1538,8 @@ void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d) {
     std::vector<std::pair<int, int>> todo;  // (bucket, fp64)
     for (int b = 0; b < d.n_buckets; b++) {
         if (d.buckets[b].n_coefs == 0 || d.buckets[b].n_elem == 0) continue;
-        if (P.coef_mode == 0 && d.buckets[b].n_elem < SLX_COEF_JIT_MIN) continue;
+        // programs beyond the interpreter's register file are always compiled
+        if (P.coef_mode == 0 && d.buckets[b].n_elem < SLX_COEF_JIT_MIN && d.buckets[b].n_regs <= SLX_MAX_REGS) continue;
         todo.push_back({b, 0});
         todo.push_back({b, 1});
     }
@@ -1650,7 +1651,11 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
             if (tm) tm->begin(T_COEF);
             const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
             if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, 1, ctab, st));
-            else HIPCHECK(launch_coef(b, in, ke64, ctab, st, 1));
+            else {
+                SLX_CHECK(b.n_regs <= SLX_MAX_REGS, "coefficient program of " << b.n_regs << " registers has no compiled kernel and the interpreter holds "
+                                                   << SLX_MAX_REGS << " (" << P.coef_jit_error << ")");
+                HIPCHECK(launch_coef(b, in, ke64, ctab, st, 1));
+            }
             if (tm) tm->end();
         }
         if (tm) tm->begin(T_MFMA);
@@ -1674,7 +1679,11 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
             if (tm) tm->begin(T_COEF);
             const hipFunction_t cfn = bi < P.coef_fn.size() ? P.coef_fn[bi][ke64 ? 1 : 0] : nullptr;
             if (cfn) HIPCHECK(coef_launch(cfn, b, in, b.q_inv ? 1 : b.n_gp, qfast, ctab, bst));
-            else HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
+            else {
+                SLX_CHECK(b.n_regs <= SLX_MAX_REGS, "coefficient program of " << b.n_regs << " registers has no compiled kernel and the interpreter holds "
+                                                   << SLX_MAX_REGS << " (" << P.coef_jit_error << ")");
+                HIPCHECK(launch_coef(b, in, ke64, ctab, bst, qfast));
+            }
             if (tm) tm->end();
         }
         if (arith >= 0 && P.mfma[bi].M > 0 && ((P.mfma[bi].arith_ok >> arith) & 1u)) {
@@ -2332,6 +2341,9 @@ slx_status slx_plan_coef_mode(slx_plan p, int32_t mode, int32_t* n_compiled) {
         SLX_CHECK(p, "null plan");
         SLX_CHECK(mode >= 0 && mode <= 2, "coefficient mode must be 0 (auto), 1 (interpreter) or 2 (compiled)");
         SLX_CHECK(!p->host_only, "plan was created without a device context");
+        for (int b = 0; mode == 1 && b < p->src->n_buckets; b++)
+            SLX_CHECK(p->src->buckets[b].n_regs <= SLX_MAX_REGS, "coefficient program of bucket " << b << " needs " << p->src->buckets[b].n_regs
+                                                              << " registers; the interpreter holds " << SLX_MAX_REGS);
         HIPCHECK(hipSetDevice(p->ctx->device));
         if (mode != p->coef_mode) {
             HIPCHECK(hipStreamSynchronize(p->stream));
@@ -2446,6 +2458,15 @@ slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orienta
         SLX_CHECK(pts && out && npts >= 0, "bad argument");
         std::vector<double> p(pts, pts + 3 * (size_t)npts), o;
         h1_tables(elem_type, order, total_orientation, p, o);
+        std::copy(o.begin(), o.end(), out);
+    });
+}
+
+slx_status slx_geometry_tables(int32_t elem_type, int32_t n_geo, int32_t npts, const double* pts, double* out) {
+    return guarded([&] {
+        SLX_CHECK(pts && out && npts >= 0 && elem_type >= 1 && elem_type <= 7, "bad argument");
+        std::vector<double> p(pts, pts + 3 * (size_t)npts), o;
+        geometry_tables(elem_type, n_geo, p, o);
         std::copy(o.begin(), o.end(), out);
     });
 }
