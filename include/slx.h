@@ -363,6 +363,15 @@ slx_status slx_h1_count(int32_t elem_type, int32_t order, int32_t* nff);
  * for all four derivative slots: out[deriv][ff][pt], deriv 0 value, 1..3 d/dki,eta,phi. */
 slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t npts, const double* pts,
                          double* out);
+/* Number of H(curl) form functions of order <= `order` (order >= 0) on a
+ * line, triangle, quadrangle, tetrahedron or hexahedron
+ * (hcurl*::count, e.g. src/shapefunction/hierarchical/hcurl/hcurltriangle.cpp:6-14). */
+slx_status slx_hcurl_count(int32_t elem_type, int32_t order, int32_t* nff);
+/* H(curl) tables (hcurl*::evalat + tomatrix per component): family 1 the
+ * function values, family 2 their curls, both in reference coordinates:
+ * out[slot][ff][pt], slot 0 zero, slots 1..3 the ki, eta, phi components. */
+slx_status slx_hcurl_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t family, int32_t npts,
+                            const double* pts, double* out);
 /* Geometry (Lagrange) functions of an element type with n_geo nodes per
  * element: the corner nodes (order 1) or every node of a curved order-2
  * element in gmsh order (3 line, 6 triangle, 9 quadrangle, 10 tetrahedron,

@@ -139,6 +139,8 @@ def lib():
         "slx_h1_count": (i32, [i32, i32, p32]),
         "slx_h1_tables": (i32, [i32, i32, i32, i32, pd, pd]),
         "slx_geometry_tables": (i32, [i32, i32, i32, pd, pd]),
+        "slx_hcurl_count": (i32, [i32, i32, p32]),
+        "slx_hcurl_tables": (i32, [i32, i32, i32, i32, i32, pd, pd]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -155,7 +157,7 @@ EXPORTED = [
     "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_fused_info", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
-    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_geometry_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
+    "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_geometry_tables", "slx_hcurl_count", "slx_hcurl_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
     "slx_selftest_spmv_stream",
 ]
 
@@ -216,6 +218,26 @@ def h1_tables(elem_type, order, pts, total_orientation=0):
     nff = h1_count(elem_type, order)
     out = np.zeros((4, nff, len(pts)), np.float64)
     check(lib().slx_h1_tables(elem_type, order, total_orientation, len(pts), _p(pts, C.c_double), _p(out, C.c_double)))
+    return out
+
+
+def hcurl_count(elem_type, order):
+    n = C.c_int32()
+    check(lib().slx_hcurl_count(elem_type, order, C.byref(n)))
+    return n.value
+
+
+HCURL_VALUE, HCURL_CURL = 1, 2
+
+
+def hcurl_tables(elem_type, order, pts, family=HCURL_VALUE, total_orientation=0):
+    """H(curl) values (family 1) or curls (family 2) in reference coordinates:
+    array [4, nff, npts], slot 0 zero, slots 1..3 the ki, eta, phi components."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+    nff = hcurl_count(elem_type, order)
+    out = np.zeros((4, nff, len(pts)), np.float64)
+    check(lib().slx_hcurl_tables(elem_type, order, total_orientation, family, len(pts), _p(pts, C.c_double),
+                                 _p(out, C.c_double)))
     return out
 
 

@@ -67,6 +67,24 @@ H1Fold h1_fold(int type, int order, const std::vector<double>& pts);
 // perm[f], sign[f] of every function for the given total orientation.
 void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm, std::vector<int8_t>& sign);
 
+// Hierarchical H(curl) functions (hcurl.cpp; src/shapefunction/hierarchical/
+// hcurl/hcurl{line,triangle,quadrangle,tetrahedron,hexahedron}.cpp) for lines,
+// triangles, quadrangles, tetrahedra and hexahedra, orders >= 0.
+// Count on the whole element (hcurl*::count(order)) and on one sub-entity
+// (hcurl*::count(order, dim, num)).
+int hcurl_count(int type, int order);
+int hcurl_count(int type, int order, int dim, int num);
+// Form functions in the reference iterator order (edge, face, volume; inside
+// an entity by increasing order).
+std::vector<FormFunction> hcurl_iterate(int type, int order);
+// hcurl functions are orientation dependent at every order
+// (hierarchicalformfunction::isorientationdependent default).
+enum { HCURL_VALUE = 1, HCURL_CURL = 2 };
+// out[slot][ff][pt], slot 0 unused, slots 1..3 the reference components of
+// the function value (HCURL_VALUE) or of its reference curl (HCURL_CURL).
+void hcurl_tables(int type, int order, int totalorientation, int family, const std::vector<double>& pts,
+                  std::vector<double>& out);
+
 // Order-1 Lagrange geometry functions (corner nodes) and their reference
 // derivatives: out[deriv][node][pt] (src/shapefunction/lagrange/lagrange*.cpp, order 1).
 void lagrange1_tables(int type, const std::vector<double>& pts, std::vector<double>& out);

@@ -217,8 +217,10 @@ DofTfSlices extract_dof_tf(OpP root, int elementdimension) {
             const OpP& t0 = out.tfs[s][0];
             auto dfield = [](const OpP& o) { return o ? o->field.get() : nullptr; };
             auto dphys = [](const OpP& o) { return o ? o->physreg : -1; };
+            auto dfam = [](const OpP& o) { return o ? o->hc : 0; };
+            // hcurl values and curls use different tables: separate slices
             if (dfield(d0) == dfield(curdof) && t0->field.get() == curtf->field.get() && dphys(d0) == dphys(curdof) &&
-                t0->physreg == curtf->physreg) {
+                t0->physreg == curtf->physreg && dfam(d0) == dfam(curdof) && t0->hc == curtf->hc) {
                 slice = (int)s;
                 break;
             }
@@ -311,6 +313,7 @@ expression expression::getcopy() const { return *this; }
 
 expression expression::transpose() const {
     expression o = *this;
+    o.hc_field_.reset();
     o.rows_ = cols_;
     o.cols_ = rows_;
     for (int r = 0; r < rows_; r++)
@@ -325,6 +328,7 @@ void expression::reuseit(bool flag) {
 expression operator+(const expression& a, const expression& b) {
     SLX_CHECK(a.rows_ == b.rows_ && a.cols_ == b.cols_, "trying to add a " << a.rows_ << "x" << a.cols_ << " expression to a " << b.rows_ << "x" << b.cols_);
     expression o = a;
+    o.hc_field_.reset();
     for (size_t i = 0; i < o.ops_.size(); i++) o.ops_[i] = make_op(OpKind::Sum, {a.ops_[i], b.ops_[i]});
     return o;
 }
@@ -333,11 +337,13 @@ expression operator*(const expression& a, const expression& b) {
     // expression.cpp:2399-2459
     if (a.isscalar()) {
         expression o = b;
+        o.hc_field_.reset();
         for (size_t i = 0; i < o.ops_.size(); i++) o.ops_[i] = make_op(OpKind::Product, {a.ops_[0], b.ops_[i]});
         return o;
     }
     if (b.isscalar()) {
         expression o = a;
+        o.hc_field_.reset();
         for (size_t i = 0; i < o.ops_.size(); i++) o.ops_[i] = make_op(OpKind::Product, {a.ops_[i], b.ops_[0]});
         return o;
     }
@@ -366,12 +372,14 @@ expression operator-(const expression& a, const expression& b) { return a + b * 
 expression operator/(const expression& a, const expression& b) {
     SLX_CHECK(b.isscalar(), "only divisions by scalars are allowed");
     expression o = a;
+    o.hc_field_.reset();
     for (size_t i = 0; i < o.ops_.size(); i++) o.ops_[i] = make_op(OpKind::Product, {a.ops_[i], make_op(OpKind::Inversion, {b.ops_[0]})});
     return o;
 }
 
 expression expression::spacederivative(int which) const {
     SLX_CHECK(rows_ <= 3 && cols_ == 1, "can only take space derivatives of column vectors with up to 3 components");
+    SLX_CHECK(!hc_field_, "space derivatives of hcurl fields are not supported (use curl)");
     expression o = *this;
     for (auto& op : o.ops_) {
         if (op->isconstant()) {
@@ -391,7 +399,29 @@ expression expression::spacederivative(int which) const {
     return o;
 }
 
+// Copy of an op tree with every hcurl leaf turned into a Dof / Tf leaf.
+static OpP retag_hcurl(const OpP& op, OpKind k, int physreg) {
+    if (op->kind == OpKind::Field && op->hc) {
+        OpP n = op->copy();
+        n->kind = k;
+        n->physreg = physreg;
+        return n;
+    }
+    if (op->args.empty()) return op;
+    OpP n = op->copy();
+    for (auto& a : n->args) a = retag_hcurl(a, k, physreg);
+    return n;
+}
+
 static expression tag(const expression& in, OpKind k, int physreg) {
+    if (in.hcurl_field()) {
+        SLX_CHECK(in.hcurl_kind() == OpKind::Field, "dof()/tf() of a dof()/tf() expression");
+        expression o = in;
+        for (int r = 0; r < in.countrows(); r++)
+            for (int c = 0; c < in.countcolumns(); c++) o.op(r, c) = retag_hcurl(in.op(r, c), k, physreg);
+        o.set_hcurl(in.hcurl_field(), k, physreg);
+        return o;
+    }
     expression o = in;
     for (int r = 0; r < in.countrows(); r++)
         for (int c = 0; c < in.countcolumns(); c++) {

@@ -33,6 +33,9 @@ struct Op : std::enable_shared_from_this<Op> {
     std::shared_ptr<RawParameter> param; // Parameter (scalar)
     int spacederiv = 0;                  // 0 none, 1 d/dx, 2 d/dy, 3 d/dz
     int kietaphideriv = 0;               // 0 none, 1 d/dki, 2 d/deta, 3 d/dphi
+                                         // (hcurl ops: the reference component 1..3)
+    int hc = 0;                          // Dof / Tf / Field of an hcurl field: 1 the
+                                         // function value, 2 its reference curl
     int physreg = -1;                    // dof(u, physreg) / tf(u, physreg)
     int row = 0, col = 0;                // Jac / InvJac entry, Coord axis
     bool reuse = false;
@@ -92,9 +95,25 @@ class expression {
     expression dof(int physreg) const;
     expression tf(int physreg) const;
 
+    // An hcurl field, or its dof() / tf(), keeps the field it maps from the
+    // reference element (expression::inrefcoord, expression.cpp:30-35): curl()
+    // is then taken in reference coordinates (sl.cpp:1184-1222).  Every other
+    // operation returns a plain expression.
+    const std::shared_ptr<RawField>& hcurl_field() const { return hc_field_; }
+    OpKind hcurl_kind() const { return hc_kind_; }
+    int hcurl_physreg() const { return hc_physreg_; }
+    void set_hcurl(const std::shared_ptr<RawField>& f, OpKind k, int physreg) {
+        hc_field_ = f;
+        hc_kind_ = k;
+        hc_physreg_ = physreg;
+    }
+
    private:
     int rows_, cols_;
     std::vector<OpP> ops_;
+    std::shared_ptr<RawField> hc_field_;
+    OpKind hc_kind_ = OpKind::Field;
+    int hc_physreg_ = -1;
 };
 
 }  // namespace slx

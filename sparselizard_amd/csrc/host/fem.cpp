@@ -15,6 +15,17 @@ namespace slx {
 
 // ---------------------------------------------------------------- fields ----
 
+// Form functions of field f carried by one sub-entity of type t (its own
+// dimension) at the given order, and on a whole element of type t.
+static int entity_nff(const RawField& f, int t, int order) {
+    if (f.hcurl()) return t == POINT ? 0 : hcurl_count(t, order, topo::dimension(t), 0);
+    return h1_count(t, order, topo::dimension(t));
+}
+static int element_nff(const RawField& f, int t, int order) { return f.hcurl() ? hcurl_count(t, order) : h1_count(t, order); }
+static std::vector<FormFunction> element_ffs(const RawField& f, int t, int order) {
+    return f.hcurl() ? hcurl_iterate(t, order) : h1_iterate(t, order);
+}
+
 RawField::RawField(const Mesh* m, const std::string& type, const std::string& name) : mesh_(m), type_(type), name_(name) {
     orders_.assign(m->count_disjregs(), 1);
     constrained_.assign(m->count_disjregs(), 0);
@@ -22,7 +33,8 @@ RawField::RawField(const Mesh* m, const std::string& type, const std::string& na
 }
 
 void RawField::setorder(int physreg, int order) {
-    SLX_CHECK(order >= 1, "H1 fields need an interpolation order of at least 1");
+    // hierarchicalformfunction::getminorder: 0 for hcurl, 1 for h1
+    SLX_CHECK(order >= (hcurl() ? 0 : 1), "cannot use interpolation order " << order << " for shape function " << type_);
     for (auto& s : subfields_) s->setorder(physreg, order);
     if (!subfields_.empty()) return;
     int regdim = mesh_->physreg_dim(physreg);
@@ -32,6 +44,7 @@ void RawField::setorder(int physreg, int order) {
 }
 
 void RawField::setconstraint(int physreg, double value) {
+    SLX_CHECK(!hcurl() || value == 0.0, "an hcurl field is constrained with a 3x1 expression (only 0 is provided)");
     for (auto& s : subfields_) s->setconstraint(physreg, value);
     if (!subfields_.empty()) return;
     for (int dr : mesh_->physreg_disjregs(physreg, -1)) {
@@ -48,8 +61,7 @@ void RawField::build_data_numbering() {
     data_rangebegin.assign(nd, {});
     int64_t n = 0;
     for (int dr = 0; dr < nd; dr++) {
-        int t = mesh_->disjreg_type(dr);
-        int nff = h1_count(t, orders_[dr], topo::dimension(t));
+        int nff = entity_nff(*this, mesh_->disjreg_type(dr), orders_[dr]);
         for (int ff = 0; ff < nff; ff++) {
             data_rangebegin[dr].push_back((int)n);
             n += mesh_->disjreg_count(dr);
@@ -64,7 +76,8 @@ field::field(const Mesh& m, const std::string& type, const std::string& name) {
     std::string base = type;
     if (type.size() > 2 && type.compare(type.size() - 3, 3, "xyz") == 0) { ncomp = 3; base = type.substr(0, type.size() - 3); }
     else if (type.size() > 2 && type.compare(type.size() - 2, 2, "xy") == 0) { ncomp = 2; base = type.substr(0, type.size() - 2); }
-    SLX_CHECK(base == "h1", "only h1 fields are provided (got '" << type << "')");
+    SLX_CHECK(base == "h1" || (base == "hcurl" && ncomp == 1),
+              "field types are h1, h1xy, h1xyz and hcurl (got '" << type << "')");
     raw_ = std::make_shared<RawField>(&m, base, name);
     raw_->setself(raw_);
     if (ncomp > 1) {
@@ -88,6 +101,37 @@ static OpP field_op(const std::shared_ptr<RawField>& f) {
 }
 
 field::operator expression() const {
+    if (raw_->hcurl()) {
+        // reference components v_r mapped by J^-T (expression.cpp:30-35 with
+        // expression::invjac, expression.cpp:2031-2050): row k = sum_r
+        // invjac(k, r) v_r over the mesh dimension; the third row of a 2D
+        // mesh multiplies the (identically zero) third reference component
+        const int md = raw_->meshdim();
+        SLX_CHECK(md >= 2, "hcurl fields need a 2D or 3D mesh");
+        std::vector<expression> rows;
+        for (int k = 0; k < 3; k++) {
+            if (k >= md) {
+                rows.push_back(expression(0.0));
+                continue;
+            }
+            expression e(0.0);
+            for (int r = 0; r < md; r++) {
+                OpP v = field_op(raw_);
+                v->hc = 1;
+                v->kietaphideriv = r + 1;
+                auto ij = std::make_shared<Op>();
+                ij->kind = OpKind::InvJac;
+                ij->row = k;
+                ij->col = r;
+                expression t = expression(ij) * expression(v);
+                e = r == 0 ? t : e + t;
+            }
+            rows.push_back(e);
+        }
+        expression out(3, 1, rows);
+        out.set_hcurl(raw_, OpKind::Field, -1);
+        return out;
+    }
     int n = raw_->ncomponents();
     if (n == 1) return expression(field_op(raw_));
     std::vector<expression> e;
@@ -95,7 +139,10 @@ field::operator expression() const {
     return expression(n, 1, e);
 }
 
-expression field::comp(int i) const { return expression(field_op(raw_->component(i))); }
+expression field::comp(int i) const {
+    if (raw_->hcurl()) return expression(*this).at(i, 0);
+    return expression(field_op(raw_->component(i)));
+}
 
 parameter::operator expression() const {
     auto o = std::make_shared<Op>();
@@ -130,6 +177,52 @@ expression grad(const expression& in) {
     expression out(in.countrows(), pd, ex);
     if (in.countrows() == 1) out = out.transpose();
     return out;
+}
+
+expression curl(const expression& in) {
+    if (const std::shared_ptr<RawField>& f = in.hcurl_field()) {
+        // curl in the reference element, brought back with J^T / det J
+        // (sl.cpp:1215-1222); a 2D element's curl is along z only
+        const int md = f->meshdim();
+        auto leaf = [&](int r) {
+            auto o = std::make_shared<Op>();
+            o->kind = in.hcurl_kind();
+            o->field = f;
+            o->physreg = in.hcurl_physreg();
+            o->hc = 2;
+            o->kietaphideriv = r + 1;
+            return expression(o);
+        };
+        auto detj = std::make_shared<Op>();
+        detj->kind = OpKind::DetJac;
+        const expression invdet = expression(1.0) / expression(detj);
+        std::vector<expression> rows;
+        for (int k = 0; k < 3; k++) {
+            if (md == 2) {
+                rows.push_back(k == 2 ? leaf(2) * invdet : expression(0.0));
+                continue;
+            }
+            expression e(0.0);
+            for (int r = 0; r < 3; r++) {
+                auto j = std::make_shared<Op>();
+                j->kind = OpKind::Jac;
+                j->row = r;
+                j->col = k;
+                expression t = expression(j) * leaf(r);
+                e = r == 0 ? t : e + t;
+            }
+            rows.push_back(e * invdet);
+        }
+        return expression(3, 1, rows);
+    }
+    SLX_CHECK(in.countcolumns() == 1 && in.countrows() <= 3, "can only take the curl of an up to length 3 column vector");
+    // sl.cpp:1203-1211 (not axisymmetric)
+    const expression zero(0.0);
+    switch (in.countrows()) {
+        case 1: return expression(3, 1, {zero, zero, zero});
+        case 2: return expression(3, 1, {zero, zero, dx(in).at(1, 0) - dy(in).at(0, 0)});
+    }
+    return expression(3, 1, {dy(in).at(2, 0) - dz(in).at(1, 0), dz(in).at(0, 0) - dx(in).at(2, 0), dx(in).at(1, 0) - dy(in).at(0, 0)});
 }
 
 expression strain(const expression& input) {
@@ -205,8 +298,7 @@ void DofManager::addtostructure(const std::shared_ptr<RawField>& f, const std::v
     } else
         SLX_CHECK(f->orders() == orders_at_add_[fi], "field order was changed and does not match dof structure anymore");
     for (int dr : disjregs) {
-        int t = mesh_->disjreg_type(dr);
-        int nff = h1_count(t, f->order(dr), topo::dimension(t));
+        int nff = entity_nff(*f, mesh_->disjreg_type(dr), f->order(dr));
         std::vector<int>& rb = rangebegin_[fi][dr];
         int before = (int)rb.size();
         if (before >= nff) continue;
@@ -243,7 +335,7 @@ std::vector<double> DofManager::constraint_values() const {
     for (size_t fi = 0; fi < fields_.size(); fi++)
         for (int dr = 0; dr < (int)rangebegin_[fi].size(); dr++) {
             if (rangebegin_[fi][dr].empty() || !fields_[fi]->constrained(dr)) continue;
-            if (mesh_->disjreg_type(dr) != 0) continue;
+            if (mesh_->disjreg_type(dr) != 0 || fields_[fi]->hcurl()) continue;
             int n = mesh_->disjreg_count(dr);
             int rb = rangebegin_[fi][dr][0];
             for (int i = 0; i < n; i++) out[rb + i] = fields_[fi]->constraint_value(dr);
@@ -275,7 +367,7 @@ std::vector<int32_t> DofManager::getaddresses(const std::shared_ptr<RawField>& f
     SLX_CHECK(fi >= 0, "selected field is not defined in the dof manager");
     std::vector<char> defined(mesh_->count_disjregs(), 0);
     for (int dr : mesh_->physreg_disjregs(fieldphysreg, -1)) defined[dr] = 1;
-    std::vector<FormFunction> ffs = h1_iterate(elemtype, order);
+    std::vector<FormFunction> ffs = element_ffs(*f, elemtype, order);
     int ne = (int)elems.size();
     std::vector<int32_t> out((size_t)ffs.size() * ne);
     for (size_t k = 0; k < ffs.size(); k++) {
@@ -284,7 +376,7 @@ std::vector<int32_t> DofManager::getaddresses(const std::shared_ptr<RawField>& f
         for (int i = 0; i < ne; i++) {
             int sub = mesh_->subelement(ff.assoc_type, elemtype, elems[i], ff.sub);
             int dr = mesh_->disjreg_of(ff.assoc_type, sub);
-            if (i == 0 || dr != prevdr) cnt = h1_count(ff.assoc_type, f->order(dr), ff.dim);
+            if (i == 0 || dr != prevdr) cnt = entity_nff(*f, ff.assoc_type, f->order(dr));
             prevdr = dr;
             int32_t a = -1;
             if (defined[dr] && ff.index < cnt) a = rangebegin_[fi][dr][ff.index] + (sub - mesh_->disjreg_begin(dr));
@@ -311,6 +403,8 @@ formulation& formulation::operator+=(const integration& in) {
         c.integral_id = id;
         c.doffield = s.dofs[sl][0] ? s.dofs[sl][0]->field : nullptr;
         c.tffield = s.tfs[sl][0]->field;
+        c.doffam = s.dofs[sl][0] ? s.dofs[sl][0]->hc : 0;
+        c.tffam = s.tfs[sl][0]->hc;
         c.coeffs = s.coeffs[sl];
         c.dofs = s.dofs[sl];
         c.tfs = s.tfs[sl];
@@ -413,7 +507,8 @@ struct Compiler {
                 for (size_t i = 0; i < fields->size(); i++)
                     if ((*fields)[i] == op->field) k = (int)i;
                 if (k < 0) { k = (int)fields->size(); fields->push_back(op->field); }
-                SLX_CHECK(op->kietaphideriv == 0, "reference-coordinate derivatives of fields are not supported in coefficients");
+                SLX_CHECK(op->kietaphideriv == 0 && !op->hc,
+                          "reference-coordinate derivatives of fields and hcurl field values are not supported in coefficients");
                 s << "f" << k << "_" << op->spacederiv;
                 return emit(SLX_OP_FIELD, op->spacederiv, 0, k, s.str());
             }
@@ -576,7 +671,9 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
                 f.pdof = c.doffield ? c.doffield->order(g[0]) : f.ptf;
                 f.intorder = f.ptf + f.pdof + 2 + c.delta;
                 SLX_CHECK(f.intorder >= 0, "trying to integrate at negative order " << f.intorder);
-                f.orientdep = h1_orientation_dependent(f.ptf) || (c.doffield && h1_orientation_dependent(f.pdof));
+                // hcurl functions are orientation dependent at every order
+                const bool hc = c.tffield->hcurl() || (c.doffield && c.doffield->hcurl());
+                f.orientdep = hc || h1_orientation_dependent(f.ptf) || (c.doffield && h1_orientation_dependent(f.pdof));
                 std::vector<int> sorted = g;
                 std::sort(sorted.begin(), sorted.end());
                 std::vector<int> elems;
@@ -598,7 +695,7 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
                 // folded: the batches, in the same order, as one fragment on the
                 // orientation-0 tables (element matrices equal up to the signed
                 // renumbering, which the addresses and signs carry)
-                if (fold_ && !reads_fields(c) && fold_of(f.type, f.ptf, f.intorder).ok &&
+                if (fold_ && !hc && !reads_fields(c) && fold_of(f.type, f.ptf, f.intorder).ok &&
                     (!c.doffield || fold_of(f.type, f.pdof, f.intorder).ok)) {
                     Frag fb = f;
                     fb.orient = 0;
@@ -700,13 +797,16 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         ps->dbls.push_back(rule.weights);
         bd.gp_weights = ps->dbls.back().data();
         // shape tables for the tf and dof orders (orientation of the batch)
-        std::vector<int> table_orders;
-        auto table_for = [&](int order) {
+        // (family, order): family 0 H1 values and reference derivatives,
+        // HCURL_VALUE / HCURL_CURL the hcurl reference components
+        std::vector<std::pair<int, int>> table_orders;
+        auto table_for = [&](int fam, int order) {
             for (size_t t = 0; t < table_orders.size(); t++)
-                if (table_orders[t] == order) return (int)t;
-            table_orders.push_back(order);
+                if (table_orders[t] == std::make_pair(fam, order)) return (int)t;
+            table_orders.push_back({fam, order});
             std::vector<double> tab;
-            h1_tables(f0.type, order, f0.orient, rule.coords, tab);
+            if (fam == 0) h1_tables(f0.type, order, f0.orient, rule.coords, tab);
+            else hcurl_tables(f0.type, order, f0.orient, fam, rule.coords, tab);
             ps->dbls.push_back(tab);
             ps->table_ptrs[b].push_back(ps->dbls.back().data());
             return (int)table_orders.size() - 1;
@@ -724,16 +824,16 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
             slx_slice_desc sd{};
             sd.frag_rank = frag_rank[fi];
             sd.is_rhs = c.doffield ? 0 : 1;
-            sd.tf_table = table_for(f.ptf);
-            sd.n_tf = h1_count(f.type, f.ptf);
+            sd.tf_table = table_for(c.tffam, f.ptf);
+            sd.n_tf = element_nff(*c.tffield, f.type, f.ptf);
             ps->ints.push_back(dofm_->getaddresses(c.tffield, f.ptf, f.type, f.elems, c.tfphysreg));
             sd.tf_addr = ps->ints.back().data();
             sd.tf_sign = nullptr;
             sd.dof_sign = nullptr;
             if (f.folded && h1_orientation_dependent(f.ptf)) sd.tf_sign = fold_addresses(f, f.ptf, ps->ints.back());
             if (c.doffield) {
-                sd.dof_table = table_for(f.pdof);
-                sd.n_dof = h1_count(f.type, f.pdof);
+                sd.dof_table = table_for(c.doffam, f.pdof);
+                sd.n_dof = element_nff(*c.doffield, f.type, f.pdof);
                 ps->ints.push_back(dofm_->getaddresses(c.doffield, f.pdof, f.type, f.elems, c.dofphysreg));
                 sd.dof_addr = ps->ints.back().data();
                 if (f.folded && h1_orientation_dependent(f.pdof)) sd.dof_sign = fold_addresses(f, f.pdof, ps->ints.back());
@@ -766,7 +866,7 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         bd.n_slices = (int)ps->slices[b].size();
         bd.slices = ps->slices[b].data();
         ps->ints.emplace_back();
-        for (int o : table_orders) ps->ints.back().push_back(h1_count(f0.type, o));
+        for (auto& fo : table_orders) ps->ints.back().push_back(fo.first ? hcurl_count(f0.type, fo.second) : h1_count(f0.type, fo.second));
         bd.n_tables = (int)table_orders.size();
         bd.table_nff = ps->ints.back().data();
         bd.tables = ps->table_ptrs[b].data();

@@ -24,6 +24,7 @@ class RawField {
     RawField(const Mesh* m, const std::string& type, const std::string& name);
     const std::string& name() const { return name_; }
     const std::string& type() const { return type_; }
+    bool hcurl() const { return type_ == "hcurl"; }
     int meshdim() const { return mesh_->dimension(); }
     const Mesh* mesh() const { return mesh_; }
     int ncomponents() const { return subfields_.empty() ? 1 : (int)subfields_.size(); }
@@ -108,6 +109,9 @@ expression dx(const expression& e);
 expression dy(const expression& e);
 expression dz(const expression& e);
 expression grad(const expression& e);
+// sl::curl (sl.cpp:1167-1226): of an hcurl field (or its dof/tf) in reference
+// coordinates, mapped by J^T / det J; else from the space derivatives.
+expression curl(const expression& e);
 expression strain(const expression& e);
 expression predefinedelasticity(const expression& dofu, const expression& tfu, const expression& E, const expression& nu);
 expression pow(const expression& b, const expression& e);
@@ -194,6 +198,7 @@ class formulation {
     struct Contribution {
         int integral_id;
         std::shared_ptr<RawField> doffield, tffield;
+        int doffam = 0, tffam = 0;  // hcurl table family of the dof / tf (0: H1)
         std::vector<OpP> coeffs, dofs, tfs;
         int intphysreg, dofphysreg, tfphysreg, delta;
     };

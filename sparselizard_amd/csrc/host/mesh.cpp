@@ -15,6 +15,25 @@ void Mesh::set_nodes(const std::vector<double>& xyz) {
     coords_ = xyz;
 }
 
+void Mesh::permute_nodes(const std::vector<int>& newindex) {
+    SLX_CHECK(!finalized_, "mesh already finalized");
+    const int n = count(0);
+    SLX_CHECK((int)newindex.size() == n, "node permutation of size " << newindex.size() << " for " << n << " nodes");
+    std::vector<char> seen(n, 0);
+    for (int v : newindex) {
+        SLX_CHECK(v >= 0 && v < n && !seen[v], "not a node permutation");
+        seen[v] = 1;
+    }
+    std::vector<double> c(coords_.size());
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < 3; d++) c[3 * (size_t)newindex[i] + d] = coords_[3 * (size_t)i + d];
+    coords_.swap(c);
+    for (int t = 1; t < 8; t++) {
+        for (int& v : sub_[t][0]) v = newindex[v];
+        for (int& v : curv_[t]) v = newindex[v];
+    }
+}
+
 int Mesh::count(int type) const {
     if (type == 0) return (int)coords_.size() / 3;
     int nn = topo::count_nodes(type);

@@ -43,6 +43,9 @@ class Mesh {
     // A repeated element line of a file: the previous element also joins
     // another physical region (gmshinterface.cpp:274-281).
     void add_to_physreg(int type, int elem, int physreg);
+    // Before finalize: node i becomes node newindex[i] (coordinates and every
+    // element node list follow).
+    void permute_nodes(const std::vector<int>& newindex);
     // Run the pipeline once every element has been added.
     void finalize();
     // Gmsh 2.2 ASCII reader / writer (gmshinterface.cpp:157-298).

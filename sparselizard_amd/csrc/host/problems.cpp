@@ -1,5 +1,6 @@
 // Synthetic benchmark workloads (see problems.h).
 #include "problems.h"
+#include <numeric>
 #include <cmath>
 #include <sstream>
 
@@ -453,6 +454,71 @@ std::shared_ptr<Problem> projection(const std::map<std::string, std::string>& s)
     return p;
 }
 
+// H(curl) field E of the given order on a unit square / cube grid (nodes
+// randomly renumbered with shuffle=1, so that edge and face orientations
+// vary), with a vector polynomial f of degree `deg` (third component 0 in 2D):
+//   term=mass     integral(dof(E) tf(E) - f tf(E)): the L2 projection, exact
+//                 when f lies in the discrete space (degree <= order on
+//                 simplices and affine quadrangles / hexahedra, order >= 1);
+//   term=curl     integral(curl(dof(E)) curl(tf(E))) + integral(-f tf(E)),
+//                 the same DOF numbering as term=mass;
+//   term=maxwell  integral(-curl(dof(E)) curl(tf(E)) + k2 dof(E) tf(E) - f tf(E))
+//                 (the waveguide examples' formulation with a source).
+std::shared_ptr<Problem> hcurlprojection(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    const std::string elem = s.count("elem") ? s.at("elem") : "tri";
+    const std::string term = s.count("term") ? s.at("term") : "mass";
+    const int n = geti(s, "n", 3), order = geti(s, "order", 1), deg = geti(s, "deg", std::max(order, 0));
+    const double jit = getd(s, "jitter", (elem == "tri" || elem == "tet") ? 0.2 : 0.0);
+    Rng rng((uint64_t)geti(s, "seed", 11));
+    p->mesh = std::make_shared<Mesh>();
+    Mesh& m = *p->mesh;
+    grid_mesh(m, elem, n, jit, rng, 1, -1);
+    if (geti(s, "shuffle", 1)) {
+        std::vector<int> perm(m.count(0));
+        std::iota(perm.begin(), perm.end(), 0);
+        for (int i = (int)perm.size() - 1; i > 0; i--) std::swap(perm[i], perm[rng.uniform_int(0, i)]);
+        m.permute_nodes(perm);
+    }
+    m.finalize();
+    field E(m, "hcurl", "E");
+    E.setorder(1, order);
+    p->fields = {E};
+    p->form = std::make_shared<formulation>(m);
+    using namespace sl;
+    const bool is3d = elem == "tet" || elem == "hex";
+    std::vector<expression> fc;
+    for (int comp = 0; comp < 3; comp++) {
+        expression f(0.0);
+        bool first = true;
+        for (int a = 0; a <= deg && (is3d || comp < 2); a++)
+            for (int b = 0; a + b <= deg; b++)
+                for (int c = 0; a + b + c <= deg && (is3d || c == 0); c++) {
+                    expression mono(((comp + a + b + c) % 2 ? -1.0 : 1.0) / (1.0 + a + 2.0 * b + 3.0 * c + 5.0 * comp));
+                    for (int k = 0; k < a; k++) mono = mono * x();
+                    for (int k = 0; k < b; k++) mono = mono * y();
+                    for (int k = 0; k < c; k++) mono = mono * z();
+                    f = first ? mono : f + mono;
+                    first = false;
+                }
+        fc.push_back(f);
+    }
+    const expression f(3, 1, fc);
+    const int delta = geti(s, "delta", 0);
+    if (term == "mass") {
+        *p->form += integral(1, dof(E) * tf(E) - f * tf(E), delta);
+    } else if (term == "curl") {
+        *p->form += integral(1, curl(dof(E)) * curl(tf(E)), delta);
+        *p->form += integral(1, -f * tf(E), delta);
+    } else {
+        SLX_CHECK(term == "maxwell", "term is mass, curl or maxwell");
+        const expression k2(getd(s, "k2", 25.0));
+        *p->form += integral(1, -curl(dof(E)) * curl(tf(E)) + k2 * dof(E) * tf(E) - f * tf(E), delta);
+    }
+    p->describe = "hcurl " + term + " on " + elem + " order " + std::to_string(order) + ", f of degree " + std::to_string(deg);
+    return p;
+}
+
 // Config: 3D hp-Poisson on hexahedra, per-element order uniform in
 // [pmin, pmax] (one physical region per order, orders set descendingly so
 // interfaces take the lower order, rawfield.cpp:670-676), coefficient
@@ -654,6 +720,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "elasticity3d") p = elasticity3d(s);
     else if (kind == "quadbatch") p = quadbatch(s);
     else if (kind == "projection") p = projection(s);
+    else if (kind == "hcurlprojection") p = hcurlprojection(s);
     else if (kind == "hppoisson3d") p = hppoisson3d(s);
     else if (kind == "prism3d") p = prism3d(s);
     else if (kind == "msh") p = from_msh(s);

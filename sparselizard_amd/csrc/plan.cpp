@@ -2462,6 +2462,23 @@ slx_status slx_h1_tables(int32_t elem_type, int32_t order, int32_t total_orienta
     });
 }
 
+slx_status slx_hcurl_count(int32_t elem_type, int32_t order, int32_t* nff) {
+    return guarded([&] {
+        SLX_CHECK(nff && order >= 0, "bad argument");
+        *nff = hcurl_count(elem_type, order);
+    });
+}
+
+slx_status slx_hcurl_tables(int32_t elem_type, int32_t order, int32_t total_orientation, int32_t family, int32_t npts,
+                            const double* pts, double* out) {
+    return guarded([&] {
+        SLX_CHECK(pts && out && npts >= 0 && order >= 0, "bad argument");
+        std::vector<double> p(pts, pts + 3 * (size_t)npts), o;
+        hcurl_tables(elem_type, order, total_orientation, family, p, o);
+        std::copy(o.begin(), o.end(), out);
+    });
+}
+
 slx_status slx_geometry_tables(int32_t elem_type, int32_t n_geo, int32_t npts, const double* pts, double* out) {
     return guarded([&] {
         SLX_CHECK(pts && out && npts >= 0 && elem_type >= 1 && elem_type <= 7, "bad argument");
