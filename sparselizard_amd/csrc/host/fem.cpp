@@ -30,6 +30,8 @@ RawField::RawField(const Mesh* m, const std::string& type, const std::string& na
     orders_.assign(m->count_disjregs(), 1);
     constrained_.assign(m->count_disjregs(), 0);
     cvalue_.assign(m->count_disjregs(), 0.0);
+    cexpr_.assign(m->count_disjregs(), nullptr);
+    ceio_.assign(m->count_disjregs(), 0);
 }
 
 void RawField::setorder(int physreg, int order) {
@@ -50,6 +52,21 @@ void RawField::setconstraint(int physreg, double value) {
     for (int dr : mesh_->physreg_disjregs(physreg, -1)) {
         constrained_[dr] = 1;
         cvalue_[dr] = value;
+        cexpr_[dr] = nullptr;
+    }
+}
+
+void RawField::setconstraint(int physreg, const expression& value, int extraintegrationdegree) {
+    SLX_CHECK(hcurl(), "expression constraints are provided for hcurl fields (h1 fields take a constant)");
+    SLX_CHECK(value.countrows() == 3 && value.countcolumns() == 1, "the rawfield must be constrained using a 3x1 expression");
+    for (int r = 0; r < 3; r++)
+        SLX_CHECK(!value.op(r, 0)->isdofincluded() && !value.op(r, 0)->istfincluded(), "a constraint cannot contain dof() or tf()");
+    auto e = std::make_shared<expression>(value);
+    for (int dr : mesh_->physreg_disjregs(physreg, -1)) {
+        constrained_[dr] = 1;
+        cvalue_[dr] = 0.0;
+        cexpr_[dr] = e;
+        ceio_[dr] = extraintegrationdegree;
     }
 }
 
@@ -92,6 +109,9 @@ field::field(const Mesh& m, const std::string& type, const std::string& name) {
 
 void field::setorder(int physreg, int order) { raw_->setorder(physreg, order); }
 void field::setconstraint(int physreg, double value) { raw_->setconstraint(physreg, value); }
+void field::setconstraint(int physreg, const expression& value, int extraintegrationdegree) {
+    raw_->setconstraint(physreg, value, extraintegrationdegree);
+}
 
 static OpP field_op(const std::shared_ptr<RawField>& f) {
     auto o = std::make_shared<Op>();
@@ -325,6 +345,110 @@ std::vector<uint8_t> DofManager::isconstrained() const {
     return out;
 }
 
+// Value of a coefficient expression (no dof, tf or field) at a physical point.
+static double eval_at(const OpP& op, const double* xyz) {
+    switch (op->kind) {
+        case OpKind::Constant: return op->value;
+        case OpKind::Coord: return xyz[op->row];
+        case OpKind::Sum: {
+            double v = 0.0;
+            for (const OpP& a : op->args) v += eval_at(a, xyz);
+            return v;
+        }
+        case OpKind::Product: {
+            double v = 1.0;
+            for (const OpP& a : op->args) v *= eval_at(a, xyz);
+            return v;
+        }
+        case OpKind::Inversion: return 1.0 / eval_at(op->args[0], xyz);
+        case OpKind::Power: return std::pow(eval_at(op->args[0], xyz), eval_at(op->args[1], xyz));
+        case OpKind::Abs: return std::abs(eval_at(op->args[0], xyz));
+        case OpKind::Exp: return std::exp(eval_at(op->args[0], xyz));
+        case OpKind::Log: return std::log(eval_at(op->args[0], xyz));
+        case OpKind::Sin: return std::sin(eval_at(op->args[0], xyz));
+        case OpKind::Cos: return std::cos(eval_at(op->args[0], xyz));
+        case OpKind::Tan: return std::tan(eval_at(op->args[0], xyz));
+        case OpKind::Sqrt: return std::sqrt(eval_at(op->args[0], xyz));
+        default: fail("constraint expressions may hold constants, coordinates and functions of them (got " + op_str(op) + ")");
+    }
+}
+
+// Dense solve with partial pivoting (the small per-edge projection systems).
+static std::vector<double> dense_solve(std::vector<double> A, std::vector<double> b) {
+    const int n = (int)b.size();
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::abs(A[(size_t)r * n + c]) > std::abs(A[(size_t)piv * n + c])) piv = r;
+        SLX_CHECK(A[(size_t)piv * n + c] != 0.0, "singular constraint projection");
+        if (piv != c) {
+            for (int k = 0; k < n; k++) std::swap(A[(size_t)c * n + k], A[(size_t)piv * n + k]);
+            std::swap(b[c], b[piv]);
+        }
+        for (int r = c + 1; r < n; r++) {
+            const double f = A[(size_t)r * n + c] / A[(size_t)c * n + c];
+            for (int k = c; k < n; k++) A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+            b[r] -= f * b[c];
+        }
+    }
+    for (int r = n - 1; r >= 0; r--) {
+        double v = b[r];
+        for (int k = r + 1; k < n; k++) v -= A[(size_t)r * n + k] * b[k];
+        b[r] = v / A[(size_t)r * n + r];
+    }
+    return b;
+}
+
+// hcurl constraint expression on the edges of a 2D mesh: per line element the
+// L2 projection of the expression on the element's edge functions
+// (rawfield::updateothershapefunctions, rawfield.cpp:164-260, dimension 1:
+// block-diagonal projection integral(dof(E) tf(E) - expr tf(E)) at integration
+// order 2p + 2 + extra).  On a line of a 2D mesh the Jacobian is completed by
+// the unit normal (jacobian.cpp:138-185), so E = v t / |dx/dki| with t the unit
+// tangent and det J = |dx/dki|: the mass is int v_i v_j / |dx/dki| dki and the
+// load int (expr . t) v_i dki.
+static void project_hcurl_edges(const Mesh& m, const RawField& f, int dr, const std::vector<int>& rb, std::vector<double>& out) {
+    const expression& ex = *f.constraint_expression(dr);
+    const int p = f.order(dr), nff = p + 1;
+    SLX_CHECK((int)rb.size() == nff, "constrained edge region without its edge functions");
+    const GaussRule rule = gauss_rule(LINE, 2 * p + 2 + f.constraint_integration_degree(dr));
+    const int np = rule.count();
+    const std::vector<double>& xyz = m.coords();
+    std::map<int, std::vector<double>> tabs;  // per total orientation
+    for (int e = m.disjreg_begin(dr); e <= m.disjreg_end(dr); e++) {
+        const int o = m.total_orientation(LINE, e);
+        auto it = tabs.find(o);
+        if (it == tabs.end()) {
+            std::vector<double> t;
+            hcurl_tables(LINE, p, o, HCURL_VALUE, rule.coords, t);
+            it = tabs.emplace(o, std::move(t)).first;
+        }
+        const std::vector<double>& tab = it->second;
+        const int n0 = m.subelement(0, LINE, e, 0), n1 = m.subelement(0, LINE, e, 1);
+        double a[3], x0[3];
+        for (int d = 0; d < 3; d++) {
+            x0[d] = xyz[3 * (size_t)n0 + d];
+            a[d] = 0.5 * (xyz[3 * (size_t)n1 + d] - x0[d]);  // dx/dki of the order-1 line
+        }
+        const double len = std::sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        std::vector<double> M((size_t)nff * nff, 0.0), b(nff, 0.0);
+        for (int q = 0; q < np; q++) {
+            const double ki = rule.coords[3 * (size_t)q], w = rule.weights[q];
+            double x[3];
+            for (int d = 0; d < 3; d++) x[d] = x0[d] + (ki + 1.0) * a[d];
+            double et = 0.0;
+            for (int d = 0; d < 3; d++) et += eval_at(ex.op(d, 0), x) * a[d] / len;
+            for (int i = 0; i < nff; i++) {
+                const double vi = tab[((size_t)1 * nff + i) * np + q];
+                b[i] += w * et * vi;
+                for (int j = 0; j < nff; j++) M[(size_t)i * nff + j] += w * vi * tab[((size_t)1 * nff + j) * np + q] / len;
+            }
+        }
+        const std::vector<double> c = dense_solve(M, b);
+        for (int i = 0; i < nff; i++) out[(size_t)rb[i] + (e - m.disjreg_begin(dr))] = c[i];
+    }
+}
+
 std::vector<double> DofManager::constraint_values() const {
     // The reference projects the constraint expression onto the constrained
     // region (rawfield::setdisjregconstraint, rawfield.cpp:1166-1248).  For a
@@ -335,7 +459,17 @@ std::vector<double> DofManager::constraint_values() const {
     for (size_t fi = 0; fi < fields_.size(); fi++)
         for (int dr = 0; dr < (int)rangebegin_[fi].size(); dr++) {
             if (rangebegin_[fi][dr].empty() || !fields_[fi]->constrained(dr)) continue;
-            if (mesh_->disjreg_type(dr) != 0 || fields_[fi]->hcurl()) continue;
+            if (fields_[fi]->hcurl()) {
+                if (!fields_[fi]->constraint_expression(dr)) continue;
+                const int t = mesh_->disjreg_type(dr);
+                if (t == POINT) continue;
+                SLX_CHECK(t == LINE && mesh_->dimension() == 2,
+                          "hcurl constraints with an expression are provided on line regions of 2D meshes");
+                SLX_CHECK(!mesh_->curved(), "hcurl constraint expressions need straight line elements");
+                project_hcurl_edges(*mesh_, *fields_[fi], dr, rangebegin_[fi][dr], out);
+                continue;
+            }
+            if (mesh_->disjreg_type(dr) != 0) continue;
             int n = mesh_->disjreg_count(dr);
             int rb = rangebegin_[fi][dr][0];
             for (int i = 0; i < n; i++) out[rb + i] = fields_[fi]->constraint_value(dr);

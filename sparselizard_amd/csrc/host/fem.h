@@ -38,6 +38,12 @@ class RawField {
     void setorder(int physreg, int order);
     // rawfield::setdisjregconstraint with a constant value (rawfield.cpp:1166-1248).
     void setconstraint(int physreg, double value = 0.0);
+    // rawfield::setdisjregconstraint with an expression (hcurl fields: a 3x1
+    // vector, projected on the constrained edges when the DOF values are
+    // computed; later calls override earlier ones region by region).
+    void setconstraint(int physreg, const expression& value, int extraintegrationdegree = 0);
+    const expression* constraint_expression(int disjreg) const { return cexpr_[disjreg].get(); }
+    int constraint_integration_degree(int disjreg) const { return ceio_[disjreg]; }
     std::shared_ptr<RawField> self() { return self_.lock(); }
     void setself(const std::shared_ptr<RawField>& s) { self_ = s; }
     void addsub(const std::shared_ptr<RawField>& s) { subfields_.push_back(s); }
@@ -55,6 +61,8 @@ class RawField {
     std::vector<int> orders_;
     std::vector<char> constrained_;
     std::vector<double> cvalue_;
+    std::vector<std::shared_ptr<expression>> cexpr_;
+    std::vector<int> ceio_;
     std::weak_ptr<RawField> self_;
     int64_t data_size_ = 0;
 };
@@ -66,6 +74,7 @@ class field {
     field(const Mesh& m, const std::string& type, const std::string& name = "u");
     void setorder(int physreg, int order);
     void setconstraint(int physreg, double value = 0.0);
+    void setconstraint(int physreg, const expression& value, int extraintegrationdegree = 0);
     std::shared_ptr<RawField> raw() const { return raw_; }
     operator expression() const;
     expression comp(int i) const;

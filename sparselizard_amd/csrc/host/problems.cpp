@@ -686,6 +686,31 @@ std::shared_ptr<Problem> electrostatics1d(const std::map<std::string, std::strin
     return p;
 }
 
+// examples/electromagnetic-waveguide-2d/main.cpp:13-40 on its mesh file: hcurl
+// electric field of order 2 on the whole domain (region 3), tangential E = 0
+// on the skin (region 2), E = sin(y / 0.1 pi) (0, 1, 0) projected on the
+// left edge (region 1), integral(-curl(dof(E)) curl(tf(E)) + k^2 dof(E) tf(E))
+// with k = 2 pi f / c, f = 0.9 GHz, c = 3e8 and the example's pi = 3.14159.
+std::shared_ptr<Problem> waveguide2d(const std::map<std::string, std::string>& s) {
+    auto p = std::make_shared<Problem>();
+    auto it = s.find("file");
+    SLX_CHECK(it != s.end(), "waveguide2d needs file=<path>");
+    p->mesh = std::make_shared<Mesh>(Mesh::read_msh(it->second));
+    Mesh& m = *p->mesh;
+    const int left = 1, skin = 2, wholedomain = 3;
+    field E(m, "hcurl", "E");
+    E.setorder(wholedomain, geti(s, "order", 2));
+    const double freq = getd(s, "freq", 0.9e9), c = 3e8, pi = 3.14159, k = 2 * pi * freq / c;
+    using namespace sl;
+    E.setconstraint(skin);
+    E.setconstraint(left, sl::sin(y() / expression(0.1) * expression(pi)) * expression(3, 1, {expression(0.0), expression(1.0), expression(0.0)}));
+    p->fields = {E};
+    p->form = std::make_shared<formulation>(m);
+    *p->form += integral(wholedomain, -curl(dof(E)) * curl(tf(E)) + expression(k * k) * dof(E) * tf(E));
+    p->describe = "electromagnetic waveguide 2D (hcurl), mesh file " + it->second;
+    return p;
+}
+
 // examples/high-order-interpolations-2d/main.cpp:17-48 on its mesh file:
 // L2 projection of sin(10x) sin(13y) onto h1 of the given order on face 1,
 // integration order +20.
@@ -721,6 +746,7 @@ std::shared_ptr<Problem> make_problem(const std::string& kind, const std::string
     else if (kind == "quadbatch") p = quadbatch(s);
     else if (kind == "projection") p = projection(s);
     else if (kind == "hcurlprojection") p = hcurlprojection(s);
+    else if (kind == "waveguide2d") p = waveguide2d(s);
     else if (kind == "hppoisson3d") p = hppoisson3d(s);
     else if (kind == "prism3d") p = prism3d(s);
     else if (kind == "msh") p = from_msh(s);

@@ -7,6 +7,12 @@ run on the meshes those examples ship (copied as data into tests/golden/).
   examples/high-order-interpolations-2d/main.cpp:68
       relative L2 error < 1e-5 of the order-5 projection of sin(10x) sin(13y)
       on the 81-quad mesh (integration order +20).
+  examples/electromagnetic-waveguide-2d/main.cpp:49
+      0.6825 < ||x||_2 < 0.6826 for the order-2 hcurl electric field of
+      integral(-curl(dof(E)) curl(tf(E)) + k^2 dof(E) tf(E)) on the 2444-quad
+      waveguide, E = 0 on the skin and the projected sin(y / 0.1 pi) (0, 1, 0)
+      on the left edge (hcurl bases, orientations, curl mapping and the edge
+      projection of the constraint together).
 
 The solve x_A = A^-1 (b_A - D x_D) (mat::eliminate + sl::solve,
 src/formulation/mat.cpp:78-103, src/expression/operation/sl.cpp:1565-1627)
@@ -30,6 +36,7 @@ from oracle import py_oracle
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LINE_MSH = os.path.join(GOLDEN, "electrostatics1d_line.msh")
 QUAD_MSH = os.path.join(GOLDEN, "hointerp2d_quad.msh")
+WAVEGUIDE_MSH = os.path.join(GOLDEN, "waveguide2d.msh")
 
 
 def solve_full(csr, ndofs, nfree):
@@ -112,6 +119,22 @@ def test_oracle_projection_error_decreases_with_order():
     assert errs[0] > errs[1] > errs[2], errs
 
 
+def test_oracle_waveguide_2d_golden():
+    p = sa.Problem(None, "waveguide2d", "file=%s;meshpipe=host" % WAVEGUIDE_MSH)
+    info = register_free(p)
+    assert info["nelem"] == 2444
+    o = py_oracle.assemble(p.desc_ptr(), p.numeric(), info["ndofs"], info["nfree"])
+    x, _, _ = solve_full(o, info["ndofs"], info["nfree"])
+    nrm = float(np.linalg.norm(x))
+    assert 0.6825 < nrm < 0.6826, nrm
+    # planted check of the test itself: another frequency leaves the bracket
+    p2 = sa.Problem(None, "waveguide2d", "file=%s;meshpipe=host;freq=0.91e9" % WAVEGUIDE_MSH)
+    info2 = register_free(p2)
+    o2 = py_oracle.assemble(p2.desc_ptr(), p2.numeric(), info2["ndofs"], info2["nfree"])
+    x2, _, _ = solve_full(o2, info2["ndofs"], info2["nfree"])
+    assert not 0.6825 < float(np.linalg.norm(x2)) < 0.6826
+
+
 # ------------------------------------------------------------------- GPU ----
 
 @pytest.mark.gpu
@@ -135,3 +158,15 @@ def test_gpu_high_order_projection_2d_golden(ctx):
     plan = p.generate("fp64")
     err = projection_error(plan.csr(info["ndofs"]), info["nfree"])
     assert err < 1e-5, err
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "f16split", "fp32"])
+def test_gpu_waveguide_2d_golden(ctx, precision):
+    print("case waveguide2d", precision, flush=True)
+    p = sa.Problem(ctx, "waveguide2d", "file=" + WAVEGUIDE_MSH)
+    info = register_free(p)
+    plan = p.generate(precision)
+    x, _, _ = solve_full(plan.csr(info["ndofs"]), info["ndofs"], info["nfree"])
+    nrm = float(np.linalg.norm(x))
+    assert 0.6825 < nrm < 0.6826, nrm

@@ -30,6 +30,8 @@ CASES = [
     ("prism3d", "n=3;order=3"),
     ("convdiff2d", "n=12;order=2;bx=20;by=10"),
     ("projection", "elem=tet;n=3;order=3"),
+    ("hcurlprojection", "elem=tet;n=3;order=2;term=maxwell"),
+    ("hcurlprojection", "elem=hex;n=3;order=1;term=maxwell"),
     ("projection", "elem=quad;n=4;order=3"),
     ("electrostatics1d", "file=" + os.path.join(GOLDEN, "electrostatics1d_line.msh")),
     ("hoprojection2d", "file=%s;order=5" % os.path.join(GOLDEN, "hointerp2d_quad.msh")),
