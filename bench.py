@@ -43,8 +43,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--problem", default="elasticity3d",
                     help="elasticity3d (the headline workload), hppoisson3d (hp hexahedra, orders 1..5), poisson2d, "
-                         "quadbatch (512 independent problems), newton2d (Newton re-assembly with a field) or prism3d "
-                         "(order-1 prisms)")
+                         "quadbatch (512 independent problems), newton2d (Newton re-assembly with a field), prism3d "
+                         "(order-1 prisms) or hcurlprojection (hcurl fields; --extra ';elem=tet;order=2;term=maxwell')")
     ap.add_argument("--n", type=int, default=0,
                     help="cells per edge (0: the workload's full size: 64 elasticity3d, 48 hppoisson3d, 200 poisson2d, "
                          "708 newton2d, 96 prism3d; quadbatch takes its sizes from --extra)")
@@ -76,7 +76,8 @@ def main():
         return
     ctx = sa.Context(0)
     if args.n <= 0:
-        args.n = {"elasticity3d": 64, "hppoisson3d": 48, "poisson2d": 200, "newton2d": 708, "prism3d": 96}.get(args.problem, 0)
+        args.n = {"elasticity3d": 64, "hppoisson3d": 48, "poisson2d": 200, "newton2d": 708, "prism3d": 96,
+                  "hcurlprojection": 32}.get(args.problem, 0)
     if args.problem == "elasticity3d":
         spec = "n=%d;order=%d" % (args.n, args.order)
     elif args.problem == "quadbatch":
@@ -323,6 +324,14 @@ def workload_name(args, n_elem):
                 order = int(kv[6:])
         return ("3D Poisson + mass, order-%d prisms, %d^3 x 2 jittered prisms (%d), c lognormal(0,1), u=0 on the side faces"
                 % (order, args.n, 2 * args.n ** 3))
+    if args.problem == "hcurlprojection":
+        opts = dict(kv.split("=", 1) for kv in args.extra.split(";") if "=" in kv)
+        elem = opts.get("elem", "tri")
+        cells = "%d^3" % args.n if elem in ("tet", "hex") else "%dx%d" % (args.n, args.n)
+        return ("hcurl %s of order %s on %s %s cells (%d elements, nodes randomly renumbered): %s"
+                % (opts.get("term", "mass"), opts.get("order", "1"), cells, elem, n_elem,
+                   "integral(-curl(dof(E)) curl(tf(E)) + k2 dof(E) tf(E) - f tf(E))" if opts.get("term") == "maxwell"
+                   else "hcurl formulation"))
     return "%s %s" % (args.problem, args.extra)
 
 
@@ -380,7 +389,9 @@ def cpu_baseline(sa, args):
     # sample size knob, its start, cap and how the element count grows with it
     knob = {"elasticity3d": ("n", 4, 28, 3), "hppoisson3d": ("n", 3, 24, 3), "poisson2d": ("n", 20, 1200, 2),
             "newton2d": ("n", 20, 708, 2), "quadbatch": ("instances", 2, 96, 1),
-            "prism3d": ("n", 4, 110, 3)}[args.problem]
+            "prism3d": ("n", 4, 110, 3),
+            "hcurlprojection": ("n", 3, 40, 3) if any(e in args.extra for e in ("elem=tet", "elem=hex")) else ("n", 4, 400, 2)
+            }[args.problem]
 
     def spec_of(v):
         if args.problem == "elasticity3d":

@@ -66,6 +66,10 @@ struct H1Fold {
 H1Fold h1_fold(int type, int order, const std::vector<double>& pts);
 // perm[f], sign[f] of every function for the given total orientation.
 void h1_fold_apply(const H1Fold& F, int type, int total, std::vector<int>& perm, std::vector<int8_t>& sign);
+// The same folding for hcurl functions (components and their reference
+// gradients compared): found on quadrangles and hexahedra; h1_fold_apply
+// applies it.
+H1Fold hcurl_fold(int type, int order, const std::vector<double>& pts);
 
 // Hierarchical H(curl) functions (hcurl.cpp; src/shapefunction/hierarchical/
 // hcurl/hcurl{line,triangle,quadrangle,tetrahedron,hexahedron}.cpp) for lines,

@@ -766,14 +766,19 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
         std::vector<int> eorient;   // folded: total orientation of every element
     };
     std::vector<Frag> frags;
-    // orientation folding maps per (type, order, integration order)
-    std::map<std::tuple<int, int, int>, H1Fold> folds;
-    auto fold_of = [&](int type, int order, int intorder) -> const H1Fold& {
-        auto key = std::make_tuple(type, order, intorder);
+    // orientation folding maps per (basis, type, order, integration order);
+    // basis 0 H1, 1 hcurl
+    std::map<std::tuple<int, int, int, int>, H1Fold> folds;
+    auto fold_of = [&](int basis, int type, int order, int intorder) -> const H1Fold& {
+        auto key = std::make_tuple(basis, type, order, intorder);
         auto it = folds.find(key);
-        if (it == folds.end()) it = folds.emplace(key, h1_fold(type, order, gauss_rule(type, intorder).coords)).first;
+        if (it == folds.end()) {
+            const std::vector<double>& pts = gauss_rule(type, intorder).coords;
+            it = folds.emplace(key, basis ? hcurl_fold(type, order, pts) : h1_fold(type, order, pts)).first;
+        }
         return it->second;
     };
+    auto basis_of = [](const std::shared_ptr<RawField>& f) { return f && f->hcurl() ? 1 : 0; };
     auto reads_fields = [](const Contribution& c) {
         std::function<bool(const OpP&)> walk = [&](const OpP& op) {
             if (op->kind == OpKind::Field) return true;
@@ -829,8 +834,8 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
                 // folded: the batches, in the same order, as one fragment on the
                 // orientation-0 tables (element matrices equal up to the signed
                 // renumbering, which the addresses and signs carry)
-                if (fold_ && !hc && !reads_fields(c) && fold_of(f.type, f.ptf, f.intorder).ok &&
-                    (!c.doffield || fold_of(f.type, f.pdof, f.intorder).ok)) {
+                if (fold_ && !reads_fields(c) && fold_of(basis_of(c.tffield), f.type, f.ptf, f.intorder).ok &&
+                    (!c.doffield || fold_of(basis_of(c.doffield), f.type, f.pdof, f.intorder).ok)) {
                     Frag fb = f;
                     fb.orient = 0;
                     fb.folded = true;
@@ -887,8 +892,8 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
     for (size_t i = 0; i < frags.size(); i++) frag_rank[i] = frags[i].m == 0 ? rank_rhs++ : rank_mat++;
     // folded fragment: renumber the [nff][n_elem] address table to the
     // orientation-0 functions and return the signs (same layout)
-    auto fold_addresses = [&](const Frag& f, int order, std::vector<int32_t>& addr) -> const int8_t* {
-        const H1Fold& F = fold_of(f.type, order, f.intorder);
+    auto fold_addresses = [&](const Frag& f, int basis, int order, std::vector<int32_t>& addr) -> const int8_t* {
+        const H1Fold& F = fold_of(basis, f.type, order, f.intorder);
         const int ne = (int)f.elems.size(), nff = (int)F.map.size();
         std::vector<int32_t> src = addr;
         ps->signs.emplace_back((size_t)nff * ne, (int8_t)1);
@@ -964,13 +969,15 @@ std::shared_ptr<PlanStorage> formulation::buildplan() {
             sd.tf_addr = ps->ints.back().data();
             sd.tf_sign = nullptr;
             sd.dof_sign = nullptr;
-            if (f.folded && h1_orientation_dependent(f.ptf)) sd.tf_sign = fold_addresses(f, f.ptf, ps->ints.back());
+            if (f.folded && (c.tffield->hcurl() || h1_orientation_dependent(f.ptf)))
+                sd.tf_sign = fold_addresses(f, basis_of(c.tffield), f.ptf, ps->ints.back());
             if (c.doffield) {
                 sd.dof_table = table_for(c.doffam, f.pdof);
                 sd.n_dof = element_nff(*c.doffield, f.type, f.pdof);
                 ps->ints.push_back(dofm_->getaddresses(c.doffield, f.pdof, f.type, f.elems, c.dofphysreg));
                 sd.dof_addr = ps->ints.back().data();
-                if (f.folded && h1_orientation_dependent(f.pdof)) sd.dof_sign = fold_addresses(f, f.pdof, ps->ints.back());
+                if (f.folded && (c.doffield->hcurl() || h1_orientation_dependent(f.pdof)))
+                    sd.dof_sign = fold_addresses(f, basis_of(c.doffield), f.pdof, ps->ints.back());
             } else {
                 sd.dof_table = -1;
                 sd.n_dof = 1;

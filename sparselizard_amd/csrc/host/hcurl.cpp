@@ -353,6 +353,78 @@ std::vector<FormFunction> hcurl_iterate(int type, int order) {
     return out;
 }
 
+H1Fold hcurl_fold(int type, int order, const std::vector<double>& pts) {
+    // the same search as h1_fold on the three components and their reference
+    // gradients: quadrangle and hexahedron functions under another edge / face
+    // orientation are signed functions of orientation 0 (Legendre parity,
+    // exchanged face axes); triangle faces are not, and the search fails
+    H1Fold F;
+    const std::vector<FormFunction> ffs = hcurl_iterate(type, order);
+    const int nff = (int)ffs.size(), np = (int)pts.size() / 3;
+    std::vector<D2> lam, sig;
+    auto values = [&](const FormFunction& ff, int orient, std::vector<double>& v) {
+        v.resize((size_t)12 * np);
+        for (int p = 0; p < np; p++) {
+            D2 x[3];
+            for (int d = 0; d < 3; d++) {
+                x[d] = D2(pts[3 * p + d]);
+                x[d].g[d] = 1.0;
+            }
+            lambdas(type, x, lam, sig);
+            const Vec r = eval_hcurl(type, order, ff, orient, x, lam, sig);
+            for (int c = 0; c < 3; c++) {
+                v[12 * (size_t)p + 4 * c] = r[c].v;
+                for (int d = 0; d < 3; d++) v[12 * (size_t)p + 4 * c + 1 + d] = r[c].g[d];
+            }
+        }
+    };
+    std::vector<std::vector<double>> v0(nff);
+    for (int f = 0; f < nff; f++)
+        if (ffs[f].dim <= 2) values(ffs[f], 0, v0[f]);
+    F.map.assign(nff, {});
+    F.dim.resize(nff);
+    F.num.resize(nff);
+    for (int f = 0; f < nff; f++) {
+        F.dim[f] = ffs[f].dim;
+        F.num[f] = ffs[f].num;
+    }
+    std::vector<double> a;
+    for (int f = 0; f < nff; f++) {
+        const FormFunction& ff = ffs[f];
+        const int nv = ff.dim == 1 ? 2 : (ff.dim == 2 ? (ff.assoc_type == TRI ? 6 : 8) : 1);
+        F.map[f].assign(nv, {f, 1});
+        for (int v = 1; v < nv; v++) {
+            values(ff, v, a);
+            double amax = 0.0;
+            for (double x : a) amax = std::max(amax, std::fabs(x));
+            const double tol = 1e-10 * std::max(1.0, amax);
+            int best = -1, bsign = 0, nmatch = 0;
+            for (int g = 0; g < nff; g++) {
+                if (ffs[g].dim != ff.dim || ffs[g].num != ff.num) continue;
+                for (int sg = 1; sg >= -1; sg -= 2) {
+                    double err = 0.0;
+                    for (size_t k = 0; k < a.size() && err <= tol; k++) err = std::max(err, std::fabs(a[k] - sg * v0[g][k]));
+                    if (err <= tol) {
+                        best = g;
+                        bsign = sg;
+                        nmatch++;
+                    }
+                }
+            }
+            if (nmatch != 1) return F;  // no (or no unique) signed match: not foldable
+            F.map[f][v] = {best, bsign};
+        }
+    }
+    for (int f = 0; f < nff; f++)
+        for (size_t v = 0; v < F.map[f].size(); v++)
+            for (int g = f + 1; g < nff; g++)
+                if (ffs[g].dim == ffs[f].dim && ffs[g].num == ffs[f].num && F.map[g].size() > v &&
+                    F.map[g][v].first == F.map[f][v].first)
+                    return F;
+    F.ok = true;
+    return F;
+}
+
 void hcurl_tables(int type, int order, int total_orientation, int family, const std::vector<double>& pts,
                   std::vector<double>& out) {
     SLX_CHECK(family == HCURL_VALUE || family == HCURL_CURL, "unknown H(curl) table family " << family);

@@ -215,6 +215,21 @@ def test_hcurl_maxwell_is_curl_and_mass_slices(elem, order):
     assert abs(A - ref).max() <= 1e-12 * abs(ref).max()
 
 
+@pytest.mark.parametrize("elem,order", [("quad", 3), ("hex", 2)])
+def test_hcurl_orientation_folding(elem, order):
+    # quadrangle and hexahedron functions fold to orientation 0 (signed
+    # renumbering): one bucket instead of one per orientation, same matrix;
+    # triangle faces (order >= 2) do not fold: tetrahedra keep their batches
+    spec = "elem=%s;n=%d;order=%d;term=maxwell" % (elem, 3 if elem == "quad" else 2, order)
+    pf, Af, bf = oracle_system("hcurlprojection", spec)
+    pu, Au, bu = oracle_system("hcurlprojection", spec + ";fold=0")
+    assert pf.info()["nbuckets"] == 1 and pu.info()["nbuckets"] > 4
+    assert abs(Af - Au).max() <= 1e-13 * abs(Au).max()
+    assert abs(bf - bu).max() <= 1e-13 * abs(bu).max()
+    pt, _, _ = oracle_system("hcurlprojection", "elem=tet;n=2;order=2;term=mass")
+    assert pt.info()["nbuckets"] > 1
+
+
 def test_hcurl_rejects_unsupported_uses():
     with pytest.raises(sa.SlxError):
         sa.Problem(None, "hcurlprojection", "elem=tri;n=2;order=-1;meshpipe=host")
@@ -223,7 +238,7 @@ def test_hcurl_rejects_unsupported_uses():
 # ---- GPU: HIP assembly against the oracle ----
 
 GPU_CASES = [("tri", 2, "maxwell"), ("tri", 0, "mass"), ("quad", 2, "maxwell"), ("tet", 2, "maxwell"),
-             ("tet", 3, "curl"), ("hex", 1, "maxwell"), ("hex", 2, "mass")]
+             ("tet", 3, "curl"), ("hex", 1, "maxwell"), ("hex", 2, "mass"), ("hex", 3, "maxwell")]
 
 
 @pytest.mark.gpu
