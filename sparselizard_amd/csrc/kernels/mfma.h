@@ -25,7 +25,7 @@ inline int mfma_words_per_pair(int arith) { return arith == AR_BF16 ? 2 : 1; }
 
 struct MfmaBucket {
     int32_t M, K;               // rows (n_tf * n_dof) and K = n_pairs * kq
-    int32_t kq;                 // K entries per derivative pair (n_gp, padded to 8 when compact)
+    int32_t kq;                 // K entries per derivative pair (n_gp; 8 when compact: the Gauss-point sum, 7 zero rows)
     int32_t n_tf, n_dof;
     int32_t mtiles, ksteps;     // ceil(M / 16), ceil(K / 32)
     int32_t n_pairs, nq;        // derivative pairs; 1 when the coefficients do not vary over Gauss points

@@ -92,11 +92,11 @@ __device__ inline void store_pad(uint32_t* dst, int np) {
 // lives in LDS as 32-bit words:
 //   compact  (coefficients constant over the Gauss points): the words of
 //            pair p (F16 one word hi | lo' << 16, BF16 two words, F32 the
-//            value).  The K index is k = p * kq + q with the Gauss points of
-//            a pair padded to kq = n_gp rounded up to 8 (zero rows of Phi), so
-//            the eight consecutive k of a lane (16x16x32) or the four k of
-//            one 16x16x4 step share one pair: the B fragment is a broadcast of
-//            one pair's words.  Pair n_pairs is 0 (K padding).
+//            value).  Phi holds each pair's Gauss-point sum in row k = 8 p
+//            (kq = 8; rows 8 p + 1 .. 8 p + 7 are zero), so the eight
+//            consecutive k of a lane (16x16x32) or the four k of one 16x16x4
+//            step share one pair: the B fragment is a broadcast of one pair's
+//            words.  Pair n_pairs is 0 (K padding).
 //   expanded otherwise (F16 only): word k = C[k / n_gp][k % n_gp] for k < K,
 //            0 up to the padded K, read as two 16-byte LDS loads per K step.
 // Output: the element rows of every segment group are assembled in LDS for

@@ -1024,9 +1024,12 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         if (!(t && std::atoi(t) != 0)) return;
     }
     // compact columns (coefficients constant over the Gauss points): the
-    // Gauss points of a pair are padded to a multiple of 8 in K
+    // Gauss-point sum of every derivative pair is taken here, in FP64
+    // (sum_q c w_q tf_i dof_j = c sum_q w_q tf_i dof_j), so a pair needs one K
+    // row; it keeps a block of 8 (the kernel broadcasts one pair's coefficient
+    // over the eight k of a lane) whose first row holds the sum
     mb.compact = mb.nq == 1 ? 1 : 0;
-    mb.kq = mb.compact ? ((bd.n_gp + 7) / 8) * 8 : bd.n_gp;
+    mb.kq = mb.compact ? 8 : bd.n_gp;
     mb.K = mb.n_pairs * mb.kq;
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
@@ -1099,21 +1102,22 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
             csrc[si * mb.n_pairs + p] = sd.terms[t].coef;
         }
     }
-    // Phi[m = i*n_dof + j][k = p*n_gp + q] = tf_r[i][q] w_q dof_s[j][q]
+    // Phi[m = i*n_dof + j][k = p*n_gp + q] = tf_r[i][q] w_q dof_s[j][q];
+    // compact: Phi[m][p*8] = sum over q of the same products
     const double* tft = bd.tables[tf_table];
     const double* dft = bd.tables[dof_table];
     int ng = bd.n_gp;
-    std::vector<double> phi((size_t)mb.M * mb.K);
-    double mx = 0.0;
+    std::vector<double> phi((size_t)mb.M * mb.K, 0.0);
     for (int i = 0; i < n_tf; i++)
         for (int j = 0; j < n_dof; j++)
             for (int p = 0; p < mb.n_pairs; p++)
                 for (int q = 0; q < ng; q++) {
                     double v = tft[((size_t)pairs[p].first * n_tf + i) * ng + q] * bd.gp_weights[q] *
                                dft[((size_t)pairs[p].second * n_dof + j) * ng + q];
-                    phi[((size_t)i * n_dof + j) * mb.K + p * mb.kq + q] = v;
-                    mx = std::max(mx, std::fabs(v));
+                    phi[((size_t)i * n_dof + j) * mb.K + p * mb.kq + (mb.compact ? 0 : q)] += v;
                 }
+    double mx = 0.0;
+    for (double v : phi) mx = std::max(mx, std::fabs(v));
     int ex = 0;
     if (mx > 0) std::frexp(mx, &ex);
     double sc = std::ldexp(1.0, -ex);
