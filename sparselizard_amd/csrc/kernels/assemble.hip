@@ -540,7 +540,20 @@ __global__ void reduce_rhs(DevReduce rd, const KT* __restrict__ ke, double* __re
     int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rd.nrows) return;
     double s = 0.0;
-    for (int64_t k = rd.rhs_erow_ptr[r]; k < rd.rhs_erow_ptr[r + 1]; k++) {
+    const int64_t k1 = rd.rhs_erow_ptr[r + 1];
+    int64_t k = rd.rhs_erow_ptr[r];
+    // four entries' loads in flight at once; the sum keeps the entry order
+    for (; k + 4 <= k1; k += 4) {
+        uint64_t pk[4];
+        double v[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) pk[u] = rd.rhs_erows[k + u];
+#pragma unroll
+        for (int u = 0; u < 4; u++) v[u] = (double)ke[(int64_t)(pk[u] & 0xFFFFFFFFFFFFULL)];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s += (pk[u] >> 63) ? -v[u] : v[u];
+    }
+    for (; k < k1; k++) {
         uint64_t pk = rd.rhs_erows[k];
         const double v = (double)ke[(int64_t)(pk & 0xFFFFFFFFFFFFULL)];
         s += (pk >> 63) ? -v : v;

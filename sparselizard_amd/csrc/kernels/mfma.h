@@ -54,6 +54,11 @@ struct MfmaBucket {
     // host side: the fragments of every arithmetic and which arithmetics fit
     const void* afrag_x[AR_COUNT];
     uint32_t arith_ok;          // bit a: arithmetic a has a kernel variant for this bucket
+    // presum: the F16 launch reads presummed columns (one K row per pair, kq =
+    // 1, 16 words per slice: a lane reads the words of eight pairs); ck_*: the
+    // compact layout (kq = 8) of the BF16 and F32 launches
+    int32_t presum;
+    int32_t ck_kq, ck_K, ck_ksteps;
 };
 
 // The launch description of one arithmetic (fragments, column strides, waves).

@@ -77,6 +77,21 @@ __device__ inline void store_pair(uint32_t* dst, int p, float x) {
     }
 }
 template <int AR>
+__device__ inline void store_pad(uint32_t* dst, int np);
+// Padding of a compact column: the zero pad pair; presummed F16 columns: zero
+// words up to 16 and, after the last slice of an element, 16 more zero words
+// (read by the lanes of k 16..31).
+template <int AR>
+__device__ inline void presum_pad(uint32_t* dst, int np, int presum, bool last) {
+    if (AR == AR_F16 && presum) {
+        for (int k = np; k < 16; k++) dst[k] = 0u;
+        if (last)
+            for (int k = 16; k < 32; k++) dst[k] = 0u;
+    } else {
+        store_pad<AR>(dst, np);
+    }
+}
+template <int AR>
 __device__ inline void store_pad(uint32_t* dst, int np) {
     if constexpr (AR == AR_BF16) {
         dst[2 * np] = 0u;
@@ -97,8 +112,15 @@ __device__ inline void store_pad(uint32_t* dst, int np) {
 //            consecutive k of a lane (16x16x32) or the four k of one 16x16x4
 //            step share one pair: the B fragment is a broadcast of one pair's
 //            words.  Pair n_pairs is 0 (K padding).
-//   expanded otherwise (F16 only): word k = C[k / n_gp][k % n_gp] for k < K,
-//            0 up to the padded K, read as two 16-byte LDS loads per K step.
+//   presum   (F16, compact columns whose Phi holds each pair's Gauss-point
+//            sum in row k = p, kq = 1): 16 words per slice, pairs 0..15, zero
+//            past the last pair; a lane's eight k are eight pairs, read as two
+//            16-byte LDS loads and split by byte permutes; one K step.  The
+//            lanes of k 16..31 read the next slice's words (or 16 zero words
+//            after the last slice) against zero rows of Phi.
+//   expanded otherwise (F16 only): word k = C[k / kq][k % kq] for k < K,
+//            0 up to the padded K, read as two 16-byte LDS loads per K step
+//            (kq = n_gp).
 // Output: the element rows of every segment group are assembled in LDS for
 // 16 elements and written with coalesced 16-byte stores (one contiguous
 // range per group).
@@ -119,7 +141,7 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
     constexpr int NPL = AR == AR_BF16 ? 3 : 2;  // A fragment planes
     extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
     constexpr int NE = 16;
-    const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq, ngp = b.n_gp;
+    const int ns = mb.n_mslices, np = mb.n_pairs, nq = mb.nq;
     const int estride = mb.estride, cstride = mb.cstride;
     const int n_tf = mb.n_tf;
     uint32_t* Csb = reinterpret_cast<uint32_t*>(smem_raw);                     // [2][NE][estride]
@@ -188,7 +210,32 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
             // pipelined: the MFMAs of slice s and the output stores of slice
             // s - 1 form one basic block (branch-free stores: rows past M go to
             // the sink), so the stores issue while the MFMA pipe is busy.
+            const bool presum = AR == AR_F16 && mb.presum;
             auto contract = [&](const uint32_t* colbase, float4v& hh, float4v& xx) {
+                if constexpr (AR == AR_F16) {
+                    if (presum) {
+                        // the words of pairs 8 lgrp .. 8 lgrp + 7 (one K step)
+                        const uint4v x0 = *reinterpret_cast<const uint4v*>(colbase + 8 * lgrp);
+                        const uint4v x1 = *reinterpret_cast<const uint4v*>(colbase + 8 * lgrp + 4);
+                        uint4v hw, lw;
+                        hw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x05040100u);
+                        hw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x05040100u);
+                        hw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x05040100u);
+                        hw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x05040100u);
+                        lw[0] = __builtin_amdgcn_perm(x0[1], x0[0], 0x07060302u);
+                        lw[1] = __builtin_amdgcn_perm(x0[3], x0[2], 0x07060302u);
+                        lw[2] = __builtin_amdgcn_perm(x1[1], x1[0], 0x07060302u);
+                        lw[3] = __builtin_amdgcn_perm(x1[3], x1[2], 0x07060302u);
+                        const half8 bhi = __builtin_bit_cast(half8, hw);
+                        const half8 blo = __builtin_bit_cast(half8, lw);
+                        const half8 ahi = __builtin_bit_cast(half8, Af[0][0][0]);
+                        const half8 alo = __builtin_bit_cast(half8, Af[0][0][1]);
+                        hh = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, bhi, float4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi, blo, float4v{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+                        xx = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo, bhi, xx, 0, 0, 0);
+                        return;
+                    }
+                }
                 uint32_t cw[MAXK][AR == AR_BF16 ? 2 : (AR == AR_F32 ? 4 : 1)];
 #pragma unroll
                 for (int ks = 0; ks < MAXK; ks++) {
@@ -448,7 +495,7 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
 #pragma unroll
                     for (int p = 0; p < SLX_MFMA_MAXP; p++)
                         if (p < np) store_pair<AR>(dst, p, cv[p] * sc);
-                    store_pad<AR>(dst, np);
+                    presum_pad<AR>(dst, np, mb.presum, s == ns - 1);
                 } else {
                     // Gauss-point values in batches of 8 independent loads
                     for (int p = 0; p < np; p++) {
@@ -464,19 +511,20 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
                     if (mx > 0.f) (void)frexpf(mx, &ex);  // mx in [2^(ex-1), 2^ex)
                     const float sc = ldexpf(1.f, -ex);
                     colsc[s * NE + el] = ldexpf(1.f, ex) * mb.a_unscale;
+                    const int kq = mb.kq;  // the Gauss points of a pair
                     for (int p = 0; p < np; p++) {
                         const int c = csl[s * np + p];
-                        for (int q0 = 0; q0 < ngp; q0 += 8) {
+                        for (int q0 = 0; q0 < kq; q0 += 8) {
 #pragma unroll
                             for (int u = 0; u < 8; u++) {
                                 const int q = q0 + u;
-                                if (q >= ngp) break;
+                                if (q >= kq) break;
                                 const float x = (c >= 0 && valid) ? ctab[((int64_t)c * nq + (nq == 1 ? 0 : q)) * nel + e] * sc : 0.f;
-                                dst[p * ngp + q] = pack_split(x);
+                                dst[p * kq + q] = pack_split(x);
                             }
                         }
                     }
-                    for (int k = np * ngp; k < cstride; k++) dst[k] = 0u;
+                    for (int k = np * kq; k < cstride; k++) dst[k] = 0u;
                 }
             }
         };
@@ -493,14 +541,24 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
                     float4v* d4 = reinterpret_cast<float4v*>(dst);
                     int eo = pt / per, w = pt - eo * per;
                     const int step_e = write_threads / per, step_w = write_threads - step_e * per;
-                    for (int x = pt; x < nel_here * per; x += write_threads) {
-                        d4[x] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
-                        eo += step_e;
-                        w += step_w;
-                        if (w >= per) {
-                            w -= per;
-                            eo++;
+                    const int nx = nel_here * per;
+                    // four LDS reads in flight before their four stores
+                    constexpr int WU = 4;
+                    for (int x = pt; x < nx; x += WU * write_threads) {
+                        float4v v[WU];
+#pragma unroll
+                        for (int u = 0; u < WU; u++) {
+                            if (x + u * write_threads < nx) v[u] = *reinterpret_cast<const float4v*>(gsrc + (size_t)eo * srow + 4 * w);
+                            eo += step_e;
+                            w += step_w;
+                            if (w >= per) {
+                                w -= per;
+                                eo++;
+                            }
                         }
+#pragma unroll
+                        for (int u = 0; u < WU; u++)
+                            if (x + u * write_threads < nx) d4[x + u * write_threads] = v[u];
                     }
                 } else {
                     for (int x = pt; x < nel_here * erow; x += write_threads) {
@@ -541,7 +599,7 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
 #pragma unroll
             for (int p = 0; p < SLX_MFMA_MAXP; p++)
                 if (p < np) store_pair<AR>(dst, p, cvn[p] * sc);
-            store_pad<AR>(dst, np);
+            presum_pad<AR>(dst, np, mb.presum, s == ns - 1);
         };
         const bool pipelined = COMPACT && NE * ns <= prep_threads;
         auto tile_of = [&](int st) { return (int)blockIdx.x + st * (int)gridDim.x; };
@@ -631,6 +689,14 @@ MfmaBucket mfma_for(const MfmaBucket& mb0, int arith) {
     mb.afrag = mb0.afrag_x[arith];
     if (arith != AR_F16) {
         // BF16 / F32: compact columns only, one M tile per wave, one slice per group
+        if (mb0.presum) {
+            // the F16 launch reads presummed columns; these take the compact
+            // layout (eight K rows per pair)
+            mb.presum = 0;
+            mb.kq = mb0.ck_kq;
+            mb.K = mb0.ck_K;
+            mb.ksteps = mb.maxk = mb0.ck_ksteps;
+        }
         mb.nw = mb.mtiles;
         mb.mtw = 1;
         mb.sgroup = 1;

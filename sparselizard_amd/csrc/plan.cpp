@@ -1102,61 +1102,111 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
             csrc[si * mb.n_pairs + p] = sd.terms[t].coef;
         }
     }
-    // Phi[m = i*n_dof + j][k = p*n_gp + q] = tf_r[i][q] w_q dof_s[j][q];
-    // compact: Phi[m][p*8] = sum over q of the same products
+    // Phi[m = i*n_dof + j][k = p*kq + q] = tf_r[i][q] w_q dof_s[j][q]; with
+    // constant coefficients the Gauss-point sum of a pair sits in row p*kq
+    // (kq = 8 compact, kq = 1 expanded) and the pair's other rows are zero
     const double* tft = bd.tables[tf_table];
     const double* dft = bd.tables[dof_table];
-    int ng = bd.n_gp;
-    std::vector<double> phi((size_t)mb.M * mb.K, 0.0);
+    const int ng = bd.n_gp;
+    const bool presum = mb.nq == 1;
+    std::vector<double> psum;  // [m][p] Gauss-point sums, or [m][p][q] products
+    const int qcols = presum ? 1 : ng;
+    psum.assign((size_t)mb.M * mb.n_pairs * qcols, 0.0);
     for (int i = 0; i < n_tf; i++)
         for (int j = 0; j < n_dof; j++)
             for (int p = 0; p < mb.n_pairs; p++)
                 for (int q = 0; q < ng; q++) {
                     double v = tft[((size_t)pairs[p].first * n_tf + i) * ng + q] * bd.gp_weights[q] *
                                dft[((size_t)pairs[p].second * n_dof + j) * ng + q];
-                    phi[((size_t)i * n_dof + j) * mb.K + p * mb.kq + (mb.compact ? 0 : q)] += v;
+                    psum[(((size_t)i * n_dof + j) * mb.n_pairs + p) * qcols + (presum ? 0 : q)] += v;
                 }
     double mx = 0.0;
-    for (double v : phi) mx = std::max(mx, std::fabs(v));
+    for (double v : psum) mx = std::max(mx, std::fabs(v));
     int ex = 0;
     if (mx > 0) std::frexp(mx, &ex);
-    double sc = std::ldexp(1.0, -ex);
+    const double sc = std::ldexp(1.0, -ex);
     mb.a_unscale = (float)std::ldexp(1.0, ex);
-    auto phix = [&](int m, int k) { return (m < mb.M && k < mb.K) ? (float)(phi[(size_t)m * mb.K + k] * sc) : 0.f; };
-    // fragments [mt][ks][plane][lane] x 16 bytes.  16x16x32 (F16, BF16): lane
-    // l holds k = 8 (l / 16) + v of row l % 16; 16x16x4 (F32): plane pl, entry
-    // v is step j = 4 pl + v, k = 4 j + l / 16
-    const size_t nfr = (size_t)mb.mtiles * mb.ksteps * 64;
-    std::vector<uint16_t> frag(nfr * 2 * 8, 0), fragb(nfr * 3 * 8, 0);
-    std::vector<float> fragf(nfr * 2 * 4, 0.f);
-    for (int mt = 0; mt < mb.mtiles; mt++)
-        for (int ks = 0; ks < mb.ksteps; ks++)
-            for (int lane = 0; lane < 64; lane++) {
-                const int m = mt * 16 + lane % 16;
-                const size_t w = (size_t)(mt * mb.ksteps + ks);
-                for (int v = 0; v < 8; v++) {
-                    const float x = phix(m, ks * 32 + 8 * (lane / 16) + v);
-                    uint16_t hi, mid, lo;
-                    split_f16(x, hi, lo);
-                    frag[((w * 2 + 0) * 64 + lane) * 8 + v] = hi;
-                    frag[((w * 2 + 1) * 64 + lane) * 8 + v] = lo;
-                    split_bf16(x, hi, mid, lo);
-                    fragb[((w * 3 + 0) * 64 + lane) * 8 + v] = hi;
-                    fragb[((w * 3 + 1) * 64 + lane) * 8 + v] = mid;
-                    fragb[((w * 3 + 2) * 64 + lane) * 8 + v] = lo;
+    // fragments [mt][ks][plane][lane] x 16 bytes of the K layout (kq, K,
+    // ksteps).  16x16x32 (F16, BF16): lane l holds k = 8 (l / 16) + v of row
+    // l % 16; 16x16x4 (F32): plane pl, entry v is step j = 4 pl + v, k = 4 j + l / 16
+    auto make_frags = [&](int kq, int K, int ksteps, bool f16, bool others) {
+        auto phix = [&](int m, int k) -> float {
+            if (m >= mb.M || k >= K) return 0.f;
+            const int p = k / kq, q = k % kq;
+            if (presum) return q == 0 ? (float)(psum[(size_t)m * mb.n_pairs + p] * sc) : 0.f;
+            return (float)(psum[((size_t)m * mb.n_pairs + p) * qcols + q] * sc);
+        };
+        const size_t nfr = (size_t)mb.mtiles * ksteps * 64;
+        std::vector<uint16_t> frag(f16 ? nfr * 2 * 8 : 0, 0), fragb(others ? nfr * 3 * 8 : 0, 0);
+        std::vector<float> fragf(others ? nfr * 2 * 4 : 0, 0.f);
+        for (int mt = 0; mt < mb.mtiles; mt++)
+            for (int ks = 0; ks < ksteps; ks++)
+                for (int lane = 0; lane < 64; lane++) {
+                    const int m = mt * 16 + lane % 16;
+                    const size_t w = (size_t)(mt * ksteps + ks);
+                    for (int v = 0; v < 8; v++) {
+                        const float x = phix(m, ks * 32 + 8 * (lane / 16) + v);
+                        uint16_t hi, mid, lo;
+                        if (f16) {
+                            split_f16(x, hi, lo);
+                            frag[((w * 2 + 0) * 64 + lane) * 8 + v] = hi;
+                            frag[((w * 2 + 1) * 64 + lane) * 8 + v] = lo;
+                        }
+                        if (others) {
+                            split_bf16(x, hi, mid, lo);
+                            fragb[((w * 3 + 0) * 64 + lane) * 8 + v] = hi;
+                            fragb[((w * 3 + 1) * 64 + lane) * 8 + v] = mid;
+                            fragb[((w * 3 + 2) * 64 + lane) * 8 + v] = lo;
+                        }
+                    }
+                    if (others)
+                        for (int pl = 0; pl < 2; pl++)
+                            for (int v = 0; v < 4; v++)
+                                fragf[((w * 2 + pl) * 64 + lane) * 4 + v] = phix(m, ks * 32 + 4 * (4 * pl + v) + lane / 16);
                 }
-                for (int pl = 0; pl < 2; pl++)
-                    for (int v = 0; v < 4; v++) fragf[((w * 2 + pl) * 64 + lane) * 4 + v] = phix(m, ks * 32 + 4 * (4 * pl + v) + lane / 16);
+        if (f16) mb.afrag_x[AR_F16] = P.dev(frag.data(), frag.size());
+        if (others) {
+            mb.afrag_x[AR_BF16] = P.dev(fragb.data(), fragb.size());
+            mb.afrag_x[AR_F32] = P.dev(fragf.data(), fragf.size());
+        }
+    };
+    make_frags(mb.kq, mb.K, mb.ksteps, true, true);
+    mb.ck_kq = mb.kq;
+    mb.ck_K = mb.K;
+    mb.ck_ksteps = mb.ksteps;
+    // F16 with constant coefficients: presummed columns with one K row per
+    // pair (kq = 1; a lane reads the words of eight pairs, so up to 16 pairs
+    // take one K step instead of one per four pairs), when that layout fits
+    // the LDS; BF16 and F32 keep the compact layout above.
+    // SLX_TUNE_MFMA_PRESUM=0 keeps F16 on the compact layout (tuning knob).
+    bool try_expanded = presum && mb.sgroup == 1;
+    if (const char* t = std::getenv("SLX_TUNE_MFMA_PRESUM")) try_expanded = try_expanded && std::atoi(t) != 0;
+    mb.presum = 0;
+    if (try_expanded) {
+        MfmaBucket e = mb;
+        e.presum = 1;
+        e.kq = 1;
+        e.K = e.n_pairs;
+        e.ksteps = 1;  // at most 16 pairs
+        // the software-pipelined consumer (one M tile per wave, one slice at a time)
+        if (mfma_variant_fits(e.mtiles, e.ksteps, &e.nw, &e.mtw, &e.maxk, prefer_mtw) && e.mtw == 1) {
+            // 16 words per slice; the lanes of k 16..31 read the next slice's
+            // words (or the 16 zero words after the last slice), against zero
+            // rows of Phi
+            e.cstride = 16;
+            e.estride = ((e.n_mslices * e.cstride + 12 + 63) / 64) * 64 + 4;  // = 4 (mod 64), >= ns * 16 + 16
+            if (mfma_lds_bytes(e) <= 160 * 1024) {
+                mb = e;
+                make_frags(mb.kq, mb.K, mb.ksteps, true, false);
             }
-    mb.afrag_x[AR_F16] = P.dev(frag.data(), frag.size());
-    mb.afrag_x[AR_BF16] = P.dev(fragb.data(), fragb.size());
-    mb.afrag_x[AR_F32] = P.dev(fragf.data(), fragf.size());
+        }
+    }
     mb.afrag = mb.afrag_x[AR_F16];
     mb.arith = AR_F16;
     // arithmetics with a kernel variant: F16 always; BF16 and F32 take compact
     // columns with one M tile per wave, within the LDS of one CU
     mb.arith_ok = 1u << AR_F16;
-    if (mb.compact && mb.mtiles <= 8)
+    if (presum && mb.mtiles <= 8)
         for (int a : {AR_BF16, AR_F32})
             if (mfma_lds_bytes(mfma_for(mb, a)) <= 160 * 1024) mb.arith_ok |= 1u << a;
     mb.csrc = P.dev(csrc.data(), csrc.size());
