@@ -5,8 +5,9 @@
 
 #define SLX_MFMA_MAXP 16   // derivative pairs of a batched-MFMA bucket
 #define SLX_MFMA_PREP_WAVES 3   // producer waves of the batched-MFMA workgroup that prepare columns
-#define SLX_MFMA_WRITE_WAVES 2  // producer waves that write element matrices out
-#define SLX_MFMA_PWAVES (SLX_MFMA_PREP_WAVES + SLX_MFMA_WRITE_WAVES)
+// producer waves that write element matrices out: the rest of a 16-wave
+// workgroup when the consumers' A fragments are small (at least 5), else 2
+#define SLX_MFMA_MIN_WRITE_WAVES 2
 
 namespace slx {
 

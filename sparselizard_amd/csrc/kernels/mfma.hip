@@ -128,9 +128,11 @@ __device__ inline void store_pad(uint32_t* dst, int np) {
 // Work split (one workgroup per CU, persistent over 16-element tiles):
 //   consumer waves (one or two M tiles each) run the contraction of tile s
 //     from the column buffer Cs[s & 1] into the staging buffer stg[s & 1];
-//   producer waves (SLX_MFMA_PWAVES, in two groups) meanwhile prepare the
-//     columns of tile s + 1 into Cs[(s + 1) & 1] and write tile s - 1 from
-//     stg[(s - 1) & 1] to the element matrices with coalesced 16-byte stores.
+//   producer waves (in two groups) meanwhile prepare the columns of tile
+//     s + 1 into Cs[(s + 1) & 1] (SLX_MFMA_PREP_WAVES) and write tile s - 1
+//     from stg[(s - 1) & 1] to the element matrices with coalesced 16-byte
+//     stores (the remaining waves of the workgroup: the write-out is the
+//     longest of the three tasks, so small-fragment variants fill 16 waves).
 // One barrier per tile; every buffer is written one tile after its last read.
 // WAVES: the largest workgroup of the variant, in waves (12 leaves 168 VGPRs
 // per wave: up to 7 consumer waves; 16 leaves 128).
@@ -297,11 +299,11 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
                 const int soff = sprmc[3 * s], ld = sprmc[3 * s + 1], ssrow = sprmc[3 * s + 2];
                 const float colscale = colsc[s * NE + lrow];
                 const float xscale = AR == AR_F16 ? colscale * (1.f / 2048.f) : colscale;
-                float* out = stg + soff + lrow * ssrow;
+                float* out = stg + soff + __umul24(lrow, ssrow);  // 24-bit products: full-rate, factors < 2^24
 #pragma unroll
                 for (int v = 0; v < 4; v++) {
                     const float val = AR == AR_F32 ? hh[v] * colscale : hh[v] * colscale + xx[v] * xscale;
-                    float* dst = rowok[0][v] ? out + irow[0][v] * ld + jrow[0][v] : trash + lane;
+                    float* dst = rowok[0][v] ? out + __umul24(irow[0][v], ld) + jrow[0][v] : trash + lane;
                     *dst = val;
                 }
             };
@@ -440,7 +442,7 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
                     const int soff = sprmc[3 * s], ld = sprmc[3 * s + 1], ssrow = sprmc[3 * s + 2];
                     const float colscale = colsc[s * NE + lrow];
                     const float xscale = AR == AR_F16 ? colscale * (1.f / 2048.f) : colscale;
-                    float* out = stg + soff + lrow * ssrow;
+                    float* out = stg + soff + __umul24(lrow, ssrow);
 #pragma unroll
                     for (int i = 0; i < MTW; i++) {
                         const int mt = wave + nw * i;
@@ -452,7 +454,7 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
                             if (!full && !rowok[i][v]) continue;
                             const float val = AR == AR_F32 ? acc_hh[c][i][v] * colscale
                                                            : acc_hh[c][i][v] * colscale + acc_x[c][i][v] * xscale;
-                            out[irow[i][v] * ld + jrow[i][v]] = val;
+                            out[__umul24(irow[i][v], ld) + jrow[i][v]] = val;
                         }
                     }
                 }
@@ -462,7 +464,8 @@ __global__ void __launch_bounds__(64 * WAVES) mfma_contract_kernel(DevBucket b, 
     } else {
         // ------------------------------------------------------- producers --
         // two producer groups: the first prepares columns, the second writes out
-        constexpr int prep_threads = 64 * SLX_MFMA_PREP_WAVES, write_threads = 64 * SLX_MFMA_WRITE_WAVES;
+        constexpr int prep_threads = 64 * SLX_MFMA_PREP_WAVES;
+        const int write_threads = (int)blockDim.x - 64 * nw - prep_threads;
         const int pt0 = threadIdx.x - 64 * nw;
         const bool is_prep = pt0 < prep_threads;  // wave-uniform
         const int pt = is_prep ? pt0 : pt0 - prep_threads;
@@ -648,7 +651,12 @@ static hipError_t launch_variant(const DevBucket& b, const MfmaBucket& mb, const
     const size_t lds = mfma_lds_bytes(mb);
     const int tiles = (b.n_elem + 15) / 16;
     const int grid = tiles < mb.max_blocks ? tiles : mb.max_blocks;
-    const dim3 gd(grid), bd(64 * (mb.nw + SLX_MFMA_PWAVES));
+    // F16 with small A fragments (at most 32 VGPRs): a 16-wave workgroup,
+    // the waves beyond consumers and column producers write out (measured:
+    // 2.00 -> 1.84 ms at 64^3 P2 elasticity; FP32 MFMA is slower that way);
+    // else two writers
+    const int waves = AR == AR_F16 && MTW * MAXK <= 4 ? 16 : mb.nw + SLX_MFMA_PREP_WAVES + SLX_MFMA_MIN_WRITE_WAVES;
+    const dim3 gd(grid), bd(64 * waves);
     if constexpr (AR != AR_F16) {
         // compact columns, one M tile per wave, one slice at a time
         if (!mb.compact || MTW != 1) return hipErrorInvalidValue;
