@@ -1029,7 +1029,12 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     // row; it keeps a block of 8 (the kernel broadcasts one pair's coefficient
     // over the eight k of a lane) whose first row holds the sum
     mb.compact = mb.nq == 1 ? 1 : 0;
-    mb.kq = mb.compact ? 8 : bd.n_gp;
+    // SLX_MFMA_GAUSS_SUMS=0 keeps one K row per Gauss point in compact buckets
+    // (the per-Gauss-point contraction of the FP64 path, padded to a multiple
+    // of 8 per pair): a like-for-like comparison knob
+    bool gauss_sums = true;
+    if (const char* t = std::getenv("SLX_MFMA_GAUSS_SUMS")) gauss_sums = std::atoi(t) != 0;
+    mb.kq = mb.compact ? (gauss_sums ? 8 : ((bd.n_gp + 7) / 8) * 8) : bd.n_gp;
     mb.K = mb.n_pairs * mb.kq;
     mb.mtiles = (mb.M + 15) / 16;
     mb.ksteps = (mb.K + 31) / 32;
@@ -1108,7 +1113,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     const double* tft = bd.tables[tf_table];
     const double* dft = bd.tables[dof_table];
     const int ng = bd.n_gp;
-    const bool presum = mb.nq == 1;
+    const bool presum = mb.nq == 1 && gauss_sums;
     std::vector<double> psum;  // [m][p] Gauss-point sums, or [m][p][q] products
     const int qcols = presum ? 1 : ng;
     psum.assign((size_t)mb.M * mb.n_pairs * qcols, 0.0);
@@ -1134,6 +1139,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
             if (m >= mb.M || k >= K) return 0.f;
             const int p = k / kq, q = k % kq;
             if (presum) return q == 0 ? (float)(psum[(size_t)m * mb.n_pairs + p] * sc) : 0.f;
+            if (q >= qcols) return 0.f;  // Gauss points padded to the pair's K block
             return (float)(psum[((size_t)m * mb.n_pairs + p) * qcols + q] * sc);
         };
         const size_t nfr = (size_t)mb.mtiles * ksteps * 64;
@@ -1206,7 +1212,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     // arithmetics with a kernel variant: F16 always; BF16 and F32 take compact
     // columns with one M tile per wave, within the LDS of one CU
     mb.arith_ok = 1u << AR_F16;
-    if (presum && mb.mtiles <= 8)
+    if (mb.compact && mb.mtiles <= 8)
         for (int a : {AR_BF16, AR_F32})
             if (mfma_lds_bytes(mfma_for(mb, a)) <= 160 * 1024) mb.arith_ok |= 1u << a;
     mb.csrc = P.dev(csrc.data(), csrc.size());
