@@ -20,10 +20,20 @@
 
 namespace slx {
 
+// K order of a k-step (32 entries): 32 / segw segments of segw entries, segw
+// = 16 for the variants with two column tiles per wave and 8 for the one-tile
+// variant (one producer thread per column and segment either way); a segment
+// holds segw / na whole Gauss points, entry r of segment s being Gauss point
+// ks * qb + s * (segw / na) + r / na with tf derivative r % na (entries past
+// the segment's last whole Gauss point are zero).
+__host__ __device__ constexpr int factor_segw(int bn) { return bn >= 2 ? 16 : 8; }
+__host__ __device__ constexpr int factor_qb(int na, int bn) { return (32 / factor_segw(bn)) * (factor_segw(bn) / na); }
+
 struct FactorSlice {
     int32_t n_tf, n_dof, ngp, nq;   // nq: coefficient points per element (1 when q-invariant)
-    int32_t na, qb, nks;            // tf derivatives, Gauss points per k-step (qb * na <= 32), k-steps
+    int32_t na, qb, nks;            // tf derivatives, Gauss points per k-step (factor_qb), k-steps
     int32_t mt, nt, ntp;            // 16-row / 16-column tiles, padded dof columns (nt * 16)
+    int32_t ntq;                    // row stride of tq: dof columns padded to a multiple of 128
     int32_t n_terms, bmask;         // terms; dof derivatives in use (bit b)
     int32_t bm, bn;                 // tiles per wave (rows, columns): waves 2 x 4
     // tinfo: [n_terms] tf-derivative index (into the na derivatives), then
@@ -33,7 +43,7 @@ struct FactorSlice {
     float tqmax;                    // max |tq|
     int32_t arith;                  // MfmaArith of this launch
     const uint32_t* afrag;          // [nks][mt][planes][64][4] words: fragments of the scaled P in this launch's arithmetic
-    const float* tq;                // [ngp][ntp][4]: w_q dof_b[j][q], zero padded columns
+    const float* tq;                // [nks * qb][ntq][4]: w_q dof_b[j][q], zero padded rows and columns
     int64_t ke_base, ke_ld;
     const uint32_t* afrag_x[3];     // host side: the fragments of every arithmetic
     uint32_t arith_ok;              // bit a: arithmetic a fits (LDS, P chunk staging)

@@ -4,8 +4,9 @@
 // waves (a 2 x 4 grid over the 16x16 output tiles) run the MFMAs while 4
 // producer waves stage the next k-step.
 // The K dimension runs over (Gauss point, tf derivative) pairs in k-steps of
-// 32: a k-step holds qb Gauss points times the na tf derivatives, so the dof
-// table values of a Gauss point are read once per element.  Per k-step:
+// 32: a k-step holds qb Gauss points times the na tf derivatives, grouped in
+// segments of whole Gauss points (factor_segw), so the dof table values of a
+// Gauss point are read once per element and pass.  Per k-step:
 //   A chunk (shared P fragments, precomputed on the host) -> LDS
 //   B chunk (Q_e for the element, built from the element's coefficients in
 //            LDS and the dof table) -> LDS
@@ -31,7 +32,7 @@ typedef unsigned int uint4v __attribute__((ext_vector_type(4)));
 
 size_t factor_lds_bytes(const FactorSlice& f, int arith) {
     const size_t npl = (size_t)mfma_planes(arith);
-    return (size_t)2 * f.mt * npl * 64 * 16 + (size_t)2 * (4 * f.bn) * npl * 64 * 16 + (size_t)f.ngp * 16 * 4 + 16;
+    return (size_t)2 * f.mt * npl * 64 * 16 + (size_t)2 * (4 * f.bn) * npl * 64 * 16 + (size_t)f.nks * f.qb * 16 * 4 + 16;
 }
 
 // tile blocks per wave: rows over 2 waves, columns over 4 waves; the column
@@ -54,6 +55,119 @@ bool factor_variant_fits(int mt, int nt, int* bm, int* bn, int variant) {
     return false;
 }
 
+// Producer waves of factor_element: one k-step ahead of the consumers, stage
+// the shared P chunk and build the Q_e chunk of the element.  Thread bt takes
+// column bc = bt % CW of the pass and segment sg = bt / CW of the k-step
+// (factor_segw): entries k = SEGW sg + r, r = NA qi + ai, are Gauss point
+// ks QB + sg QH + qi with tf derivative ai.  The segment's QH dof-table values
+// (a float4 of the four derivatives per Gauss point) are loaded one k-step
+// ahead; each is combined with the NA coefficient rows Ms[q][ai] of its Gauss
+// point, and the entries are packed eight at a time into 16-byte fragment
+// words (F16 / BF16: lane 16 (k / 8) + j % 16 of column tile j / 16, element
+// k % 8) or written as floats (F32: Q[k][column] per tile).  The table and Ms
+// are zero past the last Gauss point and the table past the last dof column,
+// so the entry positions are compile-time constants and no entry is checked.
+template <int AR, int BM, int BN, int NA>
+__device__ __forceinline__ void factor_producer(const FactorSlice& f, uint4v* As, uint4v* Bs, const float* Ms, int bt) {
+    constexpr int NPL = AR == AR_BF16 ? 3 : 2;
+    constexpr int NTB = 4 * BN, BCH = NTB * NPL * 64, CW = 16 * NTB;
+    constexpr int SEGW = factor_segw(BN), QH = SEGW / NA, QB = factor_qb(NA, BN), NGS = SEGW / 8;
+    static_assert(CW * (32 / SEGW) == 64 * SLX_FK_BWAVES, "one producer thread per (column, segment)");
+    static_assert(QH >= 1, "a segment holds at least one Gauss point");
+    const int nks = f.nks, ntq = f.ntq;
+    const int achunk = f.mt * NPL * 64;  // a multiple of 64: a wave's words are all in or all out
+    const int npass = (f.nt + NTB - 1) / NTB;
+    const int wp = __builtin_amdgcn_readfirstlane(bt >> 6), lane = bt & 63;
+    const int sg = __builtin_amdgcn_readfirstlane(bt / CW), bc = bt % CW;
+    const float4v* tq4 = reinterpret_cast<const float4v*>(f.tq);
+    const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
+    float4v hq[QH];
+    auto load_h = [&](int ks, int cb) {
+        const float4v* row = tq4 + ((ks * QB + sg * QH) * ntq + cb * 16 + bc);
+#pragma unroll
+        for (int qi = 0; qi < QH; qi++) hq[qi] = row[qi * ntq];
+    };
+    auto produce = [&](int ks, int buf, int cb) {
+        // the shared P chunk goes straight from global memory to LDS
+        // (global_load_lds: lane i of a wave lands at the wave's base + 16 i),
+        // in flight while Q_e is built
+#pragma unroll
+        for (int u = 0; u < SLX_FK_AREG; u++) {
+            const int x0 = 64 * wp + u * 64 * SLX_FK_BWAVES;  // wave-uniform first entry
+            if (x0 < achunk) {
+                const uint4v* src = afrag + ((size_t)ks * achunk + x0);
+                __builtin_amdgcn_global_load_lds((const void*)(src + lane), (__attribute__((address_space(3))) void*)(As + buf * achunk + x0),
+                                                 16, 0, 0);
+            }
+        }
+        const float* mrow = Ms + (ks * QB + sg * QH) * 16;  // the segment's coefficient rows (LDS broadcast)
+        float vals[SEGW];
+#pragma unroll
+        for (int r = 0; r < SEGW; r++) {
+            const int qi = r / NA, ai = r % NA;
+            if (qi < QH) {
+                const float4v m = *reinterpret_cast<const float4v*>(mrow + (qi * 4 + ai) * 4);
+                const float4v h = hq[qi];
+                vals[r] = m[0] * h[0] + m[1] * h[1] + m[2] * h[2] + m[3] * h[3];
+            } else {
+                vals[r] = 0.f;
+            }
+        }
+        uint4v* colw = Bs + (size_t)buf * BCH + (size_t)(bc >> 4) * NPL * 64 + (bc & 15);
+        if constexpr (AR == AR_F16) {
+#pragma unroll
+            for (int g = 0; g < NGS; g++) {
+                half8 hv, lv;
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    const _Float16 hi = (_Float16)vals[8 * g + v];
+                    hv[v] = hi;
+                    lv[v] = (_Float16)(vals[8 * g + v] - (float)hi);
+                }
+                const int kg = sg * NGS + g;
+                colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
+                colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+            }
+        } else if constexpr (AR == AR_BF16) {
+#pragma unroll
+            for (int g = 0; g < NGS; g++) {
+                bf16x8 hv, mv, lv;
+#pragma unroll
+                for (int v = 0; v < 8; v++) {
+                    const float x = vals[8 * g + v];
+                    const __bf16 hi = (__bf16)x;
+                    const float r1 = x - (float)hi;
+                    const __bf16 mid = (__bf16)r1;
+                    hv[v] = hi;
+                    mv[v] = mid;
+                    lv[v] = (__bf16)(r1 - (float)mid);
+                }
+                const int kg = sg * NGS + g;
+                colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
+                colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, mv);
+                colw[128 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
+            }
+        } else {
+            float* colf = reinterpret_cast<float*>(Bs + (size_t)buf * BCH) + (size_t)(bc >> 4) * 512 + (bc & 15);
+#pragma unroll
+            for (int r = 0; r < SEGW; r++) colf[(SEGW * sg + r) * 16] = vals[r];
+        }
+        // the P chunk is in LDS before this wave reaches the barrier; the
+        // next k-step's table values are requested after that wait
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (ks + 1 < nks) load_h(ks + 1, cb);
+    };
+    for (int pass = 0; pass < npass; pass++) {
+        load_h(0, pass * NTB);
+        produce(0, 0, pass * NTB);
+        __syncthreads();  // k-step 0 staged
+        for (int ks = 0; ks < nks; ks++) {
+            if (ks + 1 < nks) produce(ks + 1, (ks & 1) ^ 1, pass * NTB);
+            __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
+        }
+    }
+}
+
 // LDS per k-step and buffer: A chunk [MT][NPL][64] 16-byte words; B chunk
 // [NTB][NPL][64] words (F16 / BF16: lane l of column tile t holds the eight
 // k = 8 (l / 16) + 0..7 of column l % 16, one plane per piece; F32: the tile
@@ -63,20 +177,19 @@ template <int AR, int BM, int BN>
 __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem, int e, const float* __restrict__ ctab,
                                                float* __restrict__ ke, unsigned char* smem_raw) {
     constexpr int NPL = AR == AR_BF16 ? 3 : 2;                  // 16-byte planes per lane
-    const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, qb = f.qb, ntp = f.ntp;
+    const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, nqp = f.nks * f.qb;
     constexpr int NTB = 4 * BN;                                 // column tiles per pass
     constexpr int BCH = NTB * NPL * 64;                         // B chunk words
     uint4v* As = reinterpret_cast<uint4v*>(smem_raw);           // [2][MT][NPL][64]
     uint4v* Bs = As + 2 * MT * NPL * 64;                        // [2][NTB][NPL][64]
-    float* Ms = reinterpret_cast<float*>(Bs + 2 * BCH);         // [ngp][4 tf derivative][4 dof derivative]
-    unsigned* smax = reinterpret_cast<unsigned*>(Ms + ngp * 16);
+    float* Ms = reinterpret_cast<float*>(Bs + 2 * BCH);         // [nqp][4 tf derivative][4 dof derivative]
+    unsigned* smax = reinterpret_cast<unsigned*>(Ms + nqp * 16);
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool mfma_wave = w < SLX_FK_WAVES;
     const int32_t* t_ai = f.tinfo;
     const int32_t* t_b = f.tinfo + f.n_terms;
     const int32_t* t_slot = f.tinfo + 2 * f.n_terms;
     const int64_t stride = (int64_t)nq * n_elem;
-    const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
     const int achunk = MT * NPL * 64;
     const int npass = (NT + NTB - 1) / NTB;
 
@@ -84,12 +197,14 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
         if (tid == 0) *smax = 0u;
         __syncthreads();  // the previous element is done with Ms, As, Bs
         // the element's coefficient matrices per Gauss point:
-        //   Ms[q][ai][b] = sum of C_t(e, q) over terms with derivatives (alist[ai], b)
+        //   Ms[q][ai][b] = sum of C_t(e, q) over terms with derivatives (alist[ai], b),
+        // zero for the padding points ngp <= q < nqp of the last k-step
         float m = 0.f;
-        for (int q = tid; q < ngp; q += blockDim.x) {
+        for (int q = tid; q < nqp; q += blockDim.x) {
             float* mq = Ms + q * 16;  // owned by this thread
 #pragma unroll
             for (int x = 0; x < 16; x++) mq[x] = 0.f;
+            if (q >= ngp) continue;
             const int qq = nq == 1 ? 0 : q;
             float cv[SLX_FK_MAXT];
 #pragma unroll
@@ -230,123 +345,14 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
                 }
             }
         } else {
-            // producer waves: stage the shared P fragments and build Q_e, one
-            // k-step ahead of the consumers.  Thread bj takes column j = bj and
-            // every Gauss point of the k-step: the dof table values (a float4 of
-            // the four derivatives per point) are loaded first, then combined
-            // with Ms[q][ai][b] into the entries k = qi * na + ai, split into hi/lo
-            // f16 in the B fragment layout (lane 16 * (k / 8) + j % 16 of column
-            // tile j / 16, element k % 8).
+            // producer waves (the element routine is specialised on the number
+            // of tf derivatives, which fixes the K order of a k-step)
             const int bt = tid - 64 * SLX_FK_WAVES;
-            const float4v* tq4 = reinterpret_cast<const float4v*>(f.tq);
-            // thread bt: column bc = bt % CW of the pass and the 8-entry k groups
-            // kg = bt / CW, kg + NG, ... of the k-step (4 groups of 8 make the 32
-            // k slots); entry k is Gauss point k / na, tf derivative k % na.
-            // A group is packed in registers and written as one 16-byte hi and
-            // one 16-byte lo fragment word.
-            constexpr int CW = 16 * NTB;
-            constexpr int NG = (64 * SLX_FK_BWAVES) / CW;
-            static_assert(NG >= 1 && CW % 64 == 0, "a pass needs whole producer waves per column group");
-            // k group and everything derived from it are wave-uniform (scalar
-            // registers); the column is the lane's
-            const int bc = bt % CW, kg0 = __builtin_amdgcn_readfirstlane(bt / CW);
-            const unsigned nmagic = (65536u + na - 1) / na;  // k / na = (k * nmagic) >> 16 for k < 32, na <= 4
-            // the dof-table values of a k-step (one float4 of the four derivatives
-            // per entry) are loaded one k-step ahead into hq, so their latency
-            // overlaps the barrier and the consumers' MFMAs
-            constexpr int GP = 4 / NG;  // k groups per thread
-            float4v hq[GP][8];
-            auto load_h = [&](int ks, int cb) {
-                const int j = cb * 16 + bc;
-                const bool jon = j < ntp;
-#pragma unroll
-                for (int g = 0; g < GP; g++) {
-                    const int kg = kg0 + g * NG;
-#pragma unroll
-                    for (int v = 0; v < 8; v++) {
-                        const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), q = ks * qb + qi;
-                        const float4v* row = tq4 + (size_t)min(q, ngp - 1) * ntp;  // uniform row of the table
-                        hq[g][v] = (jon && qi < qb && q < ngp && ks < f.nks) ? row[j] : float4v{0.f, 0.f, 0.f, 0.f};
-                    }
-                }
-            };
-            auto produce = [&](int ks, int buf, int cb) {
-                // the shared P chunk goes straight from global memory to LDS
-                // (global_load_lds: lane i of a wave lands at the wave's base + 16 i),
-                // in flight while Q_e is built
-                const int wp = __builtin_amdgcn_readfirstlane(bt >> 6);
-                const int lane = bt & 63;
-#pragma unroll
-                for (int u = 0; u < SLX_FK_AREG; u++) {
-                    const int x0 = 64 * wp + u * 64 * SLX_FK_BWAVES;  // wave-uniform first entry
-                    if (x0 + lane < achunk) {
-                        const uint4v* src = afrag + ((size_t)ks * achunk + x0);  // uniform base, lane offset below
-                        __builtin_amdgcn_global_load_lds((const void*)(src + lane),
-                                                         (__attribute__((address_space(3))) void*)(As + buf * achunk + x0), 16, 0, 0);
-                    }
-                }
-                uint4v* colw = Bs + (size_t)buf * BCH + (size_t)(bc >> 4) * NPL * 64 + (bc & 15);
-                float* colf = reinterpret_cast<float*>(Bs + (size_t)buf * BCH) + (size_t)(bc >> 4) * 512 + (bc & 15);
-#pragma unroll
-                for (int g = 0; g < GP; g++) {
-                    const int kg = kg0 + g * NG;
-                    // the group's coefficient rows first (one LDS wait), then the
-                    // products (unused derivatives have zero coefficients)
-                    float4v mm[8];
-#pragma unroll
-                    for (int v = 0; v < 8; v++) {
-                        const int k = 8 * kg + v, qi = (int)(((unsigned)k * nmagic) >> 16), ai = k - qi * na;
-                        const int q = min(ks * qb + qi, ngp - 1);
-                        mm[v] = *reinterpret_cast<const float4v*>(Ms + (q * 4 + ai) * 4);
-                    }
-                    float vals[8];
-#pragma unroll
-                    for (int v = 0; v < 8; v++) {
-                        const float4v hh = hq[g][v];
-                        vals[v] = mm[v][0] * hh[0] + mm[v][1] * hh[1] + mm[v][2] * hh[2] + mm[v][3] * hh[3];
-                    }
-                    if constexpr (AR == AR_F16) {
-                        half8 hv, lv;
-#pragma unroll
-                        for (int v = 0; v < 8; v++) {
-                            const _Float16 hi = (_Float16)vals[v];
-                            hv[v] = hi;
-                            lv[v] = (_Float16)(vals[v] - (float)hi);
-                        }
-                        colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
-                        colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
-                    } else if constexpr (AR == AR_BF16) {
-                        bf16x8 hv, mv, lv;
-#pragma unroll
-                        for (int v = 0; v < 8; v++) {
-                            const __bf16 hi = (__bf16)vals[v];
-                            const float r1 = vals[v] - (float)hi;
-                            const __bf16 mid = (__bf16)r1;
-                            hv[v] = hi;
-                            mv[v] = mid;
-                            lv[v] = (__bf16)(r1 - (float)mid);
-                        }
-                        colw[16 * kg] = __builtin_bit_cast(uint4v, hv);
-                        colw[64 + 16 * kg] = __builtin_bit_cast(uint4v, mv);
-                        colw[128 + 16 * kg] = __builtin_bit_cast(uint4v, lv);
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < 8; v++) colf[(8 * kg + v) * 16] = vals[v];
-                    }
-                }
-                // the P chunk is in LDS before this wave reaches the barrier; the
-                // next k-step's table values are requested after that wait
-                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-                load_h(ks + 1, cb);  // zeros past the last k-step
-            };
-            for (int pass = 0; pass < npass; pass++) {
-                load_h(0, pass * NTB);
-                produce(0, 0, pass * NTB);
-                __syncthreads();  // k-step 0 staged
-                for (int ks = 0; ks < f.nks; ks++) {
-                    if (ks + 1 < f.nks) produce(ks + 1, (ks & 1) ^ 1, pass * NTB);
-                    __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
-                }
+            switch (na) {
+                case 1: factor_producer<AR, BM, BN, 1>(f, As, Bs, Ms, bt); break;
+                case 2: factor_producer<AR, BM, BN, 2>(f, As, Bs, Ms, bt); break;
+                case 3: factor_producer<AR, BM, BN, 3>(f, As, Bs, Ms, bt); break;
+                default: factor_producer<AR, BM, BN, 4>(f, As, Bs, Ms, bt); break;
             }
         }
     }

@@ -1275,10 +1275,11 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
             tinfo[2 * sd.n_terms + t] = sd.terms[t].coef;
             f.bmask |= 1 << sd.terms[t].dof_deriv;
         }
-        // Gauss points per k-step: qb * na <= 32 k slots, at most SLX_FK_RU per producer thread
-        f.qb = std::min(32 / f.na, SLX_FK_RU);
-        if (f.qb < 1) continue;
+        // Gauss points per k-step: segments of whole Gauss points (factor_segw)
+        if (f.na < 1 || f.na > 4) continue;
+        f.qb = factor_qb(f.na, f.bn);
         f.nks = (f.ngp + f.qb - 1) / f.qb;
+        f.ntq = (f.ntp + 127) / 128 * 128;
         // arithmetics whose LDS and P chunk staging fit; the FP16 split must
         f.arith_ok = 0;
         for (int a = 0; a < AR_COUNT; a++)
@@ -1286,7 +1287,7 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
                 f.arith_ok |= 1u << a;
         if (!(f.arith_ok & (1u << AR_F16))) continue;
         // P fragments: lane l of tile (ks, mt) holds P^T[i = mt*16 + l%16][k = 8*(l/16) + v],
-        // k -> (Gauss point ks*qb + k/na, tf derivative alist[k%na]); power-of-two scale to [128, 256)
+        // k -> (Gauss point, tf derivative) by segments (factor_segw); power-of-two scale to [128, 256)
         const double* tft = bd.tables[sd.tf_table];
         const int nff = bd.table_nff[sd.tf_table], ngp = bd.n_gp;
         double amax = 0.0;
@@ -1297,10 +1298,12 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
         if (amax > 0) (void)std::frexp(amax, &aex);
         const double ascale = std::ldexp(1.0, 8 - aex);
         f.a_unscale = (float)std::ldexp(1.0, aex - 8);
-        // P^T[i][kl] of k-step ks (scaled), kl = local k: Gauss point kl / na, tf derivative kl % na
+        // P^T[i][kl] of k-step ks (scaled), kl = local k: segment kl / segw, entry r = kl % segw of
+        // the segment is Gauss point ks*qb + (kl / segw)*qh + r / na, tf derivative r % na (r < qh*na)
+        const int segw = factor_segw(f.bn), qh = segw / f.na;
         auto px = [&](int ks, int i, int kl) {
-            const int qi = kl / f.na, ai = kl % f.na, q = ks * f.qb + qi;
-            if (i < sd.n_tf && qi < f.qb && q < ngp) return (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+            const int r = kl % segw, qi = r / f.na, ai = r % f.na, q = ks * f.qb + (kl / segw) * qh + qi;
+            if (i < sd.n_tf && qi < qh && q < ngp) return (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
             return 0.f;
         };
         // fragments [ks][mt][plane][lane] x 16 bytes: F16 (hi, lo), BF16 (hi, mid, lo)
@@ -1328,16 +1331,17 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
                     for (int pl = 0; pl < 2; pl++)
                         for (int v = 0; v < 4; v++) fragf[((w * 2 + pl) * 64 + l) * 4 + v] = px(ks, i, 4 * (4 * pl + v) + l / 16);
                 }
-        // dof table times the weights, Gauss point major, padded columns
+        // dof table times the weights, Gauss point major; zero rows up to nks * qb and
+        // zero columns up to ntq, so the kernel reads every k-step's rows unchecked
         const double* dft = bd.tables[sd.dof_table];
         const int nfd = bd.table_nff[sd.dof_table];
-        std::vector<float> tq((size_t)ngp * 4 * f.ntp, 0.f);
+        std::vector<float> tq((size_t)f.nks * f.qb * 4 * f.ntq, 0.f);
         double tqmax = 0.0;
         for (int q = 0; q < ngp; q++)
             for (int d = 0; d < 4; d++)
                 for (int j = 0; j < sd.n_dof; j++) {
                     const double x = bd.gp_weights[q] * dft[((size_t)d * nfd + j) * ngp + q];
-                    tq[((size_t)q * f.ntp + j) * 4 + d] = (float)x;
+                    tq[((size_t)q * f.ntq + j) * 4 + d] = (float)x;
                     tqmax = std::max(tqmax, std::fabs(x));
                 }
         f.tqmax = (float)(tqmax * 1.0001);
