@@ -615,39 +615,95 @@ static void launch_cols_any(const DevBucket& b, const void* ctab, uint32_t mask,
     else launch_cols<CT, 16>(b, ctab, mask, jc, qfast, ke, st);
 }
 
-// Right-hand-side slices: one lane per (element, tf function),
-//   b_e[i] = -sum_q w_q sum_t C_t(e, q) tf_{r_t}[i][q]
-// (contribution.cpp:293-297; the element's lanes write one contiguous run).
+// Right-hand-side slices with many tf functions, as a blocked product over
+// tiles of RB rows x EB elements:
+//   b_e[i] = -sum_q sum_r tf_r[i][q] (w_q sum_{t: tf derivative r} C_t(e, q))
+// (contribution.cpp:293-297).  Gauss points go in chunks of RQ: the tile's
+// table rows and the elements' weighted coefficients are staged in LDS, and
+// every thread accumulates 4 consecutive rows of 4 elements, so each table
+// value and each coefficient is read from memory once per tile.
 template <typename CT>
-__global__ void __launch_bounds__(256) rhs_rows(DevBucket b, const CT* __restrict__ ctab, int s, int qfast, CT* __restrict__ ke) {
+struct RhsVec4 { typedef CT type __attribute__((ext_vector_type(4))); };
+
+template <typename CT>
+__global__ void __launch_bounds__(256) rhs_tiles(DevBucket b, const CT* __restrict__ ctab, int s, int qfast, CT* __restrict__ ke) {
+    constexpr int RB = 64, EB = 64, RQ = 16;
+    typedef typename RhsVec4<CT>::type CT4;
+    __shared__ __attribute__((aligned(32))) CT Ts[RQ][RB];  // table values of the chunk, [Gauss point][row]
+    __shared__ __attribute__((aligned(32))) CT Cs[RQ][EB];  // weighted coefficients, [Gauss point][element]
     const SLX_CONST DevSlice* dsl = (const SLX_CONST DevSlice*)b.dslices;
     const int n_tf = dsl[s].n_tf, n_terms = dsl[s].n_terms, tf_pad = dsl[s].tf_pad;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)b.n_elem * n_tf) return;
-    const int e = (int)(idx / n_tf), i = (int)(idx - (int64_t)e * n_tf);
-    const int nq = b.q_inv ? 1 : b.n_gp, ngp = b.n_gp;
-    const size_t ne = (size_t)b.n_elem;
     const SLX_CONST slx_term* terms = (const SLX_CONST slx_term*)dsl[s].terms;
     const SLX_CONST double* gw = (const SLX_CONST double*)b.gp_w;
-    const double* tfq = dsl[s].tf_tabq;
-    CT acc = 0;
-    for (int q = 0; q < ngp; q++) {
-        const int qq = nq == 1 ? 0 : q;
-        CT h = 0;
-        for (int t = 0; t < n_terms; t++) {
-            const int c = terms[t].coef, r = terms[t].tf_deriv;
-            const CT cv = qfast ? ctab[((size_t)c * ne + e) * nq + qq] : ctab[((size_t)c * nq + qq) * ne + e];
-            h += cv * (CT)tfq[((size_t)q * 4 + r) * tf_pad + i];
+    const CT* tab;
+    if constexpr (sizeof(CT) == 4) tab = (const CT*)dsl[s].tf_tabq32;
+    else tab = (const CT*)dsl[s].tf_tabq;
+    const int ngp = b.n_gp, nq = b.q_inv ? 1 : b.n_gp, n_elem = b.n_elem;
+    const size_t ne = (size_t)n_elem;
+    const int e0 = blockIdx.x * EB, i0 = blockIdx.y * RB;
+    const int tid = threadIdx.x, ri = tid & 15, ej = tid >> 4;  // rows i0 + 4 ri + 0..3, elements e0 + 4 ej + 0..3
+    CT acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = 0;
+    for (int r = 0; r < 4; r++) {
+        bool used = false;
+        for (int t = 0; t < n_terms; t++) used |= terms[t].tf_deriv == r;
+        if (!used) continue;
+        for (int q0 = 0; q0 < ngp; q0 += RQ) {
+#pragma unroll
+            for (int u = 0; u < RQ * RB / 256; u++) {
+                const int x = tid + 256 * u, k = x / RB, il = x % RB, q = q0 + k, i = i0 + il;
+                Ts[k][il] = (q < ngp && i < n_tf) ? tab[((size_t)q * 4 + r) * tf_pad + i] : (CT)0;
+            }
+#pragma unroll
+            for (int u = 0; u < RQ * EB / 256; u++) {
+                const int x = tid + 256 * u;
+                // Gauss point fastest where the coefficient table is [slot][element][point]
+                const int k = qfast ? x % RQ : x / EB, el = qfast ? x / RQ : x % EB, q = q0 + k, e = e0 + el;
+                CT c = 0;
+                if (q < ngp && e < n_elem) {
+                    const int qq = nq == 1 ? 0 : q;
+                    for (int t = 0; t < n_terms; t++)
+                        if (terms[t].tf_deriv == r) {
+                            const int cs = terms[t].coef;
+                            c += qfast ? ctab[((size_t)cs * ne + e) * nq + qq] : ctab[((size_t)cs * nq + qq) * ne + e];
+                        }
+                    c *= (CT)gw[q];
+                }
+                Cs[k][el] = c;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < RQ; k++) {
+                const CT4 a = *reinterpret_cast<const CT4*>(&Ts[k][4 * ri]);
+                const CT4 c = *reinterpret_cast<const CT4*>(&Cs[k][4 * ej]);
+#pragma unroll
+                for (int x = 0; x < 4; x++)
+#pragma unroll
+                    for (int y = 0; y < 4; y++) acc[x][y] += c[x] * a[y];
+            }
+            __syncthreads();
         }
-        acc += h * (CT)gw[q];
     }
-    ke[dsl[s].ke_base + ((int64_t)e * n_tf + i) * dsl[s].ke_ld] = -acc;
+    const int64_t base = dsl[s].ke_base, ld = dsl[s].ke_ld;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const int e = e0 + 4 * ej + x;
+        if (e >= n_elem) continue;
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int i = i0 + 4 * ri + y;
+            if (i < n_tf) ke[base + ((int64_t)e * n_tf + i) * ld] = -acc[x][y];
+        }
+    }
 }
 
 hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void* ke, hipStream_t st, uint32_t slice_mask, int qfast) {
     if (b.n_elem == 0 || slice_mask == 0) return hipSuccess;
     // right-hand sides with more tf functions than one column pass of elem_cols
-    // covers go to the row-per-lane kernel
+    // covers go to the tiled kernel
     uint32_t rhs_mask = 0;
     for (int s = 0; s < b.n_slices; s++)
         if (((slice_mask >> s) & 1u) && b.slices[s].is_rhs && b.slices[s].n_tf > 16) rhs_mask |= 1u << s;
@@ -660,12 +716,11 @@ hipError_t launch_elem(const DevBucket& b, const void* ctab, int precision, void
     }
     for (int s = 0; s < b.n_slices; s++) {
         if (!((rhs_mask >> s) & 1u)) continue;
-        const int64_t n = (int64_t)b.n_elem * b.slices[s].n_tf;
-        const unsigned grid = (unsigned)((n + 255) / 256);
+        const dim3 grid((unsigned)((b.n_elem + 63) / 64), (unsigned)((b.slices[s].n_tf + 63) / 64));
         if (precision == SLX_PREC_FP64)
-            hipLaunchKernelGGL((rhs_rows<double>), dim3(grid), dim3(256), 0, st, b, (const double*)ctab, s, qfast, (double*)ke);
+            hipLaunchKernelGGL((rhs_tiles<double>), grid, dim3(256), 0, st, b, (const double*)ctab, s, qfast, (double*)ke);
         else
-            hipLaunchKernelGGL((rhs_rows<float>), dim3(grid), dim3(256), 0, st, b, (const float*)ctab, s, qfast, (float*)ke);
+            hipLaunchKernelGGL((rhs_tiles<float>), grid, dim3(256), 0, st, b, (const float*)ctab, s, qfast, (float*)ke);
     }
     return hipGetLastError();
 }

@@ -79,11 +79,15 @@ __device__ __forceinline__ void factor_producer(const FactorSlice& f, uint4v* As
     const int npass = (f.nt + NTB - 1) / NTB;
     const int wp = __builtin_amdgcn_readfirstlane(bt >> 6), lane = bt & 63;
     const int sg = __builtin_amdgcn_readfirstlane(bt / CW), bc = bt % CW;
-    const float4v* tq4 = reinterpret_cast<const float4v*>(f.tq);
+    // global (not flat) loads: a flat load also counts against the LDS counter,
+    // which every barrier drains, so the one-k-step-ahead loads would be waited
+    // for at the next barrier instead of at their use
+    typedef __attribute__((address_space(1))) const float4v gfloat4v;
+    const gfloat4v* tq4 = (const gfloat4v*)(f.tq);
     const uint4v* afrag = reinterpret_cast<const uint4v*>(f.afrag);
     float4v hq[QH];
     auto load_h = [&](int ks, int cb) {
-        const float4v* row = tq4 + ((ks * QB + sg * QH) * ntq + cb * 16 + bc);
+        const gfloat4v* row = tq4 + ((ks * QB + sg * QH) * ntq + cb * 16 + bc);
 #pragma unroll
         for (int qi = 0; qi < QH; qi++) hq[qi] = row[qi * ntq];
     };
@@ -168,6 +172,95 @@ __device__ __forceinline__ void factor_producer(const FactorSlice& f, uint4v* As
     }
 }
 
+// One k-step of a consumer wave: its BM row tiles wm + 2 mi (a tile past the
+// last row tile reads the last one; its accumulator is never stored) times its
+// first NB column tiles wn + 4 nj, as straight-line code: the operands are read
+// first and the products of the split are issued product by product over the
+// tiles, so no accumulator is updated by two MFMAs back to back.
+template <int AR, int BM, int BN, int NB>
+__device__ __forceinline__ void factor_consume(const uint4v* __restrict__ Ac, const uint4v* __restrict__ Bc, float4v (&acc)[BM][BN], int wm,
+                                               int wn, int MT, int lane) {
+    constexpr int NPL = AR == AR_BF16 ? 3 : 2;
+    static_assert(NB >= 1 && NB <= BN, "column tiles of a k-step");
+    int mts[BM];
+#pragma unroll
+    for (int mi = 0; mi < BM; mi++) mts[mi] = min(wm + 2 * mi, MT - 1);
+    if constexpr (AR == AR_F32) {
+        // step j (k = 4 j + lane / 16): B lane value Q[4 j + lane / 16][lane % 16]
+        const float* Bf = reinterpret_cast<const float*>(Bc);
+        float bq[NB][8];
+#pragma unroll
+        for (int nj = 0; nj < NB; nj++)
+#pragma unroll
+            for (int j = 0; j < 8; j++) bq[nj][j] = Bf[(wn + 4 * nj) * 512 + 64 * j + lane];
+        float4v a[BM][2];
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++) {
+            a[mi][0] = __builtin_bit_cast(float4v, Ac[(mts[mi] * NPL) * 64 + lane]);
+            a[mi][1] = __builtin_bit_cast(float4v, Ac[(mts[mi] * NPL + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+#pragma unroll
+            for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+                for (int nj = 0; nj < NB; nj++)
+                    acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[mi][j >> 2][j & 3], bq[nj][j], acc[mi][nj], 0, 0, 0);
+    } else if constexpr (AR == AR_F16) {
+        half8 bh[NB], bl[NB], ah[BM], al[BM];
+#pragma unroll
+        for (int nj = 0; nj < NB; nj++) {
+            bh[nj] = __builtin_bit_cast(half8, Bc[((wn + 4 * nj) * NPL) * 64 + lane]);
+            bl[nj] = __builtin_bit_cast(half8, Bc[((wn + 4 * nj) * NPL + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++) {
+            ah[mi] = __builtin_bit_cast(half8, Ac[(mts[mi] * NPL) * 64 + lane]);
+            al[mi] = __builtin_bit_cast(half8, Ac[(mts[mi] * NPL + 1) * 64 + lane]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bh[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[mi], bl[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++)
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[mi], bh[nj], acc[mi][nj], 0, 0, 0);
+    } else {
+        // BF16: one row tile's three planes at a time (all of them would not fit
+        // the register budget); the six products alternate over the column tiles
+        bf16x8 xh[NB], xm[NB], xl[NB];
+#pragma unroll
+        for (int nj = 0; nj < NB; nj++) {
+            xh[nj] = __builtin_bit_cast(bf16x8, Bc[((wn + 4 * nj) * NPL) * 64 + lane]);
+            xm[nj] = __builtin_bit_cast(bf16x8, Bc[((wn + 4 * nj) * NPL + 1) * 64 + lane]);
+            xl[nj] = __builtin_bit_cast(bf16x8, Bc[((wn + 4 * nj) * NPL + 2) * 64 + lane]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < BM; mi++) {
+            const bf16x8 ah = __builtin_bit_cast(bf16x8, Ac[(mts[mi] * NPL) * 64 + lane]);
+            const bf16x8 am = __builtin_bit_cast(bf16x8, Ac[(mts[mi] * NPL + 1) * 64 + lane]);
+            const bf16x8 al = __builtin_bit_cast(bf16x8, Ac[(mts[mi] * NPL + 2) * 64 + lane]);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xh[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xm[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xh[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xl[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xm[nj], acc[mi][nj], 0, 0, 0);
+#pragma unroll
+            for (int nj = 0; nj < NB; nj++) acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, xh[nj], acc[mi][nj], 0, 0, 0);
+        }
+    }
+}
+
 // LDS per k-step and buffer: A chunk [MT][NPL][64] 16-byte words; B chunk
 // [NTB][NPL][64] words (F16 / BF16: lane l of column tile t holds the eight
 // k = 8 (l / 16) + 0..7 of column l % 16, one plane per piece; F32: the tile
@@ -180,15 +273,21 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
     const int MT = f.mt, NT = f.nt, ngp = f.ngp, nq = f.nq, na = f.na, nqp = f.nks * f.qb;
     constexpr int NTB = 4 * BN;                                 // column tiles per pass
     constexpr int BCH = NTB * NPL * 64;                         // B chunk words
+    static_assert(BN == 1 || BN == 2, "a partial pass leaves a wave one column tile or none");
     uint4v* As = reinterpret_cast<uint4v*>(smem_raw);           // [2][MT][NPL][64]
     uint4v* Bs = As + 2 * MT * NPL * 64;                        // [2][NTB][NPL][64]
     float* Ms = reinterpret_cast<float*>(Bs + 2 * BCH);         // [nqp][4 tf derivative][4 dof derivative]
     unsigned* smax = reinterpret_cast<unsigned*>(Ms + nqp * 16);
     const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const bool mfma_wave = w < SLX_FK_WAVES;
-    const int32_t* t_ai = f.tinfo;
-    const int32_t* t_b = f.tinfo + f.n_terms;
-    const int32_t* t_slot = f.tinfo + 2 * f.n_terms;
+    // term table through the constant address space (scalar loads), coefficients
+    // through the global one (no flat loads, which the barriers would drain)
+    typedef __attribute__((address_space(4))) const int32_t cint32;
+    typedef __attribute__((address_space(1))) const float gfloat;
+    const cint32* t_ai = (const cint32*)(f.tinfo);
+    const cint32* t_b = t_ai + f.n_terms;
+    const cint32* t_slot = t_ai + 2 * f.n_terms;
+    const gfloat* ctg = (const gfloat*)(ctab);
     const int64_t stride = (int64_t)nq * n_elem;
     const int achunk = MT * NPL * 64;
     const int npass = (NT + NTB - 1) / NTB;
@@ -209,7 +308,7 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
             float cv[SLX_FK_MAXT];
 #pragma unroll
             for (int t = 0; t < SLX_FK_MAXT; t++)
-                cv[t] = t < f.n_terms ? ctab[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq] : 0.f;
+                cv[t] = t < f.n_terms ? ctg[(int64_t)t_slot[t] * stride + (int64_t)e * nq + qq] : 0.f;
 #pragma unroll
             for (int t = 0; t < SLX_FK_MAXT; t++)
                 if (t < f.n_terms) mq[t_ai[t] * 4 + t_b[t]] += cv[t];
@@ -246,80 +345,17 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
 #pragma unroll
                     for (int nj = 0; nj < BN; nj++) acc[mi][nj] = float4v{0.f, 0.f, 0.f, 0.f};
                 __syncthreads();  // k-step 0 staged
+                // column tiles of this wave in the pass (a prefix of its BN tiles)
+                int nbv = 0;
+#pragma unroll
+                for (int nj = 0; nj < BN; nj++)
+                    if (cb + wn + 4 * nj < NT) nbv = nj + 1;
                 for (int ks = 0; ks < f.nks; ks++) {
                     const int cur = ks & 1;
                     const uint4v* Ac = As + cur * achunk;
                     const uint4v* Bc = Bs + cur * BCH;
-                    if constexpr (AR == AR_F32) {
-                        // step j (k = 4 j + lane / 16): B lane value Q[4 j + lane / 16][lane % 16]
-                        const float* Bf = reinterpret_cast<const float*>(Bc);
-                        float bq[BN][8];
-#pragma unroll
-                        for (int nj = 0; nj < BN; nj++) {
-                            const int jl = wn + 4 * nj;
-#pragma unroll
-                            for (int j = 0; j < 8; j++) bq[nj][j] = Bf[jl * 512 + 64 * j + lane];
-                        }
-#pragma unroll
-                        for (int mi = 0; mi < BM; mi++) {
-                            const int mt = wm + 2 * mi;
-                            if (mt < MT) {
-                                const float4v a0 = __builtin_bit_cast(float4v, Ac[(mt * NPL) * 64 + lane]);
-                                const float4v a1 = __builtin_bit_cast(float4v, Ac[(mt * NPL + 1) * 64 + lane]);
-#pragma unroll
-                                for (int nj = 0; nj < BN; nj++) {
-                                    if (cb + wn + 4 * nj < NT) {
-#pragma unroll
-                                        for (int j = 0; j < 4; j++)
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[j], bq[nj][j], acc[mi][nj], 0, 0, 0);
-#pragma unroll
-                                        for (int j = 0; j < 4; j++)
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[j], bq[nj][4 + j], acc[mi][nj], 0, 0, 0);
-                                    }
-                                }
-                            }
-                        }
-                    } else {
-                        uint4v bp[BN][NPL];
-#pragma unroll
-                        for (int nj = 0; nj < BN; nj++) {
-                            const int jl = wn + 4 * nj;
-#pragma unroll
-                            for (int pl = 0; pl < NPL; pl++) bp[nj][pl] = Bc[(jl * NPL + pl) * 64 + lane];
-                        }
-#pragma unroll
-                        for (int mi = 0; mi < BM; mi++) {
-                            const int mt = wm + 2 * mi;
-                            if (mt < MT) {
-                                uint4v ap[NPL];
-#pragma unroll
-                                for (int pl = 0; pl < NPL; pl++) ap[pl] = Ac[(mt * NPL + pl) * 64 + lane];
-#pragma unroll
-                                for (int nj = 0; nj < BN; nj++) {
-                                    if (cb + wn + 4 * nj < NT) {
-                                        if constexpr (AR == AR_F16) {
-                                            const half8 ah = __builtin_bit_cast(half8, ap[0]), al = __builtin_bit_cast(half8, ap[1]);
-                                            const half8 xh = __builtin_bit_cast(half8, bp[nj][0]), xl = __builtin_bit_cast(half8, bp[nj][1]);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh, acc[mi][nj], 0, 0, 0);
-                                        } else {
-                                            const bf16x8 ah = __builtin_bit_cast(bf16x8, ap[0]), am = __builtin_bit_cast(bf16x8, ap[1]),
-                                                         al = __builtin_bit_cast(bf16x8, ap[2]);
-                                            const bf16x8 xh = __builtin_bit_cast(bf16x8, bp[nj][0]), xm = __builtin_bit_cast(bf16x8, bp[nj][1]),
-                                                         xl = __builtin_bit_cast(bf16x8, bp[nj][2]);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xh, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xm, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xh, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xl, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xm, acc[mi][nj], 0, 0, 0);
-                                            acc[mi][nj] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, xh, acc[mi][nj], 0, 0, 0);
-                                        }
-                                    }
-                                }
-                            }
-                        }
-                    }
+                    if (nbv == BN) factor_consume<AR, BM, BN, BN>(Ac, Bc, acc, wm, wn, MT, lane);
+                    else if (nbv > 0) factor_consume<AR, BM, BN, 1>(Ac, Bc, acc, wm, wn, MT, lane);
                     __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
                 }
                 // element matrix rows i = mt*16 + 4*(lane/16) + r, column j = jt*16 + lane%16;
