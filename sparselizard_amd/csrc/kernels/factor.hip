@@ -358,24 +358,26 @@ __device__ __forceinline__ void factor_element(const FactorSlice& f, int n_elem,
                     else if (nbv > 0) factor_consume<AR, BM, BN, 1>(Ac, Bc, acc, wm, wn, MT, lane);
                     __syncthreads();  // k-step ks consumed, k-step ks + 1 staged
                 }
-                // element matrix rows i = mt*16 + 4*(lane/16) + r, column j = jt*16 + lane%16;
-                // 32-bit offsets from the element's (uniform) base keep the addresses cheap
+                // element matrix rows i = mt*16 + 4*(lane/16) + r, column j = jt*16 + lane%16:
+                // a uniform tile base plus one 32-bit lane offset per r, so that no
+                // per-entry address is kept live across the k-steps
                 float* eb = ke + f.ke_base + (int64_t)e * f.n_tf * f.ke_ld;
-                const int ld = (int)f.ke_ld;
+                const int ld = (int)f.ke_ld, lrow = 4 * (lane >> 4), lcol = lane & 15;
 #pragma unroll
                 for (int mi = 0; mi < BM; mi++) {
                     const int mt = wm + 2 * mi;
 #pragma unroll
                     for (int nj = 0; nj < BN; nj++) {
                         const int jt = cb + wn + 4 * nj;
-                        const int j = jt * 16 + (lane & 15);
-                        if (mt >= MT || jt >= NT || j >= f.n_dof) continue;
+                        if (mt >= MT || jt >= NT) continue;
+                        const bool jok = jt * 16 + lcol < f.n_dof;
+                        float* tb = eb + (mt * 16 * ld + jt * 16);
 #pragma unroll
                         for (int r = 0; r < 4; r++) {
-                            const int i = mt * 16 + 4 * (lane >> 4) + r;
                             // streaming store: element matrices must not evict the shared
                             // P fragments and dof table from L2
-                            if (i < f.n_tf) __builtin_nontemporal_store(acc[mi][nj][r] * unsc, eb + i * ld + j);
+                            if (jok && mt * 16 + lrow + r < f.n_tf)
+                                __builtin_nontemporal_store(acc[mi][nj][r] * unsc, tb + ((lrow + r) * ld + lcol));
                         }
                     }
                 }
