@@ -112,6 +112,7 @@ def lib():
         "slx_plan_destroy": (i32, [vp]),
         "slx_plan_kernels": (i32, [vp, p32, p32]),
         "slx_plan_kernel_counts": (i32, [vp, i32, p32, p32]),
+        "slx_plan_tile_count": (i32, [vp, i32, p32]),
         "slx_plan_fused_info": (i32, [vp, i32, p32, p64]),
         "slx_plan_solve": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
         "slx_plan_solve_noalloc": (i32, [vp, i32, C.c_double, i32, i32, pd, vp, p32, p32, pd, p32]),
@@ -154,7 +155,7 @@ EXPORTED = [
     "slx_last_error", "slx_create", "slx_destroy", "slx_plan_create", "slx_plan_create_ex", "slx_plan_symbolic_info",
     "slx_plan_symbolic_array", "slx_plan_shape", "slx_assemble",
     "slx_assemble_device", "slx_assemble_async", "slx_plan_dof_maps", "slx_plan_sync", "slx_plan_profile", "slx_csr_device", "slx_csr_copyout", "slx_plan_stats",
-    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_fused_info", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
+    "slx_plan_destroy", "slx_plan_kernels", "slx_plan_kernel_counts", "slx_plan_tile_count", "slx_plan_fused_info", "slx_plan_solve", "slx_plan_solve_noalloc", "slx_plan_eliminate", "slx_plan_matvec", "slx_plan_coef_mode", "slx_plan_reduction_maps", "slx_plan_element_matrices", "slx_plan_coefficients", "slx_problem_create", "slx_problem_destroy", "slx_problem_plan", "slx_problem_numeric",
     "slx_problem_generate", "slx_problem_info", "slx_problem_desc", "slx_problem_setdata", "slx_problem_setdata_vec",
     "slx_plan_graph_capture", "slx_plan_graph_launch",
     "slx_gauss_rule", "slx_h1_count", "slx_h1_tables", "slx_geometry_tables", "slx_hcurl_count", "slx_hcurl_tables", "slx_selftest_mfma_layout", "slx_selftest_mfma_layout_kind",
@@ -446,6 +447,12 @@ class Plan:
         check(lib().slx_plan_kernel_counts(self.h, PRECISIONS[precision] if isinstance(precision, str) else precision,
                                            C.byref(a), C.byref(b)))
         return dict(batched=a.value, factorized=b.value)
+
+    def tile_count(self, precision):
+        """Matrix slices on the one-tile-per-element MFMA kernel in a precision mode."""
+        a = C.c_int32()
+        check(lib().slx_plan_tile_count(self.h, PRECISIONS[precision] if isinstance(precision, str) else precision, C.byref(a)))
+        return a.value
 
     def fused_info(self, precision):
         """(slices, work items) of the mode's fused launch; (0, 0) without one."""

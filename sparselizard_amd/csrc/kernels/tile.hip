@@ -1,0 +1,139 @@
+// One 16x16 MFMA tile per element for small slices with Gauss-point-varying
+// coefficients (kernels/tile.h).
+//
+// A workgroup of SLX_TK_WAVES waves loops over the bucket's elements, one
+// element per wave at a time.  Once per workgroup: the weighted dof table
+// (w_q dof_b[j][q], [q][j][4 derivatives]) and the P fragments (the shared tf
+// table, FP16 hi/lo, scaled on the host) go to LDS, where they stay for the
+// whole launch.  Per element: the lanes of the Gauss points sum
+// the element's term coefficients into the 4x4 coefficient matrices
+// D[q][a][b] (LDS, one region per wave) and the wave takes the element's
+// power-of-two scale; then per k-chunk every lane builds its 8 entries of Q_e
+//   Q_e[k][j] = sc * sum_b D[q][a][b] * w_q dof_b[j][q]
+// splits them into f16 hi/lo and issues three v_mfma_f32_16x16x32_f16
+// (hi*hi + hi*lo + lo*hi) into the element's FP32 accumulator.
+#include "tile.h"
+
+namespace slx {
+
+typedef _Float16 half8t __attribute__((ext_vector_type(8)));
+typedef float float4t __attribute__((ext_vector_type(4)));
+typedef unsigned int uint4t __attribute__((ext_vector_type(4)));
+
+template <int NA>
+__global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, TileSlice t, const float* __restrict__ ctab, int qfast, float* __restrict__ ke) {
+    constexpr int QH = tile_qh(NA);
+    static_assert(QH >= 1, "a lane group holds at least one Gauss point");
+    // dynamic LDS: [ngpp][16] float4 weighted dof table, [nkc][2][64] P fragment
+    // words, [SLX_TK_WAVES][ngpp][4] float4 coefficient rows
+    extern __shared__ __attribute__((aligned(16))) unsigned char tile_lds[];
+    const int nkc = t.nkc, ngpp = t.ngpp, ngp = t.ngp, nq = t.nq, n_terms = t.n_terms;
+    float4t* Tl = reinterpret_cast<float4t*>(tile_lds);
+    uint4t* Pl = reinterpret_cast<uint4t*>(Tl + ngpp * 16);
+    float4t* Dall = reinterpret_cast<float4t*>(Pl + nkc * 2 * 64);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, j = lane & 15;
+    typedef __attribute__((address_space(4))) const int32_t cint32;
+    const cint32* t_ai = (const cint32*)(t.tinfo);
+    const cint32* t_b = t_ai + n_terms;
+    const cint32* t_slot = t_ai + 2 * n_terms;
+    {
+        const float4t* tq4 = reinterpret_cast<const float4t*>(t.tq);
+        for (int x = tid; x < ngpp * 16; x += blockDim.x) Tl[x] = tq4[x];
+        const uint4t* af = reinterpret_cast<const uint4t*>(t.afrag);
+        for (int x = tid; x < nkc * 2 * 64; x += blockDim.x) Pl[x] = af[x];
+    }
+    float4t* Dl = Dall + w * ngpp * 4;  // this wave's [q][a]
+    float* dw = reinterpret_cast<float*>(Dl);
+    const int64_t stride = (int64_t)nq * n_elem;
+    // a contiguous range of elements per workgroup, SLX_TK_WAVES at a time: with
+    // the point-major coefficient layout ([slot][point][element]) consecutive
+    // iterations read neighbouring words of the same cache lines
+    const int per = (int)(((int64_t)n_elem + gridDim.x - 1) / gridDim.x + SLX_TK_WAVES - 1) / SLX_TK_WAVES * SLX_TK_WAVES;
+    const int64_t e_hi = (int64_t)per * (blockIdx.x + 1);
+    const int e_end = (int)(e_hi < n_elem ? e_hi : (int64_t)n_elem);
+    for (int e0 = per * blockIdx.x; e0 < e_end; e0 += SLX_TK_WAVES) {
+        const int e = e0 + w;
+        const bool live = e < e_end;
+        __syncthreads();  // the tables are in LDS; the previous element's coefficient rows are read
+        // coefficient matrices of the element: lane q sums the terms of its Gauss point
+        float m = 0.f;
+        if (lane < ngpp) {
+            float* mq = dw + lane * 16;
+#pragma unroll
+            for (int x = 0; x < 16; x++) mq[x] = 0.f;
+            if (live && lane < ngp) {
+                const int qq = nq == 1 ? 0 : lane;
+                for (int tt = 0; tt < n_terms; tt++)
+                    mq[t_ai[tt] * 4 + t_b[tt]] += qfast ? ctab[(int64_t)t_slot[tt] * stride + (int64_t)e * nq + qq]
+                                                        : ctab[((int64_t)t_slot[tt] * nq + qq) * n_elem + e];
+            }
+#pragma unroll
+            for (int a = 0; a < 4; a++) m = fmaxf(m, fabsf(mq[4 * a]) + fabsf(mq[4 * a + 1]) + fabsf(mq[4 * a + 2]) + fabsf(mq[4 * a + 3]));
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+        // power-of-two scale: |Q| <= max_q,a sum_b |D| * max |tq|  ->  [128, 256)
+        const float bound = m * t.tqmax;
+        float sc = 1.f, unsc = t.a_unscale;
+        if (bound > 0.f) {
+            int ex = 0;
+            (void)frexpf(bound, &ex);
+            sc = ldexpf(1.f, 8 - ex);
+            unsc = ldexpf(t.a_unscale, ex - 8);
+        }
+        __syncthreads();  // the coefficient rows are in LDS
+        float4t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 1
+        for (int c = 0; c < nkc; c++) {
+            const int q0 = (4 * c + g) * QH;
+            const half8t ph = __builtin_bit_cast(half8t, Pl[(c * 2) * 64 + lane]);
+            const half8t pl = __builtin_bit_cast(half8t, Pl[(c * 2 + 1) * 64 + lane]);
+            half8t hv, lv;
+#pragma unroll
+            for (int v = 0; v < 8; v++) {
+                const int qi = v / NA, a = v % NA;
+                float val = 0.f;
+                if (qi < QH) {
+                    const float4t d = Dl[(q0 + qi) * 4 + a];
+                    const float4t h = Tl[(q0 + qi) * 16 + j];
+                    val = sc * (d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3]);
+                }
+                const _Float16 hi = (_Float16)val;
+                hv[v] = hi;
+                lv[v] = (_Float16)(val - (float)hi);
+            }
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, lv, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(pl, hv, acc, 0, 0, 0);
+        }
+        // rows i = 4 g + r, column j
+        if (live && j < t.n_dof) {
+            float* out = ke + t.ke_base + (int64_t)e * t.n_tf * t.ke_ld + j;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 4 * g + r;
+                if (i < t.n_tf) __builtin_nontemporal_store(acc[r] * unsc, out + (int64_t)i * t.ke_ld);
+            }
+        }
+    }
+}
+
+hipError_t launch_tile(const DevBucket& b, const TileSlice& t, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
+    if (b.n_elem == 0) return hipSuccess;
+    if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.n_tf > 16 ||
+        t.n_dof > 16)
+        return hipErrorInvalidValue;
+    const size_t lds = (size_t)t.ngpp * 16 * 16 + (size_t)t.nkc * 2 * 64 * 16 + (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
+    const int64_t groups = ((int64_t)b.n_elem + SLX_TK_WAVES - 1) / SLX_TK_WAVES;
+    const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)n_cu * 16);
+    const dim3 block(64 * SLX_TK_WAVES);
+    switch (t.na) {
+        case 1: hipLaunchKernelGGL(tile_qv_kernel<1>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
+        case 2: hipLaunchKernelGGL(tile_qv_kernel<2>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
+        case 3: hipLaunchKernelGGL(tile_qv_kernel<3>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
+        case 4: hipLaunchKernelGGL(tile_qv_kernel<4>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace slx

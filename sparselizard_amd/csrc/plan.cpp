@@ -18,6 +18,7 @@
 #include "kernels/device.h"
 #include "kernels/mfma.h"
 #include "kernels/factor.h"
+#include "kernels/tile.h"
 #include "kernels/solve.h"
 #include "kernels/symbolic.h"
 #include "coefjit.h"
@@ -200,6 +201,8 @@ struct slx_plan_s {
     std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
     std::vector<std::vector<std::pair<int, FactorSlice>>> factor;  // per bucket: (slice, factorized MFMA data)
     std::vector<uint32_t> factor_mask; // matrix slices handled by the factorized MFMA kernel
+    std::vector<std::vector<std::pair<int, TileSlice>>> tile;  // per bucket: (slice, one-tile-per-element data), FP16 split
+    std::vector<uint32_t> tile_mask;   // matrix slices handled by the one-tile kernel in the FP16 split
     int last_qfast = 0;                // coefficient table layout of the last bucket assembled
     bool factor64 = true;              // FP64 mode: factorized slices on v_mfma_f64 (SLX_TUNE_FP64_FACTOR=0: VALU)
     // fused launch: the factorized slices of every bucket in one persistent
@@ -1381,6 +1384,91 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
     }
 }
 
+// One-tile-per-element data (kernels/tile.h) for the small matrix slices of a
+// bucket whose coefficients vary over the Gauss points and that neither MFMA
+// kernel above takes (FP16 split; the other modes keep the VALU kernel there).
+void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
+    if (bd.q_invariant) return;
+    if (const char* t = std::getenv("SLX_TUNE_TILE"))
+        if (std::atoi(t) == 0) return;
+    // smaller slices waste most of the tile's Q_e build: the VALU element kernel
+    // measured as fast or faster there (6 x 6: P2 triangles, order-1 prisms)
+    int min_m = 100;
+    if (const char* t = std::getenv("SLX_TUNE_TILE_MIN")) min_m = std::atoi(t);
+    for (int s = 0; s < bd.n_slices; s++) {
+        const slx_slice_desc& sd = bd.slices[s];
+        if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u) || ((P.factor_mask[b] >> s) & 1u)) continue;
+        if (sd.n_tf > 16 || sd.n_dof > 16 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_FK_MAXT) continue;
+        std::vector<int> alist;
+        for (int t = 0; t < sd.n_terms; t++)
+            if (std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) == alist.end()) alist.push_back(sd.terms[t].tf_deriv);
+        std::sort(alist.begin(), alist.end());
+        const int na = (int)alist.size(), ngp = bd.n_gp;
+        if (na < 1 || na > 4) continue;
+        const int qh = tile_qh(na), nkc = (ngp + 4 * qh - 1) / (4 * qh), ngpp = nkc * 4 * qh;
+        if (nkc > SLX_TK_MAXC || ngpp > SLX_TK_MAXQ) continue;
+        TileSlice t;
+        std::memset(&t, 0, sizeof(t));
+        t.n_tf = sd.n_tf;
+        t.n_dof = sd.n_dof;
+        t.ngp = ngp;
+        t.nq = bd.n_gp;
+        t.na = na;
+        t.nkc = nkc;
+        t.ngpp = ngpp;
+        t.n_terms = sd.n_terms;
+        std::vector<int32_t> tinfo(3 * sd.n_terms);
+        for (int k = 0; k < sd.n_terms; k++) {
+            tinfo[k] = (int32_t)(std::find(alist.begin(), alist.end(), sd.terms[k].tf_deriv) - alist.begin());
+            tinfo[sd.n_terms + k] = sd.terms[k].dof_deriv;
+            tinfo[2 * sd.n_terms + k] = sd.terms[k].coef;
+        }
+        // P fragments: lane l of chunk c holds P^T[i = l % 16][k = 32 c + 8 (l / 16) + v], k -> Gauss
+        // point (4 c + l / 16) qh + v / na, tf derivative alist[v % na]; power-of-two scale to [128, 256)
+        const double* tft = bd.tables[sd.tf_table];
+        const int nff = bd.table_nff[sd.tf_table];
+        double amax = 0.0;
+        for (int a : alist)
+            for (int i = 0; i < sd.n_tf; i++)
+                for (int q = 0; q < ngp; q++) amax = std::max(amax, std::fabs(tft[((size_t)a * nff + i) * ngp + q]));
+        int aex = 0;
+        if (amax > 0) (void)std::frexp(amax, &aex);
+        const double ascale = std::ldexp(1.0, 8 - aex);
+        t.a_unscale = (float)std::ldexp(1.0, aex - 8);
+        std::vector<uint16_t> frag((size_t)nkc * 2 * 64 * 8, 0);
+        for (int c = 0; c < nkc; c++)
+            for (int l = 0; l < 64; l++)
+                for (int v = 0; v < 8; v++) {
+                    const int i = l % 16, qi = v / na, ai = v % na, q = (4 * c + l / 16) * qh + qi;
+                    float x = 0.f;
+                    if (i < sd.n_tf && qi < qh && q < ngp) x = (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+                    const uint16_t hi = f32_to_f16(x);
+                    frag[(((size_t)c * 2 + 0) * 64 + l) * 8 + v] = hi;
+                    frag[(((size_t)c * 2 + 1) * 64 + l) * 8 + v] = f32_to_f16(x - f16_to_f32(hi));
+                }
+        // weighted dof table [q][j][4], zero past the last Gauss point and dof
+        const double* dft = bd.tables[sd.dof_table];
+        const int nfd = bd.table_nff[sd.dof_table];
+        std::vector<float> tq((size_t)ngpp * 16 * 4, 0.f);
+        double tqmax = 0.0;
+        for (int q = 0; q < ngp; q++)
+            for (int j = 0; j < sd.n_dof; j++)
+                for (int dd = 0; dd < 4; dd++) {
+                    const double x = bd.gp_weights[q] * dft[((size_t)dd * nfd + j) * ngp + q];
+                    tq[((size_t)q * 16 + j) * 4 + dd] = (float)x;
+                    tqmax = std::max(tqmax, std::fabs(x));
+                }
+        t.tqmax = (float)(tqmax * 1.0001);
+        t.tinfo = P.dev(tinfo.data(), tinfo.size());
+        t.afrag = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
+        t.tq = P.dev(tq.data(), tq.size());
+        t.ke_base = L.ke_base[b][s];
+        t.ke_ld = L.ke_ld[b][s];
+        P.tile[b].push_back({s, t});
+        P.tile_mask[b] |= 1u << s;
+    }
+}
+
 void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d);
 
 // The fused launch (kernels/factor.hip, launch_factor_fused): when two or more
@@ -1465,6 +1553,8 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
     P.mfma_mask.assign(d.n_buckets, 0);
     P.factor.assign(d.n_buckets, {});
     P.factor_mask.assign(d.n_buckets, 0);
+    P.tile.assign(d.n_buckets, {});
+    P.tile_mask.assign(d.n_buckets, 0);
     {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, P.ctx->device) == hipSuccess && prop.multiProcessorCount > 0) P.n_cu = prop.multiProcessorCount;
@@ -1579,6 +1669,7 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
             SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
         build_mfma(P, bd, b, L);
         build_factor(P, bd, b, L);
+        build_tile(P, bd, b, L);
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));
@@ -1731,6 +1822,7 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         const DevBucket& b = P.dbuckets[bi];
         const hipStream_t bst = (overlap && P.rhs_only[bi]) ? P.side : st;
         const uint32_t fmask = factor_mask(bi);  // slices on the factorized kernel
+        const uint32_t tmask = arith == AR_F16 ? P.tile_mask[bi] & ~fmask : 0u;  // slices on the one-tile kernel
         const bool in_fused = fused && P.fuse_bucket[bi] && fmask != 0;
         void* ctab = in_fused ? (void*)((char*)P.d_ctab_fused.p + P.fuse_off[bi]) : (P.rhs_only[bi] ? P.d_ctab_side.p : P.d_ctab.p);
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
@@ -1767,6 +1859,13 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
                 }
             if (tm) tm->end();
             done |= fmask;
+        }
+        if (tmask != 0) {
+            if (tm) tm->begin(T_MFMA);
+            for (auto& ts : P.tile[bi])
+                if ((tmask >> ts.first) & 1u) HIPCHECK(launch_tile(b, ts.second, (const float*)ctab, qfast, P.d_ke.as<float>(), P.n_cu, bst));
+            if (tm) tm->end();
+            done |= tmask;
         }
         int p = (precision == SLX_PREC_FP64) ? SLX_PREC_FP64 : SLX_PREC_FP32;
         if ((all & ~done) != 0) {
@@ -2288,6 +2387,17 @@ slx_status slx_plan_kernel_counts(slx_plan p, int32_t precision, int32_t* n_batc
         }
         if (n_batched) *n_batched = nb;
         if (n_factorized) *n_factorized = nf;
+    });
+}
+
+slx_status slx_plan_tile_count(slx_plan p, int32_t precision, int32_t* n_tile) {
+    return guarded([&] {
+        SLX_CHECK(p && n_tile, "null argument");
+        SLX_CHECK(precision >= 0 && precision <= 4, "unknown precision " << precision);
+        int n = 0;
+        if (precision == SLX_PREC_F16_SPLIT)
+            for (auto& v : p->tile) n += (int)v.size();
+        *n_tile = n;
     });
 }
 

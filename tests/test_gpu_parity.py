@@ -91,11 +91,14 @@ def test_f16split_mfma_matches_oracle(ctx, kind, spec, monkeypatch):
     print("errors", errs)
 
 
-def test_f16split_qvarying_buckets_on_valu(ctx):
-    # default kernel choice: Gauss-point-varying coefficients run on the FP32
-    # VALU element kernel, still within the split path's 1e-6 gate
+def test_f16split_qvarying_buckets_on_valu(ctx, monkeypatch):
+    # with the one-tile kernel off (SLX_TUNE_TILE=0), Gauss-point-varying
+    # coefficients run on the FP32 VALU element kernel, still within the split
+    # path's 1e-6 gate (the one-tile kernel has its own tests)
+    monkeypatch.setenv("SLX_TUNE_TILE", "0")
     prob, plan, got, errs = run_case(ctx, "quadbatch", "instances=3;nmin=2;nmax=5", "f16split", TOL_SPLIT)
     assert plan.kernels()["mfma_buckets"] == 0
+    assert plan.tile_count("f16split") == 0
     print("errors", errs)
 
 

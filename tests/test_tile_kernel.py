@@ -1,0 +1,96 @@
+"""One 16x16 MFMA tile per element (kernels/tile.hip) for the small matrix
+slices whose coefficients vary over the Gauss points, FP16 split: the
+element-matrix contraction of contribution.cpp:172-246 regrouped by tf
+derivative, Ke = P^T Q_e with the shared tf table P and the element's
+Q_e[k][j] = w_q sum_b D_e[q][a][b] dof_b[j][q].
+
+Against the CPU FP64 oracle: the CSR pattern bit-exact and the values within
+the split path's 1e-6 relative Frobenius gate, on every element family that
+takes the kernel (order-3 quadrangles, P2 triangles with a Newton field, order-1
+prisms and hexahedra), plus the element matrices against the FP32 VALU kernel
+(SLX_TUNE_TILE=0) that handles the same slices otherwise."""
+import numpy as np
+import pytest
+
+import sparselizard_amd as sa
+from oracle import py_oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL_SPLIT = 1e-6
+
+
+def relerr(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def entries(prob, plan):
+    """Indices of every element-matrix entry (padding between segments excluded)."""
+    kb, kl = plan.ke_layout()
+    d = prob.desc()
+    idx, k = [], 0
+    for b in range(d.n_buckets):
+        bd = d.buckets[b]
+        for s in range(bd.n_slices):
+            sd = bd.slices[s]
+            ndf = 1 if sd.is_rhs else sd.n_dof
+            e, i, j = np.meshgrid(np.arange(bd.n_elem), np.arange(sd.n_tf), np.arange(ndf), indexing="ij")
+            idx.append((kb[k] + (e * sd.n_tf + i) * kl[k] + j).ravel())
+            k += 1
+    return np.concatenate(idx)
+
+
+CASES = [("quadbatch", "instances=3;nmin=2;nmax=5"), ("newton2d", "n=6"), ("prism3d", "n=3"),
+         ("hppoisson3d", "n=3;pmin=1;pmax=2"), ("quadbatch", "instances=2;nmin=7;nmax=7")]
+
+
+@pytest.mark.parametrize("kind,spec", CASES)
+def test_one_tile_kernel_matches_oracle(ctx, kind, spec, monkeypatch):
+    # every slice size (by default the kernel takes n_tf * n_dof >= 100 only)
+    monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
+    print("case tile", kind, spec, flush=True)
+    prob = sa.Problem(ctx, kind, spec)
+    info = prob.info()
+    plan = prob.generate("f16split")
+    assert plan.tile_count("f16split") >= 1, "one-tile kernel not used"
+    assert plan.tile_count("bf16split") == 0 and plan.tile_count("fp64") == 0
+    got = plan.csr(info["ndofs"])
+    ref = py_oracle.assemble(prob.desc_ptr(), prob.numeric(), info["ndofs"], info["nfree"])
+    for name in ("Arow", "Acol", "Drow", "Dcol"):
+        assert np.array_equal(getattr(got, name), getattr(ref, name)), name
+    for name in ("Aval", "Dval", "rhs"):
+        b = getattr(ref, name)
+        if b.size:
+            e = relerr(getattr(got, name), b)
+            print(name, "rel err %.3e" % e)
+            assert e <= TOL_SPLIT, "%s rel err %.3e" % (name, e)
+
+
+@pytest.mark.parametrize("kind,spec", CASES[:3])
+def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
+    # same slices, same layout: the one-tile kernel's element matrices against
+    # the FP32 VALU kernel's (both against the FP64 path's, 1e-6 of its norm)
+    print("case tile vs valu", kind, spec, flush=True)
+    monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
+    prob = sa.Problem(ctx, kind, spec)
+    plan = prob.generate("fp64")
+    sel = entries(prob, plan)
+    k64 = plan.element_matrices()[sel]
+    ktile = prob.generate("f16split").element_matrices()[sel]
+    monkeypatch.setenv("SLX_TUNE_TILE", "0")
+    prob2 = sa.Problem(ctx, kind, spec)
+    plan2 = prob2.generate("f16split")
+    assert plan2.tile_count("f16split") == 0
+    kvalu = plan2.element_matrices()[entries(prob2, plan2)]
+    assert ktile.shape == kvalu.shape == k64.shape
+    e_tile, e_valu = relerr(ktile, k64), relerr(kvalu, k64)
+    print("element matrices: tile %.3e, valu %.3e" % (e_tile, e_valu))
+    assert e_tile <= TOL_SPLIT and e_valu <= TOL_SPLIT
+
+
+def test_one_tile_default_slice_sizes(ctx):
+    # default threshold: order-3 quadrangles (16 x 16) take the kernel, P2
+    # triangles (6 x 6) stay on the VALU element kernel
+    assert sa.Problem(ctx, "quadbatch", "instances=2;nmin=2;nmax=3").generate("f16split").tile_count("f16split") >= 1
+    assert sa.Problem(ctx, "newton2d", "n=4").generate("f16split").tile_count("f16split") == 0
