@@ -15,6 +15,7 @@
 #define SLX_TK_MAXC 8      // k-chunks of 32 entries at most
 #define SLX_TK_MAXQ 64     // padded Gauss points at most (one lane per Gauss point sums the coefficients)
 #define SLX_TK_WAVES 4     // waves (elements in flight) per workgroup
+#define SLX_TK_MAXT 16     // terms of a slice
 
 namespace slx {
 

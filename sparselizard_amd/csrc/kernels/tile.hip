@@ -51,6 +51,25 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     const int per = (int)(((int64_t)n_elem + gridDim.x - 1) / gridDim.x + SLX_TK_WAVES - 1) / SLX_TK_WAVES * SLX_TK_WAVES;
     const int64_t e_hi = (int64_t)per * (blockIdx.x + 1);
     const int e_end = (int)(e_hi < n_elem ? e_hi : (int64_t)n_elem);
+    // the term coefficients of lane q's Gauss point for element e, every load
+    // issued before any is used (and those of the next element while the
+    // current one's MFMAs run)
+    // entry (slot, element, point) at slot * stride + element * es + (lane's point part)
+    const int qq = nq == 1 ? 0 : lane;
+    const int64_t es = qfast ? nq : 1;
+    const float* cl = ctab + (qfast ? (int64_t)qq : (int64_t)qq * n_elem);
+    auto load_c = [&](int e, float (&cv)[SLX_TK_MAXT]) {
+        const bool on = e < e_end && lane < ngp;
+        const int64_t eo = (int64_t)e * es;
+#pragma unroll
+        for (int tt = 0; tt < SLX_TK_MAXT; tt++) {
+            float x = 0.f;
+            if (tt < n_terms && on) x = cl[(int64_t)t_slot[tt] * stride + eo];
+            cv[tt] = x;
+        }
+    };
+    float cv[SLX_TK_MAXT];
+    load_c(per * blockIdx.x + w, cv);
     for (int e0 = per * blockIdx.x; e0 < e_end; e0 += SLX_TK_WAVES) {
         const int e = e0 + w;
         const bool live = e < e_end;
@@ -61,12 +80,9 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
             float* mq = dw + lane * 16;
 #pragma unroll
             for (int x = 0; x < 16; x++) mq[x] = 0.f;
-            if (live && lane < ngp) {
-                const int qq = nq == 1 ? 0 : lane;
-                for (int tt = 0; tt < n_terms; tt++)
-                    mq[t_ai[tt] * 4 + t_b[tt]] += qfast ? ctab[(int64_t)t_slot[tt] * stride + (int64_t)e * nq + qq]
-                                                        : ctab[((int64_t)t_slot[tt] * nq + qq) * n_elem + e];
-            }
+#pragma unroll
+            for (int tt = 0; tt < SLX_TK_MAXT; tt++)
+                if (tt < n_terms) mq[t_ai[tt] * 4 + t_b[tt]] += cv[tt];
 #pragma unroll
             for (int a = 0; a < 4; a++) m = fmaxf(m, fabsf(mq[4 * a]) + fabsf(mq[4 * a + 1]) + fabsf(mq[4 * a + 2]) + fabsf(mq[4 * a + 3]));
         }
@@ -81,6 +97,7 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
             unsc = ldexpf(t.a_unscale, ex - 8);
         }
         __syncthreads();  // the coefficient rows are in LDS
+        load_c(e + SLX_TK_WAVES, cv);  // the next element's, in flight during the MFMAs
         float4t acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int c = 0; c < nkc; c++) {

@@ -1398,7 +1398,7 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u) || ((P.factor_mask[b] >> s) & 1u)) continue;
-        if (sd.n_tf > 16 || sd.n_dof > 16 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_FK_MAXT) continue;
+        if (sd.n_tf > 16 || sd.n_dof > 16 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_TK_MAXT) continue;
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
             if (std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) == alist.end()) alist.push_back(sd.terms[t].tf_deriv);
