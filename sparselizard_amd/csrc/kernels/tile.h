@@ -1,10 +1,11 @@
-// Element matrices of small slices (n_tf, n_dof <= 16) whose coefficients
-// vary over the Gauss points, one 16x16 MFMA tile per element:
+// Element matrices of small slices (n_tf, n_dof <= 32) whose coefficients
+// vary over the Gauss points, one wave per element holding its mtt x ntt
+// 16x16 MFMA tiles (mtt, ntt = 1 or 2):
 //   Ke = P^T Q_e,   P[k][i] = tf_a(k)[i][q(k)]  (shared by the bucket's elements),
 //                   Q_e[k][j] = w_q(k) sum_{terms t with tf derivative a(k)} C_t(e, q(k)) dof_{b_t}[j][q(k)]
 // the reference's sum over terms and Gauss points of tf_t[i][q] w_q C_t(e,q)
 // dof_t[j][q] (contribution.cpp:172-246) regrouped by tf derivative, as in the
-// factorized kernel (kernels/factor.h) but with the whole element in one tile:
+// factorized kernel (kernels/factor.h) but with the whole element in one wave:
 // P and the weighted dof table stay in LDS for the whole launch, and each
 // element builds its Q_e fragments from its coefficients.  FP16
 // hi/lo split with FP32 accumulation (kernels/mfma.h, AR_F16).
@@ -28,18 +29,20 @@ __host__ __device__ constexpr int tile_qh(int na) { return 8 / na; }
 struct TileSlice {
     int32_t n_tf, n_dof, ngp, nq;   // nq: coefficient points per element
     int32_t na, nkc, ngpp;          // tf derivatives, k-chunks, padded Gauss points (nkc * 4 * qh)
+    int32_t mtt, ntt;               // 16-row and 16-column tiles (1 or 2)
     int32_t n_terms;
     // tinfo: [n_terms] tf-derivative index (into the na derivatives), then
     // [n_terms] dof derivative, then [n_terms] coefficient slot
     const int32_t* tinfo;
     float a_unscale;                // 1 / power-of-two scale of P
     float tqmax;                    // max |tq|
-    const uint32_t* afrag;          // [nkc][2 planes][64 lanes][4 words]: lane l of chunk c holds the f16 hi (plane 0) and
-                                    // lo (plane 1) of the scaled P^T[l % 16][32 c + 8 (l / 16) + v], v = 0..7
-    const float* tq;                // [ngpp][16][4]: w_q dof_b[j][q], zero for q >= ngp or j >= n_dof
+    const uint32_t* afrag;          // [nkc][mtt][2 planes][64 lanes][4 words]: lane l of chunk c and row tile mt holds the f16 hi
+                                    // (plane 0) and lo (plane 1) of the scaled P^T[16 mt + l % 16][32 c + 8 (l / 16) + v], v = 0..7
+    const float* tq;                // [ngpp][16 ntt][4]: w_q dof_b[j][q], zero for q >= ngp or j >= n_dof
     int64_t ke_base, ke_ld;
 };
 
+size_t tile_lds_bytes(const TileSlice& t);
 // Element matrices of one slice (coefficient table [slot][element][point]
 // when qfast, else [slot][point][element]).
 hipError_t launch_tile(const DevBucket& b, const TileSlice& t, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st);

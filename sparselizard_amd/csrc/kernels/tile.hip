@@ -1,5 +1,5 @@
-// One 16x16 MFMA tile per element for small slices with Gauss-point-varying
-// coefficients (kernels/tile.h).
+// One to four 16x16 MFMA tiles per element for small slices with
+// Gauss-point-varying coefficients (kernels/tile.h).
 //
 // A workgroup of SLX_TK_WAVES waves loops over the bucket's elements, one
 // element per wave at a time.  Once per workgroup: the weighted dof table
@@ -20,27 +20,28 @@ typedef _Float16 half8t __attribute__((ext_vector_type(8)));
 typedef float float4t __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4t __attribute__((ext_vector_type(4)));
 
-template <int NA>
+template <int NA, int MTT, int NTT>
 __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, TileSlice t, const float* __restrict__ ctab, int qfast, float* __restrict__ ke) {
     constexpr int QH = tile_qh(NA);
+    constexpr int NC = 16 * NTT;  // table columns
     static_assert(QH >= 1, "a lane group holds at least one Gauss point");
-    // dynamic LDS: [ngpp][16] float4 weighted dof table, [nkc][2][64] P fragment
-    // words, [SLX_TK_WAVES][ngpp][4] float4 coefficient rows
+    // dynamic LDS: [ngpp][NC] float4 weighted dof table, [nkc][MTT][2][64] P
+    // fragment words, [SLX_TK_WAVES][ngpp][4] float4 coefficient rows
     extern __shared__ __attribute__((aligned(16))) unsigned char tile_lds[];
     const int nkc = t.nkc, ngpp = t.ngpp, ngp = t.ngp, nq = t.nq, n_terms = t.n_terms;
     float4t* Tl = reinterpret_cast<float4t*>(tile_lds);
-    uint4t* Pl = reinterpret_cast<uint4t*>(Tl + ngpp * 16);
-    float4t* Dall = reinterpret_cast<float4t*>(Pl + nkc * 2 * 64);
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, j = lane & 15;
+    uint4t* Pl = reinterpret_cast<uint4t*>(Tl + ngpp * NC);
+    float4t* Dall = reinterpret_cast<float4t*>(Pl + nkc * MTT * 2 * 64);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, jl = lane & 15;
     typedef __attribute__((address_space(4))) const int32_t cint32;
     const cint32* t_ai = (const cint32*)(t.tinfo);
     const cint32* t_b = t_ai + n_terms;
     const cint32* t_slot = t_ai + 2 * n_terms;
     {
         const float4t* tq4 = reinterpret_cast<const float4t*>(t.tq);
-        for (int x = tid; x < ngpp * 16; x += blockDim.x) Tl[x] = tq4[x];
+        for (int x = tid; x < ngpp * NC; x += blockDim.x) Tl[x] = tq4[x];
         const uint4t* af = reinterpret_cast<const uint4t*>(t.afrag);
-        for (int x = tid; x < nkc * 2 * 64; x += blockDim.x) Pl[x] = af[x];
+        for (int x = tid; x < nkc * MTT * 2 * 64; x += blockDim.x) Pl[x] = af[x];
     }
     float4t* Dl = Dall + w * ngpp * 4;  // this wave's [q][a]
     float* dw = reinterpret_cast<float*>(Dl);
@@ -53,8 +54,8 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     const int e_end = (int)(e_hi < n_elem ? e_hi : (int64_t)n_elem);
     // the term coefficients of lane q's Gauss point for element e, every load
     // issued before any is used (and those of the next element while the
-    // current one's MFMAs run)
-    // entry (slot, element, point) at slot * stride + element * es + (lane's point part)
+    // current one's MFMAs run); entry (slot, element, point) at
+    // slot * stride + element * es + (lane's point part)
     const int qq = nq == 1 ? 0 : lane;
     const int64_t es = qfast ? nq : 1;
     const float* cl = ctab + (qfast ? (int64_t)qq : (int64_t)qq * n_elem);
@@ -98,58 +99,92 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
         }
         __syncthreads();  // the coefficient rows are in LDS
         load_c(e + SLX_TK_WAVES, cv);  // the next element's, in flight during the MFMAs
-        float4t acc = {0.f, 0.f, 0.f, 0.f};
+        float4t acc[MTT][NTT];
+#pragma unroll
+        for (int mt = 0; mt < MTT; mt++)
+#pragma unroll
+            for (int jt = 0; jt < NTT; jt++) acc[mt][jt] = float4t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll 1
         for (int c = 0; c < nkc; c++) {
             const int q0 = (4 * c + g) * QH;
-            const half8t ph = __builtin_bit_cast(half8t, Pl[(c * 2) * 64 + lane]);
-            const half8t pl = __builtin_bit_cast(half8t, Pl[(c * 2 + 1) * 64 + lane]);
-            half8t hv, lv;
+            half8t hv[NTT], lv[NTT];
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 const int qi = v / NA, a = v % NA;
-                float val = 0.f;
-                if (qi < QH) {
-                    const float4t d = Dl[(q0 + qi) * 4 + a];
-                    const float4t h = Tl[(q0 + qi) * 16 + j];
-                    val = sc * (d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3]);
-                }
-                const _Float16 hi = (_Float16)val;
-                hv[v] = hi;
-                lv[v] = (_Float16)(val - (float)hi);
-            }
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, lv, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(pl, hv, acc, 0, 0, 0);
-        }
-        // rows i = 4 g + r, column j
-        if (live && j < t.n_dof) {
-            float* out = ke + t.ke_base + (int64_t)e * t.n_tf * t.ke_ld + j;
+                float4t d = {0.f, 0.f, 0.f, 0.f};
+                if (qi < QH) d = Dl[(q0 + qi) * 4 + a] * sc;
 #pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int i = 4 * g + r;
-                if (i < t.n_tf) __builtin_nontemporal_store(acc[r] * unsc, out + (int64_t)i * t.ke_ld);
+                for (int jt = 0; jt < NTT; jt++) {
+                    float val = 0.f;
+                    if (qi < QH) {
+                        const float4t h = Tl[(q0 + qi) * NC + jt * 16 + jl];
+                        val = d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3];
+                    }
+                    const _Float16 hi = (_Float16)val;
+                    hv[jt][v] = hi;
+                    lv[jt][v] = (_Float16)(val - (float)hi);
+                }
+            }
+#pragma unroll
+            for (int mt = 0; mt < MTT; mt++) {
+                const half8t ph = __builtin_bit_cast(half8t, Pl[((c * MTT + mt) * 2) * 64 + lane]);
+                const half8t pl = __builtin_bit_cast(half8t, Pl[((c * MTT + mt) * 2 + 1) * 64 + lane]);
+#pragma unroll
+                for (int jt = 0; jt < NTT; jt++) {
+                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv[jt], acc[mt][jt], 0, 0, 0);
+                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, lv[jt], acc[mt][jt], 0, 0, 0);
+                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pl, hv[jt], acc[mt][jt], 0, 0, 0);
+                }
+            }
+        }
+        // rows i = 16 mt + 4 g + r, column j = 16 jt + lane % 16
+        if (live) {
+            float* out = ke + t.ke_base + (int64_t)e * t.n_tf * t.ke_ld;
+#pragma unroll
+            for (int jt = 0; jt < NTT; jt++) {
+                const int j = jt * 16 + jl;
+                if (j >= t.n_dof) continue;
+#pragma unroll
+                for (int mt = 0; mt < MTT; mt++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) {
+                        const int i = mt * 16 + 4 * g + r;
+                        if (i < t.n_tf) __builtin_nontemporal_store(acc[mt][jt][r] * unsc, out + (int64_t)i * t.ke_ld + j);
+                    }
             }
         }
     }
 }
 
+size_t tile_lds_bytes(const TileSlice& t) {
+    return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * 2 * 64 * 16 + (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
+}
+
 hipError_t launch_tile(const DevBucket& b, const TileSlice& t, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
     if (b.n_elem == 0) return hipSuccess;
-    if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.n_tf > 16 ||
-        t.n_dof > 16)
+    if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.mtt < 1 || t.mtt > 2 ||
+        t.ntt < 1 || t.ntt > 2 || t.n_tf > 16 * t.mtt || t.n_dof > 16 * t.ntt || tile_lds_bytes(t) > 64 * 1024)
         return hipErrorInvalidValue;
-    const size_t lds = (size_t)t.ngpp * 16 * 16 + (size_t)t.nkc * 2 * 64 * 16 + (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
+    const size_t lds = tile_lds_bytes(t);
     const int64_t groups = ((int64_t)b.n_elem + SLX_TK_WAVES - 1) / SLX_TK_WAVES;
     const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)n_cu * 16);
     const dim3 block(64 * SLX_TK_WAVES);
+#define SLX_TILE_LAUNCH(NA_, M_, N_) \
+    hipLaunchKernelGGL((tile_qv_kernel<NA_, M_, N_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
+#define SLX_TILE_SHAPES(NA_)                              \
+    if (t.mtt == 1 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 1, 1);      \
+    else if (t.mtt == 1) SLX_TILE_LAUNCH(NA_, 1, 2);               \
+    else if (t.ntt == 1) SLX_TILE_LAUNCH(NA_, 2, 1);               \
+    else SLX_TILE_LAUNCH(NA_, 2, 2)
     switch (t.na) {
-        case 1: hipLaunchKernelGGL(tile_qv_kernel<1>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
-        case 2: hipLaunchKernelGGL(tile_qv_kernel<2>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
-        case 3: hipLaunchKernelGGL(tile_qv_kernel<3>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
-        case 4: hipLaunchKernelGGL(tile_qv_kernel<4>, dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); break;
+        case 1: SLX_TILE_SHAPES(1); break;
+        case 2: SLX_TILE_SHAPES(2); break;
+        case 3: SLX_TILE_SHAPES(3); break;
+        case 4: SLX_TILE_SHAPES(4); break;
         default: return hipErrorInvalidValue;
     }
+#undef SLX_TILE_SHAPES
+#undef SLX_TILE_LAUNCH
     return hipGetLastError();
 }
 

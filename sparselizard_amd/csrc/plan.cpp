@@ -1289,6 +1289,8 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
             if (factor_lds_bytes(f, a) <= 160 * 1024 && f.mt * mfma_planes(a) * 64 <= SLX_FK_AREG * 64 * SLX_FK_BWAVES)
                 f.arith_ok |= 1u << a;
         if (!(f.arith_ok & (1u << AR_F16))) continue;
+        // the FP16 split of a slice the one-tile kernel takes runs there
+        if ((P.tile_mask[b] >> s) & 1u) f.arith_ok &= ~(1u << AR_F16);
         // P fragments: lane l of tile (ks, mt) holds P^T[i = mt*16 + l%16][k = 8*(l/16) + v],
         // k -> (Gauss point, tf derivative) by segments (factor_segw); power-of-two scale to [128, 256)
         const double* tft = bd.tables[sd.tf_table];
@@ -1397,8 +1399,8 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     if (const char* t = std::getenv("SLX_TUNE_TILE_MIN")) min_m = std::atoi(t);
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
-        if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u) || ((P.factor_mask[b] >> s) & 1u)) continue;
-        if (sd.n_tf > 16 || sd.n_dof > 16 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_TK_MAXT) continue;
+        if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
+        if (sd.n_tf > 32 || sd.n_dof > 32 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_TK_MAXT) continue;
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
             if (std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) == alist.end()) alist.push_back(sd.terms[t].tf_deriv);
@@ -1409,6 +1411,8 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         if (nkc > SLX_TK_MAXC || ngpp > SLX_TK_MAXQ) continue;
         TileSlice t;
         std::memset(&t, 0, sizeof(t));
+        t.mtt = (sd.n_tf + 15) / 16;
+        t.ntt = (sd.n_dof + 15) / 16;
         t.n_tf = sd.n_tf;
         t.n_dof = sd.n_dof;
         t.ngp = ngp;
@@ -1417,14 +1421,15 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         t.nkc = nkc;
         t.ngpp = ngpp;
         t.n_terms = sd.n_terms;
+        if (tile_lds_bytes(t) > 64 * 1024) continue;
         std::vector<int32_t> tinfo(3 * sd.n_terms);
         for (int k = 0; k < sd.n_terms; k++) {
             tinfo[k] = (int32_t)(std::find(alist.begin(), alist.end(), sd.terms[k].tf_deriv) - alist.begin());
             tinfo[sd.n_terms + k] = sd.terms[k].dof_deriv;
             tinfo[2 * sd.n_terms + k] = sd.terms[k].coef;
         }
-        // P fragments: lane l of chunk c holds P^T[i = l % 16][k = 32 c + 8 (l / 16) + v], k -> Gauss
-        // point (4 c + l / 16) qh + v / na, tf derivative alist[v % na]; power-of-two scale to [128, 256)
+        // P fragments: lane l of chunk c and row tile mt holds P^T[i = 16 mt + l % 16][k = 32 c + 8 (l / 16) + v],
+        // k -> Gauss point (4 c + l / 16) qh + v / na, tf derivative alist[v % na]; power-of-two scale to [128, 256)
         const double* tft = bd.tables[sd.tf_table];
         const int nff = bd.table_nff[sd.tf_table];
         double amax = 0.0;
@@ -1435,27 +1440,30 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         if (amax > 0) (void)std::frexp(amax, &aex);
         const double ascale = std::ldexp(1.0, 8 - aex);
         t.a_unscale = (float)std::ldexp(1.0, aex - 8);
-        std::vector<uint16_t> frag((size_t)nkc * 2 * 64 * 8, 0);
+        std::vector<uint16_t> frag((size_t)nkc * t.mtt * 2 * 64 * 8, 0);
         for (int c = 0; c < nkc; c++)
-            for (int l = 0; l < 64; l++)
-                for (int v = 0; v < 8; v++) {
-                    const int i = l % 16, qi = v / na, ai = v % na, q = (4 * c + l / 16) * qh + qi;
-                    float x = 0.f;
-                    if (i < sd.n_tf && qi < qh && q < ngp) x = (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
-                    const uint16_t hi = f32_to_f16(x);
-                    frag[(((size_t)c * 2 + 0) * 64 + l) * 8 + v] = hi;
-                    frag[(((size_t)c * 2 + 1) * 64 + l) * 8 + v] = f32_to_f16(x - f16_to_f32(hi));
-                }
+            for (int mt = 0; mt < t.mtt; mt++)
+                for (int l = 0; l < 64; l++)
+                    for (int v = 0; v < 8; v++) {
+                        const int i = 16 * mt + l % 16, qi = v / na, ai = v % na, q = (4 * c + l / 16) * qh + qi;
+                        float x = 0.f;
+                        if (i < sd.n_tf && qi < qh && q < ngp) x = (float)(tft[((size_t)alist[ai] * nff + i) * ngp + q] * ascale);
+                        const uint16_t hi = f32_to_f16(x);
+                        const size_t w = ((size_t)c * t.mtt + mt) * 2;
+                        frag[((w + 0) * 64 + l) * 8 + v] = hi;
+                        frag[((w + 1) * 64 + l) * 8 + v] = f32_to_f16(x - f16_to_f32(hi));
+                    }
         // weighted dof table [q][j][4], zero past the last Gauss point and dof
         const double* dft = bd.tables[sd.dof_table];
         const int nfd = bd.table_nff[sd.dof_table];
-        std::vector<float> tq((size_t)ngpp * 16 * 4, 0.f);
+        const int nc = 16 * t.ntt;
+        std::vector<float> tq((size_t)ngpp * nc * 4, 0.f);
         double tqmax = 0.0;
         for (int q = 0; q < ngp; q++)
             for (int j = 0; j < sd.n_dof; j++)
                 for (int dd = 0; dd < 4; dd++) {
                     const double x = bd.gp_weights[q] * dft[((size_t)dd * nfd + j) * ngp + q];
-                    tq[((size_t)q * 16 + j) * 4 + dd] = (float)x;
+                    tq[((size_t)q * nc + j) * 4 + dd] = (float)x;
                     tqmax = std::max(tqmax, std::fabs(x));
                 }
         t.tqmax = (float)(tqmax * 1.0001);
@@ -1668,8 +1676,8 @@ void upload_buckets(slx_plan_s& P, const slx_plan_desc& d, const KeLayout& L) {
         for (size_t k = 0; k < (size_t)bd.n_elem * bd.n_geo; k++)
             SLX_CHECK(bd.geo_nodes[k] >= 0 && bd.geo_nodes[k] < d.n_nodes, "element node out of range");
         build_mfma(P, bd, b, L);
-        build_factor(P, bd, b, L);
         build_tile(P, bd, b, L);
+        build_factor(P, bd, b, L);
     }
     upload(P.consts, d.consts, (size_t)std::max(d.n_consts, 1), P.stream);
     P.d_ctab.alloc(ctab_words * sizeof(double));

@@ -41,8 +41,13 @@ def entries(prob, plan):
     return np.concatenate(idx)
 
 
+# one tile (quadrangles order 3, P2 triangles, order-1 prisms and hexahedra) and
+# 2 x 2 tiles (order-2 hexahedra 27 x 27, order-2 prisms 18 x 18, order-2 hcurl
+# tetrahedra 30 x 30 with value and curl tables)
 CASES = [("quadbatch", "instances=3;nmin=2;nmax=5"), ("newton2d", "n=6"), ("prism3d", "n=3"),
-         ("hppoisson3d", "n=3;pmin=1;pmax=2"), ("quadbatch", "instances=2;nmin=7;nmax=7")]
+         ("hppoisson3d", "n=3;pmin=1;pmax=2"), ("quadbatch", "instances=2;nmin=7;nmax=7"),
+         ("prism3d", "n=2;order=2"), ("hcurlprojection", "elem=tet;n=2;order=2;term=maxwell"),
+         ("hppoisson3d", "n=2;pmin=2;pmax=2")]
 
 
 @pytest.mark.parametrize("kind,spec", CASES)
@@ -67,10 +72,12 @@ def test_one_tile_kernel_matches_oracle(ctx, kind, spec, monkeypatch):
             assert e <= TOL_SPLIT, "%s rel err %.3e" % (name, e)
 
 
-@pytest.mark.parametrize("kind,spec", CASES[:3])
+@pytest.mark.parametrize("kind,spec", CASES[:3] + CASES[5:7])
 def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
     # same slices, same layout: the one-tile kernel's element matrices against
-    # the FP32 VALU kernel's (both against the FP64 path's, 1e-6 of its norm)
+    # those of the kernel that takes the slices without it (the FP32 VALU kernel,
+    # or the factorized MFMA kernel for 2 x 2 tiles), both against the FP64
+    # path's (1e-6 of its norm)
     print("case tile vs valu", kind, spec, flush=True)
     monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
     prob = sa.Problem(ctx, kind, spec)
@@ -85,7 +92,7 @@ def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
     kvalu = plan2.element_matrices()[entries(prob2, plan2)]
     assert ktile.shape == kvalu.shape == k64.shape
     e_tile, e_valu = relerr(ktile, k64), relerr(kvalu, k64)
-    print("element matrices: tile %.3e, valu %.3e" % (e_tile, e_valu))
+    print("element matrices: tile %.3e, other kernel %.3e" % (e_tile, e_valu))
     assert e_tile <= TOL_SPLIT and e_valu <= TOL_SPLIT
 
 
