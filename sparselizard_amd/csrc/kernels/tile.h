@@ -1,20 +1,21 @@
-// Element matrices of small slices (n_tf, n_dof <= 32) whose coefficients
+// Element matrices of small slices (n_tf, n_dof <= 64) whose coefficients
 // vary over the Gauss points, one wave per element holding its mtt x ntt
-// 16x16 MFMA tiles (mtt, ntt = 1 or 2):
+// 16x16 MFMA tiles (mtt, ntt = 1 or 2, or 3 x 3 or 4 x 4):
 //   Ke = P^T Q_e,   P[k][i] = tf_a(k)[i][q(k)]  (shared by the bucket's elements),
 //                   Q_e[k][j] = w_q(k) sum_{terms t with tf derivative a(k)} C_t(e, q(k)) dof_{b_t}[j][q(k)]
 // the reference's sum over terms and Gauss points of tf_t[i][q] w_q C_t(e,q)
 // dof_t[j][q] (contribution.cpp:172-246) regrouped by tf derivative, as in the
 // factorized kernel (kernels/factor.h) but with the whole element in one wave:
-// P and the weighted dof table stay in LDS for the whole launch, and each
-// element builds its Q_e fragments from its coefficients.  FP16
+// P and the weighted dof table stay in LDS for the whole launch (up to 2 x 2
+// tiles; read through the caches for the larger ones), and each element
+// builds its Q_e fragments from its coefficients.  FP16
 // hi/lo split with FP32 accumulation (kernels/mfma.h, AR_F16).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device.h"
 
-#define SLX_TK_MAXC 8      // k-chunks of 32 entries at most
-#define SLX_TK_MAXQ 64     // padded Gauss points at most (one lane per Gauss point sums the coefficients)
+#define SLX_TK_MAXC 16     // k-chunks of 32 entries at most
+#define SLX_TK_MAXQ 128    // padded Gauss points at most (the lanes sum the coefficients in two passes)
 #define SLX_TK_WAVES 4     // waves (elements in flight) per workgroup
 #define SLX_TK_MAXT 16     // terms of a slice
 
@@ -29,7 +30,7 @@ __host__ __device__ constexpr int tile_qh(int na) { return 8 / na; }
 struct TileSlice {
     int32_t n_tf, n_dof, ngp, nq;   // nq: coefficient points per element
     int32_t na, nkc, ngpp;          // tf derivatives, k-chunks, padded Gauss points (nkc * 4 * qh)
-    int32_t mtt, ntt;               // 16-row and 16-column tiles (1 or 2)
+    int32_t mtt, ntt;               // 16-row and 16-column tiles (1 or 2 each, or 3 x 3, or 4 x 4)
     int32_t n_terms;
     // tinfo: [n_terms] tf-derivative index (into the na derivatives), then
     // [n_terms] dof derivative, then [n_terms] coefficient slot

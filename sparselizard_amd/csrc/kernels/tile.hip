@@ -1,5 +1,5 @@
-// One to four 16x16 MFMA tiles per element for small slices with
-// Gauss-point-varying coefficients (kernels/tile.h).
+// One wave per element holding its 16x16 MFMA tiles (up to 4 x 4) for the
+// smaller slices with Gauss-point-varying coefficients (kernels/tile.h).
 //
 // A workgroup of SLX_TK_WAVES waves loops over the bucket's elements, one
 // element per wave at a time.  Once per workgroup: the weighted dof table
@@ -20,29 +20,33 @@ typedef _Float16 half8t __attribute__((ext_vector_type(8)));
 typedef float float4t __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4t __attribute__((ext_vector_type(4)));
 
-template <int NA, int MTT, int NTT>
+template <int NA, int MTT, int NTT, bool GT>
 __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, TileSlice t, const float* __restrict__ ctab, int qfast, float* __restrict__ ke) {
     constexpr int QH = tile_qh(NA);
     constexpr int NC = 16 * NTT;  // table columns
     static_assert(QH >= 1, "a lane group holds at least one Gauss point");
-    // dynamic LDS: [ngpp][NC] float4 weighted dof table, [nkc][MTT][2][64] P
-    // fragment words, [SLX_TK_WAVES][ngpp][4] float4 coefficient rows
+    // dynamic LDS: [ngpp][NC] float4 weighted dof table and [nkc][MTT][2][64] P
+    // fragment words (both read from global memory instead when GT: the larger
+    // tiles' tables would not fit), then [SLX_TK_WAVES][ngpp][4] float4
+    // coefficient rows
     extern __shared__ __attribute__((aligned(16))) unsigned char tile_lds[];
     const int nkc = t.nkc, ngpp = t.ngpp, ngp = t.ngp, nq = t.nq, n_terms = t.n_terms;
+    const float4t* tq4 = reinterpret_cast<const float4t*>(t.tq);
+    const uint4t* af = reinterpret_cast<const uint4t*>(t.afrag);
     float4t* Tl = reinterpret_cast<float4t*>(tile_lds);
-    uint4t* Pl = reinterpret_cast<uint4t*>(Tl + ngpp * NC);
-    float4t* Dall = reinterpret_cast<float4t*>(Pl + nkc * MTT * 2 * 64);
+    uint4t* Pl = reinterpret_cast<uint4t*>(Tl + (GT ? 0 : ngpp * NC));
+    float4t* Dall = reinterpret_cast<float4t*>(Pl + (GT ? 0 : nkc * MTT * 2 * 64));
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, jl = lane & 15;
     typedef __attribute__((address_space(4))) const int32_t cint32;
     const cint32* t_ai = (const cint32*)(t.tinfo);
     const cint32* t_b = t_ai + n_terms;
     const cint32* t_slot = t_ai + 2 * n_terms;
-    {
-        const float4t* tq4 = reinterpret_cast<const float4t*>(t.tq);
+    if constexpr (!GT) {
         for (int x = tid; x < ngpp * NC; x += blockDim.x) Tl[x] = tq4[x];
-        const uint4t* af = reinterpret_cast<const uint4t*>(t.afrag);
         for (int x = tid; x < nkc * MTT * 2 * 64; x += blockDim.x) Pl[x] = af[x];
     }
+    const float4t* Tsrc = GT ? tq4 : Tl;
+    const uint4t* Psrc = GT ? af : Pl;
     float4t* Dl = Dall + w * ngpp * 4;  // this wave's [q][a]
     float* dw = reinterpret_cast<float*>(Dl);
     const int64_t stride = (int64_t)nq * n_elem;
@@ -52,38 +56,43 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     const int per = (int)(((int64_t)n_elem + gridDim.x - 1) / gridDim.x + SLX_TK_WAVES - 1) / SLX_TK_WAVES * SLX_TK_WAVES;
     const int64_t e_hi = (int64_t)per * (blockIdx.x + 1);
     const int e_end = (int)(e_hi < n_elem ? e_hi : (int64_t)n_elem);
-    // the term coefficients of lane q's Gauss point for element e, every load
-    // issued before any is used (and those of the next element while the
-    // current one's MFMAs run); entry (slot, element, point) at
-    // slot * stride + element * es + (lane's point part)
-    const int qq = nq == 1 ? 0 : lane;
-    const int64_t es = qfast ? nq : 1;
-    const float* cl = ctab + (qfast ? (int64_t)qq : (int64_t)qq * n_elem);
-    auto load_c = [&](int e, float (&cv)[SLX_TK_MAXT]) {
-        const bool on = e < e_end && lane < ngp;
-        const int64_t eo = (int64_t)e * es;
+    // the term coefficients of Gauss point q for element e; entry (slot, element,
+    // point) at slot * stride + element * es + point * ps.  Lane q's are loaded
+    // before any is used (and those of the next element while the current one's
+    // MFMAs run); points q >= 64 (a second pass of the lanes) load on use.
+    const int64_t es = qfast ? nq : 1, ps = nq == 1 ? 0 : (qfast ? 1 : (int64_t)n_elem);
+    auto load_c = [&](int e, int q, float (&cv)[SLX_TK_MAXT]) {
+        const bool on = e < e_end && q < ngp;
+        const float* cq = ctab + (int64_t)e * es + (int64_t)q * ps;
 #pragma unroll
         for (int tt = 0; tt < SLX_TK_MAXT; tt++) {
             float x = 0.f;
-            if (tt < n_terms && on) x = cl[(int64_t)t_slot[tt] * stride + eo];
+            if (tt < n_terms && on) x = cq[(int64_t)t_slot[tt] * stride];
             cv[tt] = x;
         }
     };
     float cv[SLX_TK_MAXT];
-    load_c(per * blockIdx.x + w, cv);
+    load_c(per * blockIdx.x + w, lane, cv);
     for (int e0 = per * blockIdx.x; e0 < e_end; e0 += SLX_TK_WAVES) {
         const int e = e0 + w;
         const bool live = e < e_end;
         __syncthreads();  // the tables are in LDS; the previous element's coefficient rows are read
         // coefficient matrices of the element: lane q sums the terms of its Gauss point
         float m = 0.f;
-        if (lane < ngpp) {
-            float* mq = dw + lane * 16;
+        for (int q = lane; q < ngpp; q += 64) {
+            float cw[SLX_TK_MAXT];
+            if (q == lane) {
+#pragma unroll
+                for (int tt = 0; tt < SLX_TK_MAXT; tt++) cw[tt] = cv[tt];
+            } else {
+                load_c(e, q, cw);
+            }
+            float* mq = dw + q * 16;
 #pragma unroll
             for (int x = 0; x < 16; x++) mq[x] = 0.f;
 #pragma unroll
             for (int tt = 0; tt < SLX_TK_MAXT; tt++)
-                if (tt < n_terms) mq[t_ai[tt] * 4 + t_b[tt]] += cv[tt];
+                if (tt < n_terms) mq[t_ai[tt] * 4 + t_b[tt]] += cw[tt];
 #pragma unroll
             for (int a = 0; a < 4; a++) m = fmaxf(m, fabsf(mq[4 * a]) + fabsf(mq[4 * a + 1]) + fabsf(mq[4 * a + 2]) + fabsf(mq[4 * a + 3]));
         }
@@ -98,7 +107,7 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
             unsc = ldexpf(t.a_unscale, ex - 8);
         }
         __syncthreads();  // the coefficient rows are in LDS
-        load_c(e + SLX_TK_WAVES, cv);  // the next element's, in flight during the MFMAs
+        load_c(e + SLX_TK_WAVES, lane, cv);  // the next element's, in flight during the MFMAs
         float4t acc[MTT][NTT];
 #pragma unroll
         for (int mt = 0; mt < MTT; mt++)
@@ -112,13 +121,13 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
             for (int v = 0; v < 8; v++) {
                 const int qi = v / NA, a = v % NA;
                 float4t d = {0.f, 0.f, 0.f, 0.f};
-                if (qi < QH) d = Dl[(q0 + qi) * 4 + a] * sc;
+                if (qi < QH) d = Dl[(q0 + qi) * 4 + a];
 #pragma unroll
                 for (int jt = 0; jt < NTT; jt++) {
                     float val = 0.f;
                     if (qi < QH) {
-                        const float4t h = Tl[(q0 + qi) * NC + jt * 16 + jl];
-                        val = d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3];
+                        const float4t h = Tsrc[(q0 + qi) * NC + jt * 16 + jl];
+                        val = sc * (d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3]);
                     }
                     const _Float16 hi = (_Float16)val;
                     hv[jt][v] = hi;
@@ -127,8 +136,8 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
             }
 #pragma unroll
             for (int mt = 0; mt < MTT; mt++) {
-                const half8t ph = __builtin_bit_cast(half8t, Pl[((c * MTT + mt) * 2) * 64 + lane]);
-                const half8t pl = __builtin_bit_cast(half8t, Pl[((c * MTT + mt) * 2 + 1) * 64 + lane]);
+                const half8t ph = __builtin_bit_cast(half8t, Psrc[((c * MTT + mt) * 2) * 64 + lane]);
+                const half8t pl = __builtin_bit_cast(half8t, Psrc[((c * MTT + mt) * 2 + 1) * 64 + lane]);
 #pragma unroll
                 for (int jt = 0; jt < NTT; jt++) {
                     acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv[jt], acc[mt][jt], 0, 0, 0);
@@ -156,26 +165,35 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     }
 }
 
+// P and the weighted dof table in LDS for up to 2 x 2 tiles, read through the
+// caches for 3 x 3 and 4 x 4 (the square tile count of the larger side)
+static bool tile_global_tables(const TileSlice& t) { return t.mtt > 2 || t.ntt > 2; }
+
 size_t tile_lds_bytes(const TileSlice& t) {
-    return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * 2 * 64 * 16 + (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
+    const size_t rows = (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
+    if (tile_global_tables(t)) return rows;
+    return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * 2 * 64 * 16 + rows;
 }
 
 hipError_t launch_tile(const DevBucket& b, const TileSlice& t, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
     if (b.n_elem == 0) return hipSuccess;
-    if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.mtt < 1 || t.mtt > 2 ||
-        t.ntt < 1 || t.ntt > 2 || t.n_tf > 16 * t.mtt || t.n_dof > 16 * t.ntt || tile_lds_bytes(t) > 64 * 1024)
+    if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.mtt < 1 || t.mtt > 4 ||
+        t.ntt < 1 || t.ntt > 4 || (tile_global_tables(t) && t.mtt != t.ntt) || t.n_tf > 16 * t.mtt || t.n_dof > 16 * t.ntt ||
+        tile_lds_bytes(t) > 64 * 1024)
         return hipErrorInvalidValue;
     const size_t lds = tile_lds_bytes(t);
     const int64_t groups = ((int64_t)b.n_elem + SLX_TK_WAVES - 1) / SLX_TK_WAVES;
     const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)n_cu * 16);
     const dim3 block(64 * SLX_TK_WAVES);
-#define SLX_TILE_LAUNCH(NA_, M_, N_) \
-    hipLaunchKernelGGL((tile_qv_kernel<NA_, M_, N_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
-#define SLX_TILE_SHAPES(NA_)                              \
-    if (t.mtt == 1 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 1, 1);      \
-    else if (t.mtt == 1) SLX_TILE_LAUNCH(NA_, 1, 2);               \
-    else if (t.ntt == 1) SLX_TILE_LAUNCH(NA_, 2, 1);               \
-    else SLX_TILE_LAUNCH(NA_, 2, 2)
+#define SLX_TILE_LAUNCH(NA_, M_, N_, G_) \
+    hipLaunchKernelGGL((tile_qv_kernel<NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
+#define SLX_TILE_SHAPES(NA_)                                         \
+    if (t.mtt == 1 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 1, 1, false);         \
+    else if (t.mtt == 1 && t.ntt == 2) SLX_TILE_LAUNCH(NA_, 1, 2, false);    \
+    else if (t.mtt == 2 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 2, 1, false);    \
+    else if (t.mtt == 2 && t.ntt == 2) SLX_TILE_LAUNCH(NA_, 2, 2, false);    \
+    else if (t.mtt == 3) SLX_TILE_LAUNCH(NA_, 3, 3, true);                   \
+    else SLX_TILE_LAUNCH(NA_, 4, 4, true)
     switch (t.na) {
         case 1: SLX_TILE_SHAPES(1); break;
         case 2: SLX_TILE_SHAPES(2); break;

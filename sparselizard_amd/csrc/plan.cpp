@@ -913,7 +913,7 @@ void split_bf16(float x, uint16_t& hi, uint16_t& mid, uint16_t& lo) {
 // of the matrix slices of buckets of that class: mfma (batched, mtw M tiles
 // per wave), valu (FP32 element kernel) or factor (factorized MFMA).
 // Classes the table does not list use the built-in rules below.
-enum { TK_NONE = 0, TK_MFMA = 1, TK_VALU = 2, TK_FACTOR = 3 };
+enum { TK_NONE = 0, TK_MFMA = 1, TK_VALU = 2, TK_FACTOR = 3, TK_TILE = 4 };
 // kernel_table.txt rows: elem_type n_tf n_dof q_invariant kernel mtw [variant]
 // (variant: tile blocks per wave of the factorized kernel, 0 = (2, 1),
 // 1 = (4, 2), 2 = (7, 2); absent or -1: the smallest that covers the matrix)
@@ -955,7 +955,7 @@ const std::vector<TuneEntry>& tune_table() {
             char kern[32] = {0};
             if (std::sscanf(line, "%d %d %d %d %31s %d %d", &x.type, &x.n_tf, &x.n_dof, &x.q_inv, kern, &x.mtw, &x.variant) < 6) continue;
             const std::string k = kern;
-            x.kernel = k == "mfma" ? TK_MFMA : k == "valu" ? TK_VALU : k == "factor" ? TK_FACTOR : TK_NONE;
+            x.kernel = k == "mfma" ? TK_MFMA : k == "valu" ? TK_VALU : k == "factor" ? TK_FACTOR : k == "tile" ? TK_TILE : TK_NONE;
             if (x.kernel != TK_NONE) t.push_back(x);
         }
         std::fclose(f);
@@ -1009,7 +1009,7 @@ void build_mfma(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     if (ms.empty() || (int)ms.size() > SLX_MAX_SLICES) return;
     int tuned_mtw = 1;
     const int tuned = tune_lookup(bd.elem_type, n_tf, n_dof, bd.q_invariant ? 1 : 0, &tuned_mtw);
-    if (tuned == TK_VALU || tuned == TK_FACTOR) return;
+    if (tuned == TK_VALU || tuned == TK_FACTOR || tuned == TK_TILE) return;
     if (tuned == TK_NONE && n_tf * n_dof > batched_max_m()) return;  // the factorized kernel takes larger slices
     mb.M = n_tf * n_dof;
     mb.n_tf = n_tf;
@@ -1400,7 +1400,10 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
     for (int s = 0; s < bd.n_slices; s++) {
         const slx_slice_desc& sd = bd.slices[s];
         if (sd.is_rhs || ((P.mfma_mask[b] >> s) & 1u)) continue;
-        if (sd.n_tf > 32 || sd.n_dof > 32 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_TK_MAXT) continue;
+        if (sd.n_tf > 64 || sd.n_dof > 64 || sd.n_tf * sd.n_dof < min_m || sd.n_terms < 1 || sd.n_terms > SLX_TK_MAXT) continue;
+        // a kernel-table entry for the class decides; without one the slice takes the kernel
+        const int tuned = tune_lookup(bd.elem_type, sd.n_tf, sd.n_dof, 0, nullptr);
+        if (tuned != TK_NONE && tuned != TK_TILE) continue;
         std::vector<int> alist;
         for (int t = 0; t < sd.n_terms; t++)
             if (std::find(alist.begin(), alist.end(), sd.terms[t].tf_deriv) == alist.end()) alist.push_back(sd.terms[t].tf_deriv);
@@ -1413,6 +1416,7 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         std::memset(&t, 0, sizeof(t));
         t.mtt = (sd.n_tf + 15) / 16;
         t.ntt = (sd.n_dof + 15) / 16;
+        if (t.mtt > 2 || t.ntt > 2) t.mtt = t.ntt = std::max(t.mtt, t.ntt);  // 3 x 3 or 4 x 4
         t.n_tf = sd.n_tf;
         t.n_dof = sd.n_dof;
         t.ngp = ngp;

@@ -6,7 +6,9 @@ times the split-precision assembly (HIP events per launch, slx_plan_profile)
 with each candidate for that class alone -- the batched MFMA kernel with one
 or two M tiles per wave (16 elements per workgroup tile), the factorized MFMA
 kernel with each tile shape that covers the matrix ((2, 1), (4, 2) or (7, 2)
-16x16 tiles per wave in rows and columns), or the FP32 VALU element kernel --
+16x16 tiles per wave in rows and columns), the one-tile kernel (one element per
+wave, up to 4 x 4 tiles; Gauss-point-varying classes up to 64 x 64), or the FP32
+VALU element kernel --
 while the workload's other classes keep the built-in choice.  Candidates whose
 global-matrix error against the FP64 assembly exceeds 1e-6 are dropped, and
 the fastest per class goes to kernel_table.txt (next to this script), which
@@ -50,11 +52,12 @@ def candidates(c):
     larger element matrices (a shape must cover the rows with its 2 row waves), the
     VALU kernel takes everything.  The first is the reference of the 3 % rule."""
     m = c[1] * c[2]
+    tile = [("tile", 1, -1)] if c[3] == 0 and c[1] <= 64 and c[2] <= 64 else []
     if m <= 256:
-        return [("mfma", 1, -1), ("mfma", 2, -1), ("valu", 1, -1)]
+        return [("mfma", 1, -1), ("mfma", 2, -1)] + tile + [("valu", 1, -1)]
     mt = (c[1] + 15) // 16
     out = [("factor", 1, v) for v, (bm, bn) in enumerate(FACTOR_SHAPES) if 2 * bm >= mt]
-    return out + [("valu", 1, -1)]
+    return out + tile + [("valu", 1, -1)]
 
 
 def classes_of(kind, spec):

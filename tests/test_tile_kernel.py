@@ -41,19 +41,23 @@ def entries(prob, plan):
     return np.concatenate(idx)
 
 
-# one tile (quadrangles order 3, P2 triangles, order-1 prisms and hexahedra) and
+# one tile (quadrangles order 3, P2 triangles, order-1 prisms and hexahedra),
 # 2 x 2 tiles (order-2 hexahedra 27 x 27, order-2 prisms 18 x 18, order-2 hcurl
-# tetrahedra 30 x 30 with value and curl tables)
+# tetrahedra 30 x 30 with value and curl tables), 3 x 3 (order-3 prisms 40 x 40
+# at 80 Gauss points: two passes of the lanes over the points) and 4 x 4
+# (order-3 hexahedra 64 x 64)
 CASES = [("quadbatch", "instances=3;nmin=2;nmax=5"), ("newton2d", "n=6"), ("prism3d", "n=3"),
          ("hppoisson3d", "n=3;pmin=1;pmax=2"), ("quadbatch", "instances=2;nmin=7;nmax=7"),
          ("prism3d", "n=2;order=2"), ("hcurlprojection", "elem=tet;n=2;order=2;term=maxwell"),
-         ("hppoisson3d", "n=2;pmin=2;pmax=2")]
+         ("hppoisson3d", "n=2;pmin=2;pmax=2"), ("prism3d", "n=2;order=3"), ("hppoisson3d", "n=2;pmin=3;pmax=3")]
 
 
 @pytest.mark.parametrize("kind,spec", CASES)
 def test_one_tile_kernel_matches_oracle(ctx, kind, spec, monkeypatch):
-    # every slice size (by default the kernel takes n_tf * n_dof >= 100 only)
+    # every slice size (by default the kernel takes n_tf * n_dof >= 100 only),
+    # whatever the kernel table names for the class
     monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
+    monkeypatch.setenv("SLX_TUNE_TABLE", "none")
     print("case tile", kind, spec, flush=True)
     prob = sa.Problem(ctx, kind, spec)
     info = prob.info()
@@ -72,7 +76,7 @@ def test_one_tile_kernel_matches_oracle(ctx, kind, spec, monkeypatch):
             assert e <= TOL_SPLIT, "%s rel err %.3e" % (name, e)
 
 
-@pytest.mark.parametrize("kind,spec", CASES[:3] + CASES[5:7])
+@pytest.mark.parametrize("kind,spec", CASES[:3] + CASES[5:7] + CASES[8:])
 def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
     # same slices, same layout: the one-tile kernel's element matrices against
     # those of the kernel that takes the slices without it (the FP32 VALU kernel,
@@ -80,6 +84,7 @@ def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
     # path's (1e-6 of its norm)
     print("case tile vs valu", kind, spec, flush=True)
     monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
+    monkeypatch.setenv("SLX_TUNE_TABLE", "none")
     prob = sa.Problem(ctx, kind, spec)
     plan = prob.generate("fp64")
     sel = entries(prob, plan)
@@ -96,8 +101,22 @@ def test_one_tile_element_matrices_match_valu(ctx, kind, spec, monkeypatch):
     assert e_tile <= TOL_SPLIT and e_valu <= TOL_SPLIT
 
 
-def test_one_tile_default_slice_sizes(ctx):
+def test_one_tile_default_slice_sizes(ctx, monkeypatch):
     # default threshold: order-3 quadrangles (16 x 16) take the kernel, P2
     # triangles (6 x 6) stay on the VALU element kernel
+    monkeypatch.setenv("SLX_TUNE_TABLE", "none")
     assert sa.Problem(ctx, "quadbatch", "instances=2;nmin=2;nmax=3").generate("f16split").tile_count("f16split") >= 1
     assert sa.Problem(ctx, "newton2d", "n=4").generate("f16split").tile_count("f16split") == 0
+
+
+def test_kernel_table_decides(ctx, tmp_path, monkeypatch):
+    # a table entry naming another kernel keeps the class off the one-tile kernel;
+    # a "tile" entry or none leaves it there
+    # (the library reads a table again when SLX_TUNE_TABLE names another file)
+    valu, tile = tmp_path / "valu.txt", tmp_path / "tile.txt"
+    valu.write_text("3 16 16 0 valu 1 -1\n")
+    tile.write_text("3 16 16 0 tile 1 -1\n")
+    monkeypatch.setenv("SLX_TUNE_TABLE", str(valu))
+    assert sa.Problem(ctx, "quadbatch", "instances=2;nmin=2;nmax=3").generate("f16split").tile_count("f16split") == 0
+    monkeypatch.setenv("SLX_TUNE_TABLE", str(tile))
+    assert sa.Problem(ctx, "quadbatch", "instances=2;nmin=2;nmax=3").generate("f16split").tile_count("f16split") >= 1
