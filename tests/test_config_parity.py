@@ -115,7 +115,7 @@ def matrix_errors(ctx, kind, spec):
              "element_matrix": relerr(ke, ke_ref),
              "global": float(np.sqrt(np.sum((g.Aval - o.Aval) ** 2) + np.sum((g.Dval - o.Dval) ** 2)) / ref_glob),
              "rhs": relerr(g.rhs, o.rhs),
-             "kernels": plan.kernel_counts(mode)}
+             "kernels": dict(plan.kernel_counts(mode), tile=plan.tile_count(mode))}
         if comps is not None:
             rows = np.repeat(np.arange(nf), np.diff(o.Arow))
             diff2 = np.bincount(comps[rows], weights=(g.Aval - o.Aval) ** 2, minlength=comps.max() + 1)
