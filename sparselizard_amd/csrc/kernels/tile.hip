@@ -11,21 +11,24 @@
 // power-of-two scale; then per k-chunk every lane builds its 8 entries of Q_e
 //   Q_e[k][j] = sc * sum_b D[q][a][b] * w_q dof_b[j][q]
 // splits them into f16 hi/lo and issues three v_mfma_f32_16x16x32_f16
-// (hi*hi + hi*lo + lo*hi) into the element's FP32 accumulator.
+// (hi*hi + hi*lo + lo*hi) into the element's FP32 accumulator, or, in the BF16
+// split, into bf16 hi/mid/lo with six v_mfma_f32_16x16x32_bf16 products.
 #include "tile.h"
 
 namespace slx {
 
 typedef _Float16 half8t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8t __attribute__((ext_vector_type(8)));
 typedef float float4t __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4t __attribute__((ext_vector_type(4)));
 
-template <int NA, int MTT, int NTT, bool GT>
+template <bool BF, int NA, int MTT, int NTT, bool GT>
 __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, TileSlice t, const float* __restrict__ ctab, int qfast, float* __restrict__ ke) {
     constexpr int QH = tile_qh(NA);
-    constexpr int NC = 16 * NTT;  // table columns
+    constexpr int NC = 16 * NTT;    // table columns
+    constexpr int NPL = BF ? 3 : 2;  // operand planes: FP16 hi/lo, BF16 hi/mid/lo
     static_assert(QH >= 1, "a lane group holds at least one Gauss point");
-    // dynamic LDS: [ngpp][NC] float4 weighted dof table and [nkc][MTT][2][64] P
+    // dynamic LDS: [ngpp][NC] float4 weighted dof table and [nkc][MTT][NPL][64] P
     // fragment words (both read from global memory instead when GT: the larger
     // tiles' tables would not fit), then [SLX_TK_WAVES][ngpp][4] float4
     // coefficient rows
@@ -35,7 +38,7 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     const uint4t* af = reinterpret_cast<const uint4t*>(t.afrag);
     float4t* Tl = reinterpret_cast<float4t*>(tile_lds);
     uint4t* Pl = reinterpret_cast<uint4t*>(Tl + (GT ? 0 : ngpp * NC));
-    float4t* Dall = reinterpret_cast<float4t*>(Pl + (GT ? 0 : nkc * MTT * 2 * 64));
+    float4t* Dall = reinterpret_cast<float4t*>(Pl + (GT ? 0 : nkc * MTT * NPL * 64));
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, g = lane >> 4, jl = lane & 15;
     typedef __attribute__((address_space(4))) const int32_t cint32;
     const cint32* t_ai = (const cint32*)(t.tinfo);
@@ -43,7 +46,7 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
     const cint32* t_slot = t_ai + 2 * n_terms;
     if constexpr (!GT) {
         for (int x = tid; x < ngpp * NC; x += blockDim.x) Tl[x] = tq4[x];
-        for (int x = tid; x < nkc * MTT * 2 * 64; x += blockDim.x) Pl[x] = af[x];
+        for (int x = tid; x < nkc * MTT * NPL * 64; x += blockDim.x) Pl[x] = af[x];
     }
     const float4t* Tsrc = GT ? tq4 : Tl;
     const uint4t* Psrc = GT ? af : Pl;
@@ -116,7 +119,8 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
 #pragma unroll 1
         for (int c = 0; c < nkc; c++) {
             const int q0 = (4 * c + g) * QH;
-            half8t hv[NTT], lv[NTT];
+            // the lane's eight Q_e entries of each column tile, split into the planes
+            uint4t qp[NTT][NPL];
 #pragma unroll
             for (int v = 0; v < 8; v++) {
                 const int qi = v / NA, a = v % NA;
@@ -129,20 +133,53 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
                         const float4t h = Tsrc[(q0 + qi) * NC + jt * 16 + jl];
                         val = sc * (d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3]);
                     }
-                    const _Float16 hi = (_Float16)val;
-                    hv[jt][v] = hi;
-                    lv[jt][v] = (_Float16)(val - (float)hi);
+                    if constexpr (BF) {
+                        bf16x8t hv = __builtin_bit_cast(bf16x8t, qp[jt][0]), mv = __builtin_bit_cast(bf16x8t, qp[jt][1]),
+                                lv = __builtin_bit_cast(bf16x8t, qp[jt][2]);
+                        const __bf16 hi = (__bf16)val;
+                        const float r1 = val - (float)hi;
+                        const __bf16 mid = (__bf16)r1;
+                        hv[v] = hi;
+                        mv[v] = mid;
+                        lv[v] = (__bf16)(r1 - (float)mid);
+                        qp[jt][0] = __builtin_bit_cast(uint4t, hv);
+                        qp[jt][1] = __builtin_bit_cast(uint4t, mv);
+                        qp[jt][2] = __builtin_bit_cast(uint4t, lv);
+                    } else {
+                        half8t hv = __builtin_bit_cast(half8t, qp[jt][0]), lv = __builtin_bit_cast(half8t, qp[jt][1]);
+                        const _Float16 hi = (_Float16)val;
+                        hv[v] = hi;
+                        lv[v] = (_Float16)(val - (float)hi);
+                        qp[jt][0] = __builtin_bit_cast(uint4t, hv);
+                        qp[jt][1] = __builtin_bit_cast(uint4t, lv);
+                    }
                 }
             }
 #pragma unroll
             for (int mt = 0; mt < MTT; mt++) {
-                const half8t ph = __builtin_bit_cast(half8t, Psrc[((c * MTT + mt) * 2) * 64 + lane]);
-                const half8t pl = __builtin_bit_cast(half8t, Psrc[((c * MTT + mt) * 2 + 1) * 64 + lane]);
+                uint4t pp[NPL];
+#pragma unroll
+                for (int pl = 0; pl < NPL; pl++) pp[pl] = Psrc[((c * MTT + mt) * NPL + pl) * 64 + lane];
 #pragma unroll
                 for (int jt = 0; jt < NTT; jt++) {
-                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv[jt], acc[mt][jt], 0, 0, 0);
-                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, lv[jt], acc[mt][jt], 0, 0, 0);
-                    acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pl, hv[jt], acc[mt][jt], 0, 0, 0);
+                    if constexpr (BF) {
+                        const bf16x8t ah = __builtin_bit_cast(bf16x8t, pp[0]), am = __builtin_bit_cast(bf16x8t, pp[1]),
+                                      al = __builtin_bit_cast(bf16x8t, pp[2]);
+                        const bf16x8t xh = __builtin_bit_cast(bf16x8t, qp[jt][0]), xm = __builtin_bit_cast(bf16x8t, qp[jt][1]),
+                                      xl = __builtin_bit_cast(bf16x8t, qp[jt][2]);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xh, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xm, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xh, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, xl, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, xm, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, xh, acc[mt][jt], 0, 0, 0);
+                    } else {
+                        const half8t ph = __builtin_bit_cast(half8t, pp[0]), pl = __builtin_bit_cast(half8t, pp[1]);
+                        const half8t hv = __builtin_bit_cast(half8t, qp[jt][0]), lv = __builtin_bit_cast(half8t, qp[jt][1]);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, hv, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ph, lv, acc[mt][jt], 0, 0, 0);
+                        acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x32_f16(pl, hv, acc[mt][jt], 0, 0, 0);
+                    }
                 }
             }
         }
@@ -169,31 +206,35 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
 // caches for 3 x 3 and 4 x 4 (the square tile count of the larger side)
 static bool tile_global_tables(const TileSlice& t) { return t.mtt > 2 || t.ntt > 2; }
 
-size_t tile_lds_bytes(const TileSlice& t) {
+size_t tile_lds_bytes(const TileSlice& t, int planes) {
     const size_t rows = (size_t)SLX_TK_WAVES * t.ngpp * 4 * 16;
     if (tile_global_tables(t)) return rows;
-    return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * 2 * 64 * 16 + rows;
+    return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * planes * 64 * 16 + rows;
 }
 
-hipError_t launch_tile(const DevBucket& b, const TileSlice& t, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
+hipError_t launch_tile(const DevBucket& b, const TileSlice& t0, int bf16, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
+    TileSlice t = t0;
+    t.afrag = t0.afrag_x[bf16 ? 1 : 0];
+    const int planes = bf16 ? 3 : 2;
     if (b.n_elem == 0) return hipSuccess;
     if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.mtt < 1 || t.mtt > 4 ||
         t.ntt < 1 || t.ntt > 4 || (tile_global_tables(t) && t.mtt != t.ntt) || t.n_tf > 16 * t.mtt || t.n_dof > 16 * t.ntt ||
-        tile_lds_bytes(t) > 64 * 1024)
+        !t.afrag || tile_lds_bytes(t, planes) > 64 * 1024)
         return hipErrorInvalidValue;
-    const size_t lds = tile_lds_bytes(t);
+    const size_t lds = tile_lds_bytes(t, planes);
     const int64_t groups = ((int64_t)b.n_elem + SLX_TK_WAVES - 1) / SLX_TK_WAVES;
     const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)n_cu * 16);
     const dim3 block(64 * SLX_TK_WAVES);
-#define SLX_TILE_LAUNCH(NA_, M_, N_, G_) \
-    hipLaunchKernelGGL((tile_qv_kernel<NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
-#define SLX_TILE_SHAPES(NA_)                                         \
-    if (t.mtt == 1 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 1, 1, false);         \
-    else if (t.mtt == 1 && t.ntt == 2) SLX_TILE_LAUNCH(NA_, 1, 2, false);    \
-    else if (t.mtt == 2 && t.ntt == 1) SLX_TILE_LAUNCH(NA_, 2, 1, false);    \
-    else if (t.mtt == 2 && t.ntt == 2) SLX_TILE_LAUNCH(NA_, 2, 2, false);    \
-    else if (t.mtt == 3) SLX_TILE_LAUNCH(NA_, 3, 3, true);                   \
-    else SLX_TILE_LAUNCH(NA_, 4, 4, true)
+#define SLX_TILE_LAUNCH(NA_, M_, N_, G_)                                                                                          \
+    if (bf16) hipLaunchKernelGGL((tile_qv_kernel<true, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); \
+    else hipLaunchKernelGGL((tile_qv_kernel<false, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
+#define SLX_TILE_SHAPES(NA_)                                            \
+    if (t.mtt == 1 && t.ntt == 1) { SLX_TILE_LAUNCH(NA_, 1, 1, false); }        \
+    else if (t.mtt == 1 && t.ntt == 2) { SLX_TILE_LAUNCH(NA_, 1, 2, false); }   \
+    else if (t.mtt == 2 && t.ntt == 1) { SLX_TILE_LAUNCH(NA_, 2, 1, false); }   \
+    else if (t.mtt == 2 && t.ntt == 2) { SLX_TILE_LAUNCH(NA_, 2, 2, false); }   \
+    else if (t.mtt == 3) { SLX_TILE_LAUNCH(NA_, 3, 3, true); }                  \
+    else { SLX_TILE_LAUNCH(NA_, 4, 4, true); }
     switch (t.na) {
         case 1: SLX_TILE_SHAPES(1); break;
         case 2: SLX_TILE_SHAPES(2); break;

@@ -1,5 +1,6 @@
-"""One 16x16 MFMA tile per element (kernels/tile.hip) for the small matrix
-slices whose coefficients vary over the Gauss points, FP16 split: the
+"""One wave per element holding its 16x16 MFMA tiles (kernels/tile.hip) for the
+small matrix slices whose coefficients vary over the Gauss points, FP16 and BF16
+splits: the
 element-matrix contraction of contribution.cpp:172-246 regrouped by tf
 derivative, Ke = P^T Q_e with the shared tf table P and the element's
 Q_e[k][j] = w_q sum_b D_e[q][a][b] dof_b[j][q].
@@ -52,18 +53,19 @@ CASES = [("quadbatch", "instances=3;nmin=2;nmax=5"), ("newton2d", "n=6"), ("pris
          ("hppoisson3d", "n=2;pmin=2;pmax=2"), ("prism3d", "n=2;order=3"), ("hppoisson3d", "n=2;pmin=3;pmax=3")]
 
 
+@pytest.mark.parametrize("precision", ["f16split", "bf16split"])
 @pytest.mark.parametrize("kind,spec", CASES)
-def test_one_tile_kernel_matches_oracle(ctx, kind, spec, monkeypatch):
+def test_one_tile_kernel_matches_oracle(ctx, kind, spec, precision, monkeypatch):
     # every slice size (by default the kernel takes n_tf * n_dof >= 100 only),
-    # whatever the kernel table names for the class
+    # whatever the kernel table names for the class; both split arithmetics
     monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
     monkeypatch.setenv("SLX_TUNE_TABLE", "none")
-    print("case tile", kind, spec, flush=True)
+    print("case tile", kind, spec, precision, flush=True)
     prob = sa.Problem(ctx, kind, spec)
     info = prob.info()
-    plan = prob.generate("f16split")
-    assert plan.tile_count("f16split") >= 1, "one-tile kernel not used"
-    assert plan.tile_count("bf16split") == 0 and plan.tile_count("fp64") == 0
+    plan = prob.generate(precision)
+    assert plan.tile_count(precision) >= 1, "one-tile kernel not used"
+    assert plan.tile_count("fp32mfma") == 0 and plan.tile_count("fp64") == 0
     got = plan.csr(info["ndofs"])
     ref = py_oracle.assemble(prob.desc_ptr(), prob.numeric(), info["ndofs"], info["nfree"])
     for name in ("Arow", "Acol", "Drow", "Dcol"):
