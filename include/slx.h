@@ -246,7 +246,8 @@ slx_status slx_plan_kernels(slx_plan plan, int32_t* n_mfma_buckets, int32_t* n_b
 slx_status slx_plan_kernel_counts(slx_plan plan, int32_t precision, int32_t* n_batched, int32_t* n_factorized);
 /* For one precision mode: matrix slices whose element matrices run on the
  * one-tile-per-element MFMA kernel (small slices with Gauss-point-varying
- * coefficients; FP16 and BF16 splits, 0 for the other modes). */
+ * coefficients; the FP16 split, BF16 split and FP32-MFMA modes, 0 for the FP32
+ * VALU and FP64 modes). */
 slx_status slx_plan_tile_count(slx_plan plan, int32_t precision, int32_t* n_tile);
 /* The fused launch of a precision mode: factorized slices of every bucket in
  * one persistent launch (contribution.cpp:61-133 loops over the element

@@ -9,8 +9,8 @@
 // P and the weighted dof table stay in LDS for the whole launch (up to 2 x 2
 // tiles; read through the caches for the larger ones), and each element
 // builds its Q_e fragments from its coefficients.  FP16
-// hi/lo or BF16 hi/mid/lo split with FP32 accumulation (kernels/mfma.h,
-// AR_F16 and AR_BF16).
+// hi/lo or BF16 hi/mid/lo split, or plain FP32, with FP32 accumulation
+// (kernels/mfma.h, AR_F16, AR_BF16 and AR_F32).
 #pragma once
 #include <hip/hip_runtime.h>
 #include "device.h"
@@ -38,10 +38,10 @@ struct TileSlice {
     const int32_t* tinfo;
     float a_unscale;                // 1 / power-of-two scale of P
     float tqmax;                    // max |tq|
-    const uint32_t* afrag;          // set per launch: afrag_x[0] (FP16 split) or afrag_x[1] (BF16 split)
-    const uint32_t* afrag_x[2];     // [nkc][mtt][planes][64 lanes][4 words]: lane l of chunk c and row tile mt holds the f16
-                                    // hi, lo (2 planes) or bf16 hi, mid, lo (3 planes) of the scaled P^T[16 mt + l % 16][32 c +
-                                    // 8 (l / 16) + v], v = 0..7
+    const uint32_t* afrag;          // set per launch: afrag_x[arith]
+    const uint32_t* afrag_x[3];     // [nkc][mtt][planes][64 lanes][4 words]: lane l of chunk c and row tile mt holds its entries
+                                    // v = 0..7, the scaled P^T[16 mt + l % 16][k(c, l / 16, v)], as f16 hi, lo (2 planes),
+                                    // bf16 hi, mid, lo (3 planes) or floats (2 words: v = 0..3, 4..7)
     const float* tq;                // [ngpp][16 ntt][4]: w_q dof_b[j][q], zero for q >= ngp or j >= n_dof
     int64_t ke_base, ke_ld;
 };
@@ -49,7 +49,8 @@ struct TileSlice {
 size_t tile_lds_bytes(const TileSlice& t, int planes);
 // Element matrices of one slice (coefficient table [slot][element][point]
 // when qfast, else [slot][point][element]).
-hipError_t launch_tile(const DevBucket& b, const TileSlice& t, int bf16, const float* ctab, int qfast, float* ke, int n_cu,
+// arith: 0 FP16 split, 1 BF16 split, 2 FP32.
+hipError_t launch_tile(const DevBucket& b, const TileSlice& t, int arith, const float* ctab, int qfast, float* ke, int n_cu,
                        hipStream_t st);
 
 }  // namespace slx

@@ -12,7 +12,8 @@
 //   Q_e[k][j] = sc * sum_b D[q][a][b] * w_q dof_b[j][q]
 // splits them into f16 hi/lo and issues three v_mfma_f32_16x16x32_f16
 // (hi*hi + hi*lo + lo*hi) into the element's FP32 accumulator, or, in the BF16
-// split, into bf16 hi/mid/lo with six v_mfma_f32_16x16x32_bf16 products.
+// split, into bf16 hi/mid/lo with six v_mfma_f32_16x16x32_bf16 products, or,
+// in FP32, eight v_mfma_f32_16x16x4_f32 steps on the entries as they are.
 #include "tile.h"
 
 namespace slx {
@@ -22,11 +23,14 @@ typedef __bf16 bf16x8t __attribute__((ext_vector_type(8)));
 typedef float float4t __attribute__((ext_vector_type(4)));
 typedef unsigned int uint4t __attribute__((ext_vector_type(4)));
 
-template <bool BF, int NA, int MTT, int NTT, bool GT>
+// arithmetic AR: 0 FP16 split, 1 BF16 split, 2 FP32 (v_mfma_f32_16x16x4_f32,
+// lane group g's entry v at step v of the chunk's eight K = 4 steps)
+template <int AR, int NA, int MTT, int NTT, bool GT>
 __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, TileSlice t, const float* __restrict__ ctab, int qfast, float* __restrict__ ke) {
     constexpr int QH = tile_qh(NA);
     constexpr int NC = 16 * NTT;    // table columns
-    constexpr int NPL = BF ? 3 : 2;  // operand planes: FP16 hi/lo, BF16 hi/mid/lo
+    constexpr bool BF = AR == 1, F32 = AR == 2;
+    constexpr int NPL = BF ? 3 : 2;  // 16-byte operand words per lane: FP16 hi/lo, BF16 hi/mid/lo, FP32 entries 0..3 / 4..7
     static_assert(QH >= 1, "a lane group holds at least one Gauss point");
     // dynamic LDS: [ngpp][NC] float4 weighted dof table and [nkc][MTT][NPL][64] P
     // fragment words (both read from global memory instead when GT: the larger
@@ -133,7 +137,11 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
                         const float4t h = Tsrc[(q0 + qi) * NC + jt * 16 + jl];
                         val = sc * (d[0] * h[0] + d[1] * h[1] + d[2] * h[2] + d[3] * h[3]);
                     }
-                    if constexpr (BF) {
+                    if constexpr (F32) {
+                        float4t x = __builtin_bit_cast(float4t, qp[jt][v >> 2]);
+                        x[v & 3] = val;
+                        qp[jt][v >> 2] = __builtin_bit_cast(uint4t, x);
+                    } else if constexpr (BF) {
                         bf16x8t hv = __builtin_bit_cast(bf16x8t, qp[jt][0]), mv = __builtin_bit_cast(bf16x8t, qp[jt][1]),
                                 lv = __builtin_bit_cast(bf16x8t, qp[jt][2]);
                         const __bf16 hi = (__bf16)val;
@@ -162,7 +170,12 @@ __global__ void __launch_bounds__(64 * SLX_TK_WAVES) tile_qv_kernel(int n_elem, 
                 for (int pl = 0; pl < NPL; pl++) pp[pl] = Psrc[((c * MTT + mt) * NPL + pl) * 64 + lane];
 #pragma unroll
                 for (int jt = 0; jt < NTT; jt++) {
-                    if constexpr (BF) {
+                    if constexpr (F32) {
+#pragma unroll
+                        for (int v = 0; v < 8; v++)
+                            acc[mt][jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(__builtin_bit_cast(float4t, pp[v >> 2])[v & 3],
+                                                                             __builtin_bit_cast(float4t, qp[jt][v >> 2])[v & 3], acc[mt][jt], 0, 0, 0);
+                    } else if constexpr (BF) {
                         const bf16x8t ah = __builtin_bit_cast(bf16x8t, pp[0]), am = __builtin_bit_cast(bf16x8t, pp[1]),
                                       al = __builtin_bit_cast(bf16x8t, pp[2]);
                         const bf16x8t xh = __builtin_bit_cast(bf16x8t, qp[jt][0]), xm = __builtin_bit_cast(bf16x8t, qp[jt][1]),
@@ -212,10 +225,11 @@ size_t tile_lds_bytes(const TileSlice& t, int planes) {
     return (size_t)t.ngpp * 16 * t.ntt * 16 + (size_t)t.nkc * t.mtt * planes * 64 * 16 + rows;
 }
 
-hipError_t launch_tile(const DevBucket& b, const TileSlice& t0, int bf16, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
+hipError_t launch_tile(const DevBucket& b, const TileSlice& t0, int arith, const float* ctab, int qfast, float* ke, int n_cu, hipStream_t st) {
+    if (arith < 0 || arith > 2) return hipErrorInvalidValue;
     TileSlice t = t0;
-    t.afrag = t0.afrag_x[bf16 ? 1 : 0];
-    const int planes = bf16 ? 3 : 2;
+    t.afrag = t0.afrag_x[arith];
+    const int planes = arith == 1 ? 3 : 2;
     if (b.n_elem == 0) return hipSuccess;
     if (t.nkc < 1 || t.nkc > SLX_TK_MAXC || t.ngpp > SLX_TK_MAXQ || t.ngpp != t.nkc * 4 * tile_qh(t.na) || t.mtt < 1 || t.mtt > 4 ||
         t.ntt < 1 || t.ntt > 4 || (tile_global_tables(t) && t.mtt != t.ntt) || t.n_tf > 16 * t.mtt || t.n_dof > 16 * t.ntt ||
@@ -225,9 +239,10 @@ hipError_t launch_tile(const DevBucket& b, const TileSlice& t0, int bf16, const 
     const int64_t groups = ((int64_t)b.n_elem + SLX_TK_WAVES - 1) / SLX_TK_WAVES;
     const unsigned grid = (unsigned)std::min<int64_t>(groups, (int64_t)n_cu * 16);
     const dim3 block(64 * SLX_TK_WAVES);
-#define SLX_TILE_LAUNCH(NA_, M_, N_, G_)                                                                                          \
-    if (bf16) hipLaunchKernelGGL((tile_qv_kernel<true, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); \
-    else hipLaunchKernelGGL((tile_qv_kernel<false, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
+#define SLX_TILE_LAUNCH(NA_, M_, N_, G_)                                                                                        \
+    if (arith == 1) hipLaunchKernelGGL((tile_qv_kernel<1, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); \
+    else if (arith == 2) hipLaunchKernelGGL((tile_qv_kernel<2, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke); \
+    else hipLaunchKernelGGL((tile_qv_kernel<0, NA_, M_, N_, G_>), dim3(grid), block, lds, st, b.n_elem, t, ctab, qfast, ke)
 #define SLX_TILE_SHAPES(NA_)                                            \
     if (t.mtt == 1 && t.ntt == 1) { SLX_TILE_LAUNCH(NA_, 1, 1, false); }        \
     else if (t.mtt == 1 && t.ntt == 2) { SLX_TILE_LAUNCH(NA_, 1, 2, false); }   \

@@ -201,8 +201,8 @@ struct slx_plan_s {
     std::vector<uint32_t> mfma_mask;   // matrix slices handled by the MFMA kernel
     std::vector<std::vector<std::pair<int, FactorSlice>>> factor;  // per bucket: (slice, factorized MFMA data)
     std::vector<uint32_t> factor_mask; // matrix slices handled by the factorized MFMA kernel
-    std::vector<std::vector<std::pair<int, TileSlice>>> tile;  // per bucket: (slice, one-tile-per-element data), FP16/BF16 splits
-    std::vector<uint32_t> tile_mask;   // matrix slices handled by the one-tile kernel in the FP16 and BF16 splits
+    std::vector<std::vector<std::pair<int, TileSlice>>> tile;  // per bucket: (slice, one-tile-per-element data), MFMA arithmetics
+    std::vector<uint32_t> tile_mask;   // matrix slices handled by the one-tile kernel in the MFMA arithmetics
     int last_qfast = 0;                // coefficient table layout of the last bucket assembled
     bool factor64 = true;              // FP64 mode: factorized slices on v_mfma_f64 (SLX_TUNE_FP64_FACTOR=0: VALU)
     // fused launch: the factorized slices of every bucket in one persistent
@@ -1289,8 +1289,8 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
             if (factor_lds_bytes(f, a) <= 160 * 1024 && f.mt * mfma_planes(a) * 64 <= SLX_FK_AREG * 64 * SLX_FK_BWAVES)
                 f.arith_ok |= 1u << a;
         if (!(f.arith_ok & (1u << AR_F16))) continue;
-        // the FP16 and BF16 splits of a slice the one-tile kernel takes run there
-        if ((P.tile_mask[b] >> s) & 1u) f.arith_ok &= ~((1u << AR_F16) | (1u << AR_BF16));
+        // the MFMA arithmetics of a slice the one-tile kernel takes run there
+        if ((P.tile_mask[b] >> s) & 1u) f.arith_ok &= ~((1u << AR_F16) | (1u << AR_BF16) | (1u << AR_F32));
         // P fragments: lane l of tile (ks, mt) holds P^T[i = mt*16 + l%16][k = 8*(l/16) + v],
         // k -> (Gauss point, tf derivative) by segments (factor_segw); power-of-two scale to [128, 256)
         const double* tft = bd.tables[sd.tf_table];
@@ -1388,8 +1388,8 @@ void build_factor(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayou
 
 // One-tile-per-element data (kernels/tile.h) for the small matrix slices of a
 // bucket whose coefficients vary over the Gauss points and that the batched
-// MFMA kernel does not take (FP16 and BF16 splits; the FP32 modes keep the
-// factorized or VALU kernel there).
+// MFMA kernel does not take (the three MFMA arithmetics; the FP32 VALU and
+// FP64 modes keep their kernels there).
 void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout& L) {
     if (bd.q_invariant) return;
     if (const char* t = std::getenv("SLX_TUNE_TILE"))
@@ -1445,8 +1445,9 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         if (amax > 0) (void)std::frexp(amax, &aex);
         const double ascale = std::ldexp(1.0, 8 - aex);
         t.a_unscale = (float)std::ldexp(1.0, aex - 8);
-        // FP16 split: planes hi, lo; BF16 split: planes hi, mid, lo
+        // FP16 split: planes hi, lo; BF16 split: planes hi, mid, lo; FP32: the entries
         std::vector<uint16_t> frag((size_t)nkc * t.mtt * 2 * 64 * 8, 0), fragb((size_t)nkc * t.mtt * 3 * 64 * 8, 0);
+        std::vector<float> fragf((size_t)nkc * t.mtt * 2 * 64 * 4, 0.f);
         for (int c = 0; c < nkc; c++)
             for (int mt = 0; mt < t.mtt; mt++)
                 for (int l = 0; l < 64; l++)
@@ -1464,6 +1465,7 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
                         fragb[((wb + 0) * 64 + l) * 8 + v] = bh;
                         fragb[((wb + 1) * 64 + l) * 8 + v] = bm;
                         fragb[((wb + 2) * 64 + l) * 8 + v] = bl;
+                        fragf[((((size_t)c * t.mtt + mt) * 2 + v / 4) * 64 + l) * 4 + v % 4] = x;
                     }
         // weighted dof table [q][j][4], zero past the last Gauss point and dof
         const double* dft = bd.tables[sd.dof_table];
@@ -1482,6 +1484,7 @@ void build_tile(slx_plan_s& P, const slx_bucket_desc& bd, int b, const KeLayout&
         t.tinfo = P.dev(tinfo.data(), tinfo.size());
         t.afrag_x[0] = reinterpret_cast<const uint32_t*>(P.dev(frag.data(), frag.size()));
         t.afrag_x[1] = reinterpret_cast<const uint32_t*>(P.dev(fragb.data(), fragb.size()));
+        t.afrag_x[2] = reinterpret_cast<const uint32_t*>(P.dev(fragf.data(), fragf.size()));
         t.afrag = t.afrag_x[0];
         t.tq = P.dev(tq.data(), tq.size());
         t.ke_base = L.ke_base[b][s];
@@ -1844,7 +1847,7 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         const DevBucket& b = P.dbuckets[bi];
         const hipStream_t bst = (overlap && P.rhs_only[bi]) ? P.side : st;
         const uint32_t fmask = factor_mask(bi);  // slices on the factorized kernel
-        const uint32_t tmask = (arith == AR_F16 || arith == AR_BF16) ? P.tile_mask[bi] & ~fmask : 0u;  // slices on the one-tile kernel
+        const uint32_t tmask = arith >= 0 ? P.tile_mask[bi] & ~fmask : 0u;  // slices on the one-tile kernel
         const bool in_fused = fused && P.fuse_bucket[bi] && fmask != 0;
         void* ctab = in_fused ? (void*)((char*)P.d_ctab_fused.p + P.fuse_off[bi]) : (P.rhs_only[bi] ? P.d_ctab_side.p : P.d_ctab.p);
         uint32_t all = (b.n_slices >= 32) ? 0xffffffffu : ((1u << b.n_slices) - 1u);
@@ -1885,7 +1888,8 @@ void numeric(slx_plan_s& P, const double* xyz, const double* params, int64_t ple
         if (tmask != 0) {
             if (tm) tm->begin(T_MFMA);
             for (auto& ts : P.tile[bi])
-                if ((tmask >> ts.first) & 1u) HIPCHECK(launch_tile(b, ts.second, arith == AR_BF16 ? 1 : 0, (const float*)ctab, qfast, P.d_ke.as<float>(), P.n_cu, bst));
+                if ((tmask >> ts.first) & 1u) HIPCHECK(launch_tile(b, ts.second, arith == AR_BF16 ? 1 : arith == AR_F32 ? 2 : 0, (const float*)ctab, qfast, P.d_ke.as<float>(),
+                                                                P.n_cu, bst));
             if (tm) tm->end();
             done |= tmask;
         }
@@ -2417,7 +2421,7 @@ slx_status slx_plan_tile_count(slx_plan p, int32_t precision, int32_t* n_tile) {
         SLX_CHECK(p && n_tile, "null argument");
         SLX_CHECK(precision >= 0 && precision <= 4, "unknown precision " << precision);
         int n = 0;
-        if (precision == SLX_PREC_F16_SPLIT || precision == SLX_PREC_BF16_SPLIT)
+        if (precision == SLX_PREC_F16_SPLIT || precision == SLX_PREC_BF16_SPLIT || precision == SLX_PREC_FP32_MFMA)
             for (auto& v : p->tile) n += (int)v.size();
         *n_tile = n;
     });

@@ -1,6 +1,6 @@
 """One wave per element holding its 16x16 MFMA tiles (kernels/tile.hip) for the
-small matrix slices whose coefficients vary over the Gauss points, FP16 and BF16
-splits: the
+small matrix slices whose coefficients vary over the Gauss points, in the FP16
+split, the BF16 split and FP32 MFMA: the
 element-matrix contraction of contribution.cpp:172-246 regrouped by tf
 derivative, Ke = P^T Q_e with the shared tf table P and the element's
 Q_e[k][j] = w_q sum_b D_e[q][a][b] dof_b[j][q].
@@ -53,11 +53,12 @@ CASES = [("quadbatch", "instances=3;nmin=2;nmax=5"), ("newton2d", "n=6"), ("pris
          ("hppoisson3d", "n=2;pmin=2;pmax=2"), ("prism3d", "n=2;order=3"), ("hppoisson3d", "n=2;pmin=3;pmax=3")]
 
 
-@pytest.mark.parametrize("precision", ["f16split", "bf16split"])
+@pytest.mark.parametrize("precision", ["f16split", "bf16split", "fp32mfma"])
 @pytest.mark.parametrize("kind,spec", CASES)
 def test_one_tile_kernel_matches_oracle(ctx, kind, spec, precision, monkeypatch):
     # every slice size (by default the kernel takes n_tf * n_dof >= 100 only),
-    # whatever the kernel table names for the class; both split arithmetics
+    # whatever the kernel table names for the class; the three MFMA arithmetics
+    # (gates: 1e-6 for the splits, 2e-5 for FP32 as for the FP32 VALU kernel)
     monkeypatch.setenv("SLX_TUNE_TILE_MIN", "0")
     monkeypatch.setenv("SLX_TUNE_TABLE", "none")
     print("case tile", kind, spec, precision, flush=True)
@@ -65,7 +66,8 @@ def test_one_tile_kernel_matches_oracle(ctx, kind, spec, precision, monkeypatch)
     info = prob.info()
     plan = prob.generate(precision)
     assert plan.tile_count(precision) >= 1, "one-tile kernel not used"
-    assert plan.tile_count("fp32mfma") == 0 and plan.tile_count("fp64") == 0
+    assert plan.tile_count("fp32") == 0 and plan.tile_count("fp64") == 0
+    tol = 2e-5 if precision == "fp32mfma" else TOL_SPLIT
     got = plan.csr(info["ndofs"])
     ref = py_oracle.assemble(prob.desc_ptr(), prob.numeric(), info["ndofs"], info["nfree"])
     for name in ("Arow", "Acol", "Drow", "Dcol"):
@@ -75,7 +77,7 @@ def test_one_tile_kernel_matches_oracle(ctx, kind, spec, precision, monkeypatch)
         if b.size:
             e = relerr(getattr(got, name), b)
             print(name, "rel err %.3e" % e)
-            assert e <= TOL_SPLIT, "%s rel err %.3e" % (name, e)
+            assert e <= tol, "%s rel err %.3e" % (name, e)
 
 
 @pytest.mark.parametrize("kind,spec", CASES[:3] + CASES[5:7] + CASES[8:])
