@@ -1,11 +1,13 @@
 // One wave per element holding its 16x16 MFMA tiles (up to 4 x 4) for the
 // smaller slices with Gauss-point-varying coefficients (kernels/tile.h).
 //
-// A workgroup of SLX_TK_WAVES waves loops over the bucket's elements, one
-// element per wave at a time.  Once per workgroup: the weighted dof table
-// (w_q dof_b[j][q], [q][j][4 derivatives]) and the P fragments (the shared tf
-// table, FP16 hi/lo, scaled on the host) go to LDS, where they stay for the
-// whole launch.  Per element: the lanes of the Gauss points sum
+// A workgroup of SLX_TK_WAVES waves loops over a contiguous range of the
+// bucket's elements, one element per wave at a time.  Up to 2 x 2 tiles, the
+// weighted dof table (w_q dof_b[j][q], [q][j][4 derivatives]) and the P
+// fragments (the shared tf table in the launch's arithmetic, scaled on the
+// host) go to LDS once per workgroup and stay there for the whole launch; the
+// 3 x 3 and 4 x 4 variants read both through the caches.  Per element: the
+// lanes of the Gauss points sum
 // the element's term coefficients into the 4x4 coefficient matrices
 // D[q][a][b] (LDS, one region per wave) and the wave takes the element's
 // power-of-two scale; then per k-chunk every lane builds its 8 entries of Q_e
