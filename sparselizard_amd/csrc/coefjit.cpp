@@ -30,8 +30,20 @@ std::map<std::string, hipFunction_t> g_cache;  // key: device id + source
 
 }  // namespace
 
-std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool runs) {
+std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool runs, const std::vector<int>& field_nff) {
     std::ostringstream o;
+    // fields whose values a thread gathers once per element (runs of Gauss
+    // points, at most SLX_JIT_GATHER_NFF form functions)
+    std::vector<int> pre;
+    if (runs)
+        for (int pc = 0; pc < n; pc++)
+            if (prog[pc].op == SLX_OP_FIELD) {
+                const int f = prog[pc].imm;
+                if (f >= 0 && f < (int)field_nff.size() && field_nff[f] > 0 && field_nff[f] <= SLX_JIT_GATHER_NFF &&
+                    std::find(pre.begin(), pre.end(), f) == pre.end())
+                    pre.push_back(f);
+            }
+    auto nff_of = [&](int f) { return std::find(pre.begin(), pre.end(), f) == pre.end() ? 0 : field_nff[f]; };
     o << "namespace slx {\ntypedef " << (fp64 ? "double" : "float") << " T;\n";
     // runs: one thread per (element, run of qc Gauss points).  Gauss point
     // fastest layout (qfast): qc = 1 and consecutive threads take consecutive
@@ -49,8 +61,12 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool run
              "    const int q0 = qfast ? (int)(idx % nq) : (int)(idx / b.n_elem) * qc;\n"
              "    const int q1 = qfast ? q0 + 1 : min(nq, q0 + qc);\n"
              "    GeoX gx;\n"
-             "    geo_load(b, in.xyz, e, gx);\n"
-             "    const int64_t stride = (int64_t)nq * b.n_elem;\n"
+             "    geo_load(b, in.xyz, e, gx);\n";
+    for (int f : pre)
+        o << "    double fv" << f << "[" << field_nff[f] << "];\n    bool fo" << f << "[" << field_nff[f] << "];\n"
+          << "    field_gather<" << field_nff[f] << ">(b, in, " << f << ", e, fv" << f << ", fo" << f << ");\n";
+    if (runs)
+        o << "    const int64_t stride = (int64_t)nq * b.n_elem;\n"
              "    for (int q = q0; q < q1; q++) {\n"
              "    Geo g;\n"
              "    geo_eval(b, gx, q, g);\n";
@@ -92,7 +108,11 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool run
                 cur[x.dst] = v;
                 continue;
             case SLX_OP_FIELD:
-                rhs = "(T)field_value(b, in, " + std::to_string(x.imm) + ", " + std::to_string(x.a) + ", e, q, g)";
+                if (nff_of(x.imm) > 0)
+                    rhs = "(T)field_value_pre<" + std::to_string(nff_of(x.imm)) + ">(b, " + std::to_string(x.imm) + ", fv" +
+                          std::to_string(x.imm) + ", fo" + std::to_string(x.imm) + ", " + std::to_string(x.a) + ", q, g)";
+                else
+                    rhs = "(T)field_value(b, in, " + std::to_string(x.imm) + ", " + std::to_string(x.a) + ", e, q, g)";
                 break;
             case SLX_OP_ADD: rhs = a + " + " + bb; break;
             case SLX_OP_MUL: rhs = a + " * " + bb; break;
@@ -109,14 +129,15 @@ std::string coef_kernel_source(const slx_instr* prog, int n, bool fp64, bool run
     return std::string(kJitPrelude) + o.str();
 }
 
-bool coef_compile(const slx_instr* prog, int n_instr, bool fp64, bool runs, hipFunction_t* fn, std::string* err) {
+bool coef_compile(const slx_instr* prog, int n_instr, bool fp64, bool runs, const std::vector<int>& field_nff, hipFunction_t* fn,
+                  std::string* err) {
     *fn = nullptr;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) {
         if (err) *err = "no current device";
         return false;
     }
-    const std::string src = coef_kernel_source(prog, n_instr, fp64, runs);
+    const std::string src = coef_kernel_source(prog, n_instr, fp64, runs, field_nff);
     const std::string key = std::to_string(dev) + "\n" + src;
     {
         std::lock_guard<std::mutex> lk(g_mu);

@@ -163,6 +163,45 @@ __device__ inline double field_value(const DevBucket& b, const DevInputs& in, in
     return s;
 }
 
+// The values of field f's NFF form functions on element e (ok: the function
+// has an address), gathered once for all Gauss points a thread takes of the
+// element (NFF must equal the bucket's b.fields[f].nff).
+template <int NFF>
+__device__ inline void field_gather(const DevBucket& b, const DevInputs& in, int f, int e, double (&c)[NFF], bool (&ok)[NFF]) {
+    const DevField& fl = b.fields[f];
+    const double* data = in.field_data[f];
+    int a[NFF];
+#pragma unroll
+    for (int k = 0; k < NFF; k++) a[k] = fl.addr[(size_t)k * b.n_elem + e];
+#pragma unroll
+    for (int k = 0; k < NFF; k++) {
+        ok[k] = a[k] >= 0;
+        c[k] = ok[k] ? data[a[k]] : 0.0;
+    }
+}
+
+// field_value from the values field_gather took (same terms, same order).
+template <int NFF>
+__device__ inline double field_value_pre(const DevBucket& b, int f, const double (&c)[NFF], const bool (&ok)[NFF], int sd, int q,
+                                         const Geo& g) {
+    const DevField& fl = b.fields[f];
+    double v[4] = {0, 0, 0, 0};
+    const int nd = sd == 0 ? 1 : 1 + b.elemdim;
+#pragma unroll
+    for (int k = 0; k < NFF; k++) {
+        if (!ok[k]) continue;
+#pragma unroll
+        for (int d = 0; d < 4; d++)
+            if (d < nd) v[d] += c[k] * fl.tab[((size_t)d * NFF + k) * b.n_gp + q];
+    }
+    if (sd == 0) return v[0];
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+        if (i < b.elemdim) s += pick9(g.inv, 3 * (sd - 1) + i) * v[1 + i];
+    return s;
+}
+
 template <typename T>
 __device__ inline T apply_unary(int op, T x) {
     switch (op) {

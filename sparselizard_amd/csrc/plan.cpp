@@ -1732,9 +1732,11 @@ void compile_coef_programs(slx_plan_s& P, const slx_plan_desc& d) {
             (void)hipSetDevice(device);
             for (size_t i = t; i < todo.size(); i += nth) {
                 const slx_bucket_desc& bd = d.buckets[todo[i].first];
+                std::vector<int> fnff(d.n_fields);
+                for (int f = 0; f < d.n_fields; f++) fnff[f] = d.field_nff[(size_t)f * d.n_buckets + todo[i].first];
                 hipFunction_t fn = nullptr;
                 std::string err;
-                if (coef_compile(bd.prog, bd.n_instr, todo[i].second != 0, bd.q_invariant == 0, &fn, &err)) {
+                if (coef_compile(bd.prog, bd.n_instr, todo[i].second != 0, bd.q_invariant == 0, fnff, &fn, &err)) {
                     P.coef_fn[todo[i].first][todo[i].second] = fn;
                 } else {
                     std::lock_guard<std::mutex> lk(mu);
